@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark: RGF energy-points/s of the batched per-(E,k) transmission engine.
+
+Workload (default): the k-resolved slab -- n = 256 orbitals per principal
+layer, 40 layers, overlap S != I, 16 x 16 transverse k-grid, 500 energies in
+[-2, 2] eV, eta = 1e-5.  One *step* = one batch of (E, k) items (default: all
+500 energies of one k-point; successive steps walk the k-grid).  The k-point
+structures of every step are resident in HBM before the timed region.
+
+Prints ONE JSON line (rank 0).  Single GPU only.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--items", type=int, default=0, help="items per step (0: config default)")
+    ap.add_argument("--nbl", type=int, default=0, help="override the number of layers")
+    ap.add_argument("--cpu-items", type=int, default=48)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plain-items", type=int, default=4,
+                    help="items timed one at a time (unbatched FP64 reference mode)")
+    ap.add_argument("--verbose", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for --gpus {a.gpus}); every (E,k) item is "
+              "independent, multi-GPU runs shard items per GPU with no exchange.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from libnegf_amd import hip, workloads as wl
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    prec = hip.FP64 if a.precision == "fp64" else hip.MIXED
+
+    # ------------------------------------------------------------ workload ----
+    if a.config == "c3":
+        slab, kgrid, Egrid, eta = wl.config3(nbl=a.nbl or None)
+        nsteps = a.warmup + a.steps
+        kidx = [(17 + 37 * i) % len(kgrid) for i in range(nsteps)]
+        t0 = time.time()
+        structs = [slab.at_k(*kgrid[j]) for j in kidx]
+        log(f"built {len(structs)} k-structures in {time.time() - t0:.1f}s")
+        ds = hip.DeviceStructure(structs)
+        per_step = a.items or len(Egrid)
+        steps_E = [Egrid[np.linspace(0, len(Egrid) - 1, per_step).astype(int)] for _ in range(nsteps)]
+        steps_slot = [np.full(per_step, i, dtype=np.int32) for i in range(nsteps)]
+        s0 = structs[0]
+        workload = dict(workload="kslab_n256_nbl40_k16x16_E500", n=slab.n, nbl=s0.nbl,
+                        kgrid="16x16", nE=len(Egrid), eta=eta, overlap=True,
+                        items_per_step=per_step)
+    else:
+        s0, Egrid, eta = wl.config1() if a.config == "c1" else wl.config2(nbl=a.nbl or None)
+        ds = hip.DeviceStructure(s0)
+        structs = [s0]
+        nsteps = a.warmup + a.steps
+        per_step = a.items or len(Egrid)
+        steps_E = [Egrid[np.linspace(0, len(Egrid) - 1, per_step).astype(int)] for _ in range(nsteps)]
+        steps_slot = [np.zeros(per_step, dtype=np.int32) for _ in range(nsteps)]
+        workload = dict(workload=f"{a.config}_ribbon_W{s0.bsz[0]}_nbl{s0.nbl}_E{len(Egrid)}",
+                        n=s0.bsz[0], nbl=s0.nbl, nE=len(Egrid), eta=eta, items_per_step=per_step)
+
+    eng = hip.Engine(0)
+    L = hip.load()
+    ncont = s0.ncont
+    dev = torch.device("cuda:0")
+    dE = [torch.from_numpy(e).to(dev) for e in steps_E]
+    dS = [torch.from_numpy(s).to(dev) for s in steps_slot]
+    dT = torch.zeros(per_step, dtype=torch.float64, device=dev)
+    dN = torch.zeros(per_step * ncont, dtype=torch.int32, device=dev)
+    dSt = torch.zeros(per_step, dtype=torch.int32, device=dev)
+    opts = hip._opts(eta, 1, 2, prec, 0)
+    import ctypes as C
+
+    def step(i):
+        hip.check(L.hipnegf_solve_transmission_device(
+            eng.handle, ds.handle, per_step, C.c_void_p(dE[i].data_ptr()),
+            C.c_void_p(dS[i].data_ptr()), C.byref(opts), C.c_void_p(dT.data_ptr()),
+            C.c_void_p(dN.data_ptr()), C.c_void_p(dSt.data_ptr())))
+
+    results = {}
+    for i in range(a.warmup):
+        t0 = time.time()
+        step(i)
+        torch.cuda.synchronize()
+        log(f"warmup step {i}: {time.time() - t0:.3f}s")
+        results[i] = (dT.cpu().numpy().copy(), dN.cpu().numpy().copy(), dSt.cpu().numpy().copy())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(a.warmup, nsteps):
+        step(i)
+        if a.verbose:
+            torch.cuda.synchronize()
+            log(f"step {i}: {time.perf_counter() - t0:.3f}s cumulative")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    last = (dT.cpu().numpy().copy(), dN.cpu().numpy().reshape(per_step, ncont), dSt.cpu().numpy())
+    n_items = a.steps * per_step
+    value = n_items / elapsed
+    log(f"timed: {a.steps} steps x {per_step} items in {elapsed:.3f}s -> {value:.2f} points/s; "
+        f"status nonzero: {(last[2] != 0).sum()}; ncyc mean {last[1].mean():.1f}")
+
+    # ---------------------------------------------- unbatched FP64 GPU mode ----
+    plain = None
+    if a.plain_items > 0:
+        Es = steps_E[-1][:a.plain_items]
+        sl = steps_slot[-1][:a.plain_items]
+        hip.solve_transmission(eng, ds, Es[:1], slot=sl[:1], delta=eta, max_batch=1)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        hip.solve_transmission(eng, ds, Es, slot=sl, delta=eta, max_batch=1)
+        torch.cuda.synchronize()
+        plain = a.plain_items / (time.perf_counter() - t1)
+        log(f"plain (one item per launch) FP64: {plain:.3f} points/s")
+
+    # --------------------------------------------------------- CPU baseline ----
+    cpu = None
+    parity = None
+    if not a.no_cpu and a.cpu_items > 0:
+        from oracle import pyoracle
+        pyoracle.build()
+        kind = "port"
+        try:
+            backend = pyoracle.set_backend("mkl")
+        except RuntimeError:
+            backend = pyoracle.set_backend("builtin")
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
+        threads = min(threads, 16)
+        i_last = nsteps - 1
+        sel = np.linspace(0, per_step - 1, min(a.cpu_items, per_step)).astype(int)
+        s_last = structs[steps_slot[i_last][0]]
+        Tc, nc, secs = pyoracle.transmission_batch(s_last, steps_E[i_last][sel], eta, nthreads=threads)
+        cpu = dict(value=len(sel) / secs, unit="energy-points/s", cores=threads, kind=kind,
+                   sample=f"{len(sel)} (E,k) points of the last timed step (one k-point), "
+                          f"oracle with {backend}, OpenMP over points, 1 BLAS thread each",
+                   seconds=secs)
+        Tg = last[0][sel]
+        big = Tc > 1e-6 * Tc.max()
+        parity = dict(max_rel_err=float((np.abs(Tg[big] - Tc[big]) / Tc[big]).max()),
+                      ncyc_equal=bool((last[1][sel] == nc).all()), points=int(len(sel)))
+        log(f"cpu baseline: {cpu['value']:.3f} points/s on {threads} threads; parity {parity}")
+
+    out = {
+        "metric": "RGF energy-points/s",
+        "value": value,
+        "unit": "energy-points/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": 1000.0 * elapsed / a.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128" if prec == hip.FP64 else "c64+c128",
+        "data": "synthetic (seeded generators, libnegf_amd/workloads.py)",
+        "config": workload,
+        "precision": a.precision,
+        "fp64_plain_points_per_s": plain,
+        "cpu_baseline": cpu,
+        "parity": parity,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

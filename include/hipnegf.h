@@ -1,0 +1,233 @@
+/*
+ * hipnegf.h -- C ABI of the MI355X-native per-(E,k) Green's-function engine.
+ *
+ * Three layers, mirroring the three boundaries of the reference:
+ *
+ *  (1) Block operations: one-for-one replacements of the cu_* symbols that
+ *      src/cudautils.F90:236-431 binds from Fortran (implemented in the
+ *      reference by src/gpuroutines.cu).  Same argument order and meaning,
+ *      column-major interleaved complex, sizes as size_t, scalars by pointer,
+ *      int status return (0 = success).  Device pointers unless stated.
+ *      T = complex float (C variants) or complex double (Z variants), passed
+ *      as void* so no vendor complex type leaks into the ABI.
+ *
+ *  (2) Batched engine: entry points the reference lacks -- device-resident
+ *      structures with one slot per transverse k, batched transmission and
+ *      elastic-dephasing SCBA solves over many (E,k) items with per-item
+ *      status words, and an explicit precision mode.
+ *
+ *  (3) The public libNEGF C API subset of src/api/libnegf_api.f90 for the
+ *      Landauer / Meir-Wingreen / dephasing path (negf_* symbols), with the
+ *      same int handler[4] convention.  Declared in hipnegf_api.h.
+ *
+ * All functions are thread-compatible; one engine handle must not be used
+ * from two threads at once.
+ */
+#ifndef HIPNEGF_H
+#define HIPNEGF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Opaque context.  Stands in for both cublasHandle and cusolverDnHandle of
+ * the reference (lib_param.F90:90-96, 164-165): one stream + workspace. */
+typedef struct hipnegf_ctx *hipnegf_handle;
+
+/* ------------------------------------------------------------------------ */
+/* (1) block operations                                                      */
+/* ------------------------------------------------------------------------ */
+
+/* memory: cu_createMat / cu_deleteMat / cu_cudaMallocAsync / cu_cudaFreeAsync /
+ * cu_copyMatH2D / cu_copyMatH2D_async / cu_copyMatD2H / cu_meminfo
+ * (gpuroutines.cu:248-304, 989-996; bound at cudautils.F90:236-431).
+ * delete/free null *d on success. */
+int hipnegf_createMat(void **d, size_t bytes);
+int hipnegf_deleteMat(void **d);
+int hipnegf_mallocAsync(void **d, size_t bytes);
+int hipnegf_freeAsync(void **d);
+int hipnegf_copyMatH2D(void *h, void *d, size_t bytes);
+int hipnegf_copyMatH2D_async(void *h, void *d, size_t bytes);
+int hipnegf_copyMatD2H(void *h, void *d, size_t bytes);
+int hipnegf_meminfo(size_t *free_bytes, size_t *total_bytes);
+/* cudaDeviceSynchronize bound directly at cudautils.F90:261-263 */
+int hipnegf_deviceSynchronize(void);
+
+/* device / handles: cu_cudaGetDeviceCount, cu_cudaGetDevice, cu_cudaSetDevice,
+ * cu_cublasInit/Finalize, cu_cusolverInit/Finalize (gpuroutines.cu:311-374).
+ * Init creates a context on the current device; Finalize destroys it. */
+int hipnegf_getDeviceCount(int *count);
+int hipnegf_getDevice(int *device);
+int hipnegf_setDevice(int device);
+int hipnegf_blasInit(hipnegf_handle *h);
+int hipnegf_blasFinalize(hipnegf_handle h);
+int hipnegf_solverInit(hipnegf_handle *h);
+int hipnegf_solverFinalize(hipnegf_handle h);
+
+/* C = alpha op(A) op(B) + beta C;  cu_{C,Z}multMat (gpuroutines.cu:395-446):
+ * dagger 0: op = N/N (A m x k, B k x n); 1: op(A) = A^H (A stored k x m);
+ * 2: op(B) = B^H (B stored n x k). */
+int hipnegf_CmultMat(hipnegf_handle h, size_t m, size_t n, size_t k, const void *alpha,
+                     const void *A, const void *B, const void *beta, void *C, int dagger);
+int hipnegf_ZmultMat(hipnegf_handle h, size_t m, size_t n, size_t k, const void *alpha,
+                     const void *A, const void *B, const void *beta, void *C, int dagger);
+
+/* Ainv = inverse(A), n x n, A unchanged; cu_{C,Z}inverse (gpuroutines.cu:449-523).
+ * The reference's caller pre-sets Ainv = I (cudautils.F90:650); not required here.
+ * Returns 0, or 1 if the matrix is exactly singular (no process exit). */
+int hipnegf_Cinverse(hipnegf_handle hb, hipnegf_handle hs, void *A, void *Ainv, size_t n);
+int hipnegf_Zinverse(hipnegf_handle hb, hipnegf_handle hs, void *A, void *Ainv, size_t n);
+
+/* C = alpha op(A) + beta op(B); cu_{C,Z}matsum (gpuroutines.cu:567-617):
+ * dagger 0: A, B m x n; 1: op(A) = A^H (A stored n x m); 2: op(B) = B^H. */
+int hipnegf_Cmatsum(hipnegf_handle h, size_t m, size_t n, const void *alpha, const void *A,
+                    const void *beta, const void *B, void *C, int dagger);
+int hipnegf_Zmatsum(hipnegf_handle h, size_t m, size_t n, const void *alpha, const void *A,
+                    const void *beta, const void *B, void *C, int dagger);
+
+/* A = identity (n x n); cu_{C,Z}initmat (gpuroutines.cu:619-639). */
+int hipnegf_Cinitmat(void *A, size_t n);
+int hipnegf_Zinitmat(void *A, size_t n);
+
+/* sum_i Re A(i,i) [over mask]; cu_{C,Z}trace (gpuroutines.cu:642-694).
+ * h_mask: host array of n int (Fortran logical, 4 bytes) -- the reference
+ * declares bool here, a width mismatch with the Fortran caller; int is what
+ * the Fortran side actually passes. */
+float hipnegf_Ctrace(hipnegf_handle h, void *A, size_t n, void *h_mask, int mask_present);
+double hipnegf_Ztrace(hipnegf_handle h, void *A, size_t n, void *h_mask, int mask_present);
+
+/* B = A (nelem complex); cu_{C,Z}matcopy (gpuroutines.cu:697-721). */
+int hipnegf_Cmatcopy(hipnegf_handle h, const void *A, void *B, size_t nelem);
+int hipnegf_Zmatcopy(hipnegf_handle h, const void *A, void *B, size_t nelem);
+
+/* *sum = sum |Re| + |Im| over n elements; cu_{C,Z}asum (gpuroutines.cu:723-739). */
+int hipnegf_Casum(hipnegf_handle h, void *A, float *sum, size_t n);
+int hipnegf_Zasum(hipnegf_handle h, void *A, double *sum, size_t n);
+
+/* Sancho-Rubio decimation with HOST matrices, cu_{C,Z}decimation
+ * (gpuroutines.cu:742-986, called via decimation_gpu, contselfenergy.F90:347-379):
+ * Go_out = surface Green's function, Ao/Bo/Co n x n (read only here),
+ * convergence as the reference GPU path: sum(|Re|+|Im|)(C1) <= SGFACC on two
+ * consecutive iterations (the Fortran caller passes SGFACC*n*n), <= 300
+ * iterations.  tf32 is accepted and ignored (no TF32 on CDNA4).
+ * Returns 0, or 2 if not converged (Go_out then holds the last iterate). */
+int hipnegf_Cdecimation(hipnegf_handle hb, hipnegf_handle hs, void *h_Go_out, void *h_Ao,
+                        void *h_Bo, void *h_Co, size_t n, int tf32, int *ncyc, float SGFACC);
+int hipnegf_Zdecimation(hipnegf_handle hb, hipnegf_handle hs, void *h_Go_out, void *h_Ao,
+                        void *h_Bo, void *h_Co, size_t n, int tf32, int *ncyc, double SGFACC);
+
+/* ------------------------------------------------------------------------ */
+/* (2) batched engine (no counterpart in the reference)                      */
+/* ------------------------------------------------------------------------ */
+
+typedef struct hipnegf_engine_s *hipnegf_engine;
+typedef struct hipnegf_struct_s *hipnegf_struct;
+
+/* block kinds for hipnegf_struct_set_block */
+enum hipnegf_block_kind {
+  HIPNEGF_HD = 0,  /* H(i,i)        bsz[i] x bsz[i]     (i = 0..nbl-1)     */
+  HIPNEGF_SD = 1,  /* S(i,i)                                                 */
+  HIPNEGF_HU = 2,  /* H(i,i+1)      bsz[i] x bsz[i+1]   (i = 0..nbl-2)     */
+  HIPNEGF_SU = 3,
+  HIPNEGF_HL = 4,  /* H(i+1,i)      bsz[i+1] x bsz[i]                      */
+  HIPNEGF_SL = 5,
+  HIPNEGF_H00 = 6, /* contact c: PL1                nc x nc                 */
+  HIPNEGF_S00 = 7,
+  HIPNEGF_H01 = 8, /* contact c: PL1 -> PL2 (away from the device)          */
+  HIPNEGF_S01 = 9,
+  HIPNEGF_HMC = 10, /* contact c: device block cblk -> PL1   bsz x nc        */
+  HIPNEGF_SMC = 11
+};
+/* smode bits: 1 = S(i,i) given (else identity), 2 = S(i,i+1)/S(i+1,i) given
+ * (else zero), 4 = contact S blocks given (else S00 = I, S01 = SMC = 0). */
+enum hipnegf_smode { HIPNEGF_S_DIAG = 1, HIPNEGF_S_OFFDIAG = 2, HIPNEGF_S_CONTACT = 4 };
+enum hipnegf_precision { HIPNEGF_FP64 = 0, HIPNEGF_MIXED = 1 };
+/* per-item status bits (replace process exit / __trap in the reference) */
+enum hipnegf_status {
+  HIPNEGF_ST_OK = 0,
+  HIPNEGF_ST_SINGULAR = 1,
+  HIPNEGF_ST_NOT_CONVERGED = 2,
+  HIPNEGF_ST_NONFINITE = 4
+};
+
+int hipnegf_engine_create(int device, hipnegf_engine *e);
+int hipnegf_engine_destroy(hipnegf_engine e);
+/* last error message of the calling thread ("" if none) */
+const char *hipnegf_last_error(void);
+
+/* Device structure: nbl principal layers of sizes bsz[nbl] (the PL partition
+ * of ln_structure.F90:80-150), ncont contacts attached to 1-based blocks
+ * cblk[ncont] with PL size nc[ncont], nslots identical slots (e.g. one per k). */
+int hipnegf_struct_create(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
+                          int smode, int nslots, hipnegf_struct *st);
+int hipnegf_struct_destroy(hipnegf_struct st);
+/* copy one block (column-major interleaved complex double) from host memory */
+int hipnegf_struct_set_block(hipnegf_struct st, int slot, int kind, int index,
+                             const double *host);
+/* same from device memory */
+int hipnegf_struct_set_block_device(hipnegf_struct st, int slot, int kind, int index,
+                                    const void *dev);
+
+/* Options of a batched solve (zero-initialise, then set what differs). */
+typedef struct {
+  double delta;      /* eta added to E on the contacts (negf%delta)            */
+  int ni, nf;        /* 1-based contact pair of the transmission              */
+  int precision;     /* hipnegf_precision                                      */
+  int max_batch;     /* 0: automatic from the HBM budget                       */
+  int ns_steps;      /* mixed: FP64 Newton-Schulz steps per block inverse      */
+  int polish;        /* mixed: FP64 decimation iterations after the FP32 ones  */
+  double eps_drop;   /* EPS drop tolerance (ln_precision), default 0           */
+} hipnegf_opts;
+
+/* Coherent transmission T(E_i, slot_i), i < nitems (iterative.F90:1669-1780
+ * per item).  Host arrays: E[nitems], slot[nitems] in; T[nitems],
+ * ncyc[nitems*ncont] (decimation iterations, may be NULL), status[nitems] out. */
+int hipnegf_solve_transmission(hipnegf_engine e, hipnegf_struct st, int nitems,
+                               const double *E, const int *slot, const hipnegf_opts *o,
+                               double *T, int *ncyc, int *status);
+/* Same with all arrays in device memory (no host synchronisation of inputs). */
+int hipnegf_solve_transmission_device(hipnegf_engine e, hipnegf_struct st, int nitems,
+                                      const double *dE, const int *dslot,
+                                      const hipnegf_opts *o, double *dT, int *dncyc,
+                                      int *dstatus);
+
+/* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):
+ * calculate_elastic_scba (iterative.F90:289-363) with the ElPhonDephD model
+ * (elphdd.F90; coupling per device orbital, squared internally), then
+ * iterative_meir_wingreen (iterative.F90:382-414).
+ * frm[ncont+1] occupations (frm[ref-1] is the reference, ref = ncont+1 for
+ * none) -- or per-item occupations if frm_per_item != 0 (frm[nitems*(ncont+1)]).
+ * Out: curr[nitems*ncont], iters[nitems], err[nitems], status[nitems];
+ * gn_diag / gr_diag [nitems*ndev] complex (may be NULL). */
+int hipnegf_solve_scba_mw(hipnegf_engine e, hipnegf_struct st, int nitems, const double *E,
+                          const int *slot, const hipnegf_opts *o, const double *coupling,
+                          int niter, double tol, const double *frm, int frm_per_item, int ref,
+                          double *curr, int *iters, double *err, int *status, double *gn_diag,
+                          double *gr_diag);
+
+/* Batched kernels for tests and callers with their own drivers (device
+ * pointers, item b at X + b*strideX elements): */
+int hipnegf_zgemm_batched(hipnegf_engine e, int m, int n, int k, const double *alpha,
+                          const void *A, long long sA, int lda, int opA, const void *B,
+                          long long sB, int ldb, int opB, const double *beta, void *C,
+                          long long sC, int ldc, int batch);
+int hipnegf_cgemm_batched(hipnegf_engine e, int m, int n, int k, const double *alpha,
+                          const void *A, long long sA, int lda, int opA, const void *B,
+                          long long sB, int ldb, int opB, const double *beta, void *C,
+                          long long sC, int ldc, int batch);
+/* Ainv_b = inverse(A_b); A is overwritten (working matrix). status[batch] out (device). */
+int hipnegf_zinverse_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv,
+                             long long sAinv, int batch, int *dstatus);
+int hipnegf_cinverse_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv,
+                             long long sAinv, int batch, int *dstatus);
+int hipnegf_engine_sync(hipnegf_engine e);
+/* stream the engine launches on (a hipStream_t) */
+void *hipnegf_engine_stream(hipnegf_engine e);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPNEGF_H */

@@ -1,0 +1,563 @@
+// hn_capi.hip -- extern "C" entry points declared in include/hipnegf.h.
+//
+// Block operations keep the synchronous semantics the reference's Fortran
+// relies on (blocking copies, host-returning reductions: gpuroutines.cu:248-996):
+// every compute call enqueues on the context's stream and waits for it.
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hipnegf.h"
+#include "hn_engine.h"
+#include "hn_gemm.h"
+#include "hn_inv.h"
+#include "hn_kernels.h"
+
+struct hipnegf_ctx {
+  std::unique_ptr<hn::Engine> eng;
+};
+struct hipnegf_engine_s {
+  std::unique_ptr<hn::Engine> eng;
+};
+struct hipnegf_struct_s {
+  std::unique_ptr<hn::DevStruct> st;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+template <typename F> int guard(F &&f) {
+  try {
+    g_err.clear();
+    return f();
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  } catch (...) {
+    g_err = "unknown error";
+    return -1;
+  }
+}
+
+constexpr int NT = 256;
+
+template <typename C>
+__global__ void k_matsum(int m, int n, C alpha, const C *A, C beta, const C *B, C *Cm,
+                         int dagger) {
+  long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)m * n) return;
+  int r = (int)(e % m), c = (int)(e / m);
+  C a = (dagger == 1) ? A[c + (long long)r * n] : A[e];
+  if (dagger == 1) a.y = -a.y;
+  C b = (dagger == 2) ? B[c + (long long)r * n] : B[e];
+  if (dagger == 2) b.y = -b.y;
+  C out;
+  out.x = alpha.x * a.x - alpha.y * a.y + beta.x * b.x - beta.y * b.y;
+  out.y = alpha.x * a.y + alpha.y * a.x + beta.x * b.y + beta.y * b.x;
+  // in-place aliasing (A or B == C) is safe only when op is N for that operand
+  // (gpuroutines.cu:576-577); the element read above happens before the write.
+  Cm[e] = out;
+}
+
+template <typename C> __global__ void k_initmat(C *A, int n) {
+  long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  int r = (int)(e % n), c = (int)(e / n);
+  A[e] = C{(r == c) ? 1.0f : 0.0f, 0.0f};
+}
+
+template <typename C, typename R>
+__global__ void k_trace(const C *A, int n, const int *mask, R *out) {
+  __shared__ double sh[NT];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += NT)
+    if (!mask || mask[i]) s += (double)A[i + (long long)i * n].x;
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = (R)sh[0];
+}
+
+template <typename C, typename R>
+__global__ void k_asum(const C *A, long long n, R *out) {
+  __shared__ double sh[NT];
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += NT) s += fabs((double)A[i].x) + fabs((double)A[i].y);
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = NT / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = (R)sh[0];
+}
+
+inline unsigned nblk(long long n) { return (unsigned)((n + NT - 1) / NT); }
+
+hipStream_t sof(hipnegf_handle h) { return h && h->eng ? h->eng->stream() : (hipStream_t)0; }
+
+template <typename R> struct CT_;
+template <> struct CT_<double> { typedef double2 t; };
+template <> struct CT_<float> { typedef float2 t; };
+
+template <typename R>
+int multmat(hipnegf_handle h, size_t m, size_t n, size_t k, const void *alpha, const void *A,
+            const void *B, const void *beta, void *C, int dagger) {
+  typedef typename CT_<R>::t CT;
+  if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX) throw std::runtime_error("size > INT_MAX");
+  hn::GemmArgs<R> g;
+  g.m = (int)m; g.n = (int)n; g.k = (int)k;
+  g.A = (const CT *)A; g.B = (const CT *)B; g.C = (CT *)C; g.ldc = (int)m;
+  const CT *al = (const CT *)alpha, *be = (const CT *)beta;
+  g.alpha = make_double2(al->x, al->y);
+  g.beta = make_double2(be->x, be->y);
+  if (dagger == 0) { g.opA = hn::OP_N; g.lda = (int)m; g.opB = hn::OP_N; g.ldb = (int)k; }
+  else if (dagger == 1) { g.opA = hn::OP_C; g.lda = (int)k; g.opB = hn::OP_N; g.ldb = (int)k; }
+  else if (dagger == 2) { g.opA = hn::OP_N; g.lda = (int)m; g.opB = hn::OP_C; g.ldb = (int)n; }
+  else throw std::runtime_error("dagger must be 0, 1 or 2");
+  g.batch = 1;
+  if constexpr (sizeof(R) == 8) hn::gemm_z(g, sof(h));
+  else hn::gemm_c(g, sof(h));
+  HN_CHECK(hipStreamSynchronize(sof(h)));
+  return 0;
+}
+
+template <typename R>
+int inverse1(hipnegf_handle h, void *A, void *Ainv, size_t n) {
+  typedef typename CT_<R>::t CT;
+  if (n == 0) return 0;
+  hipStream_t s = sof(h);
+  const size_t bytes = n * n * sizeof(CT);
+  CT *work = nullptr;
+  int *status = nullptr;
+  HN_CHECK(hipMallocAsync((void **)&work, bytes, s));
+  HN_CHECK(hipMallocAsync((void **)&status, sizeof(int), s));
+  HN_CHECK(hipMemcpyAsync(work, A, bytes, hipMemcpyDeviceToDevice, s));
+  hn::fill_int(status, 1, 0, s);
+  hn::InvWork<R> w;
+  void *wb = nullptr;
+  if ((int)n > hn::inv_lds_max_n(sizeof(R))) {
+    HN_CHECK(hipMallocAsync(&wb, hn::inv_work_bytes((int)n, 1, sizeof(R)), s));
+    w = hn::inv_work_carve<R>(wb, (int)n, 1);
+  }
+  if constexpr (sizeof(R) == 8)
+    hn::inv_z((int)n, work, (long long)n * n, (CT *)Ainv, (long long)n * n, 1, nullptr, status, w, s);
+  else
+    hn::inv_c((int)n, work, (long long)n * n, (CT *)Ainv, (long long)n * n, 1, nullptr, status, w, s);
+  int hs = 0;
+  HN_CHECK(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, s));
+  HN_CHECK(hipFreeAsync(work, s));
+  HN_CHECK(hipFreeAsync(status, s));
+  if (wb) HN_CHECK(hipFreeAsync(wb, s));
+  HN_CHECK(hipStreamSynchronize(s));
+  return hs ? 1 : 0;
+}
+
+template <typename R>
+int matsum(hipnegf_handle h, size_t m, size_t n, const void *alpha, const void *A,
+           const void *beta, const void *B, void *C, int dagger) {
+  typedef typename CT_<R>::t CT;
+  if (dagger < 0 || dagger > 2) throw std::runtime_error("dagger must be 0, 1 or 2");
+  if (A == C && dagger == 1) throw std::runtime_error("matsum: A aliases C with op(A) = A^H");
+  if (B == C && dagger == 2) throw std::runtime_error("matsum: B aliases C with op(B) = B^H");
+  hipLaunchKernelGGL(k_matsum<CT>, dim3(nblk((long long)m * n)), dim3(NT), 0, sof(h), (int)m,
+                     (int)n, *(const CT *)alpha, (const CT *)A, *(const CT *)beta,
+                     (const CT *)B, (CT *)C, dagger);
+  HN_CHECK(hipGetLastError());
+  HN_CHECK(hipStreamSynchronize(sof(h)));
+  return 0;
+}
+
+template <typename R>
+R trace1(hipnegf_handle h, void *A, size_t n, void *h_mask, int mask_present) {
+  typedef typename CT_<R>::t CT;
+  hipStream_t s = sof(h);
+  R *dout = nullptr;
+  int *dmask = nullptr;
+  HN_CHECK(hipMallocAsync((void **)&dout, sizeof(R), s));
+  if (mask_present && h_mask) {
+    HN_CHECK(hipMallocAsync((void **)&dmask, n * sizeof(int), s));
+    HN_CHECK(hipMemcpyAsync(dmask, h_mask, n * sizeof(int), hipMemcpyHostToDevice, s));
+  }
+  hipLaunchKernelGGL((k_trace<CT, R>), dim3(1), dim3(NT), 0, s, (const CT *)A, (int)n, dmask, dout);
+  HN_CHECK(hipGetLastError());
+  R out = 0;
+  HN_CHECK(hipMemcpyAsync(&out, dout, sizeof(R), hipMemcpyDeviceToHost, s));
+  HN_CHECK(hipFreeAsync(dout, s));
+  if (dmask) HN_CHECK(hipFreeAsync(dmask, s));
+  HN_CHECK(hipStreamSynchronize(s));
+  return out;
+}
+
+template <typename R>
+int asum1(hipnegf_handle h, void *A, R *sum, size_t n) {
+  typedef typename CT_<R>::t CT;
+  hipStream_t s = sof(h);
+  R *dout = nullptr;
+  HN_CHECK(hipMallocAsync((void **)&dout, sizeof(R), s));
+  hipLaunchKernelGGL((k_asum<CT, R>), dim3(1), dim3(NT), 0, s, (const CT *)A, (long long)n, dout);
+  HN_CHECK(hipGetLastError());
+  HN_CHECK(hipMemcpyAsync(sum, dout, sizeof(R), hipMemcpyDeviceToHost, s));
+  HN_CHECK(hipFreeAsync(dout, s));
+  HN_CHECK(hipStreamSynchronize(s));
+  return 0;
+}
+
+template <typename R>
+int inv_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv, long long sAinv,
+                       int batch, int *dstatus) {
+  typedef typename CT_<R>::t CT;
+  HN_CHECK(hipDeviceSynchronize());
+  hipStream_t s = e->eng->stream();
+  hn::fill_int(dstatus, batch, 0, s);
+  hn::InvWork<R> w;
+  void *wb = nullptr;
+  if (n > hn::inv_lds_max_n(sizeof(R))) {
+    HN_CHECK(hipMalloc(&wb, hn::inv_work_bytes(n, batch, sizeof(R))));
+    w = hn::inv_work_carve<R>(wb, n, batch);
+  }
+  if constexpr (sizeof(R) == 8)
+    hn::inv_z(n, (CT *)A, sA, (CT *)Ainv, sAinv, batch, nullptr, dstatus, w, s);
+  else
+    hn::inv_c(n, (CT *)A, sA, (CT *)Ainv, sAinv, batch, nullptr, dstatus, w, s);
+  HN_CHECK(hipStreamSynchronize(s));
+  if (wb) HN_CHECK(hipFree(wb));
+  return 0;
+}
+
+
+hn::SolveOpts to_opts(const hipnegf_opts *o) {
+  hn::SolveOpts so;
+  if (o) {
+    so.delta = o->delta;
+    so.ni = o->ni > 0 ? o->ni : 1;
+    so.nf = o->nf > 0 ? o->nf : 2;
+    so.precision = o->precision;
+    so.max_batch = o->max_batch;
+    if (o->ns_steps > 0) so.ns_steps = o->ns_steps;
+    if (o->polish > 0) so.polish = o->polish;
+    so.eps_drop = o->eps_drop;
+  }
+  return so;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *hipnegf_last_error(void) { return g_err.c_str(); }
+
+// ---- memory ---------------------------------------------------------------
+int hipnegf_createMat(void **d, size_t bytes) {
+  return guard([&] { HN_CHECK(hipMalloc(d, bytes)); return 0; });
+}
+int hipnegf_deleteMat(void **d) {
+  return guard([&] { HN_CHECK(hipFree(*d)); *d = nullptr; return 0; });
+}
+int hipnegf_mallocAsync(void **d, size_t bytes) {
+  return guard([&] { HN_CHECK(hipMallocAsync(d, bytes, 0)); return 0; });
+}
+int hipnegf_freeAsync(void **d) {
+  return guard([&] { HN_CHECK(hipFreeAsync(*d, 0)); *d = nullptr; return 0; });
+}
+int hipnegf_copyMatH2D(void *h, void *d, size_t bytes) {
+  return guard([&] { HN_CHECK(hipMemcpy(d, h, bytes, hipMemcpyHostToDevice)); return 0; });
+}
+int hipnegf_copyMatH2D_async(void *h, void *d, size_t bytes) {
+  return guard([&] { HN_CHECK(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, 0)); return 0; });
+}
+int hipnegf_copyMatD2H(void *h, void *d, size_t bytes) {
+  return guard([&] { HN_CHECK(hipMemcpy(h, d, bytes, hipMemcpyDeviceToHost)); return 0; });
+}
+int hipnegf_meminfo(size_t *free_bytes, size_t *total_bytes) {
+  return guard([&] { HN_CHECK(hipMemGetInfo(free_bytes, total_bytes)); return 0; });
+}
+int hipnegf_deviceSynchronize(void) {
+  return guard([&] { HN_CHECK(hipDeviceSynchronize()); return 0; });
+}
+
+// ---- device / handles -------------------------------------------------------
+int hipnegf_getDeviceCount(int *count) {
+  return guard([&] { HN_CHECK(hipGetDeviceCount(count)); return 0; });
+}
+int hipnegf_getDevice(int *device) {
+  return guard([&] { HN_CHECK(hipGetDevice(device)); return 0; });
+}
+int hipnegf_setDevice(int device) {
+  return guard([&] { HN_CHECK(hipSetDevice(device)); return 0; });
+}
+int hipnegf_blasInit(hipnegf_handle *h) {
+  return guard([&] {
+    int dev = 0;
+    HN_CHECK(hipGetDevice(&dev));
+    auto *c = new hipnegf_ctx;
+    c->eng.reset(new hn::Engine(dev));
+    *h = c;
+    return 0;
+  });
+}
+int hipnegf_blasFinalize(hipnegf_handle h) {
+  return guard([&] { delete h; return 0; });
+}
+int hipnegf_solverInit(hipnegf_handle *h) { return hipnegf_blasInit(h); }
+int hipnegf_solverFinalize(hipnegf_handle h) { return hipnegf_blasFinalize(h); }
+
+// ---- compute ----------------------------------------------------------------
+int hipnegf_CmultMat(hipnegf_handle h, size_t m, size_t n, size_t k, const void *alpha,
+                     const void *A, const void *B, const void *beta, void *C, int dagger) {
+  return guard([&] { return multmat<float>(h, m, n, k, alpha, A, B, beta, C, dagger); });
+}
+int hipnegf_ZmultMat(hipnegf_handle h, size_t m, size_t n, size_t k, const void *alpha,
+                     const void *A, const void *B, const void *beta, void *C, int dagger) {
+  return guard([&] { return multmat<double>(h, m, n, k, alpha, A, B, beta, C, dagger); });
+}
+int hipnegf_Cinverse(hipnegf_handle hb, hipnegf_handle, void *A, void *Ainv, size_t n) {
+  return guard([&] { return inverse1<float>(hb, A, Ainv, n); });
+}
+int hipnegf_Zinverse(hipnegf_handle hb, hipnegf_handle, void *A, void *Ainv, size_t n) {
+  return guard([&] { return inverse1<double>(hb, A, Ainv, n); });
+}
+int hipnegf_Cmatsum(hipnegf_handle h, size_t m, size_t n, const void *alpha, const void *A,
+                    const void *beta, const void *B, void *C, int dagger) {
+  return guard([&] { return matsum<float>(h, m, n, alpha, A, beta, B, C, dagger); });
+}
+int hipnegf_Zmatsum(hipnegf_handle h, size_t m, size_t n, const void *alpha, const void *A,
+                    const void *beta, const void *B, void *C, int dagger) {
+  return guard([&] { return matsum<double>(h, m, n, alpha, A, beta, B, C, dagger); });
+}
+int hipnegf_Cinitmat(void *A, size_t n) {
+  return guard([&] {
+    hipLaunchKernelGGL(k_initmat<float2>, dim3(nblk((long long)n * n)), dim3(NT), 0, 0, (float2 *)A, (int)n);
+    HN_CHECK(hipGetLastError());
+    HN_CHECK(hipStreamSynchronize(0));
+    return 0;
+  });
+}
+int hipnegf_Zinitmat(void *A, size_t n) {
+  return guard([&] {
+    hipLaunchKernelGGL(k_initmat<double2>, dim3(nblk((long long)n * n)), dim3(NT), 0, 0, (double2 *)A, (int)n);
+    HN_CHECK(hipGetLastError());
+    HN_CHECK(hipStreamSynchronize(0));
+    return 0;
+  });
+}
+float hipnegf_Ctrace(hipnegf_handle h, void *A, size_t n, void *h_mask, int mask_present) {
+  float r = 0.0f;
+  guard([&] { r = trace1<float>(h, A, n, h_mask, mask_present); return 0; });
+  return r;
+}
+double hipnegf_Ztrace(hipnegf_handle h, void *A, size_t n, void *h_mask, int mask_present) {
+  double r = 0.0;
+  guard([&] { r = trace1<double>(h, A, n, h_mask, mask_present); return 0; });
+  return r;
+}
+int hipnegf_Cmatcopy(hipnegf_handle h, const void *A, void *B, size_t nelem) {
+  return guard([&] {
+    HN_CHECK(hipMemcpyAsync(B, A, nelem * sizeof(float2), hipMemcpyDeviceToDevice, sof(h)));
+    HN_CHECK(hipStreamSynchronize(sof(h)));
+    return 0;
+  });
+}
+int hipnegf_Zmatcopy(hipnegf_handle h, const void *A, void *B, size_t nelem) {
+  return guard([&] {
+    HN_CHECK(hipMemcpyAsync(B, A, nelem * sizeof(double2), hipMemcpyDeviceToDevice, sof(h)));
+    HN_CHECK(hipStreamSynchronize(sof(h)));
+    return 0;
+  });
+}
+int hipnegf_Casum(hipnegf_handle h, void *A, float *sum, size_t n) {
+  return guard([&] { return asum1<float>(h, A, sum, n); });
+}
+int hipnegf_Zasum(hipnegf_handle h, void *A, double *sum, size_t n) {
+  return guard([&] { return asum1<double>(h, A, sum, n); });
+}
+
+int hipnegf_Zdecimation(hipnegf_handle hb, hipnegf_handle, void *h_Go_out, void *h_Ao,
+                        void *h_Bo, void *h_Co, size_t n, int tf32, int *ncyc, double SGFACC) {
+  (void)tf32;
+  return guard([&] {
+    int st = hb->eng->decimation_host((int)n, (double2 *)h_Go_out, (const double2 *)h_Ao,
+                                      (const double2 *)h_Bo, (const double2 *)h_Co, SGFACC, 1, ncyc);
+    return (st & hn::ST_NOT_CONVERGED) ? 2 : (st ? 1 : 0);
+  });
+}
+int hipnegf_Cdecimation(hipnegf_handle hb, hipnegf_handle, void *h_Go_out, void *h_Ao,
+                        void *h_Bo, void *h_Co, size_t n, int tf32, int *ncyc, float SGFACC) {
+  (void)tf32;
+  return guard([&] {
+    // complex-float host data: widen, run the FP64 iteration, narrow the result
+    size_t n2 = n * n;
+    std::vector<double2> a(n2), b(n2), c(n2), g(n2);
+    const float2 *fa = (const float2 *)h_Ao, *fb = (const float2 *)h_Bo, *fc = (const float2 *)h_Co;
+    for (size_t i = 0; i < n2; ++i) {
+      a[i] = make_double2(fa[i].x, fa[i].y);
+      b[i] = make_double2(fb[i].x, fb[i].y);
+      c[i] = make_double2(fc[i].x, fc[i].y);
+    }
+    int st = hb->eng->decimation_host((int)n, g.data(), a.data(), b.data(), c.data(), SGFACC, 1, ncyc);
+    float2 *go = (float2 *)h_Go_out;
+    for (size_t i = 0; i < n2; ++i) go[i] = make_float2((float)g[i].x, (float)g[i].y);
+    return (st & hn::ST_NOT_CONVERGED) ? 2 : (st ? 1 : 0);
+  });
+}
+
+// ---- batched engine ------------------------------------------------------------
+int hipnegf_engine_create(int device, hipnegf_engine *e) {
+  return guard([&] {
+    auto *x = new hipnegf_engine_s;
+    x->eng.reset(new hn::Engine(device));
+    *e = x;
+    return 0;
+  });
+}
+int hipnegf_engine_destroy(hipnegf_engine e) {
+  return guard([&] { delete e; return 0; });
+}
+int hipnegf_engine_sync(hipnegf_engine e) {
+  return guard([&] { HN_CHECK(hipStreamSynchronize(e->eng->stream())); return 0; });
+}
+void *hipnegf_engine_stream(hipnegf_engine e) { return (void *)e->eng->stream(); }
+
+int hipnegf_struct_create(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
+                          int smode, int nslots, hipnegf_struct *st) {
+  return guard([&] {
+    auto *x = new hipnegf_struct_s;
+    x->st.reset(new hn::DevStruct(nbl, bsz, ncont, cblk, nc, smode, nslots));
+    *st = x;
+    return 0;
+  });
+}
+int hipnegf_struct_destroy(hipnegf_struct st) {
+  return guard([&] { delete st; return 0; });
+}
+
+double2 *block_ptr(hn::DevStruct &s, int slot, int kind, int index, long long *n) {
+  if (slot < 0 || slot >= s.nslots) throw std::runtime_error("slot out of range");
+  if (kind < 0 || kind >= hn::BK_COUNT) throw std::runtime_error("bad block kind");
+  if (s.off[kind].empty()) throw std::runtime_error("block kind not present (smode)");
+  if (index < 0 || index >= (int)s.off[kind].size()) throw std::runtime_error("block index out of range");
+  *n = s.block_elems(kind, index);
+  return s.pool + (long long)slot * s.slot_elems + s.off[kind][index];
+}
+
+int hipnegf_struct_set_block(hipnegf_struct st, int slot, int kind, int index, const double *host) {
+  return guard([&] {
+    long long n = 0;
+    double2 *d = block_ptr(*st->st, slot, kind, index, &n);
+    HN_CHECK(hipMemcpy(d, host, n * sizeof(double2), hipMemcpyHostToDevice));
+    return 0;
+  });
+}
+int hipnegf_struct_set_block_device(hipnegf_struct st, int slot, int kind, int index,
+                                    const void *dev) {
+  return guard([&] {
+    long long n = 0;
+    double2 *d = block_ptr(*st->st, slot, kind, index, &n);
+    HN_CHECK(hipMemcpy(d, dev, n * sizeof(double2), hipMemcpyDeviceToDevice));
+    return 0;
+  });
+}
+
+int hipnegf_solve_transmission_device(hipnegf_engine e, hipnegf_struct st, int nitems,
+                                      const double *dE, const int *dslot,
+                                      const hipnegf_opts *o, double *dT, int *dncyc,
+                                      int *dstatus) {
+  return guard([&] {
+    e->eng->transmission(*st->st, nitems, dE, dslot, to_opts(o), dT, dncyc, dstatus);
+    return 0;
+  });
+}
+
+int hipnegf_solve_transmission(hipnegf_engine e, hipnegf_struct st, int nitems,
+                               const double *E, const int *slot, const hipnegf_opts *o,
+                               double *T, int *ncyc, int *status) {
+  return guard([&] {
+    if (nitems <= 0) return 0;
+    for (int i = 0; i < nitems; ++i)
+      if (slot[i] < 0 || slot[i] >= st->st->nslots) throw std::runtime_error("item slot out of range");
+    const int nc = std::max(st->st->ncont, 1);
+    double *dE = nullptr, *dT = nullptr;
+    int *ds = nullptr, *dn = nullptr, *dst = nullptr;
+    HN_CHECK(hipMalloc(&dE, sizeof(double) * nitems));
+    HN_CHECK(hipMalloc(&dT, sizeof(double) * nitems));
+    HN_CHECK(hipMalloc(&ds, sizeof(int) * nitems));
+    HN_CHECK(hipMalloc(&dn, sizeof(int) * nitems * nc));
+    HN_CHECK(hipMalloc(&dst, sizeof(int) * nitems));
+    struct F { void *p[5]; ~F() { for (void *q : p) if (q) (void)hipFree(q); } } fr{{dE, dT, ds, dn, dst}};
+    HN_CHECK(hipMemcpy(dE, E, sizeof(double) * nitems, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(ds, slot, sizeof(int) * nitems, hipMemcpyHostToDevice));
+    e->eng->transmission(*st->st, nitems, dE, ds, to_opts(o), dT, dn, dst);
+    HN_CHECK(hipMemcpy(T, dT, sizeof(double) * nitems, hipMemcpyDeviceToHost));
+    if (ncyc) HN_CHECK(hipMemcpy(ncyc, dn, sizeof(int) * nitems * nc, hipMemcpyDeviceToHost));
+    if (status) HN_CHECK(hipMemcpy(status, dst, sizeof(int) * nitems, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+int hipnegf_solve_scba_mw(hipnegf_engine e, hipnegf_struct st, int nitems, const double *E,
+                          const int *slot, const hipnegf_opts *o, const double *coupling,
+                          int niter, double tol, const double *frm, int frm_per_item, int ref,
+                          double *curr, int *iters, double *err, int *status, double *gn_diag,
+                          double *gr_diag) {
+  return guard([&]() -> int {
+    (void)e; (void)st; (void)nitems; (void)E; (void)slot; (void)o; (void)coupling; (void)niter;
+    (void)tol; (void)frm; (void)frm_per_item; (void)ref; (void)curr; (void)iters; (void)err;
+    (void)status; (void)gn_diag; (void)gr_diag;
+    throw std::runtime_error("SCBA path not available in this build");
+  });
+}
+
+int hipnegf_zgemm_batched(hipnegf_engine e, int m, int n, int k, const double *alpha,
+                          const void *A, long long sA, int lda, int opA, const void *B,
+                          long long sB, int ldb, int opB, const double *beta, void *C,
+                          long long sC, int ldc, int batch) {
+  return guard([&] {
+    HN_CHECK(hipDeviceSynchronize());
+    hn::GemmArgs<double> g;
+    g.m = m; g.n = n; g.k = k;
+    g.A = (const double2 *)A; g.sA = sA; g.lda = lda; g.opA = opA;
+    g.B = (const double2 *)B; g.sB = sB; g.ldb = ldb; g.opB = opB;
+    g.C = (double2 *)C; g.sC = sC; g.ldc = ldc;
+    g.alpha = make_double2(alpha[0], alpha[1]);
+    g.beta = make_double2(beta[0], beta[1]);
+    g.batch = batch;
+    hn::gemm_z(g, e->eng->stream());
+    HN_CHECK(hipStreamSynchronize(e->eng->stream()));
+    return 0;
+  });
+}
+int hipnegf_cgemm_batched(hipnegf_engine e, int m, int n, int k, const double *alpha,
+                          const void *A, long long sA, int lda, int opA, const void *B,
+                          long long sB, int ldb, int opB, const double *beta, void *C,
+                          long long sC, int ldc, int batch) {
+  return guard([&] {
+    HN_CHECK(hipDeviceSynchronize());
+    hn::GemmArgs<float> g;
+    g.m = m; g.n = n; g.k = k;
+    g.A = (const float2 *)A; g.sA = sA; g.lda = lda; g.opA = opA;
+    g.B = (const float2 *)B; g.sB = sB; g.ldb = ldb; g.opB = opB;
+    g.C = (float2 *)C; g.sC = sC; g.ldc = ldc;
+    g.alpha = make_double2(alpha[0], alpha[1]);
+    g.beta = make_double2(beta[0], beta[1]);
+    g.batch = batch;
+    hn::gemm_c(g, e->eng->stream());
+    HN_CHECK(hipStreamSynchronize(e->eng->stream()));
+    return 0;
+  });
+}
+
+int hipnegf_zinverse_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv,
+                             long long sAinv, int batch, int *dstatus) {
+  return guard([&] { return inv_batched<double>(e, n, A, sA, Ainv, sAinv, batch, dstatus); });
+}
+int hipnegf_cinverse_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv,
+                             long long sAinv, int batch, int *dstatus) {
+  return guard([&] { return inv_batched<float>(e, n, A, sA, Ainv, sAinv, batch, dstatus); });
+}
+
+}  // extern "C"

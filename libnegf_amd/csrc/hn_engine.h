@@ -1,0 +1,135 @@
+// hn_engine.h -- batched per-(E,k) Green's-function engine (host side).
+#pragma once
+#include <memory>
+#include <vector>
+
+#include "hn_common.h"
+#include "hn_inv.h"
+#include "hn_kernels.h"
+
+namespace hn {
+
+// Block kinds of a structure slot (device copy of one H(k), S(k)).
+enum BlockKind : int {
+  BK_HD = 0, BK_SD, BK_HU, BK_SU, BK_HL, BK_SL,  // device: diag, upper (i,i+1), lower (i+1,i)
+  BK_H00, BK_S00, BK_H01, BK_S01, BK_HMC, BK_SMC, // contacts
+  BK_COUNT
+};
+
+// S modes
+enum SMode : int { S_DIAG = 1, S_OFFDIAG = 2, S_CONTACT = 4 };
+
+// Device-resident block-tridiagonal structure with `nslots` slots of identical
+// layout (one slot per transverse k or per request variant).
+struct DevStruct {
+  int nbl = 0, ncont = 0, smode = 0, nslots = 0;
+  std::vector<int> bsz, cblk, nc;
+  std::vector<long long> off[BK_COUNT];
+  long long slot_elems = 0;
+  double2 *pool = nullptr;
+  int device = 0;
+
+  DevStruct(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc, int smode,
+            int nslots);
+  ~DevStruct();
+  long long block_elems(int kind, int idx) const;
+  BlkRef ref(int kind, int idx) const {
+    BlkRef r;
+    r.pool = pool;
+    r.slot_stride = slot_elems;
+    r.off = off[kind].empty() ? -1 : off[kind][idx];
+    return r;
+  }
+  size_t bytes() const { return (size_t)slot_elems * nslots * sizeof(double2); }
+  int nmax() const;
+  bool uniform() const;
+};
+
+enum Precision : int { PREC_FP64 = 0, PREC_MIXED = 1 };
+
+// Device arena for engine workspaces (grown on demand, never shrunk).
+struct Arena {
+  char *base = nullptr;
+  size_t cap = 0, used = 0;
+  void reserve(size_t bytes);
+  void reset() { used = 0; }
+  template <typename T> T *take(size_t count) {
+    size_t b = (count * sizeof(T) + 255) & ~(size_t)255;
+    if (used + b > cap) throw std::runtime_error("arena overflow");
+    T *p = (T *)(base + used);
+    used += b;
+    return p;
+  }
+  ~Arena();
+};
+
+struct SolveOpts {
+  double delta = 1e-4;       // eta on the contacts (lib_param.F90:628 default)
+  int ni = 1, nf = 2;        // 1-based contact pair
+  int precision = PREC_FP64;
+  int max_batch = 0;         // 0: automatic (memory budget)
+  double sgfacc = 1e-9;      // SGFACC, ln_precision.F90:40
+  int dec_maxit = 300;       // contselfenergy.F90:289
+  double eps_drop = 0.0;     // EPS (ln_precision), default 0
+  int ns_steps = 1;          // mixed precision: FP64 Newton-Schulz refinement steps
+  int polish = 2;            // mixed precision: FP64 decimation polish iterations
+  int dec_crit = 0;          // 0: max|C1| (CPU path), 1: sum|Re|+|Im| (GPU path)
+};
+
+struct SCBAOpts {
+  const double *coupling = nullptr;  // per device orbital (squared inside, elphdd.F90:94)
+  int niter = 0;                     // scba_niter (loop runs while iter <= niter)
+  double tol = 1e-7;                 // scba_elastic_tol (lib_param.F90:653)
+  const double *frm = nullptr;       // [ncont+1] occupations; frm[ref-1] reference
+  int ref = 0;                       // 1-based reference contact (ncont+1: none)
+};
+
+class Engine {
+ public:
+  explicit Engine(int device);
+  ~Engine();
+  hipStream_t stream() const { return stream_; }
+
+  // Coherent transmission for nitems (E, slot) points.  All pointers device.
+  // T[nitems], ncyc[nitems*ncont], status[nitems].
+  void transmission(const DevStruct &st, int nitems, const double *dE, const int *dslot,
+                    const SolveOpts &o, double *dT, int *dncyc, int *dstatus);
+
+  // Elastic-dephasing SCBA + Meir-Wingreen current per item.
+  // curr[nitems*ncont], iters[nitems], err[nitems], status[nitems];
+  // gn_diag/gr_diag [nitems*ndev] optional (device).
+  void scba_mw(const DevStruct &st, int nitems, const double *dE, const int *dslot,
+               const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters, double *derr,
+               int *dstatus, double2 *gn_diag, double2 *gr_diag);
+
+  // Single decimation with host matrices (cu_Zdecimation semantics).
+  int decimation_host(int n, double2 *hGo, const double2 *hAo, const double2 *hBo,
+                      const double2 *hCo, double acc, int crit, int *ncyc);
+
+  int last_batch() const { return last_batch_; }
+  size_t workspace_bytes() const { return arena_.cap; }
+
+ private:
+  int device_;
+  hipStream_t stream_;
+  Arena arena_;
+  int *h_count_ = nullptr;  // pinned
+  int last_batch_ = 0;
+
+  int pick_batch(const DevStruct &st, int nitems, const SolveOpts &o, size_t per_item) const;
+  void contacts(const DevStruct &st, int B, const double *dE, const int *dslot,
+                const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c, int *status_c);
+  void decimate_z(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, double2 *Co,
+                  double2 *Go, const SolveOpts &o, int *ncyc, int *status);
+  void decimate_mixed(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, double2 *Co,
+                      double2 *Go, const SolveOpts &o, int *ncyc, int *status);
+  void inverse(int n, double2 *A, long long sA, double2 *Ainv, long long sAinv, int batch,
+               const int *active, int *status, int precision, const SolveOpts &o);
+  void t_batch(const DevStruct &st, int B, const double *dE, const int *dslot,
+               const SolveOpts &o, double *dT, int *dncyc, int *dstatus);
+  void scba_batch(const DevStruct &st, int B, const double *dE, const int *dslot,
+                  const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
+                  double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag);
+};
+
+}  // namespace hn

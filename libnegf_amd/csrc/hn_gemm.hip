@@ -1,0 +1,215 @@
+// hn_gemm.hip -- batched complex GEMM on CDNA4 matrix cores.
+//
+// One 256-thread workgroup (4 waves, 2 x 2) computes a 64 x 64 complex tile of
+// C for one batch item.  K is walked in steps of 16 complex elements staged
+// through LDS as separate real/imaginary planes (k-major), so every MFMA
+// operand fragment is one conflict-free ds_read per lane.  Complex products
+// use four real MFMAs (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32):
+//   Cr += Ar Br - Ai Bi ,   Ci += Ar Bi + Ai Br.
+// Each wave owns a 32 x 32 sub-tile = 2 x 2 MFMA tiles x (re, im) accumulators.
+//
+// Workgroup -> (item, tile) mapping is XCD-aware: hardware dispatches
+// consecutive workgroup ids round-robin over the 8 XCDs, so the linear id is
+// remapped such that all tiles of one item land on the same XCD and share its
+// L2 (the A and B panels of an item are re-read by every tile).
+#include "hn_gemm.h"
+
+namespace hn {
+
+namespace {
+
+constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
+constexpr int PAD = 16;  // row padding (elements) of the LDS planes
+
+template <typename R> struct V4;
+template <> struct V4<double> { typedef double __attribute__((ext_vector_type(4))) t; };
+template <> struct V4<float> { typedef float __attribute__((ext_vector_type(4))) t; };
+
+__device__ inline V4<double>::t mfma(double a, double b, V4<double>::t c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ inline V4<float>::t mfma(float a, float b, V4<float>::t c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+template <typename R> struct CplxT;
+template <> struct CplxT<double> { typedef double2 t; };
+template <> struct CplxT<float> { typedef float2 t; };
+
+// XCD-aware remap of a linear workgroup id in [0, total_pad) (total_pad a
+// multiple of 8) to a logical id: logical ids [x*per, (x+1)*per) run on XCD x.
+__device__ inline int xcd_remap(int wg, int total_pad) {
+  int per = total_pad >> 3;
+  return (wg & 7) * per + (wg >> 3);
+}
+
+template <typename R>
+__global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int tiles_n,
+                                             int total, int total_pad) {
+  typedef typename CplxT<R>::t CT;
+  typedef typename V4<R>::t v4;
+  const int logical = xcd_remap(blockIdx.x, total_pad);
+  if (logical >= total) return;
+  const int tiles = tiles_m * tiles_n;
+  const int b = logical / tiles;
+  const int t = logical - b * tiles;
+  if (p.active && p.active[b] == 0) return;
+  const int tm = t % tiles_m, tn = t / tiles_m;
+  const int m0 = tm * BM, n0 = tn * BN;
+
+  __shared__ R sAr[BK][BM + PAD];
+  __shared__ R sAi[BK][BM + PAD];
+  __shared__ R sBr[BK][BN + PAD];
+  __shared__ R sBi[BK][BN + PAD];
+
+  const CT *A = p.A + (long long)b * p.sA;
+  const CT *B = p.B + (long long)b * p.sB;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+
+  v4 cr[2][2], ci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      cr[i][j] = v4{0, 0, 0, 0};
+      ci[i][j] = v4{0, 0, 0, 0};
+    }
+
+  const int M = p.m, N = p.n, K = p.k;
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    // ---- stage op(A)[m0:m0+64, k0:k0+16] ----
+    if (p.opA == OP_N) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int i = tid & 63, kk = (tid >> 6) + 4 * r;
+        int gi = m0 + i, gk = k0 + kk;
+        CT v = {0, 0};
+        if (gi < M && gk < K) v = A[gi + (long long)gk * p.lda];
+        sAr[kk][i] = v.x;
+        sAi[kk][i] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int kk = tid & 15, i = (tid >> 4) + 16 * r;
+        int gi = m0 + i, gk = k0 + kk;
+        CT v = {0, 0};
+        if (gi < M && gk < K) v = A[gk + (long long)gi * p.lda];
+        sAr[kk][i] = v.x;
+        sAi[kk][i] = (p.opA == OP_C) ? -v.y : v.y;
+      }
+    }
+    // ---- stage op(B)[k0:k0+16, n0:n0+64] ----
+    if (p.opB == OP_N) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int kk = tid & 15, j = (tid >> 4) + 16 * r;
+        int gj = n0 + j, gk = k0 + kk;
+        CT v = {0, 0};
+        if (gj < N && gk < K) v = B[gk + (long long)gj * p.ldb];
+        sBr[kk][j] = v.x;
+        sBi[kk][j] = v.y;
+      }
+    } else {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int j = tid & 63, kk = (tid >> 6) + 4 * r;
+        int gj = n0 + j, gk = k0 + kk;
+        CT v = {0, 0};
+        if (gj < N && gk < K) v = B[gj + (long long)gk * p.ldb];
+        sBr[kk][j] = v.x;
+        sBi[kk][j] = (p.opB == OP_C) ? -v.y : v.y;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kr = ks + (lane >> 4), c16 = lane & 15;
+      R ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = sAr[kr][wm * 32 + i * 16 + c16];
+        ai[i] = sAi[kr][wm * 32 + i * 16 + c16];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        br[j] = sBr[kr][wn * 32 + j * 16 + c16];
+        bi[j] = sBi[kr][wn * 32 + j * 16 + c16];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cr[i][j] = mfma(ar[i], br[j], cr[i][j]);
+          cr[i][j] = mfma(nai[i], bi[j], cr[i][j]);
+          ci[i][j] = mfma(ar[i], bi[j], ci[i][j]);
+          ci[i][j] = mfma(ai[i], br[j], ci[i][j]);
+        }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue: C = alpha acc + beta C (and the optional second output) ----
+  CT *C = p.C + (long long)b * p.sC;
+  CT *C2 = p.C2 ? p.C2 + (long long)b * p.sC2 : nullptr;
+  const double ar_ = p.alpha.x, ai_ = p.alpha.y;
+  const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
+  const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // D[row][col] of a 16x16 MFMA tile: v_mfma_f64_16x16x4_f64 keeps it in
+        // lane col + 16*(row%4), element row/4; v_mfma_f32_16x16x4_f32 in
+        // lane col + 16*(row/4), element row%4.
+        int row = m0 + wm * 32 + i * 16 + (sizeof(R) == 8 ? (lane >> 4) + 4 * r : (lane >> 4) * 4 + r);
+        int col = n0 + wn * 32 + j * 16 + (lane & 15);
+        if (row >= M || col >= N) continue;
+        if (col >= p.skip_c0 && col < p.skip_c1) continue;
+        double xr = (double)cr[i][j][r], xi = (double)ci[i][j][r];
+        double vr = ar_ * xr - ai_ * xi, vi = ar_ * xi + ai_ * xr;
+        long long o = row + (long long)col * p.ldc;
+        double yr = vr, yi = vi;
+        if (useb) {
+          CT c = C[o];
+          yr += p.beta.x * (double)c.x - p.beta.y * (double)c.y;
+          yi += p.beta.x * (double)c.y + p.beta.y * (double)c.x;
+        }
+        C[o] = CT{(R)yr, (R)yi};
+        if (C2) {
+          long long o2 = row + (long long)col * p.ldc2;
+          double zr = vr, zi = vi;
+          if (useb2) {
+            CT c = C2[o2];
+            zr += p.beta2.x * (double)c.x - p.beta2.y * (double)c.y;
+            zi += p.beta2.x * (double)c.y + p.beta2.y * (double)c.x;
+          }
+          C2[o2] = CT{(R)zr, (R)zi};
+        }
+      }
+}
+
+template <typename R>
+void launch(const GemmArgs<R> &a, hipStream_t s) {
+  if (a.m <= 0 || a.n <= 0 || a.batch <= 0) return;
+  int tiles_m = (a.m + BM - 1) / BM, tiles_n = (a.n + BN - 1) / BN;
+  long long total = (long long)tiles_m * tiles_n * a.batch;
+  if (total > (1LL << 30)) throw std::runtime_error("gemm: grid too large");
+  int total_pad = (int)((total + 7) / 8 * 8);
+  hipLaunchKernelGGL(k_gemm<R>, dim3(total_pad), dim3(NT), 0, s, a, tiles_m, tiles_n,
+                     (int)total, total_pad);
+  HN_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+void gemm_z(const GemmArgs<double> &a, hipStream_t s) { launch<double>(a, s); }
+void gemm_c(const GemmArgs<float> &a, hipStream_t s) { launch<float>(a, s); }
+
+}  // namespace hn

@@ -1,0 +1,102 @@
+// hn_kernels.h -- elementwise / reduction kernels of the engine (host side).
+#pragma once
+#include "hn_common.h"
+
+namespace hn {
+
+constexpr int kMaxCont = 10;  // MAXNCONT, lnParams.h
+constexpr size_t kNone = ~(size_t)0;
+
+// Where a block lives: item b reads pool + slot[b] * slot_stride + off.
+struct BlkRef {
+  const double2 *pool = nullptr;
+  long long slot_stride = 0;
+  long long off = -1;  // -1: absent
+};
+
+// out_b = z_b * S - H  (S: explicit block, identity (sid=1) or zero (sid=0, S absent))
+//         - Sig_b (optional)          rows x cols, column-major, ld = rows
+// z_b = E[b] + i*eta (conjz: use conj(z_b)).  trans: write the conjugate
+// transpose of (z S - H) instead (used for Co and the coupling blocks).
+struct FormArgs {
+  BlkRef H, S;
+  int s_identity = 0;
+  const int *slot = nullptr;
+  const double *E = nullptr;
+  double eta = 0.0;
+  int conjz = 0;
+  int dagger = 0;
+  int rows = 0, cols = 0;
+  double2 *out = nullptr; long long s_out = 0;
+  double2 *out2 = nullptr; long long s_out2 = 0;  // optional copy
+  const double2 *sig = nullptr; long long s_sig = 0;
+  const double2 *sig2 = nullptr; long long s_sig2 = 0;
+  const double2 *sig_diag = nullptr; long long s_sig_diag = 0;  // optional diagonal -= vector
+  const int *active = nullptr;
+  int batch = 0;
+  int item_div = 1;  // item index for E/slot = b % item_div (contact items c*B + b)
+};
+void form_block(const FormArgs &a, hipStream_t s);
+
+// strided batched copy dst_b = src_b (n complex per item)
+void copy_batched(double2 *dst, long long sd, const double2 *src, long long ss, long long n,
+                  int batch, const int *active, hipStream_t s);
+void copy_batched_c(float2 *dst, long long sd, const float2 *src, long long ss, long long n,
+                    int batch, const int *active, hipStream_t s);
+
+// Decimation convergence state machine, contselfenergy.F90:300-310:
+// m = max|C1_b|; if m <= acc { if ok_b: exit (act=0, ncyc=it) else ok_b=1 } else ok_b=0.
+// act_b stays 1 while iterating; upd_b = 1 if the update part of this
+// iteration must run.  At it == maxit items still active get status NOT_CONVERGED.
+// crit 0: max |C1| (CPU path, contselfenergy.F90:300); crit 1: sum |Re|+|Im| of C1
+// (GPU path, gpuroutines.cu:866-878).
+void decim_check(const double2 *C1, long long sC1, int n2, int batch, double acc, int it,
+                 int maxit, int *act, int *upd, int *okco, int *ncyc, int *status, int crit,
+                 hipStream_t s);
+void decim_check_c(const float2 *C1, long long sC1, int n2, int batch, double acc, int it,
+                   int maxit, int *act, int *upd, int *okco, int *ncyc, int *status, int crit,
+                   hipStream_t s);
+
+// count of nonzero flags (written to *count, device)
+void count_nonzero(const int *flags, int batch, int *count, hipStream_t s);
+
+// Gam_b = i (Sig_b - Sig_b^H)
+void gamma_batched(const double2 *Sig, long long sS, double2 *Gam, long long sG, int n,
+                   int batch, hipStream_t s);
+
+// T_b = | Re( Tr[Gam A] - Tr[W2 G^H] ) |,  A = i (G - G^H)
+// (calculate_single_transmission_2_contacts, iterative_cpu.F90:399-470)
+void trace_T2(const double2 *Gam, const double2 *G, const double2 *W2, long long s, int n,
+              int batch, double *T, hipStream_t st);
+// T_b = | Re Tr[W] | for W = Gam2 G21 Gam1 G21^H computed as Re sum (X)_ij conj(G21)_ij
+void trace_prod_conj(const double2 *X, const double2 *G, long long sX, long long sG, int rows,
+                     int cols, int batch, double *T, hipStream_t st);
+// out_b = Re Tr[X_b Y_b] (square n): sum_ij X_ij Y_ji, scaled by alpha, accumulated
+void trace_prod(const double2 *X, const double2 *Y, long long sX, long long sY, int n,
+                int batch, double alpha, double *out, hipStream_t st);
+
+// extract / scatter diagonals: d_b[i] = M_b(i,i) * scale[i]  (scale may be null)
+void diag_extract(const double2 *M, long long sM, int n, int batch, double2 *d, long long sd,
+                  const double *scale, hipStream_t st);
+
+// max |X_b - Y_b| over entries, per item, into err[b]
+void maxabs_diff(const double2 *X, const double2 *Y, long long sX, long long sY, long long n,
+                 int batch, double *err, hipStream_t st);
+
+// element-wise fill: out_b = value (n complex)
+void fill_batched(double2 *out, long long so, long long n, int batch, double2 value,
+                  hipStream_t st);
+void fill_int(int *out, int n, int value, hipStream_t st);
+void fill_double(double *out, int n, double value, hipStream_t st);
+
+// precision conversion  z -> c and back, with optional active mask
+void z2c(const double2 *src, long long ss, float2 *dst, long long sd, long long n, int batch,
+         const int *active, hipStream_t st);
+void c2z(const float2 *src, long long ss, double2 *dst, long long sd, long long n, int batch,
+         const int *active, hipStream_t st);
+
+// out_b = M_b + diag(v_b)*alpha  (add a scaled vector on the diagonal), n x n
+void add_diag(double2 *M, long long sM, int n, const double2 *v, long long sv, double2 alpha,
+              int batch, const int *active, hipStream_t st);
+
+}  // namespace hn

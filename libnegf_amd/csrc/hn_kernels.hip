@@ -1,0 +1,437 @@
+// hn_kernels.hip -- elementwise and per-item reduction kernels of the engine.
+//
+// These replace the reference's per-block helper calls (zsumdns, zdagger,
+// zspectral, ztrace, cu_Zmatsum / cu_Ztrace / cu_Zasum, sparsekit_drv.F90 and
+// gpuroutines.cu:567-739) by whole-batch launches; all are HBM-streaming.
+#include "hn_kernels.h"
+
+namespace hn {
+
+namespace {
+
+constexpr int NT = 256;
+
+__device__ inline double2 cm(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+template <typename T> __device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+template <typename T> __device__ inline T wave_max(T v) {
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1) {
+    T o = __shfl_xor(v, off);
+    v = (o > v || o != o) ? o : v;  // propagate NaN
+  }
+  return v;
+}
+
+// block-wide reductions for NT threads
+template <typename T> __device__ inline T block_sum(T v, T *sh) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0)
+    for (int i = 0; i < NT / 64; ++i) r += sh[i];
+  __syncthreads();
+  return r;  // valid in thread 0
+}
+template <typename T> __device__ inline T block_max(T v, T *sh) {
+  v = wave_max(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0) {
+    r = sh[0];
+    for (int i = 1; i < NT / 64; ++i) r = (sh[i] > r || sh[i] != sh[i]) ? sh[i] : r;
+  }
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(NT) void k_form(FormArgs a) {
+  const int b = blockIdx.y;
+  if (a.active && a.active[b] == 0) return;
+  const long long n = (long long)a.rows * a.cols;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  const int r = (int)(e % a.rows), c = (int)(e / a.rows);
+  const int sl = a.slot ? a.slot[b] : 0;
+  double2 z = make_double2(a.E ? a.E[b] : 0.0, a.eta);
+  if (a.conjz) z.y = -z.y;
+  // source element (rs, cs) of the (possibly transposed) source block
+  const int rs = a.dagger ? c : r, cs = a.dagger ? r : c;
+  const int srows = a.dagger ? a.cols : a.rows;
+  const long long se = rs + (long long)cs * srows;
+  const double2 *Hb = a.H.pool + (long long)sl * a.H.slot_stride + a.H.off;
+  double2 h = Hb[se];
+  double2 sv = make_double2(0.0, 0.0);
+  if (a.S.off >= 0) {
+    const double2 *Sb = a.S.pool + (long long)sl * a.S.slot_stride + a.S.off;
+    sv = Sb[se];
+  } else if (a.s_identity && rs == cs) {
+    sv = make_double2(1.0, 0.0);
+  }
+  double2 v = cm(z, sv);
+  v.x -= h.x;
+  v.y -= h.y;
+  if (a.dagger) v.y = -v.y;
+  if (a.sig) {
+    double2 sg = a.sig[(long long)b * a.s_sig + e];
+    v.x -= sg.x;
+    v.y -= sg.y;
+  }
+  if (a.sig2) {
+    double2 sg = a.sig2[(long long)b * a.s_sig2 + e];
+    v.x -= sg.x;
+    v.y -= sg.y;
+  }
+  if (a.sig_diag && r == c) {
+    double2 sg = a.sig_diag[(long long)b * a.s_sig_diag + r];
+    v.x -= sg.x;
+    v.y -= sg.y;
+  }
+  a.out[(long long)b * a.s_out + e] = v;
+  if (a.out2) a.out2[(long long)b * a.s_out2 + e] = v;
+}
+
+template <typename C>
+__global__ __launch_bounds__(NT) void k_copy(C *dst, long long sd, const C *src, long long ss,
+                                             long long n, const int *active) {
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  dst[(long long)b * sd + e] = src[(long long)b * ss + e];
+}
+
+template <typename C>
+__global__ __launch_bounds__(NT) void k_decim_check(const C *C1, long long sC1, int n2,
+                                                    double acc, int it, int maxit, int *act,
+                                                    int *upd, int *okco, int *ncyc,
+                                                    int *status, int crit) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.x;
+  if (act[b] == 0) {
+    if (threadIdx.x == 0) upd[b] = 0;
+    return;
+  }
+  const C *X = C1 + (long long)b * sC1;
+  double m = 0.0;
+  if (crit == 0) {
+    for (int e = threadIdx.x; e < n2; e += NT) {
+      C v = X[e];
+      double a = hypot((double)v.x, (double)v.y);
+      m = (a > m || a != a) ? a : m;
+    }
+    m = block_max(m, sh);
+  } else {
+    for (int e = threadIdx.x; e < n2; e += NT) {
+      C v = X[e];
+      m += fabs((double)v.x) + fabs((double)v.y);
+    }
+    m = block_sum(m, sh);
+  }
+  if (threadIdx.x == 0) {
+    if (m != m) {  // non-finite iterate: stop this item, flag it
+      act[b] = 0;
+      upd[b] = 0;
+      ncyc[b] = it;
+      status[b] |= ST_NONFINITE;
+      return;
+    }
+    if (m <= acc) {
+      if (okco[b]) {
+        act[b] = 0;
+        upd[b] = 0;
+        ncyc[b] = it;
+        return;
+      }
+      okco[b] = 1;
+    } else {
+      okco[b] = 0;
+    }
+    upd[b] = 1;
+    if (it >= maxit) {  // the DO loop ends after this update: ncyc = maxit + 1
+      ncyc[b] = maxit + 1;
+      status[b] |= ST_NOT_CONVERGED;
+    }
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_count(const int *flags, int batch, int *count) {
+  __shared__ int sh[NT / 64];
+  int c = 0;
+  for (int i = threadIdx.x; i < batch; i += NT) c += flags[i] != 0;
+  c = block_sum(c, sh);
+  if (threadIdx.x == 0) *count = c;
+}
+
+__global__ __launch_bounds__(NT) void k_gamma(const double2 *Sig, long long sS, double2 *Gam,
+                                              long long sG, int n) {
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  const double2 *S = Sig + (long long)b * sS;
+  double2 a = S[e], t = S[c + (long long)r * n];
+  // i (a - conj(t)) = i((a.x - t.x) + i(a.y + t.y)) = -(a.y + t.y) + i (a.x - t.x)
+  Gam[(long long)b * sG + e] = make_double2(-(a.y + t.y), a.x - t.x);
+}
+
+__global__ __launch_bounds__(NT) void k_trace_T2(const double2 *Gam, const double2 *G,
+                                                 const double2 *W2, long long s, int n,
+                                                 double *T) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.x;
+  const double2 *Gm = Gam + (long long)b * s, *Gb = G + (long long)b * s, *Wb = W2 + (long long)b * s;
+  double acc = 0.0;
+  const long long nn = (long long)n * n;
+  for (long long e = threadIdx.x; e < nn; e += NT) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    const double2 g_rc = Gb[e], g_cr = Gb[c + (long long)r * n];
+    // A_cr = i (G_cr - conj(G_rc))
+    const double2 d = make_double2(g_cr.x - g_rc.x, g_cr.y + g_rc.y);
+    const double2 A_cr = make_double2(-d.y, d.x);
+    const double2 gm = Gm[e];
+    acc += gm.x * A_cr.x - gm.y * A_cr.y;          // Re(Gam_rc A_cr)
+    const double2 w = Wb[e];
+    acc -= w.x * g_rc.x + w.y * g_rc.y;            // Re(W_rc conj(G_rc))
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) T[b] = fabs(acc);
+}
+
+__global__ __launch_bounds__(NT) void k_trace_prod_conj(const double2 *X, const double2 *G,
+                                                        long long sX, long long sG, long long nn,
+                                                        double *T) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.x;
+  const double2 *Xb = X + (long long)b * sX, *Gb = G + (long long)b * sG;
+  double acc = 0.0;
+  for (long long e = threadIdx.x; e < nn; e += NT) {
+    const double2 x = Xb[e], g = Gb[e];
+    acc += x.x * g.x + x.y * g.y;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) T[b] = fabs(acc);
+}
+
+__global__ __launch_bounds__(NT) void k_trace_prod(const double2 *X, const double2 *Y,
+                                                   long long sX, long long sY, int n,
+                                                   double alpha, double *out) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.x;
+  const double2 *Xb = X + (long long)b * sX, *Yb = Y + (long long)b * sY;
+  double acc = 0.0;
+  const long long nn = (long long)n * n;
+  for (long long e = threadIdx.x; e < nn; e += NT) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    const double2 x = Xb[e], y = Yb[c + (long long)r * n];
+    acc += x.x * y.x - x.y * y.y;
+  }
+  acc = block_sum(acc, sh);
+  if (threadIdx.x == 0) out[b] += alpha * acc;
+}
+
+__global__ __launch_bounds__(NT) void k_diag_extract(const double2 *M, long long sM, int n,
+                                                     double2 *d, long long sd,
+                                                     const double *scale) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  double2 v = M[(long long)b * sM + i + (long long)i * n];
+  double s = scale ? scale[i] : 1.0;
+  d[(long long)b * sd + i] = make_double2(v.x * s, v.y * s);
+}
+
+__global__ __launch_bounds__(NT) void k_maxabs_diff(const double2 *X, const double2 *Y,
+                                                    long long sX, long long sY, long long n,
+                                                    double *err) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.x;
+  double m = 0.0;
+  for (long long e = threadIdx.x; e < n; e += NT) {
+    double2 x = X[(long long)b * sX + e], y = Y[(long long)b * sY + e];
+    double a = hypot(x.x - y.x, x.y - y.y);
+    m = (a > m || a != a) ? a : m;
+  }
+  m = block_max(m, sh);
+  if (threadIdx.x == 0) err[b] = m;
+}
+
+__global__ __launch_bounds__(NT) void k_fill(double2 *out, long long so, long long n,
+                                             double2 v) {
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e < n) out[(long long)b * so + e] = v;
+}
+__global__ __launch_bounds__(NT) void k_fill_int(int *out, int n, int v) {
+  const int e = blockIdx.x * NT + threadIdx.x;
+  if (e < n) out[e] = v;
+}
+__global__ __launch_bounds__(NT) void k_fill_double(double *out, int n, double v) {
+  const int e = blockIdx.x * NT + threadIdx.x;
+  if (e < n) out[e] = v;
+}
+
+__global__ __launch_bounds__(NT) void k_z2c(const double2 *src, long long ss, float2 *dst,
+                                            long long sd, long long n, const int *active) {
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  double2 v = src[(long long)b * ss + e];
+  dst[(long long)b * sd + e] = make_float2((float)v.x, (float)v.y);
+}
+__global__ __launch_bounds__(NT) void k_c2z(const float2 *src, long long ss, double2 *dst,
+                                            long long sd, long long n, const int *active) {
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  float2 v = src[(long long)b * ss + e];
+  dst[(long long)b * sd + e] = make_double2((double)v.x, (double)v.y);
+}
+
+__global__ __launch_bounds__(NT) void k_add_diag(double2 *M, long long sM, int n,
+                                                 const double2 *v, long long sv, double2 alpha,
+                                                 const int *active) {
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  double2 x = v[(long long)b * sv + i];
+  double2 *p = M + (long long)b * sM + i + (long long)i * n;
+  double2 m = *p;
+  m.x += alpha.x * x.x - alpha.y * x.y;
+  m.y += alpha.x * x.y + alpha.y * x.x;
+  *p = m;
+}
+
+inline dim3 grid2(long long n, int batch) {
+  return dim3((unsigned)((n + NT - 1) / NT), (unsigned)batch);
+}
+
+}  // namespace
+
+void form_block(const FormArgs &a, hipStream_t s) {
+  if (a.batch <= 0 || a.rows <= 0 || a.cols <= 0) return;
+  hipLaunchKernelGGL(k_form, grid2((long long)a.rows * a.cols, a.batch), dim3(NT), 0, s, a);
+  HN_CHECK(hipGetLastError());
+}
+
+void copy_batched(double2 *dst, long long sd, const double2 *src, long long ss, long long n,
+                  int batch, const int *active, hipStream_t s) {
+  if (batch <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_copy<double2>, grid2(n, batch), dim3(NT), 0, s, dst, sd, src, ss, n, active);
+  HN_CHECK(hipGetLastError());
+}
+void copy_batched_c(float2 *dst, long long sd, const float2 *src, long long ss, long long n,
+                    int batch, const int *active, hipStream_t s) {
+  if (batch <= 0 || n <= 0) return;
+  hipLaunchKernelGGL(k_copy<float2>, grid2(n, batch), dim3(NT), 0, s, dst, sd, src, ss, n, active);
+  HN_CHECK(hipGetLastError());
+}
+
+void decim_check(const double2 *C1, long long sC1, int n2, int batch, double acc, int it,
+                 int maxit, int *act, int *upd, int *okco, int *ncyc, int *status, int crit,
+                 hipStream_t s) {
+  hipLaunchKernelGGL(k_decim_check<double2>, dim3(batch), dim3(NT), 0, s, C1, sC1, n2, acc, it,
+                     maxit, act, upd, okco, ncyc, status, crit);
+  HN_CHECK(hipGetLastError());
+}
+void decim_check_c(const float2 *C1, long long sC1, int n2, int batch, double acc, int it,
+                   int maxit, int *act, int *upd, int *okco, int *ncyc, int *status, int crit,
+                   hipStream_t s) {
+  hipLaunchKernelGGL(k_decim_check<float2>, dim3(batch), dim3(NT), 0, s, C1, sC1, n2, acc, it,
+                     maxit, act, upd, okco, ncyc, status, crit);
+  HN_CHECK(hipGetLastError());
+}
+
+void count_nonzero(const int *flags, int batch, int *count, hipStream_t s) {
+  hipLaunchKernelGGL(k_count, dim3(1), dim3(NT), 0, s, flags, batch, count);
+  HN_CHECK(hipGetLastError());
+}
+
+void gamma_batched(const double2 *Sig, long long sS, double2 *Gam, long long sG, int n,
+                   int batch, hipStream_t s) {
+  hipLaunchKernelGGL(k_gamma, grid2((long long)n * n, batch), dim3(NT), 0, s, Sig, sS, Gam, sG, n);
+  HN_CHECK(hipGetLastError());
+}
+
+void trace_T2(const double2 *Gam, const double2 *G, const double2 *W2, long long s, int n,
+              int batch, double *T, hipStream_t st) {
+  hipLaunchKernelGGL(k_trace_T2, dim3(batch), dim3(NT), 0, st, Gam, G, W2, s, n, T);
+  HN_CHECK(hipGetLastError());
+}
+
+void trace_prod_conj(const double2 *X, const double2 *G, long long sX, long long sG, int rows,
+                     int cols, int batch, double *T, hipStream_t st) {
+  hipLaunchKernelGGL(k_trace_prod_conj, dim3(batch), dim3(NT), 0, st, X, G, sX, sG,
+                     (long long)rows * cols, T);
+  HN_CHECK(hipGetLastError());
+}
+
+void trace_prod(const double2 *X, const double2 *Y, long long sX, long long sY, int n,
+                int batch, double alpha, double *out, hipStream_t st) {
+  hipLaunchKernelGGL(k_trace_prod, dim3(batch), dim3(NT), 0, st, X, Y, sX, sY, n, alpha, out);
+  HN_CHECK(hipGetLastError());
+}
+
+void diag_extract(const double2 *M, long long sM, int n, int batch, double2 *d, long long sd,
+                  const double *scale, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_extract, grid2(n, batch), dim3(NT), 0, st, M, sM, n, d, sd, scale);
+  HN_CHECK(hipGetLastError());
+}
+
+void maxabs_diff(const double2 *X, const double2 *Y, long long sX, long long sY, long long n,
+                 int batch, double *err, hipStream_t st) {
+  hipLaunchKernelGGL(k_maxabs_diff, dim3(batch), dim3(NT), 0, st, X, Y, sX, sY, n, err);
+  HN_CHECK(hipGetLastError());
+}
+
+void fill_batched(double2 *out, long long so, long long n, int batch, double2 value,
+                  hipStream_t st) {
+  if (n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_fill, grid2(n, batch), dim3(NT), 0, st, out, so, n, value);
+  HN_CHECK(hipGetLastError());
+}
+void fill_int(int *out, int n, int value, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_int, dim3((n + NT - 1) / NT), dim3(NT), 0, st, out, n, value);
+  HN_CHECK(hipGetLastError());
+}
+
+void fill_double(double *out, int n, double value, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill_double, dim3((n + NT - 1) / NT), dim3(NT), 0, st, out, n, value);
+  HN_CHECK(hipGetLastError());
+}
+
+void z2c(const double2 *src, long long ss, float2 *dst, long long sd, long long n, int batch,
+         const int *active, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_z2c, grid2(n, batch), dim3(NT), 0, st, src, ss, dst, sd, n, active);
+  HN_CHECK(hipGetLastError());
+}
+void c2z(const float2 *src, long long ss, double2 *dst, long long sd, long long n, int batch,
+         const int *active, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_c2z, grid2(n, batch), dim3(NT), 0, st, src, ss, dst, sd, n, active);
+  HN_CHECK(hipGetLastError());
+}
+
+void add_diag(double2 *M, long long sM, int n, const double2 *v, long long sv, double2 alpha,
+              int batch, const int *active, hipStream_t st) {
+  hipLaunchKernelGGL(k_add_diag, grid2(n, batch), dim3(NT), 0, st, M, sM, n, v, sv, alpha, active);
+  HN_CHECK(hipGetLastError());
+}
+
+}  // namespace hn

@@ -1,0 +1,243 @@
+"""ctypes binding of libhipnegf.so (the C ABI declared in include/hipnegf.h).
+
+Loading is explicit and fails loudly: there is no CPU fallback anywhere in
+this package.  The library must have been built in-tree
+(``python -m libnegf_amd.build``).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+from .structure import BlockStructure
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "libhipnegf.so")
+_LIB = None
+
+# block kinds / modes (hipnegf.h)
+HD, SD, HU, SU, HL, SL, H00, S00, H01, S01, HMC, SMC = range(12)
+S_DIAG, S_OFFDIAG, S_CONTACT = 1, 2, 4
+FP64, MIXED = 0, 1
+ST_OK, ST_SINGULAR, ST_NOT_CONVERGED, ST_NONFINITE = 0, 1, 2, 4
+
+
+class HipNegfError(RuntimeError):
+    pass
+
+
+class Opts(C.Structure):
+    _fields_ = [("delta", C.c_double), ("ni", C.c_int), ("nf", C.c_int),
+                ("precision", C.c_int), ("max_batch", C.c_int), ("ns_steps", C.c_int),
+                ("polish", C.c_int), ("eps_drop", C.c_double)]
+
+
+def load():
+    """Load libhipnegf.so; raises if it has not been built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise HipNegfError(f"{LIB_PATH} is missing: build it with `python -m libnegf_amd.build` "
+                           "(the HIP engine has no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i, d, ll = C.c_void_p, C.c_int, C.c_double, C.c_longlong
+    L.hipnegf_last_error.restype = C.c_char_p
+    L.hipnegf_engine_create.argtypes = [i, C.POINTER(vp)]
+    L.hipnegf_engine_destroy.argtypes = [vp]
+    L.hipnegf_engine_sync.argtypes = [vp]
+    L.hipnegf_engine_stream.argtypes = [vp]
+    L.hipnegf_engine_stream.restype = vp
+    L.hipnegf_struct_create.argtypes = [i, vp, i, vp, vp, i, i, C.POINTER(vp)]
+    L.hipnegf_struct_destroy.argtypes = [vp]
+    L.hipnegf_struct_set_block.argtypes = [vp, i, i, i, vp]
+    L.hipnegf_struct_set_block_device.argtypes = [vp, i, i, i, vp]
+    L.hipnegf_solve_transmission.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, vp, vp]
+    L.hipnegf_solve_transmission_device.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, vp, vp]
+    L.hipnegf_solve_scba_mw.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, i, d, vp, i, i,
+                                        vp, vp, vp, vp, vp, vp]
+    L.hipnegf_zgemm_batched.argtypes = [vp, i, i, i, vp, vp, ll, i, i, vp, ll, i, i, vp, vp, ll, i, i]
+    L.hipnegf_cgemm_batched.argtypes = L.hipnegf_zgemm_batched.argtypes
+    L.hipnegf_zinverse_batched.argtypes = [vp, i, vp, ll, vp, ll, i, vp]
+    L.hipnegf_cinverse_batched.argtypes = L.hipnegf_zinverse_batched.argtypes
+    # block-op ABI
+    L.hipnegf_blasInit.argtypes = [C.POINTER(vp)]
+    L.hipnegf_blasFinalize.argtypes = [vp]
+    L.hipnegf_ZmultMat.argtypes = [vp, C.c_size_t, C.c_size_t, C.c_size_t, vp, vp, vp, vp, vp, i]
+    L.hipnegf_CmultMat.argtypes = L.hipnegf_ZmultMat.argtypes
+    L.hipnegf_Zinverse.argtypes = [vp, vp, vp, vp, C.c_size_t]
+    L.hipnegf_Cinverse.argtypes = L.hipnegf_Zinverse.argtypes
+    L.hipnegf_Zmatsum.argtypes = [vp, C.c_size_t, C.c_size_t, vp, vp, vp, vp, vp, i]
+    L.hipnegf_Cmatsum.argtypes = L.hipnegf_Zmatsum.argtypes
+    L.hipnegf_Zinitmat.argtypes = [vp, C.c_size_t]
+    L.hipnegf_Cinitmat.argtypes = [vp, C.c_size_t]
+    L.hipnegf_Ztrace.argtypes = [vp, vp, C.c_size_t, vp, i]
+    L.hipnegf_Ztrace.restype = C.c_double
+    L.hipnegf_Ctrace.argtypes = [vp, vp, C.c_size_t, vp, i]
+    L.hipnegf_Ctrace.restype = C.c_float
+    L.hipnegf_Zmatcopy.argtypes = [vp, vp, vp, C.c_size_t]
+    L.hipnegf_Zasum.argtypes = [vp, vp, C.POINTER(C.c_double), C.c_size_t]
+    L.hipnegf_Casum.argtypes = [vp, vp, C.POINTER(C.c_float), C.c_size_t]
+    L.hipnegf_Zdecimation.argtypes = [vp, vp, vp, vp, vp, vp, C.c_size_t, i, C.POINTER(C.c_int), d]
+    L.hipnegf_Cdecimation.argtypes = [vp, vp, vp, vp, vp, vp, C.c_size_t, i, C.POINTER(C.c_int),
+                                      C.c_float]
+    _LIB = L
+    return L
+
+
+def check(rc: int):
+    if rc != 0:
+        msg = load().hipnegf_last_error().decode()
+        raise HipNegfError(msg or f"hipnegf call failed (rc={rc})")
+    return rc
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class Engine:
+    """One engine = one HIP stream + a device workspace arena."""
+
+    def __init__(self, device: int = 0):
+        L = load()
+        self._h = C.c_void_p()
+        check(L.hipnegf_engine_create(device, C.byref(self._h)))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def sync(self):
+        check(load().hipnegf_engine_sync(self._h))
+
+    def close(self):
+        if self._h:
+            load().hipnegf_engine_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _opts(delta: float, ni: int, nf: int, precision: int, max_batch: int, ns_steps=0,
+          polish=0, eps_drop=0.0) -> Opts:
+    return Opts(delta, ni, nf, precision, max_batch, ns_steps, polish, eps_drop)
+
+
+class DeviceStructure:
+    """Device copy of one or more BlockStructures of identical shape (slots)."""
+
+    def __init__(self, structs: Sequence[BlockStructure] | BlockStructure, nslots: Optional[int] = None):
+        if isinstance(structs, BlockStructure):
+            structs = [structs]
+        s0 = structs[0]
+        self.template = s0
+        self.nslots = nslots or len(structs)
+        smode = 0
+        if s0.Sd is not None:
+            smode |= S_DIAG
+        if s0.Su is not None or s0.Sl is not None:
+            smode |= S_OFFDIAG
+        if any(c.S00 is not None for c in s0.contacts):
+            smode |= S_CONTACT
+        self.smode = smode
+        bsz = np.array(s0.bsz, dtype=np.int32)
+        cblk = np.array([c.cblk for c in s0.contacts], dtype=np.int32)
+        nc = np.array([c.nc for c in s0.contacts], dtype=np.int32)
+        self._h = C.c_void_p()
+        check(load().hipnegf_struct_create(s0.nbl, _ptr(bsz), len(cblk), _ptr(cblk), _ptr(nc),
+                                           smode, self.nslots, C.byref(self._h)))
+        for k, s in enumerate(structs):
+            self.set_slot(k, s)
+
+    def set_block(self, slot: int, kind: int, index: int, a: np.ndarray):
+        a = np.asfortranarray(a, dtype=np.complex128)
+        check(load().hipnegf_struct_set_block(self._h, slot, kind, index, _ptr(a)))
+
+    def set_slot(self, slot: int, s: BlockStructure):
+        for i, a in enumerate(s.Hd):
+            self.set_block(slot, HD, i, a)
+        for i in range(s.nbl - 1):
+            self.set_block(slot, HU, i, s.Hu[i])
+            self.set_block(slot, HL, i, s.Hl[i])
+        if self.smode & S_DIAG:
+            for i, a in enumerate(s.Sd):
+                self.set_block(slot, SD, i, a)
+        if self.smode & S_OFFDIAG:
+            for i in range(s.nbl - 1):
+                self.set_block(slot, SU, i, s.Su[i] if s.Su is not None else np.zeros_like(s.Hu[i]))
+                self.set_block(slot, SL, i, s.Sl[i] if s.Sl is not None else np.zeros_like(s.Hl[i]))
+        for c, ct in enumerate(s.contacts):
+            self.set_block(slot, H00, c, ct.H00)
+            self.set_block(slot, H01, c, ct.H01)
+            self.set_block(slot, HMC, c, ct.HMC)
+            if self.smode & S_CONTACT:
+                n = ct.nc
+                self.set_block(slot, S00, c, ct.S00 if ct.S00 is not None else np.eye(n))
+                self.set_block(slot, S01, c, ct.S01 if ct.S01 is not None else np.zeros((n, n)))
+                self.set_block(slot, SMC, c, ct.SMC if ct.SMC is not None else np.zeros_like(ct.HMC))
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            load().hipnegf_struct_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_transmission(eng: Engine, ds: DeviceStructure, E: np.ndarray, slot=None,
+                       delta: float = 1e-4, ni: int = 1, nf: int = 2, precision: int = FP64,
+                       max_batch: int = 0):
+    """Host-array batched transmission.  Returns (T, ncyc[n, ncont], status)."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    n = len(E)
+    sl = np.zeros(n, dtype=np.int32) if slot is None else np.ascontiguousarray(slot, dtype=np.int32)
+    ncont = ds.template.ncont
+    T = np.zeros(n)
+    ncyc = np.zeros((n, max(ncont, 1)), dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    o = _opts(delta, ni, nf, precision, max_batch)
+    check(load().hipnegf_solve_transmission(eng.handle, ds.handle, n, _ptr(E), _ptr(sl),
+                                            C.byref(o), _ptr(T), _ptr(ncyc), _ptr(st)))
+    return T, ncyc, st
+
+
+def solve_scba_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, coupling: np.ndarray,
+                  niter: int, tol: float, frm, ref: int, slot=None, delta: float = 1e-4,
+                  precision: int = FP64, max_batch: int = 0, want_diag: bool = False):
+    """Elastic-dephasing SCBA + Meir-Wingreen per energy (host arrays).
+    frm: [ncont+1] shared or [n, ncont+1] per item.  Returns dict."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    n = len(E)
+    s = ds.template
+    sl = np.zeros(n, dtype=np.int32) if slot is None else np.ascontiguousarray(slot, dtype=np.int32)
+    fr = np.ascontiguousarray(frm, dtype=np.float64)
+    per_item = 1 if fr.ndim == 2 else 0
+    cp = np.ascontiguousarray(coupling, dtype=np.float64)
+    curr = np.zeros((n, s.ncont))
+    iters = np.zeros(n, dtype=np.int32)
+    err = np.zeros(n)
+    st = np.zeros(n, dtype=np.int32)
+    gn = np.zeros((n, s.ndev), dtype=np.complex128) if want_diag else None
+    gr = np.zeros((n, s.ndev), dtype=np.complex128) if want_diag else None
+    o = _opts(delta, 1, 2, precision, max_batch)
+    check(load().hipnegf_solve_scba_mw(
+        eng.handle, ds.handle, n, _ptr(E), _ptr(sl), C.byref(o), _ptr(cp), niter, tol, _ptr(fr),
+        per_item, ref, _ptr(curr), _ptr(iters), _ptr(err), _ptr(st),
+        None if gn is None else _ptr(gn), None if gr is None else _ptr(gr)))
+    return dict(curr=curr, iters=iters, err=err, status=st, gn_diag=gn, gr_diag=gr)

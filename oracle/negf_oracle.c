@@ -1,0 +1,923 @@
+/*
+ * negf_oracle.c -- CPU restatement of libNEGF's per-energy Green's-function path.
+ *
+ * TEST INFRASTRUCTURE ONLY (parity checker and CPU baseline).  Nothing in the
+ * product (libnegf_amd/) links or calls this file.
+ *
+ * Each function cites the reference file:line it restates.  Dense algebra
+ * goes through or_zgemm / or_zinv, which either run the built-in loops below
+ * or call ZGEMM/ZGETRF/ZGETRS of a dlopen'd MKL (libmkl_rt.so, sequential
+ * threading layer), the same LAPACK entry points the reference calls
+ * (sparsekit_drv.F90:1934, inversions.F90:821,832).
+ */
+#define _GNU_SOURCE
+#include "negf_oracle.h"
+
+#include <dlfcn.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ======================================================================= */
+/* dense backend                                                            */
+/* ======================================================================= */
+
+typedef void (*zgemm_fn)(const char *, const char *, const int *, const int *, const int *,
+                         const zc *, const zc *, const int *, const zc *, const int *, const zc *,
+                         zc *, const int *);
+typedef void (*zgetrf_fn)(const int *, const int *, zc *, const int *, int *, int *);
+typedef void (*zgetrs_fn)(const char *, const int *, const int *, const zc *, const int *,
+                          const int *, zc *, const int *, int *);
+
+static int g_backend = 0;
+static zgemm_fn p_zgemm = 0;
+static zgetrf_fn p_zgetrf = 0;
+static zgetrs_fn p_zgetrs = 0;
+
+static int load_mkl(void) {
+  static int tried = 0, ok = 0;
+  if (tried) return ok;
+  tried = 1;
+  const char *names[] = {"libmkl_rt.so", "/opt/conda/lib/libmkl_rt.so", "libmkl_rt.so.1",
+                         "/opt/conda/lib/libmkl_rt.so.1", 0};
+  void *h = 0;
+  for (int i = 0; names[i] && !h; ++i) h = dlopen(names[i], RTLD_NOW | RTLD_LOCAL);
+  if (!h) return 0;
+  /* MKL_THREADING_SEQUENTIAL = 1: one MKL thread per calling thread */
+  int (*set_layer)(int) = (int (*)(int))dlsym(h, "MKL_Set_Threading_Layer");
+  if (set_layer) set_layer(1);
+  p_zgemm = (zgemm_fn)dlsym(h, "zgemm_");
+  p_zgetrf = (zgetrf_fn)dlsym(h, "zgetrf_");
+  p_zgetrs = (zgetrs_fn)dlsym(h, "zgetrs_");
+  ok = p_zgemm && p_zgetrf && p_zgetrs;
+  return ok;
+}
+
+int or_set_backend(int which) {
+  if (which == 1) {
+    if (!load_mkl()) return -1;
+    g_backend = 1;
+  } else {
+    g_backend = 0;
+  }
+  return g_backend;
+}
+int or_get_backend(void) { return g_backend; }
+const char *or_backend_name(void) { return g_backend == 1 ? "MKL (libmkl_rt, sequential)" : "built-in"; }
+
+static inline zc opA(char t, const zc *A, int lda, int i, int l) {
+  /* element (i,l) of op(A) */
+  if (t == 'N') return A[i + (size_t)l * lda];
+  if (t == 'T') return A[l + (size_t)i * lda];
+  return conj(A[l + (size_t)i * lda]);
+}
+
+void or_zgemm(char ta, char tb, int m, int n, int k, zc alpha, const zc *A, int lda,
+              const zc *B, int ldb, zc beta, zc *C, int ldc) {
+  if (m <= 0 || n <= 0) return;
+  if (g_backend == 1) {
+    char a[2] = {ta, 0}, b[2] = {tb, 0};
+    p_zgemm(a, b, &m, &n, &k, &alpha, A, &lda, B, &ldb, &beta, C, &ldc);
+    return;
+  }
+  /* built-in: C = alpha op(A) op(B) + beta C, column-major */
+  zc *acol = (zc *)malloc(sizeof(zc) * (size_t)m);
+  for (int j = 0; j < n; ++j) {
+    zc *c = C + (size_t)j * ldc;
+    if (beta == 0.0) {
+      for (int i = 0; i < m; ++i) c[i] = 0.0;
+    } else if (beta != 1.0) {
+      for (int i = 0; i < m; ++i) c[i] *= beta;
+    }
+    for (int l = 0; l < k; ++l) {
+      zc b = alpha * opA(tb, B, ldb, l, j);
+      if (b == 0.0) continue;
+      if (ta == 'N') {
+        const zc *a = A + (size_t)l * lda;
+        for (int i = 0; i < m; ++i) c[i] += a[i] * b;
+      } else {
+        for (int i = 0; i < m; ++i) acol[i] = opA(ta, A, lda, i, l);
+        for (int i = 0; i < m; ++i) c[i] += acol[i] * b;
+      }
+    }
+  }
+  free(acol);
+}
+
+static inline double cabs1(zc z) { return fabs(creal(z)) + fabs(cimag(z)); }
+
+/* ZGETRF (partial pivoting, pivot = first max of |Re|+|Im| as IZAMAX) +
+ * ZGETRS against the identity: zinv, inversions.F90:807-839. */
+int or_zinv(int n, const zc *A, zc *Ainv) {
+  if (n <= 0) return 0;
+  zc *LU = (zc *)malloc(sizeof(zc) * (size_t)n * n);
+  int *ipiv = (int *)malloc(sizeof(int) * (size_t)n);
+  memcpy(LU, A, sizeof(zc) * (size_t)n * n);
+  int info = 0;
+  for (size_t i = 0; i < (size_t)n * n; ++i) Ainv[i] = 0.0;
+  for (int i = 0; i < n; ++i) Ainv[i + (size_t)i * n] = 1.0;
+  if (g_backend == 1) {
+    p_zgetrf(&n, &n, LU, &n, ipiv, &info);
+    if (info == 0) {
+      int i2 = 0;
+      p_zgetrs("N", &n, &n, LU, &n, ipiv, Ainv, &n, &i2);
+      info = i2;
+    }
+  } else {
+    for (int j = 0; j < n; ++j) {
+      int p = j;
+      double best = cabs1(LU[j + (size_t)j * n]);
+      for (int i = j + 1; i < n; ++i) {
+        double v = cabs1(LU[i + (size_t)j * n]);
+        if (v > best) { best = v; p = i; }
+      }
+      ipiv[j] = p;
+      if (LU[p + (size_t)j * n] == 0.0) {
+        if (info == 0) info = j + 1;
+        continue;
+      }
+      if (p != j)
+        for (int c = 0; c < n; ++c) {
+          zc t = LU[j + (size_t)c * n];
+          LU[j + (size_t)c * n] = LU[p + (size_t)c * n];
+          LU[p + (size_t)c * n] = t;
+        }
+      zc inv = 1.0 / LU[j + (size_t)j * n];
+      for (int i = j + 1; i < n; ++i) LU[i + (size_t)j * n] *= inv;
+      for (int c = j + 1; c < n; ++c) {
+        zc u = LU[j + (size_t)c * n];
+        if (u == 0.0) continue;
+        zc *col = LU + (size_t)c * n;
+        const zc *l = LU + (size_t)j * n;
+        for (int i = j + 1; i < n; ++i) col[i] -= l[i] * u;
+      }
+    }
+    if (info == 0) {
+      /* apply row interchanges to the identity, then L y = b, U x = y */
+      for (int j = 0; j < n; ++j)
+        if (ipiv[j] != j)
+          for (int c = 0; c < n; ++c) {
+            zc t = Ainv[j + (size_t)c * n];
+            Ainv[j + (size_t)c * n] = Ainv[ipiv[j] + (size_t)c * n];
+            Ainv[ipiv[j] + (size_t)c * n] = t;
+          }
+      for (int c = 0; c < n; ++c) {
+        zc *x = Ainv + (size_t)c * n;
+        for (int j = 0; j < n; ++j) {
+          zc xj = x[j];
+          if (xj == 0.0) continue;
+          const zc *l = LU + (size_t)j * n;
+          for (int i = j + 1; i < n; ++i) x[i] -= xj * l[i];
+        }
+        for (int j = n - 1; j >= 0; --j) {
+          x[j] /= LU[j + (size_t)j * n];
+          zc xj = x[j];
+          if (xj == 0.0) continue;
+          const zc *u = LU + (size_t)j * n;
+          for (int i = 0; i < j; ++i) x[i] -= xj * u[i];
+        }
+      }
+    }
+  }
+  free(LU);
+  free(ipiv);
+  return info;
+}
+
+/* ======================================================================= */
+/* small helpers                                                            */
+/* ======================================================================= */
+
+static zc *zalloc(size_t n) { return (zc *)calloc(n ? n : 1, sizeof(zc)); }
+
+/* C = A^dagger (A: m x n -> C: n x m); zdagger */
+static void zdag(int m, int n, const zc *A, zc *C) {
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < m; ++i) C[j + (size_t)i * n] = conj(A[i + (size_t)j * m]);
+}
+
+/* Gamma = i (S - S^dagger), zspectral_dns, sparsekit_drv.F90:2870-2891 */
+static void zspectral(int n, const zc *S, zc *G) {
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i)
+      G[i + (size_t)j * n] = I * (S[i + (size_t)j * n] - conj(S[j + (size_t)i * n]));
+}
+
+static double max_abs(size_t n, const zc *A) {
+  double m = 0.0;
+  for (size_t i = 0; i < n; ++i) {
+    double v = cabs(A[i]);
+    if (v > m) m = v;
+  }
+  return m;
+}
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* ======================================================================= */
+/* contacts                                                                 */
+/* ======================================================================= */
+
+/* contselfenergy.F90:271-340 (CPU decimation).  Convergence: max|C1| <=
+ * sgfacc on two consecutive iterations (okCo, :300-310); <= 300 iterations;
+ * the last iterate is used if not converged (:289-335). */
+int or_decimation(int n, zc *Go, zc *Ao, zc *Bo, zc *Co, int *okco, double sgfacc) {
+  size_t nn = (size_t)n * n;
+  int okCo = okco ? *okco : 0;
+  zc *Ao_s = zalloc(nn), *GoXCo = zalloc(nn), *C1 = zalloc(nn), *Self = zalloc(nn);
+  zc *GoXBo = zalloc(nn), *B1 = zalloc(nn);
+  memcpy(Ao_s, Ao, nn * sizeof(zc));
+  int ncyc = 0, i1;
+  for (i1 = 1; i1 <= 300; ++i1) {
+    ncyc = i1;
+    or_zinv(n, Ao, Go);
+    or_zgemm('N', 'N', n, n, n, 1.0, Go, n, Co, n, 0.0, GoXCo, n);
+    or_zgemm('N', 'N', n, n, n, 1.0, Co, n, GoXCo, n, 0.0, C1, n);
+    if (max_abs(nn, C1) <= sgfacc) {
+      if (okCo) break;
+      okCo = 1;
+    } else {
+      okCo = 0;
+    }
+    or_zgemm('N', 'N', n, n, n, 1.0, Bo, n, GoXCo, n, 0.0, Self, n);
+    for (size_t t = 0; t < nn; ++t) {
+      Ao_s[t] -= Self[t];
+      Ao[t] -= Self[t];
+    }
+    or_zgemm('N', 'N', n, n, n, 1.0, Go, n, Bo, n, 0.0, GoXBo, n);
+    or_zgemm('N', 'N', n, n, n, 1.0, Bo, n, GoXBo, n, 0.0, B1, n);
+    memcpy(Bo, B1, nn * sizeof(zc));
+    or_zgemm('N', 'N', n, n, n, -1.0, Co, n, GoXBo, n, 1.0, Ao, n);
+    memcpy(Co, C1, nn * sizeof(zc));
+  }
+  ncyc = i1; /* :335 -- after an exhausted DO loop i1 is 301 */
+  or_zinv(n, Ao_s, Go);
+  if (okco) *okco = okCo;
+  free(Ao_s); free(GoXCo); free(C1); free(Self); free(GoXBo); free(B1);
+  return ncyc;
+}
+
+/* surface_green Ao/Bo/Co (contselfenergy.F90:207-210) + decimation, then
+ * Tlc = z SMC - HMC, Tcl = (z* SMC - HMC)^dag (:475-479) and
+ * Sigma = Tlc GS Tcl (SelfEnergy_dns :552-566). */
+int or_contact_sigma(zc z, int nc, int nd, const zc *H00, const zc *S00, const zc *H01,
+                     const zc *S01, const zc *HMC, const zc *SMC, zc *Sigma, zc *GS,
+                     int *okco, double sgfacc) {
+  size_t nn = (size_t)nc * nc;
+  zc *Ao = zalloc(nn), *Bo = zalloc(nn), *Co = zalloc(nn), *Cp = zalloc(nn), *Go = zalloc(nn);
+  for (int j = 0; j < nc; ++j)
+    for (int i = 0; i < nc; ++i) {
+      size_t t = i + (size_t)j * nc;
+      zc s0 = S00 ? S00[t] : (i == j ? 1.0 : 0.0);
+      zc s1 = S01 ? S01[t] : 0.0;
+      Ao[t] = z * s0 - H00[t];
+      Bo[t] = z * s1 - H01[t];
+      Cp[t] = conj(z) * s1 - H01[t];
+    }
+  zdag(nc, nc, Cp, Co);
+  int ncyc = or_decimation(nc, Go, Ao, Bo, Co, okco, sgfacc);
+  if (GS) memcpy(GS, Go, nn * sizeof(zc));
+  size_t md = (size_t)nd * nc;
+  zc *Tlc = zalloc(md), *Tp = zalloc(md), *Tcl = zalloc(md), *TG = zalloc(md);
+  for (size_t t = 0; t < md; ++t) {
+    zc s = SMC ? SMC[t] : 0.0;
+    Tlc[t] = z * s - HMC[t];
+    Tp[t] = conj(z) * s - HMC[t];
+  }
+  zdag(nd, nc, Tp, Tcl); /* nc x nd */
+  or_zgemm('N', 'N', nd, nc, nc, 1.0, Tlc, nd, Go, nc, 0.0, TG, nd);
+  or_zgemm('N', 'N', nd, nd, nc, 1.0, TG, nd, Tcl, nc, 0.0, Sigma, nd);
+  free(Ao); free(Bo); free(Co); free(Cp); free(Go);
+  free(Tlc); free(Tp); free(Tcl); free(TG);
+  return ncyc;
+}
+
+/* ======================================================================= */
+/* device sweeps                                                            */
+/* ======================================================================= */
+
+typedef struct {
+  int nbl;
+  const int *b;
+  zc **d, **u, **l; /* ESH(i,i), ESH(i,i+1), ESH(i+1,i) */
+} esh_t;
+
+/* build_ESH (iterative.F90:1956-1982): ESH = E S - H on the tridiagonal
+ * blocks, then ESH(cblk,cblk) -= Sigma(contact). */
+static esh_t build_esh(const or_struct *s, zc E, zc *const *Sigma) {
+  esh_t e;
+  int nbl = s->nbl;
+  e.nbl = nbl;
+  e.b = s->bsz;
+  e.d = (zc **)calloc(nbl, sizeof(zc *));
+  e.u = (zc **)calloc(nbl > 1 ? nbl - 1 : 1, sizeof(zc *));
+  e.l = (zc **)calloc(nbl > 1 ? nbl - 1 : 1, sizeof(zc *));
+  for (int i = 0; i < nbl; ++i) {
+    int n = s->bsz[i];
+    e.d[i] = zalloc((size_t)n * n);
+    for (int c = 0; c < n; ++c)
+      for (int r = 0; r < n; ++r) {
+        size_t t = r + (size_t)c * n;
+        zc sv = (s->Sd && s->Sd[i]) ? s->Sd[i][t] : (r == c ? 1.0 : 0.0);
+        e.d[i][t] = E * sv - s->Hd[i][t];
+      }
+  }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    int n0 = s->bsz[i], n1 = s->bsz[i + 1];
+    e.u[i] = zalloc((size_t)n0 * n1);
+    e.l[i] = zalloc((size_t)n0 * n1);
+    for (size_t t = 0; t < (size_t)n0 * n1; ++t) {
+      zc su = (s->Su && s->Su[i]) ? s->Su[i][t] : 0.0;
+      zc sl = (s->Sl && s->Sl[i]) ? s->Sl[i][t] : 0.0;
+      e.u[i][t] = E * su - s->Hu[i][t];
+      e.l[i][t] = E * sl - s->Hl[i][t];
+    }
+  }
+  if (Sigma)
+    for (int c = 0; c < s->ncont; ++c) {
+      int bk = s->cblk[c] - 1, n = s->bsz[bk];
+      for (size_t t = 0; t < (size_t)n * n; ++t) e.d[bk][t] -= Sigma[c][t];
+    }
+  return e;
+}
+
+static void free_esh(esh_t *e) {
+  for (int i = 0; i < e->nbl; ++i) free(e->d[i]);
+  for (int i = 0; i + 1 < e->nbl; ++i) { free(e->u[i]); free(e->l[i]); }
+  free(e->d); free(e->u); free(e->l);
+}
+
+/* calculate_gsmr_blocks (iterative_cpu.F90:51-124) from sbl=nbl down to ebl
+ * (1-based); g[i] allocated for i in [ebl-1, nbl-1] (0-based). */
+static void gsmr_blocks(const esh_t *e, int ebl, zc **g) {
+  int nbl = e->nbl;
+  if (nbl < ebl || nbl == 1) return;
+  int n = e->b[nbl - 1];
+  g[nbl - 1] = zalloc((size_t)n * n);
+  or_zinv(n, e->d[nbl - 1], g[nbl - 1]);
+  for (int i = nbl - 1; i >= ebl; --i) { /* 1-based i -> 0-based i-1 */
+    int a = i - 1, ni = e->b[a], nj = e->b[a + 1];
+    zc *w1 = zalloc((size_t)ni * nj), *w2 = zalloc((size_t)ni * ni);
+    or_zgemm('N', 'N', ni, nj, nj, -1.0, e->u[a], ni, g[a + 1], nj, 0.0, w1, ni);
+    or_zgemm('N', 'N', ni, ni, nj, 1.0, w1, ni, e->l[a], nj, 0.0, w2, ni);
+    for (size_t t = 0; t < (size_t)ni * ni; ++t) w2[t] += e->d[a][t];
+    g[a] = zalloc((size_t)ni * ni);
+    or_zinv(ni, w2, g[a]);
+    free(w1); free(w2);
+  }
+}
+
+/* calculate_Gr_tridiag_blocks sbl=1 form (iterative_cpu.F90:172-198):
+ * G11 = inv(ESH11 - ESH12 g2 ESH21). */
+static void gr_11(const esh_t *e, zc *const *g, zc *G11) {
+  int n = e->b[0];
+  if (e->nbl == 1) {
+    or_zinv(n, e->d[0], G11);
+    return;
+  }
+  int n1 = e->b[1];
+  zc *w2 = zalloc((size_t)n * n1), *w3 = zalloc((size_t)n * n);
+  or_zgemm('N', 'N', n, n1, n1, 1.0, e->u[0], n, g[1], n1, 0.0, w2, n);
+  or_zgemm('N', 'N', n, n, n1, 1.0, w2, n, e->l[0], n1, 0.0, w3, n);
+  for (size_t t = 0; t < (size_t)n * n; ++t) w3[t] = e->d[0][t] - w3[t];
+  or_zinv(n, w3, G11);
+  free(w2); free(w3);
+}
+
+/* range form, iterative_cpu.F90:200-219; sbl..ebl 1-based, sbl >= 2.
+ * Gd[i], Gu[i]=Gr(i,i+1), Gl[i]=Gr(i+1,i) (0-based) */
+static void gr_tridiag(const esh_t *e, zc *const *g, int sbl, int ebl, zc **Gd, zc **Gu,
+                       zc **Gl) {
+  for (int i1 = sbl; i1 <= ebl; ++i1) {
+    int i = i1 - 1, im = i - 1, ni = e->b[i], nm = e->b[im];
+    zc *w1 = zalloc((size_t)ni * nm), *w2 = zalloc((size_t)nm * ni), *w3 = zalloc((size_t)ni * ni);
+    Gl[im] = zalloc((size_t)ni * nm);
+    Gu[im] = zalloc((size_t)nm * ni);
+    Gd[i] = zalloc((size_t)ni * ni);
+    or_zgemm('N', 'N', ni, nm, ni, 1.0, g[i], ni, e->l[im], ni, 0.0, w1, ni);
+    or_zgemm('N', 'N', ni, nm, nm, -1.0, w1, ni, Gd[im], nm, 0.0, Gl[im], ni);
+    or_zgemm('N', 'N', nm, ni, ni, 1.0, e->u[im], nm, g[i], ni, 0.0, w2, nm);
+    or_zgemm('N', 'N', nm, ni, nm, -1.0, Gd[im], nm, w2, nm, 0.0, Gu[im], nm);
+    or_zgemm('N', 'N', ni, ni, nm, -1.0, Gl[im], ni, w2, nm, 0.0, w3, ni);
+    for (size_t t = 0; t < (size_t)ni * ni; ++t) Gd[i][t] = g[i][t] + w3[t];
+    free(w1); free(w2); free(w3);
+  }
+}
+
+static double trace_re(int n, const zc *A) {
+  double t = 0.0;
+  for (int i = 0; i < n; ++i) t += creal(A[i + (size_t)i * n]);
+  return t;
+}
+
+/* calculate_single_transmission_2_contacts (iterative_cpu.F90:399-470):
+ * T = |Re Tr[Gam A - Gam G Gam G^dag]| on block 1. */
+static double trans_2c(int n, const zc *G11, const zc *Sig) {
+  size_t nn = (size_t)n * n;
+  zc *GA = zalloc(nn), *Gam = zalloc(nn), *w1 = zalloc(nn), *w2 = zalloc(nn), *AA = zalloc(nn);
+  zdag(n, n, G11, GA);
+  zspectral(n, Sig, Gam);
+  or_zgemm('N', 'N', n, n, n, 1.0, Gam, n, G11, n, 0.0, w1, n);
+  or_zgemm('N', 'N', n, n, n, 1.0, w1, n, Gam, n, 0.0, w2, n);
+  or_zgemm('N', 'N', n, n, n, 1.0, w2, n, GA, n, 0.0, w1, n);
+  for (size_t t = 0; t < nn; ++t) AA[t] = I * (G11[t] - GA[t]);
+  or_zgemm('N', 'N', n, n, n, 1.0, Gam, n, AA, n, 0.0, w2, n);
+  for (size_t t = 0; t < nn; ++t) w2[t] -= w1[t];
+  double T = fabs(trace_re(n, w2));
+  free(GA); free(Gam); free(w1); free(w2); free(AA);
+  return T;
+}
+
+double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int ni, int nf,
+                             double eps_drop) {
+  int nbl = s->nbl, ncont = s->ncont;
+  if (ncont == 1) return 0.0;
+  esh_t e = build_esh(s, E, Sigma);
+  int cbi = s->cblk[ni - 1], cbf = s->cblk[nf - 1];
+  int nt = cbi < cbf ? cbi : cbf;
+  zc **g = (zc **)calloc(nbl, sizeof(zc *));
+  double T = 0.0;
+  if (ncont == 2 && nt == 1) {
+    /* fast path, iterative.F90:1704-1715 */
+    gsmr_blocks(&e, 2, g);
+    zc *G11 = zalloc((size_t)e.b[0] * e.b[0]);
+    gr_11(&e, g, G11);
+    int ct1 = (cbi < cbf) ? ni : nf;
+    T = trans_2c(e.b[0], G11, Sigma[ct1 - 1]);
+    free(G11);
+  } else {
+    /* multiterminal path, iterative.F90:1716-1766, one (ni,nf) pair */
+    gsmr_blocks(&e, 2, g);
+    zc **Gd = (zc **)calloc(nbl, sizeof(zc *));
+    zc **Gu = (zc **)calloc(nbl, sizeof(zc *)), **Gl = (zc **)calloc(nbl, sizeof(zc *));
+    Gd[0] = zalloc((size_t)e.b[0] * e.b[0]);
+    gr_11(&e, g, Gd[0]);
+    int nt1 = cbi > cbf ? cbi : cbf;
+    if (nt1 > 1) gr_tridiag(&e, g, 2, nt1, Gd, Gu, Gl);
+    /* calculate_single_transmission_N_contacts, iterative_cpu.F90:477-583 */
+    int ct1, ct2;
+    if (cbi < cbf) { ct1 = ni; ct2 = nf; } else { ct1 = nf; ct2 = ni; }
+    int bl1 = s->cblk[ct1 - 1], bl2 = s->cblk[ct2 - 1];
+    /* column blocks Gr(i,bl1), i = bl1..bl2 */
+    zc **Gc = (zc **)calloc(nbl + 1, sizeof(zc *));
+    Gc[bl1] = Gd[bl1 - 1];
+    if (bl2 > bl1) Gc[bl1 + 1] = Gl[bl1 - 1];
+    int early = 0;
+    if (nbl > 1 && bl2 - bl1 > 1) {
+      for (int i = bl1 + 1; i <= bl2; ++i) {
+        int nprev = e.b[i - 2], nr = e.b[bl1 - 1];
+        double mx = max_abs((size_t)nprev * nr, Gc[i - 1]);
+        if (mx < eps_drop) { T = eps_drop * eps_drop; early = 1; break; }
+        if (!Gc[i]) {
+          int nii = e.b[i - 1];
+          zc *w1 = zalloc((size_t)nii * nprev);
+          or_zgemm('N', 'N', nii, nprev, nii, -1.0, g[i - 1], nii, e.l[i - 2], nii, 0.0, w1, nii);
+          Gc[i] = zalloc((size_t)nii * nr);
+          or_zgemm('N', 'N', nii, nr, nprev, 1.0, w1, nii, Gc[i - 1], nprev, 0.0, Gc[i], nii);
+          free(w1);
+        }
+      }
+    }
+    if (!early) {
+      int n1 = e.b[bl1 - 1], n2 = e.b[bl2 - 1];
+      zc *G21 = Gc[bl2];
+      zc *Gam1 = zalloc((size_t)n1 * n1), *Gam2 = zalloc((size_t)n2 * n2);
+      zspectral(n1, Sigma[ct1 - 1], Gam1);
+      zspectral(n2, Sigma[ct2 - 1], Gam2);
+      zc *w1 = zalloc((size_t)n2 * n1), *w2 = zalloc((size_t)n2 * n1), *GA = zalloc((size_t)n1 * n2);
+      zc *TRS = zalloc((size_t)n2 * n2);
+      or_zgemm('N', 'N', n2, n1, n2, 1.0, Gam2, n2, G21, n2, 0.0, w1, n2);
+      or_zgemm('N', 'N', n2, n1, n1, 1.0, w1, n2, Gam1, n1, 0.0, w2, n2);
+      zdag(n2, n1, G21, GA);
+      or_zgemm('N', 'N', n2, n2, n1, 1.0, w2, n2, GA, n1, 0.0, TRS, n2);
+      T = fabs(trace_re(n2, TRS));
+      free(Gam1); free(Gam2); free(w1); free(w2); free(GA); free(TRS);
+    }
+    for (int i = bl1 + 2; i <= bl2 && i <= nbl; ++i)
+      if (Gc[i]) free(Gc[i]);
+    free(Gc);
+    for (int i = 0; i < nbl; ++i) free(Gd[i]);
+    for (int i = 0; i + 1 < nbl; ++i) { free(Gu[i]); free(Gl[i]); }
+    free(Gd); free(Gu); free(Gl);
+  }
+  for (int i = 0; i < nbl; ++i) free(g[i]);
+  free(g);
+  free_esh(&e);
+  return T;
+}
+
+/* compute_contacts (contselfenergy.F90:445-491) at E + i delta, then
+ * calculate_transmissions at real E (integrations.F90:1750-1755). */
+double or_transmission(const or_struct *s, double E, double delta, int ni, int nf,
+                       int *ncyc_out, double sgfacc, double eps_drop) {
+  zc z = E + I * delta;
+  zc **Sig = (zc **)calloc(s->ncont, sizeof(zc *));
+  for (int c = 0; c < s->ncont; ++c) {
+    int nd = s->bsz[s->cblk[c] - 1];
+    Sig[c] = zalloc((size_t)nd * nd);
+    int ncyc = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
+                                s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
+                                0, 0, sgfacc);
+    if (ncyc_out) ncyc_out[c] = ncyc;
+  }
+  double T = or_transmission_sigma(s, E, Sig, ni, nf, eps_drop);
+  for (int c = 0; c < s->ncont; ++c) free(Sig[c]);
+  free(Sig);
+  return T;
+}
+
+int or_gr_tridiag(const or_struct *s, double E, zc *const *Sigma, zc *const *Grd,
+                  zc *const *Gru, zc *const *Grl) {
+  int nbl = s->nbl;
+  esh_t e = build_esh(s, E, Sigma);
+  zc **g = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gd = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gu = (zc **)calloc(nbl, sizeof(zc *)), **Gl = (zc **)calloc(nbl, sizeof(zc *));
+  gsmr_blocks(&e, 2, g);
+  Gd[0] = zalloc((size_t)e.b[0] * e.b[0]);
+  gr_11(&e, g, Gd[0]);
+  if (nbl > 1) gr_tridiag(&e, g, 2, nbl, Gd, Gu, Gl);
+  for (int i = 0; i < nbl; ++i) {
+    memcpy(Grd[i], Gd[i], sizeof(zc) * (size_t)e.b[i] * e.b[i]);
+    free(Gd[i]);
+    free(g[i]);
+  }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    if (Gru) memcpy(Gru[i], Gu[i], sizeof(zc) * (size_t)e.b[i] * e.b[i + 1]);
+    if (Grl) memcpy(Grl[i], Gl[i], sizeof(zc) * (size_t)e.b[i] * e.b[i + 1]);
+    free(Gu[i]); free(Gl[i]);
+  }
+  free(g); free(Gd); free(Gu); free(Gl);
+  free_esh(&e);
+  return 0;
+}
+
+/* ======================================================================= */
+/* G^n and the elastic SCBA                                                 */
+/* ======================================================================= */
+
+/* calculate_Gn_tridiag_blocks (iterative_cpu.F90:230-372) including the
+ * inner calculate_sigma_n fold.  Sn*: Sigma_n tridiagonal blocks (modified
+ * in place by the fold, as the reference does).  Output Gnd/Gnu/Gnl. */
+static void gn_tridiag(const esh_t *e, zc *const *g, zc *const *Grd, zc *const *Gru,
+                       zc *const *Grl, zc **Snd, zc **Snu, zc **Snl, zc **Gnd, zc **Gnu,
+                       zc **Gnl) {
+  int nbl = e->nbl;
+  /* calculate_sigma_n, :338-370 */
+  if (nbl > 1) {
+    int n = e->b[nbl - 1];
+    zc *gns = zalloc((size_t)n * n), *w = zalloc((size_t)n * n), *gd = zalloc((size_t)n * n);
+    zdag(n, n, g[nbl - 1], gd);
+    or_zgemm('N', 'N', n, n, n, 1.0, g[nbl - 1], n, Snd[nbl - 1], n, 0.0, w, n);
+    or_zgemm('N', 'N', n, n, n, 1.0, w, n, gd, n, 0.0, gns, n);
+    free(w); free(gd);
+    for (int i = nbl - 2; i >= 0; --i) {
+      int ni = e->b[i], nj = e->b[i + 1];
+      zc *Ed = zalloc((size_t)nj * ni), *wk = zalloc((size_t)ni * nj);
+      zdag(ni, nj, e->u[i], Ed);
+      /* Sigma(i,i) += ESH(i,i+1) gns ESH(i,i+1)^dag */
+      or_zgemm('N', 'N', ni, nj, nj, 1.0, e->u[i], ni, gns, nj, 0.0, wk, ni);
+      or_zgemm('N', 'N', ni, ni, nj, 1.0, wk, ni, Ed, nj, 1.0, Snd[i], ni);
+      free(gns);
+      gns = 0;
+      /* Sigma(i,i) -= Sigma(i,i+1) gsmr(i+1)^dag ESH(i,i+1)^dag */
+      zc *gdag = zalloc((size_t)nj * nj);
+      zdag(nj, nj, g[i + 1], gdag);
+      or_zgemm('N', 'N', ni, nj, nj, 1.0, Snu[i], ni, gdag, nj, 0.0, wk, ni);
+      or_zgemm('N', 'N', ni, ni, nj, -1.0, wk, ni, Ed, nj, 1.0, Snd[i], ni);
+      free(gdag);
+      /* Sigma(i,i) -= ESH(i,i+1) gsmr(i+1) Sigma(i+1,i) */
+      or_zgemm('N', 'N', ni, nj, nj, 1.0, e->u[i], ni, g[i + 1], nj, 0.0, wk, ni);
+      or_zgemm('N', 'N', ni, ni, nj, -1.0, wk, ni, Snl[i], nj, 1.0, Snd[i], ni);
+      free(wk); free(Ed);
+      if (i > 0) {
+        zc *gd2 = zalloc((size_t)ni * ni), *w2 = zalloc((size_t)ni * ni);
+        gns = zalloc((size_t)ni * ni);
+        zdag(ni, ni, g[i], gd2);
+        or_zgemm('N', 'N', ni, ni, ni, 1.0, g[i], ni, Snd[i], ni, 0.0, w2, ni);
+        or_zgemm('N', 'N', ni, ni, ni, 1.0, w2, ni, gd2, ni, 0.0, gns, ni);
+        free(gd2); free(w2);
+      }
+    }
+    if (gns) free(gns);
+  }
+  /* Gn(1,1) = Gr(1,1) Sigma_n(1,1) Gr(1,1)^dag, :305-308 */
+  {
+    int n = e->b[0];
+    zc *Ga = zalloc((size_t)n * n), *w = zalloc((size_t)n * n);
+    zdag(n, n, Grd[0], Ga);
+    or_zgemm('N', 'N', n, n, n, 1.0, Snd[0], n, Ga, n, 0.0, w, n);
+    or_zgemm('N', 'N', n, n, n, 1.0, Grd[0], n, w, n, 0.0, Gnd[0], n);
+    free(Ga); free(w);
+  }
+  /* downward recursion, :319-349 */
+  for (int i = 0; i + 1 < nbl; ++i) {
+    int ni = e->b[i], nj = e->b[i + 1];
+    zc *w1 = zalloc((size_t)nj * ni), *Ga = zalloc((size_t)ni * ni);
+    zdag(ni, ni, Grd[i], Ga);
+    or_zgemm('N', 'N', nj, ni, ni, 1.0, Snl[i], nj, Ga, ni, 0.0, w1, nj);
+    free(Ga);
+    Ga = zalloc((size_t)nj * ni);
+    zdag(ni, nj, Gru[i], Ga); /* Gr(i,i+1)^dag : nj x ni */
+    or_zgemm('N', 'N', nj, ni, nj, 1.0, Snd[i + 1], nj, Ga, nj, 1.0, w1, nj);
+    free(Ga);
+    or_zgemm('N', 'N', nj, ni, ni, -1.0, e->l[i], nj, Gnd[i], ni, 1.0, w1, nj);
+    or_zgemm('N', 'N', nj, ni, nj, 1.0, g[i + 1], nj, w1, nj, 0.0, Gnl[i], nj);
+    free(w1);
+    zdag(nj, ni, Gnl[i], Gnu[i]);
+    zc *w2 = zalloc((size_t)nj * nj);
+    Ga = zalloc((size_t)ni * nj);
+    zdag(nj, ni, Grl[i], Ga); /* Gr(i+1,i)^dag: ni x nj */
+    or_zgemm('N', 'N', nj, nj, ni, 1.0, Snl[i], nj, Ga, ni, 0.0, w2, nj);
+    free(Ga);
+    Ga = zalloc((size_t)nj * nj);
+    zdag(nj, nj, Grd[i + 1], Ga);
+    or_zgemm('N', 'N', nj, nj, nj, 1.0, Snd[i + 1], nj, Ga, nj, 1.0, w2, nj);
+    free(Ga);
+    or_zgemm('N', 'N', nj, nj, ni, -1.0, e->l[i], nj, Gnu[i], ni, 1.0, w2, nj);
+    or_zgemm('N', 'N', nj, nj, nj, 1.0, g[i + 1], nj, w2, nj, 0.0, Gnd[i + 1], nj);
+    free(w2);
+  }
+}
+
+/* calculate_elastic_scba (iterative.F90:289-363) + ElPhonDephD (elphdd.F90)
+ * + iterative_meir_wingreen (iterative.F90:382-414). */
+int or_scba_mw_sigma(const or_struct *s, double E, zc *const *Sigma, const double *coupling,
+                     int niter, double tol, const double *frm, int ref, double *curr,
+                     double *err_out, zc *gn_diag, zc *gr_diag, double eps_drop) {
+  int nbl = s->nbl, ncont = s->ncont;
+  int ndev = 0;
+  int *off = (int *)calloc(nbl + 1, sizeof(int));
+  for (int i = 0; i < nbl; ++i) { off[i] = ndev; ndev += s->bsz[i]; }
+  off[nbl] = ndev;
+  double *c2 = (double *)calloc(ndev ? ndev : 1, sizeof(double));
+  for (int d = 0; d < ndev; ++d) c2[d] = coupling ? coupling[d] * coupling[d] : 0.0; /* elphdd.F90:94 */
+  zc *sig_r = zalloc(ndev), *sig_n = zalloc(ndev);
+  zc *mat_old = zalloc(ndev); /* masked previous Gn (diagonal part) */
+  /* off-diagonal S pattern entries also enter the convergence test (blk2csr) */
+  int have_old = 0, converged = 0;
+  double err = 0.0;
+  int iter = 0;
+  zc **Grd = (zc **)calloc(nbl, sizeof(zc *)), **Gru = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Grl = (zc **)calloc(nbl, sizeof(zc *)), **Gnd = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gnu = (zc **)calloc(nbl, sizeof(zc *)), **Gnl = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Snd = (zc **)calloc(nbl, sizeof(zc *)), **Snu = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Snl = (zc **)calloc(nbl, sizeof(zc *));
+  zc **g = (zc **)calloc(nbl, sizeof(zc *));
+  /* masked off-diagonal entries: store previous values of Gn(i,i+1), Gn(i+1,i) at S-pattern */
+  size_t noffd = 0;
+  for (int i = 0; i + 1 < nbl; ++i) noffd += 2 * (size_t)s->bsz[i] * s->bsz[i + 1];
+  zc *old_off = zalloc(noffd);
+  /* intra-block off-diagonal S entries of diagonal blocks */
+  size_t ndblk = 0;
+  for (int i = 0; i < nbl; ++i) ndblk += (size_t)s->bsz[i] * s->bsz[i];
+  zc *old_dblk = zalloc(ndblk);
+
+  while (!converged && iter <= niter) {
+    esh_t e = build_esh(s, E, Sigma);
+    if (iter > 0) /* add_sigma_r, elphdd.F90:108-128 */
+      for (int i = 0; i < nbl; ++i)
+        for (int d = 0; d < s->bsz[i]; ++d) e.d[i][d + (size_t)d * s->bsz[i]] -= sig_r[off[i] + d];
+    for (int i = 0; i < nbl; ++i) { free(g[i]); g[i] = 0; }
+    gsmr_blocks(&e, 2, g);
+    for (int i = 0; i < nbl; ++i) { free(Grd[i]); Grd[i] = 0; free(Gnd[i]); Gnd[i] = 0; }
+    for (int i = 0; i + 1 < nbl; ++i) {
+      free(Gru[i]); free(Grl[i]); free(Gnu[i]); free(Gnl[i]);
+      Gru[i] = Grl[i] = 0;
+      Gnu[i] = zalloc((size_t)s->bsz[i] * s->bsz[i + 1]);
+      Gnl[i] = zalloc((size_t)s->bsz[i] * s->bsz[i + 1]);
+    }
+    for (int i = 0; i < nbl; ++i) Gnd[i] = zalloc((size_t)s->bsz[i] * s->bsz[i]);
+    Grd[0] = zalloc((size_t)e.b[0] * e.b[0]);
+    gr_11(&e, g, Grd[0]);
+    if (nbl > 1) gr_tridiag(&e, g, 2, nbl, Grd, Gru, Grl);
+    if (iter <= niter) /* set_Gr, elphdd.F90:227-250 */
+      for (int i = 0; i < nbl; ++i)
+        for (int d = 0; d < s->bsz[i]; ++d)
+          sig_r[off[i] + d] = Grd[i][d + (size_t)d * s->bsz[i]] * c2[off[i] + d];
+    /* Sigma_n: init_tridiag_blk zeros + add_sigma_n + contact terms */
+    for (int i = 0; i < nbl; ++i) { free(Snd[i]); Snd[i] = zalloc((size_t)s->bsz[i] * s->bsz[i]); }
+    for (int i = 0; i + 1 < nbl; ++i) {
+      free(Snu[i]); free(Snl[i]);
+      Snu[i] = zalloc((size_t)s->bsz[i] * s->bsz[i + 1]);
+      Snl[i] = zalloc((size_t)s->bsz[i] * s->bsz[i + 1]);
+    }
+    if (iter > 0) /* add_sigma_n, elphdd.F90:135-156 (uses sigma_n of previous pass) */
+      for (int i = 0; i < nbl; ++i)
+        for (int d = 0; d < s->bsz[i]; ++d) Snd[i][d + (size_t)d * s->bsz[i]] += sig_n[off[i] + d];
+    for (int j = 0; j < ncont; ++j) {
+      double fd = frm[j] - frm[ref - 1];
+      if (j != ref - 1 && fabs(fd) > eps_drop) {
+        int cb = s->cblk[j] - 1, n = s->bsz[cb];
+        zc *Gam = zalloc((size_t)n * n);
+        zspectral(n, Sigma[j], Gam);
+        for (size_t t = 0; t < (size_t)n * n; ++t) Snd[cb][t] += fd * Gam[t];
+        free(Gam);
+      }
+    }
+    gn_tridiag(&e, g, Grd, Gru, Grl, Snd, Snu, Snl, Gnd, Gnu, Gnl);
+    if (iter <= niter) /* set_Gn, elphdd.F90:253-276 */
+      for (int i = 0; i < nbl; ++i)
+        for (int d = 0; d < s->bsz[i]; ++d)
+          sig_n[off[i] + d] = Gnd[i][d + (size_t)d * s->bsz[i]] * c2[off[i] + d];
+    /* check_Mat_convergence on blk2csr(Gn) masked by S (scba.F90:120-147) */
+    {
+      double e_max = 0.0;
+      size_t po = 0, pd = 0;
+      for (int i = 0; i < nbl; ++i) {
+        int n = s->bsz[i];
+        for (int c = 0; c < n; ++c)
+          for (int r = 0; r < n; ++r) {
+            size_t t = r + (size_t)c * n;
+            int inpat = (s->Sd && s->Sd[i]) ? (s->Sd[i][t] != 0.0) : (r == c);
+            if (inpat) {
+              zc v = Gnd[i][t];
+              if (have_old) { double dv = cabs(v - old_dblk[pd + t]); if (dv > e_max) e_max = dv; }
+              old_dblk[pd + t] = v;
+            }
+          }
+        pd += (size_t)n * n;
+      }
+      for (int i = 0; i + 1 < nbl; ++i) {
+        size_t m = (size_t)s->bsz[i] * s->bsz[i + 1];
+        for (size_t t = 0; t < m; ++t) {
+          int pu = (s->Su && s->Su[i]) ? (s->Su[i][t] != 0.0) : 0;
+          int pl = (s->Sl && s->Sl[i]) ? (s->Sl[i][t] != 0.0) : 0;
+          if (pu) {
+            if (have_old) { double dv = cabs(Gnu[i][t] - old_off[po + t]); if (dv > e_max) e_max = dv; }
+            old_off[po + t] = Gnu[i][t];
+          }
+          if (pl) {
+            if (have_old) { double dv = cabs(Gnl[i][t] - old_off[po + m + t]); if (dv > e_max) e_max = dv; }
+            old_off[po + m + t] = Gnl[i][t];
+          }
+        }
+        po += 2 * m;
+      }
+      if (have_old) {
+        err = e_max;
+        converged = err < tol;
+      }
+      have_old = 1;
+    }
+    free_esh(&e);
+    iter++;
+  }
+  /* iterative_meir_wingreen, iterative.F90:382-414 */
+  for (int ii = 0; ii < ncont; ++ii) {
+    int b = s->cblk[ii] - 1, n = s->bsz[b];
+    size_t nn = (size_t)n * n;
+    zc *Gam = zalloc(nn), *A = zalloc(nn), *w = zalloc(nn);
+    zspectral(n, Sigma[ii], Gam);
+    or_zgemm('N', 'N', n, n, n, 1.0, Gam, n, Gnd[b], n, 0.0, w, n);
+    zspectral(n, Grd[b], A);
+    or_zgemm('N', 'N', n, n, n, -frm[ii], Gam, n, A, n, 1.0, w, n);
+    curr[ii] = -trace_re(n, w);
+    free(Gam); free(A); free(w);
+  }
+  if (gn_diag)
+    for (int i = 0; i < nbl; ++i)
+      for (int d = 0; d < s->bsz[i]; ++d) gn_diag[off[i] + d] = Gnd[i][d + (size_t)d * s->bsz[i]];
+  if (gr_diag)
+    for (int i = 0; i < nbl; ++i)
+      for (int d = 0; d < s->bsz[i]; ++d) gr_diag[off[i] + d] = Grd[i][d + (size_t)d * s->bsz[i]];
+  if (err_out) *err_out = err;
+  for (int i = 0; i < nbl; ++i) { free(g[i]); free(Grd[i]); free(Gnd[i]); free(Snd[i]); }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    free(Gru[i]); free(Grl[i]); free(Gnu[i]); free(Gnl[i]); free(Snu[i]); free(Snl[i]);
+  }
+  free(g); free(Grd); free(Gru); free(Grl); free(Gnd); free(Gnu); free(Gnl);
+  free(Snd); free(Snu); free(Snl);
+  free(old_off); free(old_dblk); free(mat_old);
+  free(sig_r); free(sig_n); free(c2); free(off);
+  return iter;
+}
+
+int or_scba_meir_wingreen(const or_struct *s, double E, double delta, const double *coupling,
+                          int niter, double tol, const double *frm, int ref, double *curr,
+                          double *err_out, zc *gn_diag, zc *gr_diag, double sgfacc,
+                          double eps_drop) {
+  zc z = E + I * delta;
+  zc **Sig = (zc **)calloc(s->ncont, sizeof(zc *));
+  for (int c = 0; c < s->ncont; ++c) {
+    int nd = s->bsz[s->cblk[c] - 1];
+    Sig[c] = zalloc((size_t)nd * nd);
+    or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
+                     s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c], 0, 0,
+                     sgfacc);
+  }
+  int it = or_scba_mw_sigma(s, E, Sig, coupling, niter, tol, frm, ref, curr, err_out, gn_diag,
+                            gr_diag, eps_drop);
+  for (int c = 0; c < s->ncont; ++c) free(Sig[c]);
+  free(Sig);
+  return it;
+}
+
+/* ======================================================================= */
+/* energy grid and currents                                                 */
+/* ======================================================================= */
+
+/* fermi_f, distributions.F90:46-74 */
+double or_fermi(double E, double Ef, double kT) {
+  if (kT == 0.0) return E > Ef ? 0.0 : 1.0;
+  if (fabs((E - Ef) / kT) > 30) return E > Ef ? exp(-(E - Ef) / kT) : 1.0;
+  return 1.0 / (exp((E - Ef) / kT) + 1.0);
+}
+
+/* tunneling_int_def, integrations.F90:1619 */
+int or_npoints(double emin, double emax, double estep) {
+  return (int)lround((emax - emin) / estep) + 1;
+}
+
+/* integrate_el, integrations.F90:2688-2767 (Kb in Hartree/K, ln_constants.F90:36) */
+double or_integrate_el(const double *T, int npts, double mu1, double mu2, double kT1,
+                       double kT2, double emin, double emax, double estep) {
+  (void)npts;
+  const double Kb = 3.166811389002312e-6;
+  double curr = 0.0, destep = 1.0e10;
+  int N = 0;
+  int Np = (int)lround((emax - emin) / estep);
+  double kbT1 = kT1 < Kb ? Kb : kT1, kbT2 = kT2 < Kb ? Kb : kT2;
+  int swapped = 0;
+  if (mu2 < mu1) {
+    swapped = 1;
+    double t = mu1; mu1 = mu2; mu2 = t;
+    t = kbT1; kbT1 = kbT2; kbT2 = t;
+  }
+  for (int i = 0; i < Np; ++i) {
+    double E1 = emin + estep * i, TT1 = T[i];
+    double E2 = emin + estep * (i + 1), TT2 = T[i + 1];
+    while (destep >= (kbT1 + kbT2) / 20.0) {
+      N = N + 1;
+      destep = (E2 - E1) / N;
+    }
+    for (int i1 = 0; i1 < N; ++i1) {
+      double E3 = E1 + (E2 - E1) * i1 / N;
+      double E4 = E3 + (E2 - E1) / N;
+      double TT3 = (TT2 - TT1) * i1 / N + TT1;
+      double TT4 = TT3 + (TT2 - TT1) / N;
+      double c1 = (or_fermi(E3, mu2, kbT2) - or_fermi(E3, mu1, kbT1)) * TT3;
+      double c2 = (or_fermi(E4, mu2, kbT2) - or_fermi(E4, mu1, kbT1)) * TT4;
+      curr = curr + (c1 + c2) * (E4 - E3) / 2.0;
+    }
+  }
+  return swapped ? -curr : curr;
+}
+
+/* integrate_el_meir_wingreen, integrations.F90:2774-2832.
+ * integration: 0 trapezoidal, 1 simpson13, 2 simpson38 */
+double or_integrate_el_mw(int integration, const double *Iv, int npts, double estep) {
+  double *w = (double *)calloc(npts, sizeof(double));
+  if (integration == 0) {
+    for (int i = 0; i < npts; ++i) w[i] = estep;
+    w[0] = 0.5 * estep;
+    w[npts - 1] = 0.5 * estep;
+  } else if (integration == 1) {
+    for (int i = 1; i <= npts; ++i) {
+      int m = (i - 1) % 2 + 1;
+      w[i - 1] = estep / 3.0 * (double)(m * m);
+    }
+    for (int i = 3; i <= npts - 1; i += 2) w[i - 1] += estep / 3.0;
+  } else {
+    for (int i = 0; i < npts; ++i) w[i] = 9.0 / 8.0 * estep;
+    w[0] = 3.0 / 8.0 * estep;
+    for (int i = 4; i <= npts; i += 3) w[i - 1] = 6.0 / 8.0 * estep;
+  }
+  double c = 0.0;
+  for (int i = 0; i < npts; ++i) c += Iv[i] * w[i];
+  free(w);
+  return c;
+}
+
+int or_omp_max_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+double or_transmission_batch(const or_struct *s, int npts, const double *E, double delta, int ni,
+                             int nf, double *T, int *ncyc, int nthreads, double sgfacc) {
+  double t0 = now_s();
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+  for (int i = 0; i < npts; ++i) {
+    int nc[16];
+    T[i] = or_transmission(s, E[i], delta, ni, nf, nc, sgfacc, 0.0);
+    if (ncyc)
+      for (int c = 0; c < s->ncont; ++c) ncyc[(size_t)i * s->ncont + c] = nc[c];
+  }
+  (void)nthreads;
+  return now_s() - t0;
+}

@@ -1,0 +1,110 @@
+/*
+ * negf_oracle.h -- CPU restatement of libNEGF's per-energy Green's-function path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the HIP
+ * engine in libnegf_amd/.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product never links it.
+ *
+ * Conventions (same as the reference's z_DNS, mat_def.F90:199-204):
+ *   complex double, column-major, leading dimension = number of rows.
+ *   Block indices passed in are 1-based where the reference uses 1-based
+ *   indices (cblk, ni, nf), 0-based elsewhere.
+ */
+#ifndef NEGF_ORACLE_H
+#define NEGF_ORACLE_H
+
+#include <complex.h>
+
+typedef double _Complex zc;
+
+/* ---- dense backend -------------------------------------------------- */
+/* 0 = built-in loops (always available), 1 = dlopen'd MKL (libmkl_rt.so). */
+int  or_set_backend(int which);
+int  or_get_backend(void);
+const char *or_backend_name(void);
+void or_zgemm(char ta, char tb, int m, int n, int k, zc alpha, const zc *A, int lda,
+              const zc *B, int ldb, zc beta, zc *C, int ldc);
+/* inverse via ZGETRF + ZGETRS(identity); returns LAPACK info (0 = ok) */
+int  or_zinv(int n, const zc *A, zc *Ainv);
+
+/* ---- contacts --------------------------------------------------------- */
+/* Sancho-Rubio decimation, contselfenergy.F90:271-340.  Ao,Bo,Co are
+ * overwritten.  okco is the SAVE'd flag (contselfenergy.F90:284); pass NULL
+ * for a fresh .false. each call.  Returns ncyc. */
+int or_decimation(int n, zc *Go, zc *Ao, zc *Bo, zc *Co, int *okco, double sgfacc);
+
+/* surface_green (contselfenergy.F90:207-210, no surface block) followed by
+ * the self-energy Sigma = Tlc GS Tcl (contselfenergy.F90:475-484, 552-566).
+ * H00,S00,H01,S01: nc x nc.  HMC,SMC: nd x nc.  Sigma: nd x nd. GS: nc x nc
+ * (may be NULL).  Returns ncyc. */
+int or_contact_sigma(zc z, int nc, int nd, const zc *H00, const zc *S00, const zc *H01,
+                     const zc *S01, const zc *HMC, const zc *SMC, zc *Sigma, zc *GS,
+                     int *okco, double sgfacc);
+
+/* ---- device structure --------------------------------------------------- */
+typedef struct {
+  int nbl;                 /* number of PLs                                  */
+  const int *bsz;          /* [nbl] block sizes                              */
+  const zc *const *Hd;     /* [nbl]   H(i,i)                                 */
+  const zc *const *Sd;     /* [nbl]   S(i,i) or NULL entries -> identity     */
+  const zc *const *Hu;     /* [nbl-1] H(i,i+1)  bsz[i] x bsz[i+1]            */
+  const zc *const *Su;     /* [nbl-1] or NULL -> zero                        */
+  const zc *const *Hl;     /* [nbl-1] H(i+1,i)  bsz[i+1] x bsz[i]            */
+  const zc *const *Sl;     /* [nbl-1] or NULL -> zero                        */
+  int ncont;
+  const int *cblk;         /* [ncont] 1-based block of each contact           */
+  const int *nc;           /* [ncont] contact PL size                         */
+  const zc *const *H00, *const *S00, *const *H01, *const *S01; /* [ncont]     */
+  const zc *const *HMC, *const *SMC;                            /* [ncont]     */
+} or_struct;
+
+/* Per-energy coherent transmission between contacts ni -> nf (1-based),
+ * iterative.F90:1669-1780 (fast path or N-contact path chosen exactly as the
+ * reference does).  E real; contacts are evaluated at E + i*delta.
+ * ncyc_out[ncont] (may be NULL) receives the decimation iteration counts.
+ * Returns T (>=0, |Re Tr| as on the CPU path). */
+double or_transmission(const or_struct *s, double E, double delta, int ni, int nf,
+                       int *ncyc_out, double sgfacc, double eps_drop);
+
+/* Transmission from precomputed self-energies (build_ESH + sweeps only). */
+double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int ni, int nf,
+                             double eps_drop);
+
+/* Elastic dephasing SCBA + Meir-Wingreen current at one energy:
+ * calculate_elastic_scba (iterative.F90:289-363) with ElPhonDephD (elphdd.F90),
+ * then iterative_meir_wingreen (iterative.F90:382-414).
+ * coupling[ndev]: per-orbital coupling, squared internally (elphdd.F90:94).
+ * frm[ncont+1]: occupations, frm[ref-1] is the reference (ref = ncont+1 -> 0).
+ * curr[ncont] out.  gn_diag/gr_diag [ndev] (may be NULL) receive the final
+ * diagonals.  Returns the number of SCBA passes executed. */
+int or_scba_meir_wingreen(const or_struct *s, double E, double delta, const double *coupling,
+                          int niter, double tol, const double *frm, int ref, double *curr,
+                          double *err_out, zc *gn_diag, zc *gr_diag, double sgfacc,
+                          double eps_drop);
+
+/* Same, with self-energies supplied (no decimation). */
+int or_scba_mw_sigma(const or_struct *s, double E, zc *const *Sigma, const double *coupling,
+                     int niter, double tol, const double *frm, int ref, double *curr,
+                     double *err_out, zc *gn_diag, zc *gr_diag, double eps_drop);
+
+/* Gr tridiagonal blocks, full sweep (iterative.F90:226-233): outputs the
+ * diagonal blocks Gr(i,i) into Grd[i] (bsz[i]^2 each) and the off-diagonal
+ * blocks Gr(i,i+1), Gr(i+1,i) into Gru/Grl. */
+int or_gr_tridiag(const or_struct *s, double E, zc *const *Sigma, zc *const *Grd,
+                  zc *const *Gru, zc *const *Grl);
+
+/* ---- energy grid and currents --------------------------------------------- */
+double or_fermi(double E, double Ef, double kT);                  /* distributions.F90:46-74 */
+double or_integrate_el(const double *T, int npts, double mu1, double mu2, double kT1,
+                       double kT2, double emin, double emax, double estep); /* integrations.F90:2688-2767 */
+double or_integrate_el_mw(int integration, const double *Iv, int npts, double estep); /* :2774-2832 */
+int    or_npoints(double emin, double emax, double estep);           /* integrations.F90:1619 */
+
+/* Batched drivers for the CPU baseline: OpenMP over energies, one
+ * single-threaded dense backend per energy.  Returns seconds of wall time. */
+double or_transmission_batch(const or_struct *s, int npts, const double *E, double delta, int ni,
+                             int nf, double *T, int *ncyc, int nthreads, double sgfacc);
+
+int or_omp_max_threads(void);
+
+#endif

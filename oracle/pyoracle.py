@@ -1,0 +1,254 @@
+"""ctypes front end of the CPU oracle (liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+zptr = C.POINTER(C.c_double)  # complex arrays are passed as interleaved doubles
+
+
+def build(force: bool = False) -> str:
+    so = os.path.join(_HERE, "liboracle.so")
+    if force or not os.path.exists(so):
+        subprocess.check_call(["make", "-C", _HERE, "-B" if force else "all"],
+                              stdout=subprocess.DEVNULL)
+    return so
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        so = os.path.join(_HERE, "liboracle.so")
+        if not os.path.exists(so):
+            build()
+        L = C.CDLL(so)
+        L.or_set_backend.argtypes = [C.c_int]
+        L.or_backend_name.restype = C.c_char_p
+        L.or_transmission.restype = C.c_double
+        L.or_transmission.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_void_p, C.c_double, C.c_double]
+        L.or_transmission_sigma.restype = C.c_double
+        L.or_transmission_sigma.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.c_int,
+                                            C.c_double]
+        L.or_transmission_batch.restype = C.c_double
+        L.or_transmission_batch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_int,
+                                            C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double]
+        L.or_contact_sigma.restype = C.c_int
+        L.or_contact_sigma.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 8 + \
+            [C.c_void_p, C.c_double]
+        L.or_decimation.restype = C.c_int
+        L.or_decimation.argtypes = [C.c_int] + [C.c_void_p] * 5 + [C.c_double]
+        L.or_zinv.restype = C.c_int
+        L.or_zinv.argtypes = [C.c_int, C.c_void_p, C.c_void_p]
+        L.or_scba_meir_wingreen.restype = C.c_int
+        L.or_scba_meir_wingreen.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int,
+                                            C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                            C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        L.or_scba_mw_sigma.restype = C.c_int
+        L.or_scba_mw_sigma.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_int,
+                                       C.c_double, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_void_p, C.c_void_p, C.c_double]
+        L.or_gr_tridiag.restype = C.c_int
+        L.or_gr_tridiag.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p]
+        L.or_fermi.restype = C.c_double
+        L.or_fermi.argtypes = [C.c_double] * 3
+        L.or_integrate_el.restype = C.c_double
+        L.or_integrate_el.argtypes = [C.c_void_p, C.c_int] + [C.c_double] * 7
+        L.or_integrate_el_mw.restype = C.c_double
+        L.or_integrate_el_mw.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_double]
+        L.or_npoints.restype = C.c_int
+        L.or_npoints.argtypes = [C.c_double] * 3
+        L.or_omp_max_threads.restype = C.c_int
+        _LIB = L
+    return _LIB
+
+
+def set_backend(which: str) -> str:
+    """'builtin' or 'mkl'; returns the backend name actually in use."""
+    r = lib().or_set_backend(1 if which == "mkl" else 0)
+    if r < 0:
+        raise RuntimeError("MKL backend requested but libmkl_rt.so could not be loaded")
+    return lib().or_backend_name().decode()
+
+
+class _OrStruct(C.Structure):
+    _fields_ = [("nbl", C.c_int), ("bsz", C.c_void_p), ("Hd", C.c_void_p), ("Sd", C.c_void_p),
+                ("Hu", C.c_void_p), ("Su", C.c_void_p), ("Hl", C.c_void_p), ("Sl", C.c_void_p),
+                ("ncont", C.c_int), ("cblk", C.c_void_p), ("nc", C.c_void_p),
+                ("H00", C.c_void_p), ("S00", C.c_void_p), ("H01", C.c_void_p),
+                ("S01", C.c_void_p), ("HMC", C.c_void_p), ("SMC", C.c_void_p)]
+
+
+def _cz(a: np.ndarray) -> np.ndarray:
+    return np.asfortranarray(a, dtype=np.complex128)
+
+
+class OracleStruct:
+    """Keeps the numpy buffers alive while the C struct points at them."""
+
+    def __init__(self, s):
+        keep = []
+
+        def parr(mats: Optional[Sequence[Optional[np.ndarray]]], n: int):
+            if mats is None:
+                return None
+            arr = (C.c_void_p * max(n, 1))()
+            for i, m in enumerate(mats):
+                if m is None:
+                    arr[i] = None
+                else:
+                    m = _cz(m)
+                    keep.append(m)
+                    arr[i] = m.ctypes.data
+            keep.append(arr)
+            return C.cast(arr, C.c_void_p)
+
+        nbl = s.nbl
+        bsz = np.array(s.bsz, dtype=np.int32)
+        cblk = np.array([c.cblk for c in s.contacts], dtype=np.int32)
+        nc = np.array([c.nc for c in s.contacts], dtype=np.int32)
+        keep += [bsz, cblk, nc]
+        st = _OrStruct()
+        st.nbl = nbl
+        st.bsz = bsz.ctypes.data
+        st.Hd = parr(s.Hd, nbl)
+        st.Sd = parr(s.Sd, nbl)
+        st.Hu = parr(s.Hu, nbl - 1)
+        st.Su = parr(s.Su, nbl - 1)
+        st.Hl = parr(s.Hl, nbl - 1)
+        st.Sl = parr(s.Sl, nbl - 1)
+        st.ncont = len(s.contacts)
+        st.cblk = cblk.ctypes.data
+        st.nc = nc.ctypes.data
+        cs = s.contacts
+        st.H00 = parr([c.H00 for c in cs], len(cs))
+        st.H01 = parr([c.H01 for c in cs], len(cs))
+        st.HMC = parr([c.HMC for c in cs], len(cs))
+        anyS = any(c.S00 is not None for c in cs)
+        st.S00 = parr([c.S00 for c in cs], len(cs)) if anyS else None
+        st.S01 = parr([c.S01 for c in cs], len(cs)) if anyS else None
+        st.SMC = parr([c.SMC for c in cs], len(cs)) if anyS else None
+        self._keep = keep
+        self.c = st
+        self.s = s
+
+    @property
+    def ptr(self):
+        return C.byref(self.c)
+
+
+SGFACC = 1e-9  # ln_precision.F90:40
+
+
+def transmission(s, E: float, delta: float, ni: int = 1, nf: int = 2, eps_drop: float = 0.0):
+    os_ = OracleStruct(s)
+    ncyc = np.zeros(s.ncont, dtype=np.int32)
+    T = lib().or_transmission(os_.ptr, E, delta, ni, nf, ncyc.ctypes.data, SGFACC, eps_drop)
+    return T, ncyc
+
+
+def transmission_batch(s, E: np.ndarray, delta: float, ni: int = 1, nf: int = 2,
+                       nthreads: int = 0):
+    os_ = OracleStruct(s)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    T = np.zeros(len(E))
+    ncyc = np.zeros((len(E), s.ncont), dtype=np.int32)
+    secs = lib().or_transmission_batch(os_.ptr, len(E), E.ctypes.data, delta, ni, nf,
+                                       T.ctypes.data, ncyc.ctypes.data, nthreads, SGFACC)
+    return T, ncyc, secs
+
+
+def contact_sigma(c, nd: int, z: complex, okco: Optional[np.ndarray] = None):
+    """Returns (Sigma nd x nd, GS nc x nc, ncyc)."""
+    Sig = np.zeros((nd, nd), dtype=np.complex128, order="F")
+    GS = np.zeros((c.nc, c.nc), dtype=np.complex128, order="F")
+    bufs = [_cz(c.H00), None if c.S00 is None else _cz(c.S00), _cz(c.H01),
+            None if c.S01 is None else _cz(c.S01), _cz(c.HMC), None if c.SMC is None else _cz(c.SMC)]
+    ptrs = [None if b is None else b.ctypes.data for b in bufs]
+    ok = None if okco is None else okco.ctypes.data
+    ncyc = lib().or_contact_sigma(z.real, z.imag, c.nc, nd, *ptrs, Sig.ctypes.data,
+                                  GS.ctypes.data, ok, SGFACC)
+    return Sig, GS, ncyc
+
+
+def transmission_sigma(s, E: float, sigmas: List[np.ndarray], ni=1, nf=2, eps_drop=0.0):
+    os_ = OracleStruct(s)
+    sg = [_cz(x) for x in sigmas]
+    arr = (C.c_void_p * len(sg))(*[x.ctypes.data for x in sg])
+    return lib().or_transmission_sigma(os_.ptr, E, C.cast(arr, C.c_void_p), ni, nf, eps_drop)
+
+
+def zinv(A: np.ndarray):
+    A = _cz(A)
+    out = np.zeros_like(A)
+    info = lib().or_zinv(A.shape[0], A.ctypes.data, out.ctypes.data)
+    return out, info
+
+
+def scba_mw(s, E: float, delta: float, coupling: np.ndarray, niter: int, tol: float,
+            frm: Sequence[float], ref: int, eps_drop: float = 0.0, sigmas=None):
+    """Returns dict(curr, iters, err, gn_diag, gr_diag)."""
+    os_ = OracleStruct(s)
+    cp = np.ascontiguousarray(coupling, dtype=np.float64)
+    fr = np.ascontiguousarray(frm, dtype=np.float64)
+    curr = np.zeros(s.ncont)
+    err = np.zeros(1)
+    gn = np.zeros(s.ndev, dtype=np.complex128)
+    gr = np.zeros(s.ndev, dtype=np.complex128)
+    if sigmas is None:
+        it = lib().or_scba_meir_wingreen(os_.ptr, E, delta, cp.ctypes.data, niter, tol,
+                                         fr.ctypes.data, ref, curr.ctypes.data, err.ctypes.data,
+                                         gn.ctypes.data, gr.ctypes.data, SGFACC, eps_drop)
+    else:
+        sg = [_cz(x) for x in sigmas]
+        arr = (C.c_void_p * len(sg))(*[x.ctypes.data for x in sg])
+        it = lib().or_scba_mw_sigma(os_.ptr, E, C.cast(arr, C.c_void_p), cp.ctypes.data, niter,
+                                    tol, fr.ctypes.data, ref, curr.ctypes.data, err.ctypes.data,
+                                    gn.ctypes.data, gr.ctypes.data, eps_drop)
+    return dict(curr=curr, iters=it, err=float(err[0]), gn_diag=gn, gr_diag=gr)
+
+
+def gr_tridiag(s, E: float, sigmas: List[np.ndarray]):
+    os_ = OracleStruct(s)
+    sg = [_cz(x) for x in sigmas]
+    arr = (C.c_void_p * len(sg))(*[x.ctypes.data for x in sg])
+    b = s.bsz
+    Grd = [np.zeros((n, n), dtype=np.complex128, order="F") for n in b]
+    Gru = [np.zeros((b[i], b[i + 1]), dtype=np.complex128, order="F") for i in range(len(b) - 1)]
+    Grl = [np.zeros((b[i + 1], b[i]), dtype=np.complex128, order="F") for i in range(len(b) - 1)]
+    pd = (C.c_void_p * len(b))(*[x.ctypes.data for x in Grd])
+    pu = (C.c_void_p * max(1, len(b) - 1))(*[x.ctypes.data for x in Gru])
+    pl = (C.c_void_p * max(1, len(b) - 1))(*[x.ctypes.data for x in Grl])
+    lib().or_gr_tridiag(os_.ptr, E, C.cast(arr, C.c_void_p), C.cast(pd, C.c_void_p),
+                        C.cast(pu, C.c_void_p), C.cast(pl, C.c_void_p))
+    return Grd, Gru, Grl
+
+
+def fermi(E, Ef, kT):
+    return lib().or_fermi(E, Ef, kT)
+
+
+def integrate_el(T, mu1, mu2, kT1, kT2, emin, emax, estep):
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    return lib().or_integrate_el(T.ctypes.data, len(T), mu1, mu2, kT1, kT2, emin, emax, estep)
+
+
+def integrate_el_mw(integration, Iv, estep):
+    Iv = np.ascontiguousarray(Iv, dtype=np.float64)
+    return lib().or_integrate_el_mw(integration, Iv.ctypes.data, len(Iv), estep)
+
+
+def npoints(emin, emax, estep):
+    return lib().or_npoints(emin, emax, estep)

@@ -1,0 +1,52 @@
+"""Small test structures (test-only helpers)."""
+import numpy as np
+
+from libnegf_amd.structure import BlockStructure, Contact
+
+
+def chain_structure(npl=6, plsz=10, t=1.0, onsite=None):
+    """Reconstruction of the reference tests' 1-D chain: 60-site device in 6
+    PLs of 10 sites, contact 1 (sites 61..80) on block 6, contact 2 (sites
+    81..100) on block 1, hopping t, S = I (tests/f90elph_deph/hello.F90:42-48,
+    tests/c_int_elph_deph/c_int_elph_deph.c:15-21).  The reference's H data
+    are git-LFS objects; this rebuilds an equivalent uniform chain."""
+    H00 = np.zeros((plsz, plsz))
+    for i in range(plsz - 1):
+        H00[i, i + 1] = H00[i + 1, i] = t
+    Hd = []
+    for p in range(npl):
+        h = H00.copy()
+        if onsite is not None:
+            h += np.diag(onsite[p * plsz:(p + 1) * plsz])
+        Hd.append(h)
+    U = np.zeros((plsz, plsz))
+    U[plsz - 1, 0] = t
+    Hu = [U.copy() for _ in range(npl - 1)]
+    Hl = [U.T.copy() for _ in range(npl - 1)]
+    H01 = U.copy()               # PL1 -> PL2 within a contact (last site -> first site)
+    HMC_r = np.zeros((plsz, plsz)); HMC_r[plsz - 1, 0] = t   # site 60 <-> 61
+    HMC_l = np.zeros((plsz, plsz)); HMC_l[0, 0] = t          # site 1  <-> 81
+    c1 = Contact(cblk=npl, H00=H00, H01=H01, HMC=HMC_r)
+    c2 = Contact(cblk=1, H00=H00, H01=H01, HMC=HMC_l)
+    return BlockStructure(Hd=Hd, Hu=Hu, Hl=Hl, contacts=[c1, c2], name="chain")
+
+
+def dense_G(s, E, sigmas):
+    """Brute force: G = (E S - H - sum Sigma)^-1 of the whole device (numpy/LAPACK)."""
+    A = s.dense_device(E)
+    off = s.block_offsets()
+    for c, Sg in zip(s.contacts, sigmas):
+        b = c.cblk - 1
+        A[off[b]:off[b + 1], off[b]:off[b + 1]] -= Sg
+    return np.linalg.inv(A), off
+
+
+def gamma(S):
+    return 1j * (S - S.conj().T)
+
+
+def brute_T(s, E, sigmas, ci=0, cf=1):
+    G, off = dense_G(s, E, sigmas)
+    bi, bf = s.contacts[ci].cblk - 1, s.contacts[cf].cblk - 1
+    Gfi = G[off[bf]:off[bf + 1], off[bi]:off[bi + 1]]
+    return float(np.real(np.trace(gamma(sigmas[cf]) @ Gfi @ gamma(sigmas[ci]) @ Gfi.conj().T)))

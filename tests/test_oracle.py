@@ -1,0 +1,276 @@
+"""CPU tests that pin the oracle (oracle/negf_oracle.c) before it is trusted.
+
+Each check is independent of the oracle's own formulas: LAPACK (numpy) for
+inverses and brute-force dense Green's functions, closed forms for the
+surface Green's function and the clean-ribbon transmission, invariants
+(reciprocity, current conservation, Hermiticity) and the reference tests'
+known-answer current windows (rebuilt chain, tests/chains.py).
+"""
+import numpy as np
+import pytest
+
+from libnegf_amd import workloads as wl
+from tests.chains import brute_T, chain_structure, dense_G, gamma
+
+
+def _rand(rng, n, m=None):
+    m = m or n
+    return rng.normal(size=(n, m)) + 1j * rng.normal(size=(n, m))
+
+
+# ---------------------------------------------------------------- inverse ----
+@pytest.mark.parametrize("n", [1, 2, 7, 37])
+def test_zinv_matches_lapack(oracle, n):
+    rng = np.random.default_rng(n)
+    A = _rand(rng, n)
+    Ai, info = oracle.zinv(A)
+    assert info == 0
+    np.testing.assert_allclose(Ai, np.linalg.inv(A), rtol=1e-11, atol=1e-12)
+
+
+def test_zinv_pivoting_and_singular(oracle):
+    A = np.array([[0, 1, 0], [0, 0, 1], [1, 0, 0]], dtype=complex)  # needs row pivoting
+    Ai, info = oracle.zinv(A)
+    assert info == 0
+    np.testing.assert_allclose(Ai, np.linalg.inv(A), atol=1e-15)
+    S = np.ones((3, 3), dtype=complex)
+    _, info = oracle.zinv(S)
+    assert info > 0  # ZGETRF reports the zero pivot (inversions.F90:823-825)
+
+
+# ------------------------------------------------------------ decimation ----
+@pytest.mark.parametrize("E,eps", [(0.3, 0.0), (-1.7, 0.2), (2.5, 0.0), (0.0, -0.4)])
+def test_chain_surface_gf_closed_form(oracle, E, eps):
+    """1-orbital chain: g_s = [(z-eps) - sqrt((z-eps)^2 - 4t^2)]/(2t^2), |t g| < 1."""
+    from libnegf_amd.structure import Contact
+    t = -1.0
+    c = Contact(cblk=1, H00=[[eps]], H01=[[t]], HMC=[[t]])
+    z = E + 1e-5j
+    _, GS, ncyc = oracle.contact_sigma(c, 1, z)
+    ref = wl.analytic_chain_surface_g(z, eps, t)
+    assert abs(GS[0, 0] - ref) <= 1e-7 * abs(ref)
+    assert 1 <= ncyc < 300
+
+
+def test_ribbon_surface_gf_mode_basis(oracle):
+    """W=10 ribbon: H00 and H01 = t I commute; in the sine basis g_s is diagonal
+    with the 1-D chain value at eps_m = 2t cos(m pi/(W+1))."""
+    s, _, _ = wl.config1()
+    c = s.contacts[0]
+    W = 10
+    U = np.sqrt(2.0 / (W + 1)) * np.sin(np.outer(np.arange(1, W + 1), np.arange(1, W + 1)) * np.pi / (W + 1))
+    for E in (-2.1, 0.45, 1.3):
+        z = E + 1e-5j
+        _, GS, _ = oracle.contact_sigma(c, W, z)
+        eps = 2 * (-1.0) * np.cos(np.arange(1, W + 1) * np.pi / (W + 1))
+        g = np.array([wl.analytic_chain_surface_g(z, e, -1.0) for e in eps])
+        ref = U @ np.diag(g) @ U.T
+        assert np.abs(GS - ref).max() <= 1e-7 * np.abs(ref).max()
+
+
+def test_decimation_fixed_point_residual(oracle):
+    """General contact with overlap: g = (Ao - Bo g Co)^-1 at the original blocks."""
+    slab = wl.KSlab.make(n=12, nbl=3, seed=11)
+    s = slab.at_k(0.4, -1.1)
+    c = s.contacts[0]
+    for E in (-0.8, 0.1, 1.2):
+        z = E + 1e-4j
+        _, g, ncyc = oracle.contact_sigma(c, s.bsz[c.cblk - 1], z)
+        Ao = z * c.S00 - c.H00
+        Bo = z * c.S01 - c.H01
+        Co = (np.conj(z) * c.S01 - c.H01).conj().T
+        res = np.linalg.inv(g) - (Ao - Bo @ g @ Co)
+        assert np.abs(res).max() <= 1e-7 * np.abs(Ao).max(), (E, ncyc)
+
+
+# ---------------------------------------------------------- transmission ----
+def _sigmas(oracle, s, E, delta):
+    return [oracle.contact_sigma(c, s.bsz[c.cblk - 1], E + 1j * delta)[0] for c in s.contacts]
+
+
+@pytest.mark.parametrize("which", ["c1", "ribbon_disorder", "kslab"])
+def test_transmission_bruteforce(oracle, which):
+    if which == "c1":
+        s, _, eta = wl.config1()
+        Es = [-2.7, -0.31, 0.77, 1.93]
+    elif which == "ribbon_disorder":
+        s = wl.ribbon(8, 12, disorder=0.5, seed=5)
+        eta, Es = 1e-6, [-2.2, -0.05, 1.4]
+    else:
+        s = wl.KSlab.make(n=10, nbl=5, seed=7).at_k(1.0, 0.3)
+        eta, Es = 1e-5, [-1.1, 0.0, 0.9]
+    for E in Es:
+        T, ncyc = oracle.transmission(s, E, eta)
+        Tb = brute_T(s, E, _sigmas(oracle, s, E, eta))
+        assert abs(T - Tb) <= 1e-10 * max(abs(Tb), 1e-6), (which, E, T, Tb)
+        assert all(1 <= n < 300 for n in ncyc)
+
+
+def test_c1_staircase(oracle):
+    """Clean ribbon: T(E) = number of open sine modes, away from the band edges."""
+    s, E, eta = wl.config1()
+    T = np.array([oracle.transmission(s, e, eta)[0] for e in E])
+    Ta = wl.analytic_ribbon_T(10, E)
+    eps = 2 * (-1.0) * np.cos(np.arange(1, 11) * np.pi / 11)
+    edges = np.concatenate([eps - 2, eps + 2])
+    far = np.array([np.min(np.abs(e - edges)) > 1e-2 for e in E])
+    assert np.abs(T[far] - Ta[far]).max() < 1e-5
+
+
+def test_reciprocity_and_bounds(oracle):
+    s = wl.ribbon(6, 8, disorder=0.8, seed=9)
+    for E in (-1.3, 0.2, 2.2):
+        t12, _ = oracle.transmission(s, E, 1e-5, 1, 2)
+        t21, _ = oracle.transmission(s, E, 1e-5, 2, 1)
+        assert abs(t12 - t21) <= 1e-12 * max(t12, 1.0)
+        assert -1e-12 <= t12 <= 6 + 1e-9
+
+
+def test_n_contact_path_bruteforce(oracle):
+    """min(cblk) != 1 routes through calculate_single_transmission_N_contacts
+    (iterative.F90:1716-1766); a third contact adds a true multi-terminal case."""
+    base = wl.ribbon(6, 7, disorder=0.4, seed=13)
+    from libnegf_amd.structure import BlockStructure, Contact
+    # contacts on blocks 2 and 7 (block 1 dangling)
+    c1 = Contact(cblk=7, H00=base.contacts[0].H00, H01=base.contacts[0].H01, HMC=base.contacts[0].HMC)
+    c2 = Contact(cblk=2, H00=base.contacts[1].H00, H01=base.contacts[1].H01, HMC=base.contacts[1].HMC)
+    s2 = BlockStructure(Hd=base.Hd, Hu=base.Hu, Hl=base.Hl, contacts=[c1, c2])
+    # three contacts: extra lead on block 4 coupled weakly
+    c3 = Contact(cblk=4, H00=base.contacts[0].H00, H01=base.contacts[0].H01,
+                 HMC=0.3 * np.eye(6))
+    s3 = BlockStructure(Hd=base.Hd, Hu=base.Hu, Hl=base.Hl, contacts=[c1, base.contacts[1], c3])
+    for E in (-1.5, 0.35):
+        T, _ = oracle.transmission(s2, E, 1e-5)
+        Tb = brute_T(s2, E, _sigmas(oracle, s2, E, 1e-5))
+        assert abs(T - Tb) <= 1e-10 * max(Tb, 1e-6)
+        for (i, f) in ((1, 2), (1, 3), (3, 2)):
+            T, _ = oracle.transmission(s3, E, 1e-5, i, f)
+            Tb = brute_T(s3, E, _sigmas(oracle, s3, E, 1e-5), i - 1, f - 1)
+            assert abs(T - Tb) <= 1e-10 * max(Tb, 1e-6), (E, i, f, T, Tb)
+
+
+def test_mkl_backend_agrees(oracle):
+    try:
+        oracle.set_backend("mkl")
+    except RuntimeError:
+        pytest.skip("MKL not loadable")
+    try:
+        s = wl.KSlab.make(n=16, nbl=4, seed=3).at_k(0.2, 0.9)
+        tm, nm = oracle.transmission(s, 0.3, 1e-5)
+    finally:
+        oracle.set_backend("builtin")
+    tb, nb = oracle.transmission(s, 0.3, 1e-5)
+    assert abs(tm - tb) <= 1e-11 * max(tb, 1.0)
+    assert list(nm) == list(nb)
+
+
+def test_gr_tridiag_bruteforce(oracle):
+    s = wl.KSlab.make(n=8, nbl=5, seed=21).at_k(-0.7, 2.0)
+    E = 0.25
+    sig = _sigmas(oracle, s, E, 1e-5)
+    Grd, Gru, Grl = oracle.gr_tridiag(s, E, sig)
+    G, off = dense_G(s, E, sig)
+    for i in range(s.nbl):
+        ref = G[off[i]:off[i + 1], off[i]:off[i + 1]]
+        assert np.abs(Grd[i] - ref).max() <= 1e-10 * np.abs(ref).max()
+    for i in range(s.nbl - 1):
+        ru = G[off[i]:off[i + 1], off[i + 1]:off[i + 2]]
+        rl = G[off[i + 1]:off[i + 2], off[i]:off[i + 1]]
+        assert np.abs(Gru[i] - ru).max() <= 1e-10 * np.abs(ru).max()
+        assert np.abs(Grl[i] - rl).max() <= 1e-10 * np.abs(rl).max()
+
+
+# ------------------------------------------------------------------- SCBA ----
+def test_scba_zero_coupling_is_landauer(oracle):
+    s = wl.ribbon(5, 6, disorder=0.6, seed=17)
+    nd = s.ndev
+    for E in (-1.1, 0.6):
+        T, _ = oracle.transmission(s, E, 1e-5)
+        r = oracle.scba_mw(s, E, 1e-5, np.zeros(nd), 5, 1e-7, [1.0, 0.0, 0.0], 3)
+        assert abs(r["curr"][0] - T) <= 1e-10 * max(T, 1e-6)
+        assert abs(r["curr"][1] + T) <= 1e-10 * max(T, 1e-6)
+        assert r["iters"] == 2  # pass 0 stores, pass 1 sees zero change
+
+
+def test_scba_gn_bruteforce_zero_coupling(oracle):
+    s = wl.KSlab.make(n=6, nbl=4, seed=5).at_k(0.1, 0.2)
+    E = 0.4
+    sig = _sigmas(oracle, s, E, 1e-5)
+    r = oracle.scba_mw(s, E, 1e-5, np.zeros(s.ndev), 3, 1e-7, [0.7, 0.2, 0.0], 3, sigmas=sig)
+    G, off = dense_G(s, E, sig)
+    Sn = np.zeros_like(G)
+    for c, f, Sg in zip(s.contacts, (0.7, 0.2), sig):
+        b = c.cblk - 1
+        Sn[off[b]:off[b + 1], off[b]:off[b + 1]] += f * gamma(Sg)
+    Gn = G @ Sn @ G.conj().T
+    np.testing.assert_allclose(r["gn_diag"], np.diag(Gn), rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(r["gr_diag"], np.diag(G), rtol=1e-10, atol=1e-13)
+
+
+def test_scba_current_conservation(oracle):
+    """Elastic dephasing conserves current at convergence: I_1 + I_2 = 0."""
+    s = wl.ribbon(5, 6, disorder=0.3, seed=19)
+    cp = np.full(s.ndev, 0.1)
+    for E in (-0.9, 0.35):
+        r = oracle.scba_mw(s, E, 1e-5, cp, 200, 1e-12, [1.0, 0.0, 0.0], 3)
+        assert r["err"] < 1e-12
+        I1, I2 = r["curr"]
+        assert abs(I1 + I2) <= 1e-8 * abs(I1)
+        assert np.abs(r["gn_diag"].imag).max() <= 1e-12 * np.abs(r["gn_diag"]).max()
+
+
+@pytest.mark.parametrize("coupling,niter,estep,lo,hi", [
+    (0.0, 5, 0.1, 2.0 - 1e-4, 2.0 + 1e-4),    # tests/f90elph_deph/hello.F90:83-100
+    (0.05, 5, 0.1, 1.90, 1.95),                # tests/f90elph_deph/hello.F90:108-120
+    (0.05, 10, 0.01, 1.90, 1.95),              # tests/c_int_elph_deph/c_int_elph_deph.c:121-152
+])
+def test_known_answer_dephasing_current(oracle, coupling, niter, estep, lo, hi):
+    """compute_current with dephasing (libnegf.F90:1904-1963): effective T from
+    Meir-Wingreen with occupations (1, 0), then integrate_el x g_spin, on the
+    rebuilt 60-site chain; mu = (-0.5, 0.5), kT = 1e-3, delta = 1e-4."""
+    s = chain_structure()
+    emin, emax = -2.0, 2.0
+    npts = oracle.npoints(emin, emax, estep)
+    E = emin + estep * np.arange(npts)
+    cp = np.full(60, coupling)
+    teff = np.array([oracle.scba_mw(s, e, 1e-4, cp, niter, 1e-7, [1.0, 0.0, 0.0], 3)["curr"][0]
+                     for e in E])
+    cur = 2.0 * oracle.integrate_el(teff, -0.5, 0.5, 1e-3, 1e-3, emin, emax, estep)
+    assert lo <= abs(cur) <= hi, cur
+
+
+# --------------------------------------------------------------- integrals ----
+def test_integrate_el_closed_form(oracle):
+    E = np.linspace(-2, 2, 401)
+    T = np.ones_like(E)
+    c = oracle.integrate_el(T, -0.5, 0.5, 0.01, 0.01, -2.0, 2.0, 0.01)
+    assert abs(c - 1.0) < 1e-9
+    c2 = oracle.integrate_el(T, 0.5, -0.5, 0.01, 0.01, -2.0, 2.0, 0.01)
+    assert abs(c2 + 1.0) < 1e-9
+    # piecewise-linear T between the potentials: integral of (f2 - f1) T ~ int T over window
+    T3 = np.clip(E, 0, None)
+    c3 = oracle.integrate_el(T3, -0.5, 0.5, 1e-4, 1e-4, -2.0, 2.0, 0.01)
+    assert abs(c3 - 0.125) < 1e-6
+
+
+def test_integrate_el_mw_weights(oracle):
+    x = np.linspace(0.0, 2.0, 21)
+    h = x[1] - x[0]
+    assert abs(oracle.integrate_el_mw(0, 3 * x + 1, h) - 8.0) < 1e-12       # trapezoid: linear exact
+    assert abs(oracle.integrate_el_mw(1, x ** 3, h) - 4.0) < 1e-12          # Simpson 1/3: cubic exact
+    # Simpson 3/8 as the reference writes it (integrations.F90:2812-2817): the
+    # last weight is never reset to 3h/8, so for N = 3m+1 points the result is
+    # the exact 3/8 rule plus 3h/8 f(x_N) -- a quirk of the reference, kept.
+    x = np.linspace(0.0, 3.0, 22)
+    h = x[1] - x[0]
+    assert abs(oracle.integrate_el_mw(2, x ** 2, h) - (9.0 + 3 * h / 8 * 9.0)) < 1e-12
+
+
+def test_fermi_and_grid(oracle):
+    assert oracle.fermi(0.0, 0.0, 0.1) == 0.5
+    assert abs(oracle.fermi(1.0, 0.0, 0.1) - 1 / (np.exp(10) + 1)) < 1e-16
+    assert oracle.fermi(5.0, 0.0, 0.1) == np.exp(-50.0)  # |x| > 30 branch
+    assert oracle.fermi(-5.0, 0.0, 0.1) == 1.0
+    assert oracle.fermi(0.1, 0.0, 0.0) == 0.0
+    assert oracle.npoints(-2.0, 2.0, 0.01) == 401
+    assert oracle.npoints(-3.0, 3.0, 6.0 / 199) == 200
