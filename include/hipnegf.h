@@ -179,7 +179,12 @@ typedef struct {
   int ns_steps;      /* mixed: FP64 Newton-Schulz steps per block inverse      */
   int polish;        /* mixed: FP64 decimation iterations after the FP32 ones  */
   double eps_drop;   /* EPS drop tolerance (ln_precision), default 0           */
+  int flags;         /* hipnegf_opt_flags                                      */
 } hipnegf_opts;
+/* HIPNEGF_NO_SHARED_DECIMATION: run one Sancho-Rubio recursion per contact
+ * even when two contacts are mirror images (by default such a pair shares one
+ * recursion; each contact keeps its own convergence test and exit iteration). */
+enum hipnegf_opt_flags { HIPNEGF_NO_SHARED_DECIMATION = 1 };
 
 /* Coherent transmission T(E_i, slot_i), i < nitems (iterative.F90:1669-1780
  * per item).  Host arrays: E[nitems], slot[nitems] in; T[nitems],

@@ -1,0 +1,132 @@
+/*
+ * hipnegf_api.h -- the libNEGF public C API subset of the per-energy path,
+ * implemented on the MI355X engine (libhipnegf.so).
+ *
+ * Same entry-point names, argument order and meaning as the Fortran-generated
+ * C API of the reference (src/api/libnegf_api.f90; handler convention
+ * src/api/libnegfAPICommon.f90:34-39; parameter block src/api/lnParams.h), so
+ * a C/C++/ctypes caller of the reference links against this library
+ * unchanged for the Landauer / Meir-Wingreen / elastic-dephasing path.
+ * Entry points outside that path (density matrix, LDOS, k-point meshes,
+ * inelastic models, file I/O) are not provided.
+ *
+ * Errors: the reference stops the process (error stop); here an entry point
+ * that fails records a message (negf_get_last_error) and leaves its outputs
+ * untouched; negf_get_status returns it as an int (0 = ok).
+ */
+#ifndef HIPNEGF_API_H
+#define HIPNEGF_API_H
+
+#ifdef __cplusplus
+#include <cstdbool>
+extern "C" {
+#else
+#include <stdbool.h>
+#endif
+
+#define MAXNCONT 10
+#define MAXNLAYERS 10000
+#define NEGF_HSIZE 4 /* DAC_handlerSize, libnegfAPICommon.f90:39 */
+
+/* identical layout to src/api/lnParams.h:15-55 */
+struct lnparams {
+  int verbose;
+  int readold_dm_sgfs;
+  int readold_t_sgfs;
+  int sgfcache;
+  int spin;
+  int kpoint;
+  double g_spin;
+  double delta;
+  int deltaModel;
+  double wmax;
+  double dos_delta;
+  double eneconv;
+  double kwght;
+  double ec;
+  double ev;
+  double deltaec;
+  double deltaev;
+  double emin;
+  double emax;
+  double estep;
+  double estep_coarse;
+  double mu_n[MAXNCONT];
+  double mu_p[MAXNCONT];
+  double mu[MAXNCONT];
+  double contact_dos[MAXNCONT];
+  bool fictcont[MAXNCONT];
+  double kbt_dm[MAXNCONT];
+  double kbt_t[MAXNCONT];
+  double scba_inelastic_tol;
+  double scba_elastic_tol;
+  int np_n[2];
+  int np_p[2];
+  int np_real;
+  int n_kt;
+  int n_poles;
+  int ni[MAXNLAYERS];
+  int nf[MAXNLAYERS];
+  char dore[1];
+  int min_or_max;
+  bool is_s_is;
+};
+
+/* session (libnegf_api.f90:62-134, 633-666) */
+void negf_gethandlersize(int *handlerSize);
+void negf_init_session(int *handler);
+void negf_init(int *handler);
+void negf_destruct_libnegf(int *handler);
+void negf_destruct_session(int *handler);
+void negf_set_verbosity(int *handler, int verbose_lev);
+
+/* structure (libnegf_api.f90:146-185, 272-285) -- 1-based indices */
+void negf_init_contacts(int *handler, int ncont);
+void negf_init_structure(int *handler, int ncont, int *surfstart, int *surfend, int *contend,
+                         int npl, int *plend, int *cblk);
+
+/* parameters (libnegf_api.f90:338-365) */
+void negf_set_params(int *handler, struct lnparams *params);
+void negf_get_params(int *handler, struct lnparams *params);
+
+/* H and S in CSR; 0-based if rowpnt[0] == 0, else 1-based (libnegf.F90:303-434).
+ * A: interleaved complex double (same bytes as double _Complex / complex(8)).
+ * iK: 1-based k index (negf_create_hs sets the count). */
+void negf_create_hs(int *handler, int nHk);
+void negf_set_h(int *handler, int nrow, const double *A, int *JA, int *IA, int iK);
+void negf_set_s(int *handler, int nrow, const double *A, int *JA, int *IA, int iK);
+void negf_set_h_cp(int *handler, int nrow, double *a_re, double *a_im, int *ja, int *ia);
+void negf_set_s_cp(int *handler, int nrow, double *a_re, double *a_im, int *ja, int *ia);
+void negf_set_s_id(int *handler, int nrow, int iK);
+/* select the k index used by the next solve (negf%HS(iK)) and its weight */
+void negf_set_kpoint(int *handler, int kpoint);
+
+/* solves (libnegf_api.f90:689-717, 1205-1236) */
+void negf_solve_landauer(int *handler);                  /* compute_current            */
+void negf_calculate_dephasing_transmission(int *handler);
+void negf_current(int *handler, double *current, char *unitsOfH, char *unitsOfJ);
+
+/* elastic dephasing (libnegf_api.f90:1289-1310, 1440-1457) */
+void negf_set_elph_dephasing(int *handler, double *coupling, int coupling_size, int scba_niter);
+void negf_set_scba_tolerances(int *handler, double elastic_tol, double inelastic_tol);
+
+/* results (libnegf_api.f90:802-935); associate_* return library-owned
+ * column-major arrays valid until the next solve or destruct. */
+void negf_get_energies(int *handler, int *npoints, double *re_en, double *im_en, int copy);
+void negf_get_currents(int *handler, int *npairs, double *currents, int copy);
+void negf_associate_transmission(int *handler, int tr_shape[2], double **tr_pointer);
+void negf_associate_energy_current(int *handler, int currents_shape[2], double **currents_pointer);
+
+/* extensions of this implementation */
+void negf_set_precision(int *handler, int precision); /* hipnegf_precision (FP64 default) */
+void negf_set_batch(int *handler, int max_batch);     /* 0: automatic                  */
+int negf_get_status(int *handler);                    /* 0 ok, else last failure       */
+const char *negf_get_last_error(int *handler);
+/* per-energy status bits and decimation iteration counts of the last solve */
+void negf_get_item_status(int *handler, int *npoints, int *status, int *ncyc, int copy);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPNEGF_API_H */

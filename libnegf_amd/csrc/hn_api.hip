@@ -1,0 +1,649 @@
+// hn_api.hip -- libNEGF public C API subset (include/hipnegf_api.h) on the
+// batched MI355X engine.
+//
+// Host logic mirrors the reference's drivers:
+//   negf_init / set_defaults            lib_param.F90:602-669, libnegf.F90:148-181
+//   set_H / set_S / set_S_id            libnegf.F90:303-434 (0/1-based CSR)
+//   init_structure / create_TStruct     libnegf.F90:555-590, ln_structure.F90:80-150
+//   extract_cont + zcsr2blk_sod         ln_extract.F90:41-76, sparsekit_drv.F90:3409-3436
+//   compute_current                     libnegf.F90:1904-1963
+//   tunneling_int_def / tunneling_and_dos  integrations.F90:1613-1783
+//   meir_wingreen (fixed occupations)   integrations.F90:1804-1966
+//   electron_current / integrate_el     integrations.F90:2427-2457, 2688-2767
+// Every (E) point of a solve is sent to the engine in one batched call.
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/hipnegf.h"
+#include "../../include/hipnegf_api.h"
+#include "hn_engine.h"
+
+namespace {
+
+struct Csr {
+  int n = 0;
+  std::vector<double2> v;
+  std::vector<int> col, ptr;  // 0-based
+  bool present() const { return n > 0; }
+};
+
+struct Session {
+  lnparams p{};
+  int ncont = 0, npl = 0;
+  bool structure_set = false;
+  std::vector<int> surfstart, surfend, contend, plend, cblk;  // 1-based
+  std::vector<Csr> H, S;
+  std::vector<char> s_id;
+  int cur = 0;  // k slot of the working H/S pointers (negf%H => HS(iK)%H)
+  bool deph = false;
+  std::vector<double> coupling;
+  int scba_niter = 0;
+  double tol_el = 1e-7, tol_inel = 1e-7;
+  int precision = 0, max_batch = 0;
+  // results
+  int npoints = 0, npairs = 0;
+  std::vector<double> en, tunn, curr_mat, currents;
+  std::vector<int> item_status, item_ncyc;
+  int status = 0;
+  std::string err;
+  std::unique_ptr<hn::Engine> eng;
+};
+
+Session *sess(int *handler) {
+  Session *s = nullptr;
+  std::memcpy(&s, handler, sizeof(s));
+  return s;
+}
+
+template <typename F> void api(int *handler, F &&f) {
+  Session *s = sess(handler);
+  if (!s) return;
+  try {
+    s->status = 0;
+    s->err.clear();
+    f(*s);
+  } catch (const std::exception &e) {
+    s->status = 1;
+    s->err = e.what();
+    if (s->p.verbose > 0) std::fprintf(stderr, "hipnegf: %s\n", e.what());
+  }
+}
+
+// set_defaults (lib_param.F90:602-669) and init_contacts defaults (libnegf.F90:612-646)
+void set_defaults(lnparams &p) {
+  std::memset(&p, 0, sizeof(p));
+  p.verbose = 10;
+  p.readold_dm_sgfs = 1;
+  p.readold_t_sgfs = 1;
+  p.sgfcache = 0;
+  p.spin = 1;
+  p.kpoint = 1;
+  p.g_spin = 2.0;
+  p.delta = 1.0e-4;
+  p.deltaModel = 1;
+  p.wmax = 0.009;
+  p.dos_delta = 0.0;
+  p.eneconv = 1.0;
+  p.kwght = 1.0;
+  p.scba_inelastic_tol = 1.0e-7;
+  p.scba_elastic_tol = 1.0e-7;
+  p.np_n[0] = p.np_n[1] = 20;
+  p.np_p[0] = p.np_p[1] = 20;
+  p.n_kt = 10;
+  p.n_poles = 3;
+  p.ni[0] = 1;
+  p.nf[0] = 2;
+  p.dore[0] = 'D';
+  p.min_or_max = 1;
+}
+
+// set_H: 0-based when rowpnt(1) == 0 (libnegf.F90:324-341)
+Csr make_csr(int nrow, const double2 *A, const int *JA, const int *IA) {
+  Csr c;
+  c.n = nrow;
+  const int base = (IA[0] == 0) ? 0 : 1;
+  const int nnz = IA[nrow] - IA[0];
+  c.ptr.resize(nrow + 1);
+  for (int i = 0; i <= nrow; ++i) c.ptr[i] = IA[i] - IA[0];
+  c.col.resize(nnz);
+  c.v.resize(nnz);
+  for (int i = 0; i < nnz; ++i) {
+    c.col[i] = JA[i] - base;
+    c.v[i] = A[i];
+  }
+  return c;
+}
+
+// dense column-major extract of rows [r1, r2], cols [c1, c2] (1-based inclusive)
+std::vector<double2> extract(const Csr &A, int r1, int r2, int c1, int c2) {
+  const int m = r2 - r1 + 1, n = c2 - c1 + 1;
+  std::vector<double2> out((size_t)m * n, make_double2(0.0, 0.0));
+  if (m <= 0 || n <= 0) return out;
+  if (r2 > A.n || c2 > A.n) throw std::runtime_error("extract: block outside the matrix");
+  for (int i = r1 - 1; i < r2; ++i)
+    for (int k = A.ptr[i]; k < A.ptr[i + 1]; ++k) {
+      const int j = A.col[k];
+      if (j >= c1 - 1 && j < c2) {
+        double2 &d = out[(i - r1 + 1) + (size_t)(j - c1 + 1) * m];
+        d.x += A.v[k].x;  // duplicates are summed (CSR sum semantics)
+        d.y += A.v[k].y;
+      }
+    }
+  return out;
+}
+
+std::vector<double2> eye(int n) {
+  std::vector<double2> out((size_t)n * n, make_double2(0.0, 0.0));
+  for (int i = 0; i < n; ++i) out[i + (size_t)i * n] = make_double2(1.0, 0.0);
+  return out;
+}
+
+hn::Engine &engine(Session &s) {
+  if (!s.eng) {
+    int dev = 0;
+    HN_CHECK(hipGetDevice(&dev));
+    s.eng.reset(new hn::Engine(dev));
+  }
+  return *s.eng;
+}
+
+// Device structure of the working H/S: the tridiagonal PL blocks
+// (zcsr2blk_sod) and the contact blocks (extract_cont), no surface blocks.
+std::unique_ptr<hn::DevStruct> build_struct(Session &s) {
+  if (!s.structure_set) throw std::runtime_error("structure not initialised (negf_init_structure)");
+  if (s.cur >= (int)s.H.size() || !s.H[s.cur].present()) throw std::runtime_error("H not set");
+  const Csr &H = s.H[s.cur];
+  const bool sid = s.cur >= (int)s.S.size() || s.s_id[s.cur] || !s.S[s.cur].present();
+  const Csr *S = sid ? nullptr : &s.S[s.cur];
+  const int nbl = s.npl;
+  std::vector<int> pls(nbl), ple(nbl), bsz(nbl);
+  for (int i = 0; i < nbl; ++i) {
+    pls[i] = (i == 0) ? 1 : s.plend[i - 1] + 1;
+    ple[i] = s.plend[i];
+    bsz[i] = ple[i] - pls[i] + 1;
+  }
+  std::vector<int> nc(s.ncont);
+  for (int c = 0; c < s.ncont; ++c) {
+    const int surfdim = (s.surfend[c] + 1) - s.surfstart[c];
+    if (surfdim != 0)
+      throw std::runtime_error("contacts with a surface block (surfend >= surfstart) are not supported");
+    const int ncdim = s.contend[c] - s.surfstart[c] + 1;
+    if (ncdim <= 0 || ncdim % 2) throw std::runtime_error("contact must hold 2 principal layers");
+    nc[c] = ncdim / 2;
+    if (s.cblk[c] < 1 || s.cblk[c] > nbl) throw std::runtime_error("cblk out of range");
+  }
+  const int smode = sid ? 0 : (hn::S_DIAG | hn::S_OFFDIAG | hn::S_CONTACT);
+  std::unique_ptr<hn::DevStruct> st(
+      new hn::DevStruct(nbl, bsz.data(), s.ncont, s.cblk.data(), nc.data(), smode, 1));
+  auto put = [&](int kind, int idx, const std::vector<double2> &h) {
+    const long long n = st->block_elems(kind, idx);
+    if ((long long)h.size() != n) throw std::runtime_error("block size mismatch");
+    HN_CHECK(hipMemcpy(st->pool + st->off[kind][idx], h.data(), n * sizeof(double2),
+                       hipMemcpyHostToDevice));
+  };
+  for (int i = 0; i < nbl; ++i) {
+    put(hn::BK_HD, i, extract(H, pls[i], ple[i], pls[i], ple[i]));
+    if (S) put(hn::BK_SD, i, extract(*S, pls[i], ple[i], pls[i], ple[i]));
+  }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    put(hn::BK_HU, i, extract(H, pls[i], ple[i], pls[i + 1], ple[i + 1]));
+    put(hn::BK_HL, i, extract(H, pls[i + 1], ple[i + 1], pls[i], ple[i]));
+    if (S) {
+      put(hn::BK_SU, i, extract(*S, pls[i], ple[i], pls[i + 1], ple[i + 1]));
+      put(hn::BK_SL, i, extract(*S, pls[i + 1], ple[i + 1], pls[i], ple[i]));
+    }
+  }
+  for (int c = 0; c < s.ncont; ++c) {
+    const int c1 = s.surfstart[c], n = nc[c];
+    const int b = s.cblk[c] - 1;
+    put(hn::BK_H00, c, extract(H, c1, c1 + n - 1, c1, c1 + n - 1));
+    put(hn::BK_H01, c, extract(H, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1));
+    put(hn::BK_HMC, c, extract(H, pls[b], ple[b], c1, c1 + n - 1));
+    if (S) {
+      put(hn::BK_S00, c, extract(*S, c1, c1 + n - 1, c1, c1 + n - 1));
+      put(hn::BK_S01, c, extract(*S, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1));
+      put(hn::BK_SMC, c, extract(*S, pls[b], ple[b], c1, c1 + n - 1));
+    }
+  }
+  return st;
+}
+
+// fermi_f, distributions.F90:46-74
+double fermi(double E, double Ef, double kT) {
+  if (kT == 0.0) return E > Ef ? 0.0 : 1.0;
+  if (std::fabs((E - Ef) / kT) > 30) return E > Ef ? std::exp(-(E - Ef) / kT) : 1.0;
+  return 1.0 / (std::exp((E - Ef) / kT) + 1.0);
+}
+
+// integrate_el, integrations.F90:2688-2767 (Kb in Hartree/K, ln_constants.F90:36)
+double integrate_el(const double *T, double mu1, double mu2, double kT1, double kT2, double emin,
+                    double emax, double estep) {
+  const double Kb = 3.166811389002312e-6;
+  double curr = 0.0, destep = 1.0e10;
+  int N = 0;
+  const int Np = (int)std::lround((emax - emin) / estep);
+  double kbT1 = kT1 < Kb ? Kb : kT1, kbT2 = kT2 < Kb ? Kb : kT2;
+  bool swapped = false;
+  if (mu2 < mu1) {
+    swapped = true;
+    std::swap(mu1, mu2);
+    std::swap(kbT1, kbT2);
+  }
+  for (int i = 0; i < Np; ++i) {
+    const double E1 = emin + estep * i, TT1 = T[i];
+    const double E2 = emin + estep * (i + 1), TT2 = T[i + 1];
+    while (destep >= (kbT1 + kbT2) / 20.0) {
+      N = N + 1;
+      destep = (E2 - E1) / N;
+    }
+    for (int i1 = 0; i1 < N; ++i1) {
+      const double E3 = E1 + (E2 - E1) * i1 / N;
+      const double E4 = E3 + (E2 - E1) / N;
+      const double TT3 = (TT2 - TT1) * i1 / N + TT1;
+      const double TT4 = TT3 + (TT2 - TT1) / N;
+      const double c1 = (fermi(E3, mu2, kbT2) - fermi(E3, mu1, kbT1)) * TT3;
+      const double c2 = (fermi(E4, mu2, kbT2) - fermi(E4, mu1, kbT1)) * TT4;
+      curr = curr + (c1 + c2) * (E4 - E3) / 2.0;
+    }
+  }
+  return swapped ? -curr : curr;
+}
+
+std::vector<int> pairs_of(const int *v) {
+  std::vector<int> out;
+  for (int i = 0; i < MAXNLAYERS && v[i] != 0; ++i) out.push_back(v[i]);
+  return out;
+}
+
+hn::SolveOpts opts_of(const Session &s) {
+  hn::SolveOpts o;
+  o.delta = s.p.delta;
+  o.precision = s.precision;
+  o.max_batch = s.max_batch;
+  return o;
+}
+
+// electron_current, integrations.F90:2427-2457
+void electron_current(Session &s) {
+  const std::vector<int> ni = pairs_of(s.p.ni), nf = pairs_of(s.p.nf);
+  s.currents.assign(s.npairs, 0.0);
+  if (s.ncont < 2 || s.tunn.empty()) return;
+  for (int ii = 0; ii < s.npairs; ++ii) {
+    const int a = ni[ii] - 1, b = nf[ii] - 1;
+    s.currents[ii] = integrate_el(s.tunn.data() + (size_t)ii * s.npoints, s.p.mu[a], s.p.mu[b],
+                                  s.p.kbt_t[a], s.p.kbt_t[b], s.p.emin, s.p.emax, s.p.estep) *
+                     s.p.g_spin;
+  }
+}
+
+void energy_grid(Session &s) {
+  // tunneling_int_def, integrations.F90:1613-1668 (one process: no padding)
+  s.npoints = (int)std::lround((s.p.emax - s.p.emin) / s.p.estep) + 1;
+  s.en.resize(s.npoints);
+  for (int i = 0; i < s.npoints; ++i) s.en[i] = s.p.emin + s.p.estep * i;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  explicit DevBuf(size_t bytes) { HN_CHECK(hipMalloc(&p, bytes ? bytes : 1)); }
+  ~DevBuf() { if (p) (void)hipFree(p); }
+  template <typename T> T *as() { return (T *)p; }
+};
+
+// compute_landauer: extract_cont + tunneling_int_def + tunneling_and_dos
+void compute_landauer(Session &s) {
+  const std::vector<int> ni = pairs_of(s.p.ni), nf = pairs_of(s.p.nf);
+  s.npairs = (int)ni.size();
+  s.tunn.clear();
+  if (s.p.emax <= s.p.emin) {  // tunneling_and_dos early return, integrations.F90:1695-1701
+    s.npoints = 0;
+    s.en.clear();
+    s.currents.assign(1, 0.0);
+    s.npairs = 1;
+    return;
+  }
+  energy_grid(s);
+  s.tunn.assign((size_t)s.npoints * s.npairs, 0.0);
+  s.item_status.assign(s.npoints, 0);
+  s.item_ncyc.assign((size_t)s.npoints * std::max(s.ncont, 1), 0);
+  if (s.ncont < 2) return;  // calculate_transmissions: one contact -> T = 0
+  auto st = build_struct(s);
+  hn::Engine &eng = engine(s);
+  const int n = s.npoints;
+  DevBuf dE(sizeof(double) * n), dsl(sizeof(int) * n), dT(sizeof(double) * n),
+      dN(sizeof(int) * n * s.ncont), dS(sizeof(int) * n);
+  HN_CHECK(hipMemcpy(dE.p, s.en.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
+  std::vector<double> T(n);
+  std::vector<int> stt(n);
+  for (int ip = 0; ip < s.npairs; ++ip) {
+    hn::SolveOpts o = opts_of(s);
+    o.ni = ni[ip];
+    o.nf = nf[ip];
+    eng.transmission(*st, n, dE.as<double>(), dsl.as<int>(), o, dT.as<double>(), dN.as<int>(),
+                     dS.as<int>());
+    HN_CHECK(hipMemcpy(T.data(), dT.p, sizeof(double) * n, hipMemcpyDeviceToHost));
+    HN_CHECK(hipMemcpy(stt.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    if (ip == 0)
+      HN_CHECK(hipMemcpy(s.item_ncyc.data(), dN.p, sizeof(int) * n * s.ncont, hipMemcpyDeviceToHost));
+    for (int i = 0; i < n; ++i) {
+      s.tunn[(size_t)ip * n + i] = T[i] * s.p.kwght;  // tunn_mat(i,:) = T kwght
+      s.item_status[i] |= stt[i];
+    }
+  }
+}
+
+// compute_dephasing_transmission: Meir-Wingreen with occupations (1, 0),
+// then tunn_mat = curr_mat(:, 1) (libnegf.F90:1940-1963)
+void compute_dephasing_transmission(Session &s) {
+  if (s.ncont != 2) throw std::runtime_error("Effective transmission is only supported for 2 electrodes");
+  const std::vector<int> ni = pairs_of(s.p.ni), nf = pairs_of(s.p.nf);
+  energy_grid(s);
+  const int n = s.npoints;
+  std::vector<double> frm(s.ncont + 1, 0.0);
+  frm[ni[0] - 1] = 1.0;
+  frm[nf[0] - 1] = 0.0;
+  auto st = build_struct(s);
+  int ndev = 0;
+  for (int b : st->bsz) ndev += b;
+  if ((int)s.coupling.size() < ndev) throw std::runtime_error("dephasing coupling shorter than the device");
+  hn::Engine &eng = engine(s);
+  DevBuf dE(sizeof(double) * n), dsl(sizeof(int) * n), dC(sizeof(double) * n * s.ncont),
+      dI(sizeof(int) * n), dErr(sizeof(double) * n), dS(sizeof(int) * n),
+      dCp(sizeof(double) * ndev), dF(sizeof(double) * (s.ncont + 1));
+  HN_CHECK(hipMemcpy(dE.p, s.en.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
+  HN_CHECK(hipMemcpy(dCp.p, s.coupling.data(), sizeof(double) * ndev, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dF.p, frm.data(), sizeof(double) * (s.ncont + 1), hipMemcpyHostToDevice));
+  hn::SCBAOpts so;
+  so.coupling = dCp.as<double>();
+  so.niter = s.scba_niter;
+  so.tol = s.p.scba_elastic_tol;
+  so.frm = dF.as<double>();
+  so.frm_per_item = 0;
+  so.ref = s.ncont + 1;
+  hn::SolveOpts o = opts_of(s);
+  eng.scba_mw(*st, n, dE.as<double>(), dsl.as<int>(), o, so, dC.as<double>(), dI.as<int>(),
+              dErr.as<double>(), dS.as<int>(), nullptr, nullptr);
+  std::vector<double> cm((size_t)n * s.ncont);
+  HN_CHECK(hipMemcpy(cm.data(), dC.p, sizeof(double) * n * s.ncont, hipMemcpyDeviceToHost));
+  s.item_status.assign(n, 0);
+  HN_CHECK(hipMemcpy(s.item_status.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  s.curr_mat.assign((size_t)n * s.ncont, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int c = 0; c < s.ncont; ++c)
+      s.curr_mat[(size_t)c * n + i] = cm[(size_t)i * s.ncont + c] * s.p.kwght;  // column-major
+  s.npairs = 1;
+  s.tunn.assign(s.curr_mat.begin(), s.curr_mat.begin() + n);
+}
+
+// compute_current, libnegf.F90:1904-1919
+void compute_current(Session &s) {
+  if (s.deph) compute_dephasing_transmission(s);
+  else compute_landauer(s);
+  if (!s.tunn.empty()) electron_current(s);
+}
+
+// convertCurrent, ln_constants.F90:167-194 (tables :88-106)
+double convert_current(const std::string &h, const std::string &j) {
+  const double HAR = 27.21138506, eovh = 1.054181532603995e-3, Kb = 3.166811389002312e-6;
+  const double J_eV = 1.0 / 1.60217656e-19;
+  if (h == "unknown") return 1.0;
+  double curr = std::nan("");
+  struct U { const char *n; double v; };
+  const U eu[] = {{"unknown", 1.0}, {"au", 1.0}, {"H", 1.0}, {"Ry", 0.5}, {"eV", 1.0 / HAR},
+                  {"kcal/mol", 0.0433634 / HAR}, {"K", Kb}, {"cm^-1", 1.239841930e-4 / HAR},
+                  {"J", J_eV / HAR}};
+  const U cu[] = {{"unknown", 1.0}, {"A", 1.0}, {"mA", 1.0e-3}, {"nA", 1.0e-9}};
+  for (const U &u : eu)
+    if (h == u.n) curr = eovh * u.v;
+  for (const U &u : cu)
+    if (j == u.n) curr = curr * u.v;
+  return curr;
+}
+
+std::string cstr(const char *c) {
+  std::string s = c ? c : "";
+  while (!s.empty() && (s.back() == ' ' || s.back() == '\0')) s.pop_back();
+  return s;
+}
+
+}  // namespace
+
+extern "C" {
+
+void negf_gethandlersize(int *handlerSize) { *handlerSize = NEGF_HSIZE; }
+
+void negf_init_session(int *handler) {
+  static_assert(sizeof(Session *) <= NEGF_HSIZE * sizeof(int), "handler too small");
+  std::memset(handler, 0, NEGF_HSIZE * sizeof(int));
+  Session *s = new Session;
+  set_defaults(s->p);
+  std::memcpy(handler, &s, sizeof(s));
+}
+
+void negf_init(int *handler) {
+  api(handler, [](Session &s) {
+    set_defaults(s.p);
+    s.H.assign(1, Csr());
+    s.S.assign(1, Csr());
+    s.s_id.assign(1, 0);
+    s.cur = 0;
+  });
+}
+
+void negf_destruct_libnegf(int *handler) {
+  api(handler, [](Session &s) {
+    s.H.clear();
+    s.S.clear();
+    s.tunn.clear();
+    s.curr_mat.clear();
+    s.currents.clear();
+    s.deph = false;
+    s.eng.reset();
+  });
+}
+
+void negf_destruct_session(int *handler) {
+  Session *s = sess(handler);
+  delete s;
+  std::memset(handler, 0, NEGF_HSIZE * sizeof(int));
+}
+
+void negf_set_verbosity(int *handler, int v) {
+  api(handler, [&](Session &s) { s.p.verbose = v; });
+}
+
+void negf_init_contacts(int *handler, int ncont) {
+  api(handler, [&](Session &s) {
+    if (ncont < 0 || ncont > MAXNCONT) throw std::runtime_error("bad number of contacts");
+    s.ncont = ncont;
+    for (int c = 0; c < MAXNCONT; ++c) {
+      s.p.mu[c] = s.p.mu_n[c] = s.p.mu_p[c] = 0.0;
+      s.p.kbt_dm[c] = s.p.kbt_t[c] = 0.0;
+      s.p.contact_dos[c] = 0.0;
+      s.p.fictcont[c] = false;
+    }
+  });
+}
+
+void negf_init_structure(int *handler, int ncont, int *surfstart, int *surfend, int *contend,
+                         int npl, int *plend, int *cblk) {
+  api(handler, [&](Session &s) {
+    if (s.ncont != 0 && s.ncont != ncont)
+      throw std::runtime_error("Error in set_structure: ncont not compatible with previous initialization.");
+    if (npl <= 0) throw std::runtime_error("automatic PL partitioning (npl = 0) is not supported");
+    s.ncont = ncont;
+    s.npl = npl;
+    s.surfstart.assign(surfstart, surfstart + ncont);
+    s.surfend.assign(surfend, surfend + ncont);
+    s.contend.assign(contend, contend + ncont);
+    s.cblk.assign(cblk, cblk + ncont);
+    s.plend.assign(plend, plend + npl);
+    s.structure_set = true;
+  });
+}
+
+void negf_set_params(int *handler, struct lnparams *params) {
+  api(handler, [&](Session &s) { s.p = *params; });
+}
+
+void negf_get_params(int *handler, struct lnparams *params) {
+  api(handler, [&](Session &s) { *params = s.p; });
+}
+
+void negf_create_hs(int *handler, int nHk) {
+  api(handler, [&](Session &s) {
+    if (nHk < 1) throw std::runtime_error("nHk < 1");
+    s.H.assign(nHk, Csr());
+    s.S.assign(nHk, Csr());
+    s.s_id.assign(nHk, 0);
+  });
+}
+
+static void set_mat(Session &s, bool is_h, int nrow, const double2 *A, const int *JA,
+                    const int *IA, int iK) {
+  const int k = iK - 1;
+  if (k < 0 || k >= (int)s.H.size())
+    throw std::runtime_error("Error: set_H with index > allocated array. Call create_HS with correct size");
+  if (is_h) s.H[k] = make_csr(nrow, A, JA, IA);
+  else {
+    s.S[k] = make_csr(nrow, A, JA, IA);
+    s.s_id[k] = 0;
+  }
+  s.cur = k;
+}
+
+void negf_set_h(int *handler, int nrow, const double *A, int *JA, int *IA, int iK) {
+  api(handler, [&](Session &s) { set_mat(s, true, nrow, (const double2 *)A, JA, IA, iK); });
+}
+void negf_set_s(int *handler, int nrow, const double *A, int *JA, int *IA, int iK) {
+  api(handler, [&](Session &s) { set_mat(s, false, nrow, (const double2 *)A, JA, IA, iK); });
+}
+
+static void set_cp(Session &s, bool is_h, int nrow, const double *re, const double *im,
+                   const int *ja, const int *ia) {
+  const int nnz = ia[nrow] - ia[0];
+  std::vector<double2> A(nnz);
+  for (int i = 0; i < nnz; ++i) A[i] = make_double2(re[i], im[i]);
+  set_mat(s, is_h, nrow, A.data(), ja, ia, 1);
+}
+void negf_set_h_cp(int *handler, int nrow, double *a_re, double *a_im, int *ja, int *ia) {
+  api(handler, [&](Session &s) { set_cp(s, true, nrow, a_re, a_im, ja, ia); });
+}
+void negf_set_s_cp(int *handler, int nrow, double *a_re, double *a_im, int *ja, int *ia) {
+  api(handler, [&](Session &s) { set_cp(s, false, nrow, a_re, a_im, ja, ia); });
+}
+
+void negf_set_s_id(int *handler, int nrow, int iK) {
+  api(handler, [&](Session &s) {
+    const int k = iK - 1;
+    if (k < 0 || k >= (int)s.S.size())
+      throw std::runtime_error("Error: set_S_id with index > allocated array. Call create_HS with correct size");
+    s.S[k] = Csr();
+    s.S[k].n = 0;
+    s.s_id[k] = 1;
+    (void)nrow;
+  });
+}
+
+void negf_set_kpoint(int *handler, int kpoint) {
+  api(handler, [&](Session &s) {
+    s.p.kpoint = kpoint;
+    if (kpoint >= 1 && kpoint <= (int)s.H.size()) s.cur = kpoint - 1;
+  });
+}
+
+void negf_solve_landauer(int *handler) {
+  api(handler, [](Session &s) { compute_current(s); });
+}
+
+void negf_calculate_dephasing_transmission(int *handler) {
+  api(handler, [](Session &s) { compute_dephasing_transmission(s); });
+}
+
+void negf_current(int *handler, double *current, char *unitsOfH, char *unitsOfJ) {
+  api(handler, [&](Session &s) {
+    compute_current(s);
+    *current = (s.currents.empty() ? 0.0 : s.currents[0]) * convert_current(cstr(unitsOfH), cstr(unitsOfJ));
+  });
+}
+
+void negf_set_elph_dephasing(int *handler, double *coupling, int coupling_size, int scba_niter) {
+  api(handler, [&](Session &s) {
+    s.deph = true;
+    s.coupling.assign(coupling, coupling + coupling_size);
+    s.scba_niter = scba_niter;
+  });
+}
+
+void negf_set_scba_tolerances(int *handler, double elastic_tol, double inelastic_tol) {
+  api(handler, [&](Session &s) {
+    s.p.scba_elastic_tol = elastic_tol;
+    s.p.scba_inelastic_tol = inelastic_tol;
+  });
+}
+
+void negf_get_energies(int *handler, int *npoints, double *re_en, double *im_en, int copy) {
+  api(handler, [&](Session &s) {
+    *npoints = (int)s.en.size();
+    if (copy)
+      for (int i = 0; i < *npoints; ++i) {
+        re_en[i] = s.en[i];
+        im_en[i] = 0.0;
+      }
+  });
+}
+
+void negf_get_currents(int *handler, int *npairs, double *currents, int copy) {
+  api(handler, [&](Session &s) {
+    *npairs = (int)s.currents.size();
+    if (copy)
+      for (int i = 0; i < *npairs; ++i) currents[i] = s.currents[i];
+  });
+}
+
+void negf_associate_transmission(int *handler, int tr_shape[2], double **tr_pointer) {
+  api(handler, [&](Session &s) {
+    tr_shape[0] = s.npoints;
+    tr_shape[1] = s.tunn.empty() ? 0 : s.npairs;
+    *tr_pointer = s.tunn.empty() ? nullptr : s.tunn.data();
+  });
+}
+
+void negf_associate_energy_current(int *handler, int currents_shape[2], double **currents_pointer) {
+  api(handler, [&](Session &s) {
+    currents_shape[0] = s.npoints;
+    currents_shape[1] = s.curr_mat.empty() ? 0 : s.ncont;
+    *currents_pointer = s.curr_mat.empty() ? nullptr : s.curr_mat.data();
+  });
+}
+
+void negf_set_precision(int *handler, int precision) {
+  api(handler, [&](Session &s) { s.precision = precision; });
+}
+void negf_set_batch(int *handler, int max_batch) {
+  api(handler, [&](Session &s) { s.max_batch = max_batch; });
+}
+int negf_get_status(int *handler) {
+  Session *s = sess(handler);
+  return s ? s->status : -1;
+}
+const char *negf_get_last_error(int *handler) {
+  Session *s = sess(handler);
+  return s ? s->err.c_str() : "invalid handler";
+}
+void negf_get_item_status(int *handler, int *npoints, int *status, int *ncyc, int copy) {
+  api(handler, [&](Session &s) {
+    *npoints = (int)s.item_status.size();
+    if (!copy) return;
+    for (int i = 0; i < *npoints; ++i) status[i] = s.item_status[i];
+    if (ncyc)
+      for (size_t i = 0; i < s.item_ncyc.size(); ++i) ncyc[i] = s.item_ncyc[i];
+  });
+}
+
+}  // extern "C"

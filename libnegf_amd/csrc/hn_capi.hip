@@ -242,6 +242,7 @@ hn::SolveOpts to_opts(const hipnegf_opts *o) {
     if (o->ns_steps > 0) so.ns_steps = o->ns_steps;
     if (o->polish > 0) so.polish = o->polish;
     so.eps_drop = o->eps_drop;
+    so.share_mirror = (o->flags & HIPNEGF_NO_SHARED_DECIMATION) ? 0 : 1;
   }
   return so;
 }
@@ -450,6 +451,7 @@ int hipnegf_struct_set_block(hipnegf_struct st, int slot, int kind, int index, c
     long long n = 0;
     double2 *d = block_ptr(*st->st, slot, kind, index, &n);
     HN_CHECK(hipMemcpy(d, host, n * sizeof(double2), hipMemcpyHostToDevice));
+    st->st->mirror_state = -1;
     return 0;
   });
 }
@@ -459,6 +461,7 @@ int hipnegf_struct_set_block_device(hipnegf_struct st, int slot, int kind, int i
     long long n = 0;
     double2 *d = block_ptr(*st->st, slot, kind, index, &n);
     HN_CHECK(hipMemcpy(d, dev, n * sizeof(double2), hipMemcpyDeviceToDevice));
+    st->st->mirror_state = -1;
     return 0;
   });
 }
@@ -504,11 +507,52 @@ int hipnegf_solve_scba_mw(hipnegf_engine e, hipnegf_struct st, int nitems, const
                           int niter, double tol, const double *frm, int frm_per_item, int ref,
                           double *curr, int *iters, double *err, int *status, double *gn_diag,
                           double *gr_diag) {
-  return guard([&]() -> int {
-    (void)e; (void)st; (void)nitems; (void)E; (void)slot; (void)o; (void)coupling; (void)niter;
-    (void)tol; (void)frm; (void)frm_per_item; (void)ref; (void)curr; (void)iters; (void)err;
-    (void)status; (void)gn_diag; (void)gr_diag;
-    throw std::runtime_error("SCBA path not available in this build");
+  return guard([&] {
+    if (nitems <= 0) return 0;
+    hn::DevStruct &S = *st->st;
+    for (int i = 0; i < nitems; ++i)
+      if (slot[i] < 0 || slot[i] >= S.nslots) throw std::runtime_error("item slot out of range");
+    if (ref < 1 || ref > S.ncont + 1) throw std::runtime_error("reference contact out of range");
+    if (niter < 0) throw std::runtime_error("niter < 0");
+    int ndev = 0;
+    for (int b : S.bsz) ndev += b;
+    const int nc = std::max(S.ncont, 1);
+    const long long nf = (long long)(frm_per_item ? nitems : 1) * (S.ncont + 1);
+    std::vector<double> zero_c;
+    if (!coupling) zero_c.assign(ndev, 0.0);
+    void *p[11] = {};
+    struct F { void **p; ~F() { for (int i = 0; i < 11; ++i) if (p[i]) (void)hipFree(p[i]); } } fr{p};
+    HN_CHECK(hipMalloc(&p[0], sizeof(double) * nitems));            // E
+    HN_CHECK(hipMalloc(&p[1], sizeof(int) * nitems));               // slot
+    HN_CHECK(hipMalloc(&p[2], sizeof(double) * ndev));              // coupling
+    HN_CHECK(hipMalloc(&p[3], sizeof(double) * nf));                // frm
+    HN_CHECK(hipMalloc(&p[4], sizeof(double) * nitems * nc));       // curr
+    HN_CHECK(hipMalloc(&p[5], sizeof(int) * nitems));               // iters
+    HN_CHECK(hipMalloc(&p[6], sizeof(double) * nitems));            // err
+    HN_CHECK(hipMalloc(&p[7], sizeof(int) * nitems));               // status
+    if (gn_diag) HN_CHECK(hipMalloc(&p[8], sizeof(double2) * (size_t)nitems * ndev));
+    if (gr_diag) HN_CHECK(hipMalloc(&p[9], sizeof(double2) * (size_t)nitems * ndev));
+    HN_CHECK(hipMemcpy(p[0], E, sizeof(double) * nitems, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(p[1], slot, sizeof(int) * nitems, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(p[2], coupling ? coupling : zero_c.data(), sizeof(double) * ndev,
+                       hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(p[3], frm, sizeof(double) * nf, hipMemcpyHostToDevice));
+    hn::SCBAOpts so;
+    so.coupling = (const double *)p[2];
+    so.niter = niter;
+    so.tol = tol;
+    so.frm = (const double *)p[3];
+    so.frm_per_item = frm_per_item ? 1 : 0;
+    so.ref = ref;
+    e->eng->scba_mw(S, nitems, (const double *)p[0], (const int *)p[1], to_opts(o), so, (double *)p[4],
+                    (int *)p[5], (double *)p[6], (int *)p[7], (double2 *)p[8], (double2 *)p[9]);
+    HN_CHECK(hipMemcpy(curr, p[4], sizeof(double) * nitems * S.ncont, hipMemcpyDeviceToHost));
+    if (iters) HN_CHECK(hipMemcpy(iters, p[5], sizeof(int) * nitems, hipMemcpyDeviceToHost));
+    if (err) HN_CHECK(hipMemcpy(err, p[6], sizeof(double) * nitems, hipMemcpyDeviceToHost));
+    if (status) HN_CHECK(hipMemcpy(status, p[7], sizeof(int) * nitems, hipMemcpyDeviceToHost));
+    if (gn_diag) HN_CHECK(hipMemcpy(gn_diag, p[8], sizeof(double2) * (size_t)nitems * ndev, hipMemcpyDeviceToHost));
+    if (gr_diag) HN_CHECK(hipMemcpy(gr_diag, p[9], sizeof(double2) * (size_t)nitems * ndev, hipMemcpyDeviceToHost));
+    return 0;
   });
 }
 

@@ -28,6 +28,7 @@ struct DevStruct {
   long long slot_elems = 0;
   double2 *pool = nullptr;
   int device = 0;
+  int mirror_state = -1;  // contacts 1/2 mirror images in every slot: -1 unknown, 0 no, 1 yes
 
   DevStruct(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc, int smode,
             int nslots);
@@ -74,13 +75,15 @@ struct SolveOpts {
   int ns_steps = 1;          // mixed precision: FP64 Newton-Schulz refinement steps
   int polish = 2;            // mixed precision: FP64 decimation polish iterations
   int dec_crit = 0;          // 0: max|C1| (CPU path), 1: sum|Re|+|Im| (GPU path)
+  int share_mirror = 1;      // one decimation for mirror-image contact pairs
 };
 
 struct SCBAOpts {
-  const double *coupling = nullptr;  // per device orbital (squared inside, elphdd.F90:94)
+  const double *coupling = nullptr;  // device, per device orbital (squared inside, elphdd.F90:94)
   int niter = 0;                     // scba_niter (loop runs while iter <= niter)
   double tol = 1e-7;                 // scba_elastic_tol (lib_param.F90:653)
-  const double *frm = nullptr;       // [ncont+1] occupations; frm[ref-1] reference
+  const double *frm = nullptr;       // device, [ncont+1] occupations (frm[ref-1]: reference)
+  int frm_per_item = 0;              // 1: frm holds nitems x (ncont+1) values
   int ref = 0;                       // 1-based reference contact (ncont+1: none)
 };
 
@@ -92,13 +95,13 @@ class Engine {
 
   // Coherent transmission for nitems (E, slot) points.  All pointers device.
   // T[nitems], ncyc[nitems*ncont], status[nitems].
-  void transmission(const DevStruct &st, int nitems, const double *dE, const int *dslot,
+  void transmission(DevStruct &st, int nitems, const double *dE, const int *dslot,
                     const SolveOpts &o, double *dT, int *dncyc, int *dstatus);
 
   // Elastic-dephasing SCBA + Meir-Wingreen current per item.
   // curr[nitems*ncont], iters[nitems], err[nitems], status[nitems];
   // gn_diag/gr_diag [nitems*ndev] optional (device).
-  void scba_mw(const DevStruct &st, int nitems, const double *dE, const int *dslot,
+  void scba_mw(DevStruct &st, int nitems, const double *dE, const int *dslot,
                const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters, double *derr,
                int *dstatus, double2 *gn_diag, double2 *gr_diag);
 
@@ -117,17 +120,21 @@ class Engine {
   int last_batch_ = 0;
 
   int pick_batch(const DevStruct &st, int nitems, const SolveOpts &o, size_t per_item) const;
-  void contacts(const DevStruct &st, int B, const double *dE, const int *dslot,
+  void contacts(DevStruct &st, int B, const double *dE, const int *dslot,
                 const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c, int *status_c);
   void decimate_z(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, double2 *Co,
                   double2 *Go, const SolveOpts &o, int *ncyc, int *status);
+  bool mirror_contacts(DevStruct &st);
+  void decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 *Aos1, double2 *Bo,
+                       double2 *Co, double2 *Go0, double2 *Go1, const SolveOpts &o, int *ncyc0,
+                       int *ncyc1, int *status0, int *status1);
   void decimate_mixed(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, double2 *Co,
                       double2 *Go, const SolveOpts &o, int *ncyc, int *status);
   void inverse(int n, double2 *A, long long sA, double2 *Ainv, long long sAinv, int batch,
                const int *active, int *status, int precision, const SolveOpts &o);
-  void t_batch(const DevStruct &st, int B, const double *dE, const int *dslot,
+  void t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                const SolveOpts &o, double *dT, int *dncyc, int *dstatus);
-  void scba_batch(const DevStruct &st, int B, const double *dE, const int *dslot,
+  void scba_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                   const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
                   double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag);
 };

@@ -229,6 +229,96 @@ void Engine::decimate_z(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, 
   arena_.used = mark;
 }
 
+// Contacts whose lead Hamiltonians are mirror images (H00, S00 equal;
+// H01, S01 of contact 2 equal to the conjugate transpose of contact 1's)
+// satisfy (Bo, Co)_2 = (Co, Bo)_1 at every z, so their Sancho-Rubio
+// recursions share the bulk iterate (Ao, Go) and each one's products.
+bool Engine::mirror_contacts(DevStruct &st) {
+  if (st.mirror_state >= 0) return st.mirror_state == 1;
+  st.mirror_state = 0;
+  if (st.ncont != 2 || st.nc[0] != st.nc[1]) return false;
+  const int n = st.nc[0];
+  int *flags = nullptr;
+  HN_CHECK(hipMalloc(&flags, sizeof(int) * st.nslots));
+  fill_int(flags, st.nslots, 1, stream_);
+  mirror_check(st.pool, st.slot_elems, st.nslots, n, st.off[BK_H00][0], st.off[BK_H00][1],
+               st.off[BK_H01][1], st.off[BK_H01][0], flags, stream_);
+  if (st.smode & S_CONTACT)
+    mirror_check(st.pool, st.slot_elems, st.nslots, n, st.off[BK_S00][0], st.off[BK_S00][1],
+                 st.off[BK_S01][1], st.off[BK_S01][0], flags, stream_);
+  std::vector<int> h(st.nslots);
+  HN_CHECK(hipMemcpyAsync(h.data(), flags, sizeof(int) * st.nslots, hipMemcpyDeviceToHost, stream_));
+  HN_CHECK(hipStreamSynchronize(stream_));
+  HN_CHECK(hipFree(flags));
+  bool all = true;
+  for (int v : h) all = all && v;
+  st.mirror_state = all ? 1 : 0;
+  return all;
+}
+
+// Joint Sancho-Rubio recursion of a mirror-image contact pair.  Contact 0
+// (primary) follows contselfenergy.F90:271-340 step by step; contact 1 sees
+// Bo and Co exchanged: its C1 is the primary's B1 = Bo Go Bo and its surface
+// update is Co Go Bo.  Each contact keeps its own convergence state (two
+// consecutive max|C1| <= SGFACC), exit iteration and frozen surface block.
+void Engine::decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 *Aos1,
+                             double2 *Bo, double2 *Co, double2 *Go0, double2 *Go1,
+                             const SolveOpts &o, int *ncyc0, int *ncyc1, int *status0,
+                             int *status1) {
+  const long long s2 = (long long)nc * nc;
+  size_t mark = arena_.used;
+  double2 *M = arena_.take<double2>(s2 * B);
+  double2 *GoXCo = arena_.take<double2>(s2 * B);
+  double2 *C1 = arena_.take<double2>(s2 * B);
+  double2 *GoXBo = arena_.take<double2>(s2 * B);
+  double2 *B1 = arena_.take<double2>(s2 * B);
+  int *act0 = arena_.take<int>(B), *act1 = arena_.take<int>(B), *act = arena_.take<int>(B);
+  int *upd0 = arena_.take<int>(B), *upd1 = arena_.take<int>(B), *upd = arena_.take<int>(B);
+  int *ok0 = arena_.take<int>(B), *ok1 = arena_.take<int>(B);
+  int *dcount = arena_.take<int>(4);
+  fill_int(act0, B, 1, stream_);
+  fill_int(act1, B, 1, stream_);
+  fill_int(ok0, B, 0, stream_);
+  fill_int(ok1, B, 0, stream_);
+  const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+  for (int it = 1; it <= o.dec_maxit; ++it) {
+    or_flags(act0, act1, act, B, stream_);
+    copy_batched(M, s2, Ao, s2, s2, B, act, stream_);
+    inverse(nc, M, s2, Go0, s2, B, act, status0, PREC_FP64, o);
+    gemm_z(G(nc, nc, nc, Go0, s2, nc, OP_N, Co, s2, nc, OP_N, GoXCo, s2, nc, one, zero, B, act), stream_);
+    gemm_z(G(nc, nc, nc, Co, s2, nc, OP_N, GoXCo, s2, nc, OP_N, C1, s2, nc, one, zero, B, act), stream_);
+    gemm_z(G(nc, nc, nc, Go0, s2, nc, OP_N, Bo, s2, nc, OP_N, GoXBo, s2, nc, one, zero, B, act), stream_);
+    gemm_z(G(nc, nc, nc, Bo, s2, nc, OP_N, GoXBo, s2, nc, OP_N, B1, s2, nc, one, zero, B, act), stream_);
+    decim_check(C1, s2, (int)s2, B, o.sgfacc, it, o.dec_maxit, act0, upd0, ok0, ncyc0, status0,
+                o.dec_crit, stream_);
+    decim_check(B1, s2, (int)s2, B, o.sgfacc, it, o.dec_maxit, act1, upd1, ok1, ncyc1, status1,
+                o.dec_crit, stream_);
+    or_flags(upd0, upd1, upd, B, stream_);
+    {  // Ao -= Bo GoXCo ; Ao_s(0) -= Bo GoXCo
+      GemmArgs<double> g = G(nc, nc, nc, Bo, s2, nc, OP_N, GoXCo, s2, nc, OP_N, Ao, s2, nc, mone,
+                             one, B, upd);
+      g.C2 = Aos0; g.sC2 = s2; g.ldc2 = nc; g.beta2 = one; g.active2 = upd0;
+      gemm_z(g, stream_);
+    }
+    {  // Ao -= Co GoXBo ; Ao_s(1) -= Co GoXBo
+      GemmArgs<double> g = G(nc, nc, nc, Co, s2, nc, OP_N, GoXBo, s2, nc, OP_N, Ao, s2, nc, mone,
+                             one, B, upd);
+      g.C2 = Aos1; g.sC2 = s2; g.ldc2 = nc; g.beta2 = one; g.active2 = upd1;
+      gemm_z(g, stream_);
+    }
+    std::swap(Bo, B1);
+    std::swap(Co, C1);
+    or_flags(act0, act1, act, B, stream_);
+    count_nonzero(act, B, dcount, stream_);
+    HN_CHECK(hipMemcpyAsync(h_count_, dcount, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+    if (*h_count_ == 0) break;
+  }
+  inverse(nc, Aos0, s2, Go0, s2, B, nullptr, status0, PREC_FP64, o);
+  inverse(nc, Aos1, s2, Go1, s2, B, nullptr, status1, PREC_FP64, o);
+  arena_.used = mark;
+}
+
 void Engine::decimate_mixed(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo,
                             double2 *Co, double2 *Go, const SolveOpts &o, int *ncyc,
                             int *status) {
@@ -240,9 +330,55 @@ void Engine::decimate_mixed(int nc, int CB, double2 *Ao, double2 *Aos, double2 *
 
 // Contact self-energies of all contacts for B items.
 // Sig: ncont x B blocks of nd x nd (nd = bsz[cblk-1]), stride sSig, contact-major.
-void Engine::contacts(const DevStruct &st, int B, const double *dE, const int *dslot,
+void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
                       const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c,
                       int *status_c) {
+  const bool sc = (st.smode & S_CONTACT) != 0;
+  auto self_energy = [&](int c, const double2 *GS) {
+    // SelfEnergy: Tlc = z SMC - HMC, Tp = z* SMC - HMC, Sigma = Tlc GS Tp^H
+    const int nc = st.nc[c], nd = st.bsz[st.cblk[c] - 1];
+    const long long s2 = (long long)nc * nc, sm = (long long)nd * nc;
+    size_t mark = arena_.used;
+    double2 *Tlc = arena_.take<double2>(sm * B), *Tp = arena_.take<double2>(sm * B);
+    double2 *TG = arena_.take<double2>(sm * B);
+    FormArgs t;
+    t.slot = dslot; t.E = dE; t.eta = o.delta; t.batch = B; t.rows = nd; t.cols = nc;
+    t.H = st.ref(BK_HMC, c); t.S = sc ? st.ref(BK_SMC, c) : BlkRef(); t.s_identity = 0;
+    t.out = Tlc; t.s_out = sm;
+    form_block(t, stream_);
+    t.conjz = 1; t.out = Tp;
+    form_block(t, stream_);
+    gemm_z(G(nd, nc, nc, Tlc, sm, nd, OP_N, GS, s2, nc, OP_N, TG, sm, nd, Z(1), Z(0), B, nullptr), stream_);
+    gemm_z(G(nd, nd, nc, TG, sm, nd, OP_N, Tp, sm, nd, OP_C, Sig + (long long)c * B * sSig, sSig,
+             nd, Z(1), Z(0), B, nullptr), stream_);
+    arena_.used = mark;
+  };
+  if (o.share_mirror && o.precision == PREC_FP64 && mirror_contacts(st)) {
+    const int nc = st.nc[0];
+    const long long s2 = (long long)nc * nc;
+    size_t mark = arena_.used;
+    double2 *Ao = arena_.take<double2>(s2 * B), *Aos0 = arena_.take<double2>(s2 * B);
+    double2 *Aos1 = arena_.take<double2>(s2 * B), *Bo = arena_.take<double2>(s2 * B);
+    double2 *Co = arena_.take<double2>(s2 * B), *Go0 = arena_.take<double2>(s2 * B);
+    double2 *Go1 = arena_.take<double2>(s2 * B);
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = o.delta; f.batch = B; f.rows = nc; f.cols = nc;
+    f.H = st.ref(BK_H00, 0); f.S = sc ? st.ref(BK_S00, 0) : BlkRef(); f.s_identity = !sc;
+    f.out = Ao; f.s_out = s2; f.out2 = Aos0; f.s_out2 = s2;
+    form_block(f, stream_);
+    copy_batched(Aos1, s2, Ao, s2, s2, B, nullptr, stream_);
+    f.H = st.ref(BK_H01, 0); f.S = sc ? st.ref(BK_S01, 0) : BlkRef(); f.s_identity = 0;
+    f.out = Bo; f.out2 = nullptr;
+    form_block(f, stream_);
+    f.conjz = 1; f.dagger = 1; f.out = Co;
+    form_block(f, stream_);
+    decimate_shared(nc, B, Ao, Aos0, Aos1, Bo, Co, Go0, Go1, o, ncyc_c, ncyc_c + B, status_c,
+                    status_c + B);
+    self_energy(0, Go0);
+    self_energy(1, Go1);
+    arena_.used = mark;
+    return;
+  }
   for (int c = 0; c < st.ncont; ++c) {
     const int nc = st.nc[c], nd = st.bsz[st.cblk[c] - 1];
     const long long s2 = (long long)nc * nc;
@@ -250,7 +386,6 @@ void Engine::contacts(const DevStruct &st, int B, const double *dE, const int *d
     double2 *Ao = arena_.take<double2>(s2 * B), *Aos = arena_.take<double2>(s2 * B);
     double2 *Bo = arena_.take<double2>(s2 * B), *Co = arena_.take<double2>(s2 * B);
     double2 *Go = arena_.take<double2>(s2 * B);
-    const bool sc = (st.smode & S_CONTACT) != 0;
     // surface_green: Ao = z S00 - H00 (also Ao_s), Bo = z S01 - H01, Co = (z* S01 - H01)^H
     FormArgs f;
     f.slot = dslot; f.E = dE; f.eta = o.delta; f.batch = B; f.rows = nc; f.cols = nc;
@@ -268,20 +403,7 @@ void Engine::contacts(const DevStruct &st, int B, const double *dE, const int *d
       decimate_mixed(nc, B, Ao, Aos, Bo, Co, Go, o, nc_item, st_item);
     else
       decimate_z(nc, B, Ao, Aos, Bo, Co, Go, o, nc_item, st_item);
-    // SelfEnergy: Tlc = z SMC - HMC, Tp = z* SMC - HMC, Sigma = Tlc GS Tp^H
-    const long long sm = (long long)nd * nc;
-    double2 *Tlc = arena_.take<double2>(sm * B), *Tp = arena_.take<double2>(sm * B);
-    double2 *TG = arena_.take<double2>(sm * B);
-    FormArgs t;
-    t.slot = dslot; t.E = dE; t.eta = o.delta; t.batch = B; t.rows = nd; t.cols = nc;
-    t.H = st.ref(BK_HMC, c); t.S = sc ? st.ref(BK_SMC, c) : BlkRef(); t.s_identity = 0;
-    t.out = Tlc; t.s_out = sm;
-    form_block(t, stream_);
-    t.conjz = 1; t.out = Tp;
-    form_block(t, stream_);
-    gemm_z(G(nd, nc, nc, Tlc, sm, nd, OP_N, Go, s2, nc, OP_N, TG, sm, nd, Z(1), Z(0), B, nullptr), stream_);
-    gemm_z(G(nd, nd, nc, TG, sm, nd, OP_N, Tp, sm, nd, OP_C, Sig + (long long)c * B * sSig, sSig,
-             nd, Z(1), Z(0), B, nullptr), stream_);
+    self_energy(c, Go);
     arena_.used = mark;
   }
 }
@@ -289,7 +411,7 @@ void Engine::contacts(const DevStruct &st, int B, const double *dE, const int *d
 // ---------------------------------------------------------------------------
 // Coherent transmission for one batch.
 // ---------------------------------------------------------------------------
-void Engine::t_batch(const DevStruct &st, int B, const double *dE, const int *dslot,
+void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                      const SolveOpts &o, double *dT, int *dncyc, int *dstatus) {
   const int nbl = st.nbl, ncont = st.ncont;
   const int nm = st.nmax();
@@ -406,7 +528,7 @@ void Engine::t_batch(const DevStruct &st, int B, const double *dE, const int *ds
   }
 }
 
-void Engine::transmission(const DevStruct &st, int nitems, const double *dE, const int *dslot,
+void Engine::transmission(DevStruct &st, int nitems, const double *dE, const int *dslot,
                           const SolveOpts &o, double *dT, int *dncyc, int *dstatus) {
   if (nitems <= 0) return;
   HN_CHECK(hipSetDevice(device_));
@@ -455,16 +577,247 @@ int Engine::decimation_host(int n, double2 *hGo, const double2 *hAo, const doubl
   return hs;
 }
 
-void Engine::scba_batch(const DevStruct &, int, const double *, const int *, const SolveOpts &,
-                        const SCBAOpts &, double *, int *, double *, int *, double2 *,
-                        double2 *) {
-  throw std::runtime_error("SCBA path not available in this build");
+// ---------------------------------------------------------------------------
+// Elastic dephasing SCBA + Meir-Wingreen for one batch.
+// Pass p (= scba_iter) of calculate_elastic_scba (iterative.F90:289-363):
+//   ESH(i,i) -= sigma_r (elphdd add_sigma_r; zero at p = 0), gsmr (nbl -> 2),
+//   Gr(1,1) and the Gr tridiagonal blocks (iterative_cpu.F90:51-224),
+//   sigma_r = c^2 diag(Gr)  (set_Gr, elphdd.F90:227-250),
+//   Sigma_n = diag(sigma_n of pass p-1) + sum_j (f_j - f_ref) Gam_j,
+//   calculate_Gn_tridiag_blocks (iterative_cpu.F90:230-372), sigma_n = c^2 diag(Gn),
+//   convergence on |Gn - Gn_old| over the S pattern (blk2csr + check_Mat_convergence,
+//   iterative.F90:1375-1450, scba.F90:120-147).
+// Sigma_n has no off-diagonal blocks (diagonal dephasing + contact terms), so the
+// products with Sigma_n(i,i+1) / Sigma_n(i+1,i) in the reference are exact zeros
+// and are not formed.  Items leave the batch when they converge or exhaust niter;
+// Meir-Wingreen then uses each item's last Gr, Gn (iterative.F90:382-414).
+// ---------------------------------------------------------------------------
+void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot,
+                        const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
+                        double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag) {
+  if (!st.uniform()) throw std::runtime_error("SCBA path requires equal PL and contact sizes");
+  const int nbl = st.nbl, ncont = st.ncont, n = st.bsz[0];
+  const long long s2 = (long long)n * n, SB = s2 * B;
+  const int ndev = n * nbl;
+  arena_.reset();
+  double2 *Sig = arena_.take<double2>(SB * std::max(ncont, 1));
+  int *ncyc_c = arena_.take<int>((size_t)B * std::max(ncont, 1));
+  int *status_c = arena_.take<int>((size_t)B * std::max(ncont, 1));
+  fill_int(ncyc_c, B * std::max(ncont, 1), 0, stream_);
+  fill_int(status_c, B * std::max(ncont, 1), 0, stream_);
+  fill_int(dstatus, B, 0, stream_);
+  contacts(st, B, dE, dslot, o, Sig, s2, ncyc_c, status_c);
+  double2 *Gam = arena_.take<double2>(SB * std::max(ncont, 1));
+  for (int c = 0; c < ncont; ++c) gamma_batched(Sig + c * SB, s2, Gam + c * SB, s2, n, B, stream_);
+
+  double2 *g = arena_.take<double2>(SB * nbl), *Grd = arena_.take<double2>(SB * nbl);
+  double2 *Gru = arena_.take<double2>(SB * std::max(nbl - 1, 1));
+  double2 *Grl = arena_.take<double2>(SB * std::max(nbl - 1, 1));
+  double2 *Sn = arena_.take<double2>(SB * nbl), *Gnd = arena_.take<double2>(SB * nbl);
+  double2 *Gnd_old = arena_.take<double2>(SB * nbl);
+  const bool s_general = (st.smode & S_OFFDIAG) != 0;
+  double2 *Gnu_old = s_general ? arena_.take<double2>(SB * std::max(nbl - 1, 1)) : nullptr;
+  double2 *D = arena_.take<double2>(SB), *U = arena_.take<double2>(SB), *L = arena_.take<double2>(SB);
+  double2 *W1 = arena_.take<double2>(SB), *W2 = arena_.take<double2>(SB);
+  double2 *gns = arena_.take<double2>(SB), *Gnl = arena_.take<double2>(SB), *Gnu = arena_.take<double2>(SB);
+  double2 *sig_r = arena_.take<double2>((size_t)ndev * B), *sig_n = arena_.take<double2>((size_t)ndev * B);
+  double *c2 = arena_.take<double>(ndev);
+  int *act = arena_.take<int>(B), *dcount = arena_.take<int>(4);
+  unsigned long long *err = arena_.take<unsigned long long>(B);
+  square(so.coupling, c2, ndev, stream_);
+  fill_batched(sig_r, ndev, ndev, B, Z(0), stream_);
+  fill_batched(sig_n, ndev, ndev, B, Z(0), stream_);
+  fill_int(act, B, 1, stream_);
+  fill_int(diters, B, 0, stream_);
+  fill_double(derr, B, 0.0, stream_);
+  const bool sd = (st.smode & S_DIAG) != 0, sof = (st.smode & S_OFFDIAG) != 0;
+  std::vector<std::vector<int>> on_blk(nbl);
+  for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
+  const int frm_stride = so.frm_per_item ? ncont + 1 : 0;
+  const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+  auto blk = [&](double2 *base, int i) { return base + (long long)i * SB; };
+
+  auto form_diag = [&](int i, double2 *out) {
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B; f.active = act;
+    f.rows = n; f.cols = n;
+    f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+    f.out = out; f.s_out = s2;
+    const auto &cs = on_blk[i];
+    if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+    if (cs.size() >= 1) { f.sig = Sig + cs[0] * SB; f.s_sig = s2; }
+    if (cs.size() == 2) { f.sig2 = Sig + cs[1] * SB; f.s_sig2 = s2; }
+    f.sig_diag = sig_r + (long long)i * n; f.s_sig_diag = ndev;  // add_sigma_r
+    form_block(f, stream_);
+  };
+  auto form_off = [&](int i, double2 *u, double2 *l) {  // u = ESH(i,i+1), l = ESH(i+1,i)
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B; f.active = act;
+    f.rows = n; f.cols = n;
+    f.s_identity = 0;
+    f.s_out = s2;
+    if (u) {
+      f.H = st.ref(BK_HU, i); f.S = sof ? st.ref(BK_SU, i) : BlkRef(); f.out = u;
+      form_block(f, stream_);
+    }
+    if (l) {
+      f.H = st.ref(BK_HL, i); f.S = sof ? st.ref(BK_SL, i) : BlkRef(); f.out = l;
+      form_block(f, stream_);
+    }
+  };
+  auto mm = [&](const double2 *A, int opA, const double2 *Bm, int opB, double2 *C, double2 al,
+                double2 be) {
+    gemm_z(G(n, n, n, A, s2, n, opA, Bm, s2, n, opB, C, s2, n, al, be, B, act), stream_);
+  };
+
+  for (int pass = 0; pass <= so.niter; ++pass) {
+    count_nonzero(act, B, dcount, stream_);
+    HN_CHECK(hipMemcpyAsync(h_count_, dcount, sizeof(int), hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+    if (*h_count_ == 0) break;
+    // ---- gsmr (nbl -> 2) ----
+    if (nbl > 1) {
+      form_diag(nbl - 1, D);
+      inverse(n, D, s2, blk(g, nbl - 1), s2, B, act, dstatus, PREC_FP64, o);
+      for (int i = nbl - 2; i >= 1; --i) {
+        form_off(i, U, L);
+        form_diag(i, D);
+        mm(U, OP_N, blk(g, i + 1), OP_N, W1, mone, zero);
+        mm(W1, OP_N, L, OP_N, D, one, one);
+        inverse(n, D, s2, blk(g, i), s2, B, act, dstatus, PREC_FP64, o);
+      }
+    }
+    // ---- Gr(1,1) ----
+    form_diag(0, D);
+    if (nbl > 1) {
+      form_off(0, U, L);
+      mm(U, OP_N, blk(g, 1), OP_N, W1, mone, zero);
+      mm(W1, OP_N, L, OP_N, D, one, one);
+    }
+    inverse(n, D, s2, blk(Grd, 0), s2, B, act, dstatus, PREC_FP64, o);
+    // ---- Gr tridiagonal blocks, i = 2..nbl (1-based) ----
+    for (int i = 1; i < nbl; ++i) {
+      form_off(i - 1, U, L);  // U = ESH(i-1,i), L = ESH(i,i-1)
+      mm(blk(g, i), OP_N, L, OP_N, W1, one, zero);
+      mm(W1, OP_N, blk(Grd, i - 1), OP_N, blk(Grl, i - 1), mone, zero);
+      mm(U, OP_N, blk(g, i), OP_N, W2, one, zero);
+      mm(blk(Grd, i - 1), OP_N, W2, OP_N, blk(Gru, i - 1), mone, zero);
+      copy_batched(blk(Grd, i), s2, blk(g, i), s2, s2, B, act, stream_);
+      mm(blk(Grl, i - 1), OP_N, W2, OP_N, blk(Grd, i), mone, one);
+    }
+    // ---- set_Gr: sigma_r = c^2 diag(Gr) ----
+    for (int i = 0; i < nbl; ++i)
+      diag_extract_masked(blk(Grd, i), s2, n, B, sig_r + (long long)i * n, ndev, c2 + (long long)i * n,
+                          act, stream_);
+    // ---- Sigma_n(i,i) = diag(sigma_n) + contact terms ----
+    for (int i = 0; i < nbl; ++i) {
+      SnArgs a;
+      a.Sn = blk(Sn, i); a.sSn = s2; a.n = n;
+      a.v = sig_n + (long long)i * n; a.sv = ndev;
+      a.sG = s2;
+      for (int c : on_blk[i]) {
+        a.G[a.nq] = Gam + c * SB;
+        a.j[a.nq] = c;
+        ++a.nq;
+      }
+      a.frm = so.frm; a.frm_stride = frm_stride; a.ref = so.ref; a.eps = o.eps_drop;
+      a.active = act; a.batch = B;
+      sn_form(a, stream_);
+    }
+    // ---- calculate_sigma_n: upward fold ----
+    if (nbl > 1) {
+      mm(blk(g, nbl - 1), OP_N, blk(Sn, nbl - 1), OP_N, W1, one, zero);
+      mm(W1, OP_N, blk(g, nbl - 1), OP_C, gns, one, zero);
+      for (int i = nbl - 2; i >= 0; --i) {
+        form_off(i, U, nullptr);
+        mm(U, OP_N, gns, OP_N, W1, one, zero);
+        mm(W1, OP_N, U, OP_C, blk(Sn, i), one, one);
+        if (i > 0) {
+          mm(blk(g, i), OP_N, blk(Sn, i), OP_N, W1, one, zero);
+          mm(W1, OP_N, blk(g, i), OP_C, gns, one, zero);
+        }
+      }
+    }
+    // ---- Gn(1,1) = Gr(1,1) Sigma_n(1,1) Gr(1,1)^H ----
+    mm(blk(Sn, 0), OP_N, blk(Grd, 0), OP_C, W1, one, zero);
+    mm(blk(Grd, 0), OP_N, W1, OP_N, blk(Gnd, 0), one, zero);
+    fill_int(reinterpret_cast<int *>(err), 2 * B, 0, stream_);
+    // ---- downward recursion ----
+    for (int i = 0; i + 1 < nbl; ++i) {
+      form_off(i, nullptr, L);  // L = ESH(i+1,i)
+      mm(blk(Sn, i + 1), OP_N, blk(Gru, i), OP_C, W1, one, zero);
+      mm(L, OP_N, blk(Gnd, i), OP_N, W1, mone, one);
+      mm(blk(g, i + 1), OP_N, W1, OP_N, Gnl, one, zero);
+      dagger_batched(Gnl, s2, Gnu, s2, n, n, B, act, stream_);
+      mm(blk(Sn, i + 1), OP_N, blk(Grd, i + 1), OP_C, W1, one, zero);
+      mm(L, OP_N, Gnu, OP_N, W1, mone, one);
+      mm(blk(g, i + 1), OP_N, W1, OP_N, blk(Gnd, i + 1), one, zero);
+      if (s_general) {  // off-diagonal S entries enter the convergence test
+        // Gn(i+1,i) = Gn(i,i+1)^H, so |dGn(i+1,i)(r,c)| = |dGn(i,i+1)(c,r)|: one pass over
+        // Gn(i,i+1) with the union of the S(i,i+1) pattern and the transposed S(i+1,i) pattern
+        masked_diff(Gnu, blk(Gnu_old, i), s2, n, n, 2, st.ref(BK_SU, i), st.ref(BK_SL, i), dslot, err,
+                    act, B, stream_);
+      }
+    }
+    // ---- set_Gn and the convergence test ----
+    for (int i = 0; i < nbl; ++i) {
+      diag_extract_masked(blk(Gnd, i), s2, n, B, sig_n + (long long)i * n, ndev, c2 + (long long)i * n,
+                          act, stream_);
+      masked_diff(blk(Gnd, i), blk(Gnd_old, i), s2, n, n, sd ? 1 : 0, sd ? st.ref(BK_SD, i) : BlkRef(),
+                  BlkRef(), dslot, err, act, B, stream_);
+    }
+    scba_state(err, so.tol, pass, so.niter, act, diters, derr, B, stream_);
+  }
+  // ---- Meir-Wingreen current per contact ----
+  for (int c = 0; c < ncont; ++c) {
+    const int b = st.cblk[c] - 1;
+    trace_mw(Gam + c * SB, blk(Gnd, b), blk(Grd, b), s2, n, so.frm, frm_stride, c, ncont, B, dcurr,
+             stream_);
+  }
+  if (gn_diag)
+    for (int i = 0; i < nbl; ++i)
+      diag_extract_masked(blk(Gnd, i), s2, n, B, gn_diag + (long long)i * n, ndev, nullptr, nullptr, stream_);
+  if (gr_diag)
+    for (int i = 0; i < nbl; ++i)
+      diag_extract_masked(blk(Grd, i), s2, n, B, gr_diag + (long long)i * n, ndev, nullptr, nullptr, stream_);
+  // per-item status: contacts | inverses
+  {
+    std::vector<int> hs((size_t)B * ncont), hst(B);
+    HN_CHECK(hipMemcpyAsync(hs.data(), status_c, sizeof(int) * B * ncont, hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipMemcpyAsync(hst.data(), dstatus, sizeof(int) * B, hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+    for (int b = 0; b < B; ++b)
+      for (int c = 0; c < ncont; ++c) hst[b] |= hs[(size_t)c * B + b];
+    HN_CHECK(hipMemcpyAsync(dstatus, hst.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+  }
 }
 
-void Engine::scba_mw(const DevStruct &st, int nitems, const double *dE, const int *dslot,
+void Engine::scba_mw(DevStruct &st, int nitems, const double *dE, const int *dslot,
                      const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
                      double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag) {
-  scba_batch(st, nitems, dE, dslot, o, so, dcurr, diters, derr, dstatus, gn_diag, gr_diag);
+  if (nitems <= 0) return;
+  if (st.ncont < 1) throw std::runtime_error("SCBA needs at least one contact");
+  HN_CHECK(hipSetDevice(device_));
+  const int n = st.nmax(), nbl = st.nbl;
+  const size_t b2 = (size_t)n * n * sizeof(double2);
+  // G^r/G^n block lists (9 nbl) + sweep temporaries (8) + contact decimation and
+  // self-energy workspace (12 per contact) + inverse workspace + per-orbital vectors
+  const size_t per_item = b2 * (size_t)(9 * nbl + 12 * st.ncont + 16) +
+                          inv_work_bytes(n, std::max(st.ncont, 1), 8) + (size_t)n * nbl * 48 + 1024;
+  const int B = pick_batch(st, nitems, o, per_item);
+  last_batch_ = B;
+  arena_.reserve(per_item * (size_t)B + (1 << 22));
+  const int ndev = n * nbl;
+  for (int i0 = 0; i0 < nitems; i0 += B) {
+    const int nb = std::min(B, nitems - i0);
+    SCBAOpts s2 = so;
+    if (so.frm_per_item) s2.frm = so.frm + (long long)i0 * (st.ncont + 1);
+    scba_batch(st, nb, dE + i0, dslot + i0, o, s2, dcurr + (long long)i0 * st.ncont, diters + i0,
+               derr + i0, dstatus + i0, gn_diag ? gn_diag + (long long)i0 * ndev : nullptr,
+               gr_diag ? gr_diag + (long long)i0 * ndev : nullptr);
+  }
+  HN_CHECK(hipStreamSynchronize(stream_));
 }
 
 }  // namespace hn

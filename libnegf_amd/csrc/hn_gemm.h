@@ -22,6 +22,7 @@ struct GemmArgs {
   CT *C2 = nullptr; long long sC2 = 0; int ldc2 = 0;
   double2 alpha = {1.0, 0.0}, beta = {0.0, 0.0}, beta2 = {0.0, 0.0};
   const int *active = nullptr;
+  const int *active2 = nullptr;  // optional separate mask of the second output C2
   int skip_c0 = 0, skip_c1 = 0;
   int batch = 1;
 };

@@ -3,10 +3,14 @@
 // One 256-thread workgroup (4 waves, 2 x 2) computes a 64 x 64 complex tile of
 // C for one batch item.  K is walked in steps of 16 complex elements staged
 // through LDS as separate real/imaginary planes (k-major), so every MFMA
-// operand fragment is one conflict-free ds_read per lane.  Complex products
-// use four real MFMAs (v_mfma_f64_16x16x4_f64 / v_mfma_f32_16x16x4_f32):
-//   Cr += Ar Br - Ai Bi ,   Ci += Ar Bi + Ai Br.
-// Each wave owns a 32 x 32 sub-tile = 2 x 2 MFMA tiles x (re, im) accumulators.
+// operand fragment is one conflict-free ds_read per lane; the next K step is
+// prefetched into registers while the matrix cores work on the current one.
+// Complex products use four real MFMAs (v_mfma_f64_16x16x4_f64 /
+// v_mfma_f32_16x16x4_f32):  Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br,
+// issued so that no MFMA depends on the one just before it.  Each wave owns a
+// 32 x 32 sub-tile = 2 x 2 MFMA tiles x (re, im) accumulators.  The epilogue
+// stages the tile through LDS so C is read and written along columns
+// (coalesced): the rank-16 updates of the blocked inverse are pure C streaming.
 //
 // Workgroup -> (item, tile) mapping is XCD-aware: hardware dispatches
 // consecutive workgroup ids round-robin over the 8 XCDs, so the linear id is
@@ -43,6 +47,20 @@ __device__ inline int xcd_remap(int wg, int total_pad) {
   return (wg & 7) * per + (wg >> 3);
 }
 
+// row of element r (0..3) of lane `lane` inside a 16 x 16 MFMA output tile:
+// v_mfma_f64_16x16x4_f64 keeps D[row][col] in lane col + 16*(row%4), element
+// row/4; v_mfma_f32_16x16x4_f32 in lane col + 16*(row/4), element row%4.
+template <typename R> __device__ inline int out_row(int lane, int r) {
+  return sizeof(R) == 8 ? (lane >> 4) + 4 * r : (lane >> 4) * 4 + r;
+}
+
+template <typename R> struct Smem {
+  R Ar[BK][BM + PAD];
+  R Ai[BK][BM + PAD];
+  R Br[BK][BN + PAD];
+  R Bi[BK][BN + PAD];
+};
+
 template <typename R>
 __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int tiles_n,
                                              int total, int total_pad) {
@@ -57,10 +75,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   const int tm = t % tiles_m, tn = t / tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
 
-  __shared__ R sAr[BK][BM + PAD];
-  __shared__ R sAi[BK][BM + PAD];
-  __shared__ R sBr[BK][BN + PAD];
-  __shared__ R sBi[BK][BN + PAD];
+  __shared__ Smem<R> sm;
 
   const CT *A = p.A + (long long)b * p.sA;
   const CT *B = p.B + (long long)b * p.sB;
@@ -68,6 +83,52 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave & 1, wn = wave >> 1;
+  const int M = p.m, N = p.n, K = p.k;
+  const bool a_n = p.opA == OP_N, b_n = p.opB == OP_N;
+  const R sa = (p.opA == OP_C) ? R(-1) : R(1), sb = (p.opB == OP_C) ? R(-1) : R(1);
+
+  // global -> register staging of one K step (4 complex of A and of B per thread)
+  CT ra[4], rb[4];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int i, kk;
+      if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
+      else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+      const int gi = m0 + i, gk = k0 + kk;
+      CT v = {0, 0};
+      if (gi < M && gk < K) v = a_n ? A[gi + (long long)gk * p.lda] : A[gk + (long long)gi * p.lda];
+      ra[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int j, kk;
+      if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
+      else { j = tid & 63; kk = (tid >> 6) + 4 * r; }
+      const int gj = n0 + j, gk = k0 + kk;
+      CT v = {0, 0};
+      if (gj < N && gk < K) v = b_n ? B[gk + (long long)gj * p.ldb] : B[gj + (long long)gk * p.ldb];
+      rb[r] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int i, kk;
+      if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
+      else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+      sm.Ar[kk][i] = ra[r].x;
+      sm.Ai[kk][i] = sa * ra[r].y;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int j, kk;
+      if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
+      else { j = tid & 63; kk = (tid >> 6) + 4 * r; }
+      sm.Br[kk][j] = rb[r].x;
+      sm.Bi[kk][j] = sb * rb[r].y;
+    }
+  };
 
   v4 cr[2][2], ci[2][2];
 #pragma unroll
@@ -78,121 +139,101 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
       ci[i][j] = v4{0, 0, 0, 0};
     }
 
-  const int M = p.m, N = p.n, K = p.k;
+  if (K > 0) load(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    // ---- stage op(A)[m0:m0+64, k0:k0+16] ----
-    if (p.opA == OP_N) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int i = tid & 63, kk = (tid >> 6) + 4 * r;
-        int gi = m0 + i, gk = k0 + kk;
-        CT v = {0, 0};
-        if (gi < M && gk < K) v = A[gi + (long long)gk * p.lda];
-        sAr[kk][i] = v.x;
-        sAi[kk][i] = v.y;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int kk = tid & 15, i = (tid >> 4) + 16 * r;
-        int gi = m0 + i, gk = k0 + kk;
-        CT v = {0, 0};
-        if (gi < M && gk < K) v = A[gk + (long long)gi * p.lda];
-        sAr[kk][i] = v.x;
-        sAi[kk][i] = (p.opA == OP_C) ? -v.y : v.y;
-      }
-    }
-    // ---- stage op(B)[k0:k0+16, n0:n0+64] ----
-    if (p.opB == OP_N) {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int kk = tid & 15, j = (tid >> 4) + 16 * r;
-        int gj = n0 + j, gk = k0 + kk;
-        CT v = {0, 0};
-        if (gj < N && gk < K) v = B[gk + (long long)gj * p.ldb];
-        sBr[kk][j] = v.x;
-        sBi[kk][j] = v.y;
-      }
-    } else {
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        int j = tid & 63, kk = (tid >> 6) + 4 * r;
-        int gj = n0 + j, gk = k0 + kk;
-        CT v = {0, 0};
-        if (gj < N && gk < K) v = B[gj + (long long)gk * p.ldb];
-        sBr[kk][j] = v.x;
-        sBi[kk][j] = (p.opB == OP_C) ? -v.y : v.y;
-      }
-    }
+    store();
     __syncthreads();
+    if (k0 + BK < K) load(k0 + BK);  // in flight while the matrix cores run
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4), c16 = lane & 15;
       R ar[2], ai[2], nai[2], br[2], bi[2];
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        ar[i] = sAr[kr][wm * 32 + i * 16 + c16];
-        ai[i] = sAi[kr][wm * 32 + i * 16 + c16];
+        ar[i] = sm.Ar[kr][wm * 32 + i * 16 + c16];
+        ai[i] = sm.Ai[kr][wm * 32 + i * 16 + c16];
         nai[i] = -ai[i];
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        br[j] = sBr[kr][wn * 32 + j * 16 + c16];
-        bi[j] = sBi[kr][wn * 32 + j * 16 + c16];
+        br[j] = sm.Br[kr][wn * 32 + j * 16 + c16];
+        bi[j] = sm.Bi[kr][wn * 32 + j * 16 + c16];
       }
+      // 8 independent accumulators: no MFMA waits on the one issued just before
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
           cr[i][j] = mfma(ar[i], br[j], cr[i][j]);
-          cr[i][j] = mfma(nai[i], bi[j], cr[i][j]);
           ci[i][j] = mfma(ar[i], bi[j], ci[i][j]);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          cr[i][j] = mfma(nai[i], bi[j], cr[i][j]);
           ci[i][j] = mfma(ai[i], br[j], ci[i][j]);
         }
     }
     __syncthreads();
   }
 
-  // ---- epilogue: C = alpha acc + beta C (and the optional second output) ----
+  // ---- epilogue through LDS: the four waves stage a 64 x 32 slab (tile
+  // columns j*16..j*16+15 of both wave columns) per pass, then every thread
+  // reads/writes C along columns (coalesced) ----
+  constexpr int EL = BM + 1;  // leading dimension of the staged slab
+  static_assert(32 * EL <= 2 * BK * (BM + PAD), "epilogue slab does not fit");  // Ar+Ai, Br+Bi
+  R *Er = &sm.Ar[0][0];
+  R *Ei = &sm.Br[0][0];
   CT *C = p.C + (long long)b * p.sC;
-  CT *C2 = p.C2 ? p.C2 + (long long)b * p.sC2 : nullptr;
+  CT *C2 = (p.C2 && (!p.active2 || p.active2[b] != 0)) ? p.C2 + (long long)b * p.sC2 : nullptr;
   const double ar_ = p.alpha.x, ai_ = p.alpha.y;
   const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
   const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int j = 0; j < 2; ++j) {
 #pragma unroll
-    for (int j = 0; j < 2; ++j)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        // D[row][col] of a 16x16 MFMA tile: v_mfma_f64_16x16x4_f64 keeps it in
-        // lane col + 16*(row%4), element row/4; v_mfma_f32_16x16x4_f32 in
-        // lane col + 16*(row/4), element row%4.
-        int row = m0 + wm * 32 + i * 16 + (sizeof(R) == 8 ? (lane >> 4) + 4 * r : (lane >> 4) * 4 + r);
-        int col = n0 + wn * 32 + j * 16 + (lane & 15);
-        if (row >= M || col >= N) continue;
-        if (col >= p.skip_c0 && col < p.skip_c1) continue;
-        double xr = (double)cr[i][j][r], xi = (double)ci[i][j][r];
-        double vr = ar_ * xr - ai_ * xi, vi = ar_ * xi + ai_ * xr;
-        long long o = row + (long long)col * p.ldc;
+        const int row = wm * 32 + i * 16 + out_row<R>(lane, r);
+        const int sc = wn * 16 + (lane & 15);
+        Er[sc * EL + row] = cr[i][j][r];
+        Ei[sc * EL + row] = ci[i][j][r];
+      }
+    __syncthreads();
+    // slab column sc holds tile column (sc / 16) * 32 + j * 16 + sc % 16
+#pragma unroll
+    for (int q = 0; q < (64 * 32) / NT; ++q) {
+      const int e = tid + q * NT;
+      const int row = e & 63, sc = e >> 6;
+      const int gr = m0 + row;
+      const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
+      if (gr < M && gc < N && !(gc >= p.skip_c0 && gc < p.skip_c1)) {
+        const double xr = (double)Er[sc * EL + row], xi = (double)Ei[sc * EL + row];
+        const double vr = ar_ * xr - ai_ * xi, vi = ar_ * xi + ai_ * xr;
+        const long long o = gr + (long long)gc * p.ldc;
         double yr = vr, yi = vi;
         if (useb) {
-          CT c = C[o];
+          const CT c = C[o];
           yr += p.beta.x * (double)c.x - p.beta.y * (double)c.y;
           yi += p.beta.x * (double)c.y + p.beta.y * (double)c.x;
         }
         C[o] = CT{(R)yr, (R)yi};
         if (C2) {
-          long long o2 = row + (long long)col * p.ldc2;
+          const long long o2 = gr + (long long)gc * p.ldc2;
           double zr = vr, zi = vi;
           if (useb2) {
-            CT c = C2[o2];
+            const CT c = C2[o2];
             zr += p.beta2.x * (double)c.x - p.beta2.y * (double)c.y;
             zi += p.beta2.x * (double)c.y + p.beta2.y * (double)c.x;
           }
           C2[o2] = CT{(R)zr, (R)zi};
         }
       }
+    }
+    __syncthreads();
+  }
 }
 
 template <typename R>

@@ -4,7 +4,8 @@
 
 namespace hn {
 
-constexpr int kInvPanel = 16;  // panel width of the blocked path
+constexpr int kInvPanel = 32;  // max inner panel width of the blocked path
+constexpr int kInvSuper = 64;  // super-panel width: deferred rank-64 updates
 
 // Largest n handled by the single-workgroup LDS kernel.
 int inv_lds_max_n(int real_bytes);
@@ -13,8 +14,9 @@ int inv_lds_max_n(int real_bytes);
 template <typename R>
 struct InvWork {
   typedef typename std::conditional<sizeof(R) == 8, double2, float2>::type CT;
-  CT *Z = nullptr;      // n x nb per item
-  CT *Rb = nullptr;     // nb x n per item
+  CT *Z = nullptr;      // n x kInvPanel per item (inner panel multipliers)
+  CT *Zh = nullptr;     // n x kInvSuper per item (accumulated super-panel transform)
+  CT *Rb = nullptr;     // kInvSuper x n per item (pivot rows, leading dim kInvSuper)
   int *used = nullptr;  // n per item
   int *piv = nullptr;   // n per item
   int n = 0, batch = 0;

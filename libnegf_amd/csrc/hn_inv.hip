@@ -7,15 +7,20 @@
 // in-place Gauss-Jordan elimination with partial pivoting entirely in LDS,
 // followed by the column interchanges that undo the row pivoting.
 //
-// Large blocks: blocked in-place Gauss-Jordan with *lazy* pivoting.  Rows are
-// never moved; the pivot row of each logical column is recorded instead.
-// Per panel of nb columns:
-//   panel kernel (one workgroup per item, panel in LDS): partial-pivoting LU
-//     of the panel restricted to not-yet-pivoted rows picks nb pivot rows;
-//     P = inverse of the pivot block; writes Z = [-B(r,K) P ; P - I] (n x nb),
-//     a copy R of the pivot rows, and the new panel columns.
-//   update: B(:, J) += Z R(:, J) for all J outside the panel -- a rank-nb
-//     GEMM on the matrix cores (hn_gemm with the panel columns masked).
+// Large blocks: two-level blocked in-place Gauss-Jordan with *lazy* pivoting.
+// Rows are never moved; the pivot row of each logical column is recorded.
+// Per inner panel K of nb columns:
+//   panel kernel (one workgroup per item, panel in registers): partial-
+//     pivoting LU of the panel over not-yet-pivoted rows picks nb pivot rows;
+//     with Q = B(piv, K) it writes Z = [-B(r,K) Q^-1 ; Q^-1 - I] (n x nb), the
+//     new panel columns, and a copy R of the pivot rows.
+//   inner update: B(:, J) += Z R(:, J) only for the other columns J of the
+//     current super-panel S (kInvSuper columns).
+// After the inner panels of S, its columns hold Zhat + E where
+// I + Zhat P^T is the product of the inner rank-nb transforms, so the rest of
+// the matrix takes ONE deferred rank-kInvSuper update
+//   B(:, J) += Zhat R(:, J),  J outside S  (R = pivot rows before S started),
+// a GEMM on the matrix cores with 4x the intensity of a rank-16 update.
 // After the last panel, Ainv(i, c) = B(pi(i), pi^-1(c)) with pi(c) the pivot
 // row of logical column c.
 #include "hn_gemm.h"
@@ -158,168 +163,176 @@ __global__ __launch_bounds__(NT) void k_inv_reset(int n, int batch, int *used,
   used[e] = 0;
 }
 
-template <typename R, int NB>
-__global__ __launch_bounds__(NT) void k_inv_panel(int n, int c0, int nbp,
-                                                  typename CT_<R>::t *A, long long sA,
-                                                  typename CT_<R>::t *Z,
-                                                  typename CT_<R>::t *Rb, int *used_g,
-                                                  int *piv_g, const int *active, int *status) {
+// Register-resident panel kernel: thread t owns rows t + NT*q (q < RPT) of the
+// n x nb panel in registers.  Column j: block argmax over unused rows (one
+// wave shuffle reduction + one LDS exchange), the owner of the pivot row
+// publishes its row, every thread eliminates its own rows (multipliers kept in
+// place, LU storage).  Two barriers per column.  Afterwards the pivot block is
+// Q = L U (rows in selection order) and each thread solves W Q = B(r, K) for
+// its rows by one forward and one backward substitution (pivot rows solve
+// with e_j, giving row j of Q^-1), so Q is never inverted explicitly.
+template <typename R, int NB, int RPT>
+__global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, int rb_row,
+                                                      typename CT_<R>::t *A, long long sA,
+                                                      typename CT_<R>::t *Z,
+                                                      typename CT_<R>::t *Rb, int *used_g,
+                                                      int *piv_g, const int *active,
+                                                      int *status) {
   typedef typename CT_<R>::t C;
-  extern __shared__ __align__(16) unsigned char smem[];
   const int b = blockIdx.x;
   if (active && active[b] == 0) return;
-  C *P = (C *)smem;               // n x NB
-  int *used = (int *)(P + (size_t)n * NB);
-  int *pos = used + n;
-  __shared__ C Q[NB * NB];
-  __shared__ C qrow[NB], qcol[NB];
-  __shared__ int piv[NB], qperm[NB];
+  __shared__ C LU[NB][NB];  // LU[j][l]: row j of the pivot block's LU factors
+  __shared__ int piv[NB];
   __shared__ R redv[NT / 64];
   __shared__ int redi[NT / 64];
-  __shared__ int s_sing, s_p;
+  __shared__ int s_sing;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   C *Ab = A + (long long)b * sA;
-  C *Zb = Z + (long long)b * n * NB;
-  C *Rbb = Rb + (long long)b * NB * n;
+  C *Zb = Z + (long long)b * n * kInvPanel;
+  C *Rbb = Rb + (long long)b * kInvSuper * n + rb_row;
   int *ug = used_g + (long long)b * n;
-  for (int j = 0; j < nbp; ++j)
-    for (int r = tid; r < n; r += NT) P[r + (size_t)j * n] = Ab[r + (size_t)(c0 + j) * n];
-  for (int r = tid; r < n; r += NT) {
-    used[r] = ug[r];
-    pos[r] = -1;
+
+  C P[RPT][NB];
+  int used[RPT], pos[RPT];
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + NT * q;
+    const bool valid = r < n;
+    used[q] = valid ? ug[r] : 1;
+    pos[q] = -1;
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+      P[q][j] = (valid && j < nbp) ? Ab[r + (size_t)(c0 + j) * n] : C{0, 0};
   }
   if (tid == 0) s_sing = 0;
-  __syncthreads();
-  // ---- pivot selection: LU with partial pivoting over unused rows ----
-  for (int j = 0; j < nbp; ++j) {
+
+#pragma unroll
+  for (int j = 0; j < NB; ++j) {
+    if (j < nbp) {
+    // ---- argmax of |.|_1 over unused rows (NaN ranks below every number) ----
     R best = R(-1);
     int bi = -1;
-    for (int r = tid; r < n; r += NT)
-      if (!used[r]) {
-        R v = cabs1(P[r + (size_t)j * n]);
-        if (v > best || (v == best && (bi < 0 || r < bi))) { best = v; bi = r; }
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+      if (!used[q]) {
+        R v = cabs1(P[q][j]);
+        if (!(v == v)) v = R(-0.5);
+        if (v > best) { best = v; bi = tid + NT * q; }
       }
     wave_argmax(best, bi);
     if (lane == 0) { redv[wave] = best; redi[wave] = bi; }
     __syncthreads();
-    if (tid == 0) {
-      R bv = redv[0];
-      int bx = redi[0];
-      for (int w = 1; w < NT / 64; ++w)
-        if (redv[w] > bv || (redv[w] == bv && redi[w] >= 0 && (bx < 0 || redi[w] < bx))) {
-          bv = redv[w];
-          bx = redi[w];
-        }
-      if (bx < 0) {  // every candidate is NaN: take the first unused row
-        s_sing = 1;
-        for (int r = 0; r < n; ++r)
-          if (!used[r]) { bx = r; break; }
-        if (bx < 0) bx = 0;  // unreachable: fewer than n rows pivoted so far
+    R bv = redv[0];
+    int p = redi[0];
+#pragma unroll
+    for (int w = 1; w < NT / 64; ++w)
+      if (redv[w] > bv || (redv[w] == bv && redi[w] >= 0 && (p < 0 || redi[w] < p))) {
+        bv = redv[w];
+        p = redi[w];
       }
-      if (!(bv > R(0))) s_sing = 1;
-      piv[j] = bx;
-      used[bx] = 1;
-      pos[bx] = j;
-      s_p = bx;
+    if (tid == 0 && !(bv > R(0))) s_sing = 1;
+    // ---- the owner publishes its (L|U) row ----
+    if (p >= 0 && (p % NT) == tid) {
+#pragma unroll
+      for (int q = 0; q < RPT; ++q)
+        if (q == p / NT) {
+          used[q] = 1;
+          pos[q] = j;
+#pragma unroll
+          for (int l = 0; l < NB; ++l) LU[j][l] = P[q][l];
+        }
+      piv[j] = p;
     }
     __syncthreads();
-    const int p = s_p;
-    const C pinv = crecip(P[p + (size_t)j * n]);
-    for (int r = tid; r < n; r += NT)
-      if (!used[r]) {
-        C l = cmul(P[r + (size_t)j * n], pinv);
-        for (int jj = j + 1; jj < nbp; ++jj)
-          P[r + (size_t)jj * n] = csub(P[r + (size_t)jj * n], cmul(l, P[p + (size_t)jj * n]));
+    // ---- eliminate column j from the remaining rows ----
+    const C pinv = crecip(LU[j][j]);
+#pragma unroll
+    for (int q = 0; q < RPT; ++q)
+      if (!used[q]) {
+        const C l = cmul(P[q][j], pinv);
+        P[q][j] = l;
+#pragma unroll
+        for (int jj = j + 1; jj < NB; ++jj)
+          if (jj < nbp) P[q][jj] = csub(P[q][jj], cmul(l, LU[j][jj]));
       }
-    __syncthreads();
-  }
-  // ---- pivot block Q = B(piv, K) (original values) and its inverse ----
-  for (int e = tid; e < nbp * nbp; e += NT) {
-    int jr = e % nbp, l = e / nbp;
-    Q[jr + l * NB] = Ab[piv[jr] + (size_t)(c0 + l) * n];
+    }
   }
   __syncthreads();
-  for (int k = 0; k < nbp; ++k) {
-    if (tid == 0) {
-      int p = k;
-      R best = cabs1(Q[k + k * NB]);
-      for (int i = k + 1; i < nbp; ++i) {
-        R v = cabs1(Q[i + k * NB]);
-        if (v > best) { best = v; p = i; }
-      }
-      if (!(best > R(0))) s_sing = 1;
-      qperm[k] = p;
-      if (p != k)
-        for (int j = 0; j < nbp; ++j) {
-          C t = Q[k + j * NB];
-          Q[k + j * NB] = Q[p + j * NB];
-          Q[p + j * NB] = t;
-        }
-    }
-    __syncthreads();
-    const C pinv = crecip(Q[k + k * NB]);
-    if (tid < nbp) {
-      qrow[tid] = (tid == k) ? pinv : cmul(Q[k + tid * NB], pinv);
-      qcol[tid] = (tid == k) ? C{0, 0} : Q[tid + k * NB];
-    }
-    __syncthreads();
-    if (tid < nbp * nbp) {
-      int i = tid % nbp, j = tid / nbp;
-      C v = Q[i + j * NB];
-      if (i == k) v = qrow[j];
-      else if (j == k) v = cneg(cmul(qcol[i], pinv));
-      else v = csub(v, cmul(qcol[i], qrow[j]));
-      Q[i + j * NB] = v;
-    }
-    __syncthreads();
-  }
-  for (int k = nbp - 1; k >= 0; --k) {
-    const int p = qperm[k];
-    if (p != k && tid < nbp) {
-      C t = Q[tid + k * NB];
-      Q[tid + k * NB] = Q[tid + p * NB];
-      Q[tid + p * NB] = t;
-    }
-    __syncthreads();
-  }
-  // ---- copy of the pivot rows (all columns; the panel columns are unused) ----
+
+  // ---- copy of the pivot rows outside the panel columns ----
   for (int e = tid; e < nbp * n; e += NT) {
-    int j = e % nbp, c = e / nbp;
-    Rbb[j + (size_t)c * NB] = Ab[piv[j] + (size_t)c * n];
+    const int j = e % nbp, c = e / nbp;
+    if (c >= c0 && c < c0 + nbp) continue;
+    Rbb[j + (size_t)c * kInvSuper] = Ab[piv[j] + (size_t)c * n];
   }
-  __syncthreads();
-  // ---- Z and the new panel columns ----
-  for (int r = tid; r < n; r += NT) {
-    const int j = pos[r];
-    if (j >= 0) {
-      for (int l = 0; l < nbp; ++l) {
-        C v = Q[j + l * NB];
-        Ab[r + (size_t)(c0 + l) * n] = v;
-        if (l == j) v.x -= R(1);
-        Zb[r + (size_t)l * n] = v;
+
+  // ---- W Q = x  with Q = L U:  y U = x (forward), W L = y (backward) ----
+#pragma unroll
+  for (int q = 0; q < RPT; ++q) {
+    const int r = tid + NT * q;
+    if (r < n) {
+    const int pj = pos[q];
+    // the eliminated row is dead now: the right-hand side reuses its registers
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+      if (pj >= 0) P[q][l] = C{(l == pj) ? R(1) : R(0), R(0)};
+      else P[q][l] = (l < nbp) ? Ab[r + (size_t)(c0 + l) * n] : C{0, 0};
+    }
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+      if (l < nbp) {
+        C acc = P[q][l];
+#pragma unroll
+        for (int m = 0; m < l; ++m) acc = csub(acc, cmul(P[q][m], LU[m][l]));
+        P[q][l] = cmul(acc, crecip(LU[l][l]));
       }
-    } else {
-      C row[NB];
+    }
 #pragma unroll
-      for (int m = 0; m < NB; ++m) row[m] = (m < nbp) ? Ab[r + (size_t)(c0 + m) * n] : C{0, 0};
-      for (int l = 0; l < nbp; ++l) {
-        C w = {0, 0};
+    for (int l = NB - 1; l >= 0; --l) {
+      if (l < nbp) {
+        C acc = P[q][l];
 #pragma unroll
-        for (int m = 0; m < NB; ++m)
-          if (m < nbp) {
-            C q = Q[m + l * NB];
-            w.x += row[m].x * q.x - row[m].y * q.y;
-            w.y += row[m].x * q.y + row[m].y * q.x;
-          }
-        C nw = cneg(w);
+        for (int m = l + 1; m < NB; ++m)
+          if (m < nbp) acc = csub(acc, cmul(P[q][m], LU[m][l]));
+        P[q][l] = acc;
+      }
+    }
+#pragma unroll
+    for (int l = 0; l < NB; ++l) {
+      if (l >= nbp) continue;
+      C w = P[q][l];
+      if (pj >= 0) {
+        Ab[r + (size_t)(c0 + l) * n] = w;
+        if (l == pj) w.x -= R(1);
+        Zb[r + (size_t)l * n] = w;
+      } else {
+        const C nw = cneg(w);
         Ab[r + (size_t)(c0 + l) * n] = nw;
         Zb[r + (size_t)l * n] = nw;
       }
     }
+    ug[r] = used[q];
+    }
   }
-  for (int r = tid; r < n; r += NT) ug[r] = used[r];
   if (tid < nbp) piv_g[(long long)b * n + c0 + tid] = piv[tid];
   if (tid == 0 && s_sing && status) atomicOr(&status[b], (int)ST_SINGULAR);
+}
+
+// Zhat = B(:, s0:s0+sw) - E (E: ones at (pivot row, logical column))
+template <typename R>
+__global__ __launch_bounds__(NT) void k_inv_zhat(int n, int s0, int sw,
+                                                 const typename CT_<R>::t *A, long long sA,
+                                                 typename CT_<R>::t *Zh, const int *piv_g,
+                                                 const int *active) {
+  typedef typename CT_<R>::t C;
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)n * sw) return;
+  const int r = (int)(e % n), i = (int)(e / n);
+  C v = A[(long long)b * sA + r + (size_t)(s0 + i) * n];
+  if (piv_g[(long long)b * n + s0 + i] == r) v.x -= R(1);
+  Zh[(long long)b * n * kInvSuper + e] = v;
 }
 
 template <typename R>
@@ -387,30 +400,57 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
     HN_CHECK(hipGetLastError());
     return;
   }
-  constexpr int NB = kInvPanel;
   typedef typename CT_<R>::t C;
-  size_t lds = (size_t)n * NB * sizeof(C) + 2 * (size_t)n * sizeof(int);
-  if (lds > (size_t)kLdsBudget) throw std::runtime_error("inverse: block order too large");
+  (void)sizeof(C);
+  if (n > 512) throw std::runtime_error("inverse: block order > 512 is not supported");
   if (sA < (long long)n * n || sAinv < (long long)n * n)
     throw std::runtime_error("inverse: item stride smaller than n*n");
   if (!w.Z || w.n < n || w.batch < batch) throw std::runtime_error("inverse: workspace too small");
+  // panel width: FP32 32 up to n = 256; FP64 16 (a 32-wide FP64 panel row
+  // exceeds the register file); rows per thread RPT = 1 up to n = 256, else 2
+  const int NB = (sizeof(R) == 4 && n <= NT) ? 32 : 16;
   long long nb_tot = (long long)n * batch;
   hipLaunchKernelGGL(k_inv_reset<R>, dim3((unsigned)((nb_tot + NT - 1) / NT)), dim3(NT), 0, s,
                      n, batch, w.used, active);
-  allow_lds(k_inv_panel<R, NB>, lds);
-  for (int c0 = 0; c0 < n; c0 += NB) {
-    int nbp = std::min(NB, n - c0);
-    hipLaunchKernelGGL((k_inv_panel<R, NB>), dim3(batch), dim3(NT), lds, s, n, c0, nbp, A, sA,
-                       w.Z, w.Rb, w.used, w.piv, active, status);
-    HN_CHECK(hipGetLastError());
-    if (n > nbp) {
+  for (int s0 = 0; s0 < n; s0 += kInvSuper) {
+    const int sw = std::min(kInvSuper, n - s0);
+    for (int c0 = s0; c0 < s0 + sw; c0 += NB) {
+      const int nbp = std::min(NB, s0 + sw - c0);
+      if (n <= NT && NB == 32)
+        hipLaunchKernelGGL((k_inv_panel_reg<R, 32, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
+                           c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+      else if (n <= NT)
+        hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
+                           c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+      else
+        hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 2>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
+                           c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+      HN_CHECK(hipGetLastError());
+      if (sw > nbp) {  // inner update: the other columns of the super-panel
+        GemmArgs<R> g;
+        g.m = n; g.n = sw; g.k = nbp;
+        g.A = w.Z; g.sA = (long long)n * kInvPanel; g.lda = n; g.opA = OP_N;
+        g.B = w.Rb + (c0 - s0) + (size_t)s0 * kInvSuper; g.sB = (long long)kInvSuper * n;
+        g.ldb = kInvSuper; g.opB = OP_N;
+        g.C = A + (size_t)s0 * n; g.sC = sA; g.ldc = n;
+        g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
+        g.active = active; g.skip_c0 = c0 - s0; g.skip_c1 = c0 - s0 + nbp;
+        g.batch = batch;
+        if constexpr (sizeof(R) == 8) gemm_z(g, s);
+        else gemm_c(g, s);
+      }
+    }
+    if (n > sw) {  // deferred rank-sw update of every column outside the super-panel
+      hipLaunchKernelGGL(k_inv_zhat<R>, dim3((unsigned)(((long long)n * sw + NT - 1) / NT), batch),
+                         dim3(NT), 0, s, n, s0, sw, A, sA, w.Zh, w.piv, active);
+      HN_CHECK(hipGetLastError());
       GemmArgs<R> g;
-      g.m = n; g.n = n; g.k = nbp;
-      g.A = w.Z; g.sA = (long long)n * NB; g.lda = n; g.opA = OP_N;
-      g.B = w.Rb; g.sB = (long long)NB * n; g.ldb = NB; g.opB = OP_N;
+      g.m = n; g.n = n; g.k = sw;
+      g.A = w.Zh; g.sA = (long long)n * kInvSuper; g.lda = n; g.opA = OP_N;
+      g.B = w.Rb; g.sB = (long long)kInvSuper * n; g.ldb = kInvSuper; g.opB = OP_N;
       g.C = A; g.sC = sA; g.ldc = n;
       g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
-      g.active = active; g.skip_c0 = c0; g.skip_c1 = c0 + nbp;
+      g.active = active; g.skip_c0 = s0; g.skip_c1 = s0 + sw;
       g.batch = batch;
       if constexpr (sizeof(R) == 8) gemm_z(g, s);
       else gemm_c(g, s);
@@ -427,9 +467,9 @@ int inv_lds_max_n(int real_bytes) { return real_bytes == 8 ? lds_max_n<double>()
 
 size_t inv_work_bytes(int n, int batch, int real_bytes) {
   size_t c = 2 * (size_t)real_bytes;
-  size_t z = (size_t)n * kInvPanel * c * batch;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  return al(z) * 2 + al((size_t)n * batch * sizeof(int)) * 2;
+  return al((size_t)n * kInvPanel * c * batch) + 2 * al((size_t)n * kInvSuper * c * batch) +
+         al((size_t)n * batch * sizeof(int)) * 2;
 }
 
 template <typename R>
@@ -438,9 +478,9 @@ InvWork<R> inv_work_carve(void *buf, int n, int batch) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   InvWork<R> w;
   char *p = (char *)buf;
-  size_t z = al((size_t)n * kInvPanel * sizeof(C) * batch);
-  w.Z = (C *)p; p += z;
-  w.Rb = (C *)p; p += z;
+  w.Z = (C *)p; p += al((size_t)n * kInvPanel * sizeof(C) * batch);
+  w.Zh = (C *)p; p += al((size_t)n * kInvSuper * sizeof(C) * batch);
+  w.Rb = (C *)p; p += al((size_t)n * kInvSuper * sizeof(C) * batch);
   w.used = (int *)p; p += al((size_t)n * batch * sizeof(int));
   w.piv = (int *)p;
   w.n = n;

@@ -57,6 +57,15 @@ void decim_check_c(const float2 *C1, long long sC1, int n2, int batch, double ac
                    int maxit, int *act, int *upd, int *okco, int *ncyc, int *status, int crit,
                    hipStream_t s);
 
+// out[i] = (a[i] != 0) | (b[i] != 0)
+void or_flags(const int *a, const int *b, int *out, int n, hipStream_t s);
+
+// Mirror test of two contacts in every slot: flags[slot] = 1 iff
+// X0 == Y0 and X1 == Y1^H bitwise (X0/Y0: PL blocks, X1/Y1: PL1->PL2 couplings),
+// n x n blocks at the given offsets.  flags must be preset to 1.
+void mirror_check(const double2 *pool, long long slot_stride, int nslots, int n, long long oX0,
+                  long long oY0, long long oX1, long long oY1, int *flags, hipStream_t s);
+
 // count of nonzero flags (written to *count, device)
 void count_nonzero(const int *flags, int batch, int *count, hipStream_t s);
 
@@ -94,6 +103,44 @@ void z2c(const double2 *src, long long ss, float2 *dst, long long sd, long long 
          const int *active, hipStream_t st);
 void c2z(const float2 *src, long long ss, double2 *dst, long long sd, long long n, int batch,
          const int *active, hipStream_t st);
+
+// ---- SCBA helpers -------------------------------------------------------------
+// Y_b = X_b^H  (X rows x cols -> Y cols x rows)
+void dagger_batched(const double2 *X, long long sX, double2 *Y, long long sY, int rows, int cols,
+                    int batch, const int *active, hipStream_t st);
+// d_b[i] = M_b(i,i) * scale[i] for active items
+void diag_extract_masked(const double2 *M, long long sM, int n, int batch, double2 *d,
+                         long long sd, const double *scale, const int *active, hipStream_t st);
+// Sn_b = diag(v_b) + sum_q w_q(b) G_q,b   (q < nq <= 2; w_q = frm_b[j_q] - frm_b[ref-1],
+// dropped when |w_q| <= eps); frm_b = frm + b * frm_stride
+struct SnArgs {
+  double2 *Sn = nullptr; long long sSn = 0;
+  int n = 0;
+  const double2 *v = nullptr; long long sv = 0;  // diagonal (may be null)
+  const double2 *G[2] = {nullptr, nullptr}; long long sG = 0;
+  int j[2] = {0, 0};
+  int nq = 0;
+  const double *frm = nullptr; int frm_stride = 0; int ref = 0;
+  double eps = 0.0;
+  const int *active = nullptr;
+  int batch = 0;
+};
+void sn_form(const SnArgs &a, hipStream_t st);
+// err_b = max(err_b, max |X_b - Xold_b| over the mask), then Xold_b = X_b
+// mask mode 0: diagonal entries; 1: entries where the slot's block `mask` != 0;
+// 2: entries (r,c) where mask(r,c) != 0 or mask2(c,r) != 0 (mask2: cols x rows)
+void masked_diff(const double2 *X, double2 *Xold, long long s, int rows, int cols, int mode,
+                 BlkRef mask, BlkRef mask2, const int *slot, unsigned long long *err,
+                 const int *active, int batch, hipStream_t st);
+// per-item SCBA state after pass p: conv = p > 0 && err < tol; act = !conv && p+1 <= niter
+void scba_state(const unsigned long long *err, double tol, int pass, int niter, int *act,
+                int *iters, double *err_out, int batch, hipStream_t st);
+// I_b,c = -Re( Tr[Gam Gn] - f_c Tr[Gam A] ), A = i(Gr - Gr^H)  (iterative.F90:382-414)
+void trace_mw(const double2 *Gam, const double2 *Gn, const double2 *Gr, long long s, int n,
+              const double *frm, int frm_stride, int c, int ncont, int batch, double *curr,
+              hipStream_t st);
+// out[i] = x[i]^2
+void square(const double *x, double *out, int n, hipStream_t st);
 
 // out_b = M_b + diag(v_b)*alpha  (add a scaled vector on the diagonal), n x n
 void add_diag(double2 *M, long long sM, int n, const double2 *v, long long sv, double2 alpha,

@@ -3,6 +3,8 @@
 // These replace the reference's per-block helper calls (zsumdns, zdagger,
 // zspectral, ztrace, cu_Zmatsum / cu_Ztrace / cu_Zasum, sparsekit_drv.F90 and
 // gpuroutines.cu:567-739) by whole-batch launches; all are HBM-streaming.
+#include <algorithm>
+
 #include "hn_kernels.h"
 
 namespace hn {
@@ -165,6 +167,25 @@ __global__ __launch_bounds__(NT) void k_decim_check(const C *C1, long long sC1, 
   }
 }
 
+__global__ __launch_bounds__(NT) void k_or_flags(const int *a, const int *b, int *out, int n) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) out[i] = (a[i] != 0) | (b[i] != 0);
+}
+
+__global__ __launch_bounds__(NT) void k_mirror(const double2 *pool, long long slot_stride, int n,
+                                               long long oX0, long long oY0, long long oX1,
+                                               long long oY1, int *flags) {
+  const int slot = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  const double2 *base = pool + (long long)slot * slot_stride;
+  const double2 x0 = base[oX0 + e], y0 = base[oY0 + e];
+  const double2 x1 = base[oX1 + e], y1t = base[oY1 + c + (long long)r * n];
+  const bool ok = x0.x == y0.x && x0.y == y0.y && x1.x == y1t.x && x1.y == -y1t.y;
+  if (!ok) atomicAnd(&flags[slot], 0);
+}
+
 __global__ __launch_bounds__(NT) void k_count(const int *flags, int batch, int *count) {
   __shared__ int sh[NT / 64];
   int c = 0;
@@ -315,6 +336,135 @@ __global__ __launch_bounds__(NT) void k_add_diag(double2 *M, long long sM, int n
   *p = m;
 }
 
+
+__global__ __launch_bounds__(NT) void k_dagger(const double2 *X, long long sX, double2 *Y,
+                                               long long sY, int rows, int cols,
+                                               const int *active) {
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)rows * cols) return;
+  const int r = (int)(e % cols), c = (int)(e / cols);  // Y is cols x rows
+  const double2 x = X[(long long)b * sX + c + (long long)r * rows];
+  Y[(long long)b * sY + e] = make_double2(x.x, -x.y);
+}
+
+__global__ __launch_bounds__(NT) void k_diag_extract_m(const double2 *M, long long sM, int n,
+                                                       double2 *d, long long sd,
+                                                       const double *scale, const int *active) {
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  double2 v = M[(long long)b * sM + i + (long long)i * n];
+  double s = scale ? scale[i] : 1.0;
+  d[(long long)b * sd + i] = make_double2(v.x * s, v.y * s);
+}
+
+__global__ __launch_bounds__(NT) void k_sn_form(SnArgs a) {
+  const int b = blockIdx.y;
+  if (a.active && a.active[b] == 0) return;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)a.n * a.n) return;
+  const int r = (int)(e % a.n), c = (int)(e / a.n);
+  double2 out = make_double2(0.0, 0.0);
+  if (a.v && r == c) out = a.v[(long long)b * a.sv + r];
+  const double *f = a.frm + (long long)b * a.frm_stride;
+  for (int q = 0; q < a.nq; ++q) {
+    const double w = f[a.j[q]] - f[a.ref - 1];
+    if (fabs(w) > a.eps) {
+      const double2 g = a.G[q][(long long)b * a.sG + e];
+      out.x += w * g.x;
+      out.y += w * g.y;
+    }
+  }
+  a.Sn[(long long)b * a.sSn + e] = out;
+}
+
+__global__ __launch_bounds__(NT) void k_masked_diff(const double2 *X, double2 *Xold, long long s,
+                                                    int rows, int cols, int mode, BlkRef mask,
+                                                    BlkRef mask2, const int *slot,
+                                                    unsigned long long *err, const int *active) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.y;
+  if (active && active[b] == 0) return;
+  const long long nn = (long long)rows * cols;
+  const double2 *Xb = X + (long long)b * s;
+  double2 *Ob = Xold + (long long)b * s;
+  const long long sl = slot ? slot[b] : 0;
+  const double2 *Mb = mode ? mask.pool + sl * mask.slot_stride + mask.off : nullptr;
+  const double2 *M2 = mode == 2 ? mask2.pool + sl * mask2.slot_stride + mask2.off : nullptr;
+  double m = 0.0;
+  for (long long e = (long long)blockIdx.x * NT + threadIdx.x; e < nn; e += (long long)gridDim.x * NT) {
+    const int r = (int)(e % rows), c = (int)(e / rows);
+    bool in = false;
+    if (mode == 0) in = (r == c);
+    else if (mode == 1) { const double2 v = Mb[e]; in = (v.x != 0.0 || v.y != 0.0); }
+    else {
+      const double2 v = Mb[e], w = M2[c + (long long)r * cols];
+      in = (v.x != 0.0 || v.y != 0.0 || w.x != 0.0 || w.y != 0.0);
+    }
+    const double2 x = Xb[e];
+    if (in) {
+      const double2 o = Ob[e];
+      const double d = hypot(x.x - o.x, x.y - o.y);
+      m = (d > m || d != d) ? d : m;
+    }
+    Ob[e] = x;
+  }
+  m = block_max(m, sh);
+  if (threadIdx.x == 0) {
+    // non-negative doubles order like their bit patterns; NaN maps above +inf
+    unsigned long long bits = (m != m) ? 0x7ff8000000000000ULL : (unsigned long long)__double_as_longlong(m);
+    atomicMax(&err[b], bits);
+  }
+}
+
+__global__ __launch_bounds__(NT) void k_scba_state(const unsigned long long *err, double tol,
+                                                   int pass, int niter, int *act, int *iters,
+                                                   double *err_out, int batch) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  if (b >= batch || act[b] == 0) return;
+  const double e = __longlong_as_double((long long)err[b]);
+  bool conv = false;
+  if (pass > 0) {
+    err_out[b] = e;
+    conv = e < tol;
+  }
+  iters[b] = pass + 1;
+  act[b] = (!conv && pass + 1 <= niter) ? 1 : 0;
+}
+
+__global__ __launch_bounds__(NT) void k_trace_mw(const double2 *Gam, const double2 *Gn,
+                                                 const double2 *Gr, long long s, int n,
+                                                 const double *frm, int frm_stride, int c,
+                                                 int ncont, double *curr) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.x;
+  const double2 *Gm = Gam + (long long)b * s, *Nb = Gn + (long long)b * s, *Rb = Gr + (long long)b * s;
+  const double f = frm[(long long)b * frm_stride + c];
+  double tg = 0.0, ta = 0.0;
+  const long long nn = (long long)n * n;
+  for (long long e = threadIdx.x; e < nn; e += NT) {
+    const int r = (int)(e % n), cc = (int)(e / n);
+    const double2 g = Gm[e];
+    const double2 gn = Nb[cc + (long long)r * n];          // Gn(c, r)
+    tg += g.x * gn.x - g.y * gn.y;
+    const double2 a1 = Rb[cc + (long long)r * n], a2 = Rb[e];  // Gr(c,r), Gr(r,c)
+    // A(c,r) = i (Gr(c,r) - conj(Gr(r,c)))
+    const double2 d = make_double2(a1.x - a2.x, a1.y + a2.y);
+    const double2 A = make_double2(-d.y, d.x);
+    ta += g.x * A.x - g.y * A.y;
+  }
+  const double v = block_sum(tg - f * ta, sh);
+  if (threadIdx.x == 0) curr[(long long)b * ncont + c] = -v;
+}
+
+__global__ __launch_bounds__(NT) void k_square(const double *x, double *out, int n) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) out[i] = x[i] * x[i];
+}
+
 inline dim3 grid2(long long n, int batch) {
   return dim3((unsigned)((n + NT - 1) / NT), (unsigned)batch);
 }
@@ -352,6 +502,19 @@ void decim_check_c(const float2 *C1, long long sC1, int n2, int batch, double ac
                    hipStream_t s) {
   hipLaunchKernelGGL(k_decim_check<float2>, dim3(batch), dim3(NT), 0, s, C1, sC1, n2, acc, it,
                      maxit, act, upd, okco, ncyc, status, crit);
+  HN_CHECK(hipGetLastError());
+}
+
+void or_flags(const int *a, const int *b, int *out, int n, hipStream_t s) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_or_flags, dim3((n + NT - 1) / NT), dim3(NT), 0, s, a, b, out, n);
+  HN_CHECK(hipGetLastError());
+}
+
+void mirror_check(const double2 *pool, long long slot_stride, int nslots, int n, long long oX0,
+                  long long oY0, long long oX1, long long oY1, int *flags, hipStream_t s) {
+  hipLaunchKernelGGL(k_mirror, grid2((long long)n * n, nslots), dim3(NT), 0, s, pool, slot_stride,
+                     n, oX0, oY0, oX1, oY1, flags);
   HN_CHECK(hipGetLastError());
 }
 
@@ -425,6 +588,50 @@ void c2z(const float2 *src, long long ss, double2 *dst, long long sd, long long 
          const int *active, hipStream_t st) {
   if (n <= 0 || batch <= 0) return;
   hipLaunchKernelGGL(k_c2z, grid2(n, batch), dim3(NT), 0, st, src, ss, dst, sd, n, active);
+  HN_CHECK(hipGetLastError());
+}
+
+void dagger_batched(const double2 *X, long long sX, double2 *Y, long long sY, int rows, int cols,
+                    int batch, const int *active, hipStream_t st) {
+  hipLaunchKernelGGL(k_dagger, grid2((long long)rows * cols, batch), dim3(NT), 0, st, X, sX, Y, sY,
+                     rows, cols, active);
+  HN_CHECK(hipGetLastError());
+}
+void diag_extract_masked(const double2 *M, long long sM, int n, int batch, double2 *d,
+                         long long sd, const double *scale, const int *active, hipStream_t st) {
+  hipLaunchKernelGGL(k_diag_extract_m, grid2(n, batch), dim3(NT), 0, st, M, sM, n, d, sd, scale,
+                     active);
+  HN_CHECK(hipGetLastError());
+}
+void sn_form(const SnArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL(k_sn_form, grid2((long long)a.n * a.n, a.batch), dim3(NT), 0, st, a);
+  HN_CHECK(hipGetLastError());
+}
+void masked_diff(const double2 *X, double2 *Xold, long long s, int rows, int cols, int mode,
+                 BlkRef mask, BlkRef mask2, const int *slot, unsigned long long *err,
+                 const int *active, int batch, hipStream_t st) {
+  long long nn = (long long)rows * cols;
+  unsigned gx = (unsigned)std::min<long long>((nn + NT - 1) / NT, 64);
+  hipLaunchKernelGGL(k_masked_diff, dim3(gx, batch), dim3(NT), 0, st, X, Xold, s, rows, cols, mode,
+                     mask, mask2, slot, err, active);
+  HN_CHECK(hipGetLastError());
+}
+void scba_state(const unsigned long long *err, double tol, int pass, int niter, int *act,
+                int *iters, double *err_out, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(k_scba_state, dim3((batch + NT - 1) / NT), dim3(NT), 0, st, err, tol, pass,
+                     niter, act, iters, err_out, batch);
+  HN_CHECK(hipGetLastError());
+}
+void trace_mw(const double2 *Gam, const double2 *Gn, const double2 *Gr, long long s, int n,
+              const double *frm, int frm_stride, int c, int ncont, int batch, double *curr,
+              hipStream_t st) {
+  hipLaunchKernelGGL(k_trace_mw, dim3(batch), dim3(NT), 0, st, Gam, Gn, Gr, s, n, frm, frm_stride,
+                     c, ncont, curr);
+  HN_CHECK(hipGetLastError());
+}
+void square(const double *x, double *out, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_square, dim3((n + NT - 1) / NT), dim3(NT), 0, st, x, out, n);
   HN_CHECK(hipGetLastError());
 }
 

@@ -50,3 +50,50 @@ def brute_T(s, E, sigmas, ci=0, cf=1):
     bi, bf = s.contacts[ci].cblk - 1, s.contacts[cf].cblk - 1
     Gfi = G[off[bf]:off[bf + 1], off[bi]:off[bi + 1]]
     return float(np.real(np.trace(gamma(sigmas[cf]) @ Gfi @ gamma(sigmas[ci]) @ Gfi.conj().T)))
+
+
+def from_dense(H, S, plend, surfstart, contend, cblk):
+    """BlockStructure of a dense PL-ordered H (and S, or None for identity) with
+    the reference's structure description (1-based, no surface blocks):
+    device PLs end at plend, contact c occupies surfstart[c]..contend[c]
+    (two principal layers) and couples to PL cblk[c]."""
+    pls = [0] + [int(p) for p in plend[:-1]]
+    ple = [int(p) for p in plend]
+    blk = lambda M, a, b, c, d: np.array(M[a:b, c:d], dtype=np.complex128)
+    nbl = len(ple)
+    Hd = [blk(H, pls[i], ple[i], pls[i], ple[i]) for i in range(nbl)]
+    Hu = [blk(H, pls[i], ple[i], pls[i + 1], ple[i + 1]) for i in range(nbl - 1)]
+    Hl = [blk(H, pls[i + 1], ple[i + 1], pls[i], ple[i]) for i in range(nbl - 1)]
+    Sd = Su = Sl = None
+    if S is not None:
+        Sd = [blk(S, pls[i], ple[i], pls[i], ple[i]) for i in range(nbl)]
+        Su = [blk(S, pls[i], ple[i], pls[i + 1], ple[i + 1]) for i in range(nbl - 1)]
+        Sl = [blk(S, pls[i + 1], ple[i + 1], pls[i], ple[i]) for i in range(nbl - 1)]
+    cs = []
+    for c in range(len(cblk)):
+        a = int(surfstart[c]) - 1
+        n = (int(contend[c]) - a) // 2
+        b = int(cblk[c]) - 1
+        kw = dict(cblk=int(cblk[c]), H00=blk(H, a, a + n, a, a + n), H01=blk(H, a, a + n, a + n, a + 2 * n),
+                  HMC=blk(H, pls[b], ple[b], a, a + n))
+        if S is not None:
+            kw.update(S00=blk(S, a, a + n, a, a + n), S01=blk(S, a, a + n, a + n, a + 2 * n),
+                      SMC=blk(S, pls[b], ple[b], a, a + n))
+        cs.append(Contact(**kw))
+    return BlockStructure(Hd=Hd, Hu=Hu, Hl=Hl, contacts=cs, Sd=Sd, Su=Su, Sl=Sl, name="dense")
+
+
+def chain_dense(n_dev=60, n_cont=20, t=1.0, onsite=None):
+    """Dense 100-site H of the reference tests' chain with the site order of
+    tests/c_int_elph_deph/c_int_elph_deph.c:15-21 (device 1..60, contact 1
+    61..80 attached at site 60, contact 2 81..100 attached at site 1)."""
+    N = n_dev + 2 * n_cont
+    H = np.zeros((N, N), dtype=np.complex128)
+    for i in range(1, n_dev + n_cont):           # 1..80 chain
+        H[i, i - 1] = H[i - 1, i] = t
+    for i in range(n_dev + n_cont + 1, N):       # 81..100 chain
+        H[i, i - 1] = H[i - 1, i] = t
+    H[0, n_dev + n_cont] = H[n_dev + n_cont, 0] = t   # site 1 <-> 81
+    if onsite is not None:
+        H[np.arange(n_dev), np.arange(n_dev)] += onsite
+    return H

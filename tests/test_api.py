@@ -1,0 +1,191 @@
+"""The C ABI boundary: library exports, the lnparams layout, host-side error
+handling of the negf_* API (CPU), and the reference's own API-level scenarios
+run through negf_* on the GPU (tests/c_int_elph_deph/c_int_elph_deph.c,
+tests/f90elph_deph/hello.F90, tests/cpython/hello_ctypes.py)."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from tests.chains import chain_dense, from_dense
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for h in ("hipnegf.h", "hipnegf_api.h"):
+        txt = open(os.path.join(ROOT, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b((?:hipnegf|negf)_\w+)\s*\(", txt))
+    return sorted(names)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    from libnegf_amd import hip
+    return hip.load()
+
+
+def test_library_exports_every_declared_symbol(lib):
+    names = _declared()
+    assert len(names) > 60
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_lnparams_layout_matches_header(tmp_path):
+    """The ctypes mirror has the size and field offsets of struct lnparams."""
+    from libnegf_amd.negf import LNParams
+    fields = [f[0] for f in LNParams._fields_]
+    src = tmp_path / "p.c"
+    body = "".join(f'  printf("%zu\\n", offsetof(struct lnparams, {f}));\n' for f in fields)
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "hipnegf_api.h"\n'
+                   'int main(void) {\n  printf("%zu\\n", sizeof(struct lnparams));\n' + body +
+                   "  return 0;\n}\n")
+    exe = tmp_path / "p"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True,
+                                          check=True).stdout.split()]
+    assert out[0] == C.sizeof(LNParams)
+    assert out[1:] == [getattr(LNParams, f).offset for f in fields]
+
+
+def test_defaults_and_host_errors():
+    """Defaults of set_defaults (lib_param.F90:602-669) and host-side error
+    reporting; nothing here touches the GPU."""
+    from libnegf_amd.negf import NEGF, NegfError
+    n = NEGF()
+    p = n.params
+    assert p.delta == 1e-4 and p.g_spin == 2.0 and p.kwght == 1.0 and p.eneconv == 1.0
+    assert p.ni[0] == 1 and p.nf[0] == 2 and p.scba_elastic_tol == 1e-7
+    with pytest.raises(NegfError, match="npl"):
+        n.init_structure(2, [80, 100], [60, 80], 0, [60], [1, 1])
+    n._lib.negf_init_contacts(n._handler, 2)
+    p.emin, p.emax, p.estep = -1.0, 1.0, 0.5
+    with pytest.raises(NegfError, match="structure not initialised"):
+        n.solve_landauer()
+    with pytest.raises(NegfError, match="create_HS"):
+        n.set_s_id(10, 3)
+    n.close()
+
+
+# ------------------------------------------------------------- GPU scenarios ----
+@pytest.fixture(scope="module")
+def gpu_negf():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from libnegf_amd.negf import NEGF
+    from oracle import pyoracle
+    pyoracle.build()
+    pyoracle.set_backend("builtin")
+    return NEGF, pyoracle
+
+
+def _chain_session(NEGF, H, plend, cblk, emin, emax, estep):
+    n = NEGF()
+    n.set_h(sp.csr_matrix(H))
+    n.set_s_id(H.shape[0])
+    n.init_structure(2, np.array([80, 100]), np.array([60, 80]), len(plend), np.array(plend),
+                     np.array(cblk), surfstart=np.array([61, 81]))
+    p = n.params
+    p.emin, p.emax, p.estep = emin, emax, estep
+    p.kbt_t[0] = p.kbt_t[1] = 1e-3
+    p.mu[0], p.mu[1] = -0.5, 0.5
+    return n
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("coupling,niter,estep,lo,hi", [
+    (0.0, 5, 0.1, 2.0 - 1e-4, 2.0 + 1e-4),    # tests/f90elph_deph/hello.F90:83-100
+    (0.05, 5, 0.1, 1.90, 1.95),                # tests/f90elph_deph/hello.F90:108-120
+    (0.05, 10, 0.01, 1.90, 1.95),              # tests/c_int_elph_deph/c_int_elph_deph.c:121-152
+])
+def test_dephasing_current_scenarios(gpu_negf, coupling, niter, estep, lo, hi):
+    NEGF, orc = gpu_negf
+    plend = [10, 20, 30, 40, 50, 60]
+    n = _chain_session(NEGF, chain_dense(), plend, [6, 1], -2.0, 2.0, estep)
+    n.set_elph_dephasing(np.full(60, coupling), niter)
+    n.solve_landauer()
+    cur = n.get_currents()
+    assert cur.shape == (1,)
+    assert lo <= abs(cur[0]) <= hi, cur
+    # per-energy effective transmission and Meir-Wingreen currents vs the oracle
+    E, _ = n.get_energies()
+    tr = n.get_transmission()
+    ec = n.get_energy_current()
+    s = from_dense(chain_dense(), None, plend, [61, 81], [80, 100], [6, 1])
+    ref = np.array([orc.scba_mw(s, e, 1e-4, np.full(60, coupling), niter, 1e-7, [1.0, 0.0, 0.0], 3)["curr"]
+                    for e in E[::7]])
+    assert np.abs(ec[:, ::7].T - ref).max() <= 1e-10 * np.abs(ref).max()
+    np.testing.assert_array_equal(tr[0], ec[0])
+    assert (n.get_item_status() == 0).all()
+    n.close()
+
+
+@pytest.mark.gpu
+def test_landauer_double_barrier(gpu_negf):
+    """tests/cpython/hello_ctypes.py structure: 4 PLs of 15 sites, barriers on
+    sites 26 and 36, contact 1 on PL 4, contact 2 on PL 1; T(E) and the
+    integrated current vs the oracle."""
+    NEGF, orc = gpu_negf
+    ons = np.zeros(60)
+    ons[25] = ons[35] = 1.0
+    H = chain_dense(onsite=ons)
+    plend = [15, 30, 45, 60]
+    n = _chain_session(NEGF, H, plend, [4, 1], -3.0, 3.0, 0.01)
+    n.params.mu[0], n.params.mu[1] = 0.1, 0.0
+    n.solve_landauer()
+    E, _ = n.get_energies()
+    T = n.get_transmission()
+    assert T.shape == (1, 601)
+    s = from_dense(H, None, plend, [61, 81], [80, 100], [4, 1])
+    Tref, _, _ = orc.transmission_batch(s, E, 1e-4)
+    big = Tref > 1e-6 * Tref.max()
+    assert (np.abs(T[0][big] - Tref[big]) / Tref[big]).max() <= 1e-10
+    cur = n.get_currents()
+    cref = 2.0 * orc.integrate_el(Tref, 0.1, 0.0, 1e-3, 1e-3, -3.0, 3.0, 0.01)
+    assert abs(cur[0] - cref) <= 1e-10 * abs(cref)
+    n.close()
+
+
+@pytest.mark.gpu
+def test_kpoint_slots_and_overlap(gpu_negf):
+    """negf_create_hs with two k indices and an explicit overlap; each solve
+    uses the selected H(k)/S(k) pair (negf%HS(iK))."""
+    NEGF, orc = gpu_negf
+    rng = np.random.default_rng(5)
+    plend = [10, 20, 30, 40, 50, 60]
+    Hs, Ss = [], []
+    for k in range(2):
+        H = chain_dense(onsite=0.3 * rng.standard_normal(60))
+        S = np.eye(100, dtype=np.complex128)
+        for i in range(1, 100):
+            if H[i, i - 1] != 0:
+                S[i, i - 1] = S[i - 1, i] = 0.05 * (k + 1)
+        Hs.append(H)
+        Ss.append(S)
+    n = NEGF()
+    n.create_hs(2)
+    for k in range(2):
+        n.set_h(sp.csr_matrix(Hs[k]), ik=k + 1)
+        n.set_s(sp.csr_matrix(Ss[k]), ik=k + 1)
+    n.init_structure(2, np.array([80, 100]), np.array([60, 80]), 6, np.array(plend), np.array([6, 1]),
+                     surfstart=np.array([61, 81]))
+    p = n.params
+    p.emin, p.emax, p.estep = -1.5, 1.5, 0.05
+    for k in (2, 1):
+        n.set_kpoint(k)
+        n.solve_landauer()
+        E, _ = n.get_energies()
+        T = n.get_transmission()[0]
+        s = from_dense(Hs[k - 1], Ss[k - 1], plend, [61, 81], [80, 100], [6, 1])
+        Tref, _, _ = orc.transmission_batch(s, E, 1e-4)
+        big = Tref > 1e-6 * Tref.max()
+        assert (np.abs(T[big] - Tref[big]) / Tref[big]).max() <= 1e-10
+    n.close()

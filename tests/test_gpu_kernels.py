@@ -97,7 +97,7 @@ def _inv_case(rng, n, kind):
         return (0.3 + 1e-3j) * np.eye(n) - H
 
 
-@pytest.mark.parametrize("n", [1, 5, 10, 32, 64, 96, 100, 101, 128, 200, 256, 300])
+@pytest.mark.parametrize("n", [1, 5, 10, 32, 64, 96, 100, 101, 128, 130, 200, 256, 300, 383, 449, 512])
 @pytest.mark.parametrize("kind", ["rand", "perm", "green"])
 def test_zinverse_batched(gpu, n, kind):
     torch, hip, eng = gpu
@@ -132,7 +132,7 @@ def test_zinverse_singular_flag(gpu):
         assert int(st.cpu()[0]) & hip.ST_SINGULAR
 
 
-@pytest.mark.parametrize("n", [10, 64, 128, 256])
+@pytest.mark.parametrize("n", [10, 64, 128, 200, 256, 320, 512])
 def test_cinverse_batched(gpu, n):
     torch, hip, eng = gpu
     rng = np.random.default_rng(n)

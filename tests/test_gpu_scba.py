@@ -1,0 +1,157 @@
+"""GPU parity tests of the batched elastic-dephasing SCBA + Meir-Wingreen path
+(C ABI hipnegf_solve_scba_mw) against the CPU oracle on the same inputs.
+
+Bar (complex FP64): per-contact currents within 1e-10 relative to the largest
+|I| of the grid, G^n / G^r diagonals within 1e-10 relative to their largest
+entry, identical SCBA pass counts, and the final SCBA error within 1e-6
+relative (it is a difference of nearly equal G^n values, so it carries the
+cancellation of that subtraction).
+"""
+import numpy as np
+import pytest
+
+from libnegf_amd import workloads as wl
+from tests.chains import chain_structure
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from libnegf_amd import hip
+    return hip, hip.Engine(0)
+
+
+@pytest.fixture(scope="module")
+def orc():
+    from oracle import pyoracle
+    pyoracle.build()
+    pyoracle.set_backend("builtin")
+    return pyoracle
+
+
+def _ref(orc, s, E, delta, cp, niter, tol, frm, ref):
+    rs = [orc.scba_mw(s, e, delta, cp, niter, tol, frm[j] if np.ndim(frm) == 2 else frm, ref)
+          for j, e in enumerate(E)]
+    return dict(curr=np.array([r["curr"] for r in rs]), iters=np.array([r["iters"] for r in rs]),
+                err=np.array([r["err"] for r in rs]), gn=np.array([r["gn_diag"] for r in rs]),
+                gr=np.array([r["gr_diag"] for r in rs]))
+
+
+def _check(r, q, err_rtol=1e-6):
+    assert (r["status"] == 0).all(), r["status"]
+    np.testing.assert_array_equal(r["iters"], q["iters"])
+    scale = np.abs(q["curr"]).max()
+    assert np.abs(r["curr"] - q["curr"]).max() <= RTOL * scale
+    for k, kk in (("gn_diag", "gn"), ("gr_diag", "gr")):
+        sc = np.abs(q[kk]).max()
+        assert np.abs(r[k] - q[kk]).max() <= RTOL * sc, k
+    big = q["err"] > 1e-13
+    assert np.all(np.abs(r["err"][big] - q["err"][big]) <= err_rtol * q["err"][big] + 1e-14)
+
+
+@pytest.mark.parametrize("coupling,niter", [(0.0, 5), (0.05, 5), (0.05, 10)])
+def test_chain_known_answer_setup(gpu, orc, coupling, niter):
+    """The reference's dephasing tests' 60-site chain (tests/f90elph_deph/hello.F90:83-120,
+    tests/c_int_elph_deph/c_int_elph_deph.c:121-152), per-energy parity and the
+    integrated current window."""
+    hip, eng = gpu
+    s = chain_structure()
+    estep = 0.1
+    npts = orc.npoints(-2.0, 2.0, estep)
+    E = -2.0 + estep * np.arange(npts)
+    cp = np.full(60, coupling)
+    ds = hip.DeviceStructure(s)
+    r = hip.solve_scba_mw(eng, ds, E, cp, niter, 1e-7, [1.0, 0.0, 0.0], 3, delta=1e-4,
+                          want_diag=True)
+    q = _ref(orc, s, E, 1e-4, cp, niter, 1e-7, [1.0, 0.0, 0.0], 3)
+    _check(r, q)
+    cur = 2.0 * orc.integrate_el(r["curr"][:, 0], -0.5, 0.5, 1e-3, 1e-3, -2.0, 2.0, estep)
+    lo, hi = (2.0 - 1e-4, 2.0 + 1e-4) if coupling == 0.0 else (1.90, 1.95)
+    assert lo <= abs(cur) <= hi, cur
+
+
+def test_ribbon_converged_conservation(gpu, orc):
+    """Converged elastic dephasing conserves current: I_1 + I_2 = 0."""
+    hip, eng = gpu
+    s = wl.ribbon(12, 8, disorder=0.3, seed=19)
+    cp = np.full(s.ndev, 0.1)
+    E = np.array([-1.7, -0.9, 0.35, 1.2])
+    ds = hip.DeviceStructure(s)
+    r = hip.solve_scba_mw(eng, ds, E, cp, 300, 1e-11, [1.0, 0.0, 0.0], 3, delta=1e-5,
+                          want_diag=True)
+    q = _ref(orc, s, E, 1e-5, cp, 300, 1e-11, [1.0, 0.0, 0.0], 3)
+    _check(r, q, err_rtol=1e-3)
+    assert (r["err"] < 1e-11).all()
+    I1, I2 = r["curr"][:, 0], r["curr"][:, 1]
+    assert np.abs(I1 + I2).max() <= 1e-8 * np.abs(I1).max()
+
+
+def _sparsify(s, seed):
+    """Zero a Hermitian-consistent random subset of the device overlap entries so the
+    S pattern differs from the H pattern (Sd(i) keeps its diagonal, Sl(i) = Su(i)^H)."""
+    from libnegf_amd.structure import BlockStructure
+    rng = np.random.default_rng(seed)
+    n = s.Hd[0].shape[0]
+    Sd, Su, Sl = [], [], []
+    for x in s.Sd:
+        q = rng.random((n, n)) < 0.5
+        q = q | q.T | np.eye(n, dtype=bool)
+        Sd.append(np.where(q, x, 0))
+    for x in s.Su:
+        p = rng.random((n, n)) < 0.4
+        Su.append(np.where(p, x, 0))
+        Sl.append(np.where(p, x, 0).conj().T.copy())
+    return BlockStructure(Hd=s.Hd, Hu=s.Hu, Hl=s.Hl, contacts=s.contacts, Sd=Sd, Su=Su, Sl=Sl)
+
+
+def test_overlap_pattern_convergence(gpu, orc):
+    """Non-orthogonal basis: the convergence test runs over the S sparsity
+    pattern (diagonal and off-diagonal blocks), k-dependent blocks in two slots."""
+    hip, eng = gpu
+    slab = wl.KSlab.make(n=40, nbl=4, seed=21, with_overlap=True)
+    structs = [_sparsify(slab.at_k(0.4, -0.3), 1), _sparsify(slab.at_k(-1.1, 2.0), 2)]
+    ds = hip.DeviceStructure(structs)
+    E = np.array([-0.8, 0.1, 0.7])
+    EE = np.concatenate([E, E])
+    sl = np.repeat([0, 1], 3)
+    cp = np.linspace(0.02, 0.08, structs[0].ndev)
+    r = hip.solve_scba_mw(eng, ds, EE, cp, 8, 1e-9, [0.9, 0.1, 0.0], 3, slot=sl, delta=1e-5,
+                          want_diag=True)
+    for j, s in enumerate(structs):
+        q = _ref(orc, s, E, 1e-5, cp, 8, 1e-9, [0.9, 0.1, 0.0], 3)
+        part = {k: (v[3 * j:3 * j + 3] if v is not None else None) for k, v in r.items()}
+        _check(part, q)
+
+
+def test_per_item_occupations_and_reference(gpu, orc):
+    """Per-item occupation vectors and a reference contact (ref = 2)."""
+    hip, eng = gpu
+    s = wl.ribbon(10, 6, disorder=0.5, seed=23)
+    E = np.linspace(-2.0, 2.0, 7)
+    frm = np.stack([[0.5 + 0.05 * i, 0.2, 0.0] for i in range(len(E))])
+    cp = np.full(s.ndev, 0.07)
+    ds = hip.DeviceStructure(s)
+    r = hip.solve_scba_mw(eng, ds, E, cp, 6, 1e-8, frm, 2, delta=1e-5, want_diag=True)
+    q = _ref(orc, s, E, 1e-5, cp, 6, 1e-8, frm, 2)
+    _check(r, q)
+
+
+def test_scba_batch_invariance(gpu):
+    """Items leave the batch independently; grouping does not change results."""
+    hip, eng = gpu
+    s = wl.ribbon(16, 7, disorder=0.4, seed=29)
+    E = np.linspace(-2.5, 2.5, 19)
+    cp = np.full(s.ndev, 0.06)
+    ds = hip.DeviceStructure(s)
+    r0 = hip.solve_scba_mw(eng, ds, E, cp, 20, 1e-9, [1.0, 0.0, 0.0], 3, delta=1e-5)
+    for mb in (1, 4):
+        r1 = hip.solve_scba_mw(eng, ds, E, cp, 20, 1e-9, [1.0, 0.0, 0.0], 3, delta=1e-5,
+                               max_batch=mb)
+        np.testing.assert_array_equal(r0["curr"], r1["curr"])
+        np.testing.assert_array_equal(r0["iters"], r1["iters"])

@@ -125,3 +125,36 @@ def test_reciprocity(gpu):
     T12, _, _ = hip.solve_transmission(eng, ds, E, delta=1e-5, ni=1, nf=2)
     T21, _, _ = hip.solve_transmission(eng, ds, E, delta=1e-5, ni=2, nf=1)
     assert np.abs(T12 - T21).max() <= 1e-11 * max(T12.max(), 1.0)
+
+
+def test_shared_decimation_matches_separate(gpu):
+    """Mirror-image contacts share one Sancho-Rubio recursion; results and each
+    contact's exit iteration must match the one-recursion-per-contact run."""
+    hip, eng = gpu
+    slab = wl.KSlab.make(n=96, nbl=4, seed=8)
+    s = slab.at_k(0.7, -2.1)
+    E = np.linspace(-1.8, 1.8, 11)
+    ds = hip.DeviceStructure(s)
+    T1, n1, s1 = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    T2, n2, s2 = hip.solve_transmission(eng, ds, E, delta=1e-5, flags=hip.NO_SHARED_DECIMATION)
+    assert (s1 == 0).all() and (s2 == 0).all()
+    assert np.array_equal(n1, n2)
+    # the mirror lead applies its two bulk updates in the opposite order; that
+    # rounding difference is amplified by the eta = 1e-5 conditioning
+    _compare(T1, T2)
+
+
+def test_non_mirror_contacts(gpu):
+    """Contacts that are not mirror images take the per-contact recursion."""
+    hip, eng = gpu
+    from libnegf_amd.structure import BlockStructure, Contact
+    base = wl.ribbon(24, 6, disorder=0.4, seed=12)
+    c2 = base.contacts[1]
+    c2 = Contact(cblk=c2.cblk, H00=c2.H00 + np.diag(np.full(24, 0.3)), H01=c2.H01, HMC=c2.HMC)
+    s = BlockStructure(Hd=base.Hd, Hu=base.Hu, Hl=base.Hl, contacts=[base.contacts[0], c2])
+    E = np.linspace(-2.5, 2.5, 7)
+    ds = hip.DeviceStructure(s)
+    T, ncyc, st = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    Tref, nref = _oracle(s, E, 1e-5)
+    _compare(T, Tref)
+    assert (ncyc == nref).all()
