@@ -36,8 +36,39 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-items", type=int, default=4,
                     help="items timed one at a time (unbatched FP64 reference mode)")
+    ap.add_argument("--no-profile", action="store_true",
+                    help="no per-launch HIP events in the timed region (no roofline field)")
+    ap.add_argument("--copy-calib", action="store_true",
+                    help="only run a plain 256 MiB device copy (PMC byte-counter calibration)")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
+
+
+PEAK_FP64_TFLOPS = 78.6   # MI355X dense FP64 matrix
+PEAK_HBM_GBS = 8000.0     # MI355X HBM3E
+
+
+def copy_calibration():
+    """A plain copy of known size for scaling FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = 256 << 20
+    src = torch.ones(n // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}), flush=True)
+
+
+def traffic_from_profiles(kernel_tag: str):
+    """HBM bytes per launch of the dominant kernel from the committed PMC summary
+    (profiles/pmc_<kernel>_*.txt, line 'traffic_bytes_per_launch: <x>'), or None."""
+    import glob
+    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", f"pmc_{kernel_tag}_*.txt"))):
+        for line in open(f):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1]), os.path.basename(f)
+    return None, None
 
 
 def log(*a):
@@ -54,6 +85,9 @@ def main():
     from libnegf_amd import hip, workloads as wl
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if a.copy_calib:
+        copy_calibration()
+        return
     prec = hip.FP64 if a.precision == "fp64" else hip.MIXED
 
     # ------------------------------------------------------------ workload ----
@@ -109,6 +143,9 @@ def main():
         log(f"warmup step {i}: {time.time() - t0:.3f}s")
         results[i] = (dT.cpu().numpy().copy(), dN.cpu().numpy().copy(), dSt.cpu().numpy().copy())
     torch.cuda.synchronize()
+    if not a.no_profile:
+        hip.profile_clear()
+        hip.profile_enable(True)
     t0 = time.perf_counter()
     for i in range(a.warmup, nsteps):
         step(i)
@@ -117,6 +154,33 @@ def main():
             log(f"step {i}: {time.perf_counter() - t0:.3f}s cumulative")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
+    roofline, kern = None, None
+    if not a.no_profile:
+        hip.profile_enable(False)
+        kern = {hip.PT_NAMES[t]: hip.profile_read(t) for t in hip.PT_NAMES}
+        hip.profile_clear()
+        for k, v in kern.items():
+            log(f"  {k:22s} launches {v['launches']:6d}  total {v['ms']:9.2f} ms  "
+                f"avg {v['ms'] / max(v['launches'], 1) * 1e3:8.1f} us  "
+                f"{(v['flops'] / (v['ms'] * 1e-3) / 1e12) if v['flops'] > 0 and v['ms'] > 0 else float('nan'):.2f} TF/s")
+        # dominant kernel = largest summed duration among launches with counted flops
+        counted = {k: v for k, v in kern.items() if v["flops"] > 0 and v["ms"] > 0}
+        if counted:
+            name = max(counted, key=lambda k: counted[k]["ms"])
+            v = counted[name]
+            avg_s = v["ms"] * 1e-3 / v["launches"]
+            fl = v["flops"] / v["launches"]
+            achieved = fl / avg_s / 1e12
+            tag = name.replace("<double, ", "_").replace(">", "")
+            traffic, tsrc = traffic_from_profiles(tag)
+            roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
+                        "traffic": traffic, "kernel": name,
+                        "flops_per_launch": fl, "avg_launch_us": avg_s * 1e6,
+                        "launches": v["launches"],
+                        "algorithmic_bytes_per_launch": v["bytes"] / v["launches"],
+                        "share_of_step_time": v["ms"] * 1e-3 / elapsed,
+                        "traffic_source": tsrc}
     last = (dT.cpu().numpy().copy(), dN.cpu().numpy().reshape(per_step, ncont), dSt.cpu().numpy())
     n_items = a.steps * per_step
     value = n_items / elapsed
@@ -179,8 +243,10 @@ def main():
         "config": workload,
         "precision": a.precision,
         "fp64_plain_points_per_s": plain,
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
+        "kernel_ms": None if kern is None else {k: round(v["ms"], 3) for k, v in kern.items()},
     }
     print(json.dumps(out), flush=True)
 

@@ -231,6 +231,17 @@ int hipnegf_engine_sync(hipnegf_engine e);
 /* stream the engine launches on (a hipStream_t) */
 void *hipnegf_engine_stream(hipnegf_engine e);
 
+/* Launch timing (no counterpart in the reference; measurement hook for
+ * benchmarks).  While enabled, every tagged launch records a HIP event pair on
+ * its stream.  Tags: 0 masked GEMMs (decimation, SCBA), 1 RGF-sweep GEMMs,
+ * 2 blocked-inverse trailing updates, 3 inverse panels, 4 small LDS inverses.
+ * hipnegf_profile_read waits for the events of `tag` and returns the summed
+ * duration (ms), algorithmic flops (-1 if any launch is not counted) and
+ * algorithmic bytes, and the launch count. */
+int hipnegf_profile_enable(int on);
+int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long long *launches);
+int hipnegf_profile_clear(void);
+
 #ifdef __cplusplus
 }
 #endif

@@ -30,9 +30,9 @@ CFLAGS = [
     f"-I{INCLUDE}",
 ]
 
-SOURCES = ["hn_gemm.hip", "hn_inv.hip", "hn_kernels.hip", "hn_engine.hip", "hn_capi.hip",
+SOURCES = ["hn_prof.hip", "hn_gemm.hip", "hn_inv.hip", "hn_kernels.hip", "hn_engine.hip", "hn_capi.hip",
            "hn_api.hip"]
-HEADERS = ["hn_common.h", "hn_gemm.h", "hn_inv.h", "hn_kernels.h", "hn_engine.h"]
+HEADERS = ["hn_common.h", "hn_prof.h", "hn_gemm.h", "hn_inv.h", "hn_kernels.h", "hn_engine.h"]
 
 
 def _newer(src: str, dst: str, deps) -> bool:

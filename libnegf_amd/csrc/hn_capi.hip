@@ -13,6 +13,7 @@
 #include "hn_gemm.h"
 #include "hn_inv.h"
 #include "hn_kernels.h"
+#include "hn_prof.h"
 
 struct hipnegf_ctx {
   std::unique_ptr<hn::Engine> eng;
@@ -602,6 +603,20 @@ int hipnegf_zinverse_batched(hipnegf_engine e, int n, void *A, long long sA, voi
 int hipnegf_cinverse_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv,
                              long long sAinv, int batch, int *dstatus) {
   return guard([&] { return inv_batched<float>(e, n, A, sA, Ainv, sAinv, batch, dstatus); });
+}
+
+int hipnegf_profile_enable(int on) {
+  return guard([&] { hn::profiler().enable(on != 0); return 0; });
+}
+int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long long *launches) {
+  return guard([&] {
+    if (tag < 0 || tag >= hn::PT_COUNT) throw std::runtime_error("bad profile tag");
+    hn::profiler().summary(tag, ms, flops, bytes, launches);
+    return 0;
+  });
+}
+int hipnegf_profile_clear(void) {
+  return guard([&] { hn::profiler().clear(); return 0; });
 }
 
 }  // extern "C"

@@ -25,6 +25,7 @@ struct GemmArgs {
   const int *active2 = nullptr;  // optional separate mask of the second output C2
   int skip_c0 = 0, skip_c1 = 0;
   int batch = 1;
+  int tag = -1;  // profile tag (ProfTag); -1: masked if `active` is set, else sweep
 };
 
 void gemm_z(const GemmArgs<double> &a, hipStream_t s);  // complex FP64

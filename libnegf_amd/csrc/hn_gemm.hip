@@ -16,7 +16,10 @@
 // consecutive workgroup ids round-robin over the 8 XCDs, so the linear id is
 // remapped such that all tiles of one item land on the same XCD and share its
 // L2 (the A and B panels of an item are re-read by every tile).
+#include <algorithm>
+
 #include "hn_gemm.h"
+#include "hn_prof.h"
 
 namespace hn {
 
@@ -61,7 +64,8 @@ template <typename R> struct Smem {
   R Bi[BK][BN + PAD];
 };
 
-template <typename R>
+// USE only separates the uses (masked, sweep, inverse update) in profiles.
+template <typename R, int USE>
 __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int tiles_n,
                                              int total, int total_pad) {
   typedef typename CplxT<R>::t CT;
@@ -243,8 +247,26 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   long long total = (long long)tiles_m * tiles_n * a.batch;
   if (total > (1LL << 30)) throw std::runtime_error("gemm: grid too large");
   int total_pad = (int)((total + 7) / 8 * 8);
-  hipLaunchKernelGGL(k_gemm<R>, dim3(total_pad), dim3(NT), 0, s, a, tiles_m, tiles_n,
-                     (int)total, total_pad);
+  const int use = a.tag >= 0 ? a.tag : (a.active ? PT_GEMM_MASKED : PT_GEMM_SWEEP);
+  // algorithmic work: 8 real flops per complex multiply-add over the columns written
+  const int skipped = std::max(0, std::min(a.skip_c1, a.n) - std::max(a.skip_c0, 0));
+  const double ncol = (double)(a.n - skipped);
+  const double flops = (a.active || a.active2) ? -1.0 : 8.0 * a.m * ncol * a.k * a.batch;
+  const double cb = (double)sizeof(typename GemmArgs<R>::CT);
+  const bool rc = a.beta.x != 0.0 || a.beta.y != 0.0;
+  const double bytes = cb * ((a.sA ? (double)a.batch : 1.0) * a.m * a.k +
+                             (a.sB ? (double)a.batch : 1.0) * a.k * a.n +
+                             (rc ? 2.0 : 1.0) * a.batch * a.m * ncol);
+  ProfScope ps(use, flops, bytes, s);
+  if (use == PT_GEMM_SWEEP)
+    hipLaunchKernelGGL((k_gemm<R, PT_GEMM_SWEEP>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+                       tiles_n, (int)total, total_pad);
+  else if (use == PT_GEMM_INV)
+    hipLaunchKernelGGL((k_gemm<R, PT_GEMM_INV>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+                       tiles_n, (int)total, total_pad);
+  else
+    hipLaunchKernelGGL((k_gemm<R, PT_GEMM_MASKED>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+                       tiles_n, (int)total, total_pad);
   HN_CHECK(hipGetLastError());
 }
 

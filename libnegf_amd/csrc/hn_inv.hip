@@ -25,6 +25,7 @@
 // row of logical column c.
 #include "hn_gemm.h"
 #include "hn_inv.h"
+#include "hn_prof.h"
 
 namespace hn {
 
@@ -395,6 +396,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
   if (n <= lds_max_n<R>()) {
     size_t lds = lds_bytes_small<R>(n);
     allow_lds(k_inv_lds<R>, lds);
+    ProfScope ps(PT_INV_SMALL, -1.0, 0.0, s);
     hipLaunchKernelGGL(k_inv_lds<R>, dim3(batch), dim3(NT), lds, s, n, A, sA, Ainv, sAinv,
                        active, status);
     HN_CHECK(hipGetLastError());
@@ -416,15 +418,18 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
     const int sw = std::min(kInvSuper, n - s0);
     for (int c0 = s0; c0 < s0 + sw; c0 += NB) {
       const int nbp = std::min(NB, s0 + sw - c0);
-      if (n <= NT && NB == 32)
-        hipLaunchKernelGGL((k_inv_panel_reg<R, 32, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                           c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
-      else if (n <= NT)
-        hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                           c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
-      else
-        hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 2>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                           c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+      {
+        ProfScope ps(PT_INV_PANEL, -1.0, 0.0, s);
+        if (n <= NT && NB == 32)
+          hipLaunchKernelGGL((k_inv_panel_reg<R, 32, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
+                             c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+        else if (n <= NT)
+          hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
+                             c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+        else
+          hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 2>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
+                             c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+      }
       HN_CHECK(hipGetLastError());
       if (sw > nbp) {  // inner update: the other columns of the super-panel
         GemmArgs<R> g;
@@ -435,7 +440,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
         g.C = A + (size_t)s0 * n; g.sC = sA; g.ldc = n;
         g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
         g.active = active; g.skip_c0 = c0 - s0; g.skip_c1 = c0 - s0 + nbp;
-        g.batch = batch;
+        g.batch = batch; g.tag = PT_GEMM_INV;
         if constexpr (sizeof(R) == 8) gemm_z(g, s);
         else gemm_c(g, s);
       }
@@ -451,7 +456,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
       g.C = A; g.sC = sA; g.ldc = n;
       g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
       g.active = active; g.skip_c0 = s0; g.skip_c1 = s0 + sw;
-      g.batch = batch;
+      g.batch = batch; g.tag = PT_GEMM_INV;
       if constexpr (sizeof(R) == 8) gemm_z(g, s);
       else gemm_c(g, s);
     }

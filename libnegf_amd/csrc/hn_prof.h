@@ -1,0 +1,57 @@
+// hn_prof.h -- optional per-launch timing of the engine's kernels.
+//
+// When enabled, a tagged launch site records a HIP event pair on the stream it
+// launches on, with the launch's algorithmic flop count (negative: not
+// counted, e.g. launches that skip converged items) and algorithmic bytes.
+// Summaries per tag give the average launch duration and the achieved rate.
+#pragma once
+#include <mutex>
+#include <vector>
+
+#include "hn_common.h"
+
+namespace hn {
+
+enum ProfTag : int {
+  PT_GEMM_MASKED = 0,  // batched GEMMs with per-item activity masks (decimation, SCBA)
+  PT_GEMM_SWEEP = 1,   // batched GEMMs of the RGF sweeps and traces
+  PT_GEMM_INV = 2,     // trailing updates of the blocked inverse
+  PT_INV_PANEL = 3,    // panel factorisation of the blocked inverse
+  PT_INV_SMALL = 4,    // LDS-resident small-block inverse
+  PT_COUNT = 5
+};
+
+class Profiler {
+ public:
+  void enable(bool on);
+  bool on() const { return on_; }
+  int begin(int tag, double flops, double bytes, hipStream_t s);
+  void end(int token, hipStream_t s);
+  // Waits for the recorded events; sums over launches of `tag`.
+  void summary(int tag, double *ms, double *flops, double *bytes, long long *launches);
+  void clear();
+
+ private:
+  struct Rec { int tag; double flops, bytes; hipEvent_t a, b; };
+  bool on_ = false;
+  std::mutex mu_;
+  std::vector<Rec> recs_;
+  std::vector<hipEvent_t> pool_;
+  hipEvent_t get();
+};
+
+Profiler &profiler();
+
+// RAII helper for one launch
+struct ProfScope {
+  int tok = -1;
+  hipStream_t s;
+  ProfScope(int tag, double flops, double bytes, hipStream_t st) : s(st) {
+    if (profiler().on()) tok = profiler().begin(tag, flops, bytes, st);
+  }
+  ~ProfScope() {
+    if (tok >= 0) profiler().end(tok, s);
+  }
+};
+
+}  // namespace hn

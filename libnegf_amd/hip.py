@@ -85,6 +85,9 @@ def load():
     L.hipnegf_Zdecimation.argtypes = [vp, vp, vp, vp, vp, vp, C.c_size_t, i, C.POINTER(C.c_int), d]
     L.hipnegf_Cdecimation.argtypes = [vp, vp, vp, vp, vp, vp, C.c_size_t, i, C.POINTER(C.c_int),
                                       C.c_float]
+    L.hipnegf_profile_enable.argtypes = [i]
+    L.hipnegf_profile_read.argtypes = [i, C.POINTER(d), C.POINTER(d), C.POINTER(d),
+                                       C.POINTER(ll)]
     _LIB = L
     return L
 
@@ -94,6 +97,27 @@ def check(rc: int):
         msg = load().hipnegf_last_error().decode()
         raise HipNegfError(msg or f"hipnegf call failed (rc={rc})")
     return rc
+
+
+# launch-timing tags (hipnegf_profile_*)
+PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV, PT_INV_PANEL, PT_INV_SMALL = range(5)
+PT_NAMES = {PT_GEMM_MASKED: "k_gemm<double, 0>", PT_GEMM_SWEEP: "k_gemm<double, 1>",
+            PT_GEMM_INV: "k_gemm<double, 2>", PT_INV_PANEL: "k_inv_panel_reg",
+            PT_INV_SMALL: "k_inv_lds"}
+
+
+def profile_enable(on: bool):
+    check(load().hipnegf_profile_enable(1 if on else 0))
+
+
+def profile_clear():
+    check(load().hipnegf_profile_clear())
+
+
+def profile_read(tag: int) -> dict:
+    ms, fl, by, n = C.c_double(), C.c_double(), C.c_double(), C.c_longlong()
+    check(load().hipnegf_profile_read(tag, C.byref(ms), C.byref(fl), C.byref(by), C.byref(n)))
+    return dict(ms=ms.value, flops=fl.value, bytes=by.value, launches=n.value)
 
 
 def _ptr(a: np.ndarray):
