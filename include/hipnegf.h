@@ -198,6 +198,17 @@ int hipnegf_solve_transmission_device(hipnegf_engine e, hipnegf_struct st, int n
                                       const hipnegf_opts *o, double *dT, int *dncyc,
                                       int *dstatus);
 
+/* Several contact pairs in one solve (host arrays): the contacts' self-energies
+ * and the gsmr blocks are computed once per item and shared by all pairs, as in
+ * the multi-terminal branch of calculate_transmissions (iterative.F90:1716-1766).
+ * ni[npairs], nf[npairs] 1-based; T[nitems*npairs] item-major.  One pair with
+ * two contacts and the lower one on block 1 takes the 2-contact fast path
+ * (iterative.F90:1704-1715); o->ni / o->nf are ignored. */
+int hipnegf_solve_transmission_pairs(hipnegf_engine e, hipnegf_struct st, int nitems,
+                                     const double *E, const int *slot, const hipnegf_opts *o,
+                                     int npairs, const int *ni, const int *nf, double *T,
+                                     int *ncyc, int *status);
+
 /* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):
  * calculate_elastic_scba (iterative.F90:289-363) with the ElPhonDephD model
  * (elphdd.F90; coupling per device orbital, squared internally), then

@@ -313,27 +313,22 @@ void compute_landauer(Session &s) {
   auto st = build_struct(s);
   hn::Engine &eng = engine(s);
   const int n = s.npoints;
-  DevBuf dE(sizeof(double) * n), dsl(sizeof(int) * n), dT(sizeof(double) * n),
-      dN(sizeof(int) * n * s.ncont), dS(sizeof(int) * n);
+  DevBuf dE(sizeof(double) * n), dsl(sizeof(int) * n), dN(sizeof(int) * n * s.ncont),
+      dS(sizeof(int) * n);
   HN_CHECK(hipMemcpy(dE.p, s.en.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
-  std::vector<double> T(n);
-  std::vector<int> stt(n);
-  for (int ip = 0; ip < s.npairs; ++ip) {
-    hn::SolveOpts o = opts_of(s);
-    o.ni = ni[ip];
-    o.nf = nf[ip];
-    eng.transmission(*st, n, dE.as<double>(), dsl.as<int>(), o, dT.as<double>(), dN.as<int>(),
-                     dS.as<int>());
-    HN_CHECK(hipMemcpy(T.data(), dT.p, sizeof(double) * n, hipMemcpyDeviceToHost));
-    HN_CHECK(hipMemcpy(stt.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    if (ip == 0)
-      HN_CHECK(hipMemcpy(s.item_ncyc.data(), dN.p, sizeof(int) * n * s.ncont, hipMemcpyDeviceToHost));
-    for (int i = 0; i < n; ++i) {
-      s.tunn[(size_t)ip * n + i] = T[i] * s.p.kwght;  // tunn_mat(i,:) = T kwght
-      s.item_status[i] |= stt[i];
-    }
-  }
+  DevBuf dTp(sizeof(double) * n * s.npairs);
+  hn::SolveOpts o = opts_of(s);
+  // every lead pair in one batched solve: contacts and gsmr blocks are shared
+  eng.transmission_pairs(*st, n, dE.as<double>(), dsl.as<int>(), o, s.npairs, ni.data(), nf.data(),
+                         dTp.as<double>(), dN.as<int>(), dS.as<int>());
+  std::vector<double> T((size_t)n * s.npairs);
+  HN_CHECK(hipMemcpy(T.data(), dTp.p, sizeof(double) * n * s.npairs, hipMemcpyDeviceToHost));
+  HN_CHECK(hipMemcpy(s.item_status.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HN_CHECK(hipMemcpy(s.item_ncyc.data(), dN.p, sizeof(int) * n * s.ncont, hipMemcpyDeviceToHost));
+  for (int ip = 0; ip < s.npairs; ++ip)
+    for (int i = 0; i < n; ++i)
+      s.tunn[(size_t)ip * n + i] = T[(size_t)i * s.npairs + ip] * s.p.kwght;  // tunn_mat(i,ip) = T kwght
 }
 
 // compute_dephasing_transmission: Meir-Wingreen with occupations (1, 0),

@@ -503,6 +503,34 @@ int hipnegf_solve_transmission(hipnegf_engine e, hipnegf_struct st, int nitems,
   });
 }
 
+int hipnegf_solve_transmission_pairs(hipnegf_engine e, hipnegf_struct st, int nitems,
+                                     const double *E, const int *slot, const hipnegf_opts *o,
+                                     int npairs, const int *ni, const int *nf, double *T,
+                                     int *ncyc, int *status) {
+  return guard([&] {
+    if (nitems <= 0) return 0;
+    if (npairs < 1) throw std::runtime_error("npairs < 1");
+    for (int i = 0; i < nitems; ++i)
+      if (slot[i] < 0 || slot[i] >= st->st->nslots) throw std::runtime_error("item slot out of range");
+    const int nc = std::max(st->st->ncont, 1);
+    void *p[5] = {};
+    struct F { void **p; ~F() { for (int i = 0; i < 5; ++i) if (p[i]) (void)hipFree(p[i]); } } fr{p};
+    HN_CHECK(hipMalloc(&p[0], sizeof(double) * nitems));
+    HN_CHECK(hipMalloc(&p[1], sizeof(double) * nitems * npairs));
+    HN_CHECK(hipMalloc(&p[2], sizeof(int) * nitems));
+    HN_CHECK(hipMalloc(&p[3], sizeof(int) * nitems * nc));
+    HN_CHECK(hipMalloc(&p[4], sizeof(int) * nitems));
+    HN_CHECK(hipMemcpy(p[0], E, sizeof(double) * nitems, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(p[2], slot, sizeof(int) * nitems, hipMemcpyHostToDevice));
+    e->eng->transmission_pairs(*st->st, nitems, (const double *)p[0], (const int *)p[2], to_opts(o),
+                               npairs, ni, nf, (double *)p[1], (int *)p[3], (int *)p[4]);
+    HN_CHECK(hipMemcpy(T, p[1], sizeof(double) * nitems * npairs, hipMemcpyDeviceToHost));
+    if (ncyc) HN_CHECK(hipMemcpy(ncyc, p[3], sizeof(int) * nitems * nc, hipMemcpyDeviceToHost));
+    if (status) HN_CHECK(hipMemcpy(status, p[4], sizeof(int) * nitems, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
 int hipnegf_solve_scba_mw(hipnegf_engine e, hipnegf_struct st, int nitems, const double *E,
                           const int *slot, const hipnegf_opts *o, const double *coupling,
                           int niter, double tol, const double *frm, int frm_per_item, int ref,

@@ -94,9 +94,16 @@ class Engine {
   hipStream_t stream() const { return stream_; }
 
   // Coherent transmission for nitems (E, slot) points.  All pointers device.
-  // T[nitems], ncyc[nitems*ncont], status[nitems].
+  // T[nitems], ncyc[nitems*ncont], status[nitems]; the pair is (o.ni, o.nf).
   void transmission(DevStruct &st, int nitems, const double *dE, const int *dslot,
                     const SolveOpts &o, double *dT, int *dncyc, int *dstatus);
+  // Same for npairs contact pairs (host arrays ni[], nf[], 1-based):
+  // T[nitems*npairs] item-major.  One pair with two contacts and the lower
+  // contact on block 1 takes the fast path, everything else the multi-terminal
+  // column sweep (calculate_transmissions, iterative.F90:1669-1780).
+  void transmission_pairs(DevStruct &st, int nitems, const double *dE, const int *dslot,
+                          const SolveOpts &o, int npairs, const int *ni, const int *nf,
+                          double *dT, int *dncyc, int *dstatus);
 
   // Elastic-dephasing SCBA + Meir-Wingreen current per item.
   // curr[nitems*ncont], iters[nitems], err[nitems], status[nitems];
@@ -133,7 +140,11 @@ class Engine {
   void inverse(int n, double2 *A, long long sA, double2 *Ainv, long long sAinv, int batch,
                const int *active, int *status, int precision, const SolveOpts &o);
   void t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
-               const SolveOpts &o, double *dT, int *dncyc, int *dstatus);
+               const SolveOpts &o, int npairs, const int *ni, const int *nf, double *dT,
+               int *dncyc, int *dstatus);
+  void t_multi(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
+               int npairs, const int *ni, const int *nf, const double2 *Sig, long long sSig,
+               double *dT, int *dstatus);
   void scba_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                   const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
                   double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag);

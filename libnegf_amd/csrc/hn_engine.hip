@@ -148,12 +148,24 @@ inline GemmArgs<double> G(int m, int n, int k, const double2 *A, long long sA, i
 }
 
 // per-item bytes of the transmission path at uniform order n
-size_t t_item_bytes(int n, int ncont) {
-  size_t b2 = (size_t)n * n * sizeof(double2);
-  size_t contact = (size_t)ncont * (10 * b2 + 3 * b2 + b2);  // decimation + couplings + Sigma
-  size_t sweep = 6 * b2;
-  size_t inv = inv_work_bytes(n, std::max(ncont, 1), 8);
-  return contact + sweep + inv + 64 * sizeof(int);
+// Workspace blocks (n_max^2 complex) per item of one transmission batch:
+// Sigma per contact, the contact stage's temporaries (released before the
+// sweep), then the sweep: 6 for the fast path, or all g_i, G_ii and G_i+1,i
+// blocks down to the deepest contact block plus 9 temporaries.
+size_t t_item_blocks(const DevStruct &st, bool fast, int ntmax) {
+  const size_t nc = (size_t)std::max(st.ncont, 1);
+  const size_t sweep = fast ? 6 : (size_t)st.nbl + 2 * (size_t)ntmax + 9;
+  return nc + std::max<size_t>(10 + 2 * nc, sweep);
+}
+size_t t_item_bytes(const DevStruct &st, bool fast, int ntmax) {
+  const int n = st.nmax();
+  const size_t b2 = (size_t)n * n * sizeof(double2);
+  return t_item_blocks(st, fast, ntmax) * b2 + inv_work_bytes(n, std::max(st.ncont, 1), 8) +
+         64 * sizeof(int) + 4096;
+}
+bool fast_path(const DevStruct &st, int npairs, const int *ni, const int *nf) {
+  if (st.ncont != 2 || npairs != 1) return false;
+  return std::min(st.cblk[ni[0] - 1], st.cblk[nf[0] - 1]) == 1;
 }
 
 }  // namespace
@@ -412,7 +424,8 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
 // Coherent transmission for one batch.
 // ---------------------------------------------------------------------------
 void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
-                     const SolveOpts &o, double *dT, int *dncyc, int *dstatus) {
+                     const SolveOpts &o, int npairs, const int *pni, const int *pnf, double *dT,
+                     int *dncyc, int *dstatus) {
   const int nbl = st.nbl, ncont = st.ncont;
   const int nm = st.nmax();
   const long long s2 = (long long)nm * nm;
@@ -424,88 +437,88 @@ void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
   fill_int(status_c, B * std::max(ncont, 1), 0, stream_);
   fill_int(dstatus, B, 0, stream_);
   if (ncont < 2) {  // calculate_transmissions: ncont == 1 -> T = 0 (iterative.F90:1685-1688)
-    fill_double(dT, B, 0.0, stream_);
+    fill_double(dT, B * npairs, 0.0, stream_);
     return;
   }
   contacts(st, B, dE, dslot, o, Sig, s2, ncyc_c, status_c);
+  if (!fast_path(st, npairs, pni, pnf)) {
+    t_multi(st, B, dE, dslot, o, npairs, pni, pnf, Sig, s2, dT, dstatus);
+  } else {
+    // which contacts sit on which block
+    std::vector<std::vector<int>> on_blk(nbl);
+    for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
+    auto sig_of = [&](int c) { return Sig + (long long)c * B * s2; };
 
-  // which contacts sit on which block
-  std::vector<std::vector<int>> on_blk(nbl);
-  for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
-  auto sig_of = [&](int c) { return Sig + (long long)c * B * s2; };
+    double2 *D = arena_.take<double2>(s2 * B), *U = arena_.take<double2>(s2 * B);
+    double2 *L = arena_.take<double2>(s2 * B), *g = arena_.take<double2>(s2 * B);
+    double2 *W1 = arena_.take<double2>(s2 * B), *Gs = arena_.take<double2>(s2 * B);
+    const bool sd = (st.smode & S_DIAG) != 0, so = (st.smode & S_OFFDIAG) != 0;
 
-  double2 *D = arena_.take<double2>(s2 * B), *U = arena_.take<double2>(s2 * B);
-  double2 *L = arena_.take<double2>(s2 * B), *g = arena_.take<double2>(s2 * B);
-  double2 *W1 = arena_.take<double2>(s2 * B), *Gs = arena_.take<double2>(s2 * B);
-  const bool sd = (st.smode & S_DIAG) != 0, so = (st.smode & S_OFFDIAG) != 0;
+    auto form_diag = [&](int i, double2 *out) {
+      FormArgs f;
+      f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+      f.rows = st.bsz[i]; f.cols = st.bsz[i];
+      f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+      f.out = out; f.s_out = s2;
+      const auto &cs = on_blk[i];
+      if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+      if (cs.size() >= 1) { f.sig = sig_of(cs[0]); f.s_sig = s2; }
+      if (cs.size() == 2) { f.sig2 = sig_of(cs[1]); f.s_sig2 = s2; }
+      form_block(f, stream_);
+    };
+    auto form_off = [&](int i) {  // U = ESH(i,i+1), L = ESH(i+1,i)
+      FormArgs f;
+      f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+      f.rows = st.bsz[i]; f.cols = st.bsz[i + 1];
+      f.H = st.ref(BK_HU, i); f.S = so ? st.ref(BK_SU, i) : BlkRef(); f.s_identity = 0;
+      f.out = U; f.s_out = s2;
+      form_block(f, stream_);
+      f.rows = st.bsz[i + 1]; f.cols = st.bsz[i];
+      f.H = st.ref(BK_HL, i); f.S = so ? st.ref(BK_SL, i) : BlkRef();
+      f.out = L;
+      form_block(f, stream_);
+    };
 
-  auto form_diag = [&](int i, double2 *out) {
-    FormArgs f;
-    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
-    f.rows = st.bsz[i]; f.cols = st.bsz[i];
-    f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
-    f.out = out; f.s_out = s2;
-    const auto &cs = on_blk[i];
-    if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
-    if (cs.size() >= 1) { f.sig = sig_of(cs[0]); f.s_sig = s2; }
-    if (cs.size() == 2) { f.sig2 = sig_of(cs[1]); f.s_sig2 = s2; }
-    form_block(f, stream_);
-  };
-  auto form_off = [&](int i) {  // U = ESH(i,i+1), L = ESH(i+1,i)
-    FormArgs f;
-    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
-    f.rows = st.bsz[i]; f.cols = st.bsz[i + 1];
-    f.H = st.ref(BK_HU, i); f.S = so ? st.ref(BK_SU, i) : BlkRef(); f.s_identity = 0;
-    f.out = U; f.s_out = s2;
-    form_block(f, stream_);
-    f.rows = st.bsz[i + 1]; f.cols = st.bsz[i];
-    f.H = st.ref(BK_HL, i); f.S = so ? st.ref(BK_SL, i) : BlkRef();
-    f.out = L;
-    form_block(f, stream_);
-  };
+    const int ni = pni[0], nf = pnf[0];
+    const int cbi = st.cblk[ni - 1], cbf = st.cblk[nf - 1];
 
-  const int ni = o.ni, nf = o.nf;
-  const int cbi = st.cblk[ni - 1], cbf = st.cblk[nf - 1];
-  const int nt = std::min(cbi, cbf);
-  if (!(ncont == 2 && nt == 1))
-    throw std::runtime_error("only the 2-contact fast path (min cblk == 1) is available");
-
-  // ---- calculate_gsmr_blocks(nbl -> 2), iterative_cpu.F90:51-124 ----
-  if (nbl > 1) {
-    form_diag(nbl - 1, D);
-    const int n = st.bsz[nbl - 1];
-    inverse(n, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
-    for (int i = nbl - 2; i >= 1; --i) {
-      const int ni_ = st.bsz[i], nj = st.bsz[i + 1];
-      form_off(i);
-      form_diag(i, D);
-      // work1 = -ESH(i,i+1) g(i+1) ; M = ESH(i,i) + work1 ESH(i+1,i)
-      gemm_z(G(ni_, nj, nj, U, s2, ni_, OP_N, g, s2, nj, OP_N, W1, s2, ni_, Z(-1), Z(0), B, nullptr), stream_);
-      gemm_z(G(ni_, ni_, nj, W1, s2, ni_, OP_N, L, s2, nj, OP_N, D, s2, ni_, Z(1), Z(1), B, nullptr), stream_);
-      inverse(ni_, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
-    }
-  }
-  // ---- Gr(1,1), iterative_cpu.F90:172-198 ----
-  {
-    const int n0 = st.bsz[0];
-    form_diag(0, D);
+    // ---- calculate_gsmr_blocks(nbl -> 2), iterative_cpu.F90:51-124 ----
     if (nbl > 1) {
-      const int n1 = st.bsz[1];
-      form_off(0);
-      gemm_z(G(n0, n1, n1, U, s2, n0, OP_N, g, s2, n1, OP_N, W1, s2, n0, Z(-1), Z(0), B, nullptr), stream_);
-      gemm_z(G(n0, n0, n1, W1, s2, n0, OP_N, L, s2, n1, OP_N, D, s2, n0, Z(1), Z(1), B, nullptr), stream_);
+      form_diag(nbl - 1, D);
+      const int n = st.bsz[nbl - 1];
+      inverse(n, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
+      for (int i = nbl - 2; i >= 1; --i) {
+        const int ni_ = st.bsz[i], nj = st.bsz[i + 1];
+        form_off(i);
+        form_diag(i, D);
+        // work1 = -ESH(i,i+1) g(i+1) ; M = ESH(i,i) + work1 ESH(i+1,i)
+        gemm_z(G(ni_, nj, nj, U, s2, ni_, OP_N, g, s2, nj, OP_N, W1, s2, ni_, Z(-1), Z(0), B, nullptr), stream_);
+        gemm_z(G(ni_, ni_, nj, W1, s2, ni_, OP_N, L, s2, nj, OP_N, D, s2, ni_, Z(1), Z(1), B, nullptr), stream_);
+        inverse(ni_, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
+      }
     }
-    inverse(n0, D, s2, Gs, s2, B, nullptr, dstatus, o.precision, o);
-  }
-  // ---- T = |Re Tr[Gam A - Gam G Gam G^H]| on block 1 ----
-  {
-    const int n0 = st.bsz[0];
-    const int ct1 = (cbi < cbf) ? ni : nf;
-    double2 *Gam = U, *W2 = L;
-    gamma_batched(sig_of(ct1 - 1), s2, Gam, s2, n0, B, stream_);
-    gemm_z(G(n0, n0, n0, Gam, s2, n0, OP_N, Gs, s2, n0, OP_N, W1, s2, n0, Z(1), Z(0), B, nullptr), stream_);
-    gemm_z(G(n0, n0, n0, W1, s2, n0, OP_N, Gam, s2, n0, OP_N, W2, s2, n0, Z(1), Z(0), B, nullptr), stream_);
-    trace_T2(Gam, Gs, W2, s2, n0, B, dT, stream_);
+    // ---- Gr(1,1), iterative_cpu.F90:172-198 ----
+    {
+      const int n0 = st.bsz[0];
+      form_diag(0, D);
+      if (nbl > 1) {
+        const int n1 = st.bsz[1];
+        form_off(0);
+        gemm_z(G(n0, n1, n1, U, s2, n0, OP_N, g, s2, n1, OP_N, W1, s2, n0, Z(-1), Z(0), B, nullptr), stream_);
+        gemm_z(G(n0, n0, n1, W1, s2, n0, OP_N, L, s2, n1, OP_N, D, s2, n0, Z(1), Z(1), B, nullptr), stream_);
+      }
+      inverse(n0, D, s2, Gs, s2, B, nullptr, dstatus, o.precision, o);
+    }
+    // ---- T = |Re Tr[Gam A - Gam G Gam G^H]| on block 1 ----
+    {
+      const int n0 = st.bsz[0];
+      const int ct1 = (cbi < cbf) ? ni : nf;
+      double2 *Gam = U, *W2 = L;
+      gamma_batched(sig_of(ct1 - 1), s2, Gam, s2, n0, B, stream_);
+      gemm_z(G(n0, n0, n0, Gam, s2, n0, OP_N, Gs, s2, n0, OP_N, W1, s2, n0, Z(1), Z(0), B, nullptr), stream_);
+      gemm_z(G(n0, n0, n0, W1, s2, n0, OP_N, Gam, s2, n0, OP_N, W2, s2, n0, Z(1), Z(0), B, nullptr), stream_);
+      trace_T2(Gam, Gs, W2, s2, n0, B, dT, stream_);
+    }
   }
   // ---- per-item outputs: ncyc (item-major) and status ----
   {
@@ -528,23 +541,165 @@ void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-terminal transmission: the MULTITERMINAL branch of calculate_transmissions
+// (iterative.F90:1716-1766) with calculate_single_transmission_N_contacts
+// (iterative_cpu.F90:477-583).  gsmr blocks g_2..g_nbl are all kept; Gr(1,1)
+// and the tridiagonal blocks Gr(i,i), Gr(i,i-1) are computed once down to the
+// deepest contact block of all pairs (the reference extends them pair by pair
+// to the same values; Gr(i-1,i) is not needed for T and is not formed).  Per
+// pair the column Gr(i,bl1) = -g_i ESH(i,i-1) Gr(i-1,bl1) is swept from bl1 to
+// bl2 and T = |Re Tr[Gam2 Gr(bl2,bl1) Gam1 Gr(bl2,bl1)^H]|.  The reference
+// frees the gsmr blocks inside its pair loop (iterative.F90:1762-1763), which
+// leaves a later pair's column sweep without them; here every pair uses them.
+// ---------------------------------------------------------------------------
+void Engine::t_multi(DevStruct &st, int B, const double *dE, const int *dslot,
+                     const SolveOpts &o, int npairs, const int *pni, const int *pnf,
+                     const double2 *Sig, long long sSig, double *dT, int *dstatus) {
+  const int nbl = st.nbl, ncont = st.ncont;
+  const long long s2 = sSig, SB = s2 * B;
+  std::vector<std::vector<int>> on_blk(nbl);
+  for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
+  int ntmax = 1;
+  for (int p = 0; p < npairs; ++p)
+    ntmax = std::max(ntmax, std::max(st.cblk[pni[p] - 1], st.cblk[pnf[p] - 1]));
+  double2 *g = arena_.take<double2>(SB * nbl);
+  double2 *Gd = arena_.take<double2>(SB * ntmax);
+  double2 *Gl = arena_.take<double2>(SB * std::max(ntmax - 1, 1));
+  double2 *D = arena_.take<double2>(SB), *U = arena_.take<double2>(SB), *L = arena_.take<double2>(SB);
+  double2 *W1 = arena_.take<double2>(SB), *W2 = arena_.take<double2>(SB);
+  double2 *Ca = arena_.take<double2>(SB), *Cb = arena_.take<double2>(SB);
+  double2 *Gam1 = arena_.take<double2>(SB), *Gam2 = arena_.take<double2>(SB);
+  int *flag = arena_.take<int>(B);
+  const bool sd = (st.smode & S_DIAG) != 0, sof = (st.smode & S_OFFDIAG) != 0;
+  auto blk = [&](double2 *base, int i) { return base + (long long)i * SB; };
+  auto bs = [&](int i) { return st.bsz[i]; };
+  const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+
+  auto form_diag = [&](int i, double2 *out) {
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+    f.rows = bs(i); f.cols = bs(i);
+    f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+    f.out = out; f.s_out = s2;
+    const auto &cs = on_blk[i];
+    if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+    if (cs.size() >= 1) { f.sig = Sig + cs[0] * SB; f.s_sig = s2; }
+    if (cs.size() == 2) { f.sig2 = Sig + cs[1] * SB; f.s_sig2 = s2; }
+    form_block(f, stream_);
+  };
+  auto form_off = [&](int i, double2 *u, double2 *l) {  // u = ESH(i,i+1), l = ESH(i+1,i)
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+    f.s_identity = 0; f.s_out = s2;
+    if (u) {
+      f.rows = bs(i); f.cols = bs(i + 1);
+      f.H = st.ref(BK_HU, i); f.S = sof ? st.ref(BK_SU, i) : BlkRef(); f.out = u;
+      form_block(f, stream_);
+    }
+    if (l) {
+      f.rows = bs(i + 1); f.cols = bs(i);
+      f.H = st.ref(BK_HL, i); f.S = sof ? st.ref(BK_SL, i) : BlkRef(); f.out = l;
+      form_block(f, stream_);
+    }
+  };
+
+  // ---- calculate_gsmr_blocks(nbl -> 2), every g_i kept ----
+  if (nbl > 1) {
+    form_diag(nbl - 1, D);
+    inverse(bs(nbl - 1), D, s2, blk(g, nbl - 1), s2, B, nullptr, dstatus, o.precision, o);
+    for (int i = nbl - 2; i >= 1; --i) {
+      const int a = bs(i), c = bs(i + 1);
+      form_off(i, U, L);
+      form_diag(i, D);
+      gemm_z(G(a, c, c, U, s2, a, OP_N, blk(g, i + 1), s2, c, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+      gemm_z(G(a, a, c, W1, s2, a, OP_N, L, s2, c, OP_N, D, s2, a, one, one, B, nullptr), stream_);
+      inverse(a, D, s2, blk(g, i), s2, B, nullptr, dstatus, o.precision, o);
+    }
+  }
+  // ---- Gr(1,1) ----
+  {
+    const int a = bs(0);
+    form_diag(0, D);
+    if (nbl > 1) {
+      const int c = bs(1);
+      form_off(0, U, L);
+      gemm_z(G(a, c, c, U, s2, a, OP_N, blk(g, 1), s2, c, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+      gemm_z(G(a, a, c, W1, s2, a, OP_N, L, s2, c, OP_N, D, s2, a, one, one, B, nullptr), stream_);
+    }
+    inverse(a, D, s2, blk(Gd, 0), s2, B, nullptr, dstatus, o.precision, o);
+  }
+  // ---- Gr tridiagonal blocks 2..ntmax (iterative_cpu.F90:200-219) ----
+  for (int i = 1; i < ntmax; ++i) {
+    const int ni_ = bs(i), nm = bs(i - 1);
+    form_off(i - 1, U, L);  // U = ESH(i-1,i) nm x ni, L = ESH(i,i-1) ni x nm
+    gemm_z(G(ni_, nm, ni_, blk(g, i), s2, ni_, OP_N, L, s2, ni_, OP_N, W1, s2, ni_, one, zero, B, nullptr), stream_);
+    gemm_z(G(ni_, nm, nm, W1, s2, ni_, OP_N, blk(Gd, i - 1), s2, nm, OP_N, blk(Gl, i - 1), s2, ni_, mone, zero,
+             B, nullptr), stream_);
+    gemm_z(G(nm, ni_, ni_, U, s2, nm, OP_N, blk(g, i), s2, ni_, OP_N, W2, s2, nm, one, zero, B, nullptr), stream_);
+    copy_batched(blk(Gd, i), s2, blk(g, i), s2, (long long)ni_ * ni_, B, nullptr, stream_);
+    gemm_z(G(ni_, ni_, nm, blk(Gl, i - 1), s2, ni_, OP_N, W2, s2, nm, OP_N, blk(Gd, i), s2, ni_, mone, one, B,
+             nullptr), stream_);
+  }
+  // ---- per pair: column sweep and trace ----
+  for (int p = 0; p < npairs; ++p) {
+    const int ni = pni[p], nf = pnf[p];
+    int ct1, ct2;
+    if (st.cblk[ni - 1] < st.cblk[nf - 1]) { ct1 = ni; ct2 = nf; } else { ct1 = nf; ct2 = ni; }
+    const int bl1 = st.cblk[ct1 - 1], bl2 = st.cblk[ct2 - 1];  // 1-based, bl1 <= bl2
+    const int n1 = bs(bl1 - 1), n2 = bs(bl2 - 1);
+    const double2 *cur = (bl2 == bl1) ? blk(Gd, bl1 - 1) : blk(Gl, bl1 - 1);
+    const bool chk = o.eps_drop > 0.0 && nbl > 1 && bl2 - bl1 > 1;
+    if (chk) fill_int(flag, B, 0, stream_);
+    if (nbl > 1 && bl2 - bl1 > 1) {
+      if (chk) small_flag(blk(Gd, bl1 - 1), s2, (long long)n1 * n1, o.eps_drop, B, flag, stream_);
+      double2 *nxt = Ca;
+      for (int i = bl1 + 2; i <= bl2; ++i) {  // 1-based block i; Gr(i-1,bl1) = cur
+        const int a = bs(i - 1), c = bs(i - 2);
+        if (chk) small_flag(cur, s2, (long long)c * n1, o.eps_drop, B, flag, stream_);
+        form_off(i - 2, nullptr, L);  // ESH(i,i-1): a x c
+        gemm_z(G(a, c, a, blk(g, i - 1), s2, a, OP_N, L, s2, a, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+        gemm_z(G(a, n1, c, W1, s2, a, OP_N, cur, s2, c, OP_N, nxt, s2, a, one, zero, B, nullptr), stream_);
+        cur = nxt;
+        nxt = (nxt == Ca) ? Cb : Ca;
+      }
+    }
+    gamma_batched(Sig + (ct1 - 1) * SB, s2, Gam1, s2, n1, B, stream_);
+    gamma_batched(Sig + (ct2 - 1) * SB, s2, Gam2, s2, n2, B, stream_);
+    gemm_z(G(n2, n1, n2, Gam2, s2, n2, OP_N, cur, s2, n2, OP_N, W1, s2, n2, one, zero, B, nullptr), stream_);
+    gemm_z(G(n2, n1, n1, W1, s2, n2, OP_N, Gam1, s2, n1, OP_N, W2, s2, n2, one, zero, B, nullptr), stream_);
+    trace_prod_conj(W2, cur, s2, s2, n2, n1, B, dT + p, stream_, npairs);
+    if (chk) flag_set(dT + p, npairs, flag, o.eps_drop * o.eps_drop, B, stream_);
+  }
+}
+
 void Engine::transmission(DevStruct &st, int nitems, const double *dE, const int *dslot,
                           const SolveOpts &o, double *dT, int *dncyc, int *dstatus) {
+  transmission_pairs(st, nitems, dE, dslot, o, 1, &o.ni, &o.nf, dT, dncyc, dstatus);
+}
+
+void Engine::transmission_pairs(DevStruct &st, int nitems, const double *dE, const int *dslot,
+                                const SolveOpts &o, int npairs, const int *ni, const int *nf,
+                                double *dT, int *dncyc, int *dstatus) {
   if (nitems <= 0) return;
+  if (npairs < 1) throw std::runtime_error("transmission: npairs < 1");
+  for (int p = 0; p < npairs; ++p)
+    if (ni[p] < 1 || ni[p] > st.ncont || nf[p] < 1 || nf[p] > st.ncont || ni[p] == nf[p])
+      if (st.ncont >= 2) throw std::runtime_error("transmission: bad contact pair");
   HN_CHECK(hipSetDevice(device_));
-  const int B = pick_batch(st, nitems, o, t_item_bytes(st.nmax(), st.ncont));
+  const bool fast = fast_path(st, npairs, ni, nf);
+  int ntmax = 1;
+  for (int p = 0; p < npairs && st.ncont >= 2; ++p)
+    ntmax = std::max(ntmax, std::max(st.cblk[ni[p] - 1], st.cblk[nf[p] - 1]));
+  const size_t per_item = t_item_bytes(st, fast, ntmax);
+  const int B = pick_batch(st, nitems, o, per_item);
   last_batch_ = B;
-  // size the arena for a full batch once
-  {
-    const long long s2 = (long long)st.nmax() * st.nmax();
-    size_t need = (size_t)B * (size_t)s2 * sizeof(double2) * (size_t)(std::max(st.ncont, 1) * 8 + 16) +
-                  inv_work_bytes(st.nmax(), B, 8) + (size_t)B * 64 * sizeof(int) + (1 << 20);
-    arena_.reserve(need);
-  }
+  // size the arena for a full batch once (the inverse workspace is per batch)
+  arena_.reserve(per_item * (size_t)B + inv_work_bytes(st.nmax(), B, 8) + (1 << 20));
   for (int i0 = 0; i0 < nitems; i0 += B) {
     const int nb = std::min(B, nitems - i0);
-    t_batch(st, nb, dE + i0, dslot + i0, o, dT + i0, dncyc ? dncyc + (long long)i0 * st.ncont : nullptr,
-            dstatus + i0);
+    t_batch(st, nb, dE + i0, dslot + i0, o, npairs, ni, nf, dT + (long long)i0 * npairs,
+            dncyc ? dncyc + (long long)i0 * st.ncont : nullptr, dstatus + i0);
   }
   HN_CHECK(hipStreamSynchronize(stream_));
 }

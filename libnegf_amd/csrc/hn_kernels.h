@@ -77,9 +77,15 @@ void gamma_batched(const double2 *Sig, long long sS, double2 *Gam, long long sG,
 // (calculate_single_transmission_2_contacts, iterative_cpu.F90:399-470)
 void trace_T2(const double2 *Gam, const double2 *G, const double2 *W2, long long s, int n,
               int batch, double *T, hipStream_t st);
-// T_b = | Re Tr[W] | for W = Gam2 G21 Gam1 G21^H computed as Re sum (X)_ij conj(G21)_ij
+// T_b = | Re Tr[W] | for W = Gam2 G21 Gam1 G21^H computed as Re sum (X)_ij conj(G21)_ij,
+// written to T[b * tstride]
 void trace_prod_conj(const double2 *X, const double2 *G, long long sX, long long sG, int rows,
-                     int cols, int batch, double *T, hipStream_t st);
+                     int cols, int batch, double *T, hipStream_t st, int tstride = 1);
+// flag_b = 1 where max_ij |X_b(ij)| < eps (flags are only set, never cleared)
+void small_flag(const double2 *X, long long sX, long long nn, double eps, int batch, int *flag,
+                hipStream_t st);
+// T[b * tstride] = v where flag_b != 0
+void flag_set(double *T, int tstride, const int *flag, double v, int batch, hipStream_t st);
 // out_b = Re Tr[X_b Y_b] (square n): sum_ij X_ij Y_ji, scaled by alpha, accumulated
 void trace_prod(const double2 *X, const double2 *Y, long long sX, long long sY, int n,
                 int batch, double alpha, double *out, hipStream_t st);

@@ -231,7 +231,7 @@ __global__ __launch_bounds__(NT) void k_trace_T2(const double2 *Gam, const doubl
 
 __global__ __launch_bounds__(NT) void k_trace_prod_conj(const double2 *X, const double2 *G,
                                                         long long sX, long long sG, long long nn,
-                                                        double *T) {
+                                                        double *T, int tstride) {
   __shared__ double sh[NT / 64];
   const int b = blockIdx.x;
   const double2 *Xb = X + (long long)b * sX, *Gb = G + (long long)b * sG;
@@ -241,7 +241,28 @@ __global__ __launch_bounds__(NT) void k_trace_prod_conj(const double2 *X, const 
     acc += x.x * g.x + x.y * g.y;
   }
   acc = block_sum(acc, sh);
-  if (threadIdx.x == 0) T[b] = fabs(acc);
+  if (threadIdx.x == 0) T[(long long)b * tstride] = fabs(acc);
+}
+
+// flag_b |= (max |X_b| < eps)
+__global__ __launch_bounds__(NT) void k_small_flag(const double2 *X, long long sX, long long nn,
+                                                   double eps, int *flag) {
+  __shared__ double sh[NT / 64];
+  const int b = blockIdx.x;
+  double m = 0.0;
+  for (long long e = threadIdx.x; e < nn; e += NT) {
+    const double2 x = X[(long long)b * sX + e];
+    const double a = hypot(x.x, x.y);
+    m = (a > m || a != a) ? a : m;
+  }
+  m = block_max(m, sh);
+  if (threadIdx.x == 0 && m < eps) flag[b] = 1;
+}
+
+__global__ __launch_bounds__(NT) void k_flag_set(double *T, int tstride, const int *flag,
+                                                 double v, int batch) {
+  const int b = blockIdx.x * NT + threadIdx.x;
+  if (b < batch && flag[b]) T[(long long)b * tstride] = v;
 }
 
 __global__ __launch_bounds__(NT) void k_trace_prod(const double2 *X, const double2 *Y,
@@ -536,9 +557,19 @@ void trace_T2(const double2 *Gam, const double2 *G, const double2 *W2, long long
 }
 
 void trace_prod_conj(const double2 *X, const double2 *G, long long sX, long long sG, int rows,
-                     int cols, int batch, double *T, hipStream_t st) {
+                     int cols, int batch, double *T, hipStream_t st, int tstride) {
   hipLaunchKernelGGL(k_trace_prod_conj, dim3(batch), dim3(NT), 0, st, X, G, sX, sG,
-                     (long long)rows * cols, T);
+                     (long long)rows * cols, T, tstride);
+  HN_CHECK(hipGetLastError());
+}
+void small_flag(const double2 *X, long long sX, long long nn, double eps, int batch, int *flag,
+                hipStream_t st) {
+  hipLaunchKernelGGL(k_small_flag, dim3(batch), dim3(NT), 0, st, X, sX, nn, eps, flag);
+  HN_CHECK(hipGetLastError());
+}
+void flag_set(double *T, int tstride, const int *flag, double v, int batch, hipStream_t st) {
+  hipLaunchKernelGGL(k_flag_set, dim3((batch + NT - 1) / NT), dim3(NT), 0, st, T, tstride, flag,
+                     v, batch);
   HN_CHECK(hipGetLastError());
 }
 

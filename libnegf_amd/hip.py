@@ -58,6 +58,8 @@ def load():
     L.hipnegf_struct_set_block_device.argtypes = [vp, i, i, i, vp]
     L.hipnegf_solve_transmission.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, vp, vp]
     L.hipnegf_solve_transmission_device.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, vp, vp]
+    L.hipnegf_solve_transmission_pairs.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), i, vp, vp,
+                                                   vp, vp, vp]
     L.hipnegf_solve_scba_mw.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, i, d, vp, i, i,
                                         vp, vp, vp, vp, vp, vp]
     L.hipnegf_zgemm_batched.argtypes = [vp, i, i, i, vp, vp, ll, i, i, vp, ll, i, i, vp, vp, ll, i, i]
@@ -239,6 +241,27 @@ def solve_transmission(eng: Engine, ds: DeviceStructure, E: np.ndarray, slot=Non
     o = _opts(delta, ni, nf, precision, max_batch, flags=flags)
     check(load().hipnegf_solve_transmission(eng.handle, ds.handle, n, _ptr(E), _ptr(sl),
                                             C.byref(o), _ptr(T), _ptr(ncyc), _ptr(st)))
+    return T, ncyc, st
+
+
+def solve_transmission_pairs(eng: Engine, ds: DeviceStructure, E: np.ndarray, pairs, slot=None,
+                             delta: float = 1e-4, precision: int = FP64, max_batch: int = 0,
+                             flags: int = 0):
+    """T for several (ni, nf) contact pairs (1-based) per energy: returns
+    (T[n, npairs], ncyc[n, ncont], status[n])."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    n = len(E)
+    sl = np.zeros(n, dtype=np.int32) if slot is None else np.ascontiguousarray(slot, dtype=np.int32)
+    ni = np.ascontiguousarray([p[0] for p in pairs], dtype=np.int32)
+    nf = np.ascontiguousarray([p[1] for p in pairs], dtype=np.int32)
+    T = np.zeros((n, len(pairs)))
+    nc = max(ds.template.ncont, 1)
+    ncyc = np.zeros((n, nc), dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    o = _opts(delta, 1, 2, precision, max_batch, flags=flags)
+    check(load().hipnegf_solve_transmission_pairs(eng.handle, ds.handle, n, _ptr(E), _ptr(sl),
+                                                  C.byref(o), len(pairs), _ptr(ni), _ptr(nf),
+                                                  _ptr(T), _ptr(ncyc), _ptr(st)))
     return T, ncyc, st
 
 

@@ -158,3 +158,70 @@ def test_non_mirror_contacts(gpu):
     Tref, nref = _oracle(s, E, 1e-5)
     _compare(T, Tref)
     assert (ncyc == nref).all()
+
+
+def _three_terminal(W=12, nbl=6, seed=31):
+    """Ribbon with a third lead attached to block 3 (side coupling 0.6 per site)."""
+    from libnegf_amd.structure import BlockStructure, Contact
+    base = wl.ribbon(W, nbl, disorder=0.4, seed=seed)
+    c0 = base.contacts[0]
+    side = Contact(cblk=3, H00=c0.H00, H01=c0.H01, HMC=-0.6 * np.eye(W))
+    return BlockStructure(Hd=base.Hd, Hu=base.Hu, Hl=base.Hl,
+                          contacts=[base.contacts[0], base.contacts[1], side])
+
+
+def test_three_terminal_pairs(gpu):
+    """N-contact path (calculate_single_transmission_N_contacts): every lead
+    pair of a three-terminal device in one solve, vs the oracle pair by pair."""
+    hip, eng = gpu
+    from oracle import pyoracle
+    s = _three_terminal()
+    E = np.linspace(-2.6, 2.6, 9)
+    pairs = [(1, 2), (1, 3), (2, 3), (3, 1)]
+    ds = hip.DeviceStructure(s)
+    T, ncyc, st = hip.solve_transmission_pairs(eng, ds, E, pairs, delta=1e-5)
+    assert (st == 0).all()
+    pyoracle.build()
+    pyoracle.set_backend("builtin")
+    for j, (a, b) in enumerate(pairs):
+        Tref, nref, _ = pyoracle.transmission_batch(s, E, 1e-5, ni=a, nf=b)
+        _compare(T[:, j], Tref)
+        assert (ncyc == nref).all()
+    # a single pair through the single-pair entry point takes the same path
+    T13, _, _ = hip.solve_transmission(eng, ds, E, delta=1e-5, ni=1, nf=3)
+    np.testing.assert_array_equal(T13, T[:, 1])
+
+
+def test_two_contacts_off_first_block(gpu):
+    """Two contacts, neither on block 1: multi-terminal branch with a column
+    sweep of several blocks (cblk = [5, 2] of 7)."""
+    hip, eng = gpu
+    from libnegf_amd.structure import BlockStructure, Contact
+    base = wl.KSlab.make(n=48, nbl=7, seed=5).at_k(0.2, 0.9)
+    c0, c1 = base.contacts
+    s = BlockStructure(Hd=base.Hd, Hu=base.Hu, Hl=base.Hl, Sd=base.Sd, Su=base.Su, Sl=base.Sl,
+                       contacts=[Contact(cblk=5, H00=c0.H00, H01=c0.H01, HMC=c0.HMC, S00=c0.S00,
+                                         S01=c0.S01, SMC=c0.SMC),
+                                 Contact(cblk=2, H00=c1.H00, H01=c1.H01, HMC=c1.HMC, S00=c1.S00,
+                                         S01=c1.S01, SMC=c1.SMC)])
+    E = np.array([-0.9, -0.1, 0.6])
+    ds = hip.DeviceStructure(s)
+    T, ncyc, st = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    Tref, nref = _oracle(s, E, 1e-5)
+    assert (st == 0).all()
+    _compare(T, Tref)
+    assert (ncyc == nref).all()
+
+
+def test_fast_path_equals_multiterminal(gpu):
+    """The 2-contact fast path (Tr[Gam A - Gam G Gam G^H] on block 1) and the
+    multi-terminal column sweep agree on the same device (two pairs force the
+    multi-terminal branch)."""
+    hip, eng = gpu
+    s = wl.ribbon(16, 8, disorder=0.5, seed=37)
+    E = np.linspace(-2.5, 2.5, 11)
+    ds = hip.DeviceStructure(s)
+    Tf, _, _ = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    Tm, _, _ = hip.solve_transmission_pairs(eng, ds, E, [(1, 2), (2, 1)], delta=1e-5)
+    _compare(Tm[:, 0], Tf)
+    _compare(Tm[:, 1], Tf)
