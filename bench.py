@@ -28,7 +28,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--items", type=int, default=0, help="items per step (0: config default)")
     ap.add_argument("--nbl", type=int, default=0, help="override the number of layers")
@@ -40,6 +40,16 @@ def parse():
                     help="no per-launch HIP events in the timed region (no roofline field)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--serve", action="store_true",
+                    help="serving workload: Poisson request stream through the server loop")
+    ap.add_argument("--requests", type=int, default=24, help="requests per serving pass")
+    ap.add_argument("--rates", default="0.5,1.0,1.5",
+                    help="arrival rates as multiples of the measured saturation rate")
+    ap.add_argument("--workers", type=int, default=3)
+    ap.add_argument("--policy", type=int, default=0, help="0 FCFS, 1 least attained service")
+    ap.add_argument("--n-choices", default="32,64,128,256,512")
+    ap.add_argument("--nbl-range", default="10,200")
+    ap.add_argument("--ne-range", default="50,2000")
     ap.add_argument("--verbose", action="store_true")
     return ap.parse_args()
 
@@ -75,6 +85,178 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def fermi(E, mu, kT):
+    """fermi_f (distributions.F90:46-74)."""
+    x = (E - mu) / kT
+    return np.where(np.abs(x) > 30, np.where(E > mu, np.exp(-np.clip(x, -700, 700)), 1.0),
+                    1.0 / (np.exp(np.clip(x, -700, 700)) + 1.0))
+
+
+def bench_serve(a, hip, wl, torch):
+    """C5: Poisson stream of transmission requests (n, nbl, nE, k from a
+    per-request seed; blocks generated in HBM at admission) through the
+    persistent server loop.  Saturation throughput from a burst, then
+    p50 / p99 latency at arrival rates relative to it."""
+    import threading
+    from libnegf_amd import serve
+    nch = tuple(int(x) for x in a.n_choices.split(","))
+    nblr = tuple(int(x) for x in a.nbl_range.split(","))
+    ner = tuple(int(x) for x in a.ne_range.split(","))
+    eta = 1e-5
+
+    def specs(base):
+        return [wl.serving_spec(base + i, nch, nblr, ner) for i in range(a.requests)]
+
+    def flops(sp):
+        n, nbl, nE = sp[0], sp[1], sp[2]
+        return (56.0 * 2 * 25 + 24.0 * nbl + 64.0) * n ** 3 * nE
+
+    srv = serve.Server(workers=a.workers, policy=a.policy)
+    # warm-up: one small request per PL size (first-use costs: code objects, workspaces)
+    for n in nch:
+        srv.wait(srv.submit_kslab(1, n, 4, np.linspace(-1, 1, 8), delta=eta))
+
+    def run(sp, base, gaps):
+        ids, t_sub = [None] * len(sp), [0.0] * len(sp)
+
+        def submitter():
+            t0 = time.perf_counter()
+            t_next = t0
+            for i, (n, nbl, nE, kx, ky) in enumerate(sp):
+                t_next += gaps[i]
+                d = t_next - time.perf_counter()
+                if d > 0:
+                    time.sleep(d)
+                t_sub[i] = time.perf_counter()
+                ids[i] = srv.submit_kslab(base + i, n, nbl, wl.energy_grid(nE, -2.0, 2.0), k=(kx, ky),
+                                          delta=eta)
+
+        th = threading.Thread(target=submitter)
+        t0 = time.perf_counter()
+        th.start()
+        res = []
+        for i in range(len(sp)):
+            while ids[i] is None:
+                time.sleep(0.001)
+            res.append(srv.wait(ids[i]))
+        th.join()
+        wall = time.perf_counter() - t0
+        lat = np.array([r["latency"] for r in res])
+        pts = sum(s[2] for s in sp)
+        bad = sum(int((r["status"] != 0).sum()) for r in res)
+        return dict(wall=wall, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                    mean=float(lat.mean()), points_per_s=pts / wall, requests=len(sp),
+                    tflops_algorithmic=sum(flops(s) for s in sp) / wall / 1e12, bad_items=bad)
+
+    sp = specs(100_000)
+    burst = run(sp, 100_000, [0.0] * len(sp))
+    mu = len(sp) / burst["wall"]
+    log(f"saturation (burst of {len(sp)}): {burst['points_per_s']:.1f} points/s, {mu:.3f} req/s, "
+        f"p50 {burst['p50']:.2f}s p99 {burst['p99']:.2f}s")
+    rates = {}
+    rng = np.random.default_rng(5)
+    for j, f in enumerate(float(x) for x in a.rates.split(",")):
+        lam = f * mu
+        gaps = rng.exponential(1.0 / lam, len(sp))
+        gaps[0] = 0.0
+        base = 200_000 + 10_000 * j
+        r = run(specs(base), base, gaps)
+        rates[f"{f:g}x"] = r
+        log(f"rate {f:g} x saturation ({lam:.3f} req/s): p50 {r['p50']:.2f}s p99 {r['p99']:.2f}s "
+            f"p99/p50 {r['p99'] / r['p50']:.2f}, {r['points_per_s']:.1f} points/s")
+    srv.close()
+    out = {"metric": "RGF energy-points/s", "value": burst["points_per_s"], "unit": "energy-points/s",
+           "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": burst["wall"] * 1e3,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic (per-request seeds; slabs generated on the device)",
+           "config": dict(workload="c5_serving_poisson", requests_per_pass=a.requests,
+                          n_choices=list(nch), nbl_range=list(nblr), ne_range=list(ner),
+                          workers=a.workers, policy=["fcfs", "las"][a.policy]),
+           "saturation": burst, "latency_by_rate": rates}
+    print(json.dumps(out), flush=True)
+
+
+def bench_c4(a, hip, wl, torch):
+    """Elastic-dephasing SCBA + Meir-Wingreen current per energy (meir_wingreen
+    semantics: contact occupations f(E, mu_c, kT), reference = none)."""
+    s, Egrid, eta, mu, kT = wl.config4(nbl=a.nbl or None)
+    c = wl.C4
+    per_step = a.items or len(Egrid)
+    E = Egrid[np.linspace(0, len(Egrid) - 1, per_step).astype(int)]
+    frm = np.stack([fermi(E, mu[0], kT), fermi(E, mu[1], kT), np.zeros_like(E)], axis=1)
+    cp = np.full(s.ndev, c["coupling"])
+    eng = hip.Engine(0)
+    ds = hip.DeviceStructure(s)
+    run = lambda: hip.solve_scba_mw(eng, ds, E, cp, c["scba_niter"], c["scba_tol"], frm, 3, delta=eta)
+    for i in range(a.warmup):
+        t0 = time.time()
+        run()
+        log(f"warmup {i}: {time.time() - t0:.2f}s")
+    torch.cuda.synchronize()
+    if not a.no_profile:
+        hip.profile_clear()
+        hip.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r = run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern = None
+    if not a.no_profile:
+        hip.profile_enable(False)
+        kern = {hip.PT_NAMES[t]: hip.profile_read(t) for t in hip.PT_NAMES}
+        hip.profile_clear()
+    value = a.steps * per_step / elapsed
+    it = r["iters"]
+    log(f"c4: {per_step} energies/step, {elapsed / a.steps:.2f}s/step -> {value:.2f} points/s; "
+        f"SCBA passes mean {it.mean():.1f} max {it.max()}; status nonzero {(r['status'] != 0).sum()}")
+    roofline = None
+    if kern:
+        v = kern["k_gemm<double, 1>"]
+        if v["flops"] > 0:
+            avg = v["ms"] * 1e-3 / v["launches"]
+            ach = v["flops"] / v["launches"] / avg / 1e12
+            roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "k_gemm<double, 1>",
+                        "avg_launch_us": avg * 1e6, "launches": v["launches"],
+                        "share_of_step_time": v["ms"] * 1e-3 / elapsed}
+    cpu = None
+    if not a.no_cpu and a.cpu_items > 0:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import pyoracle
+        pyoracle.build()
+        try:
+            backend = pyoracle.set_backend("mkl")
+        except RuntimeError:
+            backend = pyoracle.set_backend("builtin")
+        sel = np.linspace(0, per_step - 1, min(a.cpu_items, 16)).astype(int)
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16, len(sel))
+        t1 = time.perf_counter()
+        with ThreadPoolExecutor(threads) as ex:
+            refs = list(ex.map(lambda j: pyoracle.scba_mw(s, E[j], eta, cp, c["scba_niter"],
+                                                           c["scba_tol"], frm[j], 3), sel))
+        secs = time.perf_counter() - t1
+        cur = np.array([q["curr"] for q in refs])
+        scale = np.abs(cur).max()
+        err = float(np.abs(r["curr"][sel] - cur).max() / scale) if scale > 0 else 0.0
+        cpu = dict(value=len(sel) / secs, unit="energy-points/s", cores=threads, kind="port",
+                   sample=f"{len(sel)} SCBA energies spread over the grid, oracle with {backend}, "
+                          "one thread per energy", seconds=secs,
+                   parity_max_rel_err_current=err,
+                   iters_equal=bool((np.array([q["iters"] for q in refs]) == it[sel]).all()))
+        log(f"cpu baseline {cpu}")
+    out = {"metric": "RGF energy-points/s", "value": value, "unit": "energy-points/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1000.0 * elapsed / a.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic (seeded generators, libnegf_amd/workloads.py)",
+           "config": dict(workload=f"c4_elphdd_scba_W{s.bsz[0]}_nbl{s.nbl}_E{len(Egrid)}",
+                          coupling=c["coupling"], scba_tol=c["scba_tol"], niter=c["scba_niter"],
+                          items_per_step=per_step, scba_passes_mean=float(it.mean())),
+           "roofline": roofline, "cpu_baseline": cpu,
+           "kernel_ms": None if kern is None else {k: round(v["ms"], 3) for k, v in kern.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -89,6 +271,12 @@ def main():
         copy_calibration()
         return
     prec = hip.FP64 if a.precision == "fp64" else hip.MIXED
+    if a.config == "c4":
+        bench_c4(a, hip, wl, torch)
+        return
+    if a.serve:
+        bench_serve(a, hip, wl, torch)
+        return
 
     # ------------------------------------------------------------ workload ----
     if a.config == "c3":

@@ -242,6 +242,52 @@ int hipnegf_engine_sync(hipnegf_engine e);
 /* stream the engine launches on (a hipStream_t) */
 void *hipnegf_engine_stream(hipnegf_engine e);
 
+/* ---- synthetic structures (no counterpart in the reference) ------------- */
+/* Fill `slot` of a structure (uniform PL size n, contact 1 on the last block,
+ * contact 2 on block 1; smode 7 with overlap, else 0) with the k-resolved
+ * random slab of `seed` at transverse k = (kx, ky), generated on the device. */
+int hipnegf_struct_generate_kslab(hipnegf_engine e, hipnegf_struct st, int slot,
+                                  unsigned long long seed, double kx, double ky, int with_overlap);
+/* copy one block of a slot back to the host (same kinds as set_block) */
+int hipnegf_struct_get_block(hipnegf_struct st, int slot, int kind, int index, double *host);
+
+/* ---- serving loop (no counterpart in the reference) ----------------------
+ * A persistent request loop: `workers` threads, each with its own engine and
+ * HIP stream, serve transmission requests.  A request is admitted when its
+ * device footprint fits the HBM budget (FCFS), its energies are cut into
+ * chunks of bounded work, and the workers interleave chunks of all admitted
+ * requests (policy 0: oldest request first; 1: least GPU time so far), so
+ * requests that arrive while others run start between chunks.  A request's
+ * results do not depend on the mix it ran in. */
+typedef struct hipnegf_server_s *hipnegf_server;
+typedef struct {
+  int workers;                /* 1..16; 0: 2 */
+  int policy;                 /* 0 FCFS, 1 least attained service */
+  int max_chunk;              /* items per chunk cap; 0: automatic */
+  int reserved;
+  double chunk_flops;         /* target algorithmic flops per chunk; 0: 2e12 */
+  long long mem_budget;       /* bytes; 0: 85 % of free HBM */
+  long long worker_workspace; /* bytes per worker; 0: automatic */
+} hipnegf_server_opts;
+int hipnegf_server_create(int device, const hipnegf_server_opts *o, hipnegf_server *s);
+/* finishes the admitted requests, drops queued ones, joins the workers */
+int hipnegf_server_destroy(hipnegf_server s);
+/* request on a generated slab (built in HBM at admission, freed when done) */
+int hipnegf_server_submit_kslab(hipnegf_server s, unsigned long long seed, int n, int nbl,
+                                double kx, double ky, int with_overlap, int nE, const double *E,
+                                const hipnegf_opts *o, unsigned long long *id);
+/* request on a caller-owned resident structure (must outlive the request) */
+int hipnegf_server_submit(hipnegf_server s, hipnegf_struct st, int slot, int nE, const double *E,
+                          const hipnegf_opts *o, unsigned long long *id);
+int hipnegf_server_poll(hipnegf_server s, unsigned long long id, int *done);
+/* blocks until done; T[nE], ncyc[nE*ncont], status[nE] (any may be NULL);
+ * times[4] = {queue s, latency s, GPU s, chunks}.  Returns nonzero with the
+ * request's error message if it failed.  The id is released. */
+int hipnegf_server_wait(hipnegf_server s, unsigned long long id, double *T, int *ncyc, int *status,
+                        double *times);
+/* {submitted, completed, running, queued} */
+int hipnegf_server_stats(hipnegf_server s, long long *out);
+
 /* Launch timing (no counterpart in the reference; measurement hook for
  * benchmarks).  While enabled, every tagged launch records a HIP event pair on
  * its stream.  Tags: 0 masked GEMMs (decimation, SCBA), 1 RGF-sweep GEMMs,

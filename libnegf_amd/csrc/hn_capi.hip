@@ -14,6 +14,8 @@
 #include "hn_inv.h"
 #include "hn_kernels.h"
 #include "hn_prof.h"
+#include "hn_gen.h"
+#include "hn_serve.h"
 
 struct hipnegf_ctx {
   std::unique_ptr<hn::Engine> eng;
@@ -645,6 +647,106 @@ int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long
 }
 int hipnegf_profile_clear(void) {
   return guard([&] { hn::profiler().clear(); return 0; });
+}
+
+struct hipnegf_server_s {
+  std::unique_ptr<hn::Server> srv;
+};
+
+int hipnegf_struct_generate_kslab(hipnegf_engine e, hipnegf_struct st, int slot,
+                                  unsigned long long seed, double kx, double ky, int with_overlap) {
+  return guard([&] {
+    hn::gen_kslab(*st->st, slot, seed, kx, ky, with_overlap, e->eng->stream());
+    HN_CHECK(hipStreamSynchronize(e->eng->stream()));
+    return 0;
+  });
+}
+
+int hipnegf_struct_get_block(hipnegf_struct st, int slot, int kind, int index, double *host) {
+  return guard([&] {
+    long long n = 0;
+    double2 *d = block_ptr(*st->st, slot, kind, index, &n);
+    HN_CHECK(hipMemcpy(host, d, n * sizeof(double2), hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+int hipnegf_server_create(int device, const hipnegf_server_opts *o, hipnegf_server *s) {
+  return guard([&] {
+    hn::ServeOpts so;
+    if (o) {
+      if (o->workers) so.workers = o->workers;
+      so.policy = o->policy;
+      so.max_chunk = o->max_chunk;
+      if (o->chunk_flops > 0) so.chunk_flops = o->chunk_flops;
+      so.mem_budget = (size_t)std::max(0LL, o->mem_budget);
+      so.worker_ws = (size_t)std::max(0LL, o->worker_workspace);
+    }
+    auto *x = new hipnegf_server_s;
+    try {
+      x->srv.reset(new hn::Server(device, so));
+    } catch (...) {
+      delete x;
+      throw;
+    }
+    *s = x;
+    return 0;
+  });
+}
+
+int hipnegf_server_destroy(hipnegf_server s) {
+  return guard([&] { delete s; return 0; });
+}
+
+int hipnegf_server_submit_kslab(hipnegf_server s, unsigned long long seed, int n, int nbl,
+                                double kx, double ky, int with_overlap, int nE, const double *E,
+                                const hipnegf_opts *o, unsigned long long *id) {
+  return guard([&] {
+    if (nE <= 0) throw std::runtime_error("nE <= 0");
+    hn::ReqStruct rs;
+    rs.gen = 1; rs.seed = seed; rs.n = n; rs.nbl = nbl; rs.kx = kx; rs.ky = ky;
+    rs.overlap = with_overlap ? 1 : 0;
+    *id = s->srv->submit(rs, std::vector<double>(E, E + nE), to_opts(o));
+    return 0;
+  });
+}
+
+int hipnegf_server_submit(hipnegf_server s, hipnegf_struct st, int slot, int nE, const double *E,
+                          const hipnegf_opts *o, unsigned long long *id) {
+  return guard([&] {
+    if (nE <= 0) throw std::runtime_error("nE <= 0");
+    hn::ReqStruct rs;
+    rs.resident = st->st.get();
+    rs.slot = slot;
+    *id = s->srv->submit(rs, std::vector<double>(E, E + nE), to_opts(o));
+    return 0;
+  });
+}
+
+int hipnegf_server_poll(hipnegf_server s, unsigned long long id, int *done) {
+  return guard([&] { *done = s->srv->done(id) ? 1 : 0; return 0; });
+}
+
+int hipnegf_server_wait(hipnegf_server s, unsigned long long id, double *T, int *ncyc, int *status,
+                        double *times) {
+  return guard([&] {
+    hn::ReqResult r = s->srv->wait(id);
+    if (times) {
+      times[0] = r.t_queue;
+      times[1] = r.t_latency;
+      times[2] = r.t_gpu;
+      times[3] = (double)r.chunks;
+    }
+    if (!r.error.empty()) throw std::runtime_error(r.error);
+    if (T) std::copy(r.T.begin(), r.T.end(), T);
+    if (ncyc) std::copy(r.ncyc.begin(), r.ncyc.end(), ncyc);
+    if (status) std::copy(r.status.begin(), r.status.end(), status);
+    return 0;
+  });
+}
+
+int hipnegf_server_stats(hipnegf_server s, long long *out) {
+  return guard([&] { s->srv->stats(out); return 0; });
 }
 
 }  // extern "C"

@@ -30,9 +30,17 @@ struct DevStruct {
   int device = 0;
   int mirror_state = -1;  // contacts 1/2 mirror images in every slot: -1 unknown, 0 no, 1 yes
 
+  bool owned = true;       // pool allocated (and freed) by this object
+
   DevStruct(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc, int smode,
             int nslots);
+  // Same layout on caller-provided memory of at least layout_bytes(...) bytes,
+  // zeroed on `zs` (stream-ordered; not freed by the destructor).
+  DevStruct(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc, int smode,
+            int nslots, void *external, hipStream_t zs);
   ~DevStruct();
+  static size_t layout_bytes(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
+                             int smode, int nslots);
   long long block_elems(int kind, int idx) const;
   BlkRef ref(int kind, int idx) const {
     BlkRef r;
@@ -97,6 +105,12 @@ class Engine {
   // T[nitems], ncyc[nitems*ncont], status[nitems]; the pair is (o.ni, o.nf).
   void transmission(DevStruct &st, int nitems, const double *dE, const int *dslot,
                     const SolveOpts &o, double *dT, int *dncyc, int *dstatus);
+  // One-time per-structure checks (mirror-image contacts) so that several
+  // engines can then share the structure read-only.
+  void prepare(DevStruct &st) { (void)mirror_contacts(st); }
+  // device workspace bytes per item of a transmission batch on `st`
+  static size_t transmission_item_bytes(const DevStruct &st, int npairs, const int *ni,
+                                        const int *nf);
   // Same for npairs contact pairs (host arrays ni[], nf[], 1-based):
   // T[nitems*npairs] item-major.  One pair with two contacts and the lower
   // contact on block 1 takes the fast path, everything else the multi-terminal

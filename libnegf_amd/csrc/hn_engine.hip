@@ -18,30 +18,21 @@
 
 #include "hn_engine.h"
 #include "hn_gemm.h"
+#include "hn_prof.h"
 
 namespace hn {
 
 // ============================================================================
 // DevStruct
 // ============================================================================
-DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, const int *nc_,
-                     int smode_, int nslots_)
-    : nbl(nbl_), ncont(ncont_), smode(smode_), nslots(nslots_) {
-  if (nbl < 1) throw std::runtime_error("structure: nbl < 1");
-  if (ncont < 0 || ncont > kMaxCont) throw std::runtime_error("structure: bad ncont");
-  if (nslots < 1) throw std::runtime_error("structure: nslots < 1");
-  bsz.assign(bsz_, bsz_ + nbl);
-  cblk.assign(cblk_, cblk_ + ncont);
-  nc.assign(nc_, nc_ + ncont);
-  for (int i = 0; i < nbl; ++i)
-    if (bsz[i] < 1) throw std::runtime_error("structure: empty block");
-  for (int c = 0; c < ncont; ++c)
-    if (cblk[c] < 1 || cblk[c] > nbl || nc[c] < 1)
-      throw std::runtime_error("structure: bad contact description");
+namespace {
+// Block offsets of one slot (256-byte aligned blocks); returns the slot size.
+long long layout(int nbl, const std::vector<int> &bsz, int ncont, const std::vector<int> &cblk,
+                 const std::vector<int> &nc, int smode, std::vector<long long> *off) {
   long long o = 0;
   auto put = [&](int kind, long long n) {
-    off[kind].push_back(o);
-    o += (n + 15) & ~15LL;  // 256-byte aligned blocks
+    if (off) off[kind].push_back(o);
+    o += (n + 15) & ~15LL;
   };
   for (int i = 0; i < nbl; ++i) put(BK_HD, (long long)bsz[i] * bsz[i]);
   if (smode & S_DIAG)
@@ -63,14 +54,58 @@ DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, co
       put(BK_SMC, nm);
     }
   }
-  slot_elems = o;
+  return o;
+}
+
+void check_shape(int nbl, const std::vector<int> &bsz, int ncont, const std::vector<int> &cblk,
+                 const std::vector<int> &nc, int nslots) {
+  if (nbl < 1) throw std::runtime_error("structure: nbl < 1");
+  if (ncont < 0 || ncont > kMaxCont) throw std::runtime_error("structure: bad ncont");
+  if (nslots < 1) throw std::runtime_error("structure: nslots < 1");
+  for (int i = 0; i < nbl; ++i)
+    if (bsz[i] < 1) throw std::runtime_error("structure: empty block");
+  for (int c = 0; c < ncont; ++c)
+    if (cblk[c] < 1 || cblk[c] > nbl || nc[c] < 1)
+      throw std::runtime_error("structure: bad contact description");
+}
+}  // namespace
+
+size_t DevStruct::layout_bytes(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
+                               int smode, int nslots) {
+  std::vector<int> b(bsz, bsz + nbl), cb(cblk, cblk + ncont), n(nc, nc + ncont);
+  check_shape(nbl, b, ncont, cb, n, nslots);
+  return (size_t)layout(nbl, b, ncont, cb, n, smode, nullptr) * nslots * sizeof(double2);
+}
+
+DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, const int *nc_,
+                     int smode_, int nslots_)
+    : nbl(nbl_), ncont(ncont_), smode(smode_), nslots(nslots_) {
+  bsz.assign(bsz_, bsz_ + std::max(nbl, 0));
+  cblk.assign(cblk_, cblk_ + std::max(ncont, 0));
+  nc.assign(nc_, nc_ + std::max(ncont, 0));
+  check_shape(nbl, bsz, ncont, cblk, nc, nslots);
+  slot_elems = layout(nbl, bsz, ncont, cblk, nc, smode, off);
   HN_CHECK(hipGetDevice(&device));
   HN_CHECK(hipMalloc(&pool, bytes()));
   HN_CHECK(hipMemset(pool, 0, bytes()));
 }
 
+DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, const int *nc_,
+                     int smode_, int nslots_, void *external, hipStream_t zs)
+    : nbl(nbl_), ncont(ncont_), smode(smode_), nslots(nslots_) {
+  bsz.assign(bsz_, bsz_ + std::max(nbl, 0));
+  cblk.assign(cblk_, cblk_ + std::max(ncont, 0));
+  nc.assign(nc_, nc_ + std::max(ncont, 0));
+  check_shape(nbl, bsz, ncont, cblk, nc, nslots);
+  slot_elems = layout(nbl, bsz, ncont, cblk, nc, smode, off);
+  HN_CHECK(hipGetDevice(&device));
+  pool = (double2 *)external;
+  owned = false;
+  HN_CHECK(hipMemsetAsync(pool, 0, bytes(), zs));
+}
+
 DevStruct::~DevStruct() {
-  if (pool) (void)hipFree(pool);
+  if (pool && owned) (void)hipFree(pool);
 }
 
 long long DevStruct::block_elems(int kind, int idx) const {
@@ -166,6 +201,12 @@ size_t t_item_bytes(const DevStruct &st, bool fast, int ntmax) {
 bool fast_path(const DevStruct &st, int npairs, const int *ni, const int *nf) {
   if (st.ncont != 2 || npairs != 1) return false;
   return std::min(st.cblk[ni[0] - 1], st.cblk[nf[0] - 1]) == 1;
+}
+int deepest_block(const DevStruct &st, int npairs, const int *ni, const int *nf) {
+  int ntmax = 1;
+  for (int p = 0; p < npairs && st.ncont >= 2; ++p)
+    ntmax = std::max(ntmax, std::max(st.cblk[ni[p] - 1], st.cblk[nf[p] - 1]));
+  return ntmax;
 }
 
 }  // namespace
@@ -673,6 +714,11 @@ void Engine::t_multi(DevStruct &st, int B, const double *dE, const int *dslot,
   }
 }
 
+size_t Engine::transmission_item_bytes(const DevStruct &st, int npairs, const int *ni,
+                                       const int *nf) {
+  return t_item_bytes(st, fast_path(st, npairs, ni, nf), deepest_block(st, npairs, ni, nf));
+}
+
 void Engine::transmission(DevStruct &st, int nitems, const double *dE, const int *dslot,
                           const SolveOpts &o, double *dT, int *dncyc, int *dstatus) {
   transmission_pairs(st, nitems, dE, dslot, o, 1, &o.ni, &o.nf, dT, dncyc, dstatus);
@@ -687,11 +733,7 @@ void Engine::transmission_pairs(DevStruct &st, int nitems, const double *dE, con
     if (ni[p] < 1 || ni[p] > st.ncont || nf[p] < 1 || nf[p] > st.ncont || ni[p] == nf[p])
       if (st.ncont >= 2) throw std::runtime_error("transmission: bad contact pair");
   HN_CHECK(hipSetDevice(device_));
-  const bool fast = fast_path(st, npairs, ni, nf);
-  int ntmax = 1;
-  for (int p = 0; p < npairs && st.ncont >= 2; ++p)
-    ntmax = std::max(ntmax, std::max(st.cblk[ni[p] - 1], st.cblk[nf[p] - 1]));
-  const size_t per_item = t_item_bytes(st, fast, ntmax);
+  const size_t per_item = transmission_item_bytes(st, npairs, ni, nf);
   const int B = pick_batch(st, nitems, o, per_item);
   last_batch_ = B;
   // size the arena for a full batch once (the inverse workspace is per batch)
@@ -820,9 +862,13 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
       form_block(f, stream_);
     }
   };
+  int n_active = B;  // active items of the current pass (act changes only at its end)
   auto mm = [&](const double2 *A, int opA, const double2 *Bm, int opB, double2 *C, double2 al,
                 double2 be) {
-    gemm_z(G(n, n, n, A, s2, n, opA, Bm, s2, n, opB, C, s2, n, al, be, B, act), stream_);
+    GemmArgs<double> g = G(n, n, n, A, s2, n, opA, Bm, s2, n, opB, C, s2, n, al, be, B, act);
+    g.tag = PT_GEMM_SWEEP;
+    g.prof_items = n_active;
+    gemm_z(g, stream_);
   };
 
   for (int pass = 0; pass <= so.niter; ++pass) {
@@ -830,6 +876,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     HN_CHECK(hipMemcpyAsync(h_count_, dcount, sizeof(int), hipMemcpyDeviceToHost, stream_));
     HN_CHECK(hipStreamSynchronize(stream_));
     if (*h_count_ == 0) break;
+    n_active = *h_count_;
     // ---- gsmr (nbl -> 2) ----
     if (nbl > 1) {
       form_diag(nbl - 1, D);

@@ -26,6 +26,7 @@ struct GemmArgs {
   int skip_c0 = 0, skip_c1 = 0;
   int batch = 1;
   int tag = -1;  // profile tag (ProfTag); -1: masked if `active` is set, else sweep
+  int prof_items = -1;  // items known to be active (for flop counts of masked launches)
 };
 
 void gemm_z(const GemmArgs<double> &a, hipStream_t s);  // complex FP64

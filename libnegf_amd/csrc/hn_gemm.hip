@@ -251,12 +251,14 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   // algorithmic work: 8 real flops per complex multiply-add over the columns written
   const int skipped = std::max(0, std::min(a.skip_c1, a.n) - std::max(a.skip_c0, 0));
   const double ncol = (double)(a.n - skipped);
-  const double flops = (a.active || a.active2) ? -1.0 : 8.0 * a.m * ncol * a.k * a.batch;
+  const double items = a.prof_items >= 0 ? (double)a.prof_items
+                      : ((a.active || a.active2) ? -1.0 : (double)a.batch);
+  const double flops = items < 0 ? -1.0 : 8.0 * a.m * ncol * a.k * items;
   const double cb = (double)sizeof(typename GemmArgs<R>::CT);
   const bool rc = a.beta.x != 0.0 || a.beta.y != 0.0;
-  const double bytes = cb * ((a.sA ? (double)a.batch : 1.0) * a.m * a.k +
-                             (a.sB ? (double)a.batch : 1.0) * a.k * a.n +
-                             (rc ? 2.0 : 1.0) * a.batch * a.m * ncol);
+  const double nb = items < 0 ? (double)a.batch : items;
+  const double bytes = cb * ((a.sA ? nb : 1.0) * a.m * a.k + (a.sB ? nb : 1.0) * a.k * a.n +
+                             (rc ? 2.0 : 1.0) * nb * a.m * ncol);
   ProfScope ps(use, flops, bytes, s);
   if (use == PT_GEMM_SWEEP)
     hipLaunchKernelGGL((k_gemm<R, PT_GEMM_SWEEP>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,

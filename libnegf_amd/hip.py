@@ -184,6 +184,24 @@ class DeviceStructure:
         for k, s in enumerate(structs):
             self.set_slot(k, s)
 
+    @classmethod
+    def empty(cls, n: int, nbl: int, cblk=(None, 1), smode: int = 0, nslots: int = 1):
+        """Zero-filled structure of nbl PLs of size n with contacts of size n on
+        blocks ``cblk`` (to be filled on the device, e.g. by a generator)."""
+        from types import SimpleNamespace
+        self = cls.__new__(cls)
+        cb = [nbl if c is None else c for c in cblk]
+        self.template = SimpleNamespace(ncont=len(cb), nbl=nbl, bsz=[n] * nbl, ndev=n * nbl)
+        self.nslots = nslots
+        self.smode = smode
+        bsz = np.full(nbl, n, dtype=np.int32)
+        cba = np.array(cb, dtype=np.int32)
+        nc = np.full(len(cb), n, dtype=np.int32)
+        self._h = C.c_void_p()
+        check(load().hipnegf_struct_create(nbl, _ptr(bsz), len(cba), _ptr(cba), _ptr(nc), smode,
+                                           nslots, C.byref(self._h)))
+        return self
+
     def set_block(self, slot: int, kind: int, index: int, a: np.ndarray):
         a = np.asfortranarray(a, dtype=np.complex128)
         check(load().hipnegf_struct_set_block(self._h, slot, kind, index, _ptr(a)))

@@ -203,6 +203,19 @@ def config3(n: Optional[int] = None, nbl: Optional[int] = None, with_overlap: bo
     return slab, mp_grid(c["kgrid"]), energy_grid(c["nE"], c["Emin"], c["Emax"]), c["eta"]
 
 
+def serving_spec(seed: int, n_choices=(32, 64, 128, 256, 512), nbl_range=(10, 200),
+                 nE_range=(50, 2000)):
+    """C5 request shape from its seed: (n, nbl, nE, kx, ky) -- the same draws as
+    ``serving_request``, without building the blocks (the serving loop
+    generates them in HBM from the seed)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    n = int(rng.choice(n_choices))
+    nbl = int(rng.integers(nbl_range[0], nbl_range[1] + 1))
+    nE = int(rng.integers(nE_range[0], nE_range[1] + 1))
+    kx, ky = rng.uniform(-np.pi, np.pi, 2)
+    return n, nbl, nE, float(kx), float(ky)
+
+
 def serving_request(seed: int, n_choices=(32, 64, 128, 256, 512), nbl_range=(10, 200),
                     nE_range=(50, 2000)):
     """C5: one request = (KSlab at a single k, energy grid), from a per-request seed."""

@@ -1,0 +1,112 @@
+"""Serving loop (hipnegf_server_*) and on-device slab generation.
+
+Bar: generated structures solve to the oracle's T within 1e-10 relative; a
+request's T(E) and decimation counts are bitwise identical alone, in a mix of
+concurrent requests (both scheduling policies) and through the direct batched
+entry point.
+"""
+import numpy as np
+import pytest
+
+from libnegf_amd import workloads as wl
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from libnegf_amd import hip, serve
+    return hip, serve, hip.Engine(0)
+
+
+def _requests():
+    rng = np.random.default_rng(11)
+    out = []
+    for i in range(7):
+        n = int(rng.choice([32, 48, 64, 96]))
+        nbl = int(rng.integers(3, 9))
+        nE = int(rng.integers(15, 60))
+        k = tuple(rng.uniform(-np.pi, np.pi, 2))
+        out.append(dict(seed=500 + i, n=n, nbl=nbl, E=np.linspace(-1.5, 1.5, nE), k=k))
+    return out
+
+
+def test_generated_slab_structure_and_parity(gpu):
+    hip, serve, eng = gpu
+    from oracle import pyoracle
+    ds, s = serve.kslab_structure(eng, seed=7, n=40, nbl=5, k=(0.3, -1.2), download=True)
+    for h in s.Hd:
+        np.testing.assert_allclose(h, h.conj().T, atol=1e-15)
+    for u, l in zip(s.Hu, s.Hl):
+        np.testing.assert_array_equal(l, u.T)
+        assert np.isrealobj(u) or np.abs(u.imag).max() == 0.0
+    c1, c2 = s.contacts
+    np.testing.assert_array_equal(c2.H01, c1.H01.conj().T)
+    np.testing.assert_array_equal(c2.S01, c1.S01.conj().T)
+    for sd in s.Sd:  # S(k) Hermitian positive definite
+        assert np.linalg.eigvalsh(sd).min() > 0.3
+    # on-site sigma 0.3 (symmetrised) + Bloch terms: a loose distribution check
+    assert 0.2 < np.std(s.Hd[0].real) < 0.5
+    E = np.array([-0.8, 0.2, 1.1])
+    T, ncyc, st = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    pyoracle.build()
+    pyoracle.set_backend("builtin")
+    Tref, nref, _ = pyoracle.transmission_batch(s, E, 1e-5)
+    assert (st == 0).all()
+    assert np.abs(T - Tref).max() <= 1e-10 * max(Tref.max(), 1e-6)
+    assert (ncyc == nref).all()
+
+
+@pytest.mark.parametrize("policy", [0, 1])
+def test_alone_equals_mix(gpu, policy):
+    hip, serve, eng = gpu
+    reqs = _requests()
+    srv = serve.Server(workers=3, policy=policy, max_chunk=8)
+    alone = []
+    for r in reqs:
+        rid = srv.submit_kslab(r["seed"], r["n"], r["nbl"], r["E"], k=r["k"])
+        alone.append(srv.wait(rid))
+    ids = [srv.submit_kslab(r["seed"], r["n"], r["nbl"], r["E"], k=r["k"]) for r in reqs]
+    mixed = [srv.wait(i) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == 2 * len(reqs) and st["running"] == 0 and st["queued"] == 0
+    for a, m, r in zip(alone, mixed, reqs):
+        assert (a["status"] == 0).all()
+        np.testing.assert_array_equal(a["T"], m["T"])
+        np.testing.assert_array_equal(a["ncyc"], m["ncyc"])
+        assert m["chunks"] == -(-len(r["E"]) // 8)
+    # and the same numbers through the direct batched entry point
+    r = reqs[2]
+    ds, _ = serve.kslab_structure(eng, r["seed"], r["n"], r["nbl"], k=r["k"])
+    T, ncyc, _ = hip.solve_transmission(eng, ds, r["E"], delta=1e-5)
+    np.testing.assert_array_equal(T, alone[2]["T"])
+    np.testing.assert_array_equal(ncyc, alone[2]["ncyc"])
+
+
+def test_resident_structure_requests(gpu):
+    hip, serve, eng = gpu
+    s = wl.ribbon(24, 10, disorder=0.5, seed=41)
+    ds = hip.DeviceStructure(s)
+    grids = [np.linspace(-2.5, 2.5, 23), np.linspace(-1.0, 0.5, 17), np.linspace(0.1, 2.9, 31)]
+    srv = serve.Server(workers=2)
+    ids = [srv.submit(ds, E, delta=1e-5) for E in grids]
+    res = [srv.wait(i) for i in ids]
+    srv.close()
+    for E, r in zip(grids, res):
+        T, ncyc, _ = hip.solve_transmission(eng, ds, E, delta=1e-5)
+        np.testing.assert_array_equal(r["T"], T)
+        np.testing.assert_array_equal(r["ncyc"], ncyc)
+        assert r["latency"] >= r["t_queue"] >= 0.0
+
+
+def test_admission_rejects_oversized(gpu):
+    hip, serve, eng = gpu
+    srv = serve.Server(workers=1, mem_budget=(64 << 20) + (1 << 20), worker_workspace=64 << 20)
+    rid = srv.submit_kslab(3, 64, 12, np.linspace(-1, 1, 5))
+    with pytest.raises(hip.HipNegfError, match="HBM budget"):
+        srv.wait(rid)
+    srv.close()
