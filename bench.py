@@ -36,6 +36,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-items", type=int, default=4,
                     help="items timed one at a time (unbatched FP64 reference mode)")
+    ap.add_argument("--lanes", type=int, default=1,
+                    help="item groups solved concurrently on separate streams")
     ap.add_argument("--no-profile", action="store_true",
                     help="no per-launch HIP events in the timed region (no roofline field)")
     ap.add_argument("--copy-calib", action="store_true",
@@ -293,7 +295,7 @@ def main():
         s0 = structs[0]
         workload = dict(workload="kslab_n256_nbl40_k16x16_E500", n=slab.n, nbl=s0.nbl,
                         kgrid="16x16", nE=len(Egrid), eta=eta, overlap=True,
-                        items_per_step=per_step)
+                        items_per_step=per_step, lanes=a.lanes)
     else:
         s0, Egrid, eta = wl.config1() if a.config == "c1" else wl.config2(nbl=a.nbl or None)
         ds = hip.DeviceStructure(s0)
@@ -314,7 +316,7 @@ def main():
     dT = torch.zeros(per_step, dtype=torch.float64, device=dev)
     dN = torch.zeros(per_step * ncont, dtype=torch.int32, device=dev)
     dSt = torch.zeros(per_step, dtype=torch.int32, device=dev)
-    opts = hip._opts(eta, 1, 2, prec, 0)
+    opts = hip._opts(eta, 1, 2, prec, 0, lanes=a.lanes)
     import ctypes as C
 
     def step(i):

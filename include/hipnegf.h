@@ -180,6 +180,7 @@ typedef struct {
   int polish;        /* mixed: FP64 decimation iterations after the FP32 ones  */
   double eps_drop;   /* EPS drop tolerance (ln_precision), default 0           */
   int flags;         /* hipnegf_opt_flags                                      */
+  int lanes;         /* concurrent item groups (own stream each); 0/1: one     */
 } hipnegf_opts;
 /* HIPNEGF_NO_SHARED_DECIMATION: run one Sancho-Rubio recursion per contact
  * even when two contacts are mirror images (by default such a pair shares one

@@ -246,6 +246,7 @@ hn::SolveOpts to_opts(const hipnegf_opts *o) {
     if (o->polish > 0) so.polish = o->polish;
     so.eps_drop = o->eps_drop;
     so.share_mirror = (o->flags & HIPNEGF_NO_SHARED_DECIMATION) ? 0 : 1;
+    so.lanes = std::max(1, std::min(o->lanes, 8));
   }
   return so;
 }

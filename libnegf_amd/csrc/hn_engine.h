@@ -84,6 +84,7 @@ struct SolveOpts {
   int polish = 2;            // mixed precision: FP64 decimation polish iterations
   int dec_crit = 0;          // 0: max|C1| (CPU path), 1: sum|Re|+|Im| (GPU path)
   int share_mirror = 1;      // one decimation for mirror-image contact pairs
+  int lanes = 1;             // item groups solved concurrently, each on its own stream
 };
 
 struct SCBAOpts {
@@ -131,6 +132,8 @@ class Engine {
                       const double2 *hCo, double acc, int crit, int *ncyc);
 
   int last_batch() const { return last_batch_; }
+  // Sub-engines (own stream and workspace) for concurrent item groups.
+  Engine &lane(int i);
   size_t workspace_bytes() const { return arena_.cap; }
 
  private:
@@ -139,6 +142,7 @@ class Engine {
   Arena arena_;
   int *h_count_ = nullptr;  // pinned
   int last_batch_ = 0;
+  std::vector<std::unique_ptr<Engine>> lanes_;
 
   int pick_batch(const DevStruct &st, int nitems, const SolveOpts &o, size_t per_item) const;
   void contacts(DevStruct &st, int B, const double *dE, const int *dslot,
@@ -152,7 +156,8 @@ class Engine {
   void decimate_mixed(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, double2 *Co,
                       double2 *Go, const SolveOpts &o, int *ncyc, int *status);
   void inverse(int n, double2 *A, long long sA, double2 *Ainv, long long sAinv, int batch,
-               const int *active, int *status, int precision, const SolveOpts &o);
+               const int *active, int *status, int precision, const SolveOpts &o,
+               int prof_items = -1);
   void t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                const SolveOpts &o, int npairs, const int *ni, const int *nf, double *dT,
                int *dncyc, int *dstatus);

@@ -15,6 +15,8 @@
 // handled with device-side activity masks; kernels skip finished items.
 #include <algorithm>
 #include <cstring>
+#include <string>
+#include <thread>
 
 #include "hn_engine.h"
 #include "hn_gemm.h"
@@ -158,6 +160,11 @@ Engine::Engine(int device) : device_(device) {
   HN_CHECK(hipHostMalloc(&h_count_, sizeof(int) * 4, hipHostMallocDefault));
 }
 
+Engine &Engine::lane(int i) {
+  while ((int)lanes_.size() <= i) lanes_.emplace_back(new Engine(device_));
+  return *lanes_[i];
+}
+
 Engine::~Engine() {
   (void)hipStreamSynchronize(stream_);
   if (h_count_) (void)hipHostFree(h_count_);
@@ -223,7 +230,8 @@ int Engine::pick_batch(const DevStruct &st, int nitems, const SolveOpts &o,
 }
 
 void Engine::inverse(int n, double2 *A, long long sA, double2 *Ainv, long long sAinv, int batch,
-                     const int *active, int *status, int precision, const SolveOpts &o) {
+                     const int *active, int *status, int precision, const SolveOpts &o,
+                     int prof_items) {
   (void)precision;
   (void)o;
   InvWork<double> w;
@@ -231,6 +239,7 @@ void Engine::inverse(int n, double2 *A, long long sA, double2 *Ainv, long long s
     size_t mark = arena_.used;
     void *buf = arena_.take<char>(inv_work_bytes(n, batch, 8));
     w = inv_work_carve<double>(buf, n, batch);
+    w.prof_items = active ? prof_items : -1;
     inv_z(n, A, sA, Ainv, sAinv, batch, active, status, w, stream_);
     arena_.used = mark;
     return;
@@ -255,28 +264,48 @@ void Engine::decimate_z(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, 
   fill_int(okco, CB, 0, stream_);
   fill_int(upd, CB, 0, stream_);
   const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+  // active-item counts for the flop counts of the masked launches (profiles only)
+  const bool prof = profiler().on();
+  int n_act = CB;
+  std::vector<std::pair<int, int>> deferred;  // (iteration, profile token) of the update launches
+  auto mg = [&](GemmArgs<double> g, int items, int it) {
+    int tok = -1;
+    g.prof_items = items;
+    g.prof_token = &tok;
+    gemm_z(g, stream_);
+    if (prof && items == -2 && tok >= 0) deferred.emplace_back(it, tok);
+  };
   for (int it = 1; it <= o.dec_maxit; ++it) {
     copy_batched(M, s2, Ao, s2, s2, CB, act, stream_);
-    inverse(nc, M, s2, Go, s2, CB, act, status, PREC_FP64, o);
-    gemm_z(G(nc, nc, nc, Go, s2, nc, OP_N, Co, s2, nc, OP_N, GoXCo, s2, nc, one, zero, CB, act), stream_);
-    gemm_z(G(nc, nc, nc, Co, s2, nc, OP_N, GoXCo, s2, nc, OP_N, C1, s2, nc, one, zero, CB, act), stream_);
+    inverse(nc, M, s2, Go, s2, CB, act, status, PREC_FP64, o, n_act);
+    mg(G(nc, nc, nc, Go, s2, nc, OP_N, Co, s2, nc, OP_N, GoXCo, s2, nc, one, zero, CB, act), n_act, it);
+    mg(G(nc, nc, nc, Co, s2, nc, OP_N, GoXCo, s2, nc, OP_N, C1, s2, nc, one, zero, CB, act), n_act, it);
     decim_check(C1, s2, (int)s2, CB, o.sgfacc, it, o.dec_maxit, act, upd, okco, ncyc, status,
                 o.dec_crit, stream_);
     {
       GemmArgs<double> g = G(nc, nc, nc, Bo, s2, nc, OP_N, GoXCo, s2, nc, OP_N, Ao, s2, nc,
                              mone, one, CB, upd);
       g.C2 = Aos; g.sC2 = s2; g.ldc2 = nc; g.beta2 = one;  // Ao_s -= Self; Ao -= Self
-      gemm_z(g, stream_);
+      mg(g, -2, it);
     }
-    gemm_z(G(nc, nc, nc, Go, s2, nc, OP_N, Bo, s2, nc, OP_N, GoXBo, s2, nc, one, zero, CB, upd), stream_);
-    gemm_z(G(nc, nc, nc, Bo, s2, nc, OP_N, GoXBo, s2, nc, OP_N, B1, s2, nc, one, zero, CB, upd), stream_);
-    gemm_z(G(nc, nc, nc, Co, s2, nc, OP_N, GoXBo, s2, nc, OP_N, Ao, s2, nc, mone, one, CB, upd), stream_);
+    mg(G(nc, nc, nc, Go, s2, nc, OP_N, Bo, s2, nc, OP_N, GoXBo, s2, nc, one, zero, CB, upd), -2, it);
+    mg(G(nc, nc, nc, Bo, s2, nc, OP_N, GoXBo, s2, nc, OP_N, B1, s2, nc, one, zero, CB, upd), -2, it);
+    mg(G(nc, nc, nc, Co, s2, nc, OP_N, GoXBo, s2, nc, OP_N, Ao, s2, nc, mone, one, CB, upd), -2, it);
     std::swap(Bo, B1);  // Bo = B1
     std::swap(Co, C1);  // Co = C1
     count_nonzero(act, CB, dcount, stream_);
     HN_CHECK(hipMemcpyAsync(h_count_, dcount, sizeof(int), hipMemcpyDeviceToHost, stream_));
     HN_CHECK(hipStreamSynchronize(stream_));
-    if (*h_count_ == 0) break;
+    n_act = *h_count_;
+    if (n_act == 0) break;
+  }
+  if (prof && !deferred.empty()) {
+    // items that ran the update part of iteration it: ncyc > it (exit iterations skip it)
+    std::vector<int> h(CB);
+    HN_CHECK(hipMemcpyAsync(h.data(), ncyc, sizeof(int) * CB, hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+    for (auto &d : deferred)
+      profiler().scale(d.second, (double)std::count_if(h.begin(), h.end(), [&](int v) { return v > d.first; }));
   }
   inverse(nc, Aos, s2, Go, s2, CB, nullptr, status, PREC_FP64, o);
   arena_.used = mark;
@@ -334,14 +363,24 @@ void Engine::decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 
   fill_int(ok0, B, 0, stream_);
   fill_int(ok1, B, 0, stream_);
   const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+  const bool prof = profiler().on();
+  int n_act = B;
+  std::vector<std::pair<int, int>> deferred;  // (iteration, profile token) of the update launches
+  auto mg = [&](GemmArgs<double> g, int items, int it) {
+    int tok = -1;
+    g.prof_items = items;
+    g.prof_token = &tok;
+    gemm_z(g, stream_);
+    if (prof && items == -2 && tok >= 0) deferred.emplace_back(it, tok);
+  };
   for (int it = 1; it <= o.dec_maxit; ++it) {
     or_flags(act0, act1, act, B, stream_);
     copy_batched(M, s2, Ao, s2, s2, B, act, stream_);
-    inverse(nc, M, s2, Go0, s2, B, act, status0, PREC_FP64, o);
-    gemm_z(G(nc, nc, nc, Go0, s2, nc, OP_N, Co, s2, nc, OP_N, GoXCo, s2, nc, one, zero, B, act), stream_);
-    gemm_z(G(nc, nc, nc, Co, s2, nc, OP_N, GoXCo, s2, nc, OP_N, C1, s2, nc, one, zero, B, act), stream_);
-    gemm_z(G(nc, nc, nc, Go0, s2, nc, OP_N, Bo, s2, nc, OP_N, GoXBo, s2, nc, one, zero, B, act), stream_);
-    gemm_z(G(nc, nc, nc, Bo, s2, nc, OP_N, GoXBo, s2, nc, OP_N, B1, s2, nc, one, zero, B, act), stream_);
+    inverse(nc, M, s2, Go0, s2, B, act, status0, PREC_FP64, o, n_act);
+    mg(G(nc, nc, nc, Go0, s2, nc, OP_N, Co, s2, nc, OP_N, GoXCo, s2, nc, one, zero, B, act), n_act, it);
+    mg(G(nc, nc, nc, Co, s2, nc, OP_N, GoXCo, s2, nc, OP_N, C1, s2, nc, one, zero, B, act), n_act, it);
+    mg(G(nc, nc, nc, Go0, s2, nc, OP_N, Bo, s2, nc, OP_N, GoXBo, s2, nc, one, zero, B, act), n_act, it);
+    mg(G(nc, nc, nc, Bo, s2, nc, OP_N, GoXBo, s2, nc, OP_N, B1, s2, nc, one, zero, B, act), n_act, it);
     decim_check(C1, s2, (int)s2, B, o.sgfacc, it, o.dec_maxit, act0, upd0, ok0, ncyc0, status0,
                 o.dec_crit, stream_);
     decim_check(B1, s2, (int)s2, B, o.sgfacc, it, o.dec_maxit, act1, upd1, ok1, ncyc1, status1,
@@ -351,13 +390,13 @@ void Engine::decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 
       GemmArgs<double> g = G(nc, nc, nc, Bo, s2, nc, OP_N, GoXCo, s2, nc, OP_N, Ao, s2, nc, mone,
                              one, B, upd);
       g.C2 = Aos0; g.sC2 = s2; g.ldc2 = nc; g.beta2 = one; g.active2 = upd0;
-      gemm_z(g, stream_);
+      mg(g, -2, it);
     }
     {  // Ao -= Co GoXBo ; Ao_s(1) -= Co GoXBo
       GemmArgs<double> g = G(nc, nc, nc, Co, s2, nc, OP_N, GoXBo, s2, nc, OP_N, Ao, s2, nc, mone,
                              one, B, upd);
       g.C2 = Aos1; g.sC2 = s2; g.ldc2 = nc; g.beta2 = one; g.active2 = upd1;
-      gemm_z(g, stream_);
+      mg(g, -2, it);
     }
     std::swap(Bo, B1);
     std::swap(Co, C1);
@@ -365,7 +404,20 @@ void Engine::decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 
     count_nonzero(act, B, dcount, stream_);
     HN_CHECK(hipMemcpyAsync(h_count_, dcount, sizeof(int), hipMemcpyDeviceToHost, stream_));
     HN_CHECK(hipStreamSynchronize(stream_));
-    if (*h_count_ == 0) break;
+    n_act = *h_count_;
+    if (n_act == 0) break;
+  }
+  if (prof && !deferred.empty()) {
+    // items that ran the update part of iteration it: either contact has ncyc > it
+    std::vector<int> h0(B), h1(B);
+    HN_CHECK(hipMemcpyAsync(h0.data(), ncyc0, sizeof(int) * B, hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipMemcpyAsync(h1.data(), ncyc1, sizeof(int) * B, hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+    for (auto &d : deferred) {
+      int cnt = 0;
+      for (int b = 0; b < B; ++b) cnt += (std::max(h0[b], h1[b]) > d.first) ? 1 : 0;
+      profiler().scale(d.second, (double)cnt);
+    }
   }
   inverse(nc, Aos0, s2, Go0, s2, B, nullptr, status0, PREC_FP64, o);
   inverse(nc, Aos1, s2, Go1, s2, B, nullptr, status1, PREC_FP64, o);
@@ -733,6 +785,35 @@ void Engine::transmission_pairs(DevStruct &st, int nitems, const double *dE, con
     if (ni[p] < 1 || ni[p] > st.ncont || nf[p] < 1 || nf[p] > st.ncont || ni[p] == nf[p])
       if (st.ncont >= 2) throw std::runtime_error("transmission: bad contact pair");
   HN_CHECK(hipSetDevice(device_));
+  if (o.lanes > 1 && nitems >= 2 * o.lanes) {
+    // Item groups on concurrent streams: one group's latency-bound steps (panel
+    // factorisations, convergence polls) overlap the other groups' GEMMs.
+    prepare(st);
+    SolveOpts lo = o;
+    lo.lanes = 1;
+    if (lo.max_batch <= 0) lo.max_batch = std::max(1, (nitems + o.lanes - 1) / o.lanes);
+    const int L = o.lanes, per = (nitems + L - 1) / L;
+    std::vector<std::thread> th;
+    std::vector<std::string> err(L);
+    for (int l = 0; l < L; ++l) {
+      const int i0 = l * per, cnt = std::min(per, nitems - i0);
+      if (cnt <= 0) break;
+      Engine *e = &lane(l);  // created here, on the calling thread
+      th.emplace_back([&, l, i0, cnt, e]() {
+        try {
+          e->transmission_pairs(st, cnt, dE + i0, dslot + i0, lo, npairs, ni, nf,
+                               dT + (long long)i0 * npairs,
+                               dncyc ? dncyc + (long long)i0 * st.ncont : nullptr, dstatus + i0);
+        } catch (const std::exception &x) {
+          err[l] = x.what();
+        }
+      });
+    }
+    for (auto &t : th) t.join();
+    for (auto &m : err)
+      if (!m.empty()) throw std::runtime_error(m);
+    return;
+  }
   const size_t per_item = transmission_item_bytes(st, npairs, ni, nf);
   const int B = pick_batch(st, nitems, o, per_item);
   last_batch_ = B;
@@ -880,13 +961,13 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     // ---- gsmr (nbl -> 2) ----
     if (nbl > 1) {
       form_diag(nbl - 1, D);
-      inverse(n, D, s2, blk(g, nbl - 1), s2, B, act, dstatus, PREC_FP64, o);
+      inverse(n, D, s2, blk(g, nbl - 1), s2, B, act, dstatus, PREC_FP64, o, n_active);
       for (int i = nbl - 2; i >= 1; --i) {
         form_off(i, U, L);
         form_diag(i, D);
         mm(U, OP_N, blk(g, i + 1), OP_N, W1, mone, zero);
         mm(W1, OP_N, L, OP_N, D, one, one);
-        inverse(n, D, s2, blk(g, i), s2, B, act, dstatus, PREC_FP64, o);
+        inverse(n, D, s2, blk(g, i), s2, B, act, dstatus, PREC_FP64, o, n_active);
       }
     }
     // ---- Gr(1,1) ----
@@ -896,7 +977,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
       mm(U, OP_N, blk(g, 1), OP_N, W1, mone, zero);
       mm(W1, OP_N, L, OP_N, D, one, one);
     }
-    inverse(n, D, s2, blk(Grd, 0), s2, B, act, dstatus, PREC_FP64, o);
+    inverse(n, D, s2, blk(Grd, 0), s2, B, act, dstatus, PREC_FP64, o, n_active);
     // ---- Gr tridiagonal blocks, i = 2..nbl (1-based) ----
     for (int i = 1; i < nbl; ++i) {
       form_off(i - 1, U, L);  // U = ESH(i-1,i), L = ESH(i,i-1)

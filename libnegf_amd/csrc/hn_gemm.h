@@ -26,7 +26,9 @@ struct GemmArgs {
   int skip_c0 = 0, skip_c1 = 0;
   int batch = 1;
   int tag = -1;  // profile tag (ProfTag); -1: masked if `active` is set, else sweep
-  int prof_items = -1;  // items known to be active (for flop counts of masked launches)
+  int prof_items = -1;  // items known to be active (for flop counts of masked launches);
+                        // -2: count later (flops recorded per item, see Profiler::scale)
+  int *prof_token = nullptr;  // out: profile record of this launch (-1 if none)
 };
 
 void gemm_z(const GemmArgs<double> &a, hipStream_t s);  // complex FP64

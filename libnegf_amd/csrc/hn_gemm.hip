@@ -252,6 +252,7 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   const int skipped = std::max(0, std::min(a.skip_c1, a.n) - std::max(a.skip_c0, 0));
   const double ncol = (double)(a.n - skipped);
   const double items = a.prof_items >= 0 ? (double)a.prof_items
+                      : a.prof_items == -2 ? 1.0
                       : ((a.active || a.active2) ? -1.0 : (double)a.batch);
   const double flops = items < 0 ? -1.0 : 8.0 * a.m * ncol * a.k * items;
   const double cb = (double)sizeof(typename GemmArgs<R>::CT);
@@ -260,6 +261,7 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   const double bytes = cb * ((a.sA ? nb : 1.0) * a.m * a.k + (a.sB ? nb : 1.0) * a.k * a.n +
                              (rc ? 2.0 : 1.0) * nb * a.m * ncol);
   ProfScope ps(use, flops, bytes, s);
+  if (a.prof_token) *a.prof_token = ps.tok;
   if (use == PT_GEMM_SWEEP)
     hipLaunchKernelGGL((k_gemm<R, PT_GEMM_SWEEP>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
                        tiles_n, (int)total, total_pad);

@@ -20,6 +20,7 @@ struct InvWork {
   int *used = nullptr;  // n per item
   int *piv = nullptr;   // n per item
   int n = 0, batch = 0;
+  int prof_items = -1;  // active items (flop counts of masked launches in profiles)
 };
 size_t inv_work_bytes(int n, int batch, int real_bytes);
 // carve the workspace out of a device buffer of inv_work_bytes() bytes

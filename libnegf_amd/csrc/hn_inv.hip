@@ -174,11 +174,14 @@ __global__ __launch_bounds__(NT) void k_inv_reset(int n, int batch, int *used,
 // with e_j, giving row j of Q^-1), so Q is never inverted explicitly.
 template <typename R, int NB, int RPT>
 __global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, int rb_row,
-                                                      typename CT_<R>::t *A, long long sA,
-                                                      typename CT_<R>::t *Z,
-                                                      typename CT_<R>::t *Rb, int *used_g,
-                                                      int *piv_g, const int *active,
-                                                      int *status) {
+                                                      typename CT_<R>::t *__restrict__ A,
+                                                      long long sA,
+                                                      typename CT_<R>::t *__restrict__ Z,
+                                                      typename CT_<R>::t *__restrict__ Rb,
+                                                      int *__restrict__ used_g,
+                                                      int *__restrict__ piv_g,
+                                                      const int *__restrict__ active,
+                                                      int *__restrict__ status) {
   typedef typename CT_<R>::t C;
   const int b = blockIdx.x;
   if (active && active[b] == 0) return;
@@ -260,12 +263,32 @@ __global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, in
   }
   __syncthreads();
 
-  // ---- copy of the pivot rows outside the panel columns ----
-  for (int e = tid; e < nbp * n; e += NT) {
-    const int j = e % nbp, c = e / nbp;
-    if (c >= c0 && c < c0 + nbp) continue;
-    Rbb[j + (size_t)c * kInvSuper] = Ab[piv[j] + (size_t)c * n];
+  // ---- copy of the pivot rows outside the panel columns.  Element e is
+  // (pivot j = e % nbp, column c = e / nbp): a wave reads the pivot rows of a
+  // few adjacent columns.  Each thread issues all its loads before storing. ----
+  {
+    constexpr int kPer = 16;  // elements per thread per round
+    const int total = nbp * n;
+    for (int base = 0; base < total; base += NT * kPer) {
+      C v[kPer];
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int e = base + tid + q * NT;
+        const int j = e % nbp, c = e / nbp;
+        v[q] = (e < total) ? Ab[piv[j] + (size_t)c * n] : C{0, 0};
+      }
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const int e = base + tid + q * NT;
+        const int j = e % nbp, c = e / nbp;
+        if (e < total && (c < c0 || c >= c0 + nbp)) Rbb[j + (size_t)c * kInvSuper] = v[q];
+      }
+    }
   }
+  // reciprocals of the pivots U(l,l), once
+  __shared__ C dinv[NB];
+  if (tid < nbp) dinv[tid] = crecip(LU[tid][tid]);
+  __syncthreads();
 
   // ---- W Q = x  with Q = L U:  y U = x (forward), W L = y (backward) ----
 #pragma unroll
@@ -285,7 +308,7 @@ __global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, in
         C acc = P[q][l];
 #pragma unroll
         for (int m = 0; m < l; ++m) acc = csub(acc, cmul(P[q][m], LU[m][l]));
-        P[q][l] = cmul(acc, crecip(LU[l][l]));
+        P[q][l] = cmul(acc, dinv[l]);
       }
     }
 #pragma unroll
@@ -440,7 +463,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
         g.C = A + (size_t)s0 * n; g.sC = sA; g.ldc = n;
         g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
         g.active = active; g.skip_c0 = c0 - s0; g.skip_c1 = c0 - s0 + nbp;
-        g.batch = batch; g.tag = PT_GEMM_INV;
+        g.batch = batch; g.tag = PT_GEMM_INV; g.prof_items = w.prof_items;
         if constexpr (sizeof(R) == 8) gemm_z(g, s);
         else gemm_c(g, s);
       }
@@ -456,7 +479,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
       g.C = A; g.sC = sA; g.ldc = n;
       g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
       g.active = active; g.skip_c0 = s0; g.skip_c1 = s0 + sw;
-      g.batch = batch; g.tag = PT_GEMM_INV;
+      g.batch = batch; g.tag = PT_GEMM_INV; g.prof_items = w.prof_items;
       if constexpr (sizeof(R) == 8) gemm_z(g, s);
       else gemm_c(g, s);
     }

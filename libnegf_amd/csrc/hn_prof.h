@@ -27,6 +27,8 @@ class Profiler {
   bool on() const { return on_; }
   int begin(int tag, double flops, double bytes, hipStream_t s);
   void end(int token, hipStream_t s);
+  // multiply a launch's recorded flops and bytes (recorded per item) by its item count
+  void scale(int token, double items);
   // Waits for the recorded events; sums over launches of `tag`.
   void summary(int tag, double *ms, double *flops, double *bytes, long long *launches);
   void clear();

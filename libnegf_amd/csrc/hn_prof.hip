@@ -39,6 +39,13 @@ void Profiler::end(int token, hipStream_t s) {
   (void)hipEventRecord(recs_[token].b, s);
 }
 
+void Profiler::scale(int token, double items) {
+  std::lock_guard<std::mutex> g(mu_);
+  if (token < 0 || token >= (int)recs_.size()) return;
+  recs_[token].flops *= items;
+  recs_[token].bytes *= items;
+}
+
 void Profiler::summary(int tag, double *ms, double *flops, double *bytes, long long *launches) {
   std::lock_guard<std::mutex> g(mu_);
   double t = 0.0, f = 0.0, by = 0.0;

@@ -32,7 +32,8 @@ class HipNegfError(RuntimeError):
 class Opts(C.Structure):
     _fields_ = [("delta", C.c_double), ("ni", C.c_int), ("nf", C.c_int),
                 ("precision", C.c_int), ("max_batch", C.c_int), ("ns_steps", C.c_int),
-                ("polish", C.c_int), ("eps_drop", C.c_double), ("flags", C.c_int)]
+                ("polish", C.c_int), ("eps_drop", C.c_double), ("flags", C.c_int),
+                ("lanes", C.c_int)]
 NO_SHARED_DECIMATION = 1
 
 
@@ -154,8 +155,8 @@ class Engine:
 
 
 def _opts(delta: float, ni: int, nf: int, precision: int, max_batch: int, ns_steps=0,
-          polish=0, eps_drop=0.0, flags=0) -> Opts:
-    return Opts(delta, ni, nf, precision, max_batch, ns_steps, polish, eps_drop, flags)
+          polish=0, eps_drop=0.0, flags=0, lanes=1) -> Opts:
+    return Opts(delta, ni, nf, precision, max_batch, ns_steps, polish, eps_drop, flags, lanes)
 
 
 class DeviceStructure:
@@ -247,7 +248,7 @@ class DeviceStructure:
 
 def solve_transmission(eng: Engine, ds: DeviceStructure, E: np.ndarray, slot=None,
                        delta: float = 1e-4, ni: int = 1, nf: int = 2, precision: int = FP64,
-                       max_batch: int = 0, flags: int = 0):
+                       max_batch: int = 0, flags: int = 0, lanes: int = 1):
     """Host-array batched transmission.  Returns (T, ncyc[n, ncont], status)."""
     E = np.ascontiguousarray(E, dtype=np.float64)
     n = len(E)
@@ -256,7 +257,7 @@ def solve_transmission(eng: Engine, ds: DeviceStructure, E: np.ndarray, slot=Non
     T = np.zeros(n)
     ncyc = np.zeros((n, max(ncont, 1)), dtype=np.int32)
     st = np.zeros(n, dtype=np.int32)
-    o = _opts(delta, ni, nf, precision, max_batch, flags=flags)
+    o = _opts(delta, ni, nf, precision, max_batch, flags=flags, lanes=lanes)
     check(load().hipnegf_solve_transmission(eng.handle, ds.handle, n, _ptr(E), _ptr(sl),
                                             C.byref(o), _ptr(T), _ptr(ncyc), _ptr(st)))
     return T, ncyc, st

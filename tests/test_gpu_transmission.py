@@ -225,3 +225,18 @@ def test_fast_path_equals_multiterminal(gpu):
     Tm, _, _ = hip.solve_transmission_pairs(eng, ds, E, [(1, 2), (2, 1)], delta=1e-5)
     _compare(Tm[:, 0], Tf)
     _compare(Tm[:, 1], Tf)
+
+
+def test_lanes_bitwise_equal(gpu):
+    """Concurrent item groups (lanes, one stream each) give bitwise the results
+    of one group."""
+    hip, eng = gpu
+    s = wl.KSlab.make(n=80, nbl=5, seed=19).at_k(-0.4, 1.3)
+    E = np.linspace(-1.5, 1.5, 29)
+    ds = hip.DeviceStructure(s)
+    T1, n1, s1 = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    for lanes in (2, 3):
+        T2, n2, s2 = hip.solve_transmission(eng, ds, E, delta=1e-5, lanes=lanes)
+        np.testing.assert_array_equal(T1, T2)
+        np.testing.assert_array_equal(n1, n2)
+        np.testing.assert_array_equal(s1, s2)
