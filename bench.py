@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-items", type=int, default=4,
                     help="items timed one at a time (unbatched FP64 reference mode)")
+    ap.add_argument("--ns-steps", type=int, default=0,
+                    help="mixed precision: FP64 Newton-Schulz steps per block inverse (-1: none)")
+    ap.add_argument("--graph", action="store_true",
+                    help="replay each batch's RGF sweep from a captured hipGraph")
     ap.add_argument("--lanes", type=int, default=1,
                     help="item groups solved concurrently on separate streams")
     ap.add_argument("--no-profile", action="store_true",
@@ -316,7 +320,8 @@ def main():
     dT = torch.zeros(per_step, dtype=torch.float64, device=dev)
     dN = torch.zeros(per_step * ncont, dtype=torch.int32, device=dev)
     dSt = torch.zeros(per_step, dtype=torch.int32, device=dev)
-    opts = hip._opts(eta, 1, 2, prec, 0, lanes=a.lanes)
+    opts = hip._opts(eta, 1, 2, prec, 0, ns_steps=a.ns_steps, lanes=a.lanes,
+                     flags=hip.GRAPH if a.graph else 0)
     import ctypes as C
 
     def step(i):
@@ -432,6 +437,7 @@ def main():
         "data": "synthetic (seeded generators, libnegf_amd/workloads.py)",
         "config": workload,
         "precision": a.precision,
+        "ns_steps": a.ns_steps if a.precision == "mixed" else None,
         "fp64_plain_points_per_s": plain,
         "roofline": roofline,
         "cpu_baseline": cpu,

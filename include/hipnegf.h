@@ -176,16 +176,20 @@ typedef struct {
   int ni, nf;        /* 1-based contact pair of the transmission              */
   int precision;     /* hipnegf_precision                                      */
   int max_batch;     /* 0: automatic from the HBM budget                       */
-  int ns_steps;      /* mixed: FP64 Newton-Schulz steps per block inverse      */
-  int polish;        /* mixed: FP64 decimation iterations after the FP32 ones  */
+  int ns_steps;      /* mixed: FP64 Newton-Schulz steps per block inverse;
+                        0: default (1), -1: none (whole sweep in FP32)        */
+  int polish;        /* reserved (the decimation always runs in FP64)          */
   double eps_drop;   /* EPS drop tolerance (ln_precision), default 0           */
   int flags;         /* hipnegf_opt_flags                                      */
   int lanes;         /* concurrent item groups (own stream each); 0/1: one     */
 } hipnegf_opts;
 /* HIPNEGF_NO_SHARED_DECIMATION: run one Sancho-Rubio recursion per contact
  * even when two contacts are mirror images (by default such a pair shares one
- * recursion; each contact keeps its own convergence test and exit iteration). */
-enum hipnegf_opt_flags { HIPNEGF_NO_SHARED_DECIMATION = 1 };
+ * recursion; each contact keeps its own convergence test and exit iteration).
+ * HIPNEGF_GRAPH: replay the RGF sweep of a batch (the launch chain after the
+ * contacts, which has no host synchronisation) from a captured hipGraph,
+ * cached per structure and batch shape; results are identical. */
+enum hipnegf_opt_flags { HIPNEGF_NO_SHARED_DECIMATION = 1, HIPNEGF_GRAPH = 2 };
 
 /* Coherent transmission T(E_i, slot_i), i < nitems (iterative.F90:1669-1780
  * per item).  Host arrays: E[nitems], slot[nitems] in; T[nitems],

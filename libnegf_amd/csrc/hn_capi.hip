@@ -242,10 +242,11 @@ hn::SolveOpts to_opts(const hipnegf_opts *o) {
     so.nf = o->nf > 0 ? o->nf : 2;
     so.precision = o->precision;
     so.max_batch = o->max_batch;
-    if (o->ns_steps > 0) so.ns_steps = o->ns_steps;
+    if (o->ns_steps != 0) so.ns_steps = std::max(0, o->ns_steps);  // -1: none
     if (o->polish > 0) so.polish = o->polish;
     so.eps_drop = o->eps_drop;
     so.share_mirror = (o->flags & HIPNEGF_NO_SHARED_DECIMATION) ? 0 : 1;
+    so.graph = (o->flags & HIPNEGF_GRAPH) ? 1 : 0;
     so.lanes = std::max(1, std::min(o->lanes, 8));
   }
   return so;

@@ -80,11 +80,12 @@ struct SolveOpts {
   double sgfacc = 1e-9;      // SGFACC, ln_precision.F90:40
   int dec_maxit = 300;       // contselfenergy.F90:289
   double eps_drop = 0.0;     // EPS (ln_precision), default 0
-  int ns_steps = 1;          // mixed precision: FP64 Newton-Schulz refinement steps
-  int polish = 2;            // mixed precision: FP64 decimation polish iterations
+  int ns_steps = 1;          // mixed precision: FP64 Newton-Schulz steps per block inverse
+  int polish = 0;            // reserved
   int dec_crit = 0;          // 0: max|C1| (CPU path), 1: sum|Re|+|Im| (GPU path)
   int share_mirror = 1;      // one decimation for mirror-image contact pairs
   int lanes = 1;             // item groups solved concurrently, each on its own stream
+  int graph = 0;             // replay the RGF sweep of a batch from a captured hipGraph
 };
 
 struct SCBAOpts {
@@ -143,6 +144,12 @@ class Engine {
   int *h_count_ = nullptr;  // pinned
   int last_batch_ = 0;
   std::vector<std::unique_ptr<Engine>> lanes_;
+  struct GraphEntry {
+    std::vector<long long> key;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::vector<GraphEntry> graphs_;  // captured sweeps, most recent last
+  void clear_graphs();
 
   int pick_batch(const DevStruct &st, int nitems, const SolveOpts &o, size_t per_item) const;
   void contacts(DevStruct &st, int B, const double *dE, const int *dslot,
@@ -153,14 +160,16 @@ class Engine {
   void decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 *Aos1, double2 *Bo,
                        double2 *Co, double2 *Go0, double2 *Go1, const SolveOpts &o, int *ncyc0,
                        int *ncyc1, int *status0, int *status1);
-  void decimate_mixed(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, double2 *Co,
-                      double2 *Go, const SolveOpts &o, int *ncyc, int *status);
   void inverse(int n, double2 *A, long long sA, double2 *Ainv, long long sAinv, int batch,
                const int *active, int *status, int precision, const SolveOpts &o,
                int prof_items = -1);
   void t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                const SolveOpts &o, int npairs, const int *ni, const int *nf, double *dT,
                int *dncyc, int *dstatus);
+  void t_sweep(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
+               int ni, int nf, const double2 *Sig, long long sSig, double *dT, int *dstatus);
+  void t_sweep_mixed(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
+                     int ni, int nf, const double2 *Sig, long long sSig, double *dT, int *dstatus);
   void t_multi(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
                int npairs, const int *ni, const int *nf, const double2 *Sig, long long sSig,
                double *dT, int *dstatus);

@@ -15,6 +15,7 @@
 // handled with device-side activity masks; kernels skip finished items.
 #include <algorithm>
 #include <cstring>
+#include <functional>
 #include <string>
 #include <thread>
 
@@ -165,8 +166,14 @@ Engine &Engine::lane(int i) {
   return *lanes_[i];
 }
 
+void Engine::clear_graphs() {
+  for (auto &g : graphs_) (void)hipGraphExecDestroy(g.exec);
+  graphs_.clear();
+}
+
 Engine::~Engine() {
   (void)hipStreamSynchronize(stream_);
+  clear_graphs();
   if (h_count_) (void)hipHostFree(h_count_);
   (void)hipStreamDestroy(stream_);
 }
@@ -424,15 +431,6 @@ void Engine::decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 
   arena_.used = mark;
 }
 
-void Engine::decimate_mixed(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo,
-                            double2 *Co, double2 *Go, const SolveOpts &o, int *ncyc,
-                            int *status) {
-  // The mixed-precision variant is not wired in this build: fall back loudly.
-  (void)nc; (void)CB; (void)Ao; (void)Aos; (void)Bo; (void)Co; (void)Go; (void)o; (void)ncyc;
-  (void)status;
-  throw std::runtime_error("mixed-precision decimation is not available");
-}
-
 // Contact self-energies of all contacts for B items.
 // Sig: ncont x B blocks of nd x nd (nd = bsz[cblk-1]), stride sSig, contact-major.
 void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
@@ -458,7 +456,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
              nd, Z(1), Z(0), B, nullptr), stream_);
     arena_.used = mark;
   };
-  if (o.share_mirror && o.precision == PREC_FP64 && mirror_contacts(st)) {
+  if (o.share_mirror && mirror_contacts(st)) {
     const int nc = st.nc[0];
     const long long s2 = (long long)nc * nc;
     size_t mark = arena_.used;
@@ -504,10 +502,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     form_block(f, stream_);
     int *nc_item = ncyc_c + (long long)c * B;
     int *st_item = status_c + (long long)c * B;
-    if (o.precision == PREC_MIXED)
-      decimate_mixed(nc, B, Ao, Aos, Bo, Co, Go, o, nc_item, st_item);
-    else
-      decimate_z(nc, B, Ao, Aos, Bo, Co, Go, o, nc_item, st_item);
+    decimate_z(nc, B, Ao, Aos, Bo, Co, Go, o, nc_item, st_item);
     self_energy(c, Go);
     arena_.used = mark;
   }
@@ -516,6 +511,248 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
 // ---------------------------------------------------------------------------
 // Coherent transmission for one batch.
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// 2-contact fast path of calculate_transmissions (iterative.F90:1704-1715) for
+// one batch: gsmr (nbl -> 2), Gr(1,1), T = |Re Tr[Gam A - Gam G Gam G^H]|.
+// A fixed launch chain with no host synchronisation (graph-capturable).
+// ---------------------------------------------------------------------------
+void Engine::t_sweep(DevStruct &st, int B, const double *dE, const int *dslot,
+                     const SolveOpts &o, int ni, int nf, const double2 *Sig, long long sSig,
+                     double *dT, int *dstatus) {
+  const int nbl = st.nbl, ncont = st.ncont;
+  const long long s2 = sSig;
+  size_t mark = arena_.used;
+  // which contacts sit on which block
+  std::vector<std::vector<int>> on_blk(nbl);
+  for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
+  auto sig_of = [&](int c) { return Sig + (long long)c * B * s2; };
+
+  double2 *D = arena_.take<double2>(s2 * B), *U = arena_.take<double2>(s2 * B);
+  double2 *L = arena_.take<double2>(s2 * B), *g = arena_.take<double2>(s2 * B);
+  double2 *W1 = arena_.take<double2>(s2 * B), *Gs = arena_.take<double2>(s2 * B);
+  const bool sd = (st.smode & S_DIAG) != 0, so = (st.smode & S_OFFDIAG) != 0;
+
+  auto form_diag = [&](int i, double2 *out) {
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+    f.rows = st.bsz[i]; f.cols = st.bsz[i];
+    f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+    f.out = out; f.s_out = s2;
+    const auto &cs = on_blk[i];
+    if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+    if (cs.size() >= 1) { f.sig = sig_of(cs[0]); f.s_sig = s2; }
+    if (cs.size() == 2) { f.sig2 = sig_of(cs[1]); f.s_sig2 = s2; }
+    form_block(f, stream_);
+  };
+  auto form_off = [&](int i) {  // U = ESH(i,i+1), L = ESH(i+1,i)
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+    f.rows = st.bsz[i]; f.cols = st.bsz[i + 1];
+    f.H = st.ref(BK_HU, i); f.S = so ? st.ref(BK_SU, i) : BlkRef(); f.s_identity = 0;
+    f.out = U; f.s_out = s2;
+    form_block(f, stream_);
+    f.rows = st.bsz[i + 1]; f.cols = st.bsz[i];
+    f.H = st.ref(BK_HL, i); f.S = so ? st.ref(BK_SL, i) : BlkRef();
+    f.out = L;
+    form_block(f, stream_);
+  };
+
+  const int cbi = st.cblk[ni - 1], cbf = st.cblk[nf - 1];
+
+  // ---- calculate_gsmr_blocks(nbl -> 2), iterative_cpu.F90:51-124 ----
+  if (nbl > 1) {
+    form_diag(nbl - 1, D);
+    const int n = st.bsz[nbl - 1];
+    inverse(n, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
+    for (int i = nbl - 2; i >= 1; --i) {
+      const int ni_ = st.bsz[i], nj = st.bsz[i + 1];
+      form_off(i);
+      form_diag(i, D);
+      // work1 = -ESH(i,i+1) g(i+1) ; M = ESH(i,i) + work1 ESH(i+1,i)
+      gemm_z(G(ni_, nj, nj, U, s2, ni_, OP_N, g, s2, nj, OP_N, W1, s2, ni_, Z(-1), Z(0), B, nullptr), stream_);
+      gemm_z(G(ni_, ni_, nj, W1, s2, ni_, OP_N, L, s2, nj, OP_N, D, s2, ni_, Z(1), Z(1), B, nullptr), stream_);
+      inverse(ni_, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
+    }
+  }
+  // ---- Gr(1,1), iterative_cpu.F90:172-198 ----
+  {
+    const int n0 = st.bsz[0];
+    form_diag(0, D);
+    if (nbl > 1) {
+      const int n1 = st.bsz[1];
+      form_off(0);
+      gemm_z(G(n0, n1, n1, U, s2, n0, OP_N, g, s2, n1, OP_N, W1, s2, n0, Z(-1), Z(0), B, nullptr), stream_);
+      gemm_z(G(n0, n0, n1, W1, s2, n0, OP_N, L, s2, n1, OP_N, D, s2, n0, Z(1), Z(1), B, nullptr), stream_);
+    }
+    inverse(n0, D, s2, Gs, s2, B, nullptr, dstatus, o.precision, o);
+  }
+  // ---- T = |Re Tr[Gam A - Gam G Gam G^H]| on block 1 ----
+  {
+    const int n0 = st.bsz[0];
+    const int ct1 = (cbi < cbf) ? ni : nf;
+    double2 *Gam = U, *W2 = L;
+    gamma_batched(sig_of(ct1 - 1), s2, Gam, s2, n0, B, stream_);
+    gemm_z(G(n0, n0, n0, Gam, s2, n0, OP_N, Gs, s2, n0, OP_N, W1, s2, n0, Z(1), Z(0), B, nullptr), stream_);
+    gemm_z(G(n0, n0, n0, W1, s2, n0, OP_N, Gam, s2, n0, OP_N, W2, s2, n0, Z(1), Z(0), B, nullptr), stream_);
+    trace_T2(Gam, Gs, W2, s2, n0, B, dT, stream_);
+  }
+  arena_.used = mark;
+}
+
+// ---------------------------------------------------------------------------
+// Mixed precision of the 2-contact fast path: the sweep's GEMMs and block
+// inverses in complex FP32 on the matrix cores (FP32 accumulation); the
+// blocks E S - H - Sigma are formed in FP64 and rounded once, the contacts
+// (Sancho-Rubio) stay FP64, and Gr(1,1) is widened to FP64 for the trace.
+// ---------------------------------------------------------------------------
+void Engine::t_sweep_mixed(DevStruct &st, int B, const double *dE, const int *dslot,
+                           const SolveOpts &o, int ni, int nf, const double2 *Sig, long long sSig,
+                           double *dT, int *dstatus) {
+  const int nbl = st.nbl, ncont = st.ncont;
+  const long long s2 = sSig;
+  size_t mark = arena_.used;
+  std::vector<std::vector<int>> on_blk(nbl);
+  for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
+  double2 *D64 = arena_.take<double2>(s2 * B), *U64 = arena_.take<double2>(s2 * B);
+  double2 *L64 = arena_.take<double2>(s2 * B), *G64 = arena_.take<double2>(s2 * B);
+  float2 *Dc = arena_.take<float2>(s2 * B), *Uc = arena_.take<float2>(s2 * B);
+  float2 *Lc = arena_.take<float2>(s2 * B), *gc = arena_.take<float2>(s2 * B);
+  float2 *Wc = arena_.take<float2>(s2 * B);
+  const bool sd = (st.smode & S_DIAG) != 0, so = (st.smode & S_OFFDIAG) != 0;
+  auto form_diag = [&](int i) {  // Dc = round(ESH(i,i) - Sigma)
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+    f.rows = st.bsz[i]; f.cols = st.bsz[i];
+    f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+    f.out = D64; f.s_out = s2;
+    const auto &cs = on_blk[i];
+    if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+    if (cs.size() >= 1) { f.sig = Sig + (long long)cs[0] * B * s2; f.s_sig = s2; }
+    if (cs.size() == 2) { f.sig2 = Sig + (long long)cs[1] * B * s2; f.s_sig2 = s2; }
+    form_block(f, stream_);
+    z2c(D64, s2, Dc, s2, (long long)st.bsz[i] * st.bsz[i], B, nullptr, stream_);
+  };
+  auto form_off = [&](int i) {  // Uc = ESH(i,i+1), Lc = ESH(i+1,i)
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+    f.rows = st.bsz[i]; f.cols = st.bsz[i + 1];
+    f.H = st.ref(BK_HU, i); f.S = so ? st.ref(BK_SU, i) : BlkRef(); f.s_identity = 0;
+    f.out = U64; f.s_out = s2;
+    form_block(f, stream_);
+    f.rows = st.bsz[i + 1]; f.cols = st.bsz[i];
+    f.H = st.ref(BK_HL, i); f.S = so ? st.ref(BK_SL, i) : BlkRef();
+    f.out = L64;
+    form_block(f, stream_);
+    const long long e = (long long)st.bsz[i] * st.bsz[i + 1];
+    z2c(U64, s2, Uc, s2, e, B, nullptr, stream_);
+    z2c(L64, s2, Lc, s2, e, B, nullptr, stream_);
+  };
+  auto gc_ = [&](int m, int n, int k, const float2 *A, int lda, const float2 *Bm, int ldb, float2 *C,
+                 int ldc, double2 al, double2 be) {
+    GemmArgs<float> g;
+    g.m = m; g.n = n; g.k = k;
+    g.A = A; g.sA = s2; g.lda = lda; g.opA = OP_N;
+    g.B = Bm; g.sB = s2; g.ldb = ldb; g.opB = OP_N;
+    g.C = C; g.sC = s2; g.ldc = ldc;
+    g.alpha = al; g.beta = be; g.batch = B;
+    gemm_c(g, stream_);
+  };
+  auto inv32 = [&](int n, float2 *A, float2 *Ainv) {
+    InvWork<float> w;
+    size_t m2 = arena_.used;
+    if (n > inv_lds_max_n(4)) w = inv_work_carve<float>(arena_.take<char>(inv_work_bytes(n, B, 4)), n, B);
+    inv_c(n, A, s2, Ainv, s2, B, nullptr, dstatus, w, stream_);
+    arena_.used = m2;
+  };
+  const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+  const int n0 = st.bsz[0];
+  if (o.ns_steps > 0) {
+    // FP64 recursion M = ESH - U g L with each block inverse taken in FP32 and
+    // refined by ns_steps FP64 Newton-Schulz steps X <- X + X (I - M X).
+    double2 *g64 = G64, *R64 = arena_.take<double2>(s2 * B), *X64 = arena_.take<double2>(s2 * B);
+    auto form64 = [&](int i) {
+      FormArgs f;
+      f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+      f.rows = st.bsz[i]; f.cols = st.bsz[i];
+      f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+      f.out = D64; f.s_out = s2;
+      const auto &cs = on_blk[i];
+      if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+      if (cs.size() >= 1) { f.sig = Sig + (long long)cs[0] * B * s2; f.s_sig = s2; }
+      if (cs.size() == 2) { f.sig2 = Sig + (long long)cs[1] * B * s2; f.s_sig2 = s2; }
+      form_block(f, stream_);
+    };
+    auto inv_refined = [&](int n) {  // g64 = inverse(D64), D64 kept
+      z2c(D64, s2, Dc, s2, (long long)n * n, B, nullptr, stream_);
+      inv32(n, Dc, gc);
+      c2z(gc, s2, g64, s2, (long long)n * n, B, nullptr, stream_);
+      for (int k = 0; k < o.ns_steps; ++k) {
+        // R = I - M X ; X <- X + X R
+        gemm_z(G(n, n, n, D64, s2, n, OP_N, g64, s2, n, OP_N, R64, s2, n, mone, zero, B, nullptr), stream_);
+        add_diag(R64, s2, n, nullptr, 0, one, B, nullptr, stream_);
+        copy_batched(X64, s2, g64, s2, (long long)n * n, B, nullptr, stream_);
+        gemm_z(G(n, n, n, X64, s2, n, OP_N, R64, s2, n, OP_N, g64, s2, n, one, one, B, nullptr), stream_);
+      }
+    };
+    auto reduce64 = [&](int i) {  // D64 = ESH(i,i) - Sigma - U g L with g = g64 (block i+1)
+      const int a = st.bsz[i], c = st.bsz[i + 1];
+      FormArgs f;
+      f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
+      f.s_identity = 0; f.s_out = s2;
+      f.rows = a; f.cols = c;
+      f.H = st.ref(BK_HU, i); f.S = so ? st.ref(BK_SU, i) : BlkRef(); f.out = U64;
+      form_block(f, stream_);
+      f.rows = c; f.cols = a;
+      f.H = st.ref(BK_HL, i); f.S = so ? st.ref(BK_SL, i) : BlkRef(); f.out = L64;
+      form_block(f, stream_);
+      gemm_z(G(a, c, c, U64, s2, a, OP_N, g64, s2, c, OP_N, R64, s2, a, mone, zero, B, nullptr), stream_);
+      form64(i);
+      gemm_z(G(a, a, c, R64, s2, a, OP_N, L64, s2, c, OP_N, D64, s2, a, one, one, B, nullptr), stream_);
+    };
+    if (nbl > 1) {
+      form64(nbl - 1);
+      inv_refined(st.bsz[nbl - 1]);
+      for (int i = nbl - 2; i >= 0; --i) {
+        reduce64(i);
+        inv_refined(st.bsz[i]);
+      }
+    } else {
+      form64(0);
+      inv_refined(st.bsz[0]);
+    }
+  } else {
+  if (nbl > 1) {
+    form_diag(nbl - 1);
+    inv32(st.bsz[nbl - 1], Dc, gc);
+    for (int i = nbl - 2; i >= 1; --i) {
+      const int a = st.bsz[i], c = st.bsz[i + 1];
+      form_off(i);
+      form_diag(i);
+      gc_(a, c, c, Uc, a, gc, c, Wc, a, mone, zero);
+      gc_(a, a, c, Wc, a, Lc, c, Dc, a, one, one);
+      inv32(a, Dc, gc);
+    }
+  }
+  form_diag(0);
+  if (nbl > 1) {
+    const int n1 = st.bsz[1];
+    form_off(0);
+    gc_(n0, n1, n1, Uc, n0, gc, n1, Wc, n0, mone, zero);
+    gc_(n0, n0, n1, Wc, n0, Lc, n1, Dc, n0, one, one);
+  }
+  inv32(n0, Dc, gc);
+  c2z(gc, s2, G64, s2, (long long)n0 * n0, B, nullptr, stream_);
+  }
+  // T = |Re Tr[Gam A - Gam G Gam G^H]| in FP64
+  const int ct1 = (st.cblk[ni - 1] < st.cblk[nf - 1]) ? ni : nf;
+  double2 *Gam = U64, *W1 = L64, *W2 = D64;
+  gamma_batched(Sig + (long long)(ct1 - 1) * B * s2, s2, Gam, s2, n0, B, stream_);
+  gemm_z(G(n0, n0, n0, Gam, s2, n0, OP_N, G64, s2, n0, OP_N, W1, s2, n0, one, zero, B, nullptr), stream_);
+  gemm_z(G(n0, n0, n0, W1, s2, n0, OP_N, Gam, s2, n0, OP_N, W2, s2, n0, one, zero, B, nullptr), stream_);
+  trace_T2(Gam, G64, W2, s2, n0, B, dT, stream_);
+  arena_.used = mark;
+}
+
 void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                      const SolveOpts &o, int npairs, const int *pni, const int *pnf, double *dT,
                      int *dncyc, int *dstatus) {
@@ -536,82 +773,51 @@ void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
   contacts(st, B, dE, dslot, o, Sig, s2, ncyc_c, status_c);
   if (!fast_path(st, npairs, pni, pnf)) {
     t_multi(st, B, dE, dslot, o, npairs, pni, pnf, Sig, s2, dT, dstatus);
+  } else if (o.precision == PREC_MIXED) {
+    t_sweep_mixed(st, B, dE, dslot, o, pni[0], pnf[0], Sig, s2, dT, dstatus);
+  } else if (o.graph && !profiler().on()) {
+    // Stage the batch's inputs and outputs at fixed workspace addresses so the
+    // captured launch chain can be replayed for every batch of this shape.
+    double *Ef = arena_.take<double>(B), *Tf = arena_.take<double>(B);
+    int *Sf = arena_.take<int>(B), *Stf = arena_.take<int>(B);
+    copy_real(Ef, dE, B, stream_);
+    copy_int(Sf, dslot, B, stream_);
+    fill_int(Stf, B, 0, stream_);
+    const std::vector<long long> key = {(long long)(uintptr_t)st.pool, st.slot_elems, st.nbl, st.smode,
+                                        st.nmax(), B, pni[0], pnf[0], (long long)(uintptr_t)arena_.base,
+                                        (long long)arena_.used, o.share_mirror,
+                                        (long long)std::hash<std::string>()(std::string(
+                                            (const char *)st.bsz.data(), st.bsz.size() * sizeof(int)))};
+    hipGraphExec_t exec = nullptr;
+    for (auto &g : graphs_)
+      if (g.key == key) exec = g.exec;
+    if (!exec) {
+      const size_t mark = arena_.used;
+      hipGraph_t graph = nullptr;
+      HN_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+      try {
+        t_sweep(st, B, Ef, Sf, o, pni[0], pnf[0], Sig, s2, Tf, Stf);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream_, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+      }
+      HN_CHECK(hipStreamEndCapture(stream_, &graph));
+      arena_.used = mark;
+      hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      HN_CHECK(e);
+      if (graphs_.size() >= 8) {
+        (void)hipGraphExecDestroy(graphs_.front().exec);
+        graphs_.erase(graphs_.begin());
+      }
+      graphs_.push_back({key, exec});
+    }
+    HN_CHECK(hipGraphLaunch(exec, stream_));
+    copy_real(dT, Tf, B, stream_);
+    or_flags(dstatus, Stf, dstatus, B, stream_);
   } else {
-    // which contacts sit on which block
-    std::vector<std::vector<int>> on_blk(nbl);
-    for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
-    auto sig_of = [&](int c) { return Sig + (long long)c * B * s2; };
-
-    double2 *D = arena_.take<double2>(s2 * B), *U = arena_.take<double2>(s2 * B);
-    double2 *L = arena_.take<double2>(s2 * B), *g = arena_.take<double2>(s2 * B);
-    double2 *W1 = arena_.take<double2>(s2 * B), *Gs = arena_.take<double2>(s2 * B);
-    const bool sd = (st.smode & S_DIAG) != 0, so = (st.smode & S_OFFDIAG) != 0;
-
-    auto form_diag = [&](int i, double2 *out) {
-      FormArgs f;
-      f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
-      f.rows = st.bsz[i]; f.cols = st.bsz[i];
-      f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
-      f.out = out; f.s_out = s2;
-      const auto &cs = on_blk[i];
-      if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
-      if (cs.size() >= 1) { f.sig = sig_of(cs[0]); f.s_sig = s2; }
-      if (cs.size() == 2) { f.sig2 = sig_of(cs[1]); f.s_sig2 = s2; }
-      form_block(f, stream_);
-    };
-    auto form_off = [&](int i) {  // U = ESH(i,i+1), L = ESH(i+1,i)
-      FormArgs f;
-      f.slot = dslot; f.E = dE; f.eta = 0.0; f.batch = B;
-      f.rows = st.bsz[i]; f.cols = st.bsz[i + 1];
-      f.H = st.ref(BK_HU, i); f.S = so ? st.ref(BK_SU, i) : BlkRef(); f.s_identity = 0;
-      f.out = U; f.s_out = s2;
-      form_block(f, stream_);
-      f.rows = st.bsz[i + 1]; f.cols = st.bsz[i];
-      f.H = st.ref(BK_HL, i); f.S = so ? st.ref(BK_SL, i) : BlkRef();
-      f.out = L;
-      form_block(f, stream_);
-    };
-
-    const int ni = pni[0], nf = pnf[0];
-    const int cbi = st.cblk[ni - 1], cbf = st.cblk[nf - 1];
-
-    // ---- calculate_gsmr_blocks(nbl -> 2), iterative_cpu.F90:51-124 ----
-    if (nbl > 1) {
-      form_diag(nbl - 1, D);
-      const int n = st.bsz[nbl - 1];
-      inverse(n, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
-      for (int i = nbl - 2; i >= 1; --i) {
-        const int ni_ = st.bsz[i], nj = st.bsz[i + 1];
-        form_off(i);
-        form_diag(i, D);
-        // work1 = -ESH(i,i+1) g(i+1) ; M = ESH(i,i) + work1 ESH(i+1,i)
-        gemm_z(G(ni_, nj, nj, U, s2, ni_, OP_N, g, s2, nj, OP_N, W1, s2, ni_, Z(-1), Z(0), B, nullptr), stream_);
-        gemm_z(G(ni_, ni_, nj, W1, s2, ni_, OP_N, L, s2, nj, OP_N, D, s2, ni_, Z(1), Z(1), B, nullptr), stream_);
-        inverse(ni_, D, s2, g, s2, B, nullptr, dstatus, o.precision, o);
-      }
-    }
-    // ---- Gr(1,1), iterative_cpu.F90:172-198 ----
-    {
-      const int n0 = st.bsz[0];
-      form_diag(0, D);
-      if (nbl > 1) {
-        const int n1 = st.bsz[1];
-        form_off(0);
-        gemm_z(G(n0, n1, n1, U, s2, n0, OP_N, g, s2, n1, OP_N, W1, s2, n0, Z(-1), Z(0), B, nullptr), stream_);
-        gemm_z(G(n0, n0, n1, W1, s2, n0, OP_N, L, s2, n1, OP_N, D, s2, n0, Z(1), Z(1), B, nullptr), stream_);
-      }
-      inverse(n0, D, s2, Gs, s2, B, nullptr, dstatus, o.precision, o);
-    }
-    // ---- T = |Re Tr[Gam A - Gam G Gam G^H]| on block 1 ----
-    {
-      const int n0 = st.bsz[0];
-      const int ct1 = (cbi < cbf) ? ni : nf;
-      double2 *Gam = U, *W2 = L;
-      gamma_batched(sig_of(ct1 - 1), s2, Gam, s2, n0, B, stream_);
-      gemm_z(G(n0, n0, n0, Gam, s2, n0, OP_N, Gs, s2, n0, OP_N, W1, s2, n0, Z(1), Z(0), B, nullptr), stream_);
-      gemm_z(G(n0, n0, n0, W1, s2, n0, OP_N, Gam, s2, n0, OP_N, W2, s2, n0, Z(1), Z(0), B, nullptr), stream_);
-      trace_T2(Gam, Gs, W2, s2, n0, B, dT, stream_);
-    }
+    t_sweep(st, B, dE, dslot, o, pni[0], pnf[0], Sig, s2, dT, dstatus);
   }
   // ---- per-item outputs: ncyc (item-major) and status ----
   {

@@ -9,18 +9,19 @@
 //
 // Large blocks: two-level blocked in-place Gauss-Jordan with *lazy* pivoting.
 // Rows are never moved; the pivot row of each logical column is recorded.
-// Per inner panel K of nb columns:
-//   panel kernel (one workgroup per item, panel in registers): partial-
-//     pivoting LU of the panel over not-yet-pivoted rows picks nb pivot rows;
-//     with Q = B(piv, K) it writes Z = [-B(r,K) Q^-1 ; Q^-1 - I] (n x nb), the
-//     new panel columns, and a copy R of the pivot rows.
-//   inner update: B(:, J) += Z R(:, J) only for the other columns J of the
-//     current super-panel S (kInvSuper columns).
+// Per inner panel K of nb columns (one launch, one workgroup per item, the
+// panel in registers): partial-pivoting LU of the panel over the
+// not-yet-pivoted rows picks nb pivot rows; with Q = B(piv, K) every thread
+// forms its row of Z = [-B(r,K) Q^-1 ; Q^-1 - I], writes the new panel
+// columns, applies B(:, J) += Z R(:, J) to the other columns J of the current
+// super-panel S (kInvSuper columns; R = their pivot rows, staged in LDS), and
+// copies the pivot rows of the columns outside S.
 // After the inner panels of S, its columns hold Zhat + E where
-// I + Zhat P^T is the product of the inner rank-nb transforms, so the rest of
-// the matrix takes ONE deferred rank-kInvSuper update
+// I + Zhat P^T is the product of the inner rank-nb transforms (the last panel
+// of S writes Zhat), so the rest of the matrix takes ONE deferred
+// rank-kInvSuper update
 //   B(:, J) += Zhat R(:, J),  J outside S  (R = pivot rows before S started),
-// a GEMM on the matrix cores with 4x the intensity of a rank-16 update.
+// a GEMM on the matrix cores.
 // After the last panel, Ainv(i, c) = B(pi(i), pi^-1(c)) with pi(c) the pivot
 // row of logical column c.
 #include "hn_gemm.h"
@@ -172,11 +173,19 @@ __global__ __launch_bounds__(NT) void k_inv_reset(int n, int batch, int *used,
 // Q = L U (rows in selection order) and each thread solves W Q = B(r, K) for
 // its rows by one forward and one backward substitution (pivot rows solve
 // with e_j, giving row j of Q^-1), so Q is never inverted explicitly.
+//
+// The same kernel applies the panel's transform to the other columns of the
+// current super-panel [s0, s0+sw): B(:, J) += Z R(:, J) with R = the pivot
+// rows of those columns (staged in LDS before any row is updated) and Z held
+// in registers, one row per thread -- so no separate rank-nb GEMM pass runs
+// over the super-panel.
 template <typename R, int NB, int RPT>
-__global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, int rb_row,
+__global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, int rb_row, int s0,
+                                                      int sw,
                                                       typename CT_<R>::t *__restrict__ A,
                                                       long long sA,
                                                       typename CT_<R>::t *__restrict__ Z,
+                                                      typename CT_<R>::t *__restrict__ Zh,
                                                       typename CT_<R>::t *__restrict__ Rb,
                                                       int *__restrict__ used_g,
                                                       int *__restrict__ piv_g,
@@ -186,6 +195,8 @@ __global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, in
   const int b = blockIdx.x;
   if (active && active[b] == 0) return;
   __shared__ C LU[NB][NB];  // LU[j][l]: row j of the pivot block's LU factors
+  __shared__ C Rs[NB][kInvSuper];  // pivot rows of the other super-panel columns
+  __shared__ int spiv[kInvSuper];  // pivot rows of the super-panel's earlier columns
   __shared__ int piv[NB];
   __shared__ R redv[NT / 64];
   __shared__ int redi[NT / 64];
@@ -281,14 +292,22 @@ __global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, in
       for (int q = 0; q < kPer; ++q) {
         const int e = base + tid + q * NT;
         const int j = e % nbp, c = e / nbp;
-        if (e < total && (c < c0 || c >= c0 + nbp)) Rbb[j + (size_t)c * kInvSuper] = v[q];
+        if (e < total && (c < c0 || c >= c0 + nbp)) {
+          if (c >= s0 && c < s0 + sw) Rs[j][c - s0] = v[q];
+          else Rbb[j + (size_t)c * kInvSuper] = v[q];
+        }
       }
     }
   }
   // reciprocals of the pivots U(l,l), once
   __shared__ C dinv[NB];
   if (tid < nbp) dinv[tid] = crecip(LU[tid][tid]);
+  // The last panel of a super-panel also writes Zhat = B(:, S) - E for the
+  // deferred update of the other columns (E: ones at (pivot row, column)).
+  const bool last = Zh != nullptr && c0 + nbp == s0 + sw;
+  if (last && tid < c0 - s0) spiv[tid] = piv_g[(long long)b * n + s0 + tid];
   __syncthreads();
+  C *Zhb = last ? Zh + (long long)b * n * kInvSuper : nullptr;
 
   // ---- W Q = x  with Q = L U:  y U = x (forward), W L = y (backward) ----
 #pragma unroll
@@ -328,11 +347,46 @@ __global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, in
       if (pj >= 0) {
         Ab[r + (size_t)(c0 + l) * n] = w;
         if (l == pj) w.x -= R(1);
-        Zb[r + (size_t)l * n] = w;
       } else {
-        const C nw = cneg(w);
-        Ab[r + (size_t)(c0 + l) * n] = nw;
-        Zb[r + (size_t)l * n] = nw;
+        w = cneg(w);
+        Ab[r + (size_t)(c0 + l) * n] = w;
+      }
+      P[q][l] = w;  // row r of Z
+      if (sw == 0) Zb[r + (size_t)l * n] = w;
+      if (last) Zhb[r + (size_t)(c0 - s0 + l) * n] = w;
+    }
+    // ---- B(r, J) += Z(r, :) R(:, J) for the other super-panel columns J ----
+    for (int cb = s0; cb < s0 + sw; cb += 8) {
+      C x[8];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int c = cb + k;
+        const bool on = c < s0 + sw && (c < c0 || c >= c0 + nbp);
+        x[k] = on ? Ab[r + (size_t)c * n] : C{0, 0};
+      }
+#pragma unroll
+      for (int l = 0; l < NB; ++l) {
+        if (l < nbp) {
+          const C z = P[q][l];
+#pragma unroll
+          for (int k = 0; k < 8; ++k) {
+            const C rv = Rs[l][(cb + k - s0) & (kInvSuper - 1)];
+            x[k].x += z.x * rv.x - z.y * rv.y;
+            x[k].y += z.x * rv.y + z.y * rv.x;
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const int c = cb + k;
+        if (c < s0 + sw && (c < c0 || c >= c0 + nbp)) {
+          Ab[r + (size_t)c * n] = x[k];
+          if (last) {  // only earlier columns of S remain here (the panel is the last one)
+            C zh = x[k];
+            if (spiv[c - s0] == r) zh.x -= R(1);
+            Zhb[r + (size_t)(c - s0) * n] = zh;
+          }
+        }
       }
     }
     ug[r] = used[q];
@@ -340,23 +394,6 @@ __global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, in
   }
   if (tid < nbp) piv_g[(long long)b * n + c0 + tid] = piv[tid];
   if (tid == 0 && s_sing && status) atomicOr(&status[b], (int)ST_SINGULAR);
-}
-
-// Zhat = B(:, s0:s0+sw) - E (E: ones at (pivot row, logical column))
-template <typename R>
-__global__ __launch_bounds__(NT) void k_inv_zhat(int n, int s0, int sw,
-                                                 const typename CT_<R>::t *A, long long sA,
-                                                 typename CT_<R>::t *Zh, const int *piv_g,
-                                                 const int *active) {
-  typedef typename CT_<R>::t C;
-  const int b = blockIdx.y;
-  if (active && active[b] == 0) return;
-  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
-  if (e >= (long long)n * sw) return;
-  const int r = (int)(e % n), i = (int)(e / n);
-  C v = A[(long long)b * sA + r + (size_t)(s0 + i) * n];
-  if (piv_g[(long long)b * n + s0 + i] == r) v.x -= R(1);
-  Zh[(long long)b * n * kInvSuper + e] = v;
 }
 
 template <typename R>
@@ -445,33 +482,20 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
         ProfScope ps(PT_INV_PANEL, -1.0, 0.0, s);
         if (n <= NT && NB == 32)
           hipLaunchKernelGGL((k_inv_panel_reg<R, 32, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                             c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+                             c0 - s0, s0, sw, A, sA, w.Z, n > sw ? w.Zh : nullptr, w.Rb, w.used,
+                             w.piv, active, status);
         else if (n <= NT)
           hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                             c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+                             c0 - s0, s0, sw, A, sA, w.Z, n > sw ? w.Zh : nullptr, w.Rb, w.used,
+                             w.piv, active, status);
         else
           hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 2>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                             c0 - s0, A, sA, w.Z, w.Rb, w.used, w.piv, active, status);
+                             c0 - s0, s0, sw, A, sA, w.Z, n > sw ? w.Zh : nullptr, w.Rb, w.used,
+                             w.piv, active, status);
       }
       HN_CHECK(hipGetLastError());
-      if (sw > nbp) {  // inner update: the other columns of the super-panel
-        GemmArgs<R> g;
-        g.m = n; g.n = sw; g.k = nbp;
-        g.A = w.Z; g.sA = (long long)n * kInvPanel; g.lda = n; g.opA = OP_N;
-        g.B = w.Rb + (c0 - s0) + (size_t)s0 * kInvSuper; g.sB = (long long)kInvSuper * n;
-        g.ldb = kInvSuper; g.opB = OP_N;
-        g.C = A + (size_t)s0 * n; g.sC = sA; g.ldc = n;
-        g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
-        g.active = active; g.skip_c0 = c0 - s0; g.skip_c1 = c0 - s0 + nbp;
-        g.batch = batch; g.tag = PT_GEMM_INV; g.prof_items = w.prof_items;
-        if constexpr (sizeof(R) == 8) gemm_z(g, s);
-        else gemm_c(g, s);
-      }
     }
     if (n > sw) {  // deferred rank-sw update of every column outside the super-panel
-      hipLaunchKernelGGL(k_inv_zhat<R>, dim3((unsigned)(((long long)n * sw + NT - 1) / NT), batch),
-                         dim3(NT), 0, s, n, s0, sw, A, sA, w.Zh, w.piv, active);
-      HN_CHECK(hipGetLastError());
       GemmArgs<R> g;
       g.m = n; g.n = n; g.k = sw;
       g.A = w.Zh; g.sA = (long long)n * kInvSuper; g.lda = n; g.opA = OP_N;

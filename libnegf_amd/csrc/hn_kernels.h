@@ -102,6 +102,9 @@ void maxabs_diff(const double2 *X, const double2 *Y, long long sX, long long sY,
 void fill_batched(double2 *out, long long so, long long n, int batch, double2 value,
                   hipStream_t st);
 void fill_int(int *out, int n, int value, hipStream_t st);
+// dst[i] = src[i] (kernel copies, usable inside captured graphs)
+void copy_real(double *dst, const double *src, int n, hipStream_t st);
+void copy_int(int *dst, const int *src, int n, hipStream_t st);
 void fill_double(double *out, int n, double value, hipStream_t st);
 
 // precision conversion  z -> c and back, with optional active mask
@@ -148,7 +151,7 @@ void trace_mw(const double2 *Gam, const double2 *Gn, const double2 *Gr, long lon
 // out[i] = x[i]^2
 void square(const double *x, double *out, int n, hipStream_t st);
 
-// out_b = M_b + diag(v_b)*alpha  (add a scaled vector on the diagonal), n x n
+// out_b = M_b + diag(v_b)*alpha  (add a scaled vector on the diagonal; v null: ones), n x n
 void add_diag(double2 *M, long long sM, int n, const double2 *v, long long sv, double2 alpha,
               int batch, const int *active, hipStream_t st);
 

@@ -349,7 +349,7 @@ __global__ __launch_bounds__(NT) void k_add_diag(double2 *M, long long sM, int n
   if (active && active[b] == 0) return;
   const int i = blockIdx.x * NT + threadIdx.x;
   if (i >= n) return;
-  double2 x = v[(long long)b * sv + i];
+  const double2 x = v ? v[(long long)b * sv + i] : make_double2(1.0, 0.0);  // v == null: ones
   double2 *p = M + (long long)b * sM + i + (long long)i * n;
   double2 m = *p;
   m.x += alpha.x * x.x - alpha.y * x.y;
@@ -481,6 +481,12 @@ __global__ __launch_bounds__(NT) void k_trace_mw(const double2 *Gam, const doubl
   if (threadIdx.x == 0) curr[(long long)b * ncont + c] = -v;
 }
 
+template <typename T>
+__global__ __launch_bounds__(NT) void k_copy1(T *dst, const T *src, int n) {
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(NT) void k_square(const double *x, double *out, int n) {
   const int i = blockIdx.x * NT + threadIdx.x;
   if (i < n) out[i] = x[i] * x[i];
@@ -603,6 +609,16 @@ void fill_int(int *out, int n, int value, hipStream_t st) {
   HN_CHECK(hipGetLastError());
 }
 
+void copy_real(double *dst, const double *src, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy1<double>, dim3((n + NT - 1) / NT), dim3(NT), 0, st, dst, src, n);
+  HN_CHECK(hipGetLastError());
+}
+void copy_int(int *dst, const int *src, int n, hipStream_t st) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_copy1<int>, dim3((n + NT - 1) / NT), dim3(NT), 0, st, dst, src, n);
+  HN_CHECK(hipGetLastError());
+}
 void fill_double(double *out, int n, double value, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_fill_double, dim3((n + NT - 1) / NT), dim3(NT), 0, st, out, n, value);

@@ -35,6 +35,7 @@ class Opts(C.Structure):
                 ("polish", C.c_int), ("eps_drop", C.c_double), ("flags", C.c_int),
                 ("lanes", C.c_int)]
 NO_SHARED_DECIMATION = 1
+GRAPH = 2
 
 
 def load():
@@ -248,8 +249,10 @@ class DeviceStructure:
 
 def solve_transmission(eng: Engine, ds: DeviceStructure, E: np.ndarray, slot=None,
                        delta: float = 1e-4, ni: int = 1, nf: int = 2, precision: int = FP64,
-                       max_batch: int = 0, flags: int = 0, lanes: int = 1):
-    """Host-array batched transmission.  Returns (T, ncyc[n, ncont], status)."""
+                       max_batch: int = 0, flags: int = 0, lanes: int = 1, ns_steps: int = 0):
+    """Host-array batched transmission.  Returns (T, ncyc[n, ncont], status).
+    precision=MIXED: FP32 sweep, ns_steps FP64 Newton-Schulz steps per block
+    inverse (0: default 1, -1: none)."""
     E = np.ascontiguousarray(E, dtype=np.float64)
     n = len(E)
     sl = np.zeros(n, dtype=np.int32) if slot is None else np.ascontiguousarray(slot, dtype=np.int32)
@@ -257,7 +260,7 @@ def solve_transmission(eng: Engine, ds: DeviceStructure, E: np.ndarray, slot=Non
     T = np.zeros(n)
     ncyc = np.zeros((n, max(ncont, 1)), dtype=np.int32)
     st = np.zeros(n, dtype=np.int32)
-    o = _opts(delta, ni, nf, precision, max_batch, flags=flags, lanes=lanes)
+    o = _opts(delta, ni, nf, precision, max_batch, ns_steps=ns_steps, flags=flags, lanes=lanes)
     check(load().hipnegf_solve_transmission(eng.handle, ds.handle, n, _ptr(E), _ptr(sl),
                                             C.byref(o), _ptr(T), _ptr(ncyc), _ptr(st)))
     return T, ncyc, st

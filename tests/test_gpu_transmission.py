@@ -240,3 +240,47 @@ def test_lanes_bitwise_equal(gpu):
         np.testing.assert_array_equal(T1, T2)
         np.testing.assert_array_equal(n1, n2)
         np.testing.assert_array_equal(s1, s2)
+
+
+@pytest.mark.parametrize("kind", ["ribbon", "kslab"])
+def test_graph_replay_matches_eager(gpu, kind):
+    """The RGF sweep replayed from a captured hipGraph gives bitwise the eager
+    results, on every replay, with eager work and synchronisation between."""
+    hip, eng = gpu
+    if kind == "ribbon":
+        s = wl.ribbon(12, 9, disorder=0.6, seed=5)   # small blocks: LDS inverse
+    else:
+        s = wl.KSlab.make(n=120, nbl=4, seed=6).at_k(0.5, 0.1)  # blocked inverse
+    ds = hip.DeviceStructure(s)
+    grids = [np.linspace(-2.0, 2.0, 17), np.linspace(-1.1, 0.9, 17), np.linspace(0.2, 2.6, 17)]
+    eager = [hip.solve_transmission(eng, ds, E, delta=1e-5) for E in grids]
+    for rep in range(3):
+        for E, (T0, n0, s0) in zip(grids, eager):
+            T1, n1, s1 = hip.solve_transmission(eng, ds, E, delta=1e-5, flags=hip.GRAPH)
+            np.testing.assert_array_equal(T1, T0)
+            np.testing.assert_array_equal(n1, n0)
+            np.testing.assert_array_equal(s1, s0)
+            Te, _, _ = hip.solve_transmission(eng, ds, E[::-1], delta=1e-5)  # eager work between
+            np.testing.assert_array_equal(Te[::-1], T0)
+
+
+def test_mixed_precision_modes(gpu):
+    """Mixed precision: FP32 sweep (ns_steps=-1) and FP32 inverses refined by
+    FP64 Newton-Schulz steps, against the FP64 engine on the same items.  The
+    bars are the accuracy each mode reaches here (stated per mode)."""
+    hip, eng = gpu
+    s = wl.KSlab.make(n=96, nbl=6, seed=23).at_k(0.3, -0.8)
+    E = np.linspace(-1.5, 1.5, 21)
+    ds = hip.DeviceStructure(s)
+    T64, n64, _ = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    errs = {}
+    for ns in (-1, 1, 2):
+        T, ncyc, st = hip.solve_transmission(eng, ds, E, delta=1e-5, precision=hip.MIXED, ns_steps=ns)
+        assert (st == 0).all()
+        np.testing.assert_array_equal(ncyc, n64)  # contacts stay FP64
+        big = T64 > 1e-6 * T64.max()
+        errs[ns] = float((np.abs(T[big] - T64[big]) / T64[big]).max())
+    print("mixed-precision max rel err of T:", errs)
+    assert errs[-1] < 1e-2
+    assert errs[1] < errs[-1]
+    assert errs[2] <= errs[1] + 1e-12
