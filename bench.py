@@ -145,6 +145,8 @@ def bench_serve(a, hip, wl, torch):
             while ids[i] is None:
                 time.sleep(0.001)
             res.append(srv.wait(ids[i]))
+            log(f"  request {i}: n={sp[i][0]} nbl={sp[i][1]} nE={sp[i][2]} latency {res[-1]['latency']:.2f}s "
+                f"(queue {res[-1]['t_queue']:.2f}s, {res[-1]['chunks']} chunks)")
         th.join()
         wall = time.perf_counter() - t0
         lat = np.array([r["latency"] for r in res])

@@ -3,8 +3,10 @@
 // One 256-thread workgroup (4 waves, 2 x 2) computes a 64 x 64 complex tile of
 // C for one batch item.  K is walked in steps of 16 complex elements staged
 // through LDS as separate real/imaginary planes (k-major), so every MFMA
-// operand fragment is one conflict-free ds_read per lane; the next K step is
-// prefetched into registers while the matrix cores work on the current one.
+// operand fragment is one conflict-free ds_read per lane.  The LDS stage is
+// double-buffered: while the matrix cores work on step t from one buffer,
+// step t+1 (prefetched into registers during step t-1) is stored into the
+// other and step t+2 is fetched from memory -- one barrier per K step.
 // Complex products use four real MFMAs (v_mfma_f64_16x16x4_f64 /
 // v_mfma_f32_16x16x4_f32):  Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br,
 // issued so that no MFMA depends on the one just before it.  Each wave owns a
@@ -63,6 +65,9 @@ template <typename R> struct Smem {
   R Br[BK][BN + PAD];
   R Bi[BK][BN + PAD];
 };
+template <typename R> struct Smem2 {
+  Smem<R> buf[2];
+};
 
 // USE only separates the uses (masked, sweep, inverse update) in profiles.
 template <typename R, int USE>
@@ -79,7 +84,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   const int tm = t % tiles_m, tn = t / tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
 
-  __shared__ Smem<R> sm;
+  __shared__ Smem2<R> sm2;
 
   const CT *A = p.A + (long long)b * p.sA;
   const CT *B = p.B + (long long)b * p.sB;
@@ -115,7 +120,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
       rb[r] = v;
     }
   };
-  auto store = [&]() {
+  auto store = [&](Smem<R> &sm) {
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       int i, kk;
@@ -143,11 +148,15 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
       ci[i][j] = v4{0, 0, 0, 0};
     }
 
-  if (K > 0) load(0);
-  for (int k0 = 0; k0 < K; k0 += BK) {
-    store();
+  int cur = 0;
+  if (K > 0) {
+    load(0);
+    store(sm2.buf[0]);
     __syncthreads();
-    if (k0 + BK < K) load(k0 + BK);  // in flight while the matrix cores run
+    if (BK < K) load(BK);
+  }
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    Smem<R> &sm = sm2.buf[cur];
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4), c16 = lane & 15;
@@ -163,7 +172,6 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
         br[j] = sm.Br[kr][wn * 32 + j * 16 + c16];
         bi[j] = sm.Bi[kr][wn * 32 + j * 16 + c16];
       }
-      // 8 independent accumulators: no MFMA waits on the one issued just before
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -179,8 +187,14 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
           ci[i][j] = mfma(ai[i], br[j], ci[i][j]);
         }
     }
+    if (k0 + BK < K) {
+      store(sm2.buf[cur ^ 1]);          // next K step into the other buffer
+      if (k0 + 2 * BK < K) load(k0 + 2 * BK);
+    }
     __syncthreads();
+    cur ^= 1;
   }
+  Smem<R> &sm = sm2.buf[0];
 
   // ---- epilogue through LDS: the four waves stage a 64 x 32 slab (tile
   // columns j*16..j*16+15 of both wave columns) per pass, then every thread

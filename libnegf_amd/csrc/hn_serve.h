@@ -32,6 +32,7 @@ struct ServeOpts {
   int policy = 0;               // 0 FCFS, 1 least attained service
   int max_chunk = 0;            // items per chunk cap (0: automatic)
   double chunk_flops = 2e12;    // target algorithmic work per chunk (automatic sizing)
+  int min_chunk = 64;           // items per chunk at least (keeps large-n chunks wide)
   size_t mem_budget = 0;        // bytes for structures + workspaces (0: 85 % of free HBM)
   size_t worker_ws = 0;         // workspace bytes per worker (0: budget / (4 workers))
 };

@@ -131,7 +131,8 @@ void Server::admit(Req &r, Engine &eng) {
   const int nm = r.st->nmax();
   const double n3 = (double)nm * nm * nm;
   const double per_item_flops = (56.0 * 2 * 25 + 24.0 * r.st->nbl + 64.0) * n3;
-  long long c = (long long)std::floor(o_.chunk_flops / per_item_flops);
+  long long c = std::max<long long>((long long)std::floor(o_.chunk_flops / per_item_flops),
+                                    o_.min_chunk);
   const size_t ib = Engine::transmission_item_bytes(*r.st, 1, &r.o.ni, &r.o.nf) +
                     inv_work_bytes(nm, 1, 8);
   c = std::min<long long>(c, (long long)(o_.worker_ws / std::max<size_t>(ib, 1)));
