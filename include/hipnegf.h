@@ -189,7 +189,17 @@ typedef struct {
  * HIPNEGF_GRAPH: replay the RGF sweep of a batch (the launch chain after the
  * contacts, which has no host synchronisation) from a captured hipGraph,
  * cached per structure and batch shape; results are identical. */
-enum hipnegf_opt_flags { HIPNEGF_NO_SHARED_DECIMATION = 1, HIPNEGF_GRAPH = 2 };
+enum hipnegf_opt_flags {
+  HIPNEGF_NO_SHARED_DECIMATION = 1,
+  HIPNEGF_GRAPH = 2,
+  /* surface Green's function cache (surface_green with the ln_cache memory
+   * cache, contselfenergy.F90:138-173, 251-259; needs
+   * hipnegf_engine_set_sgf_cache): LOAD = the reference's ReadOldSGF 0 (use
+   * cached g_s when every item of a batch is cached, else compute and store;
+   * loaded items report ncyc 0), STORE = ReadOldSGF 2 (compute and store). */
+  HIPNEGF_SGF_LOAD = 4,
+  HIPNEGF_SGF_STORE = 8
+};
 
 /* Coherent transmission T(E_i, slot_i), i < nitems (iterative.F90:1669-1780
  * per item).  Host arrays: E[nitems], slot[nitems] in; T[nitems],
@@ -292,6 +302,16 @@ int hipnegf_server_wait(hipnegf_server s, unsigned long long id, double *T, int 
                         double *times);
 /* {submitted, completed, running, queued} */
 int hipnegf_server_stats(hipnegf_server s, long long *out);
+
+/* Device memory for the engine's surface Green's function cache (0: off).
+ * Entries are keyed by (structure and its contents version, slot, E, eta,
+ * contact); least recently used entries are replaced. */
+int hipnegf_engine_set_sgf_cache(hipnegf_engine e, long long bytes);
+/* cache hits (items x contacts loaded) and stores since the engine started */
+int hipnegf_engine_sgf_stats(hipnegf_engine e, long long *hits, long long *stores);
+/* Fictitious contact (pnegf%cont(c)%FictCont, contselfenergy.F90:175-185):
+ * g_s = -i pi dos I instead of the decimation; contact is 0-based. */
+int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious, double dos);
 
 /* Launch timing (no counterpart in the reference; measurement hook for
  * benchmarks).  While enabled, every tagged launch records a HIP event pair on

@@ -50,6 +50,10 @@ struct Session {
   int status = 0;
   std::string err;
   std::unique_ptr<hn::Engine> eng;
+  // device copy of the working H/S, rebuilt only after H, S or the partition change
+  std::unique_ptr<hn::DevStruct> dev;
+  long long gen = 0, dev_gen = -1;
+  bool sgf_cache_on = false;
 };
 
 Session *sess(int *handler) {
@@ -258,8 +262,35 @@ std::vector<int> pairs_of(const int *v) {
   return out;
 }
 
-hn::SolveOpts opts_of(const Session &s) {
+// The device structure of the working H/S (built once per change of H, S or
+// the partition); fictitious-contact parameters are applied on every call.
+hn::DevStruct &device_struct(Session &s) {
+  if (!s.dev || s.dev_gen != s.gen) {
+    s.dev = build_struct(s);
+    s.dev_gen = s.gen;
+  }
+  hn::DevStruct &d = *s.dev;
+  for (int c = 0; c < d.ncont; ++c) {
+    const int f = s.p.fictcont[c] ? 1 : 0;
+    if (f != d.fict[c] || (f && d.contact_dos[c] != s.p.contact_dos[c])) {
+      d.fict[c] = f;
+      d.contact_dos[c] = s.p.contact_dos[c];
+      d.touched();
+    }
+  }
+  return d;
+}
+
+hn::SolveOpts opts_of(Session &s) {
   hn::SolveOpts o;
+  // readOldT_SGFs (lib_param.F90): 0 load cached g_s (else compute and save),
+  // 1 compute, 2 compute and save -- kept in device memory here
+  const int r = s.p.readold_t_sgfs;
+  o.sgf_mode = (r == 0) ? 1 : (r == 2) ? 2 : 0;
+  if (o.sgf_mode && !s.sgf_cache_on) {
+    engine(s).set_sgf_cache(2ull << 30);
+    s.sgf_cache_on = true;
+  }
   o.delta = s.p.delta;
   o.precision = s.precision;
   o.max_batch = s.max_batch;
@@ -310,7 +341,7 @@ void compute_landauer(Session &s) {
   s.item_status.assign(s.npoints, 0);
   s.item_ncyc.assign((size_t)s.npoints * std::max(s.ncont, 1), 0);
   if (s.ncont < 2) return;  // calculate_transmissions: one contact -> T = 0
-  auto st = build_struct(s);
+  hn::DevStruct *st = &device_struct(s);
   hn::Engine &eng = engine(s);
   const int n = s.npoints;
   DevBuf dE(sizeof(double) * n), dsl(sizeof(int) * n), dN(sizeof(int) * n * s.ncont),
@@ -341,7 +372,7 @@ void compute_dephasing_transmission(Session &s) {
   std::vector<double> frm(s.ncont + 1, 0.0);
   frm[ni[0] - 1] = 1.0;
   frm[nf[0] - 1] = 0.0;
-  auto st = build_struct(s);
+  hn::DevStruct *st = &device_struct(s);
   int ndev = 0;
   for (int b : st->bsz) ndev += b;
   if ((int)s.coupling.size() < ndev) throw std::runtime_error("dephasing coupling shorter than the device");
@@ -424,6 +455,7 @@ void negf_init(int *handler) {
   api(handler, [](Session &s) {
     set_defaults(s.p);
     s.H.assign(1, Csr());
+    ++s.gen;
     s.S.assign(1, Csr());
     s.s_id.assign(1, 0);
     s.cur = 0;
@@ -438,7 +470,10 @@ void negf_destruct_libnegf(int *handler) {
     s.curr_mat.clear();
     s.currents.clear();
     s.deph = false;
+    s.dev.reset();
     s.eng.reset();
+    s.sgf_cache_on = false;
+    ++s.gen;
   });
 }
 
@@ -479,6 +514,7 @@ void negf_init_structure(int *handler, int ncont, int *surfstart, int *surfend, 
     s.cblk.assign(cblk, cblk + ncont);
     s.plend.assign(plend, plend + npl);
     s.structure_set = true;
+    ++s.gen;
   });
 }
 
@@ -494,6 +530,7 @@ void negf_create_hs(int *handler, int nHk) {
   api(handler, [&](Session &s) {
     if (nHk < 1) throw std::runtime_error("nHk < 1");
     s.H.assign(nHk, Csr());
+    ++s.gen;
     s.S.assign(nHk, Csr());
     s.s_id.assign(nHk, 0);
   });
@@ -510,6 +547,7 @@ static void set_mat(Session &s, bool is_h, int nrow, const double2 *A, const int
     s.s_id[k] = 0;
   }
   s.cur = k;
+  ++s.gen;
 }
 
 void negf_set_h(int *handler, int nrow, const double *A, int *JA, int *IA, int iK) {
@@ -541,6 +579,7 @@ void negf_set_s_id(int *handler, int nrow, int iK) {
     s.S[k] = Csr();
     s.S[k].n = 0;
     s.s_id[k] = 1;
+    ++s.gen;
     (void)nrow;
   });
 }
@@ -548,7 +587,10 @@ void negf_set_s_id(int *handler, int nrow, int iK) {
 void negf_set_kpoint(int *handler, int kpoint) {
   api(handler, [&](Session &s) {
     s.p.kpoint = kpoint;
-    if (kpoint >= 1 && kpoint <= (int)s.H.size()) s.cur = kpoint - 1;
+    if (kpoint >= 1 && kpoint <= (int)s.H.size() && s.cur != kpoint - 1) {
+      s.cur = kpoint - 1;
+      ++s.gen;
+    }
   });
 }
 

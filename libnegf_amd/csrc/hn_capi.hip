@@ -247,6 +247,7 @@ hn::SolveOpts to_opts(const hipnegf_opts *o) {
     so.eps_drop = o->eps_drop;
     so.share_mirror = (o->flags & HIPNEGF_NO_SHARED_DECIMATION) ? 0 : 1;
     so.graph = (o->flags & HIPNEGF_GRAPH) ? 1 : 0;
+    so.sgf_mode = (o->flags & HIPNEGF_SGF_LOAD) ? 1 : (o->flags & HIPNEGF_SGF_STORE) ? 2 : 0;
     so.lanes = std::max(1, std::min(o->lanes, 8));
   }
   return so;
@@ -456,7 +457,7 @@ int hipnegf_struct_set_block(hipnegf_struct st, int slot, int kind, int index, c
     long long n = 0;
     double2 *d = block_ptr(*st->st, slot, kind, index, &n);
     HN_CHECK(hipMemcpy(d, host, n * sizeof(double2), hipMemcpyHostToDevice));
-    st->st->mirror_state = -1;
+    st->st->touched();
     return 0;
   });
 }
@@ -466,7 +467,7 @@ int hipnegf_struct_set_block_device(hipnegf_struct st, int slot, int kind, int i
     long long n = 0;
     double2 *d = block_ptr(*st->st, slot, kind, index, &n);
     HN_CHECK(hipMemcpy(d, dev, n * sizeof(double2), hipMemcpyDeviceToDevice));
-    st->st->mirror_state = -1;
+    st->st->touched();
     return 0;
   });
 }
@@ -635,6 +636,31 @@ int hipnegf_zinverse_batched(hipnegf_engine e, int n, void *A, long long sA, voi
 int hipnegf_cinverse_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv,
                              long long sAinv, int batch, int *dstatus) {
   return guard([&] { return inv_batched<float>(e, n, A, sA, Ainv, sAinv, batch, dstatus); });
+}
+
+int hipnegf_engine_set_sgf_cache(hipnegf_engine e, long long bytes) {
+  return guard([&] {
+    if (bytes < 0) throw std::runtime_error("negative cache size");
+    e->eng->set_sgf_cache((size_t)bytes);
+    return 0;
+  });
+}
+int hipnegf_engine_sgf_stats(hipnegf_engine e, long long *hits, long long *stores) {
+  return guard([&] {
+    if (hits) *hits = e->eng->sgf_hits();
+    if (stores) *stores = e->eng->sgf_stores();
+    return 0;
+  });
+}
+int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious, double dos) {
+  return guard([&] {
+    hn::DevStruct &S = *st->st;
+    if (contact < 0 || contact >= S.ncont) throw std::runtime_error("contact out of range");
+    S.fict[contact] = fictitious ? 1 : 0;
+    S.contact_dos[contact] = dos;
+    S.touched();
+    return 0;
+  });
 }
 
 int hipnegf_profile_enable(int on) {

@@ -1,6 +1,8 @@
 // hn_engine.h -- batched per-(E,k) Green's-function engine (host side).
 #pragma once
+#include <functional>
 #include <memory>
+#include <unordered_map>
 #include <vector>
 
 #include "hn_common.h"
@@ -29,6 +31,12 @@ struct DevStruct {
   double2 *pool = nullptr;
   int device = 0;
   int mirror_state = -1;  // contacts 1/2 mirror images in every slot: -1 unknown, 0 no, 1 yes
+  unsigned long long version = 0;  // bumped whenever block contents change (cache keys)
+  unsigned long long uid = 0;      // process-unique id of this structure
+  // fictitious contacts (contselfenergy.F90:175-185): GS = -i pi dos I, no decimation
+  int fict[kMaxCont] = {0};
+  double contact_dos[kMaxCont] = {0.0};
+  void touched() { mirror_state = -1; ++version; }
 
   bool owned = true;       // pool allocated (and freed) by this object
 
@@ -86,6 +94,9 @@ struct SolveOpts {
   int share_mirror = 1;      // one decimation for mirror-image contact pairs
   int lanes = 1;             // item groups solved concurrently, each on its own stream
   int graph = 0;             // replay the RGF sweep of a batch from a captured hipGraph
+  // surface Green's function cache (surface_green + ln_cache, contselfenergy.F90:138-173,
+  // 251-259): 0 compute only; 1 load if cached, else compute and store; 2 compute and store
+  int sgf_mode = 0;
 };
 
 struct SCBAOpts {
@@ -133,6 +144,10 @@ class Engine {
                       const double2 *hCo, double acc, int crit, int *ncyc);
 
   int last_batch() const { return last_batch_; }
+  // device memory for the surface Green's function cache (0 disables it)
+  void set_sgf_cache(size_t bytes);
+  long long sgf_hits() const { return sgf_hits_; }
+  long long sgf_stores() const { return sgf_stores_; }
   // Sub-engines (own stream and workspace) for concurrent item groups.
   Engine &lane(int i);
   size_t workspace_bytes() const { return arena_.cap; }
@@ -144,6 +159,37 @@ class Engine {
   int *h_count_ = nullptr;  // pinned
   int last_batch_ = 0;
   std::vector<std::unique_ptr<Engine>> lanes_;
+  // surface Green's function cache: fixed-size entries in one device buffer, LRU
+  struct SgfKey {
+    unsigned long long uid, version, ebits, dbits;
+    int slot, contact, nc;
+    bool operator==(const SgfKey &o) const {
+      return uid == o.uid && version == o.version && ebits == o.ebits && dbits == o.dbits &&
+             slot == o.slot && contact == o.contact && nc == o.nc;
+    }
+  };
+  struct SgfKeyHash {
+    size_t operator()(const SgfKey &k) const {
+      size_t h = std::hash<unsigned long long>()(k.uid * 1000003ULL ^ k.version);
+      h ^= std::hash<unsigned long long>()(k.ebits) + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+      h ^= std::hash<unsigned long long>()(k.dbits ^ ((unsigned long long)k.slot << 20) ^
+                                          ((unsigned long long)k.contact << 8) ^ (unsigned long long)k.nc);
+      return h;
+    }
+  };
+  double2 *sgf_pool_ = nullptr;
+  size_t sgf_bytes_ = 0;
+  long long sgf_entry_elems_ = 0;
+  int sgf_capacity_ = 0;
+  std::unordered_map<SgfKey, int, SgfKeyHash> sgf_map_;
+  std::vector<SgfKey> sgf_keys_;
+  std::vector<unsigned long long> sgf_used_;  // LRU stamps
+  std::vector<int> sgf_valid_;
+  std::vector<int> sgf_status_;  // decimation status bits stored with each entry
+  unsigned long long sgf_clock_ = 0;
+  long long sgf_hits_ = 0, sgf_stores_ = 0;
+  bool sgf_prepare(long long entry_elems);
+  int sgf_slot_for(const SgfKey &k);  // insert (evicting LRU), returns entry index
   struct GraphEntry {
     std::vector<long long> key;
     hipGraphExec_t exec = nullptr;

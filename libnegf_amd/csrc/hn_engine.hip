@@ -14,6 +14,7 @@
 // sits at base + b * stride.  Per-item convergence (decimation, SCBA) is
 // handled with device-side activity masks; kernels skip finished items.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <functional>
 #include <string>
@@ -60,6 +61,11 @@ long long layout(int nbl, const std::vector<int> &bsz, int ncont, const std::vec
   return o;
 }
 
+unsigned long long next_uid() {
+  static std::atomic<unsigned long long> counter{1};
+  return counter.fetch_add(1);
+}
+
 void check_shape(int nbl, const std::vector<int> &bsz, int ncont, const std::vector<int> &cblk,
                  const std::vector<int> &nc, int nslots) {
   if (nbl < 1) throw std::runtime_error("structure: nbl < 1");
@@ -88,6 +94,7 @@ DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, co
   nc.assign(nc_, nc_ + std::max(ncont, 0));
   check_shape(nbl, bsz, ncont, cblk, nc, nslots);
   slot_elems = layout(nbl, bsz, ncont, cblk, nc, smode, off);
+  uid = next_uid();
   HN_CHECK(hipGetDevice(&device));
   HN_CHECK(hipMalloc(&pool, bytes()));
   HN_CHECK(hipMemset(pool, 0, bytes()));
@@ -101,6 +108,7 @@ DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, co
   nc.assign(nc_, nc_ + std::max(ncont, 0));
   check_shape(nbl, bsz, ncont, cblk, nc, nslots);
   slot_elems = layout(nbl, bsz, ncont, cblk, nc, smode, off);
+  uid = next_uid();
   HN_CHECK(hipGetDevice(&device));
   pool = (double2 *)external;
   owned = false;
@@ -174,6 +182,7 @@ void Engine::clear_graphs() {
 Engine::~Engine() {
   (void)hipStreamSynchronize(stream_);
   clear_graphs();
+  if (sgf_pool_) (void)hipFree(sgf_pool_);
   if (h_count_) (void)hipHostFree(h_count_);
   (void)hipStreamDestroy(stream_);
 }
@@ -431,12 +440,67 @@ void Engine::decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 
   arena_.used = mark;
 }
 
+// ---------------------------------------------------------------------------
+// Surface Green's function cache (the role of ln_cache's TMatrixCacheMem for
+// surface_green, contselfenergy.F90:138-173, 251-259): g_s of (structure
+// version, slot, E, eta, contact) kept in one device buffer of fixed-size
+// entries, least recently used out first.
+// ---------------------------------------------------------------------------
+void Engine::set_sgf_cache(size_t bytes) {
+  HN_CHECK(hipStreamSynchronize(stream_));
+  if (sgf_pool_) HN_CHECK(hipFree(sgf_pool_));
+  sgf_pool_ = nullptr;
+  sgf_bytes_ = bytes;
+  sgf_entry_elems_ = 0;
+  sgf_capacity_ = 0;
+  sgf_map_.clear();
+  sgf_keys_.clear();
+  sgf_used_.clear();
+  sgf_valid_.clear();
+  if (bytes) HN_CHECK(hipMalloc(&sgf_pool_, bytes));
+}
+
+bool Engine::sgf_prepare(long long entry_elems) {
+  if (!sgf_pool_) return false;
+  if (entry_elems > sgf_entry_elems_) {  // (re)partition for larger blocks; drops the entries
+    sgf_entry_elems_ = entry_elems;
+    sgf_capacity_ = (int)std::min<size_t>(sgf_bytes_ / ((size_t)entry_elems * sizeof(double2)),
+                                          1u << 30);
+    sgf_map_.clear();
+    sgf_keys_.assign(sgf_capacity_, SgfKey{});
+    sgf_used_.assign(sgf_capacity_, 0);
+    sgf_valid_.assign(sgf_capacity_, 0);
+    sgf_status_.assign(sgf_capacity_, 0);
+  }
+  return sgf_capacity_ > 0;
+}
+
+int Engine::sgf_slot_for(const SgfKey &k) {
+  auto it = sgf_map_.find(k);
+  if (it != sgf_map_.end()) {
+    sgf_used_[it->second] = ++sgf_clock_;
+    return it->second;
+  }
+  int victim = 0;
+  for (int i = 0; i < sgf_capacity_; ++i) {
+    if (!sgf_valid_[i]) { victim = i; break; }
+    if (sgf_used_[i] < sgf_used_[victim]) victim = i;
+  }
+  if (sgf_valid_[victim]) sgf_map_.erase(sgf_keys_[victim]);
+  sgf_keys_[victim] = k;
+  sgf_valid_[victim] = 1;
+  sgf_used_[victim] = ++sgf_clock_;
+  sgf_map_[k] = victim;
+  return victim;
+}
+
 // Contact self-energies of all contacts for B items.
 // Sig: ncont x B blocks of nd x nd (nd = bsz[cblk-1]), stride sSig, contact-major.
 void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
                       const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c,
                       int *status_c) {
   const bool sc = (st.smode & S_CONTACT) != 0;
+  const int ncont = st.ncont;
   auto self_energy = [&](int c, const double2 *GS) {
     // SelfEnergy: Tlc = z SMC - HMC, Tp = z* SMC - HMC, Sigma = Tlc GS Tp^H
     const int nc = st.nc[c], nd = st.bsz[st.cblk[c] - 1];
@@ -456,7 +520,91 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
              nd, Z(1), Z(0), B, nullptr), stream_);
     arena_.used = mark;
   };
-  if (o.share_mirror && mirror_contacts(st)) {
+  auto fictitious = [&](int c, double2 *Go) {  // GS = -i pi dos I
+    const int nc = st.nc[c];
+    fill_diag(Go, (long long)nc * nc, nc, Z(0.0, -3.141592653589793 * st.contact_dos[c]), B, stream_);
+  };
+  const bool shared = o.share_mirror && ncont == 2 && !st.fict[0] && !st.fict[1] && mirror_contacts(st);
+
+  // ---- surface Green's function cache ----
+  int ncmax = 0;
+  for (int c = 0; c < ncont; ++c)
+    if (!st.fict[c]) ncmax = std::max(ncmax, st.nc[c]);
+  const bool cache = o.sgf_mode != 0 && ncmax > 0 && sgf_prepare((long long)ncmax * ncmax);
+  std::vector<double> hE;
+  std::vector<int> hS;
+  int *didx = nullptr;
+  if (cache) {
+    hE.resize(B);
+    hS.resize(B);
+    HN_CHECK(hipMemcpyAsync(hE.data(), dE, sizeof(double) * B, hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipMemcpyAsync(hS.data(), dslot, sizeof(int) * B, hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+    didx = arena_.take<int>(B);
+  }
+  auto key = [&](int b, int c) {
+    SgfKey k;
+    k.uid = st.uid;
+    k.version = st.version;
+    std::memcpy(&k.ebits, &hE[b], sizeof(double));
+    std::memcpy(&k.dbits, &o.delta, sizeof(double));
+    k.slot = hS[b];
+    k.contact = c + (shared ? 64 : 0);  // the shared recursion rounds differently
+    k.nc = st.nc[c];
+    return k;
+  };
+  if (cache && o.sgf_mode == 1) {  // ReadOldSGF = 0: load when every item is cached
+    bool all = true;
+    for (int c = 0; c < ncont && all; ++c)
+      if (!st.fict[c])
+        for (int b = 0; b < B && all; ++b) all = sgf_map_.count(key(b, c)) > 0;
+    if (all) {
+      size_t mark = arena_.used;
+      double2 *Go = arena_.take<double2>((long long)ncmax * ncmax * B);
+      std::vector<int> hst((size_t)B * ncont, 0), idx(B);
+      for (int c = 0; c < ncont; ++c) {
+        const long long s2 = (long long)st.nc[c] * st.nc[c];
+        if (st.fict[c]) {
+          fictitious(c, Go);
+        } else {
+          for (int b = 0; b < B; ++b) {
+            idx[b] = sgf_slot_for(key(b, c));  // a hit: refreshes the entry's LRU stamp
+            hst[(size_t)c * B + b] = sgf_status_[idx[b]];
+          }
+          HN_CHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
+          copy_indexed(Go, s2, nullptr, sgf_pool_, sgf_entry_elems_, didx, s2, B, stream_);
+          sgf_hits_ += B;
+        }
+        self_energy(c, Go);
+        HN_CHECK(hipStreamSynchronize(stream_));  // idx is reused by the next contact
+      }
+      // loaded surface functions report ncyc = 0 (contselfenergy.F90:158, 259)
+      HN_CHECK(hipMemcpyAsync(status_c, hst.data(), sizeof(int) * B * ncont, hipMemcpyHostToDevice,
+                              stream_));
+      HN_CHECK(hipStreamSynchronize(stream_));
+      arena_.used = mark;
+      return;
+    }
+  }
+  auto store = [&](int c, const double2 *Go) {
+    if (!cache || st.fict[c]) return;
+    const long long s2 = (long long)st.nc[c] * st.nc[c];
+    std::vector<int> hs(B), idx(B, -1);
+    HN_CHECK(hipMemcpyAsync(hs.data(), status_c + (long long)c * B, sizeof(int) * B,
+                            hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipStreamSynchronize(stream_));
+    int stored = 0;
+    for (int b = 0; b < B && stored < sgf_capacity_; ++b, ++stored) {
+      idx[b] = sgf_slot_for(key(b, c));
+      sgf_status_[idx[b]] = hs[b];
+    }
+    HN_CHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
+    copy_indexed(sgf_pool_, sgf_entry_elems_, didx, Go, s2, nullptr, s2, B, stream_);
+    HN_CHECK(hipStreamSynchronize(stream_));
+    sgf_stores_ += stored;
+  };
+
+  if (shared) {
     const int nc = st.nc[0];
     const long long s2 = (long long)nc * nc;
     size_t mark = arena_.used;
@@ -477,32 +625,37 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     form_block(f, stream_);
     decimate_shared(nc, B, Ao, Aos0, Aos1, Bo, Co, Go0, Go1, o, ncyc_c, ncyc_c + B, status_c,
                     status_c + B);
+    store(0, Go0);
+    store(1, Go1);
     self_energy(0, Go0);
     self_energy(1, Go1);
     arena_.used = mark;
     return;
   }
-  for (int c = 0; c < st.ncont; ++c) {
-    const int nc = st.nc[c], nd = st.bsz[st.cblk[c] - 1];
+  for (int c = 0; c < ncont; ++c) {
+    const int nc = st.nc[c];
     const long long s2 = (long long)nc * nc;
     size_t mark = arena_.used;
-    double2 *Ao = arena_.take<double2>(s2 * B), *Aos = arena_.take<double2>(s2 * B);
-    double2 *Bo = arena_.take<double2>(s2 * B), *Co = arena_.take<double2>(s2 * B);
     double2 *Go = arena_.take<double2>(s2 * B);
-    // surface_green: Ao = z S00 - H00 (also Ao_s), Bo = z S01 - H01, Co = (z* S01 - H01)^H
-    FormArgs f;
-    f.slot = dslot; f.E = dE; f.eta = o.delta; f.batch = B; f.rows = nc; f.cols = nc;
-    f.H = st.ref(BK_H00, c); f.S = sc ? st.ref(BK_S00, c) : BlkRef(); f.s_identity = !sc;
-    f.out = Ao; f.s_out = s2; f.out2 = Aos; f.s_out2 = s2;
-    form_block(f, stream_);
-    f.H = st.ref(BK_H01, c); f.S = sc ? st.ref(BK_S01, c) : BlkRef(); f.s_identity = 0;
-    f.out = Bo; f.out2 = nullptr;
-    form_block(f, stream_);
-    f.conjz = 1; f.dagger = 1; f.out = Co;
-    form_block(f, stream_);
-    int *nc_item = ncyc_c + (long long)c * B;
-    int *st_item = status_c + (long long)c * B;
-    decimate_z(nc, B, Ao, Aos, Bo, Co, Go, o, nc_item, st_item);
+    if (st.fict[c]) {
+      fictitious(c, Go);  // ncyc stays 0
+    } else {
+      double2 *Ao = arena_.take<double2>(s2 * B), *Aos = arena_.take<double2>(s2 * B);
+      double2 *Bo = arena_.take<double2>(s2 * B), *Co = arena_.take<double2>(s2 * B);
+      // surface_green: Ao = z S00 - H00 (also Ao_s), Bo = z S01 - H01, Co = (z* S01 - H01)^H
+      FormArgs f;
+      f.slot = dslot; f.E = dE; f.eta = o.delta; f.batch = B; f.rows = nc; f.cols = nc;
+      f.H = st.ref(BK_H00, c); f.S = sc ? st.ref(BK_S00, c) : BlkRef(); f.s_identity = !sc;
+      f.out = Ao; f.s_out = s2; f.out2 = Aos; f.s_out2 = s2;
+      form_block(f, stream_);
+      f.H = st.ref(BK_H01, c); f.S = sc ? st.ref(BK_S01, c) : BlkRef(); f.s_identity = 0;
+      f.out = Bo; f.out2 = nullptr;
+      form_block(f, stream_);
+      f.conjz = 1; f.dagger = 1; f.out = Co;
+      form_block(f, stream_);
+      decimate_z(nc, B, Ao, Aos, Bo, Co, Go, o, ncyc_c + (long long)c * B, status_c + (long long)c * B);
+      store(c, Go);
+    }
     self_energy(c, Go);
     arena_.used = mark;
   }

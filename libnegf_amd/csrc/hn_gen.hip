@@ -177,7 +177,7 @@ void gen_kslab(DevStruct &st, int slot, unsigned long long seed, double kx, doub
     one(BK_S01, 1, r1ct);
     one(BK_SMC, 1, r1ct);
   }
-  st.mirror_state = -1;
+  st.touched();
 }
 
 }  // namespace hn

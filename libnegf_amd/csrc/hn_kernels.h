@@ -102,6 +102,12 @@ void maxabs_diff(const double2 *X, const double2 *Y, long long sX, long long sY,
 void fill_batched(double2 *out, long long so, long long n, int batch, double2 value,
                   hipStream_t st);
 void fill_int(int *out, int n, int value, hipStream_t st);
+// dst item (didx ? didx[b] : b) = src item (sidx ? sidx[b] : b), n complex each;
+// items whose index is negative are skipped
+void copy_indexed(double2 *dst, long long sd, const int *didx, const double2 *src, long long ss,
+                  const int *sidx, long long n, int batch, hipStream_t st);
+// out_b = diag(v) (n x n) for every item
+void fill_diag(double2 *out, long long so, int n, double2 v, int batch, hipStream_t st);
 // dst[i] = src[i] (kernel copies, usable inside captured graphs)
 void copy_real(double *dst, const double *src, int n, hipStream_t st);
 void copy_int(int *dst, const int *src, int n, hipStream_t st);

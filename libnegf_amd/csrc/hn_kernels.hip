@@ -481,6 +481,24 @@ __global__ __launch_bounds__(NT) void k_trace_mw(const double2 *Gam, const doubl
   if (threadIdx.x == 0) curr[(long long)b * ncont + c] = -v;
 }
 
+__global__ __launch_bounds__(NT) void k_copy_indexed(double2 *dst, long long sd, const int *didx,
+                                                     const double2 *src, long long ss,
+                                                     const int *sidx, long long n) {
+  const int b = blockIdx.y;
+  const long long di = didx ? didx[b] : b, si = sidx ? sidx[b] : b;
+  if (di < 0 || si < 0) return;
+  for (long long e = (long long)blockIdx.x * NT + threadIdx.x; e < n; e += (long long)gridDim.x * NT)
+    dst[di * sd + e] = src[si * ss + e];
+}
+
+__global__ __launch_bounds__(NT) void k_fill_diag(double2 *out, long long so, int n, double2 v) {
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)n * n) return;
+  const int r = (int)(e % n), c = (int)(e / n);
+  out[(long long)b * so + e] = (r == c) ? v : make_double2(0.0, 0.0);
+}
+
 template <typename T>
 __global__ __launch_bounds__(NT) void k_copy1(T *dst, const T *src, int n) {
   const int i = blockIdx.x * NT + threadIdx.x;
@@ -609,6 +627,18 @@ void fill_int(int *out, int n, int value, hipStream_t st) {
   HN_CHECK(hipGetLastError());
 }
 
+void copy_indexed(double2 *dst, long long sd, const int *didx, const double2 *src, long long ss,
+                  const int *sidx, long long n, int batch, hipStream_t st) {
+  if (batch <= 0 || n <= 0) return;
+  const unsigned gx = (unsigned)std::min<long long>((n + NT - 1) / NT, 64);
+  hipLaunchKernelGGL(k_copy_indexed, dim3(gx, batch), dim3(NT), 0, st, dst, sd, didx, src, ss, sidx, n);
+  HN_CHECK(hipGetLastError());
+}
+void fill_diag(double2 *out, long long so, int n, double2 v, int batch, hipStream_t st) {
+  if (batch <= 0) return;
+  hipLaunchKernelGGL(k_fill_diag, grid2((long long)n * n, batch), dim3(NT), 0, st, out, so, n, v);
+  HN_CHECK(hipGetLastError());
+}
 void copy_real(double *dst, const double *src, int n, hipStream_t st) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_copy1<double>, dim3((n + NT - 1) / NT), dim3(NT), 0, st, dst, src, n);

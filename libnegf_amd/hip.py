@@ -36,6 +36,8 @@ class Opts(C.Structure):
                 ("lanes", C.c_int)]
 NO_SHARED_DECIMATION = 1
 GRAPH = 2
+SGF_LOAD = 4
+SGF_STORE = 8
 
 
 def load():
@@ -89,6 +91,9 @@ def load():
     L.hipnegf_Zdecimation.argtypes = [vp, vp, vp, vp, vp, vp, C.c_size_t, i, C.POINTER(C.c_int), d]
     L.hipnegf_Cdecimation.argtypes = [vp, vp, vp, vp, vp, vp, C.c_size_t, i, C.POINTER(C.c_int),
                                       C.c_float]
+    L.hipnegf_engine_set_sgf_cache.argtypes = [vp, ll]
+    L.hipnegf_engine_sgf_stats.argtypes = [vp, C.POINTER(ll), C.POINTER(ll)]
+    L.hipnegf_struct_set_fictitious.argtypes = [vp, i, i, d]
     L.hipnegf_profile_enable.argtypes = [i]
     L.hipnegf_profile_read.argtypes = [i, C.POINTER(d), C.POINTER(d), C.POINTER(d),
                                        C.POINTER(ll)]
@@ -139,6 +144,14 @@ class Engine:
     @property
     def handle(self):
         return self._h
+
+    def set_sgf_cache(self, nbytes: int):
+        check(load().hipnegf_engine_set_sgf_cache(self._h, int(nbytes)))
+
+    def sgf_stats(self):
+        h, s = C.c_longlong(), C.c_longlong()
+        check(load().hipnegf_engine_sgf_stats(self._h, C.byref(h), C.byref(s)))
+        return h.value, s.value
 
     def sync(self):
         check(load().hipnegf_engine_sync(self._h))
@@ -203,6 +216,10 @@ class DeviceStructure:
         check(load().hipnegf_struct_create(nbl, _ptr(bsz), len(cba), _ptr(cba), _ptr(nc), smode,
                                            nslots, C.byref(self._h)))
         return self
+
+    def set_fictitious(self, contact: int, dos: float, fictitious: bool = True):
+        """contact 0-based; g_s = -i pi dos I for that contact."""
+        check(load().hipnegf_struct_set_fictitious(self._h, contact, 1 if fictitious else 0, dos))
 
     def set_block(self, slot: int, kind: int, index: int, a: np.ndarray):
         a = np.asfortranarray(a, dtype=np.complex128)

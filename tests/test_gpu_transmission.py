@@ -284,3 +284,65 @@ def test_mixed_precision_modes(gpu):
     assert errs[-1] < 1e-2
     assert errs[1] < errs[-1]
     assert errs[2] <= errs[1] + 1e-12
+
+
+def test_sgf_cache_load_and_store(gpu):
+    """Surface Green's function cache (ReadOldSGF semantics): a LOAD solve on
+    energies stored before reproduces T bitwise, reports ncyc 0 for the loaded
+    surface functions, and changed blocks invalidate the entries."""
+    hip, eng = gpu
+    e2 = hip.Engine(0)
+    e2.set_sgf_cache(64 << 20)
+    s = wl.ribbon(20, 7, disorder=0.5, seed=44)
+    ds = hip.DeviceStructure(s)
+    E = np.linspace(-2.2, 2.2, 13)
+    T0, n0, _ = hip.solve_transmission(e2, ds, E, delta=1e-5)
+    T1, n1, _ = hip.solve_transmission(e2, ds, E, delta=1e-5, flags=hip.SGF_STORE)
+    np.testing.assert_array_equal(T1, T0)
+    np.testing.assert_array_equal(n1, n0)
+    h0, st0 = e2.sgf_stats()
+    assert st0 == 2 * len(E)
+    T2, n2, _ = hip.solve_transmission(e2, ds, E, delta=1e-5, flags=hip.SGF_LOAD)
+    np.testing.assert_array_equal(T2, T0)
+    assert (n2 == 0).all()
+    assert e2.sgf_stats()[0] - h0 == 2 * len(E)
+    # a new energy forces the computation (and stores it); results unchanged
+    E3 = np.append(E, 0.123)
+    T3, n3, _ = hip.solve_transmission(e2, ds, E3, delta=1e-5, flags=hip.SGF_LOAD)
+    np.testing.assert_array_equal(T3[:-1], T0)
+    assert (n3[:-1] > 0).all()
+    # changing a contact block changes the structure version: no stale hits
+    c = s.contacts[0]
+    ds.set_block(0, hip.H00, 0, c.H00 + 0.05 * np.eye(c.nc))
+    T4, n4, _ = hip.solve_transmission(e2, ds, E, delta=1e-5, flags=hip.SGF_LOAD)
+    assert (n4 > 0).all()
+    assert np.abs(T4 - T0).max() > 1e-6
+
+
+def test_fictitious_contact(gpu):
+    """FictCont (contselfenergy.F90:175-185): g_s = -i pi dos I for one
+    contact; Sigma = Tlc g_s Tcl, then the oracle's transmission on those Sigmas."""
+    hip, eng = gpu
+    from oracle import pyoracle
+    pyoracle.build()
+    pyoracle.set_backend("builtin")
+    s = wl.ribbon(12, 5, disorder=0.3, seed=45)
+    ds = hip.DeviceStructure(s)
+    dos = 0.35
+    ds.set_fictitious(1, dos)
+    E = np.linspace(-2.0, 2.0, 9)
+    T, ncyc, st = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    assert (st == 0).all()
+    assert (ncyc[:, 1] == 0).all() and (ncyc[:, 0] > 0).all()
+    Tref = []
+    for e in E:
+        z = e + 1e-5j
+        c0, c1 = s.contacts
+        nd0 = s.bsz[c0.cblk - 1]
+        sig0, _, _ = pyoracle.contact_sigma(c0, nd0, z)
+        Tlc = -c1.HMC.astype(complex)          # z SMC - HMC with SMC = 0
+        Tcl = (-c1.HMC.astype(complex)).conj().T
+        sig1 = Tlc @ (-1j * np.pi * dos * np.eye(c1.nc)) @ Tcl
+        Tref.append(pyoracle.transmission_sigma(s, e, [sig0, sig1]))
+    Tref = np.array(Tref)
+    _compare(T, Tref)
