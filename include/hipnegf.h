@@ -224,7 +224,37 @@ int hipnegf_solve_transmission_pairs(hipnegf_engine e, hipnegf_struct st, int ni
                                      int npairs, const int *ni, const int *nf, double *T,
                                      int *ncyc, int *status);
 
-/* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):
+/* Local DOS requested alongside the transmission. */
+typedef struct {
+  double dos_delta;  /* extra imaginary part of the device energy (negf%dos_delta)  */
+  int nldos;         /* number of LDOS projections (negf%ndos_proj)                 */
+  int full;          /* != 0: one value per device orbital instead of projections  */
+  const int *ptr;    /* [nldos+1], ptr[0] = 0: projection l is idx[ptr[l]..ptr[l+1]-1] */
+  const int *idx;    /* 1-based device orbitals (dos_proj(l)%indexes)              */
+} hipnegf_dos_opts;
+
+/* Transmission and local DOS per item (host arrays):
+ * calculate_transmissions_and_dos (iterative.F90:1821-1952) as
+ * tunneling_and_dos calls it (integrations.F90:1745-1766): contacts at
+ * E + i delta, the device at E + i dos_delta; T[nitems*npairs] item-major by
+ * the 2-contact formula on Gr(bl1,bl1) (two contacts) or the N-contact column
+ * sweep; ldos[nitems*nout] item-major, nout = full ? ndev : nldos, where a
+ * projection sums -Im(Gr_ii)/pi over its orbitals, skipping orbitals beyond
+ * the device (iterative.F90:1926-1944).  The `full` form is the reference's
+ * dos_proj(1)%indexes(1) == 0 case (iterative.F90:1930-1931).  The caller
+ * applies kwght.  ncyc (may be NULL) and status as for the transmission. */
+int hipnegf_solve_transmission_dos(hipnegf_engine e, hipnegf_struct st, int nitems,
+                                   const double *E, const int *slot, const hipnegf_opts *o,
+                                   int npairs, const int *ni, const int *nf,
+                                   const hipnegf_dos_opts *d, double *T, double *ldos, int *ncyc,
+                                   int *status);
+
+/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058): only the listed 1-based
+ * device orbitals of the traced block enter every transmission trace of this
+ * structure (get_tun_mask, iterative_cpu.F90:585-617).  n = 0 clears it. */
+int hipnegf_struct_set_tun_proj(hipnegf_struct st, int n, const int *idx);
+
+/* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):/* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):
  * calculate_elastic_scba (iterative.F90:289-363) with the ElPhonDephD model
  * (elphdd.F90; coupling per device orbital, squared internally), then
  * iterative_meir_wingreen (iterative.F90:382-414).

@@ -117,9 +117,23 @@ void negf_get_currents(int *handler, int *npairs, double *currents, int copy);
 void negf_associate_transmission(int *handler, int tr_shape[2], double **tr_pointer);
 void negf_associate_energy_current(int *handler, int currents_shape[2], double **currents_pointer);
 
+/* Local DOS alongside the transmission (negf_init_ldos / negf_set_ldos_intervals /
+ * negf_set_ldos_indexes / negf_associate_ldos, libnegf_api.f90:941-1021):
+ * once projections are set, negf_solve_landauer also fills ldos_mat
+ * (Npoints x nldos, column-major, times kwght) from
+ * calculate_transmissions_and_dos at E + i dos_delta (integrations.F90:1758-1766). */
+void negf_init_ldos(int *handler, int nldos);
+void negf_set_ldos_intervals(int *handler, int nldos, int *istart, int *iend);
+void negf_set_ldos_indexes(int *handler, int ildos, int idx_size, int *idx);
+void negf_associate_ldos(int *handler, int ldos_shape[2], double **ldos_pointer);
+
 /* extensions of this implementation */
 void negf_set_precision(int *handler, int precision); /* hipnegf_precision (FP64 default) */
 void negf_set_batch(int *handler, int max_batch);     /* 0: automatic                  */
+/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058, which the reference's C API
+ * does not export): 1-based device orbitals whose rows enter the transmission
+ * traces; n < 0 removes it (all rows). */
+void negf_set_tun_indexes(int *handler, int n, int *idx);
 int negf_get_status(int *handler);                    /* 0 ok, else last failure       */
 const char *negf_get_last_error(int *handler);
 /* per-energy status bits and decimation iteration counts of the last solve */

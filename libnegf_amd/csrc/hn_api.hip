@@ -8,6 +8,7 @@
 //   extract_cont + zcsr2blk_sod         ln_extract.F90:41-76, sparsekit_drv.F90:3409-3436
 //   compute_current                     libnegf.F90:1904-1963
 //   tunneling_int_def / tunneling_and_dos  integrations.F90:1613-1783
+//   init_ldos / set_ldos_* / associate_ldos libnegf.F90:971-1041, 1416-1426
 //   meir_wingreen (fixed occupations)   integrations.F90:1804-1966
 //   electron_current / integrate_el     integrations.F90:2427-2457, 2688-2767
 // Every (E) point of a solve is sent to the engine in one batched call.
@@ -54,6 +55,14 @@ struct Session {
   std::unique_ptr<hn::DevStruct> dev;
   long long gen = 0, dev_gen = -1;
   bool sgf_cache_on = false;
+  // LDOS projections (negf%dos_proj / ndos_proj, libnegf.F90:971-1041), their
+  // result ldos_mat (npoints x ndos, column-major) and tun_proj (:1044-1058)
+  int ndos = 0;
+  std::vector<std::vector<int>> dos_proj;
+  std::vector<double> ldos;
+  int ldos_cols = 0;
+  bool tun_set = false;
+  std::vector<int> tun_proj;
 };
 
 Session *sess(int *handler) {
@@ -270,6 +279,15 @@ hn::DevStruct &device_struct(Session &s) {
     s.dev_gen = s.gen;
   }
   hn::DevStruct &d = *s.dev;
+  if (s.tun_set) {
+    // an empty tun_proj selects no row (get_tun_mask with a zero-size list);
+    // index 0 marks no orbital
+    const int zero = 0;
+    if (s.tun_proj.empty()) d.set_tun_proj(1, &zero);
+    else d.set_tun_proj((int)s.tun_proj.size(), s.tun_proj.data());
+  } else if (d.tun_mask) {
+    d.set_tun_proj(0, nullptr);
+  }
   for (int c = 0; c < d.ncont; ++c) {
     const int f = s.p.fictcont[c] ? 1 : 0;
     if (f != d.fict[c] || (f && d.contact_dos[c] != s.p.contact_dos[c])) {
@@ -324,6 +342,60 @@ struct DevBuf {
   template <typename T> T *as() { return (T *)p; }
 };
 
+// tunneling_and_dos with LDOS (integrations.F90:1724-1766): T and ldos_mat
+// rows from calculate_transmissions_and_dos at E + i dos_delta, both times kwght.
+void compute_landauer_dos(Session &s, const std::vector<int> &ni, const std::vector<int> &nf) {
+  hn::DevStruct *st = &device_struct(s);
+  hn::Engine &eng = engine(s);
+  const int n = s.npoints, ndev = st->ndev();
+  // dos_proj(1)%indexes(1) == 0 with one projection per orbital: every orbital
+  // (iterative.F90:1930-1931)
+  const bool full = s.ndos == ndev && !s.dos_proj[0].empty() && s.dos_proj[0][0] == 0;
+  std::vector<int> ptr(1, 0), idx;
+  if (!full)
+    for (int l = 0; l < s.ndos; ++l) {
+      for (int i2 : s.dos_proj[l]) {
+        if (i2 < 1) throw std::runtime_error("LDOS orbital index < 1");
+        idx.push_back(i2);
+      }
+      ptr.push_back((int)idx.size());
+    }
+  const int nout = full ? ndev : s.ndos;
+  const int nc = std::max(s.ncont, 1);
+  DevBuf dE(sizeof(double) * n), dsl(sizeof(int) * n), dN(sizeof(int) * n * nc), dS(sizeof(int) * n),
+      dTp(sizeof(double) * n * s.npairs), dL(sizeof(double) * (size_t)n * nout),
+      dP(sizeof(int) * (ptr.size() + idx.size() + 1));
+  HN_CHECK(hipMemcpy(dE.p, s.en.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
+  hn::DosOpts d;
+  d.dos_delta = s.p.dos_delta;
+  d.full = full ? 1 : 0;
+  if (!full) {
+    HN_CHECK(hipMemcpy(dP.p, ptr.data(), sizeof(int) * ptr.size(), hipMemcpyHostToDevice));
+    if (!idx.empty())
+      HN_CHECK(hipMemcpy(dP.as<int>() + ptr.size(), idx.data(), sizeof(int) * idx.size(),
+                         hipMemcpyHostToDevice));
+    d.nldos = s.ndos;
+    d.ptr = dP.as<int>();
+    d.idx = dP.as<int>() + ptr.size();
+  }
+  hn::SolveOpts o = opts_of(s);
+  eng.transmission_dos(*st, n, dE.as<double>(), dsl.as<int>(), o, s.npairs, ni.data(), nf.data(), d,
+                       dTp.as<double>(), dL.as<double>(), dN.as<int>(), dS.as<int>());
+  std::vector<double> T((size_t)n * s.npairs), L((size_t)n * nout);
+  HN_CHECK(hipMemcpy(T.data(), dTp.p, sizeof(double) * T.size(), hipMemcpyDeviceToHost));
+  HN_CHECK(hipMemcpy(L.data(), dL.p, sizeof(double) * L.size(), hipMemcpyDeviceToHost));
+  HN_CHECK(hipMemcpy(s.item_status.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  if (s.ncont > 0)
+    HN_CHECK(hipMemcpy(s.item_ncyc.data(), dN.p, sizeof(int) * n * s.ncont, hipMemcpyDeviceToHost));
+  for (int ip = 0; ip < s.npairs; ++ip)
+    for (int i = 0; i < n; ++i) s.tunn[(size_t)ip * n + i] = T[(size_t)i * s.npairs + ip] * s.p.kwght;
+  s.ldos_cols = s.ndos;
+  s.ldos.assign((size_t)n * s.ndos, 0.0);
+  for (int l = 0; l < nout; ++l)
+    for (int i = 0; i < n; ++i) s.ldos[(size_t)l * n + i] = L[(size_t)i * nout + l] * s.p.kwght;
+}
+
 // compute_landauer: extract_cont + tunneling_int_def + tunneling_and_dos
 void compute_landauer(Session &s) {
   const std::vector<int> ni = pairs_of(s.p.ni), nf = pairs_of(s.p.nf);
@@ -340,6 +412,10 @@ void compute_landauer(Session &s) {
   s.tunn.assign((size_t)s.npoints * s.npairs, 0.0);
   s.item_status.assign(s.npoints, 0);
   s.item_ncyc.assign((size_t)s.npoints * std::max(s.ncont, 1), 0);
+  if (s.ndos > 0) {  // do_ledos: calculate_transmissions_and_dos (integrations.F90:1758-1766)
+    compute_landauer_dos(s, ni, nf);
+    return;
+  }
   if (s.ncont < 2) return;  // calculate_transmissions: one contact -> T = 0
   hn::DevStruct *st = &device_struct(s);
   hn::Engine &eng = engine(s);
@@ -469,6 +545,7 @@ void negf_destruct_libnegf(int *handler) {
     s.tunn.clear();
     s.curr_mat.clear();
     s.currents.clear();
+    s.ldos.clear();
     s.deph = false;
     s.dev.reset();
     s.eng.reset();
@@ -656,6 +733,54 @@ void negf_associate_energy_current(int *handler, int currents_shape[2], double *
     currents_shape[0] = s.npoints;
     currents_shape[1] = s.curr_mat.empty() ? 0 : s.ncont;
     *currents_pointer = s.curr_mat.empty() ? nullptr : s.curr_mat.data();
+  });
+}
+
+// init_ldos (libnegf.F90:971-988)
+void negf_init_ldos(int *handler, int nldos) {
+  api(handler, [&](Session &s) {
+    if (nldos < 0) throw std::runtime_error("nldos < 0");
+    s.ndos = nldos;
+    s.dos_proj.assign(nldos, std::vector<int>());
+  });
+}
+
+// set_ldos_intervals (libnegf.F90:1010-1024): interval ii is istart..iend for
+// ii = 1..ndos_proj (the count given to init_ldos)
+void negf_set_ldos_intervals(int *handler, int nldos, int *istart, int *iend) {
+  api(handler, [&](Session &s) {
+    (void)nldos;
+    for (int ii = 0; ii < s.ndos; ++ii) {
+      std::vector<int> v;
+      for (int jj = 0; jj < iend[ii] - istart[ii] + 1; ++jj) v.push_back(istart[ii] + jj);
+      s.dos_proj[ii] = v;
+    }
+  });
+}
+
+// set_ldos_indexes (libnegf.F90:1031-1041), ildos 1-based
+void negf_set_ldos_indexes(int *handler, int ildos, int idx_size, int *idx) {
+  api(handler, [&](Session &s) {
+    if (ildos < 1 || ildos > s.ndos) throw std::runtime_error("ildos out of range (negf_init_ldos)");
+    s.dos_proj[ildos - 1].assign(idx, idx + std::max(idx_size, 0));
+  });
+}
+
+// associate_ldos (libnegf.F90:1416-1426): ldos_mat, shape (npoints, ndos_proj)
+void negf_associate_ldos(int *handler, int ldos_shape[2], double **ldos_pointer) {
+  api(handler, [&](Session &s) {
+    const bool have = !s.ldos.empty();
+    ldos_shape[0] = have ? s.npoints : 0;
+    ldos_shape[1] = have ? s.ldos_cols : 0;
+    *ldos_pointer = have ? s.ldos.data() : nullptr;
+  });
+}
+
+// set_tun_indexes (libnegf.F90:1044-1058); n < 0 removes the projection
+void negf_set_tun_indexes(int *handler, int n, int *idx) {
+  api(handler, [&](Session &s) {
+    s.tun_set = n >= 0;
+    s.tun_proj.assign(idx, idx + std::max(n, 0));
   });
 }
 

@@ -536,6 +536,64 @@ int hipnegf_solve_transmission_pairs(hipnegf_engine e, hipnegf_struct st, int ni
   });
 }
 
+int hipnegf_solve_transmission_dos(hipnegf_engine e, hipnegf_struct st, int nitems,
+                                   const double *E, const int *slot, const hipnegf_opts *o,
+                                   int npairs, const int *ni, const int *nf,
+                                   const hipnegf_dos_opts *d, double *T, double *ldos, int *ncyc,
+                                   int *status) {
+  return guard([&] {
+    if (nitems <= 0) return 0;
+    if (npairs < 1) throw std::runtime_error("npairs < 1");
+    if (!d) throw std::runtime_error("LDOS options missing");
+    hn::DevStruct &S = *st->st;
+    for (int i = 0; i < nitems; ++i)
+      if (slot[i] < 0 || slot[i] >= S.nslots) throw std::runtime_error("item slot out of range");
+    const int nl = d->full ? 0 : d->nldos;
+    if (nl < 0) throw std::runtime_error("nldos < 0");
+    if (nl > 0) {
+      if (!d->ptr || !d->idx) throw std::runtime_error("LDOS projection lists missing");
+      if (d->ptr[0] != 0) throw std::runtime_error("LDOS ptr[0] must be 0");
+      for (int l = 0; l < nl; ++l)
+        if (d->ptr[l + 1] < d->ptr[l]) throw std::runtime_error("LDOS ptr not ascending");
+      for (int k = 0; k < d->ptr[nl]; ++k)
+        if (d->idx[k] < 1) throw std::runtime_error("LDOS orbital index < 1");
+    }
+    const long long nout = d->full ? S.ndev() : nl;
+    const int nc = std::max(S.ncont, 1);
+    const int nidx = nl > 0 ? d->ptr[nl] : 0;
+    void *p[7] = {};
+    struct F { void **p; ~F() { for (int i = 0; i < 7; ++i) if (p[i]) (void)hipFree(p[i]); } } fr{p};
+    HN_CHECK(hipMalloc(&p[0], sizeof(double) * nitems));
+    HN_CHECK(hipMalloc(&p[1], sizeof(double) * nitems * npairs));
+    HN_CHECK(hipMalloc(&p[2], sizeof(int) * nitems));
+    HN_CHECK(hipMalloc(&p[3], sizeof(int) * nitems * nc));
+    HN_CHECK(hipMalloc(&p[4], sizeof(int) * nitems));
+    HN_CHECK(hipMalloc(&p[5], sizeof(double) * std::max<long long>(nitems * nout, 1)));
+    HN_CHECK(hipMalloc(&p[6], sizeof(int) * (nl + 1 + std::max(nidx, 1))));
+    HN_CHECK(hipMemcpy(p[0], E, sizeof(double) * nitems, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(p[2], slot, sizeof(int) * nitems, hipMemcpyHostToDevice));
+    hn::DosOpts dd;
+    dd.dos_delta = d->dos_delta;
+    dd.full = d->full ? 1 : 0;
+    dd.nldos = nl;
+    if (nl > 0) {
+      HN_CHECK(hipMemcpy(p[6], d->ptr, sizeof(int) * (nl + 1), hipMemcpyHostToDevice));
+      HN_CHECK(hipMemcpy((int *)p[6] + nl + 1, d->idx, sizeof(int) * nidx, hipMemcpyHostToDevice));
+      dd.ptr = (const int *)p[6];
+      dd.idx = (const int *)p[6] + nl + 1;
+    }
+    e->eng->transmission_dos(S, nitems, (const double *)p[0], (const int *)p[2], to_opts(o), npairs,
+                             ni, nf, dd, (double *)p[1], nout > 0 ? (double *)p[5] : nullptr,
+                             (int *)p[3], (int *)p[4]);
+    HN_CHECK(hipMemcpy(T, p[1], sizeof(double) * nitems * npairs, hipMemcpyDeviceToHost));
+    if (ldos && nout > 0)
+      HN_CHECK(hipMemcpy(ldos, p[5], sizeof(double) * nitems * nout, hipMemcpyDeviceToHost));
+    if (ncyc) HN_CHECK(hipMemcpy(ncyc, p[3], sizeof(int) * nitems * nc, hipMemcpyDeviceToHost));
+    if (status) HN_CHECK(hipMemcpy(status, p[4], sizeof(int) * nitems, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
 int hipnegf_solve_scba_mw(hipnegf_engine e, hipnegf_struct st, int nitems, const double *E,
                           const int *slot, const hipnegf_opts *o, const double *coupling,
                           int niter, double tol, const double *frm, int frm_per_item, int ref,
@@ -659,6 +717,14 @@ int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious
     S.fict[contact] = fictitious ? 1 : 0;
     S.contact_dos[contact] = dos;
     S.touched();
+    return 0;
+  });
+}
+
+int hipnegf_struct_set_tun_proj(hipnegf_struct st, int n, const int *idx) {
+  return guard([&] {
+    if (n > 0 && !idx) throw std::runtime_error("tun_proj indexes missing");
+    st->st->set_tun_proj(n, idx);
     return 0;
   });
 }

@@ -37,6 +37,21 @@ struct DevStruct {
   int fict[kMaxCont] = {0};
   double contact_dos[kMaxCont] = {0.0};
   void touched() { mirror_state = -1; ++version; }
+  // tun_proj (libnegf.F90:1044-1058) as a device row mask over the device
+  // orbitals: only rows with a nonzero byte enter transmission traces
+  // (get_tun_mask, iterative_cpu.F90:585-617).  Null: every row.
+  unsigned char *tun_mask = nullptr;
+  // host 1-based orbital indexes; n == 0 clears the mask
+  void set_tun_proj(int n, const int *idx);
+  const unsigned char *tun_rows(int blk) const {
+    return tun_mask ? tun_mask + row_off(blk) : nullptr;
+  }
+  long long row_off(int blk) const {
+    long long o = 0;
+    for (int i = 0; i < blk; ++i) o += bsz[i];
+    return o;
+  }
+  int ndev() const { return (int)row_off(nbl); }
 
   bool owned = true;       // pool allocated (and freed) by this object
 
@@ -99,6 +114,16 @@ struct SolveOpts {
   int sgf_mode = 0;
 };
 
+// Local DOS alongside the transmission (calculate_transmissions_and_dos,
+// iterative.F90:1821-1952).
+struct DosOpts {
+  double dos_delta = 0.0;     // extra imaginary part of the device energy (negf%dos_delta)
+  int nldos = 0;              // number of projections
+  int full = 0;               // one output per device orbital instead of projections
+  const int *ptr = nullptr;   // device [nldos+1]: list l is idx[ptr[l] .. ptr[l+1]-1]
+  const int *idx = nullptr;   // device, 1-based device orbitals
+};
+
 struct SCBAOpts {
   const double *coupling = nullptr;  // device, per device orbital (squared inside, elphdd.F90:94)
   int niter = 0;                     // scba_niter (loop runs while iter <= niter)
@@ -131,6 +156,15 @@ class Engine {
   void transmission_pairs(DevStruct &st, int nitems, const double *dE, const int *dslot,
                           const SolveOpts &o, int npairs, const int *ni, const int *nf,
                           double *dT, int *dncyc, int *dstatus);
+
+  // Transmission for npairs pairs and the local DOS per item, with every
+  // Gr(i,i) block of the sweep: contacts at E + i delta, device at
+  // E + i dos_delta.  T[nitems*npairs], ldos[nitems*nout] item-major
+  // (nout = full ? ndev : nldos; ldos may be null when nout == 0).
+  void transmission_dos(DevStruct &st, int nitems, const double *dE, const int *dslot,
+                        const SolveOpts &o, int npairs, const int *ni, const int *nf,
+                        const DosOpts &d, double *dT, double *dldos, int *dncyc, int *dstatus);
+  static size_t dos_item_bytes(const DevStruct &st, int npairs, const int *ni, const int *nf);
 
   // Elastic-dephasing SCBA + Meir-Wingreen current per item.
   // curr[nitems*ncont], iters[nitems], err[nitems], status[nitems];
@@ -219,6 +253,12 @@ class Engine {
   void t_multi(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
                int npairs, const int *ni, const int *nf, const double2 *Sig, long long sSig,
                double *dT, int *dstatus);
+  void dos_batch(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
+                 int npairs, const int *ni, const int *nf, const DosOpts &d, double *dT,
+                 double *dldos, int *dncyc, int *dstatus);
+  // per-item ncyc (item-major) and status words from the contact stage's arrays
+  void finish_items(int B, int ncont, const int *ncyc_c, const int *status_c, int *dncyc,
+                    int *dstatus);
   void scba_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                   const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
                   double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag);

@@ -117,6 +117,21 @@ DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, co
 
 DevStruct::~DevStruct() {
   if (pool && owned) (void)hipFree(pool);
+  if (tun_mask) (void)hipFree(tun_mask);
+}
+
+void DevStruct::set_tun_proj(int n, const int *idx) {
+  const int nd = ndev();
+  if (n <= 0) {
+    if (tun_mask) HN_CHECK(hipFree(tun_mask));
+    tun_mask = nullptr;
+    return;
+  }
+  std::vector<unsigned char> h(nd, 0);
+  for (int k = 0; k < n; ++k)
+    if (idx[k] >= 1 && idx[k] <= nd) h[idx[k] - 1] = 1;  // other indexes select no row
+  if (!tun_mask) HN_CHECK(hipMalloc(&tun_mask, nd));
+  HN_CHECK(hipMemcpy(tun_mask, h.data(), nd, hipMemcpyHostToDevice));
 }
 
 long long DevStruct::block_elems(int kind, int idx) const {
@@ -747,7 +762,7 @@ void Engine::t_sweep(DevStruct &st, int B, const double *dE, const int *dslot,
     gamma_batched(sig_of(ct1 - 1), s2, Gam, s2, n0, B, stream_);
     gemm_z(G(n0, n0, n0, Gam, s2, n0, OP_N, Gs, s2, n0, OP_N, W1, s2, n0, Z(1), Z(0), B, nullptr), stream_);
     gemm_z(G(n0, n0, n0, W1, s2, n0, OP_N, Gam, s2, n0, OP_N, W2, s2, n0, Z(1), Z(0), B, nullptr), stream_);
-    trace_T2(Gam, Gs, W2, s2, n0, B, dT, stream_);
+    trace_T2(Gam, Gs, W2, s2, n0, B, dT, stream_, 1, st.tun_rows(0));
   }
   arena_.used = mark;
 }
@@ -902,7 +917,7 @@ void Engine::t_sweep_mixed(DevStruct &st, int B, const double *dE, const int *ds
   gamma_batched(Sig + (long long)(ct1 - 1) * B * s2, s2, Gam, s2, n0, B, stream_);
   gemm_z(G(n0, n0, n0, Gam, s2, n0, OP_N, G64, s2, n0, OP_N, W1, s2, n0, one, zero, B, nullptr), stream_);
   gemm_z(G(n0, n0, n0, W1, s2, n0, OP_N, Gam, s2, n0, OP_N, W2, s2, n0, one, zero, B, nullptr), stream_);
-  trace_T2(Gam, G64, W2, s2, n0, B, dT, stream_);
+  trace_T2(Gam, G64, W2, s2, n0, B, dT, stream_, 1, st.tun_rows(0));
   arena_.used = mark;
 }
 
@@ -939,6 +954,7 @@ void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
     const std::vector<long long> key = {(long long)(uintptr_t)st.pool, st.slot_elems, st.nbl, st.smode,
                                         st.nmax(), B, pni[0], pnf[0], (long long)(uintptr_t)arena_.base,
                                         (long long)arena_.used, o.share_mirror,
+                                        (long long)(uintptr_t)st.tun_mask,
                                         (long long)std::hash<std::string>()(std::string(
                                             (const char *)st.bsz.data(), st.bsz.size() * sizeof(int)))};
     hipGraphExec_t exec = nullptr;
@@ -972,25 +988,29 @@ void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
   } else {
     t_sweep(st, B, dE, dslot, o, pni[0], pnf[0], Sig, s2, dT, dstatus);
   }
-  // ---- per-item outputs: ncyc (item-major) and status ----
-  {
-    std::vector<int> h(B * ncont), hs(B * ncont), hst(B);
+  finish_items(B, ncont, ncyc_c, status_c, dncyc, dstatus);
+}
+
+void Engine::finish_items(int B, int ncont, const int *ncyc_c, const int *status_c, int *dncyc,
+                          int *dstatus) {
+  std::vector<int> h((size_t)B * ncont), hs((size_t)B * ncont), hst(B);
+  if (ncont > 0) {
     HN_CHECK(hipMemcpyAsync(h.data(), ncyc_c, sizeof(int) * B * ncont, hipMemcpyDeviceToHost, stream_));
     HN_CHECK(hipMemcpyAsync(hs.data(), status_c, sizeof(int) * B * ncont, hipMemcpyDeviceToHost, stream_));
-    HN_CHECK(hipMemcpyAsync(hst.data(), dstatus, sizeof(int) * B, hipMemcpyDeviceToHost, stream_));
-    HN_CHECK(hipStreamSynchronize(stream_));
-    std::vector<int> o1(B * ncont);
-    for (int b = 0; b < B; ++b) {
-      for (int c = 0; c < ncont; ++c) {
-        o1[b * ncont + c] = h[c * B + b];
-        hst[b] |= hs[c * B + b];
-      }
-    }
-    if (dncyc)
-      HN_CHECK(hipMemcpyAsync(dncyc, o1.data(), sizeof(int) * B * ncont, hipMemcpyHostToDevice, stream_));
-    HN_CHECK(hipMemcpyAsync(dstatus, hst.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
-    HN_CHECK(hipStreamSynchronize(stream_));
   }
+  HN_CHECK(hipMemcpyAsync(hst.data(), dstatus, sizeof(int) * B, hipMemcpyDeviceToHost, stream_));
+  HN_CHECK(hipStreamSynchronize(stream_));
+  std::vector<int> o1((size_t)B * ncont);
+  for (int b = 0; b < B; ++b) {
+    for (int c = 0; c < ncont; ++c) {
+      o1[(size_t)b * ncont + c] = h[(size_t)c * B + b];
+      hst[b] |= hs[(size_t)c * B + b];
+    }
+  }
+  if (dncyc && ncont > 0)
+    HN_CHECK(hipMemcpyAsync(dncyc, o1.data(), sizeof(int) * B * ncont, hipMemcpyHostToDevice, stream_));
+  HN_CHECK(hipMemcpyAsync(dstatus, hst.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
+  HN_CHECK(hipStreamSynchronize(stream_));
 }
 
 // ---------------------------------------------------------------------------
@@ -1120,9 +1140,251 @@ void Engine::t_multi(DevStruct &st, int B, const double *dE, const int *dslot,
     gamma_batched(Sig + (ct2 - 1) * SB, s2, Gam2, s2, n2, B, stream_);
     gemm_z(G(n2, n1, n2, Gam2, s2, n2, OP_N, cur, s2, n2, OP_N, W1, s2, n2, one, zero, B, nullptr), stream_);
     gemm_z(G(n2, n1, n1, W1, s2, n2, OP_N, Gam1, s2, n1, OP_N, W2, s2, n2, one, zero, B, nullptr), stream_);
-    trace_prod_conj(W2, cur, s2, s2, n2, n1, B, dT + p, stream_, npairs);
+    trace_prod_conj(W2, cur, s2, s2, n2, n1, B, dT + p, stream_, npairs, st.tun_rows(bl2 - 1));
     if (chk) flag_set(dT + p, npairs, flag, o.eps_drop * o.eps_drop, B, stream_);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Transmission and local DOS: calculate_transmissions_and_dos
+// (iterative.F90:1821-1952) for one batch, as tunneling_and_dos calls it
+// (integrations.F90:1758-1766).  The device blocks are formed at
+// E + i dos_delta (build_ESH at the complex energy), the contacts' self-energies
+// come from E + i delta.  gsmr blocks g_nbl..g_2 are kept; the Gr diagonal
+// blocks are swept from Gr(1,1) to Gr(nbl,nbl) through two rolling buffers and
+// each one's diagonal goes straight into the per-item DOS vector, so only the
+// Gr blocks a trace needs are retained:
+//   2 contacts: Gr(bl1,bl1), T by the 2-contact formula
+//               (calculate_single_transmission_2_contacts, iterative_cpu.F90:399-470);
+//   N contacts: Gr(bl1+1,bl1) (and Gr(bl1,bl1) for the EPS test or bl1 == bl2),
+//               then the column sweep of iterative_cpu.F90:477-583 per pair.
+// The Gr(i-1,i) blocks are not formed (no trace reads them).
+// ---------------------------------------------------------------------------
+namespace {
+struct DosNeeds {
+  std::vector<char> d, l;  // Gr(i,i), Gr(i+1,i) retained (0-based block i)
+  int count() const {
+    int c = 0;
+    for (char x : d) c += x;
+    for (char x : l) c += x;
+    return c;
+  }
+};
+DosNeeds dos_needs(const DevStruct &st, int npairs, const int *ni, const int *nf, double eps) {
+  DosNeeds n;
+  n.d.assign(st.nbl, 0);
+  n.l.assign(st.nbl, 0);
+  if (st.ncont < 2) return n;
+  for (int p = 0; p < npairs; ++p) {
+    const int ci = st.cblk[ni[p] - 1], cf = st.cblk[nf[p] - 1];
+    const int bl1 = std::min(ci, cf), bl2 = std::max(ci, cf);
+    if (st.ncont == 2) {
+      n.d[bl1 - 1] = 1;
+    } else if (bl1 == bl2) {
+      n.d[bl1 - 1] = 1;
+    } else {
+      n.l[bl1 - 1] = 1;
+      if (eps > 0.0 && st.nbl > 1 && bl2 - bl1 > 1) n.d[bl1 - 1] = 1;
+    }
+  }
+  return n;
+}
+}  // namespace
+
+size_t Engine::dos_item_bytes(const DevStruct &st, int npairs, const int *ni, const int *nf) {
+  const int n = st.nmax();
+  const size_t b2 = (size_t)n * n * sizeof(double2);
+  const size_t nc = (size_t)std::max(st.ncont, 1);
+  const DosNeeds nd = dos_needs(st, npairs, ni, nf, 1.0);
+  // Sigma per contact; then max(contact stage, sweep): g_i for every block,
+  // 2 rolling Gr blocks, the retained ones and 9 temporaries
+  const size_t sweep = (size_t)st.nbl + 2 + (size_t)nd.count() + 9;
+  return (nc + std::max<size_t>(10 + 2 * nc, sweep)) * b2 + inv_work_bytes(n, (int)nc, 8) +
+         (size_t)st.ndev() * sizeof(double) + 64 * sizeof(int) + 4096;
+}
+
+void Engine::dos_batch(DevStruct &st, int B, const double *dE, const int *dslot,
+                       const SolveOpts &o, int npairs, const int *pni, const int *pnf,
+                       const DosOpts &dos, double *dT, double *dldos, int *dncyc, int *dstatus) {
+  const int nbl = st.nbl, ncont = st.ncont, ndev = st.ndev();
+  const int nm = st.nmax();
+  const long long s2 = (long long)nm * nm, SB = s2 * B;
+  const int ncs = std::max(ncont, 1);
+  arena_.reset();
+  double2 *Sig = arena_.take<double2>(SB * ncs);
+  int *ncyc_c = arena_.take<int>((size_t)B * ncs);
+  int *status_c = arena_.take<int>((size_t)B * ncs);
+  fill_int(ncyc_c, B * ncs, 0, stream_);
+  fill_int(status_c, B * ncs, 0, stream_);
+  fill_int(dstatus, B, 0, stream_);
+  if (ncont > 0) contacts(st, B, dE, dslot, o, Sig, s2, ncyc_c, status_c);
+
+  std::vector<std::vector<int>> on_blk(nbl);
+  for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
+  const DosNeeds need = dos_needs(st, npairs, pni, pnf, o.eps_drop);
+  double2 *g = arena_.take<double2>(SB * nbl);
+  double2 *R0 = arena_.take<double2>(SB), *R1 = arena_.take<double2>(SB);
+  std::vector<double2 *> keep_d(nbl, nullptr), keep_l(nbl, nullptr);
+  for (int i = 0; i < nbl; ++i) {
+    if (need.d[i]) keep_d[i] = arena_.take<double2>(SB);
+    if (need.l[i]) keep_l[i] = arena_.take<double2>(SB);
+  }
+  double2 *D = arena_.take<double2>(SB), *U = arena_.take<double2>(SB), *L = arena_.take<double2>(SB);
+  double2 *W1 = arena_.take<double2>(SB), *W2 = arena_.take<double2>(SB);
+  double2 *Ca = arena_.take<double2>(SB), *Cb = arena_.take<double2>(SB);
+  double2 *Gam1 = arena_.take<double2>(SB), *Gam2 = arena_.take<double2>(SB);
+  double *diag = arena_.take<double>((size_t)ndev * B);
+  int *flag = arena_.take<int>(B);
+  const bool sd = (st.smode & S_DIAG) != 0, sof = (st.smode & S_OFFDIAG) != 0;
+  auto blk = [&](double2 *base, int i) { return base + (long long)i * SB; };
+  auto bs = [&](int i) { return st.bsz[i]; };
+  const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+
+  auto form_diag = [&](int i, double2 *out) {
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = dos.dos_delta; f.batch = B;
+    f.rows = bs(i); f.cols = bs(i);
+    f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+    f.out = out; f.s_out = s2;
+    const auto &cs = on_blk[i];
+    if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+    if (cs.size() >= 1) { f.sig = Sig + cs[0] * SB; f.s_sig = s2; }
+    if (cs.size() == 2) { f.sig2 = Sig + cs[1] * SB; f.s_sig2 = s2; }
+    form_block(f, stream_);
+  };
+  auto form_off = [&](int i, double2 *u, double2 *l) {  // u = ESH(i,i+1), l = ESH(i+1,i)
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = dos.dos_delta; f.batch = B;
+    f.s_identity = 0; f.s_out = s2;
+    if (u) {
+      f.rows = bs(i); f.cols = bs(i + 1);
+      f.H = st.ref(BK_HU, i); f.S = sof ? st.ref(BK_SU, i) : BlkRef(); f.out = u;
+      form_block(f, stream_);
+    }
+    if (l) {
+      f.rows = bs(i + 1); f.cols = bs(i);
+      f.H = st.ref(BK_HL, i); f.S = sof ? st.ref(BK_SL, i) : BlkRef(); f.out = l;
+      form_block(f, stream_);
+    }
+  };
+
+  // ---- calculate_gsmr_blocks(nbl -> 2), every g_i kept ----
+  if (nbl > 1) {
+    form_diag(nbl - 1, D);
+    inverse(bs(nbl - 1), D, s2, blk(g, nbl - 1), s2, B, nullptr, dstatus, o.precision, o);
+    for (int i = nbl - 2; i >= 1; --i) {
+      const int a = bs(i), c = bs(i + 1);
+      form_off(i, U, L);
+      form_diag(i, D);
+      gemm_z(G(a, c, c, U, s2, a, OP_N, blk(g, i + 1), s2, c, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+      gemm_z(G(a, a, c, W1, s2, a, OP_N, L, s2, c, OP_N, D, s2, a, one, one, B, nullptr), stream_);
+      inverse(a, D, s2, blk(g, i), s2, B, nullptr, dstatus, o.precision, o);
+    }
+  }
+  // ---- Gr(1,1) ----
+  double2 *cur = R0, *nxt = R1;
+  {
+    const int a = bs(0);
+    form_diag(0, D);
+    if (nbl > 1) {
+      const int c = bs(1);
+      form_off(0, U, L);
+      gemm_z(G(a, c, c, U, s2, a, OP_N, blk(g, 1), s2, c, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+      gemm_z(G(a, a, c, W1, s2, a, OP_N, L, s2, c, OP_N, D, s2, a, one, one, B, nullptr), stream_);
+    }
+    inverse(a, D, s2, cur, s2, B, nullptr, dstatus, o.precision, o);
+    dos_diag(cur, s2, a, B, diag, ndev, stream_);
+    if (keep_d[0]) copy_batched(keep_d[0], s2, cur, s2, (long long)a * a, B, nullptr, stream_);
+  }
+  // ---- Gr(i,i), i = 2..nbl (iterative_cpu.F90:200-219) ----
+  for (int i = 1; i < nbl; ++i) {
+    const int ni_ = bs(i), nmm = bs(i - 1);
+    double2 *Gl = keep_l[i - 1] ? keep_l[i - 1] : Ca;  // Gr(i,i-1)
+    form_off(i - 1, U, L);  // U = ESH(i-1,i) nmm x ni, L = ESH(i,i-1) ni x nmm
+    gemm_z(G(ni_, nmm, ni_, blk(g, i), s2, ni_, OP_N, L, s2, ni_, OP_N, W1, s2, ni_, one, zero, B, nullptr), stream_);
+    gemm_z(G(ni_, nmm, nmm, W1, s2, ni_, OP_N, cur, s2, nmm, OP_N, Gl, s2, ni_, mone, zero, B, nullptr), stream_);
+    gemm_z(G(nmm, ni_, ni_, U, s2, nmm, OP_N, blk(g, i), s2, ni_, OP_N, W2, s2, nmm, one, zero, B, nullptr), stream_);
+    copy_batched(nxt, s2, blk(g, i), s2, (long long)ni_ * ni_, B, nullptr, stream_);
+    gemm_z(G(ni_, ni_, nmm, Gl, s2, ni_, OP_N, W2, s2, nmm, OP_N, nxt, s2, ni_, mone, one, B, nullptr), stream_);
+    std::swap(cur, nxt);
+    dos_diag(cur, s2, ni_, B, diag + st.row_off(i), ndev, stream_);
+    if (keep_d[i]) copy_batched(keep_d[i], s2, cur, s2, (long long)ni_ * ni_, B, nullptr, stream_);
+  }
+  // ---- transmissions ----
+  if (ncont < 2) {
+    fill_double(dT, B * npairs, 0.0, stream_);  // case(1): tun = 0 (iterative.F90:1860-1861)
+  } else if (ncont == 2) {
+    for (int p = 0; p < npairs; ++p) {
+      const int ct1 = (st.cblk[pni[p] - 1] < st.cblk[pnf[p] - 1]) ? pni[p] : pnf[p];
+      const int b1 = st.cblk[ct1 - 1] - 1, a = bs(b1);
+      const double2 *Gb = keep_d[b1];
+      gamma_batched(Sig + (ct1 - 1) * SB, s2, Gam1, s2, a, B, stream_);
+      gemm_z(G(a, a, a, Gam1, s2, a, OP_N, Gb, s2, a, OP_N, W1, s2, a, one, zero, B, nullptr), stream_);
+      gemm_z(G(a, a, a, W1, s2, a, OP_N, Gam1, s2, a, OP_N, W2, s2, a, one, zero, B, nullptr), stream_);
+      trace_T2(Gam1, Gb, W2, s2, a, B, dT + p, stream_, npairs, st.tun_rows(b1));
+    }
+  } else {
+    for (int p = 0; p < npairs; ++p) {
+      int ct1, ct2;
+      if (st.cblk[pni[p] - 1] < st.cblk[pnf[p] - 1]) { ct1 = pni[p]; ct2 = pnf[p]; } else { ct1 = pnf[p]; ct2 = pni[p]; }
+      const int bl1 = st.cblk[ct1 - 1], bl2 = st.cblk[ct2 - 1];  // 1-based, bl1 <= bl2
+      const int n1 = bs(bl1 - 1), n2 = bs(bl2 - 1);
+      const double2 *c0 = (bl2 == bl1) ? keep_d[bl1 - 1] : keep_l[bl1 - 1];
+      const bool chk = o.eps_drop > 0.0 && nbl > 1 && bl2 - bl1 > 1;
+      if (chk) fill_int(flag, B, 0, stream_);
+      if (nbl > 1 && bl2 - bl1 > 1) {
+        if (chk) small_flag(keep_d[bl1 - 1], s2, (long long)n1 * n1, o.eps_drop, B, flag, stream_);
+        double2 *dst = Ca;
+        for (int i = bl1 + 2; i <= bl2; ++i) {  // 1-based block i; Gr(i-1,bl1) = c0
+          const int a = bs(i - 1), c = bs(i - 2);
+          if (chk) small_flag(c0, s2, (long long)c * n1, o.eps_drop, B, flag, stream_);
+          form_off(i - 2, nullptr, L);  // ESH(i,i-1): a x c
+          gemm_z(G(a, c, a, blk(g, i - 1), s2, a, OP_N, L, s2, a, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+          gemm_z(G(a, n1, c, W1, s2, a, OP_N, c0, s2, c, OP_N, dst, s2, a, one, zero, B, nullptr), stream_);
+          c0 = dst;
+          dst = (dst == Ca) ? Cb : Ca;
+        }
+      }
+      gamma_batched(Sig + (ct1 - 1) * SB, s2, Gam1, s2, n1, B, stream_);
+      gamma_batched(Sig + (ct2 - 1) * SB, s2, Gam2, s2, n2, B, stream_);
+      gemm_z(G(n2, n1, n2, Gam2, s2, n2, OP_N, c0, s2, n2, OP_N, W1, s2, n2, one, zero, B, nullptr), stream_);
+      gemm_z(G(n2, n1, n1, W1, s2, n2, OP_N, Gam1, s2, n1, OP_N, W2, s2, n2, one, zero, B, nullptr), stream_);
+      trace_prod_conj(W2, c0, s2, s2, n2, n1, B, dT + p, stream_, npairs, st.tun_rows(bl2 - 1));
+      if (chk) flag_set(dT + p, npairs, flag, o.eps_drop * o.eps_drop, B, stream_);
+    }
+  }
+  // ---- LDOS projections ----
+  const int nout = dos.full ? ndev : dos.nldos;
+  if (nout > 0 && dldos)
+    ldos_reduce(diag, ndev, ndev, dos.ptr, dos.idx, dos.nldos, dos.full, dldos, nout, B, stream_);
+  finish_items(B, ncont, ncyc_c, status_c, dncyc, dstatus);
+}
+
+void Engine::transmission_dos(DevStruct &st, int nitems, const double *dE, const int *dslot,
+                              const SolveOpts &o, int npairs, const int *ni, const int *nf,
+                              const DosOpts &d, double *dT, double *dldos, int *dncyc,
+                              int *dstatus) {
+  if (nitems <= 0) return;
+  if (npairs < 1) throw std::runtime_error("transmission_dos: npairs < 1");
+  if (st.ncont >= 2)
+    for (int p = 0; p < npairs; ++p)
+      if (ni[p] < 1 || ni[p] > st.ncont || nf[p] < 1 || nf[p] > st.ncont || ni[p] == nf[p])
+        throw std::runtime_error("transmission_dos: bad contact pair");
+  if (d.nldos < 0 || (!d.full && d.nldos > 0 && (!d.ptr || !d.idx)))
+    throw std::runtime_error("transmission_dos: bad LDOS projections");
+  HN_CHECK(hipSetDevice(device_));
+  const size_t per_item = dos_item_bytes(st, npairs, ni, nf);
+  const int B = pick_batch(st, nitems, o, per_item);
+  last_batch_ = B;
+  arena_.reserve(per_item * (size_t)B + inv_work_bytes(st.nmax(), B, 8) + (1 << 20));
+  const long long nout = d.full ? st.ndev() : d.nldos;
+  for (int i0 = 0; i0 < nitems; i0 += B) {
+    const int nb = std::min(B, nitems - i0);
+    dos_batch(st, nb, dE + i0, dslot + i0, o, npairs, ni, nf, d, dT + (long long)i0 * npairs,
+              dldos ? dldos + i0 * nout : nullptr, dncyc ? dncyc + (long long)i0 * st.ncont : nullptr,
+              dstatus + i0);
+  }
+  HN_CHECK(hipStreamSynchronize(stream_));
 }
 
 size_t Engine::transmission_item_bytes(const DevStruct &st, int npairs, const int *ni,

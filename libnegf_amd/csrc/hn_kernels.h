@@ -75,12 +75,24 @@ void gamma_batched(const double2 *Sig, long long sS, double2 *Gam, long long sG,
 
 // T_b = | Re( Tr[Gam A] - Tr[W2 G^H] ) |,  A = i (G - G^H)
 // (calculate_single_transmission_2_contacts, iterative_cpu.F90:399-470)
+// written to T[b * tstride]; rmask (device, n bytes, may be null): only rows r with
+// rmask[r] != 0 enter the trace (get_tun_mask, iterative_cpu.F90:585-617)
 void trace_T2(const double2 *Gam, const double2 *G, const double2 *W2, long long s, int n,
-              int batch, double *T, hipStream_t st);
+              int batch, double *T, hipStream_t st, int tstride = 1,
+              const unsigned char *rmask = nullptr);
 // T_b = | Re Tr[W] | for W = Gam2 G21 Gam1 G21^H computed as Re sum (X)_ij conj(G21)_ij,
-// written to T[b * tstride]
+// written to T[b * tstride]; rmask as for trace_T2 (rows of X)
 void trace_prod_conj(const double2 *X, const double2 *G, long long sX, long long sG, int rows,
-                     int cols, int batch, double *T, hipStream_t st, int tstride = 1);
+                     int cols, int batch, double *T, hipStream_t st, int tstride = 1,
+                     const unsigned char *rmask = nullptr);
+// local DOS of a diagonal block: d_b[i] = -Im(M_b(i,i)) / pi  (iterative.F90:1907-1918)
+void dos_diag(const double2 *M, long long sM, int n, int batch, double *d, long long sd,
+              hipStream_t st);
+// LDOS projections (iterative.F90:1926-1944): out_b[l] = sum of d_b[idx[k] - 1] over
+// k in [ptr[l], ptr[l+1]) with idx[k] <= ndev (1-based, device arrays); full != 0:
+// out_b[i] = d_b[i] for every device orbital
+void ldos_reduce(const double *d, long long sd, int ndev, const int *ptr, const int *idx,
+                 int nldos, int full, double *out, long long so, int batch, hipStream_t st);
 // flag_b = 1 where max_ij |X_b(ij)| < eps (flags are only set, never cleared)
 void small_flag(const double2 *X, long long sX, long long nn, double eps, int batch, int *flag,
                 hipStream_t st);

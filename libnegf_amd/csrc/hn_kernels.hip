@@ -208,7 +208,8 @@ __global__ __launch_bounds__(NT) void k_gamma(const double2 *Sig, long long sS, 
 
 __global__ __launch_bounds__(NT) void k_trace_T2(const double2 *Gam, const double2 *G,
                                                  const double2 *W2, long long s, int n,
-                                                 double *T) {
+                                                 double *T, int tstride,
+                                                 const unsigned char *rmask) {
   __shared__ double sh[NT / 64];
   const int b = blockIdx.x;
   const double2 *Gm = Gam + (long long)b * s, *Gb = G + (long long)b * s, *Wb = W2 + (long long)b * s;
@@ -216,6 +217,7 @@ __global__ __launch_bounds__(NT) void k_trace_T2(const double2 *Gam, const doubl
   const long long nn = (long long)n * n;
   for (long long e = threadIdx.x; e < nn; e += NT) {
     const int r = (int)(e % n), c = (int)(e / n);
+    if (rmask && !rmask[r]) continue;  // entry (r,c) feeds diagonal entry r of the product
     const double2 g_rc = Gb[e], g_cr = Gb[c + (long long)r * n];
     // A_cr = i (G_cr - conj(G_rc))
     const double2 d = make_double2(g_cr.x - g_rc.x, g_cr.y + g_rc.y);
@@ -226,17 +228,19 @@ __global__ __launch_bounds__(NT) void k_trace_T2(const double2 *Gam, const doubl
     acc -= w.x * g_rc.x + w.y * g_rc.y;            // Re(W_rc conj(G_rc))
   }
   acc = block_sum(acc, sh);
-  if (threadIdx.x == 0) T[b] = fabs(acc);
+  if (threadIdx.x == 0) T[(long long)b * tstride] = fabs(acc);
 }
 
 __global__ __launch_bounds__(NT) void k_trace_prod_conj(const double2 *X, const double2 *G,
-                                                        long long sX, long long sG, long long nn,
-                                                        double *T, int tstride) {
+                                                        long long sX, long long sG, int rows,
+                                                        long long nn, double *T, int tstride,
+                                                        const unsigned char *rmask) {
   __shared__ double sh[NT / 64];
   const int b = blockIdx.x;
   const double2 *Xb = X + (long long)b * sX, *Gb = G + (long long)b * sG;
   double acc = 0.0;
   for (long long e = threadIdx.x; e < nn; e += NT) {
+    if (rmask && !rmask[e % rows]) continue;
     const double2 x = Xb[e], g = Gb[e];
     acc += x.x * g.x + x.y * g.y;
   }
@@ -575,15 +579,65 @@ void gamma_batched(const double2 *Sig, long long sS, double2 *Gam, long long sG,
 }
 
 void trace_T2(const double2 *Gam, const double2 *G, const double2 *W2, long long s, int n,
-              int batch, double *T, hipStream_t st) {
-  hipLaunchKernelGGL(k_trace_T2, dim3(batch), dim3(NT), 0, st, Gam, G, W2, s, n, T);
+              int batch, double *T, hipStream_t st, int tstride, const unsigned char *rmask) {
+  hipLaunchKernelGGL(k_trace_T2, dim3(batch), dim3(NT), 0, st, Gam, G, W2, s, n, T, tstride,
+                     rmask);
   HN_CHECK(hipGetLastError());
 }
 
 void trace_prod_conj(const double2 *X, const double2 *G, long long sX, long long sG, int rows,
-                     int cols, int batch, double *T, hipStream_t st, int tstride) {
-  hipLaunchKernelGGL(k_trace_prod_conj, dim3(batch), dim3(NT), 0, st, X, G, sX, sG,
-                     (long long)rows * cols, T, tstride);
+                     int cols, int batch, double *T, hipStream_t st, int tstride,
+                     const unsigned char *rmask) {
+  hipLaunchKernelGGL(k_trace_prod_conj, dim3(batch), dim3(NT), 0, st, X, G, sX, sG, rows,
+                     (long long)rows * cols, T, tstride, rmask);
+  HN_CHECK(hipGetLastError());
+}
+
+// d_b[i] = Im((-1/pi) M_b(i,i)): the value zrconcatm_csr ('I', s = -1/pi) stores
+__global__ __launch_bounds__(NT) void k_dos_diag(const double2 *M, long long sM, int n, double *d,
+                                                 long long sd) {
+  const int b = blockIdx.y;
+  const int i = blockIdx.x * NT + threadIdx.x;
+  if (i >= n) return;
+  const double2 v = M[(long long)b * sM + i + (long long)i * n];
+  const double c = -1.0 / 3.14159265358979323846;
+  d[(long long)b * sd + i] = c * v.y + 0.0 * v.x;
+}
+
+void dos_diag(const double2 *M, long long sM, int n, int batch, double *d, long long sd,
+              hipStream_t st) {
+  hipLaunchKernelGGL(k_dos_diag, dim3((n + NT - 1) / NT, batch), dim3(NT), 0, st, M, sM, n, d, sd);
+  HN_CHECK(hipGetLastError());
+}
+
+// one thread per (item, projection): the list summed in its own order from 0
+__global__ __launch_bounds__(NT) void k_ldos_reduce(const double *d, long long sd, int ndev,
+                                                    const int *ptr, const int *idx, int nldos,
+                                                    int full, double *out, long long so,
+                                                    int batch) {
+  const long long t = (long long)blockIdx.x * NT + threadIdx.x;
+  const int nout = full ? ndev : nldos;
+  if (t >= (long long)nout * batch) return;
+  const int b = (int)(t / nout), l = (int)(t % nout);
+  const double *db = d + (long long)b * sd;
+  double acc = 0.0;
+  if (full) {
+    acc += db[l];
+  } else {
+    for (int k = ptr[l]; k < ptr[l + 1]; ++k) {
+      const int i2 = idx[k];
+      if (i2 >= 1 && i2 <= ndev) acc += db[i2 - 1];  // callers reject i2 < 1
+    }
+  }
+  out[(long long)b * so + l] = acc;
+}
+
+void ldos_reduce(const double *d, long long sd, int ndev, const int *ptr, const int *idx,
+                 int nldos, int full, double *out, long long so, int batch, hipStream_t st) {
+  const long long tot = (long long)(full ? ndev : nldos) * batch;
+  if (tot <= 0) return;
+  hipLaunchKernelGGL(k_ldos_reduce, dim3((unsigned)((tot + NT - 1) / NT)), dim3(NT), 0, st, d, sd,
+                     ndev, ptr, idx, nldos, full, out, so, batch);
   HN_CHECK(hipGetLastError());
 }
 void small_flag(const double2 *X, long long sX, long long nn, double eps, int batch, int *flag,

@@ -64,6 +64,9 @@ def load():
     L.hipnegf_solve_transmission_device.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, vp, vp]
     L.hipnegf_solve_transmission_pairs.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), i, vp, vp,
                                                    vp, vp, vp]
+    L.hipnegf_solve_transmission_dos.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), i, vp, vp,
+                                                 C.POINTER(DosOpts), vp, vp, vp, vp]
+    L.hipnegf_struct_set_tun_proj.argtypes = [vp, i, vp]
     L.hipnegf_solve_scba_mw.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, i, d, vp, i, i,
                                         vp, vp, vp, vp, vp, vp]
     L.hipnegf_zgemm_batched.argtypes = [vp, i, i, i, vp, vp, ll, i, i, vp, ll, i, i, vp, vp, ll, i, i]
@@ -217,6 +220,12 @@ class DeviceStructure:
                                            nslots, C.byref(self._h)))
         return self
 
+    def set_tun_proj(self, idx=None):
+        """tun_proj: 1-based device orbitals whose rows enter the transmission
+        traces (None or empty: all)."""
+        a = np.ascontiguousarray([] if idx is None else idx, dtype=np.int32)
+        check(load().hipnegf_struct_set_tun_proj(self._h, len(a), _ptr(a) if len(a) else None))
+
     def set_fictitious(self, contact: int, dos: float, fictitious: bool = True):
         """contact 0-based; g_s = -i pi dos I for that contact."""
         check(load().hipnegf_struct_set_fictitious(self._h, contact, 1 if fictitious else 0, dos))
@@ -302,6 +311,44 @@ def solve_transmission_pairs(eng: Engine, ds: DeviceStructure, E: np.ndarray, pa
                                                   C.byref(o), len(pairs), _ptr(ni), _ptr(nf),
                                                   _ptr(T), _ptr(ncyc), _ptr(st)))
     return T, ncyc, st
+
+
+class DosOpts(C.Structure):
+    _fields_ = [("dos_delta", C.c_double), ("nldos", C.c_int), ("full", C.c_int),
+                ("ptr", C.c_void_p), ("idx", C.c_void_p)]
+
+
+def solve_transmission_dos(eng: Engine, ds: DeviceStructure, E: np.ndarray, pairs=((1, 2),),
+                           proj=None, full: bool = False, dos_delta: float = 0.0, slot=None,
+                           delta: float = 1e-4, precision: int = FP64, max_batch: int = 0,
+                           flags: int = 0):
+    """Transmission and local DOS per energy (calculate_transmissions_and_dos).
+    proj: list of 1-based orbital lists (LDOS projections); full: one column
+    per device orbital.  Returns (T[n, npairs], ldos[n, nout], ncyc[n, ncont],
+    status[n])."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    n = len(E)
+    sl = np.zeros(n, dtype=np.int32) if slot is None else np.ascontiguousarray(slot, dtype=np.int32)
+    ni = np.ascontiguousarray([p[0] for p in pairs], dtype=np.int32)
+    nf = np.ascontiguousarray([p[1] for p in pairs], dtype=np.int32)
+    lists = [] if (proj is None or full) else list(proj)
+    ptr = np.zeros(len(lists) + 1, dtype=np.int32)
+    for k, l in enumerate(lists):
+        ptr[k + 1] = ptr[k] + len(l)
+    idx = np.ascontiguousarray(np.concatenate([np.asarray(l, dtype=np.int32) for l in lists])
+                               if lists else np.zeros(1, np.int32), dtype=np.int32)
+    nout = ds.template.ndev if full else len(lists)
+    d = DosOpts(dos_delta, len(lists), 1 if full else 0, ptr.ctypes.data, idx.ctypes.data)
+    T = np.zeros((n, len(pairs)))
+    ldos = np.zeros((n, max(nout, 1)))
+    nc = max(ds.template.ncont, 1)
+    ncyc = np.zeros((n, nc), dtype=np.int32)
+    st = np.zeros(n, dtype=np.int32)
+    o = _opts(delta, 1, 2, precision, max_batch, flags=flags)
+    check(load().hipnegf_solve_transmission_dos(eng.handle, ds.handle, n, _ptr(E), _ptr(sl), C.byref(o),
+                                                len(pairs), _ptr(ni), _ptr(nf), C.byref(d), _ptr(T),
+                                                _ptr(ldos), _ptr(ncyc), _ptr(st)))
+    return T, ldos[:, :nout], ncyc, st
 
 
 def solve_scba_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, coupling: np.ndarray,

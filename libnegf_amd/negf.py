@@ -233,6 +233,32 @@ class NEGF:
         """(npairs, npoints): one row per lead pair, as the reference's class."""
         return self._associated("negf_associate_transmission")
 
+    def get_ldos(self):
+        """(nldos, npoints): local DOS of each projection, as the reference's class."""
+        return self._associated("negf_associate_ldos")
+
+    def set_ldos_intervals(self, istart, iend):
+        """LDOS on orbital intervals istart[i]..iend[i] (1-based, inclusive)."""
+        a, b = _i32(istart), _i32(iend)
+        self._call("negf_init_ldos", C.c_int(a.size))
+        self._call("negf_set_ldos_intervals", C.c_int(a.size), hip._ptr(a), hip._ptr(b))
+
+    def init_ldos(self, nldos: int):
+        self._call("negf_init_ldos", C.c_int(nldos))
+
+    def set_ldos_indexes(self, ildos: int, idx):
+        """Projection ildos (1-based) sums the given 1-based orbitals."""
+        a = _i32(idx)
+        self._call("negf_set_ldos_indexes", C.c_int(ildos), C.c_int(a.size), hip._ptr(a))
+
+    def set_tun_indexes(self, idx=None):
+        """tun_proj: only these 1-based orbitals of the traced block enter T (None: all)."""
+        if idx is None:
+            self._call("negf_set_tun_indexes", C.c_int(-1), None)
+        else:
+            a = _i32(idx)
+            self._call("negf_set_tun_indexes", C.c_int(a.size), hip._ptr(a))
+
     def get_energy_current(self):
         """(ncont, npoints) Meir-Wingreen energy-resolved currents."""
         return self._associated("negf_associate_energy_current")

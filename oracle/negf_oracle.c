@@ -418,9 +418,32 @@ static double trace_re(int n, const zc *A) {
   return t;
 }
 
+/* get_tun_mask (iterative_cpu.F90:585-617): row r of block blk (0-based)
+ * enters the trace iff the 1-based device orbital index of that row is in
+ * tun[0..ntun-1]; ntun == 0 (tun_proj not set): every row.  Returns NULL for
+ * "every row". */
+static unsigned char *tun_mask(const esh_t *e, int blk, int ntun, const int *tun) {
+  if (ntun <= 0 || !tun) return NULL;
+  int istart = 0;
+  for (int i = 0; i < blk; ++i) istart += e->b[i];
+  int n = e->b[blk], iend = istart + n + 1;
+  unsigned char *m = (unsigned char *)calloc(n ? n : 1, 1);
+  for (int k = 0; k < ntun; ++k)
+    if (tun[k] > istart && tun[k] < iend) m[tun[k] - istart - 1] = 1;
+  return m;
+}
+
+/* ztrace_dns with an optional mask (sparsekit_drv.F90, trace(mat, mask)) */
+static double trace_re_mask(int n, const zc *A, const unsigned char *mask) {
+  zc t = 0.0;
+  for (int i = 0; i < n; ++i)
+    if (!mask || mask[i]) t += A[i + (size_t)i * n];
+  return creal(t);
+}
+
 /* calculate_single_transmission_2_contacts (iterative_cpu.F90:399-470):
- * T = |Re Tr[Gam A - Gam G Gam G^dag]| on block 1. */
-static double trans_2c(int n, const zc *G11, const zc *Sig) {
+ * T = |Re Tr[Gam A - Gam G Gam G^dag]| on the block of G (mask: tun_proj rows). */
+static double trans_2c(int n, const zc *G11, const zc *Sig, const unsigned char *mask) {
   size_t nn = (size_t)n * n;
   zc *GA = zalloc(nn), *Gam = zalloc(nn), *w1 = zalloc(nn), *w2 = zalloc(nn), *AA = zalloc(nn);
   zdag(n, n, G11, GA);
@@ -431,13 +454,68 @@ static double trans_2c(int n, const zc *G11, const zc *Sig) {
   for (size_t t = 0; t < nn; ++t) AA[t] = I * (G11[t] - GA[t]);
   or_zgemm('N', 'N', n, n, n, 1.0, Gam, n, AA, n, 0.0, w2, n);
   for (size_t t = 0; t < nn; ++t) w2[t] -= w1[t];
-  double T = fabs(trace_re(n, w2));
+  double T = fabs(trace_re_mask(n, w2, mask));
   free(GA); free(Gam); free(w1); free(w2); free(AA);
   return T;
 }
 
-double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int ni, int nf,
-                             double eps_drop) {
+/* calculate_single_transmission_N_contacts (iterative_cpu.F90:477-583) for
+ * one pair, given the gsmr blocks g and the Gr tridiagonal blocks Gd (diagonal)
+ * and Gl (Gr(i+1,i)) down to the deeper contact block at least. */
+static double trans_nc(const or_struct *s, const esh_t *e, zc *const *g, zc *const *Gd,
+                       zc *const *Gl, zc *const *Sigma, int ni, int nf, double eps_drop,
+                       int ntun, const int *tun) {
+  int nbl = s->nbl;
+  int cbi = s->cblk[ni - 1], cbf = s->cblk[nf - 1];
+  int ct1, ct2;
+  if (cbi < cbf) { ct1 = ni; ct2 = nf; } else { ct1 = nf; ct2 = ni; }
+  int bl1 = s->cblk[ct1 - 1], bl2 = s->cblk[ct2 - 1];
+  double T = 0.0;
+  /* column blocks Gr(i,bl1), i = bl1..bl2 */
+  zc **Gc = (zc **)calloc(nbl + 2, sizeof(zc *));
+  Gc[bl1] = Gd[bl1 - 1];
+  if (bl2 > bl1) Gc[bl1 + 1] = Gl[bl1 - 1];
+  int early = 0;
+  if (nbl > 1 && bl2 - bl1 > 1) {
+    for (int i = bl1 + 1; i <= bl2; ++i) {
+      int nprev = e->b[i - 2], nr = e->b[bl1 - 1];
+      double mx = max_abs((size_t)nprev * nr, Gc[i - 1]);
+      if (mx < eps_drop) { T = eps_drop * eps_drop; early = 1; break; }
+      if (!Gc[i]) {
+        int nii = e->b[i - 1];
+        zc *w1 = zalloc((size_t)nii * nprev);
+        or_zgemm('N', 'N', nii, nprev, nii, -1.0, g[i - 1], nii, e->l[i - 2], nii, 0.0, w1, nii);
+        Gc[i] = zalloc((size_t)nii * nr);
+        or_zgemm('N', 'N', nii, nr, nprev, 1.0, w1, nii, Gc[i - 1], nprev, 0.0, Gc[i], nii);
+        free(w1);
+      }
+    }
+  }
+  if (!early) {
+    int n1 = e->b[bl1 - 1], n2 = e->b[bl2 - 1];
+    zc *G21 = Gc[bl2];
+    zc *Gam1 = zalloc((size_t)n1 * n1), *Gam2 = zalloc((size_t)n2 * n2);
+    zspectral(n1, Sigma[ct1 - 1], Gam1);
+    zspectral(n2, Sigma[ct2 - 1], Gam2);
+    zc *w1 = zalloc((size_t)n2 * n1), *w2 = zalloc((size_t)n2 * n1), *GA = zalloc((size_t)n1 * n2);
+    zc *TRS = zalloc((size_t)n2 * n2);
+    or_zgemm('N', 'N', n2, n1, n2, 1.0, Gam2, n2, G21, n2, 0.0, w1, n2);
+    or_zgemm('N', 'N', n2, n1, n1, 1.0, w1, n2, Gam1, n1, 0.0, w2, n2);
+    zdag(n2, n1, G21, GA);
+    or_zgemm('N', 'N', n2, n2, n1, 1.0, w2, n2, GA, n1, 0.0, TRS, n2);
+    unsigned char *mask = tun_mask(e, bl2 - 1, ntun, tun);
+    T = fabs(trace_re_mask(n2, TRS, mask));
+    free(mask);
+    free(Gam1); free(Gam2); free(w1); free(w2); free(GA); free(TRS);
+  }
+  for (int i = bl1 + 2; i <= bl2 && i <= nbl; ++i)
+    if (Gc[i]) free(Gc[i]);
+  free(Gc);
+  return T;
+}
+
+double or_transmission_sigma_tun(const or_struct *s, double E, zc *const *Sigma, int ni, int nf,
+                                 double eps_drop, int ntun, const int *tun) {
   int nbl = s->nbl, ncont = s->ncont;
   if (ncont == 1) return 0.0;
   esh_t e = build_esh(s, E, Sigma);
@@ -451,7 +529,9 @@ double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int
     zc *G11 = zalloc((size_t)e.b[0] * e.b[0]);
     gr_11(&e, g, G11);
     int ct1 = (cbi < cbf) ? ni : nf;
-    T = trans_2c(e.b[0], G11, Sigma[ct1 - 1]);
+    unsigned char *mask = tun_mask(&e, 0, ntun, tun);
+    T = trans_2c(e.b[0], G11, Sigma[ct1 - 1], mask);
+    free(mask);
     free(G11);
   } else {
     /* multiterminal path, iterative.F90:1716-1766, one (ni,nf) pair */
@@ -462,48 +542,7 @@ double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int
     gr_11(&e, g, Gd[0]);
     int nt1 = cbi > cbf ? cbi : cbf;
     if (nt1 > 1) gr_tridiag(&e, g, 2, nt1, Gd, Gu, Gl);
-    /* calculate_single_transmission_N_contacts, iterative_cpu.F90:477-583 */
-    int ct1, ct2;
-    if (cbi < cbf) { ct1 = ni; ct2 = nf; } else { ct1 = nf; ct2 = ni; }
-    int bl1 = s->cblk[ct1 - 1], bl2 = s->cblk[ct2 - 1];
-    /* column blocks Gr(i,bl1), i = bl1..bl2 */
-    zc **Gc = (zc **)calloc(nbl + 1, sizeof(zc *));
-    Gc[bl1] = Gd[bl1 - 1];
-    if (bl2 > bl1) Gc[bl1 + 1] = Gl[bl1 - 1];
-    int early = 0;
-    if (nbl > 1 && bl2 - bl1 > 1) {
-      for (int i = bl1 + 1; i <= bl2; ++i) {
-        int nprev = e.b[i - 2], nr = e.b[bl1 - 1];
-        double mx = max_abs((size_t)nprev * nr, Gc[i - 1]);
-        if (mx < eps_drop) { T = eps_drop * eps_drop; early = 1; break; }
-        if (!Gc[i]) {
-          int nii = e.b[i - 1];
-          zc *w1 = zalloc((size_t)nii * nprev);
-          or_zgemm('N', 'N', nii, nprev, nii, -1.0, g[i - 1], nii, e.l[i - 2], nii, 0.0, w1, nii);
-          Gc[i] = zalloc((size_t)nii * nr);
-          or_zgemm('N', 'N', nii, nr, nprev, 1.0, w1, nii, Gc[i - 1], nprev, 0.0, Gc[i], nii);
-          free(w1);
-        }
-      }
-    }
-    if (!early) {
-      int n1 = e.b[bl1 - 1], n2 = e.b[bl2 - 1];
-      zc *G21 = Gc[bl2];
-      zc *Gam1 = zalloc((size_t)n1 * n1), *Gam2 = zalloc((size_t)n2 * n2);
-      zspectral(n1, Sigma[ct1 - 1], Gam1);
-      zspectral(n2, Sigma[ct2 - 1], Gam2);
-      zc *w1 = zalloc((size_t)n2 * n1), *w2 = zalloc((size_t)n2 * n1), *GA = zalloc((size_t)n1 * n2);
-      zc *TRS = zalloc((size_t)n2 * n2);
-      or_zgemm('N', 'N', n2, n1, n2, 1.0, Gam2, n2, G21, n2, 0.0, w1, n2);
-      or_zgemm('N', 'N', n2, n1, n1, 1.0, w1, n2, Gam1, n1, 0.0, w2, n2);
-      zdag(n2, n1, G21, GA);
-      or_zgemm('N', 'N', n2, n2, n1, 1.0, w2, n2, GA, n1, 0.0, TRS, n2);
-      T = fabs(trace_re(n2, TRS));
-      free(Gam1); free(Gam2); free(w1); free(w2); free(GA); free(TRS);
-    }
-    for (int i = bl1 + 2; i <= bl2 && i <= nbl; ++i)
-      if (Gc[i]) free(Gc[i]);
-    free(Gc);
+    T = trans_nc(s, &e, g, Gd, Gl, Sigma, ni, nf, eps_drop, ntun, tun);
     for (int i = 0; i < nbl; ++i) free(Gd[i]);
     for (int i = 0; i + 1 < nbl; ++i) { free(Gu[i]); free(Gl[i]); }
     free(Gd); free(Gu); free(Gl);
@@ -512,6 +551,76 @@ double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int
   free(g);
   free_esh(&e);
   return T;
+}
+
+double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int ni, int nf,
+                             double eps_drop) {
+  return or_transmission_sigma_tun(s, E, Sigma, ni, nf, eps_drop, 0, NULL);
+}
+
+/* calculate_transmissions_and_dos (iterative.F90:1821-1952) with the
+ * self-energies given: ESH at the complex device energy z, gsmr blocks
+ * (nbl -> 2), Gr(1,1) and every Gr tridiagonal block; per pair T by the
+ * 2-contact formula on Gr(bl1,bl1) (ncont == 2) or the N-contact column sweep
+ * (ncont > 2), 0 for one contact; the device diagonal
+ * diag_i = Im((-1/pi) Gr_ii) (zrconcatm_csr 'I', sparsekit_drv.F90:2659-2730)
+ * and the LDOS projections (iterative.F90:1926-1944). */
+int or_transmission_dos_sigma(const or_struct *s, double Er, double Ei, zc *const *Sigma,
+                              int npairs, const int *ni, const int *nf, int ntun, const int *tun,
+                              int nldos, const int *ptr, const int *idx, int full, double *T,
+                              double *ledos, double *diag_out, double eps_drop) {
+  int nbl = s->nbl, ncont = s->ncont;
+  int ndev = 0;
+  for (int i = 0; i < nbl; ++i) ndev += s->bsz[i];
+  if (!full)
+    for (int l = 0; l < nldos; ++l)
+      for (int k = ptr[l]; k < ptr[l + 1]; ++k)
+        if (idx[k] < 1) return -1;
+  zc z = Er + I * Ei;
+  esh_t e = build_esh(s, z, Sigma);
+  zc **g = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gd = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gu = (zc **)calloc(nbl, sizeof(zc *)), **Gl = (zc **)calloc(nbl, sizeof(zc *));
+  gsmr_blocks(&e, 2, g);
+  Gd[0] = zalloc((size_t)e.b[0] * e.b[0]);
+  gr_11(&e, g, Gd[0]);
+  if (nbl > 1) gr_tridiag(&e, g, 2, nbl, Gd, Gu, Gl);
+  for (int p = 0; p < npairs; ++p) {
+    if (ncont < 2) {
+      T[p] = 0.0;
+    } else if (ncont == 2) {
+      int ct1 = (s->cblk[ni[p] - 1] < s->cblk[nf[p] - 1]) ? ni[p] : nf[p];
+      int bl1 = s->cblk[ct1 - 1];
+      unsigned char *mask = tun_mask(&e, bl1 - 1, ntun, tun);
+      T[p] = trans_2c(e.b[bl1 - 1], Gd[bl1 - 1], Sigma[ct1 - 1], mask);
+      free(mask);
+    } else {
+      T[p] = trans_nc(s, &e, g, Gd, Gl, Sigma, ni[p], nf[p], eps_drop, ntun, tun);
+    }
+  }
+  double *diag = (double *)calloc(ndev ? ndev : 1, sizeof(double));
+  const zc zc_ = -1.0 / M_PI;
+  for (int i = 0, o = 0; i < nbl; o += e.b[i], ++i)
+    for (int r = 0; r < e.b[i]; ++r) diag[o + r] = cimag(zc_ * Gd[i][r + (size_t)r * e.b[i]]);
+  if (diag_out) memcpy(diag_out, diag, sizeof(double) * ndev);
+  if (ledos) {
+    if (full) {
+      for (int r = 0; r < ndev; ++r) ledos[r] = 0.0 + diag[r];
+    } else {
+      for (int l = 0; l < nldos; ++l) {
+        double acc = 0.0;
+        for (int k = ptr[l]; k < ptr[l + 1]; ++k)
+          if (idx[k] <= ndev) acc += diag[idx[k] - 1];
+        ledos[l] = acc;
+      }
+    }
+  }
+  free(diag);
+  for (int i = 0; i < nbl; ++i) { free(Gd[i]); free(g[i]); }
+  for (int i = 0; i + 1 < nbl; ++i) { free(Gu[i]); free(Gl[i]); }
+  free(g); free(Gd); free(Gu); free(Gl);
+  free_esh(&e);
+  return 0;
 }
 
 /* compute_contacts (contselfenergy.F90:445-491) at E + i delta, then
@@ -532,6 +641,29 @@ double or_transmission(const or_struct *s, double E, double delta, int ni, int n
   for (int c = 0; c < s->ncont; ++c) free(Sig[c]);
   free(Sig);
   return T;
+}
+
+/* tunneling_and_dos with LDOS (integrations.F90:1745-1766): contacts at
+ * E + i delta, calculate_transmissions_and_dos at E + i dos_delta. */
+int or_transmission_dos(const or_struct *s, double E, double delta, double dos_delta, int npairs,
+                        const int *ni, const int *nf, int ntun, const int *tun, int nldos,
+                        const int *ptr, const int *idx, int full, double *T, double *ledos,
+                        double *diag_out, int *ncyc_out, double sgfacc, double eps_drop) {
+  zc z = E + I * delta;
+  zc **Sig = (zc **)calloc(s->ncont ? s->ncont : 1, sizeof(zc *));
+  for (int c = 0; c < s->ncont; ++c) {
+    int nd = s->bsz[s->cblk[c] - 1];
+    Sig[c] = zalloc((size_t)nd * nd);
+    int ncyc = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
+                                s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
+                                0, 0, sgfacc);
+    if (ncyc_out) ncyc_out[c] = ncyc;
+  }
+  int r = or_transmission_dos_sigma(s, E, dos_delta, Sig, npairs, ni, nf, ntun, tun, nldos, ptr,
+                                    idx, full, T, ledos, diag_out, eps_drop);
+  for (int c = 0; c < s->ncont; ++c) free(Sig[c]);
+  free(Sig);
+  return r;
 }
 
 int or_gr_tridiag(const or_struct *s, double E, zc *const *Sigma, zc *const *Grd,
@@ -826,6 +958,283 @@ int or_scba_meir_wingreen(const or_struct *s, double E, double delta, const doub
 /* ======================================================================= */
 
 /* fermi_f, distributions.F90:46-74 */
+/* ======================================================================= */
+/* density matrix: contour and real-axis integration                        */
+/* ======================================================================= */
+
+/* gauleg (integrations.F90:1468-1526): Gauss-Legendre nodes and weights on
+ * [x1, x2] (x1 > x2 allowed), Newton refinement of the Legendre zeros to 1e-15. */
+void or_gauleg(double x1, double x2, double *x, double *w, int n) {
+  const double acc = 1e-15;
+  int m = (n + 1) / 2;
+  double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int i = 1; i <= m; ++i) {
+    double z = cos(M_PI * (i - 0.25) / (n + 0.5)), z1, pp, p0, p1, p2;
+    for (;;) {
+      p0 = 1.0;
+      p1 = 0.0;
+      for (int k = 1; k <= n; ++k) {
+        p2 = p1;
+        p1 = p0;
+        p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+      }
+      pp = n * (z * p0 - p1) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p0 / pp;
+      if (fabs(z - z1) <= acc) break;
+    }
+    x[i - 1] = xm - xl * z;
+    x[n - i] = xm + xl * z;
+    w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - i] = w[i - 1];
+  }
+}
+
+/* fermi_fc (distributions.F90:77-110) */
+zc or_fermi_c(zc E, double Ef, double kT) {
+  if (kT == 0.0) return creal(E) > Ef ? 0.0 : 1.0;
+  if (fabs((creal(E) - Ef) / kT) > 30) return creal(E) > Ef ? cexp(-(E - Ef) / kT) : 1.0;
+  return 1.0 / (cexp((E - Ef) / kT) + 1.0);
+}
+
+/* contour_int_def without inelastic terms (integrations.F90:675-835):
+ * circle from Elow (path 1, np1 points), the segment at Im z = Lambda
+ * (kT > 0) or the rest of the circle (kT = 0) (path 2, np2 points), and the
+ * npoles Fermi poles (path 3).  z[np1+np2+npoles], w[...] out.  Returns the
+ * number of points. */
+int or_contour_grid(double Elow, double muref, double kbT, int n_kt, int np1, int np2, int npoles,
+                    double g_spin, zc *z, zc *w) {
+  double nkT = n_kt * kbT, mumin = muref - nkT, Lambda;
+  Lambda = 2.0 * npoles * kbT * M_PI;
+  if (npoles == 0) Lambda = 0.5 * kbT * M_PI;
+  double Centre = (Lambda * Lambda - Elow * Elow + mumin * mumin) / (2.0 * (mumin - Elow));
+  double Rad = Centre - Elow, alpha;
+  if (kbT == 0.0) alpha = 0.1 * M_PI;
+  else alpha = atan(Lambda / (mumin - Centre));
+  double *x = (double *)calloc(np1 + np2 + 1, sizeof(double));
+  double *wt = (double *)calloc(np1 + np2 + 1, sizeof(double));
+  or_gauleg(M_PI, alpha, x, wt, np1);
+  for (int i = 0; i < np1; ++i) {
+    zc Pc = Rad * cexp(I * x[i]);
+    z[i] = Centre + Pc;
+    w[i] = I * Pc * g_spin * wt[i] / (2.0 * M_PI);
+  }
+  zc z1 = 0, zdiff = 0;
+  if (kbT == 0.0) {
+    or_gauleg(alpha, 0.0, x, wt, np2);
+  } else {
+    z1 = muref + nkT + I * Lambda;
+    zc z2 = muref - nkT + I * Lambda;
+    zdiff = z2 - z1;
+    or_gauleg(1.0, 0.0, x, wt, np2);
+  }
+  for (int i = 0; i < np2; ++i) {
+    int k = np1 + i;
+    if (kbT == 0.0) {
+      zc Pc = Rad * cexp(I * x[i]);
+      z[k] = Centre + Pc;
+      double dt = g_spin * wt[i] / (2.0 * M_PI);
+      w[k] = dt * Pc * I;
+    } else {
+      z[k] = z1 + x[i] * zdiff;
+      zc ff = or_fermi_c(z[k], muref, kbT);
+      w[k] = g_spin * zdiff * ff * wt[i] / (2.0 * M_PI);
+    }
+  }
+  for (int i = 1; i <= npoles; ++i) {
+    int k = np1 + np2 + i - 1;
+    z[k] = muref + I * kbT * M_PI * (2.0 * i - 1.0);
+    w[k] = -I * g_spin * kbT;
+  }
+  free(x);
+  free(wt);
+  return np1 + np2 + npoles;
+}
+
+/* real_axis_int_def without inelastic terms (integrations.F90:989-1084):
+ * Gauss-Legendre on [mumin - Omega, mumax + Omega], Omega = n_kt * kbt_max;
+ * z = x + i delta, w = g_spin w_x / (2 pi). */
+int or_real_axis_grid(double mumin, double mumax, double kbt_max, int n_kt, int np, double delta,
+                      double g_spin, zc *z, zc *w) {
+  double Omega = n_kt * kbt_max;
+  double *x = (double *)calloc(np + 1, sizeof(double)), *wt = (double *)calloc(np + 1, sizeof(double));
+  or_gauleg(mumin - Omega, mumax + Omega, x, wt, np);
+  for (int i = 0; i < np; ++i) {
+    z[i] = x[i] + I * delta;
+    w[i] = g_spin * wt[i] / (2.0 * M_PI);
+  }
+  free(x);
+  free(wt);
+  return np;
+}
+
+/* Tlc = z SMC - HMC (nd x nc), Tcl = (conj(z) SMC - HMC)^dag (nc x nd):
+ * compute_contacts_dns, contselfenergy.F90:470-475 */
+static void t_blocks(const or_struct *s, int c, zc z, zc *Tlc, zc *Tcl) {
+  int nc = s->nc[c], nd = s->bsz[s->cblk[c] - 1];
+  zc *Tp = zalloc((size_t)nd * nc);
+  for (size_t t = 0; t < (size_t)nd * nc; ++t) {
+    zc sm = (s->SMC && s->SMC[c]) ? s->SMC[c][t] : 0.0;
+    Tlc[t] = z * sm - s->HMC[c][t];
+    Tp[t] = conj(z) * sm - s->HMC[c][t];
+  }
+  zdag(nd, nc, Tp, Tcl);
+  free(Tp);
+}
+
+/* compute_Gr + calculate_Gr (iterative.F90:90-148) at a complex z, contacts at
+ * z (integrations.F90:2300), with the outer blocks of calculate_Gr_outer
+ * (iterative.F90:1463-1573): Glc[k] = -Gr(cb,cb) Tlc GS (nd x nc),
+ * Gcl[k] = -GS Tcl Gr(cb,cb) (nc x nd), unmasked.  Blocks are caller
+ * allocated (Gd[nbl], Gu/Gl[nbl-1]; Glc/Gcl may be NULL). */
+int or_gr_point(const or_struct *s, double zr, double zi, zc *const *Gdo, zc *const *Guo,
+                zc *const *Glo, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc) {
+  int nbl = s->nbl, ncont = s->ncont;
+  zc z = zr + I * zi;
+  zc **Sig = (zc **)calloc(ncont ? ncont : 1, sizeof(zc *)), **GS = (zc **)calloc(ncont ? ncont : 1, sizeof(zc *));
+  for (int c = 0; c < ncont; ++c) {
+    int nd = s->bsz[s->cblk[c] - 1];
+    Sig[c] = zalloc((size_t)nd * nd);
+    GS[c] = zalloc((size_t)s->nc[c] * s->nc[c]);
+    int n = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
+                             s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
+                             GS[c], 0, sgfacc);
+    if (ncyc) ncyc[c] = n;
+  }
+  esh_t e = build_esh(s, z, Sig);
+  zc **g = (zc **)calloc(nbl, sizeof(zc *)), **Gd = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gu = (zc **)calloc(nbl, sizeof(zc *)), **Gl = (zc **)calloc(nbl, sizeof(zc *));
+  gsmr_blocks(&e, 2, g);
+  Gd[0] = zalloc((size_t)e.b[0] * e.b[0]);
+  gr_11(&e, g, Gd[0]);
+  if (nbl > 1) gr_tridiag(&e, g, 2, nbl, Gd, Gu, Gl);
+  for (int c = 0; c < ncont; ++c) {
+    int nc = s->nc[c], cb = s->cblk[c] - 1, nd = e.b[cb];
+    zc *Tlc = zalloc((size_t)nd * nc), *Tcl = zalloc((size_t)nc * nd);
+    zc *w1 = zalloc((size_t)nd * nc), *w2 = zalloc((size_t)nc * nd);
+    t_blocks(s, c, z, Tlc, Tcl);
+    if (Glc) {
+      or_zgemm('N', 'N', nd, nc, nd, -1.0, Gd[cb], nd, Tlc, nd, 0.0, w1, nd);
+      or_zgemm('N', 'N', nd, nc, nc, 1.0, w1, nd, GS[c], nc, 0.0, Glc[c], nd);
+    }
+    if (Gcl) {
+      or_zgemm('N', 'N', nc, nd, nc, -1.0, GS[c], nc, Tcl, nc, 0.0, w2, nc);
+      or_zgemm('N', 'N', nc, nd, nd, 1.0, w2, nc, Gd[cb], nd, 0.0, Gcl[c], nc);
+    }
+    free(Tlc); free(Tcl); free(w1); free(w2);
+  }
+  for (int i = 0; i < nbl; ++i) {
+    memcpy(Gdo[i], Gd[i], sizeof(zc) * (size_t)e.b[i] * e.b[i]);
+    free(Gd[i]);
+    free(g[i]);
+  }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    if (Guo) memcpy(Guo[i], Gu[i], sizeof(zc) * (size_t)e.b[i] * e.b[i + 1]);
+    if (Glo) memcpy(Glo[i], Gl[i], sizeof(zc) * (size_t)e.b[i] * e.b[i + 1]);
+    free(Gu[i]); free(Gl[i]);
+  }
+  free(g); free(Gd); free(Gu); free(Gl);
+  for (int c = 0; c < ncont; ++c) { free(Sig[c]); free(GS[c]); }
+  free(Sig); free(GS);
+  free_esh(&e);
+  return 0;
+}
+
+/* compute_Gn -> calculate_Gn_neq_components (iterative.F90:159-279) without
+ * interactions: contacts at E + i delta, ESH at the real E, Sigma_n(cb,cb) =
+ * sum_{j != ref, |f_j - f_ref| > eps} (f_j - f_ref) Gam_j, the Gn tridiagonal
+ * blocks, and calculate_Gn_outer (iterative.F90:1576-1665):
+ * Gnlc[k] = -Gn(cb,cb) Tlc GS^dag + (f_ref - f_k) Gr(cb,cb) Tlc i(GS - GS^dag)
+ * (the second term only for k != ref, |f_k - f_ref| > eps), zeroed where Tlc
+ * is 0 (mask); Gncl[k] = Gnlc[k]^dag.  frm[ncont+1]; ref 1-based (ncont+1:
+ * no reference, frm[ncont] = 0). */
+int or_gn_point(const or_struct *s, double E, double delta, const double *frm, int ref,
+                zc *const *Gndo, zc *const *Gnuo, zc *const *Gnlo, zc *const *Gnlc,
+                zc *const *Gncl, int *ncyc, double sgfacc, double eps) {
+  int nbl = s->nbl, ncont = s->ncont;
+  zc z = E + I * delta;
+  zc **Sig = (zc **)calloc(ncont ? ncont : 1, sizeof(zc *)), **GS = (zc **)calloc(ncont ? ncont : 1, sizeof(zc *));
+  for (int c = 0; c < ncont; ++c) {
+    int nd = s->bsz[s->cblk[c] - 1];
+    Sig[c] = zalloc((size_t)nd * nd);
+    GS[c] = zalloc((size_t)s->nc[c] * s->nc[c]);
+    int n = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
+                             s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
+                             GS[c], 0, sgfacc);
+    if (ncyc) ncyc[c] = n;
+  }
+  esh_t e = build_esh(s, E, Sig);
+  zc **g = (zc **)calloc(nbl, sizeof(zc *)), **Gd = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gu = (zc **)calloc(nbl, sizeof(zc *)), **Gl = (zc **)calloc(nbl, sizeof(zc *));
+  gsmr_blocks(&e, 2, g);
+  Gd[0] = zalloc((size_t)e.b[0] * e.b[0]);
+  gr_11(&e, g, Gd[0]);
+  if (nbl > 1) gr_tridiag(&e, g, 2, nbl, Gd, Gu, Gl);
+  zc **Snd = (zc **)calloc(nbl, sizeof(zc *)), **Snu = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Snl = (zc **)calloc(nbl, sizeof(zc *)), **Gnd = (zc **)calloc(nbl, sizeof(zc *));
+  zc **Gnu = (zc **)calloc(nbl, sizeof(zc *)), **Gnl = (zc **)calloc(nbl, sizeof(zc *));
+  for (int i = 0; i < nbl; ++i) {
+    Snd[i] = zalloc((size_t)e.b[i] * e.b[i]);
+    Gnd[i] = zalloc((size_t)e.b[i] * e.b[i]);
+  }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    size_t m = (size_t)e.b[i] * e.b[i + 1];
+    Snu[i] = zalloc(m); Snl[i] = zalloc(m); Gnu[i] = zalloc(m); Gnl[i] = zalloc(m);
+  }
+  double fref = frm[ref - 1];
+  for (int j = 0; j < ncont; ++j) {
+    double fd = frm[j] - fref;
+    if (j != ref - 1 && fabs(fd) > eps) {
+      int cb = s->cblk[j] - 1, n = e.b[cb];
+      zc *Gam = zalloc((size_t)n * n);
+      zspectral(n, Sig[j], Gam);
+      for (size_t t = 0; t < (size_t)n * n; ++t) Snd[cb][t] += fd * Gam[t];
+      free(Gam);
+    }
+  }
+  gn_tridiag(&e, g, Gd, Gu, Gl, Snd, Snu, Snl, Gnd, Gnu, Gnl);
+  for (int k = 0; k < ncont; ++k) {
+    int nc = s->nc[k], cb = s->cblk[k] - 1, nd = e.b[cb];
+    zc *Tlc = zalloc((size_t)nd * nc), *Tcl = zalloc((size_t)nc * nd);
+    zc *gA = zalloc((size_t)nc * nc), *w2 = zalloc((size_t)nd * nc), *w3 = zalloc((size_t)nd * nc);
+    zc *w1 = zalloc((size_t)nd * nc), *A = zalloc((size_t)nc * nc);
+    t_blocks(s, k, z, Tlc, Tcl);
+    zdag(nc, nc, GS[k], gA);
+    or_zgemm('N', 'N', nd, nc, nc, 1.0, Tlc, nd, gA, nc, 0.0, w2, nd);
+    or_zgemm('N', 'N', nd, nc, nd, -1.0, Gnd[cb], nd, w2, nd, 0.0, w3, nd);
+    if (fabs(frm[k] - fref) > eps && k != ref - 1) {
+      zc fd = fref - frm[k];
+      zspectral(nc, GS[k], A);
+      or_zgemm('N', 'N', nd, nc, nc, 1.0, Tlc, nd, A, nc, 0.0, w2, nd);
+      or_zgemm('N', 'N', nd, nc, nd, fd, Gd[cb], nd, w2, nd, 0.0, w1, nd);
+    }
+    for (size_t t = 0; t < (size_t)nd * nc; ++t) {
+      zc v = w3[t] + w1[t];
+      if (Tlc[t] == 0.0) v = 0.0;
+      if (Gnlc) Gnlc[k][t] = v;
+      w1[t] = v;
+    }
+    if (Gncl) zdag(nd, nc, w1, Gncl[k]);
+    free(Tlc); free(Tcl); free(gA); free(w1); free(w2); free(w3); free(A);
+  }
+  for (int i = 0; i < nbl; ++i) {
+    memcpy(Gndo[i], Gnd[i], sizeof(zc) * (size_t)e.b[i] * e.b[i]);
+    free(Gnd[i]); free(Snd[i]); free(Gd[i]); free(g[i]);
+  }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    size_t m = (size_t)e.b[i] * e.b[i + 1];
+    if (Gnuo) memcpy(Gnuo[i], Gnu[i], sizeof(zc) * m);
+    if (Gnlo) memcpy(Gnlo[i], Gnl[i], sizeof(zc) * m);
+    free(Gnu[i]); free(Gnl[i]); free(Snu[i]); free(Snl[i]); free(Gu[i]); free(Gl[i]);
+  }
+  free(Gnd); free(Gnu); free(Gnl); free(Snd); free(Snu); free(Snl);
+  free(g); free(Gd); free(Gu); free(Gl);
+  for (int c = 0; c < ncont; ++c) { free(Sig[c]); free(GS[c]); }
+  free(Sig); free(GS);
+  free_esh(&e);
+  return 0;
+}
+
 double or_fermi(double E, double Ef, double kT) {
   if (kT == 0.0) return E > Ef ? 0.0 : 1.0;
   if (fabs((E - Ef) / kT) > 30) return E > Ef ? exp(-(E - Ef) / kT) : 1.0;

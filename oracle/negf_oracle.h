@@ -69,6 +69,30 @@ double or_transmission(const or_struct *s, double E, double delta, int ni, int n
 /* Transmission from precomputed self-energies (build_ESH + sweeps only). */
 double or_transmission_sigma(const or_struct *s, double E, zc *const *Sigma, int ni, int nf,
                              double eps_drop);
+/* Same with tun_proj (libnegf.F90:1044-1058): only the rows of the traced
+ * block whose 1-based device orbital is in tun[ntun] enter T (get_tun_mask,
+ * iterative_cpu.F90:585-617); ntun == 0: all rows. */
+double or_transmission_sigma_tun(const or_struct *s, double E, zc *const *Sigma, int ni, int nf,
+                                 double eps_drop, int ntun, const int *tun);
+
+/* Transmission and local DOS at one energy, calculate_transmissions_and_dos
+ * (iterative.F90:1821-1952) as tunneling_and_dos calls it
+ * (integrations.F90:1745-1766): contacts at E + i delta, the device at
+ * E + i dos_delta.  npairs pairs ni/nf (1-based) -> T[npairs].
+ * LDOS projections: list l holds the 1-based orbitals idx[ptr[l] .. ptr[l+1]-1]
+ * (indexes beyond the device are skipped) -> ledos[nldos]; full != 0: one
+ * value per device orbital -> ledos[ndev].  diag_out[ndev] (may be NULL):
+ * -Im(Gr_ii)/pi.  ncyc_out[ncont] may be NULL.  Returns 0, or -1 for an
+ * index < 1 in a list. */
+int or_transmission_dos(const or_struct *s, double E, double delta, double dos_delta, int npairs,
+                        const int *ni, const int *nf, int ntun, const int *tun, int nldos,
+                        const int *ptr, const int *idx, int full, double *T, double *ledos,
+                        double *diag_out, int *ncyc_out, double sgfacc, double eps_drop);
+/* Same with the self-energies given, device energy Er + i Ei. */
+int or_transmission_dos_sigma(const or_struct *s, double Er, double Ei, zc *const *Sigma,
+                              int npairs, const int *ni, const int *nf, int ntun, const int *tun,
+                              int nldos, const int *ptr, const int *idx, int full, double *T,
+                              double *ledos, double *diag_out, double eps_drop);
 
 /* Elastic dephasing SCBA + Meir-Wingreen current at one energy:
  * calculate_elastic_scba (iterative.F90:289-363) with ElPhonDephD (elphdd.F90),
@@ -92,6 +116,30 @@ int or_scba_mw_sigma(const or_struct *s, double E, zc *const *Sigma, const doubl
  * blocks Gr(i,i+1), Gr(i+1,i) into Gru/Grl. */
 int or_gr_tridiag(const or_struct *s, double E, zc *const *Sigma, zc *const *Grd,
                   zc *const *Gru, zc *const *Grl);
+
+/* ---- density matrix ---------------------------------------------------------- */
+void or_gauleg(double x1, double x2, double *x, double *w, int n);      /* integrations.F90:1468-1526 */
+zc   or_fermi_c(zc E, double Ef, double kT);                             /* distributions.F90:77-110 */
+/* contour_int_def without inelastic terms (integrations.F90:675-835); returns
+ * np1 + np2 + npoles points z[], weights w[] */
+int or_contour_grid(double Elow, double muref, double kbT, int n_kt, int np1, int np2, int npoles,
+                    double g_spin, zc *z, zc *w);
+/* real_axis_int_def without inelastic terms (integrations.F90:989-1084) */
+int or_real_axis_grid(double mumin, double mumax, double kbt_max, int n_kt, int np, double delta,
+                      double g_spin, zc *z, zc *w);
+/* G^r blocks at complex z (contacts at z): Gd[nbl], Gu[nbl-1] = Gr(i,i+1),
+ * Gl[nbl-1] = Gr(i+1,i), outer Glc[k] = -Gr(cb,cb) Tlc GS (nd x nc) and
+ * Gcl[k] = -GS Tcl Gr(cb,cb) (nc x nd) (calculate_Gr + calculate_Gr_outer,
+ * iterative.F90:90-148, 1463-1573); caller-allocated, Gu/Gl/Glc/Gcl may be NULL. */
+int or_gr_point(const or_struct *s, double zr, double zi, zc *const *Gd, zc *const *Gu,
+                zc *const *Gl, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc);
+/* G^n blocks on the real axis (calculate_Gn_neq_components + calculate_Gn_outer,
+ * iterative.F90:159-279, 1576-1665): contacts at E + i delta, ESH at E;
+ * frm[ncont+1], ref 1-based (ncont+1: none); Gnlc masked by Tlc != 0,
+ * Gncl = Gnlc^dag. */
+int or_gn_point(const or_struct *s, double E, double delta, const double *frm, int ref,
+                zc *const *Gnd, zc *const *Gnu, zc *const *Gnl, zc *const *Gnlc,
+                zc *const *Gncl, int *ncyc, double sgfacc, double eps);
 
 /* ---- energy grid and currents --------------------------------------------- */
 double or_fermi(double E, double Ef, double kT);                  /* distributions.F90:46-74 */

@@ -41,6 +41,19 @@ def lib():
         L.or_transmission_sigma.restype = C.c_double
         L.or_transmission_sigma.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.c_int,
                                             C.c_double]
+        L.or_transmission_sigma_tun.restype = C.c_double
+        L.or_transmission_sigma_tun.argtypes = [C.c_void_p, C.c_double, C.c_void_p, C.c_int, C.c_int,
+                                                C.c_double, C.c_int, C.c_void_p]
+        L.or_transmission_dos.restype = C.c_int
+        L.or_transmission_dos.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_double, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_int,
+                                          C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_void_p, C.c_double, C.c_double]
+        L.or_transmission_dos_sigma.restype = C.c_int
+        L.or_transmission_dos_sigma.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p,
+                                                C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                                C.c_void_p, C.c_void_p, C.c_double]
         L.or_transmission_batch.restype = C.c_double
         L.or_transmission_batch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double, C.c_int,
                                             C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_double]
@@ -188,6 +201,60 @@ def transmission_sigma(s, E: float, sigmas: List[np.ndarray], ni=1, nf=2, eps_dr
     sg = [_cz(x) for x in sigmas]
     arr = (C.c_void_p * len(sg))(*[x.ctypes.data for x in sg])
     return lib().or_transmission_sigma(os_.ptr, E, C.cast(arr, C.c_void_p), ni, nf, eps_drop)
+
+
+def transmission_sigma_tun(s, E: float, sigmas: List[np.ndarray], ni=1, nf=2, eps_drop=0.0,
+                           tun=None):
+    os_ = OracleStruct(s)
+    sg = [_cz(x) for x in sigmas]
+    arr = (C.c_void_p * len(sg))(*[x.ctypes.data for x in sg])
+    tv = np.ascontiguousarray(tun if tun is not None else [0], dtype=np.int32)
+    return lib().or_transmission_sigma_tun(os_.ptr, E, C.cast(arr, C.c_void_p), ni, nf, eps_drop,
+                                           0 if tun is None else len(tv), tv.ctypes.data)
+
+
+def proj_csr(lists):
+    """LDOS projections (list of 1-based orbital index lists) -> (ptr, idx) int32."""
+    ptr = np.zeros(len(lists) + 1, dtype=np.int32)
+    for i, l in enumerate(lists):
+        ptr[i + 1] = ptr[i] + len(l)
+    idx = np.concatenate([np.asarray(l, dtype=np.int32) for l in lists]) if lists else np.zeros(0, np.int32)
+    return ptr, np.ascontiguousarray(idx, dtype=np.int32)
+
+
+def transmission_dos(s, E: float, delta: float, dos_delta: float, pairs=((1, 2),), proj=None,
+                     full: bool = False, tun=None, eps_drop: float = 0.0, sigmas=None):
+    """calculate_transmissions_and_dos at one energy.  Returns dict(T[npairs],
+    ldos[nldos or ndev], diag[ndev], ncyc[ncont])."""
+    os_ = OracleStruct(s)
+    ni = np.array([p[0] for p in pairs], dtype=np.int32)
+    nf = np.array([p[1] for p in pairs], dtype=np.int32)
+    lists = proj or []
+    ptr, idx = proj_csr(lists)
+    tv = np.ascontiguousarray(tun if tun is not None else [0], dtype=np.int32)
+    ntun = 0 if tun is None else len(tv)
+    T = np.zeros(len(pairs))
+    ledos = np.zeros(s.ndev if full else max(len(lists), 1))
+    diag = np.zeros(s.ndev)
+    ncyc = np.zeros(max(s.ncont, 1), dtype=np.int32)
+    nl = len(lists)
+    if sigmas is None:
+        r = lib().or_transmission_dos(os_.ptr, E, delta, dos_delta, len(pairs), ni.ctypes.data,
+                                      nf.ctypes.data, ntun, tv.ctypes.data, nl, ptr.ctypes.data,
+                                      idx.ctypes.data, 1 if full else 0, T.ctypes.data,
+                                      ledos.ctypes.data, diag.ctypes.data, ncyc.ctypes.data, SGFACC,
+                                      eps_drop)
+    else:
+        sg = [_cz(x) for x in sigmas]
+        arr = (C.c_void_p * max(len(sg), 1))(*[x.ctypes.data for x in sg])
+        r = lib().or_transmission_dos_sigma(os_.ptr, E, dos_delta, C.cast(arr, C.c_void_p), len(pairs),
+                                            ni.ctypes.data, nf.ctypes.data, ntun, tv.ctypes.data, nl,
+                                            ptr.ctypes.data, idx.ctypes.data, 1 if full else 0,
+                                            T.ctypes.data, ledos.ctypes.data, diag.ctypes.data,
+                                            eps_drop)
+    if r != 0:
+        raise ValueError("LDOS index < 1")
+    return dict(T=T, ldos=ledos[:s.ndev if full else nl], diag=diag, ncyc=ncyc[:s.ncont])
 
 
 def zinv(A: np.ndarray):
