@@ -28,7 +28,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "ldos", "dm"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--items", type=int, default=0, help="items per step (0: config default)")
     ap.add_argument("--nbl", type=int, default=0, help="override the number of layers")
@@ -265,6 +265,111 @@ def bench_c4(a, hip, wl, torch):
     print(json.dumps(out), flush=True)
 
 
+def bench_path(a, hip, wl, torch):
+    """The C3 k-slab at one k-point through the other block-solver paths:
+    --config ldos: T + per-layer local DOS (calculate_transmissions_and_dos,
+    dos_delta 1e-4) on the C3 energy grid; --config dm: the density-matrix
+    contour integrand (G^r at complex points with the outer blocks, summed
+    with weights) on Gauss-Legendre points of a half circle below mu = 0."""
+    slab, kgrid, Egrid, eta = wl.config3(nbl=a.nbl or None)
+    s = slab.at_k(*kgrid[17])
+    ds = hip.DeviceStructure(s)
+    eng = hip.Engine(0)
+    per_step = a.items or (len(Egrid) if a.config == "ldos" else 200)
+    n = s.bsz[0]
+    if a.config == "ldos":
+        E = Egrid[np.linspace(0, len(Egrid) - 1, per_step).astype(int)]
+        proj = [list(range(1 + n * i, 1 + n * (i + 1))) for i in range(s.nbl)]
+        run = lambda sel=None: hip.solve_transmission_dos(
+            eng, ds, E if sel is None else E[sel], proj=proj, dos_delta=1e-4, delta=eta)
+        workload = f"kslab_n{n}_nbl{s.nbl}_T_and_layer_LDOS_E{per_step}"
+        unit = "energy-points/s"
+    else:
+        x, wq = np.polynomial.legendre.leggauss(per_step)
+        th = 0.5 * np.pi * (x + 1.0)                  # (0, pi)
+        centre, rad = -2.5, 2.5                       # from E = -5 up to mu = 0
+        z = centre + rad * np.exp(1j * th)
+        w = 1j * rad * np.exp(1j * th) * wq * 0.5 * np.pi / np.pi
+        run = lambda sel=None: hip.density_accumulate(
+            eng, ds, z if sel is None else z[sel], w if sel is None else w[sel], mode=0, delta=eta)
+        workload = f"kslab_n{n}_nbl{s.nbl}_density_contour_points{per_step}"
+        unit = "contour-points/s"
+    for i in range(a.warmup):
+        t0 = time.time()
+        run()
+        log(f"warmup {i}: {time.time() - t0:.2f}s")
+    torch.cuda.synchronize()
+    if not a.no_profile:
+        hip.profile_clear()
+        hip.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r = run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern = None
+    roofline = None
+    if not a.no_profile:
+        hip.profile_enable(False)
+        kern = {hip.PT_NAMES[t]: hip.profile_read(t) for t in hip.PT_NAMES}
+        hip.profile_clear()
+        counted = {k: v for k, v in kern.items() if v["flops"] > 0 and v["ms"] > 0}
+        if counted:
+            name = max(counted, key=lambda k: counted[k]["ms"])
+            v = counted[name]
+            avg = v["ms"] * 1e-3 / v["launches"]
+            ach = v["flops"] / v["launches"] / avg / 1e12
+            roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name,
+                        "avg_launch_us": avg * 1e6, "launches": v["launches"],
+                        "share_of_step_time": v["ms"] * 1e-3 / elapsed}
+    value = a.steps * per_step / elapsed
+    log(f"{a.config}: {per_step} points/step, {elapsed / a.steps:.3f}s/step -> {value:.2f} {unit}")
+    cpu = None
+    if not a.no_cpu and a.cpu_items > 0:
+        from concurrent.futures import ThreadPoolExecutor
+        from oracle import pyoracle
+        pyoracle.build()
+        try:
+            backend = pyoracle.set_backend("mkl")
+        except RuntimeError:
+            backend = pyoracle.set_backend("builtin")
+        sel = np.linspace(0, per_step - 1, min(a.cpu_items, 16)).astype(int)
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16, len(sel))
+        t1 = time.perf_counter()
+        if a.config == "ldos":
+            with ThreadPoolExecutor(threads) as ex:
+                refs = list(ex.map(lambda j: pyoracle.transmission_dos(s, E[j], eta, 1e-4, proj=proj), sel))
+            secs = time.perf_counter() - t1
+            Tg, Lg, _, _ = run(sel)
+            Tr = np.array([q["T"][0] for q in refs])
+            Lr = np.array([q["ldos"] for q in refs])
+            err = dict(parity_max_rel_err_T=float(np.max(np.abs(Tg[:, 0] - Tr) / np.maximum(Tr, 1e-300))),
+                       parity_max_rel_err_ldos=float(np.abs(Lg - Lr).max() / np.abs(Lr).max()))
+        else:
+            with ThreadPoolExecutor(threads) as ex:
+                refs = list(ex.map(lambda j: pyoracle.gr_point(s, complex(z[j])), sel))
+            secs = time.perf_counter() - t1
+            acc, _ = run(sel)
+            got = hip.density_blocks(ds, acc)
+            ref_d = [sum(w[j] * q["Gd"][i] for j, q in zip(sel, refs)) for i in range(s.nbl)]
+            scale = max(np.abs(m).max() for m in ref_d)
+            err = dict(parity_max_rel_err_blocks=float(max(np.abs(g - r).max()
+                                                           for g, r in zip(got["Gd"], ref_d)) / scale))
+        cpu = dict(value=len(sel) / secs, unit=unit, cores=threads, kind="port",
+                   sample=f"{len(sel)} points spread over the step, oracle with {backend}, "
+                          "one thread per point", seconds=secs, **err)
+        log(f"cpu baseline {cpu}")
+    out = {"metric": "RGF energy-points/s", "value": value, "unit": unit, "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1000.0 * elapsed / a.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic (seeded generators, libnegf_amd/workloads.py)",
+           "config": dict(workload=workload, items_per_step=per_step, eta=eta),
+           "roofline": roofline, "cpu_baseline": cpu,
+           "kernel_ms": None if kern is None else {k: round(v["ms"], 3) for k, v in kern.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -281,6 +386,9 @@ def main():
     prec = hip.FP64 if a.precision == "fp64" else hip.MIXED
     if a.config == "c4":
         bench_c4(a, hip, wl, torch)
+        return
+    if a.config in ("ldos", "dm"):
+        bench_path(a, hip, wl, torch)
         return
     if a.serve:
         bench_serve(a, hip, wl, torch)

@@ -249,7 +249,28 @@ int hipnegf_solve_transmission_dos(hipnegf_engine e, hipnegf_struct st, int nite
                                    const hipnegf_dos_opts *d, double *T, double *ldos, int *ncyc,
                                    int *status);
 
-/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058): only the listed 1-based
+/* Density-matrix integrands (the per-point work of contour_int and
+ * real_axis_int, integrations.F90:844-1176).  acc (host, interleaved complex,
+ * in/out) holds dense blocks packed column-major at the offsets of
+ * hipnegf_density_layout: Gd[nbl] (bsz_i x bsz_i), Gu[nbl-1] = (i,i+1),
+ * Gl[nbl-1] = (i+1,i), Glc[ncont] (device block cblk x contact PL, nd x nc),
+ * Gcl[ncont] (nc x nd); offsets[3 nbl - 2 + 2 ncont] = total elements.
+ * mode 0 (contour point E + i Ei; calculate_Gr + calculate_Gr_outer,
+ * iterative.F90:90-148, 1463-1573): acc += sum_i w_i G^r(z_i), contacts at z_i.
+ * mode 1 (real-axis point E; calculate_Gn_neq_components + calculate_Gn_outer,
+ * iterative.F90:159-279, 1576-1665): contacts at E + i o->delta, device at E,
+ * frm[nitems*(ncont+1)] occupations, ref 1-based (ncont+1: none), acc +=
+ * sum_i w_i G^n(E_i).  w[2*nitems] complex weights.  outer: 0 device blocks
+ * only, 1 + Glc, 2 + Glc and Gcl.  Points are summed in order.  The caller
+ * masks the blocks to the overlap pattern (blk2csr, iterative.F90:1375-1450).
+ * status[nitems] (may be NULL) per-item status words. */
+int hipnegf_density_layout(hipnegf_struct st, long long *offsets);
+int hipnegf_density_accumulate(hipnegf_engine e, hipnegf_struct st, int nitems, const double *Er,
+                               const double *Ei, const int *slot, const hipnegf_opts *o, int mode,
+                               const double *w, const double *frm, int ref, int outer,
+                               double *acc, int *status);
+
+/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058): only the listed 1-based/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058): only the listed 1-based
  * device orbitals of the traced block enter every transmission trace of this
  * structure (get_tun_mask, iterative_cpu.F90:585-617).  n = 0 clears it. */
 int hipnegf_struct_set_tun_proj(hipnegf_struct st, int n, const int *idx);

@@ -7,8 +7,10 @@
  * src/api/libnegfAPICommon.f90:34-39; parameter block src/api/lnParams.h), so
  * a C/C++/ctypes caller of the reference links against this library
  * unchanged for the Landauer / Meir-Wingreen / elastic-dephasing path.
- * Entry points outside that path (density matrix, LDOS, k-point meshes,
- * inelastic models, file I/O) are not provided.
+ * Also provided: the local DOS alongside T and the density matrix (contour
+ * and real-axis integration of the same block solver).  Entry points outside
+ * that path (EFA / quasi-equilibrium densities, k-point meshes, inelastic
+ * models, the surface-function disk cache, the UPT reader) are not provided.
  *
  * Errors: the reference stops the process (error stop); here an entry point
  * that fails records a message (negf_get_last_error) and leaves its outputs
@@ -116,6 +118,30 @@ void negf_get_energies(int *handler, int *npoints, double *re_en, double *im_en,
 void negf_get_currents(int *handler, int *npairs, double *currents, int copy);
 void negf_associate_transmission(int *handler, int tr_shape[2], double **tr_pointer);
 void negf_associate_energy_current(int *handler, int currents_shape[2], double **currents_pointer);
+
+/* Files (libnegf_api.f90:289-309, 589-628, 1032-1043): negf_read_hs reads H
+ * (target 0) or S (target 1) from the formatted PETSc ASCII pair of files
+ * (real part, imaginary part; read_H, input_output.F90:41-264, full matrix);
+ * negf_write_tunneling_and_dos writes tunneling_<kpoint>_000000.dat and
+ * localDOS_<kpoint>_000000.dat (Fortran E17.8 columns) into the path set by
+ * negf_set_output (default "./"). */
+void negf_read_hs(int *handler, const char *real_path, const char *imag_path, int target_matrix);
+void negf_set_output(int *handler, const char *out_path);
+void negf_write_tunneling_and_dos(int *handler);
+
+/* Density matrix (negf_solve_density_dft, negf_get_dm, negf_set_outer;
+ * libnegf_api.f90:780-791, 836-861, 1052-1065): compute_density_dft
+ * (libnegf.F90:1664-1679) without inelastic terms -- the contour integral
+ * (Np_n(1) + Np_n(2) points and n_poles poles from lnparams.ec up to the
+ * reference contact's mu, chosen by min_or_max) and, when np_real > 0, the
+ * real-axis integral of G^n between the contacts' Fermi levels.  rho is
+ * returned as 1-based CSR (nrow = the size of H) on the pattern: diagonal,
+ * S pattern, the device/contact coupling pattern (outer blocks), and their
+ * transposes.  DorE = 'E' gives the energy-weighted matrix. */
+void negf_solve_density_dft(int *handler);
+void negf_get_dm(int *handler, int *nnz, int *nrow, int *rowpnt, int *colind, double *re_nzval,
+                 double *im_nzval, int copy);
+void negf_set_outer(int *handler, int *outer);
 
 /* Local DOS alongside the transmission (negf_init_ldos / negf_set_ldos_intervals /
  * negf_set_ldos_indexes / negf_associate_ldos, libnegf_api.f90:941-1021):

@@ -12,7 +12,9 @@
 //   meir_wingreen (fixed occupations)   integrations.F90:1804-1966
 //   electron_current / integrate_el     integrations.F90:2427-2457, 2688-2767
 // Every (E) point of a solve is sent to the engine in one batched call.
+#include <algorithm>
 #include <cmath>
+#include <complex>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -46,7 +48,7 @@ struct Session {
   int precision = 0, max_batch = 0;
   // results
   int npoints = 0, npairs = 0;
-  std::vector<double> en, tunn, curr_mat, currents;
+  std::vector<double> en, en_im, tunn, curr_mat, currents;
   std::vector<int> item_status, item_ncyc;
   int status = 0;
   std::string err;
@@ -63,6 +65,12 @@ struct Session {
   int ldos_cols = 0;
   bool tun_set = false;
   std::vector<int> tun_proj;
+  std::string out_path = "./";  // negf%out_path (set_outpath appends '/')
+  // density matrix (negf%rho, CSR 0-based here) and negf%outer_blocks
+  int outer = 2;
+  int rho_n = 0;
+  std::vector<int> rho_ptr, rho_col;
+  std::vector<double2> rho_val;
 };
 
 Session *sess(int *handler) {
@@ -330,6 +338,7 @@ void electron_current(Session &s) {
 
 void energy_grid(Session &s) {
   // tunneling_int_def, integrations.F90:1613-1668 (one process: no padding)
+  s.en_im.clear();
   s.npoints = (int)std::lround((s.p.emax - s.p.emin) / s.p.estep) + 1;
   s.en.resize(s.npoints);
   for (int i = 0; i < s.npoints; ++i) s.en[i] = s.p.emin + s.p.estep * i;
@@ -480,6 +489,327 @@ void compute_dephasing_transmission(Session &s) {
       s.curr_mat[(size_t)c * n + i] = cm[(size_t)i * s.ncont + c] * s.p.kwght;  // column-major
   s.npairs = 1;
   s.tunn.assign(s.curr_mat.begin(), s.curr_mat.begin() + n);
+}
+
+// ---------------------------------------------------------------------------
+// Density matrix: compute_density_dft (libnegf.F90:1664-1679) = extract_cont,
+// set_ref_cont (libnegf.F90:2132-2157) and contour_driver
+// (integrations.F90:597-646) without inelastic terms: contour_int_def +
+// contour_int (:675-897) and real_axis_int_def + real_axis_int (:989-1176).
+// The per-point block work runs batched on the engine (Engine::density); the
+// host builds the energy grids and assembles the CSR result.
+// ---------------------------------------------------------------------------
+const double kPi = 3.14159265358979323846;
+
+// gauleg, integrations.F90:1468-1526
+void gauleg(double x1, double x2, std::vector<double> &x, std::vector<double> &w, int n) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  const int m = (n + 1) / 2;
+  const double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int i = 1; i <= m; ++i) {
+    double z = std::cos(kPi * (i - 0.25) / (n + 0.5)), z1, pp, p0, p1, p2;
+    for (;;) {
+      p0 = 1.0;
+      p1 = 0.0;
+      for (int k = 1; k <= n; ++k) {
+        p2 = p1;
+        p1 = p0;
+        p0 = ((2.0 * k - 1.0) * z * p1 - (k - 1.0) * p2) / k;
+      }
+      pp = n * (z * p0 - p1) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p0 / pp;
+      if (std::fabs(z - z1) <= 1e-15) break;
+    }
+    x[i - 1] = xm - xl * z;
+    x[n - i] = xm + xl * z;
+    w[i - 1] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - i] = w[i - 1];
+  }
+}
+
+typedef std::complex<double> cplx;
+
+// fermi_fc, distributions.F90:77-110
+cplx fermi_c(cplx E, double Ef, double kT) {
+  if (kT == 0.0) return E.real() > Ef ? cplx(0.0) : cplx(1.0);
+  if (std::fabs((E.real() - Ef) / kT) > 30)
+    return E.real() > Ef ? std::exp(-(E - Ef) / kT) : cplx(1.0);
+  return 1.0 / (std::exp((E - Ef) / kT) + 1.0);
+}
+
+// contour_int_def without inelastic terms (integrations.F90:675-835)
+void contour_grid(double Elow, double muref, double kbT, int n_kt, int np1, int np2, int npoles,
+                  double g_spin, std::vector<cplx> &z, std::vector<cplx> &w) {
+  const cplx I(0.0, 1.0);
+  const double nkT = n_kt * kbT, mumin = muref - nkT;
+  double Lambda = 2.0 * npoles * kbT * kPi;
+  if (npoles == 0) Lambda = 0.5 * kbT * kPi;
+  const double Centre = (Lambda * Lambda - Elow * Elow + mumin * mumin) / (2.0 * (mumin - Elow));
+  const double Rad = Centre - Elow;
+  const double alpha = (kbT == 0.0) ? 0.1 * kPi : std::atan(Lambda / (mumin - Centre));
+  std::vector<double> x, wt;
+  z.clear();
+  w.clear();
+  gauleg(kPi, alpha, x, wt, np1);
+  for (int i = 0; i < np1; ++i) {
+    const cplx Pc = Rad * std::exp(I * x[i]);
+    z.push_back(Centre + Pc);
+    w.push_back(I * Pc * g_spin * wt[i] / (2.0 * kPi));
+  }
+  cplx z1 = 0.0, zdiff = 0.0;
+  if (kbT == 0.0) {
+    gauleg(alpha, 0.0, x, wt, np2);
+  } else {
+    z1 = cplx(muref + nkT, Lambda);
+    const cplx z2(muref - nkT, Lambda);
+    zdiff = z2 - z1;
+    gauleg(1.0, 0.0, x, wt, np2);
+  }
+  for (int i = 0; i < np2; ++i) {
+    if (kbT == 0.0) {
+      const cplx Pc = Rad * std::exp(I * x[i]);
+      z.push_back(Centre + Pc);
+      const double dt = g_spin * wt[i] / (2.0 * kPi);
+      w.push_back(dt * Pc * I);
+    } else {
+      const cplx Ec = z1 + x[i] * zdiff;
+      z.push_back(Ec);
+      w.push_back(g_spin * zdiff * fermi_c(Ec, muref, kbT) * wt[i] / (2.0 * kPi));
+    }
+  }
+  for (int i = 1; i <= npoles; ++i) {
+    z.push_back(cplx(muref, kbT * kPi * (2.0 * i - 1.0)));
+    w.push_back(-I * g_spin * kbT);
+  }
+}
+
+// Sparse accumulation of (row, col, value) entries; duplicates are summed.
+struct SparseAcc {
+  std::vector<std::vector<std::pair<int, double2>>> rows;
+  explicit SparseAcc(int n) : rows(n) {}
+  void add(int r, int c, double2 v) { rows[r].push_back({c, v}); }
+  void finalize() {
+    for (auto &row : rows) {
+      std::stable_sort(row.begin(), row.end(), [](const std::pair<int, double2> &a,
+                                           const std::pair<int, double2> &b) { return a.first < b.first; });
+      size_t o = 0;
+      for (size_t i = 0; i < row.size(); ++i) {
+        if (o > 0 && row[o - 1].first == row[i].first) {
+          row[o - 1].second.x += row[i].second.x;
+          row[o - 1].second.y += row[i].second.y;
+        } else {
+          row[o++] = row[i];
+        }
+      }
+      row.resize(o);
+    }
+  }
+  double2 at(int r, int c) const {
+    const auto &row = rows[r];
+    auto it = std::lower_bound(row.begin(), row.end(), c, [](const std::pair<int, double2> &a, int v) {
+      return a.first < v;
+    });
+    return (it != row.end() && it->first == c) ? it->second : make_double2(0.0, 0.0);
+  }
+};
+
+// One integration path's accumulated blocks put on the density-matrix pattern:
+// the full diagonal (the zero-initialised CSR of contour_int / real_axis_int),
+// blk2csr of the device blocks on the S pattern (iterative.F90:1375-1450:
+// values on device rows and columns, explicit zeros elsewhere), and the outer
+// blocks on the coupling pattern (HMC or SMC nonzero) at (block cb rows,
+// contact PL1 columns) and, for outer == 2, at the transposed position.
+SparseAcc place_blocks(Session &s, const hn::DevStruct &st, const std::vector<double2> &acc,
+                       const std::vector<long long> &lay, int N, bool lower) {
+  SparseAcc A(N);
+  const int nbl = st.nbl, ncont = st.ncont, ndev = st.ndev();
+  std::vector<int> start(nbl + 1, 0), blk_of(ndev);
+  for (int i = 0; i < nbl; ++i) start[i + 1] = start[i] + st.bsz[i];
+  for (int i = 0; i < nbl; ++i)
+    for (int r = start[i]; r < start[i + 1]; ++r) blk_of[r] = i;
+  const double2 z0 = make_double2(0.0, 0.0);
+  for (int r = 0; r < N; ++r) A.add(r, r, z0);
+  auto blockval = [&](int r, int c) -> double2 {
+    const int x = blk_of[r], y = blk_of[c];
+    const int rr = r - start[x], cc = c - start[y];
+    if (y == x) return acc[lay[x] + rr + (long long)cc * st.bsz[x]];
+    if (y == x + 1) return acc[lay[nbl + x] + rr + (long long)cc * st.bsz[x]];
+    if (y == x - 1) return acc[lay[2 * nbl - 1 + y] + rr + (long long)cc * st.bsz[x]];
+    throw std::runtime_error("blk2csr: overlap pattern outside the block tridiagonal (wrong PL size?)");
+  };
+  const bool sid = s.cur >= (int)s.S.size() || s.s_id[s.cur] || !s.S[s.cur].present();
+  if (sid) {
+    for (int r = 0; r < N; ++r) A.add(r, r, r < ndev ? blockval(r, r) : z0);
+  } else {
+    const Csr &S = s.S[s.cur];
+    if (S.n != N) throw std::runtime_error("S and H sizes differ");
+    for (int r = 0; r < N; ++r)
+      for (int k = S.ptr[r]; k < S.ptr[r + 1]; ++k) {
+        const int c = S.col[k];
+        A.add(r, c, (r < ndev && c < ndev) ? blockval(r, c) : z0);
+      }
+  }
+  const Csr &H = s.H[s.cur];
+  for (int k = 0; k < ncont; ++k) {
+    const int cb = st.cblk[k] - 1, nd = st.bsz[cb], nc = st.nc[k];
+    const int r0 = start[cb], c0 = s.surfstart[k] - 1;
+    const std::vector<double2> hm = extract(H, r0 + 1, r0 + nd, c0 + 1, c0 + nc);
+    std::vector<double2> sm;
+    if (!sid) sm = extract(s.S[s.cur], r0 + 1, r0 + nd, c0 + 1, c0 + nc);
+    const long long olc = lay[3 * nbl - 2 + k], ocl = lay[3 * nbl - 2 + ncont + k];
+    for (int j = 0; j < nc; ++j)
+      for (int i = 0; i < nd; ++i) {
+        const size_t t = i + (size_t)j * nd;
+        const bool nz = hm[t].x != 0.0 || hm[t].y != 0.0 || (!sid && (sm[t].x != 0.0 || sm[t].y != 0.0));
+        if (!nz) continue;
+        A.add(r0 + i, c0 + j, acc[olc + t]);
+        if (lower) A.add(c0 + j, r0 + i, acc[ocl + j + (size_t)i * nc]);
+      }
+  }
+  A.finalize();
+  return A;
+}
+
+void compute_density(Session &s) {
+  if (s.ncont < 1) throw std::runtime_error("density matrix needs at least one contact");
+  hn::DevStruct *st = &device_struct(s);
+  hn::Engine &eng = engine(s);
+  const int ncont = s.ncont, N = s.H[s.cur].n;
+  // set_ref_cont
+  int refcont;
+  double muref;
+  {
+    int imin = 0, imax = 0;
+    for (int c = 1; c < ncont; ++c) {
+      if (s.p.mu[c] < s.p.mu[imin]) imin = c;
+      if (s.p.mu[c] > s.p.mu[imax]) imax = c;
+    }
+    if (s.p.min_or_max == 0) { muref = s.p.mu[imin]; refcont = imin + 1; }
+    else if (s.p.min_or_max == 1) { muref = s.p.mu[imax]; refcont = imax + 1; }
+    else { muref = s.p.mu[imax]; refcont = ncont + 1; }
+  }
+  const bool energy = s.p.dore[0] == 'E';
+  hn::SolveOpts o = opts_of(s);
+  {  // readOldDM_SGFs for the density paths
+    const int r = s.p.readold_dm_sgfs;
+    o.sgf_mode = (r == 0) ? 1 : (r == 2) ? 2 : 0;
+    if (o.sgf_mode && !s.sgf_cache_on) {
+      eng.set_sgf_cache(2ull << 30);
+      s.sgf_cache_on = true;
+    }
+  }
+  const std::vector<long long> lay = hn::Engine::density_layout(*st);
+  const long long tot = lay.back();
+  SparseAcc rho(N);
+  s.item_status.clear();
+  auto run = [&](int mode, const std::vector<cplx> &z, const std::vector<cplx> &w,
+                 const std::vector<double> &frm) {
+    const int n = (int)z.size();
+    std::vector<double> Er(n), Ei(n);
+    std::vector<double2> ww(n);
+    for (int i = 0; i < n; ++i) {
+      Er[i] = z[i].real();
+      Ei[i] = z[i].imag();
+      ww[i] = make_double2(w[i].real(), w[i].imag());
+    }
+    DevBuf dE(sizeof(double) * n), dEi(sizeof(double) * n), dsl(sizeof(int) * n), dw(sizeof(double2) * n),
+        dF(sizeof(double) * std::max<size_t>(frm.size(), 1)), dA(sizeof(double2) * std::max<long long>(tot, 1)),
+        dS(sizeof(int) * n);
+    HN_CHECK(hipMemcpy(dE.p, Er.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(dEi.p, Ei.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
+    HN_CHECK(hipMemcpy(dw.p, ww.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    if (!frm.empty()) HN_CHECK(hipMemcpy(dF.p, frm.data(), sizeof(double) * frm.size(), hipMemcpyHostToDevice));
+    HN_CHECK(hipMemset(dA.p, 0, sizeof(double2) * tot));
+    hn::DensityOpts d;
+    d.mode = mode;
+    d.w = dw.as<double2>();
+    d.frm = mode == 1 ? dF.as<double>() : nullptr;
+    d.ref = refcont;
+    d.outer = s.outer;
+    d.status = dS.as<int>();
+    eng.density(*st, n, dE.as<double>(), mode == 0 ? dEi.as<double>() : nullptr, dsl.as<int>(), o, d,
+                dA.as<double2>());
+    std::vector<double2> acc(tot);
+    HN_CHECK(hipMemcpy(acc.data(), dA.p, sizeof(double2) * tot, hipMemcpyDeviceToHost));
+    std::vector<int> stv(n);
+    HN_CHECK(hipMemcpy(stv.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    s.item_status.insert(s.item_status.end(), stv.begin(), stv.end());
+    return acc;
+  };
+  // en_grid as the reference leaves it: the last integration path's points
+  s.en.clear();
+  s.en_im.clear();
+  // ---- contour_int: rho = i (T - T^H), T = sum w Gr on the pattern ----
+  if (s.p.np_n[0] + s.p.np_n[1] + s.p.n_poles > 0) {
+    // kbT of the reference contact (the contact of max mu when there is none)
+    const int kc = refcont <= ncont ? refcont - 1 : [&] {
+      int im = 0;
+      for (int c = 1; c < ncont; ++c) if (s.p.mu[c] > s.p.mu[im]) im = c;
+      return im;
+    }();
+    std::vector<cplx> z, w;
+    contour_grid(s.p.ec, muref, s.p.kbt_dm[kc], s.p.n_kt, s.p.np_n[0], s.p.np_n[1], s.p.n_poles,
+                 s.p.g_spin, z, w);
+    if (energy)
+      for (size_t i = 0; i < z.size(); ++i) w[i] *= z[i];
+    for (const cplx &v : z) {
+      s.en.push_back(v.real());
+      s.en_im.push_back(v.imag());
+    }
+    const std::vector<double2> acc = run(0, z, w, {});
+    SparseAcc T = place_blocks(s, *st, acc, lay, N, s.outer == 2);
+    for (int r = 0; r < N; ++r)
+      for (const auto &e : T.rows[r]) {  // zspectral: i (T - T^H)
+        const double2 v = e.second;
+        rho.add(r, e.first, make_double2(-v.y, v.x));       // i T(r,c)
+        rho.add(e.first, r, make_double2(-v.y, -v.x));      // -i conj(T(r,c)) at (c,r)
+      }
+  }
+  // ---- real_axis_int: rho += sum w Gn on the pattern ----
+  if (s.p.np_real > 0) {
+    double mumin = s.p.mu[0], mumax = s.p.mu[0], kmax = s.p.kbt_dm[0];
+    for (int c = 1; c < ncont; ++c) {
+      mumin = std::min(mumin, s.p.mu[c]);
+      mumax = std::max(mumax, s.p.mu[c]);
+      kmax = std::max(kmax, s.p.kbt_dm[c]);
+    }
+    const double Omega = s.p.n_kt * kmax;
+    std::vector<double> x, wt;
+    gauleg(mumin - Omega, mumax + Omega, x, wt, s.p.np_real);
+    const int n = s.p.np_real;
+    std::vector<cplx> z(n), w(n);
+    std::vector<double> frm((size_t)n * (ncont + 1), 0.0);
+    s.en.clear();
+    s.en_im.clear();
+    for (int i = 0; i < n; ++i) {
+      z[i] = cplx(x[i], s.p.delta);
+      w[i] = s.p.g_spin * wt[i] / (2.0 * kPi);
+      if (energy) w[i] *= x[i];
+      for (int c = 0; c < ncont; ++c) frm[(size_t)i * (ncont + 1) + c] = fermi(x[i], s.p.mu[c], s.p.kbt_dm[c]);
+      s.en.push_back(x[i]);
+      s.en_im.push_back(s.p.delta);
+    }
+    o.delta = s.p.delta;
+    const std::vector<double2> acc = run(1, z, w, frm);
+    SparseAcc Tn = place_blocks(s, *st, acc, lay, N, s.outer == 2);
+    for (int r = 0; r < N; ++r)
+      for (const auto &e : Tn.rows[r]) rho.add(r, e.first, e.second);
+  }
+  rho.finalize();
+  s.rho_n = N;
+  s.rho_ptr.assign(N + 1, 0);
+  s.rho_col.clear();
+  s.rho_val.clear();
+  for (int r = 0; r < N; ++r) {
+    for (const auto &e : rho.rows[r]) {
+      s.rho_col.push_back(e.first);
+      s.rho_val.push_back(e.second);
+    }
+    s.rho_ptr[r + 1] = (int)s.rho_col.size();
+  }
 }
 
 // compute_current, libnegf.F90:1904-1919
@@ -707,7 +1037,7 @@ void negf_get_energies(int *handler, int *npoints, double *re_en, double *im_en,
     if (copy)
       for (int i = 0; i < *npoints; ++i) {
         re_en[i] = s.en[i];
-        im_en[i] = 0.0;
+        im_en[i] = s.en_im.size() == s.en.size() ? s.en_im[i] : 0.0;
       }
   });
 }
@@ -733,6 +1063,181 @@ void negf_associate_energy_current(int *handler, int currents_shape[2], double *
     currents_shape[0] = s.npoints;
     currents_shape[1] = s.curr_mat.empty() ? 0 : s.ncont;
     *currents_pointer = s.curr_mat.empty() ? nullptr : s.curr_mat.data();
+  });
+}
+
+// read_HS (libnegf.F90:202-254) with read_H (input_output.F90:41-264) for the
+// formatted PETSc ASCII form, full matrix: two files (real, imaginary), each a
+// 4-line header ("% Size = nrow ncol", "% Nonzeros = nnz", two more lines)
+// and nnz lines "j i value" with 1-based indices.  target 0: H, 1: S.
+void negf_read_hs(int *handler, const char *real_path, const char *imag_path, int target_matrix) {
+  api(handler, [&](Session &s) {
+    if (target_matrix != 0 && target_matrix != 1)
+      throw std::runtime_error("libNEGF error. Wrong target_matrix: must be 0 (H) or 1 (S)");
+    struct Part { int nrow = 0, ncol = 0, nnz = 0; std::vector<int> i, j; std::vector<double> v; };
+    auto read_part = [](const char *path) {
+      FILE *f = std::fopen(path, "r");
+      if (!f) throw std::runtime_error("libNEGF error. Matrix files not found");
+      Part p;
+      char line[4096];
+      auto next = [&]() -> std::string {
+        if (!std::fgets(line, sizeof(line), f)) throw std::runtime_error("matrix file ended early");
+        return std::string(line);
+      };
+      auto fields = [](const std::string &l) {
+        std::vector<std::string> out;
+        std::string cur;
+        for (char ch : l) {
+          if (ch == ' ' || ch == '\t' || ch == ',' || ch == '\n' || ch == '\r') {
+            if (!cur.empty()) out.push_back(cur);
+            cur.clear();
+          } else {
+            cur += ch;
+          }
+        }
+        if (!cur.empty()) out.push_back(cur);
+        return out;
+      };
+      try {
+        std::vector<std::string> h1 = fields(next()), h2 = fields(next());
+        if (h1.size() < 5 || h2.size() < 4) throw std::runtime_error("bad PETSc header");
+        p.nrow = std::stoi(h1[3]);
+        p.ncol = std::stoi(h1[4]);
+        p.nnz = std::stoi(h2[3]);
+        next();
+        next();
+        for (int k = 0; k < p.nnz; ++k) {
+          std::vector<std::string> t = fields(next());
+          if (t.size() < 3) throw std::runtime_error("bad matrix entry line");
+          p.j.push_back(std::stoi(t[0]));  // the file lists (j, i, value)
+          p.i.push_back(std::stoi(t[1]));
+          std::string v = t[2];
+          for (char &ch : v) if (ch == 'd' || ch == 'D') ch = 'e';
+          p.v.push_back(std::stod(v));
+        }
+      } catch (...) {
+        std::fclose(f);
+        throw;
+      }
+      std::fclose(f);
+      return p;
+    };
+    const Part re = read_part(real_path), im = read_part(imag_path);
+    if (re.nrow != im.nrow) throw std::runtime_error("nrow Error");
+    if (re.nnz != im.nnz) throw std::runtime_error("nnz Error");
+    if (re.nnz == 0) throw std::runtime_error("nnz Error");
+    for (int k = 0; k < re.nnz; ++k)
+      if (re.i[k] != im.i[k] || re.j[k] != im.j[k]) throw std::runtime_error("Index Error");
+    // coo2csr: row-major order, entries of a row in file order
+    const int n = re.nrow;
+    std::vector<int> ia(n + 1, 0), ja(re.nnz);
+    std::vector<double2> a(re.nnz);
+    for (int k = 0; k < re.nnz; ++k) {
+      if (re.i[k] < 1 || re.i[k] > n || re.j[k] < 1 || re.j[k] > n)
+        throw std::runtime_error("matrix index out of range");
+      ++ia[re.i[k]];
+    }
+    for (int r = 0; r < n; ++r) ia[r + 1] += ia[r];
+    std::vector<int> fill(ia.begin(), ia.end() - 1);
+    for (int k = 0; k < re.nnz; ++k) {
+      const int pos = fill[re.i[k] - 1]++;
+      ja[pos] = re.j[k] - 1;
+      a[pos] = make_double2(re.v[k], im.v[k]);
+    }
+    // create_HS(negf, 1): one k slot
+    if (s.H.size() != 1) {
+      s.H.assign(1, Csr());
+      s.S.assign(1, Csr());
+      s.s_id.assign(1, 0);
+    }
+    set_mat(s, target_matrix == 0, n, a.data(), ja.data(), ia.data(), 1);
+  });
+}
+
+// set_outpath (libnegf.F90:943-949)
+void negf_set_output(int *handler, const char *out_path) {
+  api(handler, [&](Session &s) { s.out_path = cstr(out_path) + "/"; });
+}
+
+// write_tunneling_and_dos (libnegf.F90:2041-2099): tunneling_<kpoint>_<rank>.dat
+// and localDOS_<kpoint>_<rank>.dat in the output path, Fortran E17.8 fields
+void negf_write_tunneling_and_dos(int *handler) {
+  api(handler, [&](Session &s) {
+    auto e17 = [](double v) {
+      // E17.8: 0.dddddddd mantissa, 2-digit exponent (3 digits without the E)
+      char b[64], out[64];
+      if (v == 0.0 || !std::isfinite(v)) {
+        if (v == 0.0) std::snprintf(out, sizeof(out), "%17s", "0.00000000E+00");
+        else std::snprintf(out, sizeof(out), "%17s", std::isnan(v) ? "NaN" : (v > 0 ? "Infinity" : "-Infinity"));
+        return std::string(out);
+      }
+      std::snprintf(b, sizeof(b), "%.7e", std::fabs(v));  // d.ddddddde+XX
+      const int ex = std::atoi(b + 10) + 1;
+      char mant[16] = {'0', '.', b[0], b[2], b[3], b[4], b[5], b[6], b[7], b[8], 0};
+      char body[48];
+      if (std::abs(ex) <= 99) std::snprintf(body, sizeof(body), "%s%sE%c%02d", v < 0 ? "-" : "", mant, ex < 0 ? '-' : '+', std::abs(ex));
+      else std::snprintf(body, sizeof(body), "%s%s%c%03d", v < 0 ? "-" : "", mant, ex < 0 ? '-' : '+', std::abs(ex));
+      std::snprintf(out, sizeof(out), "%17s", body);
+      return std::string(out);
+    };
+    char kp[16];
+    std::snprintf(kp, sizeof(kp), "%06d", s.p.kpoint);
+    const std::string tag = std::string(kp) + "_000000.dat";
+    if (!s.tunn.empty()) {
+      const std::string fn = s.out_path + "tunneling_" + tag;
+      FILE *f = std::fopen(fn.c_str(), "w");
+      if (!f) throw std::runtime_error("cannot open " + fn);
+      const int n = s.npoints;
+      for (int i = 0; i < n; ++i) {
+        const double E = s.p.emin + s.p.estep * i;
+        std::string l = e17(E * s.p.eneconv);
+        for (int ip = 0; ip < s.npairs; ++ip) l += e17(s.tunn[(size_t)ip * n + i]);
+        std::fprintf(f, "%s\n", l.c_str());
+      }
+      std::fclose(f);
+    }
+    if (!s.ldos.empty() && s.ndos > 0) {
+      const std::string fn = s.out_path + "localDOS_" + tag;
+      FILE *f = std::fopen(fn.c_str(), "w");
+      if (!f) throw std::runtime_error("cannot open " + fn);
+      const int n = s.npoints;
+      for (int i = 0; i < n; ++i) {
+        const double E = s.p.emin + s.p.estep * i;
+        std::string l = e17(E * s.p.eneconv);
+        for (int l2 = 0; l2 < s.ldos_cols; ++l2) l += e17(s.ldos[(size_t)l2 * n + i] / s.p.eneconv);
+        std::fprintf(f, "%s\n", l.c_str());
+      }
+      std::fclose(f);
+    }
+  });
+}
+
+// compute_density_dft (libnegf.F90:1664-1679)
+void negf_solve_density_dft(int *handler) {
+  api(handler, [](Session &s) { compute_density(s); });
+}
+
+// rho as 1-based CSR (libnegf_api.f90:836-861)
+void negf_get_dm(int *handler, int *nnz, int *nrow, int *rowpnt, int *colind, double *re_nzval,
+                 double *im_nzval, int copy) {
+  api(handler, [&](Session &s) {
+    *nnz = (int)s.rho_col.size();
+    *nrow = s.rho_n;
+    if (!copy) return;
+    for (int r = 0; r <= s.rho_n; ++r) rowpnt[r] = s.rho_ptr[r] + 1;
+    for (size_t k = 0; k < s.rho_col.size(); ++k) {
+      colind[k] = s.rho_col[k] + 1;
+      re_nzval[k] = s.rho_val[k].x;
+      im_nzval[k] = s.rho_val[k].y;
+    }
+  });
+}
+
+// negf_set_outer (libnegf_api.f90:1052-1065)
+void negf_set_outer(int *handler, int *outer) {
+  api(handler, [&](Session &s) {
+    if (*outer < 0 || *outer > 2) throw std::runtime_error("outer must be 0, 1 or 2");
+    s.outer = *outer;
   });
 }
 

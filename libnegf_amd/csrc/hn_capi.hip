@@ -721,6 +721,59 @@ int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious
   });
 }
 
+int hipnegf_density_layout(hipnegf_struct st, long long *offsets) {
+  return guard([&] {
+    const std::vector<long long> lay = hn::Engine::density_layout(*st->st);
+    for (size_t i = 0; i < lay.size(); ++i) offsets[i] = lay[i];
+    return 0;
+  });
+}
+
+int hipnegf_density_accumulate(hipnegf_engine e, hipnegf_struct st, int nitems, const double *Er,
+                               const double *Ei, const int *slot, const hipnegf_opts *o, int mode,
+                               const double *w, const double *frm, int ref, int outer,
+                               double *acc, int *status) {
+  return guard([&] {
+    if (nitems <= 0) return 0;
+    hn::DevStruct &S = *st->st;
+    for (int i = 0; i < nitems; ++i)
+      if (slot[i] < 0 || slot[i] >= S.nslots) throw std::runtime_error("item slot out of range");
+    if (mode != 0 && mode != 1) throw std::runtime_error("mode must be 0 (contour) or 1 (real axis)");
+    if (outer < 0 || outer > 2) throw std::runtime_error("outer must be 0, 1 or 2");
+    if (mode == 0 && !Ei) throw std::runtime_error("contour points need imaginary parts");
+    if (mode == 1 && (!frm || ref < 1 || ref > S.ncont + 1))
+      throw std::runtime_error("real-axis points need occupations and a reference contact");
+    const long long tot = hn::Engine::density_layout(S).back();
+    const int nf = S.ncont + 1;
+    void *p[6] = {};
+    struct F { void **p; ~F() { for (int i = 0; i < 6; ++i) if (p[i]) (void)hipFree(p[i]); } } fr{p};
+    HN_CHECK(hipMalloc(&p[0], sizeof(double) * nitems * 2));
+    HN_CHECK(hipMalloc(&p[1], sizeof(int) * nitems * 2));
+    HN_CHECK(hipMalloc(&p[2], sizeof(double) * nitems * 2));
+    HN_CHECK(hipMalloc(&p[3], sizeof(double) * std::max(nitems * nf, 1)));
+    HN_CHECK(hipMalloc(&p[4], sizeof(double) * 2 * std::max<long long>(tot, 1)));
+    double *dE = (double *)p[0], *dEi = dE + nitems;
+    int *dsl = (int *)p[1], *dst = dsl + nitems;
+    HN_CHECK(hipMemcpy(dE, Er, sizeof(double) * nitems, hipMemcpyHostToDevice));
+    if (mode == 0) HN_CHECK(hipMemcpy(dEi, Ei, sizeof(double) * nitems, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(dsl, slot, sizeof(int) * nitems, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(p[2], w, sizeof(double) * 2 * nitems, hipMemcpyHostToDevice));
+    if (mode == 1) HN_CHECK(hipMemcpy(p[3], frm, sizeof(double) * nitems * nf, hipMemcpyHostToDevice));
+    HN_CHECK(hipMemcpy(p[4], acc, sizeof(double) * 2 * tot, hipMemcpyHostToDevice));
+    hn::DensityOpts d;
+    d.mode = mode;
+    d.w = (const double2 *)p[2];
+    d.frm = mode == 1 ? (const double *)p[3] : nullptr;
+    d.ref = ref;
+    d.outer = outer;
+    d.status = dst;
+    e->eng->density(S, nitems, dE, mode == 0 ? dEi : nullptr, dsl, to_opts(o), d, (double2 *)p[4]);
+    HN_CHECK(hipMemcpy(acc, p[4], sizeof(double) * 2 * tot, hipMemcpyDeviceToHost));
+    if (status) HN_CHECK(hipMemcpy(status, dst, sizeof(int) * nitems, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
 int hipnegf_struct_set_tun_proj(hipnegf_struct st, int n, const int *idx) {
   return guard([&] {
     if (n > 0 && !idx) throw std::runtime_error("tun_proj indexes missing");

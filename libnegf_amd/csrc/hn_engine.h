@@ -112,6 +112,9 @@ struct SolveOpts {
   // surface Green's function cache (surface_green + ln_cache, contselfenergy.F90:138-173,
   // 251-259): 0 compute only; 1 load if cached, else compute and store; 2 compute and store
   int sgf_mode = 0;
+  // device [nitems] imaginary parts added to delta on the contacts (complex
+  // energies E + i (delta + Ei), e.g. contour points); null: none
+  const double *dEi = nullptr;
 };
 
 // Local DOS alongside the transmission (calculate_transmissions_and_dos,
@@ -122,6 +125,18 @@ struct DosOpts {
   int full = 0;               // one output per device orbital instead of projections
   const int *ptr = nullptr;   // device [nldos+1]: list l is idx[ptr[l] .. ptr[l+1]-1]
   const int *idx = nullptr;   // device, 1-based device orbitals
+};
+
+// Density-matrix integrand (contour_int / real_axis_int, integrations.F90:844-1176):
+// acc += sum over items of w_b X_b for the blocks of one integration path.
+struct DensityOpts {
+  int mode = 0;                 // 0: G^r, contacts and device at E + i Ei (contour points)
+                                // 1: G^n, contacts at E + i delta, device at E (real axis)
+  const double2 *w = nullptr;   // device [nitems] weights
+  const double *frm = nullptr;  // mode 1: device [nitems * (ncont+1)] occupations
+  int ref = 0;                  // mode 1: 1-based reference contact (ncont + 1: none)
+  int outer = 2;                // 0: device blocks, 1: + device/contact, 2: + contact/device
+  int *status = nullptr;        // optional device [nitems]: per-item status words
 };
 
 struct SCBAOpts {
@@ -165,6 +180,15 @@ class Engine {
                         const SolveOpts &o, int npairs, const int *ni, const int *nf,
                         const DosOpts &d, double *dT, double *dldos, int *dncyc, int *dstatus);
   static size_t dos_item_bytes(const DevStruct &st, int npairs, const int *ni, const int *nf);
+
+  // Density-matrix integrand for nitems points, summed into acc (device,
+  // packed column-major blocks in the order of density_layout: Gd[nbl],
+  // Gu[nbl-1] = (i,i+1), Gl[nbl-1] = (i+1,i), Glc[ncont] (nd x nc),
+  // Gcl[ncont] (nc x nd)).  dEi: mode 0 imaginary parts (device).
+  void density(DevStruct &st, int nitems, const double *dE, const double *dEi, const int *dslot,
+               const SolveOpts &o, const DensityOpts &d, double2 *acc);
+  // offsets of the blocks in the accumulator; back() = total elements
+  static std::vector<long long> density_layout(const DevStruct &st);
 
   // Elastic-dephasing SCBA + Meir-Wingreen current per item.
   // curr[nitems*ncont], iters[nitems], err[nitems], status[nitems];
@@ -232,8 +256,10 @@ class Engine {
   void clear_graphs();
 
   int pick_batch(const DevStruct &st, int nitems, const SolveOpts &o, size_t per_item) const;
+  // GSout (optional): the surface Green's functions, contact-major, stride sGS
   void contacts(DevStruct &st, int B, const double *dE, const int *dslot,
-                const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c, int *status_c);
+                const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c, int *status_c,
+                double2 *GSout = nullptr, long long sGS = 0);
   void decimate_z(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, double2 *Co,
                   double2 *Go, const SolveOpts &o, int *ncyc, int *status);
   bool mirror_contacts(DevStruct &st);
@@ -253,6 +279,9 @@ class Engine {
   void t_multi(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
                int npairs, const int *ni, const int *nf, const double2 *Sig, long long sSig,
                double *dT, int *dstatus);
+  void density_batch(DevStruct &st, int B, const double *dE, const double *dEi, const int *dslot,
+                     const SolveOpts &o, const DensityOpts &d, const double2 *w, const double *frm,
+                     double2 *acc, const std::vector<long long> &lay);
   void dos_batch(DevStruct &st, int B, const double *dE, const int *dslot, const SolveOpts &o,
                  int npairs, const int *ni, const int *nf, const DosOpts &d, double *dT,
                  double *dldos, int *dncyc, int *dstatus);

@@ -513,18 +513,19 @@ int Engine::sgf_slot_for(const SgfKey &k) {
 // Sig: ncont x B blocks of nd x nd (nd = bsz[cblk-1]), stride sSig, contact-major.
 void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
                       const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c,
-                      int *status_c) {
+                      int *status_c, double2 *GSout, long long sGS) {
   const bool sc = (st.smode & S_CONTACT) != 0;
   const int ncont = st.ncont;
   auto self_energy = [&](int c, const double2 *GS) {
     // SelfEnergy: Tlc = z SMC - HMC, Tp = z* SMC - HMC, Sigma = Tlc GS Tp^H
     const int nc = st.nc[c], nd = st.bsz[st.cblk[c] - 1];
     const long long s2 = (long long)nc * nc, sm = (long long)nd * nc;
+    if (GSout) copy_batched(GSout + (long long)c * B * sGS, sGS, GS, s2, s2, B, nullptr, stream_);
     size_t mark = arena_.used;
     double2 *Tlc = arena_.take<double2>(sm * B), *Tp = arena_.take<double2>(sm * B);
     double2 *TG = arena_.take<double2>(sm * B);
     FormArgs t;
-    t.slot = dslot; t.E = dE; t.eta = o.delta; t.batch = B; t.rows = nd; t.cols = nc;
+    t.slot = dslot; t.E = dE; t.eta = o.delta; t.Ei = o.dEi; t.batch = B; t.rows = nd; t.cols = nc;
     t.H = st.ref(BK_HMC, c); t.S = sc ? st.ref(BK_SMC, c) : BlkRef(); t.s_identity = 0;
     t.out = Tlc; t.s_out = sm;
     form_block(t, stream_);
@@ -545,7 +546,8 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
   int ncmax = 0;
   for (int c = 0; c < ncont; ++c)
     if (!st.fict[c]) ncmax = std::max(ncmax, st.nc[c]);
-  const bool cache = o.sgf_mode != 0 && ncmax > 0 && sgf_prepare((long long)ncmax * ncmax);
+  // (complex per-item energies are not cached: keys hold the real energy only)
+  const bool cache = o.sgf_mode != 0 && !o.dEi && ncmax > 0 && sgf_prepare((long long)ncmax * ncmax);
   std::vector<double> hE;
   std::vector<int> hS;
   int *didx = nullptr;
@@ -628,7 +630,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     double2 *Co = arena_.take<double2>(s2 * B), *Go0 = arena_.take<double2>(s2 * B);
     double2 *Go1 = arena_.take<double2>(s2 * B);
     FormArgs f;
-    f.slot = dslot; f.E = dE; f.eta = o.delta; f.batch = B; f.rows = nc; f.cols = nc;
+    f.slot = dslot; f.E = dE; f.eta = o.delta; f.Ei = o.dEi; f.batch = B; f.rows = nc; f.cols = nc;
     f.H = st.ref(BK_H00, 0); f.S = sc ? st.ref(BK_S00, 0) : BlkRef(); f.s_identity = !sc;
     f.out = Ao; f.s_out = s2; f.out2 = Aos0; f.s_out2 = s2;
     form_block(f, stream_);
@@ -659,7 +661,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
       double2 *Bo = arena_.take<double2>(s2 * B), *Co = arena_.take<double2>(s2 * B);
       // surface_green: Ao = z S00 - H00 (also Ao_s), Bo = z S01 - H01, Co = (z* S01 - H01)^H
       FormArgs f;
-      f.slot = dslot; f.E = dE; f.eta = o.delta; f.batch = B; f.rows = nc; f.cols = nc;
+      f.slot = dslot; f.E = dE; f.eta = o.delta; f.Ei = o.dEi; f.batch = B; f.rows = nc; f.cols = nc;
       f.H = st.ref(BK_H00, c); f.S = sc ? st.ref(BK_S00, c) : BlkRef(); f.s_identity = !sc;
       f.out = Ao; f.s_out = s2; f.out2 = Aos; f.s_out2 = s2;
       form_block(f, stream_);
@@ -1383,6 +1385,318 @@ void Engine::transmission_dos(DevStruct &st, int nitems, const double *dE, const
     dos_batch(st, nb, dE + i0, dslot + i0, o, npairs, ni, nf, d, dT + (long long)i0 * npairs,
               dldos ? dldos + i0 * nout : nullptr, dncyc ? dncyc + (long long)i0 * st.ncont : nullptr,
               dstatus + i0);
+  }
+  HN_CHECK(hipStreamSynchronize(stream_));
+}
+
+// ---------------------------------------------------------------------------
+// Density-matrix integrands.
+// Mode 0 (contour point z = E + i Ei): compute_Gr / calculate_Gr
+// (integrations.F90:2287-2340, iterative.F90:90-148): contacts at z itself,
+// ESH at z, gsmr blocks, Gr(1,1) and the Gr tridiagonal blocks through two
+// rolling buffers, each block summed into the accumulator as soon as it
+// exists; the outer blocks of calculate_Gr_outer (iterative.F90:1463-1573):
+// Glc = -Gr(cb,cb) Tlc GS, Gcl = -GS Tcl Gr(cb,cb), Tcl = (conj(z) SMC - HMC)^H.
+// Mode 1 (real-axis point E): calculate_Gn_neq_components
+// (iterative.F90:159-279): contacts at E + i delta, ESH at E,
+// Sigma_n(cb,cb) = sum_{j != ref, |f_j - f_ref| > EPS} (f_j - f_ref) Gam_j with
+// per-item occupations, the Sigma_n fold and the Gn tridiagonal recursion of
+// calculate_Gn_tridiag_blocks (iterative_cpu.F90:230-372; the Sigma_n
+// off-diagonal blocks are zero without interactions), then calculate_Gn_outer
+// (iterative.F90:1576-1665): Glc = -Gn(cb,cb) Tlc GS^H
+// + (f_ref - f_k) Gr(cb,cb) Tlc i(GS - GS^H), zero where Tlc is, Gcl = Glc^H.
+// The reference adds each point's blocks, masked to the S pattern, into a
+// CSR sum in point order; here every accumulator element sums the items of a
+// batch in item order, and batches run in order.
+// ---------------------------------------------------------------------------
+std::vector<long long> Engine::density_layout(const DevStruct &st) {
+  std::vector<long long> off;
+  long long o = 0;
+  auto put = [&](long long n) { off.push_back(o); o += n; };
+  for (int i = 0; i < st.nbl; ++i) put((long long)st.bsz[i] * st.bsz[i]);
+  for (int i = 0; i + 1 < st.nbl; ++i) put((long long)st.bsz[i] * st.bsz[i + 1]);
+  for (int i = 0; i + 1 < st.nbl; ++i) put((long long)st.bsz[i] * st.bsz[i + 1]);
+  for (int c = 0; c < st.ncont; ++c) put((long long)st.bsz[st.cblk[c] - 1] * st.nc[c]);
+  for (int c = 0; c < st.ncont; ++c) put((long long)st.bsz[st.cblk[c] - 1] * st.nc[c]);
+  off.push_back(o);
+  return off;
+}
+
+void Engine::density_batch(DevStruct &st, int B, const double *dE, const double *dEi,
+                           const int *dslot, const SolveOpts &o0, const DensityOpts &d,
+                           const double2 *w, const double *frm, double2 *acc,
+                           const std::vector<long long> &lay) {
+  const int nbl = st.nbl, ncont = st.ncont;
+  const int nm = st.nmax();
+  const long long s2 = (long long)nm * nm, SB = s2 * B;
+  const int ncs = std::max(ncont, 1);
+  const bool gn = d.mode == 1;
+  SolveOpts o = o0;
+  if (!gn) {  // contour: the contacts at the point itself
+    o.delta = 0.0;
+    o.dEi = dEi;
+  } else {
+    o.dEi = nullptr;
+  }
+  const double *Ei_dev = gn ? nullptr : dEi;
+  arena_.reset();
+  double2 *Sig = arena_.take<double2>(SB * ncs), *GS = arena_.take<double2>(SB * ncs);
+  int *ncyc_c = arena_.take<int>((size_t)B * ncs), *status_c = arena_.take<int>((size_t)B * ncs);
+  int *dstatus = arena_.take<int>(B);
+  fill_int(ncyc_c, B * ncs, 0, stream_);
+  fill_int(status_c, B * ncs, 0, stream_);
+  fill_int(dstatus, B, 0, stream_);
+  if (ncont > 0) contacts(st, B, dE, dslot, o, Sig, s2, ncyc_c, status_c, GS, s2);
+
+  std::vector<std::vector<int>> on_blk(nbl);
+  for (int c = 0; c < ncont; ++c) on_blk[st.cblk[c] - 1].push_back(c);
+  std::vector<char> cblock(nbl, 0);
+  for (int c = 0; c < ncont; ++c) cblock[st.cblk[c] - 1] = 1;
+  const bool sd = (st.smode & S_DIAG) != 0, sof = (st.smode & S_OFFDIAG) != 0;
+  const bool sc = (st.smode & S_CONTACT) != 0;
+  auto blk = [&](double2 *base, int i) { return base + (long long)i * SB; };
+  auto bs = [&](int i) { return st.bsz[i]; };
+  const double2 one = Z(1), zero = Z(0), mone = Z(-1);
+  const int iu = nbl, il = 2 * nbl - 1, ilc = 3 * nbl - 2, icl = 3 * nbl - 2 + ncont;
+  auto add = [&](int idx, const double2 *X) {
+    accumulate(acc + lay[idx], X, s2, lay[idx + 1] - lay[idx], w, B, stream_);
+  };
+  auto form_diag = [&](int i, double2 *out) {
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.Ei = Ei_dev; f.batch = B;
+    f.rows = bs(i); f.cols = bs(i);
+    f.H = st.ref(BK_HD, i); f.S = sd ? st.ref(BK_SD, i) : BlkRef(); f.s_identity = !sd;
+    f.out = out; f.s_out = s2;
+    const auto &cs = on_blk[i];
+    if (cs.size() > 2) throw std::runtime_error("more than two contacts on one block");
+    if (cs.size() >= 1) { f.sig = Sig + cs[0] * SB; f.s_sig = s2; }
+    if (cs.size() == 2) { f.sig2 = Sig + cs[1] * SB; f.s_sig2 = s2; }
+    form_block(f, stream_);
+  };
+  auto form_off = [&](int i, double2 *u, double2 *l) {  // u = ESH(i,i+1), l = ESH(i+1,i)
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = 0.0; f.Ei = Ei_dev; f.batch = B;
+    f.s_identity = 0; f.s_out = s2;
+    if (u) {
+      f.rows = bs(i); f.cols = bs(i + 1);
+      f.H = st.ref(BK_HU, i); f.S = sof ? st.ref(BK_SU, i) : BlkRef(); f.out = u;
+      form_block(f, stream_);
+    }
+    if (l) {
+      f.rows = bs(i + 1); f.cols = bs(i);
+      f.H = st.ref(BK_HL, i); f.S = sof ? st.ref(BK_SL, i) : BlkRef(); f.out = l;
+      form_block(f, stream_);
+    }
+  };
+  auto form_tlc = [&](int k, double2 *out, int conjz) {  // z SMC - HMC at the contacts' z
+    FormArgs t;
+    t.slot = dslot; t.E = dE; t.eta = o.delta; t.Ei = o.dEi; t.batch = B;
+    t.rows = bs(st.cblk[k] - 1); t.cols = st.nc[k];
+    t.H = st.ref(BK_HMC, k); t.S = sc ? st.ref(BK_SMC, k) : BlkRef(); t.s_identity = 0;
+    t.conjz = conjz; t.out = out; t.s_out = s2;
+    form_block(t, stream_);
+  };
+
+  double2 *g = arena_.take<double2>(SB * nbl);
+  double2 *D = arena_.take<double2>(SB), *U = arena_.take<double2>(SB), *L = arena_.take<double2>(SB);
+  double2 *W1 = arena_.take<double2>(SB), *W2 = arena_.take<double2>(SB);
+  double2 *Ca = arena_.take<double2>(SB), *Cb = arena_.take<double2>(SB);
+  // mode 0: Gr diagonal blocks rolling, contact blocks kept; mode 1: all Gr(i,i), Gr(i,i+1)
+  double2 *Grd = gn ? arena_.take<double2>(SB * nbl) : nullptr;
+  double2 *Gru = gn ? arena_.take<double2>(SB * std::max(nbl - 1, 1)) : nullptr;
+  double2 *R0 = gn ? nullptr : arena_.take<double2>(SB), *R1 = gn ? nullptr : arena_.take<double2>(SB);
+  std::vector<double2 *> keep(nbl, nullptr);
+  for (int i = 0; i < nbl; ++i)
+    if (cblock[i] && d.outer > 0) keep[i] = arena_.take<double2>(SB);
+
+  // ---- gsmr blocks (nbl -> 2) ----
+  if (nbl > 1) {
+    form_diag(nbl - 1, D);
+    inverse(bs(nbl - 1), D, s2, blk(g, nbl - 1), s2, B, nullptr, dstatus, PREC_FP64, o);
+    for (int i = nbl - 2; i >= 1; --i) {
+      const int a = bs(i), c = bs(i + 1);
+      form_off(i, U, L);
+      form_diag(i, D);
+      gemm_z(G(a, c, c, U, s2, a, OP_N, blk(g, i + 1), s2, c, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+      gemm_z(G(a, a, c, W1, s2, a, OP_N, L, s2, c, OP_N, D, s2, a, one, one, B, nullptr), stream_);
+      inverse(a, D, s2, blk(g, i), s2, B, nullptr, dstatus, PREC_FP64, o);
+    }
+  }
+  // ---- Gr(1,1) and the tridiagonal blocks ----
+  double2 *cur = gn ? Grd : R0, *nxt = gn ? nullptr : R1;
+  {
+    const int a = bs(0);
+    form_diag(0, D);
+    if (nbl > 1) {
+      const int c = bs(1);
+      form_off(0, U, L);
+      gemm_z(G(a, c, c, U, s2, a, OP_N, blk(g, 1), s2, c, OP_N, W1, s2, a, mone, zero, B, nullptr), stream_);
+      gemm_z(G(a, a, c, W1, s2, a, OP_N, L, s2, c, OP_N, D, s2, a, one, one, B, nullptr), stream_);
+    }
+    inverse(a, D, s2, cur, s2, B, nullptr, dstatus, PREC_FP64, o);
+    if (!gn) add(0, cur);
+    if (!gn && keep[0]) copy_batched(keep[0], s2, cur, s2, (long long)a * a, B, nullptr, stream_);
+  }
+  for (int i = 1; i < nbl; ++i) {
+    const int ni_ = bs(i), nmm = bs(i - 1);
+    double2 *Gl = Ca, *Gu = gn ? blk(Gru, i - 1) : Cb;
+    double2 *dst = gn ? blk(Grd, i) : nxt;
+    form_off(i - 1, U, L);  // U = ESH(i-1,i) nmm x ni, L = ESH(i,i-1) ni x nmm
+    gemm_z(G(ni_, nmm, ni_, blk(g, i), s2, ni_, OP_N, L, s2, ni_, OP_N, W1, s2, ni_, one, zero, B, nullptr), stream_);
+    gemm_z(G(ni_, nmm, nmm, W1, s2, ni_, OP_N, cur, s2, nmm, OP_N, Gl, s2, ni_, mone, zero, B, nullptr), stream_);
+    gemm_z(G(nmm, ni_, ni_, U, s2, nmm, OP_N, blk(g, i), s2, ni_, OP_N, W2, s2, nmm, one, zero, B, nullptr), stream_);
+    gemm_z(G(nmm, ni_, nmm, cur, s2, nmm, OP_N, W2, s2, nmm, OP_N, Gu, s2, nmm, mone, zero, B, nullptr), stream_);
+    copy_batched(dst, s2, blk(g, i), s2, (long long)ni_ * ni_, B, nullptr, stream_);
+    gemm_z(G(ni_, ni_, nmm, Gl, s2, ni_, OP_N, W2, s2, nmm, OP_N, dst, s2, ni_, mone, one, B, nullptr), stream_);
+    if (!gn) {
+      add(il + i - 1, Gl);
+      add(iu + i - 1, Gu);
+      add(i, dst);
+      if (keep[i]) copy_batched(keep[i], s2, dst, s2, (long long)ni_ * ni_, B, nullptr, stream_);
+      std::swap(cur, nxt);
+    } else {
+      cur = dst;
+    }
+  }
+  if (!gn) {
+    // ---- calculate_Gr_outer ----
+    if (d.outer > 0) {
+      for (int k = 0; k < ncont; ++k) {
+        const int cb = st.cblk[k] - 1, nd = bs(cb), nc = st.nc[k];
+        double2 *GSk = GS + (long long)k * SB;
+        form_tlc(k, U, 0);  // Tlc
+        gemm_z(G(nd, nc, nc, U, s2, nd, OP_N, GSk, s2, nc, OP_N, W1, s2, nd, one, zero, B, nullptr), stream_);
+        gemm_z(G(nd, nc, nd, keep[cb], s2, nd, OP_N, W1, s2, nd, OP_N, W2, s2, nd, mone, zero, B, nullptr), stream_);
+        add(ilc + k, W2);
+        if (d.outer == 2) {
+          form_tlc(k, L, 1);  // conj(z) SMC - HMC; Tcl = its conjugate transpose
+          gemm_z(G(nc, nd, nc, GSk, s2, nc, OP_N, L, s2, nd, OP_C, W1, s2, nc, one, zero, B, nullptr), stream_);
+          gemm_z(G(nc, nd, nd, W1, s2, nc, OP_N, keep[cb], s2, nd, OP_N, W2, s2, nc, mone, zero, B, nullptr), stream_);
+          add(icl + k, W2);
+        }
+      }
+    }
+    finish_items(B, ncont, ncyc_c, status_c, nullptr, dstatus);
+    if (d.status) copy_int(d.status, dstatus, B, stream_);
+    return;
+  }
+
+  // ---- mode 1: Sigma_n, its fold and the Gn recursion ----
+  double2 *Gam = arena_.take<double2>(SB * ncs);
+  for (int c = 0; c < ncont; ++c)
+    gamma_batched(Sig + c * SB, s2, Gam + c * SB, s2, bs(st.cblk[c] - 1), B, stream_);
+  double2 *Sn = arena_.take<double2>(SB * nbl);
+  double2 *gns = arena_.take<double2>(SB), *N0 = arena_.take<double2>(SB), *N1 = arena_.take<double2>(SB);
+  double2 *Gnu = arena_.take<double2>(SB);
+  std::vector<double2 *> keep_n(nbl, nullptr);
+  for (int i = 0; i < nbl; ++i)
+    if (cblock[i] && d.outer > 0) keep_n[i] = arena_.take<double2>(SB);
+  for (int i = 0; i < nbl; ++i) {
+    SnArgs a;
+    a.Sn = blk(Sn, i); a.sSn = s2; a.n = bs(i);
+    a.sG = s2;
+    for (int c : on_blk[i]) {
+      a.G[a.nq] = Gam + c * SB;
+      a.j[a.nq] = c;
+      ++a.nq;
+    }
+    a.frm = frm; a.frm_stride = ncont + 1; a.ref = d.ref; a.eps = o.eps_drop;
+    a.batch = B;
+    sn_form(a, stream_);
+  }
+  if (nbl > 1) {  // calculate_sigma_n: upward fold
+    const int a = bs(nbl - 1);
+    gemm_z(G(a, a, a, blk(g, nbl - 1), s2, a, OP_N, blk(Sn, nbl - 1), s2, a, OP_N, W1, s2, a, one, zero, B, nullptr), stream_);
+    gemm_z(G(a, a, a, W1, s2, a, OP_N, blk(g, nbl - 1), s2, a, OP_C, gns, s2, a, one, zero, B, nullptr), stream_);
+    for (int i = nbl - 2; i >= 0; --i) {
+      const int ni_ = bs(i), nj = bs(i + 1);
+      form_off(i, U, nullptr);  // U = ESH(i,i+1): ni x nj
+      gemm_z(G(ni_, nj, nj, U, s2, ni_, OP_N, gns, s2, nj, OP_N, W1, s2, ni_, one, zero, B, nullptr), stream_);
+      gemm_z(G(ni_, ni_, nj, W1, s2, ni_, OP_N, U, s2, ni_, OP_C, blk(Sn, i), s2, ni_, one, one, B, nullptr), stream_);
+      if (i > 0) {
+        gemm_z(G(ni_, ni_, ni_, blk(g, i), s2, ni_, OP_N, blk(Sn, i), s2, ni_, OP_N, W1, s2, ni_, one, zero, B, nullptr), stream_);
+        gemm_z(G(ni_, ni_, ni_, W1, s2, ni_, OP_N, blk(g, i), s2, ni_, OP_C, gns, s2, ni_, one, zero, B, nullptr), stream_);
+      }
+    }
+  }
+  double2 *gcur = N0, *gnxt = N1;
+  {  // Gn(1,1) = Gr(1,1) Sigma_n(1,1) Gr(1,1)^H
+    const int a = bs(0);
+    gemm_z(G(a, a, a, blk(Sn, 0), s2, a, OP_N, Grd, s2, a, OP_C, W1, s2, a, one, zero, B, nullptr), stream_);
+    gemm_z(G(a, a, a, Grd, s2, a, OP_N, W1, s2, a, OP_N, gcur, s2, a, one, zero, B, nullptr), stream_);
+    add(0, gcur);
+    if (keep_n[0]) copy_batched(keep_n[0], s2, gcur, s2, (long long)a * a, B, nullptr, stream_);
+  }
+  for (int i = 0; i + 1 < nbl; ++i) {
+    const int ni_ = bs(i), nj = bs(i + 1);
+    form_off(i, nullptr, L);  // L = ESH(i+1,i): nj x ni
+    // Gn(i+1,i) = g(i+1) [Sn(i+1) Gr(i,i+1)^H - ESH(i+1,i) Gn(i,i)]
+    gemm_z(G(nj, ni_, nj, blk(Sn, i + 1), s2, nj, OP_N, blk(Gru, i), s2, ni_, OP_C, W1, s2, nj, one, zero, B, nullptr), stream_);
+    gemm_z(G(nj, ni_, ni_, L, s2, nj, OP_N, gcur, s2, ni_, OP_N, W1, s2, nj, mone, one, B, nullptr), stream_);
+    gemm_z(G(nj, ni_, nj, blk(g, i + 1), s2, nj, OP_N, W1, s2, nj, OP_N, Ca, s2, nj, one, zero, B, nullptr), stream_);
+    dagger_batched(Ca, s2, Gnu, s2, nj, ni_, B, nullptr, stream_);  // Gn(i,i+1) = Gn(i+1,i)^H
+    add(il + i, Ca);
+    add(iu + i, Gnu);
+    // Gn(i+1,i+1) = g(i+1) [Sn(i+1) Gr(i+1,i+1)^H - ESH(i+1,i) Gn(i,i+1)]
+    gemm_z(G(nj, nj, nj, blk(Sn, i + 1), s2, nj, OP_N, blk(Grd, i + 1), s2, nj, OP_C, W2, s2, nj, one, zero, B, nullptr), stream_);
+    gemm_z(G(nj, nj, ni_, L, s2, nj, OP_N, Gnu, s2, ni_, OP_N, W2, s2, nj, mone, one, B, nullptr), stream_);
+    gemm_z(G(nj, nj, nj, blk(g, i + 1), s2, nj, OP_N, W2, s2, nj, OP_N, gnxt, s2, nj, one, zero, B, nullptr), stream_);
+    add(i + 1, gnxt);
+    if (keep_n[i + 1]) copy_batched(keep_n[i + 1], s2, gnxt, s2, (long long)nj * nj, B, nullptr, stream_);
+    std::swap(gcur, gnxt);
+  }
+  // ---- calculate_Gn_outer ----
+  if (d.outer > 0) {
+    for (int k = 0; k < ncont; ++k) {
+      const int cb = st.cblk[k] - 1, nd = bs(cb), nc = st.nc[k];
+      double2 *GSk = GS + (long long)k * SB;
+      form_tlc(k, U, 0);  // Tlc at E + i delta
+      gemm_z(G(nd, nc, nc, U, s2, nd, OP_N, GSk, s2, nc, OP_C, W1, s2, nd, one, zero, B, nullptr), stream_);
+      gemm_z(G(nd, nc, nd, keep_n[cb], s2, nd, OP_N, W1, s2, nd, OP_N, W2, s2, nd, mone, zero, B, nullptr), stream_);
+      gamma_batched(GSk, s2, D, s2, nc, B, stream_);  // i (GS - GS^H)
+      gemm_z(G(nd, nc, nc, U, s2, nd, OP_N, D, s2, nc, OP_N, W1, s2, nd, one, zero, B, nullptr), stream_);
+      frm_scale(W1, s2, (long long)nd * nc, frm, ncont + 1, k, d.ref, o.eps_drop, B, stream_);
+      gemm_z(G(nd, nc, nd, blk(Grd, cb), s2, nd, OP_N, W1, s2, nd, OP_N, W2, s2, nd, one, one, B, nullptr), stream_);
+      mask_zero(W2, s2, U, s2, (long long)nd * nc, B, stream_);
+      add(ilc + k, W2);
+      if (d.outer == 2) {
+        dagger_batched(W2, s2, Ca, s2, nd, nc, B, nullptr, stream_);
+        add(icl + k, Ca);
+      }
+    }
+  }
+  finish_items(B, ncont, ncyc_c, status_c, nullptr, dstatus);
+  if (d.status) copy_int(d.status, dstatus, B, stream_);
+}
+
+void Engine::density(DevStruct &st, int nitems, const double *dE, const double *dEi,
+                     const int *dslot, const SolveOpts &o, const DensityOpts &d, double2 *acc) {
+  if (nitems <= 0) return;
+  if (d.mode != 0 && d.mode != 1) throw std::runtime_error("density: bad mode");
+  if (d.mode == 0 && !dEi) throw std::runtime_error("density: contour points need imaginary parts");
+  if (d.mode == 1 && (!d.frm || d.ref < 1 || d.ref > st.ncont + 1))
+    throw std::runtime_error("density: real-axis points need occupations and a reference contact");
+  if (!d.w) throw std::runtime_error("density: weights missing");
+  HN_CHECK(hipSetDevice(device_));
+  const int n = st.nmax();
+  const size_t b2 = (size_t)n * n * sizeof(double2);
+  const size_t nc = (size_t)std::max(st.ncont, 1);
+  // Sigma and GS per contact; then max(contact stage, sweep): g_i, 7 temporaries,
+  // 2 rolling or (mode 1) Gr(i,i), Gr(i,i+1), Sigma_n blocks and Gn temporaries,
+  // kept contact blocks
+  const size_t sweep = d.mode == 0 ? (size_t)st.nbl + 9 + nc : 4 * (size_t)st.nbl + 12 + 3 * nc;
+  const size_t per_item = (2 * nc + std::max<size_t>(10 + 2 * nc, sweep)) * b2 +
+                          inv_work_bytes(n, (int)nc, 8) + 64 * sizeof(int) + 4096;
+  const int B = pick_batch(st, nitems, o, per_item);
+  last_batch_ = B;
+  arena_.reserve(per_item * (size_t)B + inv_work_bytes(n, B, 8) + (1 << 20));
+  const std::vector<long long> lay = density_layout(st);
+  for (int i0 = 0; i0 < nitems; i0 += B) {
+    const int nb = std::min(B, nitems - i0);
+    DensityOpts di = d;
+    if (d.status) di.status = d.status + i0;
+    density_batch(st, nb, dE + i0, dEi ? dEi + i0 : nullptr, dslot + i0, o, di, d.w + i0,
+                  d.frm ? d.frm + (long long)i0 * (st.ncont + 1) : nullptr, acc, lay);
   }
   HN_CHECK(hipStreamSynchronize(stream_));
 }

@@ -24,6 +24,7 @@ struct FormArgs {
   const int *slot = nullptr;
   const double *E = nullptr;
   double eta = 0.0;
+  const double *Ei = nullptr;  // optional per-item imaginary part added to eta
   int conjz = 0;
   int dagger = 0;
   int rows = 0, cols = 0;
@@ -138,6 +139,17 @@ void dagger_batched(const double2 *X, long long sX, double2 *Y, long long sY, in
 // d_b[i] = M_b(i,i) * scale[i] for active items
 void diag_extract_masked(const double2 *M, long long sM, int n, int batch, double2 *d,
                          long long sd, const double *scale, const int *active, hipStream_t st);
+// acc[e] += sum_b w[b] X_b[e] for e < n, items summed in order b = 0..batch-1
+void accumulate(double2 *acc, const double2 *X, long long sX, long long n, const double2 *w,
+                int batch, hipStream_t st);
+// X_b[e] = 0 where M_b[e] == 0 (n entries per item)
+void mask_zero(double2 *X, long long sX, const double2 *M, long long sM, long long n, int batch,
+               hipStream_t st);
+// X_b *= c_b, c_b = f_ref - f_k when k != ref-1 and |f_k - f_ref| > eps, else 0
+// (frm_b = frm + b * frm_stride; k 0-based, ref 1-based)
+void frm_scale(double2 *X, long long sX, long long n, const double *frm, int frm_stride, int k,
+               int ref, double eps, int batch, hipStream_t st);
+
 // Sn_b = diag(v_b) + sum_q w_q(b) G_q,b   (q < nq <= 2; w_q = frm_b[j_q] - frm_b[ref-1],
 // dropped when |w_q| <= eps); frm_b = frm + b * frm_stride
 struct SnArgs {

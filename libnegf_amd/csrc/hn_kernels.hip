@@ -65,7 +65,7 @@ __global__ __launch_bounds__(NT) void k_form(FormArgs a) {
   if (e >= n) return;
   const int r = (int)(e % a.rows), c = (int)(e / a.rows);
   const int sl = a.slot ? a.slot[b] : 0;
-  double2 z = make_double2(a.E ? a.E[b] : 0.0, a.eta);
+  double2 z = make_double2(a.E ? a.E[b] : 0.0, a.Ei ? a.eta + a.Ei[b] : a.eta);
   if (a.conjz) z.y = -z.y;
   // source element (rs, cs) of the (possibly transposed) source block
   const int rs = a.dagger ? c : r, cs = a.dagger ? r : c;
@@ -607,6 +607,65 @@ __global__ __launch_bounds__(NT) void k_dos_diag(const double2 *M, long long sM,
 void dos_diag(const double2 *M, long long sM, int n, int batch, double *d, long long sd,
               hipStream_t st) {
   hipLaunchKernelGGL(k_dos_diag, dim3((n + NT - 1) / NT, batch), dim3(NT), 0, st, M, sM, n, d, sd);
+  HN_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(NT) void k_accumulate(double2 *acc, const double2 *X, long long sX,
+                                                   long long n, const double2 *w, int batch) {
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  double2 a = acc[e];
+  for (int b = 0; b < batch; ++b) {
+    const double2 x = X[(long long)b * sX + e], c = w[b];
+    const double2 p = make_double2(c.x * x.x - c.y * x.y, c.x * x.y + c.y * x.x);
+    a.x += p.x;
+    a.y += p.y;
+  }
+  acc[e] = a;
+}
+
+void accumulate(double2 *acc, const double2 *X, long long sX, long long n, const double2 *w,
+                int batch, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_accumulate, dim3((unsigned)((n + NT - 1) / NT)), dim3(NT), 0, st, acc, X, sX, n,
+                     w, batch);
+  HN_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(NT) void k_mask_zero(double2 *X, long long sX, const double2 *M,
+                                                  long long sM, long long n) {
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  const double2 m = M[(long long)b * sM + e];
+  if (m.x == 0.0 && m.y == 0.0) X[(long long)b * sX + e] = make_double2(0.0, 0.0);
+}
+
+void mask_zero(double2 *X, long long sX, const double2 *M, long long sM, long long n, int batch,
+               hipStream_t st) {
+  if (n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_mask_zero, grid2(n, batch), dim3(NT), 0, st, X, sX, M, sM, n);
+  HN_CHECK(hipGetLastError());
+}
+
+__global__ __launch_bounds__(NT) void k_frm_scale(double2 *X, long long sX, long long n,
+                                                  const double *frm, int frm_stride, int k, int ref,
+                                                  double eps) {
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= n) return;
+  const double *f = frm + (long long)b * frm_stride;
+  const double fr = f[ref - 1], fk = f[k];
+  const double c = (k != ref - 1 && fabs(fk - fr) > eps) ? fr - fk : 0.0;
+  double2 &x = X[(long long)b * sX + e];
+  x = make_double2(c * x.x, c * x.y);
+}
+
+void frm_scale(double2 *X, long long sX, long long n, const double *frm, int frm_stride, int k,
+               int ref, double eps, int batch, hipStream_t st) {
+  if (n <= 0 || batch <= 0) return;
+  hipLaunchKernelGGL(k_frm_scale, grid2(n, batch), dim3(NT), 0, st, X, sX, n, frm, frm_stride, k,
+                     ref, eps);
   HN_CHECK(hipGetLastError());
 }
 

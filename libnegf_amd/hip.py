@@ -67,6 +67,9 @@ def load():
     L.hipnegf_solve_transmission_dos.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), i, vp, vp,
                                                  C.POINTER(DosOpts), vp, vp, vp, vp]
     L.hipnegf_struct_set_tun_proj.argtypes = [vp, i, vp]
+    L.hipnegf_density_layout.argtypes = [vp, vp]
+    L.hipnegf_density_accumulate.argtypes = [vp, vp, i, vp, vp, vp, C.POINTER(Opts), i, vp, vp, i, i,
+                                             vp, vp]
     L.hipnegf_solve_scba_mw.argtypes = [vp, vp, i, vp, vp, C.POINTER(Opts), vp, i, d, vp, i, i,
                                         vp, vp, vp, vp, vp, vp]
     L.hipnegf_zgemm_batched.argtypes = [vp, i, i, i, vp, vp, ll, i, i, vp, ll, i, i, vp, vp, ll, i, i]
@@ -349,6 +352,52 @@ def solve_transmission_dos(eng: Engine, ds: DeviceStructure, E: np.ndarray, pair
                                                 len(pairs), _ptr(ni), _ptr(nf), C.byref(d), _ptr(T),
                                                 _ptr(ldos), _ptr(ncyc), _ptr(st)))
     return T, ldos[:, :nout], ncyc, st
+
+
+def density_layout(ds: DeviceStructure) -> np.ndarray:
+    t = ds.template
+    off = np.zeros(3 * t.nbl - 2 + 2 * t.ncont + 1, dtype=np.int64)
+    check(load().hipnegf_density_layout(ds.handle, _ptr(off)))
+    return off
+
+
+def density_blocks(ds: DeviceStructure, acc: np.ndarray) -> dict:
+    """Unpack a density accumulator into dict(Gd, Gu, Gl, Glc, Gcl) of arrays."""
+    t = ds.template
+    off = density_layout(ds)
+    b = t.bsz
+    nd = [b[c.cblk - 1] for c in t.contacts]
+    nc = [c.nc for c in t.contacts]
+    shapes = ([(n, n) for n in b] + [(b[i], b[i + 1]) for i in range(t.nbl - 1)] +
+              [(b[i + 1], b[i]) for i in range(t.nbl - 1)] + [(nd[k], nc[k]) for k in range(t.ncont)] +
+              [(nc[k], nd[k]) for k in range(t.ncont)])
+    mats = [acc[off[j]:off[j + 1]].reshape(sh, order="F") for j, sh in enumerate(shapes)]
+    n1, n2 = t.nbl, t.nbl - 1
+    return dict(Gd=mats[:n1], Gu=mats[n1:n1 + n2], Gl=mats[n1 + n2:n1 + 2 * n2],
+                Glc=mats[n1 + 2 * n2:n1 + 2 * n2 + t.ncont], Gcl=mats[n1 + 2 * n2 + t.ncont:])
+
+
+def density_accumulate(eng: Engine, ds: DeviceStructure, z: np.ndarray, w: np.ndarray, mode: int = 0,
+                       frm=None, ref: int = 0, outer: int = 2, slot=None, delta: float = 1e-4,
+                       acc: Optional[np.ndarray] = None, max_batch: int = 0):
+    """Sum of w_i G^r(z_i) (mode 0, contour points) or w_i G^n(Re z_i)
+    (mode 1, real axis; contacts at Re z + i delta; frm[n, ncont+1]) over
+    the blocks of the structure.  Returns (acc, status)."""
+    z = np.ascontiguousarray(z, dtype=np.complex128)
+    n = len(z)
+    Er = np.ascontiguousarray(z.real)
+    Ei = np.ascontiguousarray(z.imag)
+    ww = np.ascontiguousarray(w, dtype=np.complex128)
+    sl = np.zeros(n, dtype=np.int32) if slot is None else np.ascontiguousarray(slot, dtype=np.int32)
+    tot = int(density_layout(ds)[-1])
+    a = np.zeros(tot, dtype=np.complex128) if acc is None else np.ascontiguousarray(acc, dtype=np.complex128)
+    fr = None if frm is None else np.ascontiguousarray(frm, dtype=np.float64)
+    st = np.zeros(n, dtype=np.int32)
+    o = _opts(delta, 1, 2, FP64, max_batch)
+    check(load().hipnegf_density_accumulate(eng.handle, ds.handle, n, _ptr(Er), _ptr(Ei), _ptr(sl),
+                                            C.byref(o), mode, _ptr(ww), None if fr is None else _ptr(fr),
+                                            ref, outer, _ptr(a), _ptr(st)))
+    return a, st
 
 
 def solve_scba_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, coupling: np.ndarray,

@@ -196,6 +196,46 @@ class NEGF:
         self.set_params()
         self._call("negf_calculate_dephasing_transmission")
 
+    def read_hs(self, re_fname: str, im_fname: str, target: int):
+        """H (target 0) or S (target 1) from the PETSc ASCII real/imaginary files."""
+        self._call("negf_read_hs", C.c_char_p(re_fname.encode()), C.c_char_p(im_fname.encode()),
+                   C.c_int(target))
+
+    def set_output(self, path: str):
+        self._call("negf_set_output", C.c_char_p(path.encode()))
+
+    def write_tun_and_dos(self):
+        """tunneling_*.dat and localDOS_*.dat in the output path."""
+        self._call("negf_write_tunneling_and_dos")
+
+    def solve_density(self):
+        """compute_density_dft: the density matrix by contour (and, with
+        params.np_real > 0, real-axis) integration; read it with get_dm()."""
+        self.set_params()
+        self._call("negf_solve_density_dft")
+
+    def set_outer(self, outer: int):
+        """0: device blocks only, 1: + device/contact, 2: + contact/device (default)."""
+        self._call("negf_set_outer", C.byref(C.c_int(outer)))
+
+    def get_dm(self):
+        """The density matrix as a scipy CSR matrix (complex)."""
+        import scipy.sparse as sps
+        nnz, nrow = C.c_int(), C.c_int()
+        d = np.zeros(1, dtype=np.int32)
+        dd = np.zeros(1)
+        self._call("negf_get_dm", C.byref(nnz), C.byref(nrow), hip._ptr(d), hip._ptr(d), hip._ptr(dd),
+                   hip._ptr(dd), C.c_int(0))
+        rowpnt = np.zeros(nrow.value + 1, dtype=np.int32)
+        colind = np.zeros(max(nnz.value, 1), dtype=np.int32)
+        re = np.zeros(max(nnz.value, 1))
+        im = np.zeros(max(nnz.value, 1))
+        self._call("negf_get_dm", C.byref(nnz), C.byref(nrow), hip._ptr(rowpnt), hip._ptr(colind),
+                   hip._ptr(re), hip._ptr(im), C.c_int(1))
+        n = nnz.value
+        return sps.csr_matrix(((re + 1j * im)[:n], colind[:n] - 1, rowpnt - 1),
+                              shape=(nrow.value, nrow.value), dtype=np.complex128)
+
     def current(self, units_h: str = "unknown", units_j: str = "unknown") -> float:
         self.set_params()
         out = C.c_double()

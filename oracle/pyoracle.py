@@ -84,6 +84,18 @@ def lib():
         L.or_npoints.restype = C.c_int
         L.or_npoints.argtypes = [C.c_double] * 3
         L.or_omp_max_threads.restype = C.c_int
+        L.or_gauleg.argtypes = [C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int]
+        L.or_contour_grid.restype = C.c_int
+        L.or_contour_grid.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
+                                      C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+        L.or_real_axis_grid.restype = C.c_int
+        L.or_real_axis_grid.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                        C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+        L.or_gr_point.restype = C.c_int
+        L.or_gr_point.argtypes = [C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 6 + [C.c_double]
+        L.or_gn_point.restype = C.c_int
+        L.or_gn_point.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int] + \
+            [C.c_void_p] * 6 + [C.c_double, C.c_double]
         _LIB = L
     return _LIB
 
@@ -319,3 +331,149 @@ def integrate_el_mw(integration, Iv, estep):
 
 def npoints(emin, emax, estep):
     return lib().or_npoints(emin, emax, estep)
+
+
+# ----------------------------------------------------------- density matrix --
+def gauleg(x1: float, x2: float, n: int):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().or_gauleg(x1, x2, x.ctypes.data, w.ctypes.data, n)
+    return x, w
+
+
+def contour_grid(Elow, muref, kbT, n_kt, np1, np2, npoles, g_spin):
+    """contour_int_def (no inelastic terms): (z, w) complex arrays."""
+    n = np1 + np2 + npoles
+    z = np.zeros(max(n, 1), dtype=np.complex128)
+    w = np.zeros(max(n, 1), dtype=np.complex128)
+    lib().or_contour_grid(Elow, muref, kbT, n_kt, np1, np2, npoles, g_spin, z.ctypes.data, w.ctypes.data)
+    return z[:n], w[:n]
+
+
+def real_axis_grid(mumin, mumax, kbt_max, n_kt, npts, delta, g_spin):
+    z = np.zeros(max(npts, 1), dtype=np.complex128)
+    w = np.zeros(max(npts, 1), dtype=np.complex128)
+    lib().or_real_axis_grid(mumin, mumax, kbt_max, n_kt, npts, delta, g_spin, z.ctypes.data, w.ctypes.data)
+    return z[:npts], w[:npts]
+
+
+def _blocks(shapes):
+    arrs = [np.zeros(sh, dtype=np.complex128, order="F") for sh in shapes]
+    ptr = (C.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])
+    return arrs, C.cast(ptr, C.c_void_p), ptr
+
+
+def _point_shapes(s):
+    b = s.bsz
+    nd = [b[c.cblk - 1] for c in s.contacts]
+    nc = [c.nc for c in s.contacts]
+    return ([(n, n) for n in b], [(b[i], b[i + 1]) for i in range(len(b) - 1)],
+            [(b[i + 1], b[i]) for i in range(len(b) - 1)], [(nd[k], nc[k]) for k in range(len(nc))],
+            [(nc[k], nd[k]) for k in range(len(nc))])
+
+
+def gr_point(s, z: complex):
+    """G^r blocks at complex z with the outer blocks (unmasked)."""
+    os_ = OracleStruct(s)
+    shd, shu, shl, shlc, shcl = _point_shapes(s)
+    Gd, pd, kd = _blocks(shd)
+    Gu, pu, ku = _blocks(shu)
+    Gl, pl, kl = _blocks(shl)
+    Glc, plc, klc = _blocks(shlc)
+    Gcl, pcl, kcl = _blocks(shcl)
+    ncyc = np.zeros(max(s.ncont, 1), dtype=np.int32)
+    lib().or_gr_point(os_.ptr, z.real, z.imag, pd, pu, pl, plc, pcl, ncyc.ctypes.data, SGFACC)
+    return dict(Gd=Gd, Gu=Gu, Gl=Gl, Glc=Glc, Gcl=Gcl, ncyc=ncyc[:s.ncont])
+
+
+def gn_point(s, E: float, delta: float, frm, ref: int, eps: float = 0.0):
+    """G^n blocks on the real axis with the masked outer blocks."""
+    os_ = OracleStruct(s)
+    shd, shu, shl, shlc, shcl = _point_shapes(s)
+    Gd, pd, kd = _blocks(shd)
+    Gu, pu, ku = _blocks(shu)
+    Gl, pl, kl = _blocks(shl)
+    Glc, plc, klc = _blocks(shlc)
+    Gcl, pcl, kcl = _blocks(shcl)
+    fr = np.ascontiguousarray(frm, dtype=np.float64)
+    ncyc = np.zeros(max(s.ncont, 1), dtype=np.int32)
+    lib().or_gn_point(os_.ptr, E, delta, fr.ctypes.data, ref, pd, pu, pl, plc, pcl, ncyc.ctypes.data,
+                      SGFACC, eps)
+    return dict(Gd=Gd, Gu=Gu, Gl=Gl, Glc=Glc, Gcl=Gcl, ncyc=ncyc[:s.ncont])
+
+
+def _t_pattern(s, k):
+    c = s.contacts[k]
+    p = c.HMC != 0
+    if c.SMC is not None:
+        p = p | (c.SMC != 0)
+    return p
+
+
+def _add_point(s, N, cstart, P, T, pat, blk, w, lower, masked_outer, tl_pattern=None):
+    """Adds w * blocks onto the pattern: the device part through blk2csr
+    (iterative.F90:1375-1450) on the S pattern P (rows/columns of the device
+    only), the outer blocks at (block cb rows, contact PL1 columns) and, with
+    lower, at the transposed position (calculate_Gr_outer / calculate_Gn_outer)."""
+    off = s.block_offsets()
+    nd = s.ndev
+    rr, cc = np.nonzero(P[:nd, :nd])
+    bidx = np.searchsorted(off, np.arange(nd), side="right") - 1
+    for r, c in zip(rr, cc):
+        x, y = bidx[r], bidx[c]
+        if y == x:
+            v = blk["Gd"][x][r - off[x], c - off[y]]
+        elif y == x + 1:
+            v = blk["Gu"][x][r - off[x], c - off[y]]
+        elif y == x - 1:
+            v = blk["Gl"][y][r - off[x], c - off[y]]
+        else:
+            raise ValueError("S pattern outside the block tridiagonal (blk2csr)")
+        T[r, c] += w * v
+    pat |= P
+    for k, con in enumerate(s.contacts):
+        cb = con.cblk - 1
+        r0, c0, nc, ndk = off[cb], cstart[k], con.nc, s.bsz[cb]
+        tp = _t_pattern(s, k)
+        up = blk["Glc"][k]
+        m = tp & (up != 0)
+        sub = T[r0:r0 + ndk, c0:c0 + nc]
+        sub[m] += w * up[m]
+        pat[r0:r0 + ndk, c0:c0 + nc] |= m
+        if lower:
+            lo = blk["Gcl"][k]
+            ml = (tp.T if not masked_outer else m.T) & (lo != 0)
+            sub = T[c0:c0 + nc, r0:r0 + ndk]
+            sub[ml] += w * lo[ml]
+            pat[c0:c0 + nc, r0:r0 + ndk] |= ml
+
+
+def density_matrix(s, N, cstart, P, contour=None, real=None, frm_of=None, ref=None, outer=2,
+                   dore="D", delta=1e-4, eps=0.0):
+    """Dense N x N density matrix and its pattern as compute_density_dft builds
+    it (contour_int, integrations.F90:844-897; real_axis_int, :1093-1176):
+    T = sum_i w_i Gr(z_i) on the pattern, rho = i (T - T^H), then
+    rho += sum_j w_j Gn(E_j).  P: N x N boolean S pattern; cstart[k]: 0-based
+    first column of contact k's first PL; frm_of(E) -> frm[ncont+1]."""
+    rho = np.zeros((N, N), dtype=np.complex128)
+    pat = np.zeros((N, N), dtype=bool)
+    pat[np.arange(N), np.arange(N)] = True
+    if contour is not None:
+        T = np.zeros((N, N), dtype=np.complex128)
+        pt = np.eye(N, dtype=bool)
+        for z, w in zip(*contour):
+            blk = gr_point(s, complex(z))
+            ww = w * z if dore == "E" else w
+            _add_point(s, N, cstart, P, T, pt, blk, ww, outer == 2, False)
+        rho = 1j * (T - T.conj().T)
+        pat = pt | pt.T
+    if real is not None:
+        T = np.zeros((N, N), dtype=np.complex128)
+        pt = np.eye(N, dtype=bool)
+        for z, w in zip(*real):
+            E = float(np.real(z))
+            blk = gn_point(s, E, float(np.imag(z)), frm_of(E), ref, eps)
+            ww = w * E if dore == "E" else w
+            _add_point(s, N, cstart, P, T, pt, blk, ww, outer == 2, True)
+        rho = rho + T
+        pat = pat | pt
+    return rho, pat

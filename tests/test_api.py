@@ -74,6 +74,36 @@ def test_defaults_and_host_errors():
     n.close()
 
 
+def write_petsc(path_re, path_im, M):
+    """PETSc ASCII pair as read_H reads it (input_output.F90:52-60, 84-100):
+    4 header lines, then one "j i value" line per nonzero (1-based)."""
+    ii, jj = np.nonzero(M)
+    for path, part in ((path_re, M.real), (path_im, M.imag)):
+        with open(path, "w") as f:
+            f.write(f"% Size = {M.shape[0]} {M.shape[1]}\n% Nonzeros = {len(ii)}\n")
+            f.write(f"zzz = zeros({len(ii)},3);\nzzz = [\n")
+            for i, j in zip(ii, jj):
+                f.write(f"{j + 1} {i + 1} {part[i, j]:.17e}\n")
+
+
+def test_read_hs_host_errors(tmp_path):
+    """negf_read_hs parses on the host: missing files, mismatched real and
+    imaginary headers and indexes are reported (read_H's error_msg cases)."""
+    from libnegf_amd.negf import NEGF, NegfError
+    n = NEGF()
+    with pytest.raises(NegfError, match="not found"):
+        n.read_hs(str(tmp_path / "nope_re.dat"), str(tmp_path / "nope_im.dat"), 0)
+    M = np.diag(np.arange(1.0, 6.0)) + 0.5j * np.eye(5, k=1) - 0.5j * np.eye(5, k=-1)
+    write_petsc(tmp_path / "a_re.dat", tmp_path / "a_im.dat", M)
+    n.read_hs(str(tmp_path / "a_re.dat"), str(tmp_path / "a_im.dat"), 0)
+    with pytest.raises(NegfError, match="target_matrix"):
+        n.read_hs(str(tmp_path / "a_re.dat"), str(tmp_path / "a_im.dat"), 2)
+    write_petsc(tmp_path / "b_re.dat", tmp_path / "b_im.dat", M[:4, :4])
+    with pytest.raises(NegfError, match="nrow Error"):
+        n.read_hs(str(tmp_path / "a_re.dat"), str(tmp_path / "b_im.dat"), 1)
+    n.close()
+
+
 # ------------------------------------------------------------- GPU scenarios ----
 @pytest.fixture(scope="module")
 def gpu_negf():
@@ -189,3 +219,47 @@ def test_kpoint_slots_and_overlap(gpu_negf):
         big = Tref > 1e-6 * Tref.max()
         assert (np.abs(T[big] - Tref[big]) / Tref[big]).max() <= 1e-10
     n.close()
+
+
+@pytest.mark.gpu
+def test_read_hs_and_output_files(gpu_negf, tmp_path):
+    """H from PETSc files (negf_read_hs) solves bitwise like the same H given
+    in CSR; write_tunneling_and_dos writes E17.8 columns of E, T and the LDOS
+    (libnegf.F90:2041-2099) into the output path."""
+    NEGF, orc = gpu_negf
+    ons = np.zeros(60)
+    ons[25] = ons[35] = 1.0
+    H = chain_dense(onsite=ons)
+    plend = [15, 30, 45, 60]
+    a = _chain_session(NEGF, H, plend, [4, 1], -3.0, 3.0, 0.05)
+    a.solve_landauer()
+    Ta = a.get_transmission()
+    a.close()
+    write_petsc(tmp_path / "H_re.dat", tmp_path / "H_im.dat", H)
+    b = NEGF()
+    b.read_hs(str(tmp_path / "H_re.dat"), str(tmp_path / "H_im.dat"), 0)
+    b.set_s_id(100)
+    b.init_structure(2, np.array([80, 100]), np.array([60, 80]), 4, np.array(plend), np.array([4, 1]),
+                     surfstart=np.array([61, 81]))
+    p = b.params
+    p.emin, p.emax, p.estep = -3.0, 3.0, 0.05
+    p.kbt_t[0] = p.kbt_t[1] = 1e-3
+    p.mu[0], p.mu[1] = -0.5, 0.5
+    b.set_ldos_intervals(np.array([1, 31]), np.array([30, 60]))
+    b.set_output(str(tmp_path))
+    b.solve_landauer()
+    Tb = b.get_transmission()
+    np.testing.assert_array_equal(Tb, Ta)
+    L = b.get_ldos()
+    E, _ = b.get_energies()
+    b.write_tun_and_dos()
+    b.close()
+    lines = open(tmp_path / "tunneling_000001_000000.dat").read().splitlines()
+    assert len(lines) == len(E)
+    assert all(len(l) == 34 and re.fullmatch(r"( {0,3}-?0\.\d{8}E[+-]\d{2}){2}", l) for l in lines)
+    got = np.array([[float(l[k:k + 17]) for k in (0, 17)] for l in lines])
+    np.testing.assert_allclose(got[:, 0], E, rtol=5.1e-8, atol=1e-300)
+    np.testing.assert_allclose(got[:, 1], Tb[0], rtol=5.1e-8, atol=1e-300)
+    dl = open(tmp_path / "localDOS_000001_000000.dat").read().splitlines()
+    gl = np.array([[float(l[k:k + 17]) for k in (0, 17, 34)] for l in dl])
+    np.testing.assert_allclose(gl[:, 1:], L.T, rtol=5.1e-8, atol=1e-300)
