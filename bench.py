@@ -14,6 +14,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -220,12 +221,12 @@ def bench_c4(a, hip, wl, torch):
         f"SCBA passes mean {it.mean():.1f} max {it.max()}; status nonzero {(r['status'] != 0).sum()}")
     roofline = None
     if kern:
-        v = kern["k_gemm<double, 1>"]
+        v = kern[hip.PT_NAMES[hip.PT_GEMM_SWEEP]]
         if v["flops"] > 0:
             avg = v["ms"] * 1e-3 / v["launches"]
             ach = v["flops"] / v["launches"] / avg / 1e12
             roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                        "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": "k_gemm<double, 1>",
+                        "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": hip.PT_NAMES[hip.PT_GEMM_SWEEP],
                         "avg_launch_us": avg * 1e6, "launches": v["launches"],
                         "share_of_step_time": v["ms"] * 1e-3 / elapsed}
     cpu = None
@@ -476,7 +477,7 @@ def main():
             avg_s = v["ms"] * 1e-3 / v["launches"]
             fl = v["flops"] / v["launches"]
             achieved = fl / avg_s / 1e12
-            tag = name.replace("<double, ", "_").replace(">", "")
+            tag = re.sub(r"[<>, ]+", "_", name.replace("double", "")).strip("_").replace("__", "_")
             traffic, tsrc = traffic_from_profiles(tag)
             roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,

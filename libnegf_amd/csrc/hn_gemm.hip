@@ -1,24 +1,34 @@
 // hn_gemm.hip -- batched complex GEMM on CDNA4 matrix cores.
 //
-// One 256-thread workgroup (4 waves, 2 x 2) computes a 64 x 64 complex tile of
-// C for one batch item.  K is walked in steps of 16 complex elements staged
-// through LDS as separate real/imaginary planes (k-major), so every MFMA
-// operand fragment is one conflict-free ds_read per lane.  The LDS stage is
+// Complex products use four real MFMAs:  Cr += Ar Br - Ai Bi,
+// Ci += Ar Bi + Ai Br.  K is walked in steps of 16 complex elements staged
+// through LDS as separate real/imaginary planes (k-major).  The LDS stage is
 // double-buffered: while the matrix cores work on step t from one buffer,
 // step t+1 (prefetched into registers during step t-1) is stored into the
 // other and step t+2 is fetched from memory -- one barrier per K step.
-// Complex products use four real MFMAs (v_mfma_f64_16x16x4_f64 /
-// v_mfma_f32_16x16x4_f32):  Cr += Ar Br - Ai Bi,  Ci += Ar Bi + Ai Br,
-// issued so that no MFMA depends on the one just before it.  Each wave owns a
-// 32 x 32 sub-tile = 2 x 2 MFMA tiles x (re, im) accumulators.  The epilogue
-// stages the tile through LDS so C is read and written along columns
-// (coalesced): the rank-16 updates of the blocked inverse are pure C streaming.
+//
+// FP64 uses v_mfma_f64_4x4x4 (four independent 4x4x4 blocks per
+// instruction), which sustains ~72 TF/s on MI355X against ~49 TF/s for
+// v_mfma_f64_16x16x4 (measured with register-only issue loops).  Lane
+// t = 16k + 4b + a supplies A_b(a, k) and lane t = 16k + 4b + c supplies
+// B_b(k, c); lane 16a + 4b + c receives D_b(a, c).  Taking A_b = rows
+// 4b..4b+3 of a 16-row strip and the same four columns as B_b for every
+// block, one instruction produces a 16 x 4 strip of C from the fragments
+// A(lane % 16, lane / 16) and B(lane / 16, 4q + lane % 4).  The products are
+// the same four-term FP64 dot products as with the 16x16x4 form, and the
+// results are bitwise identical to it.
+// Two tilings: 64 x 128 per workgroup (each of the 4 waves 64 x 32) for
+// K >= 128 and N >= 128, 64 x 64 (each wave 32 x 32) otherwise.  FP32 keeps
+// v_mfma_f32_16x16x4 on the 64 x 64 tiling.  The epilogue stages the tile
+// through LDS so C is read and written along columns (coalesced): the rank-16
+// updates of the blocked inverse are pure C streaming.
 //
 // Workgroup -> (item, tile) mapping is XCD-aware: hardware dispatches
 // consecutive workgroup ids round-robin over the 8 XCDs, so the linear id is
 // remapped such that all tiles of one item land on the same XCD and share its
 // L2 (the A and B panels of an item are re-read by every tile).
 #include <algorithm>
+#include <type_traits>
 
 #include "hn_gemm.h"
 #include "hn_prof.h"
@@ -139,13 +149,18 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
     }
   };
 
-  v4 cr[2][2], ci[2][2];
+  // FP64: each wave's 32 x 32 = 2 row strips x 8 column quads of 4x4x4
+  // instructions; FP32: 2 x 2 tiles of 16x16x4.
+  constexpr bool M4 = sizeof(R) == 8;
+  constexpr int NP = 2, NQ = M4 ? 8 : 2;
+  typedef typename std::conditional<M4, R, v4>::type acc_t;
+  acc_t cr[NP][NQ], ci[NP][NQ];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
+  for (int i = 0; i < NP; ++i)
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      cr[i][j] = v4{0, 0, 0, 0};
-      ci[i][j] = v4{0, 0, 0, 0};
+    for (int j = 0; j < NQ; ++j) {
+      cr[i][j] = acc_t{0};
+      ci[i][j] = acc_t{0};
     }
 
   int cur = 0;
@@ -160,32 +175,61 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4), c16 = lane & 15;
-      R ar[2], ai[2], nai[2], br[2], bi[2];
+      if constexpr (M4) {
+        R ar[2], ai[2], nai[2], br[8], bi[8];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
-        ar[i] = sm.Ar[kr][wm * 32 + i * 16 + c16];
-        ai[i] = sm.Ai[kr][wm * 32 + i * 16 + c16];
-        nai[i] = -ai[i];
-      }
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        br[j] = sm.Br[kr][wn * 32 + j * 16 + c16];
-        bi[j] = sm.Bi[kr][wn * 32 + j * 16 + c16];
-      }
-#pragma unroll
-      for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          cr[i][j] = mfma(ar[i], br[j], cr[i][j]);
-          ci[i][j] = mfma(ar[i], bi[j], ci[i][j]);
+        for (int i = 0; i < 2; ++i) {
+          ar[i] = sm.Ar[kr][wm * 32 + i * 16 + c16];
+          ai[i] = sm.Ai[kr][wm * 32 + i * 16 + c16];
+          nai[i] = -ai[i];
         }
 #pragma unroll
-      for (int i = 0; i < 2; ++i)
+        for (int q = 0; q < 8; ++q) {
+          br[q] = sm.Br[kr][wn * 32 + q * 4 + (lane & 3)];
+          bi[q] = sm.Bi[kr][wn * 32 + q * 4 + (lane & 3)];
+        }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(ar[i], br[q], cr[i][q], 0, 0, 0);
+            ci[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(ar[i], bi[q], ci[i][q], 0, 0, 0);
+          }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(nai[i], bi[q], cr[i][q], 0, 0, 0);
+            ci[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(ai[i], br[q], ci[i][q], 0, 0, 0);
+          }
+      } else {
+        R ar[2], ai[2], nai[2], br[2], bi[2];
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          ar[i] = sm.Ar[kr][wm * 32 + i * 16 + c16];
+          ai[i] = sm.Ai[kr][wm * 32 + i * 16 + c16];
+          nai[i] = -ai[i];
+        }
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
-          cr[i][j] = mfma(nai[i], bi[j], cr[i][j]);
-          ci[i][j] = mfma(ai[i], br[j], ci[i][j]);
+          br[j] = sm.Br[kr][wn * 32 + j * 16 + c16];
+          bi[j] = sm.Bi[kr][wn * 32 + j * 16 + c16];
         }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            cr[i][j] = mfma(ar[i], br[j], cr[i][j]);
+            ci[i][j] = mfma(ar[i], bi[j], ci[i][j]);
+          }
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            cr[i][j] = mfma(nai[i], bi[j], cr[i][j]);
+            ci[i][j] = mfma(ai[i], br[j], ci[i][j]);
+          }
+      }
     }
     if (k0 + BK < K) {
       store(sm2.buf[cur ^ 1]);          // next K step into the other buffer
@@ -210,15 +254,27 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
+    if constexpr (M4) {
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int row = wm * 32 + i * 16 + out_row<R>(lane, r);
-        const int sc = wn * 16 + (lane & 15);
-        Er[sc * EL + row] = cr[i][j][r];
-        Ei[sc * EL + row] = ci[i][j][r];
-      }
+        for (int q = 0; q < 4; ++q) {
+          const int row = wm * 32 + i * 16 + 4 * ((lane >> 2) & 3) + (lane >> 4);
+          const int sc = wn * 16 + 4 * q + (lane & 3);
+          Er[sc * EL + row] = cr[i][4 * j + q];
+          Ei[sc * EL + row] = ci[i][4 * j + q];
+        }
+    } else {
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm * 32 + i * 16 + out_row<R>(lane, r);
+          const int sc = wn * 16 + (lane & 15);
+          Er[sc * EL + row] = cr[i][j][r];
+          Ei[sc * EL + row] = ci[i][j][r];
+        }
+    }
     __syncthreads();
     // slab column sc holds tile column (sc / 16) * 32 + j * 16 + sc % 16
 #pragma unroll
@@ -254,10 +310,227 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   }
 }
 
+// ---- FP64 wide tile: 64 x 128 per workgroup, each wave 64 x 32 (4 row
+// strips x 8 column quads of 4x4x4 instructions) ----------------------------
+constexpr int WBN = 128, WPA = 2, WPB = 2;
+struct SmemW {
+  double Ar[BK][BM + WPA];
+  double Ai[BK][BM + WPA];
+  double Br[BK][WBN + WPB];
+  double Bi[BK][WBN + WPB];
+};
+struct SmemW2 {
+  SmemW buf[2];
+};
+
+template <int USE>
+__global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, int tiles_n,
+                                               int total, int total_pad) {
+  typedef double2 CT;
+  const int logical = xcd_remap(blockIdx.x, total_pad);
+  if (logical >= total) return;
+  const int tiles = tiles_m * tiles_n;
+  const int b = logical / tiles;
+  const int t = logical - b * tiles;
+  if (p.active && p.active[b] == 0) return;
+  const int tm = t % tiles_m, tn = t / tiles_m;
+  const int m0 = tm * BM, n0 = tn * WBN;
+
+  __shared__ SmemW2 sm2;
+
+  const CT *A = p.A + (long long)b * p.sA;
+  const CT *B = p.B + (long long)b * p.sB;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int M = p.m, N = p.n, K = p.k;
+  const bool a_n = p.opA == OP_N, b_n = p.opB == OP_N;
+  const double sa = (p.opA == OP_C) ? -1.0 : 1.0, sb = (p.opB == OP_C) ? -1.0 : 1.0;
+
+  CT ra[4], rb[8];
+  const bool full_mn = (m0 + BM <= M) && (n0 + WBN <= N);
+  auto load = [&](int k0) {
+    if (full_mn && k0 + BK <= K) {  // interior tile: no per-element bounds tests
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        int i, kk;
+        if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
+        else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+        const int gi = m0 + i, gk = k0 + kk;
+        ra[r] = a_n ? A[gi + (long long)gk * p.lda] : A[gk + (long long)gi * p.lda];
+      }
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        int j, kk;
+        if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
+        else { j = tid & 127; kk = (tid >> 7) + 2 * r; }
+        const int gj = n0 + j, gk = k0 + kk;
+        rb[r] = b_n ? B[gk + (long long)gj * p.ldb] : B[gj + (long long)gk * p.ldb];
+      }
+      return;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int i, kk;
+      if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
+      else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+      const int gi = m0 + i, gk = k0 + kk;
+      CT v = {0, 0};
+      if (gi < M && gk < K) v = a_n ? A[gi + (long long)gk * p.lda] : A[gk + (long long)gi * p.lda];
+      ra[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int j, kk;
+      if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
+      else { j = tid & 127; kk = (tid >> 7) + 2 * r; }
+      const int gj = n0 + j, gk = k0 + kk;
+      CT v = {0, 0};
+      if (gj < N && gk < K) v = b_n ? B[gk + (long long)gj * p.ldb] : B[gj + (long long)gk * p.ldb];
+      rb[r] = v;
+    }
+  };
+  auto store = [&](SmemW &sm) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      int i, kk;
+      if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
+      else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+      sm.Ar[kk][i] = ra[r].x;
+      sm.Ai[kk][i] = sa * ra[r].y;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int j, kk;
+      if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
+      else { j = tid & 127; kk = (tid >> 7) + 2 * r; }
+      sm.Br[kk][j] = rb[r].x;
+      sm.Bi[kk][j] = sb * rb[r].y;
+    }
+  };
+
+  double cr[4][8], ci[4][8];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      cr[i][q] = 0.0;
+      ci[i][q] = 0.0;
+    }
+  int cur = 0;
+  if (K > 0) {
+    load(0);
+    store(sm2.buf[0]);
+    __syncthreads();
+    if (BK < K) load(BK);
+  }
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    SmemW &sm = sm2.buf[cur];
+    // operand fragments of sub-step s + 1 are read from LDS while the MFMAs of
+    // sub-step s run (two register sets)
+    double fa[2][4], fai[2][4], fb[2][8], fbi[2][8];
+    auto frag = [&](int ks, int slot) {
+      const int kr = ks + (lane >> 4), c16 = lane & 15;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        fa[slot][i] = sm.Ar[kr][i * 16 + c16];
+        fai[slot][i] = sm.Ai[kr][i * 16 + c16];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        fb[slot][q] = sm.Br[kr][wave * 32 + q * 4 + (lane & 3)];
+        fbi[slot][q] = sm.Bi[kr][wave * 32 + q * 4 + (lane & 3)];
+      }
+    };
+    frag(0, 0);
+#pragma unroll
+    for (int s = 0; s < BK / 4; ++s) {
+      const int sl = s & 1;
+      if (s + 1 < BK / 4) frag(4 * (s + 1), sl ^ 1);
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(fa[sl][i], fb[sl][q], cr[i][q], 0, 0, 0);
+          ci[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(fa[sl][i], fbi[sl][q], ci[i][q], 0, 0, 0);
+        }
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(-fai[sl][i], fbi[sl][q], cr[i][q], 0, 0, 0);
+          ci[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(fai[sl][i], fb[sl][q], ci[i][q], 0, 0, 0);
+        }
+    }
+    if (k0 + BK < K) {
+      store(sm2.buf[cur ^ 1]);
+      if (k0 + 2 * BK < K) load(k0 + 2 * BK);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  // ---- epilogue: two passes of a 64 x 64 slab (columns 16j..16j+15 of every wave) ----
+  constexpr int EL = BM + 1;
+  static_assert(2 * 64 * EL * sizeof(double) <= sizeof(SmemW2), "epilogue slab does not fit");
+  double *Er = &sm2.buf[0].Ar[0][0];
+  double *Ei = Er + 64 * EL;
+  CT *C = p.C + (long long)b * p.sC;
+  CT *C2 = (p.C2 && (!p.active2 || p.active2[b] != 0)) ? p.C2 + (long long)b * p.sC2 : nullptr;
+  const double ar_ = p.alpha.x, ai_ = p.alpha.y;
+  const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
+  const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int row = i * 16 + 4 * ((lane >> 2) & 3) + (lane >> 4);
+        const int sc = wave * 16 + 4 * q + (lane & 3);
+        Er[sc * EL + row] = cr[i][4 * j + q];
+        Ei[sc * EL + row] = ci[i][4 * j + q];
+      }
+    __syncthreads();
+    // slab column sc holds tile column (sc / 16) * 32 + j * 16 + sc % 16
+#pragma unroll
+    for (int q = 0; q < (64 * 64) / NT; ++q) {
+      const int e = tid + q * NT;
+      const int row = e & 63, sc = e >> 6;
+      const int gr = m0 + row;
+      const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
+      if (gr < M && gc < N && !(gc >= p.skip_c0 && gc < p.skip_c1)) {
+        const double xr = Er[sc * EL + row], xi = Ei[sc * EL + row];
+        const double vr = ar_ * xr - ai_ * xi, vi = ar_ * xi + ai_ * xr;
+        const long long o = gr + (long long)gc * p.ldc;
+        double yr = vr, yi = vi;
+        if (useb) {
+          const CT c = C[o];
+          yr += p.beta.x * c.x - p.beta.y * c.y;
+          yi += p.beta.x * c.y + p.beta.y * c.x;
+        }
+        C[o] = CT{yr, yi};
+        if (C2) {
+          const long long o2 = gr + (long long)gc * p.ldc2;
+          double zr = vr, zi = vi;
+          if (useb2) {
+            const CT c = C2[o2];
+            zr += p.beta2.x * c.x - p.beta2.y * c.y;
+            zi += p.beta2.x * c.y + p.beta2.y * c.x;
+          }
+          C2[o2] = CT{zr, zi};
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 template <typename R>
 void launch(const GemmArgs<R> &a, hipStream_t s) {
   if (a.m <= 0 || a.n <= 0 || a.batch <= 0) return;
-  int tiles_m = (a.m + BM - 1) / BM, tiles_n = (a.n + BN - 1) / BN;
+  const bool W = sizeof(R) == 8 && a.k >= 128 && a.n >= 128;
+  int tiles_m = (a.m + BM - 1) / BM, tiles_n = (a.n + (W ? WBN : BN) - 1) / (W ? WBN : BN);
   long long total = (long long)tiles_m * tiles_n * a.batch;
   if (total > (1LL << 30)) throw std::runtime_error("gemm: grid too large");
   int total_pad = (int)((total + 7) / 8 * 8);
@@ -276,6 +549,21 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
                              (rc ? 2.0 : 1.0) * nb * a.m * ncol);
   ProfScope ps(use, flops, bytes, s);
   if (a.prof_token) *a.prof_token = ps.tok;
+  if constexpr (sizeof(R) == 8) {
+    if (W) {
+      if (use == PT_GEMM_SWEEP)
+        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_SWEEP>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+                           tiles_n, (int)total, total_pad);
+      else if (use == PT_GEMM_INV)
+        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_INV>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+                           tiles_n, (int)total, total_pad);
+      else
+        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_MASKED>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+                           tiles_n, (int)total, total_pad);
+      HN_CHECK(hipGetLastError());
+      return;
+    }
+  }
   if (use == PT_GEMM_SWEEP)
     hipLaunchKernelGGL((k_gemm<R, PT_GEMM_SWEEP>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
                        tiles_n, (int)total, total_pad);

@@ -116,7 +116,9 @@ def check(rc: int):
 
 # launch-timing tags (hipnegf_profile_*)
 PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV, PT_INV_PANEL, PT_INV_SMALL = range(5)
-PT_NAMES = {PT_GEMM_MASKED: "k_gemm<double, 0>", PT_GEMM_SWEEP: "k_gemm<double, 1>",
+# Kernel serving each tag for FP64 blocks with K, N >= 128 (the 64 x 128 tiling);
+# smaller GEMMs of a tag run on k_gemm<double, tag>.
+PT_NAMES = {PT_GEMM_MASKED: "k_gemm_w<0>", PT_GEMM_SWEEP: "k_gemm_w<1>",
             PT_GEMM_INV: "k_gemm<double, 2>", PT_INV_PANEL: "k_inv_panel_reg",
             PT_INV_SMALL: "k_inv_lds"}
 
