@@ -107,7 +107,8 @@ struct SolveOpts {
   int polish = 0;            // reserved
   int dec_crit = 0;          // 0: max|C1| (CPU path), 1: sum|Re|+|Im| (GPU path)
   int share_mirror = 1;      // one decimation for mirror-image contact pairs
-  int lanes = 1;             // item groups solved concurrently, each on its own stream
+  int lanes = 0;             // item groups solved concurrently, each on its own stream;
+                             // 0: automatic (2 for >= 256 items of blocks >= 128, else 1)
   int graph = 0;             // replay the RGF sweep of a batch from a captured hipGraph
   // surface Green's function cache (surface_green + ln_cache, contselfenergy.F90:138-173,
   // 251-259): 0 compute only; 1 load if cached, else compute and store; 2 compute and store

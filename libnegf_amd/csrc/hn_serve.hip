@@ -67,6 +67,7 @@ unsigned long long Server::submit(const ReqStruct &rs, const std::vector<double>
   r->rs = rs;
   r->E = E;
   r->o = o;
+  r->o.lanes = 1;  // the workers themselves are the concurrent streams
   r->nE = (int)E.size();
   r->t_submit = clk::now();
   std::lock_guard<std::mutex> g(mu_);
