@@ -10,8 +10,9 @@ namespace hn {
 //
 // Operands of item b live at X + b * strideX (strideX may be 0: shared operand).
 // Items with active[b] == 0 are skipped (per-item convergence masks).
-// Columns [skip_c0, skip_c1) of C are left untouched (panel columns of the
-// blocked inverse).
+// Columns [skip_c0, skip_c1) and rows [skip_r0, skip_r1) of C are left
+// untouched (the super-panel of the blocked inverse); tiles that lie wholly
+// inside either range are not computed.
 template <typename R>
 struct GemmArgs {
   typedef typename std::conditional<sizeof(R) == 8, double2, float2>::type CT;
@@ -24,6 +25,7 @@ struct GemmArgs {
   const int *active = nullptr;
   const int *active2 = nullptr;  // optional separate mask of the second output C2
   int skip_c0 = 0, skip_c1 = 0;
+  int skip_r0 = 0, skip_r1 = 0;
   int batch = 1;
   int tag = -1;  // profile tag (ProfTag); -1: masked if `active` is set, else sweep
   int prof_items = -1;  // items known to be active (for flop counts of masked launches);

@@ -93,6 +93,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   if (p.active && p.active[b] == 0) return;
   const int tm = t % tiles_m, tn = t / tiles_m;
   const int m0 = tm * BM, n0 = tn * BN;
+  if ((m0 >= p.skip_r0 && m0 + BM <= p.skip_r1) || (n0 >= p.skip_c0 && n0 + BN <= p.skip_c1))
+    return;  // wholly inside a skipped range
 
   __shared__ Smem2<R> sm2;
 
@@ -283,7 +285,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
       const int row = e & 63, sc = e >> 6;
       const int gr = m0 + row;
       const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
-      if (gr < M && gc < N && !(gc >= p.skip_c0 && gc < p.skip_c1)) {
+      if (gr < M && gc < N && !(gc >= p.skip_c0 && gc < p.skip_c1) &&
+          !(gr >= p.skip_r0 && gr < p.skip_r1)) {
         const double xr = (double)Er[sc * EL + row], xi = (double)Ei[sc * EL + row];
         const double vr = ar_ * xr - ai_ * xi, vi = ar_ * xi + ai_ * xr;
         const long long o = gr + (long long)gc * p.ldc;
@@ -335,6 +338,8 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
   if (p.active && p.active[b] == 0) return;
   const int tm = t % tiles_m, tn = t / tiles_m;
   const int m0 = tm * BM, n0 = tn * WBN;
+  if ((m0 >= p.skip_r0 && m0 + BM <= p.skip_r1) || (n0 >= p.skip_c0 && n0 + WBN <= p.skip_c1))
+    return;  // wholly inside a skipped range
 
   __shared__ SmemW2 sm2;
 
@@ -499,7 +504,8 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
       const int row = e & 63, sc = e >> 6;
       const int gr = m0 + row;
       const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
-      if (gr < M && gc < N && !(gc >= p.skip_c0 && gc < p.skip_c1)) {
+      if (gr < M && gc < N && !(gc >= p.skip_c0 && gc < p.skip_c1) &&
+          !(gr >= p.skip_r0 && gr < p.skip_r1)) {
         const double xr = Er[sc * EL + row], xi = Ei[sc * EL + row];
         const double vr = ar_ * xr - ai_ * xi, vi = ar_ * xi + ai_ * xr;
         const long long o = gr + (long long)gc * p.ldc;
@@ -537,7 +543,8 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   const int use = a.tag >= 0 ? a.tag : (a.active ? PT_GEMM_MASKED : PT_GEMM_SWEEP);
   // algorithmic work: 8 real flops per complex multiply-add over the columns written
   const int skipped = std::max(0, std::min(a.skip_c1, a.n) - std::max(a.skip_c0, 0));
-  const double ncol = (double)(a.n - skipped);
+  const int skipped_r = std::max(0, std::min(a.skip_r1, a.m) - std::max(a.skip_r0, 0));
+  const double ncol = (double)(a.n - skipped) * (double)(a.m - skipped_r) / (double)a.m;
   const double items = a.prof_items >= 0 ? (double)a.prof_items
                       : a.prof_items == -2 ? 1.0
                       : ((a.active || a.active2) ? -1.0 : (double)a.batch);

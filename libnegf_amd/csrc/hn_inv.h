@@ -4,8 +4,7 @@
 
 namespace hn {
 
-constexpr int kInvPanel = 32;  // max inner panel width of the blocked path
-constexpr int kInvSuper = 64;  // super-panel width: deferred rank-64 updates
+constexpr int kInvSuper = 64;  // max super-panel width: deferred rank-64 updates
 
 // Largest n handled by the single-workgroup LDS kernel.
 int inv_lds_max_n(int real_bytes);
@@ -14,9 +13,7 @@ int inv_lds_max_n(int real_bytes);
 template <typename R>
 struct InvWork {
   typedef typename std::conditional<sizeof(R) == 8, double2, float2>::type CT;
-  CT *Z = nullptr;      // n x kInvPanel per item (inner panel multipliers)
-  CT *Zh = nullptr;     // n x kInvSuper per item (accumulated super-panel transform)
-  CT *Rb = nullptr;     // kInvSuper x n per item (pivot rows, leading dim kInvSuper)
+  CT *Rb = nullptr;     // n x kInvSuper per item (pivot columns of A, leading dim n)
   int *used = nullptr;  // n per item
   int *piv = nullptr;   // n per item
   int n = 0, batch = 0;

@@ -7,23 +7,16 @@
 // in-place Gauss-Jordan elimination with partial pivoting entirely in LDS,
 // followed by the column interchanges that undo the row pivoting.
 //
-// Large blocks: two-level blocked in-place Gauss-Jordan with *lazy* pivoting.
-// Rows are never moved; the pivot row of each logical column is recorded.
-// Per inner panel K of nb columns (one launch, one workgroup per item, the
-// panel in registers): partial-pivoting LU of the panel over the
-// not-yet-pivoted rows picks nb pivot rows; with Q = B(piv, K) every thread
-// forms its row of Z = [-B(r,K) Q^-1 ; Q^-1 - I], writes the new panel
-// columns, applies B(:, J) += Z R(:, J) to the other columns J of the current
-// super-panel S (kInvSuper columns; R = their pivot rows, staged in LDS), and
-// copies the pivot rows of the columns outside S.
-// After the inner panels of S, its columns hold Zhat + E where
-// I + Zhat P^T is the product of the inner rank-nb transforms (the last panel
-// of S writes Zhat), so the rest of the matrix takes ONE deferred
-// rank-kInvSuper update
-//   B(:, J) += Zhat R(:, J),  J outside S  (R = pivot rows before S started),
-// a GEMM on the matrix cores.
-// After the last panel, Ainv(i, c) = B(pi(i), pi^-1(c)) with pi(c) the pivot
-// row of logical column c.
+// Large blocks (up to n = 512): in-place Gauss-Jordan with lazy pivoting on
+// the row-major view M = A^T, so that rows of M are contiguous columns of A
+// (inverting M and reading the result back column-major gives A^-1; the row
+// pivoting of M is column pivoting of A).  Rows are never moved; the pivot
+// row of each logical column is recorded.  Per super-panel S of up to 64
+// columns, one launch (one workgroup of 512 threads per item, the n x |S|
+// panel in registers) runs the |S| elimination steps on S and writes the
+// accumulated transform; the other columns of M then take ONE deferred
+// rank-|S| update on the matrix cores.  After the last super-panel,
+// M^-1(i, c) = B(pi(i), pi^-1(c)) with pi(c) the pivot row of column c.
 #include "hn_gemm.h"
 #include "hn_inv.h"
 #include "hn_prof.h"
@@ -155,247 +148,292 @@ __global__ __launch_bounds__(NT) void k_inv_lds(int n, typename CT_<R>::t *A, lo
 // ---------------------------------------------------------------------------
 // blocked path
 // ---------------------------------------------------------------------------
-template <typename R>
-__global__ __launch_bounds__(NT) void k_inv_reset(int n, int batch, int *used,
-                                                  const int *active) {
-  long long e = (long long)blockIdx.x * NT + threadIdx.x;
-  if (e >= (long long)n * batch) return;
-  int b = (int)(e / n);
-  if (active && active[b] == 0) return;
-  used[e] = 0;
+constexpr int NTG = 512;  // threads of the Gauss-Jordan kernel: two halves of 256 rows
+
+// Pivot search key of an unpivoted row r (r < 512): the high word of the
+// FP64 |.|_1 (exponent and 10 mantissa bits; NaN ranks as inf), a flag bit,
+// and 511 - r, so that a larger magnitude wins and ties within 2^-10 go to
+// the smaller row index; 0 marks rows that cannot be chosen.
+__device__ inline unsigned pivot_key(double v, int r) {
+  const unsigned hi = min((unsigned)((unsigned long long)__double_as_longlong(v) >> 32), 0x7FF00000u);
+  return (hi & ~0x3FFu) | 0x200u | (unsigned)(511 - r);
+}
+__device__ inline unsigned pivot_key(float v, int r) {
+  return pivot_key((double)v, r);
+}
+template <int CTRL> __device__ inline unsigned dpp_max(unsigned k) {
+  return max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, CTRL, 0xF, 0xF, false));
+}
+// maximum key over the 64 lanes of a wave (every lane active): DPP butterflies
+// inside each row of 16 lanes, then the four row maxima read as scalars
+__device__ inline unsigned wave_max_key(unsigned k) {
+  k = dpp_max<0xB1>(k);   // quad_perm [1,0,3,2]
+  k = dpp_max<0x4E>(k);   // quad_perm [2,3,0,1]
+  k = dpp_max<0x124>(k);  // row_ror:4
+  k = dpp_max<0x128>(k);  // row_ror:8
+  const unsigned a = __builtin_amdgcn_readlane((int)k, 0), b = __builtin_amdgcn_readlane((int)k, 16);
+  const unsigned c = __builtin_amdgcn_readlane((int)k, 32), d = __builtin_amdgcn_readlane((int)k, 48);
+  return max(max(a, b), max(c, d));
 }
 
-// Register-resident panel kernel: thread t owns rows t + NT*q (q < RPT) of the
-// n x nb panel in registers.  Column j: block argmax over unused rows (one
-// wave shuffle reduction + one LDS exchange), the owner of the pivot row
-// publishes its row, every thread eliminates its own rows (multipliers kept in
-// place, LU storage).  Two barriers per column.  Afterwards the pivot block is
-// Q = L U (rows in selection order) and each thread solves W Q = B(r, K) for
-// its rows by one forward and one backward substitution (pivot rows solve
-// with e_j, giving row j of Q^-1), so Q is never inverted explicitly.
+// 1/b for a pivot: b scaled by a power of two (exact) so that |b'|^2 lies in
+// [1/4, 2), the hardware reciprocal of |b'|^2 refined by two Newton-Raphson
+// steps (r <- r + r (1 - d r)), and conj(b') r scaled back; 0 for b == 0.
+__device__ inline double2 crecip_fast(double2 b) {
+  const double mx = fmax(fabs(b.x), fabs(b.y));
+  if (mx == 0.0) return double2{0, 0};
+  const int e = __builtin_amdgcn_frexp_exp(mx);
+  const double x = __builtin_amdgcn_ldexp(b.x, -e), y = __builtin_amdgcn_ldexp(b.y, -e);
+  const double d = fma(x, x, y * y);
+  double r = __builtin_amdgcn_rcp(d);
+  r = fma(r, fma(-d, r, 1.0), r);
+  r = fma(r, fma(-d, r, 1.0), r);
+  return double2{__builtin_amdgcn_ldexp(x * r, -e), __builtin_amdgcn_ldexp(-y * r, -e)};
+}
+__device__ inline float2 crecip_fast(float2 b) { return crecip(b); }
+
+// rows of the LDS staging buffer of k_inv_gj (a chunk of the n x 8CT panel,
+// rows padded by one element)
+template <typename R, int RT, int CT>
+constexpr int gj_stage_rows() {
+  constexpr int rowb = (8 * CT + 1) * 2 * (int)sizeof(R);
+  constexpr int cr = ((140 * 1024) / rowb) & ~63;
+  return cr < 64 * RT ? cr : 64 * RT;
+}
+template <typename R, int RT, int CT>
+constexpr size_t gj_stage_bytes() {
+  return (size_t)gj_stage_rows<R, RT, CT>() * (8 * CT + 1) * 2 * sizeof(R);
+}
+
+// Gauss-Jordan elimination over one super-panel S = columns [s0, s0+sw),
+// sw <= 8 CT, of the row-major view M = A^T of an item (row r of M is column r
+// of A, so rows of M are contiguous in memory).
 //
-// The same kernel applies the panel's transform to the other columns of the
-// current super-panel [s0, s0+sw): B(:, J) += Z R(:, J) with R = the pivot
-// rows of those columns (staged in LDS before any row is updated) and Z held
-// in registers, one row per thread -- so no separate rank-nb GEMM pass runs
-// over the super-panel.
-template <typename R, int NB, int RPT>
-__global__ __launch_bounds__(NT) void k_inv_panel_reg(int n, int c0, int nbp, int rb_row, int s0,
-                                                      int sw,
-                                                      typename CT_<R>::t *__restrict__ A,
-                                                      long long sA,
-                                                      typename CT_<R>::t *__restrict__ Z,
-                                                      typename CT_<R>::t *__restrict__ Zh,
-                                                      typename CT_<R>::t *__restrict__ Rb,
-                                                      int *__restrict__ used_g,
-                                                      int *__restrict__ piv_g,
-                                                      const int *__restrict__ active,
-                                                      int *__restrict__ status) {
+// Wave w (of 8) holds columns s0 + w*CT .. +CT, lane l the rows l + 64 i
+// (i < RT), in registers (loaded and stored through an LDS staging buffer so
+// that global accesses run along rows).  Every step writes each updated value
+// one register lower (register 0 moves to the last), so the wave owning the
+// current pivot column always finds it in register 0 and no register is
+// indexed at run time; CT steps restore the order.
+//
+// Step j, pivot column owned by wave wj = j / CT:
+//   [after barrier A] every thread reads the pivot key (row p, the largest
+//   |.|_1 over the unpivoted rows) and 1/B(p, j); the eight threads holding
+//   row p publish it raw (1 in place of column j);
+//   [after barrier B] every row applies B(r, :) -= (B(r, j)/B(p, j)) B(p, :)
+//   and the pivot row becomes B(p, :)/B(p, j) (1/B(p, j) in column j).  The
+//   wave owning the next pivot column updates that column first and, before
+//   its other columns, publishes the column (the next multipliers) and its
+//   pivot candidate with the reciprocal, so the search overlaps the update.
+// LDS buffers alternate between steps.
+//
+// With Ẑ = B(:, S) - E (E: ones at (pivot row, column)) the columns J outside
+// S then take B(:, J) += Ẑ B_old(piv, J).  In A's column-major layout that is
+// A(J', :) += Rb Ẑ^T with Rb(c, k) = A(c, piv_k) -- contiguous columns of A --
+// and J' the rows of A outside S.  Ẑ^T is the rows S of A less E^T, and
+// -Rb E^T only cancels A(J', piv_k): the kernel moves those entries into Rb
+// (ld n; rows of S are not written) and leaves zeros, so that the deferred
+// update on the matrix cores is the plain product A(J', :) += Rb A(S, :).
+template <typename R, int RT, int CT>
+__global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
+                                                typename CT_<R>::t *__restrict__ A, long long sA,
+                                                typename CT_<R>::t *__restrict__ Rb,
+                                                int *__restrict__ used_g, int *__restrict__ piv_g,
+                                                const int *__restrict__ active,
+                                                int *__restrict__ status) {
   typedef typename CT_<R>::t C;
+  static_assert(CT >= 2, "the look-ahead needs two columns per wave");
+  constexpr int SWC = 8 * CT, LDST = SWC + 1;
+  constexpr int CR = gj_stage_rows<R, RT, CT>();
+  constexpr int PER = CR * SWC / NTG;  // staged elements per thread and chunk
   const int b = blockIdx.x;
   if (active && active[b] == 0) return;
-  __shared__ C LU[NB][NB];  // LU[j][l]: row j of the pivot block's LU factors
-  __shared__ C Rs[NB][kInvSuper];  // pivot rows of the other super-panel columns
-  __shared__ int spiv[kInvSuper];  // pivot rows of the super-panel's earlier columns
-  __shared__ int piv[NB];
-  __shared__ R redv[NT / 64];
-  __shared__ int redi[NT / 64];
+  extern __shared__ __align__(16) unsigned char gsm[];
+  C *stage = (C *)gsm;
+  __shared__ C prow[2][SWC];
+  __shared__ C mcol[2][64 * RT];
+  __shared__ unsigned redk[2];
+  __shared__ C redinv[2];
+  __shared__ int spiv[SWC];
   __shared__ int s_sing;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  C *Ab = A + (long long)b * sA;
-  C *Zb = Z + (long long)b * n * kInvPanel;
-  C *Rbb = Rb + (long long)b * kInvSuper * n + rb_row;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  C *Xb = A + (long long)b * sA;
   int *ug = used_g + (long long)b * n;
 
-  C P[RPT][NB];
-  int used[RPT], pos[RPT];
+  C P[RT][CT];
+  int used[RT];
 #pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const int r = tid + NT * q;
-    const bool valid = r < n;
-    used[q] = valid ? ug[r] : 1;
-    pos[q] = -1;
+  for (int i = 0; i < RT; ++i) {
+    const int r = l + 64 * i;
+    used[i] = r >= n ? 1 : (s0 == 0 ? 0 : ug[r]);
 #pragma unroll
-    for (int j = 0; j < NB; ++j)
-      P[q][j] = (valid && j < nbp) ? Ab[r + (size_t)(c0 + j) * n] : C{0, 0};
+    for (int c = 0; c < CT; ++c) P[i][c] = C{0, 0};
   }
   if (tid == 0) s_sing = 0;
-
+  // ---- load the panel, CR rows at a time: coalesced along rows of M ----
+  for (int cr0 = 0; cr0 < n; cr0 += CR) {
+    const int nr = min(CR, n - cr0);
 #pragma unroll
-  for (int j = 0; j < NB; ++j) {
-    if (j < nbp) {
-    // ---- argmax of |.|_1 over unused rows (NaN ranks below every number) ----
-    R best = R(-1);
-    int bi = -1;
-#pragma unroll
-    for (int q = 0; q < RPT; ++q)
-      if (!used[q]) {
-        R v = cabs1(P[q][j]);
-        if (!(v == v)) v = R(-0.5);
-        if (v > best) { best = v; bi = tid + NT * q; }
-      }
-    wave_argmax(best, bi);
-    if (lane == 0) { redv[wave] = best; redi[wave] = bi; }
-    __syncthreads();
-    R bv = redv[0];
-    int p = redi[0];
-#pragma unroll
-    for (int w = 1; w < NT / 64; ++w)
-      if (redv[w] > bv || (redv[w] == bv && redi[w] >= 0 && (p < 0 || redi[w] < p))) {
-        bv = redv[w];
-        p = redi[w];
-      }
-    if (tid == 0 && !(bv > R(0))) s_sing = 1;
-    // ---- the owner publishes its (L|U) row ----
-    if (p >= 0 && (p % NT) == tid) {
-#pragma unroll
-      for (int q = 0; q < RPT; ++q)
-        if (q == p / NT) {
-          used[q] = 1;
-          pos[q] = j;
-#pragma unroll
-          for (int l = 0; l < NB; ++l) LU[j][l] = P[q][l];
-        }
-      piv[j] = p;
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + NTG * k;
+      const int row = e / SWC, col = e % SWC;
+      if (row < nr)
+        stage[row * LDST + col] = col < sw ? Xb[(size_t)(cr0 + row) * n + s0 + col] : C{0, 0};
     }
     __syncthreads();
-    // ---- eliminate column j from the remaining rows ----
-    const C pinv = crecip(LU[j][j]);
 #pragma unroll
-    for (int q = 0; q < RPT; ++q)
-      if (!used[q]) {
-        const C l = cmul(P[q][j], pinv);
-        P[q][j] = l;
+    for (int i = 0; i < RT; ++i) {
+      const int r = l + 64 * i;
+      if (r >= cr0 && r < cr0 + nr) {
+        const C *src = stage + (r - cr0) * LDST + w * CT;
 #pragma unroll
-        for (int jj = j + 1; jj < NB; ++jj)
-          if (jj < nbp) P[q][jj] = csub(P[q][jj], cmul(l, LU[j][jj]));
+        for (int c = 0; c < CT; ++c) P[i][c] = src[c];
       }
     }
+    __syncthreads();
   }
+
+  // the owning wave's column (register 0) -> multipliers in mcol[bb] and the
+  // pivot candidate (largest key over the wave's unpivoted rows) with its
+  // reciprocal in redk[bb] / redinv[bb]
+  auto candidate = [&](int bb) {
+    unsigned kb = 0;
+    C cv = C{0, 0};
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const int r = l + 64 * i;
+      if (r < n) mcol[bb][r] = P[i][0];
+      const unsigned key = used[i] ? 0u : pivot_key(cabs1(P[i][0]), r);
+      if (key > kb) { kb = key; cv = P[i][0]; }
+    }
+    const unsigned km = wave_max_key(kb);
+    if (km != 0 && kb == km) {  // keys of distinct rows differ: one lane
+      redk[bb] = km;
+      redinv[bb] = crecip_fast(cv);
+    }
+    if (km == 0 && l == 0) redk[bb] = 0;
+  };
+  if (w == 0) candidate(0);
   __syncthreads();
 
-  // ---- copy of the pivot rows outside the panel columns.  Element e is
-  // (pivot j = e % nbp, column c = e / nbp): a wave reads the pivot rows of a
-  // few adjacent columns.  Each thread issues all its loads before storing. ----
-  {
-    constexpr int kPer = 16;  // elements per thread per round
-    const int total = nbp * n;
-    for (int base = 0; base < total; base += NT * kPer) {
-      C v[kPer];
+  for (int j = 0; j < sw; ++j) {
+    const int bf = j & 1, wj = j / CT;
+    // ---- pivot row p and 1/B(p, j); row p published raw ----
+    const unsigned kx = redk[bf];
+    const int p = kx ? 511 - (int)(kx & 511) : -1;
+    if (tid == 0 && (kx >> 10) == 0) s_sing = 1;  // zero (or NaN) pivot
+    const C pinv = kx ? redinv[bf] : C{0, 0};
+    if (p < 0) {
+      if (tid < SWC) prow[bf][tid] = C{0, 0};  // cannot happen: j < n unpivoted rows remain
+      if (tid == 0) spiv[j] = 0;
+    } else if ((p & 63) == l) {
 #pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int e = base + tid + q * NT;
-        const int j = e % nbp, c = e / nbp;
-        v[q] = (e < total) ? Ab[piv[j] + (size_t)c * n] : C{0, 0};
-      }
+      for (int i = 0; i < RT; ++i)
+        if ((p >> 6) == i) {
+          used[i] = 1;
 #pragma unroll
-      for (int q = 0; q < kPer; ++q) {
-        const int e = base + tid + q * NT;
-        const int j = e % nbp, c = e / nbp;
-        if (e < total && (c < c0 || c >= c0 + nbp)) {
-          if (c >= s0 && c < s0 + sw) Rs[j][c - s0] = v[q];
-          else Rbb[j + (size_t)c * kInvSuper] = v[q];
+          for (int c = 0; c < CT; ++c)
+            prow[bf][w * CT + c] = (w == wj && c == 0) ? C{R(1), R(0)} : P[i][c];
         }
+      if (w == wj) spiv[j] = p;
+    }
+    __syncthreads();
+    // ---- B(r, :) -= l_r B(p, :), l_r = B(r, j) / B(p, j); old register k + 1
+    // -> register k (the owning wave's column j, taken as 0, enters the last
+    // register against the published 1) ----
+    const C *pr = &prow[bf][w * CT];
+    C lm[RT], x0[RT];
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const int r = l + 64 * i;
+      const C m = (w == wj) ? P[i][0] : (r < n ? mcol[bf][r] : C{0, 0});
+      lm[i] = cmul(m, pinv);
+      x0[i] = (w == wj) ? C{0, 0} : P[i][0];
+      if (r == p) {  // the pivot row becomes B(p, :) / B(p, j)
+        lm[i] = C{-pinv.x, -pinv.y};
+        x0[i] = C{0, 0};
+#pragma unroll
+        for (int c = 1; c < CT; ++c) P[i][c] = C{0, 0};
       }
     }
+    auto upd = [&](int k) {
+      const C y = pr[(k + 1) % CT];
+#pragma unroll
+      for (int i = 0; i < RT; ++i) {
+        C x = k + 1 < CT ? P[i][k + 1] : x0[i];
+        const R nr = -lm[i].x, ni = -lm[i].y;
+        x.x = fma(nr, y.x, x.x);
+        x.x = fma(-ni, y.y, x.x);
+        x.y = fma(nr, y.y, x.y);
+        x.y = fma(ni, y.x, x.y);
+        P[i][k] = x;
+      }
+    };
+    upd(0);  // the next pivot column of its wave lands in register 0
+    if (j + 1 < sw && w == (j + 1) / CT) candidate(bf ^ 1);
+#pragma unroll
+    for (int k = 1; k < CT; ++k) upd(k);
+    __syncthreads();
   }
-  // reciprocals of the pivots U(l,l), once
-  __shared__ C dinv[NB];
-  if (tid < nbp) dinv[tid] = crecip(LU[tid][tid]);
-  // The last panel of a super-panel also writes Zhat = B(:, S) - E for the
-  // deferred update of the other columns (E: ones at (pivot row, column)).
-  const bool last = Zh != nullptr && c0 + nbp == s0 + sw;
-  if (last && tid < c0 - s0) spiv[tid] = piv_g[(long long)b * n + s0 + tid];
-  __syncthreads();
-  C *Zhb = last ? Zh + (long long)b * n * kInvSuper : nullptr;
+  // restore the register order after a narrow super-panel
+  for (int j = sw; j % CT != 0; ++j) {
+#pragma unroll
+    for (int i = 0; i < RT; ++i) {
+      const C first = P[i][0];
+#pragma unroll
+      for (int c = 1; c < CT; ++c) P[i][c - 1] = P[i][c];
+      P[i][CT - 1] = first;
+    }
+  }
 
-  // ---- W Q = x  with Q = L U:  y U = x (forward), W L = y (backward) ----
+  // ---- write the super-panel back, CR rows at a time ----
+  for (int cr0 = 0; cr0 < n; cr0 += CR) {
+    const int nr = min(CR, n - cr0);
+    if (cr0 > 0) __syncthreads();  // the previous chunk has been read
 #pragma unroll
-  for (int q = 0; q < RPT; ++q) {
-    const int r = tid + NT * q;
-    if (r < n) {
-    const int pj = pos[q];
-    // the eliminated row is dead now: the right-hand side reuses its registers
+    for (int i = 0; i < RT; ++i) {
+      const int r = l + 64 * i;
+      if (r >= cr0 && r < cr0 + nr) {
+        C *dst = stage + (r - cr0) * LDST + w * CT;
 #pragma unroll
-    for (int l = 0; l < NB; ++l) {
-      if (pj >= 0) P[q][l] = C{(l == pj) ? R(1) : R(0), R(0)};
-      else P[q][l] = (l < nbp) ? Ab[r + (size_t)(c0 + l) * n] : C{0, 0};
-    }
-#pragma unroll
-    for (int l = 0; l < NB; ++l) {
-      if (l < nbp) {
-        C acc = P[q][l];
-#pragma unroll
-        for (int m = 0; m < l; ++m) acc = csub(acc, cmul(P[q][m], LU[m][l]));
-        P[q][l] = cmul(acc, dinv[l]);
+        for (int c = 0; c < CT; ++c) dst[c] = P[i][c];
       }
     }
+    __syncthreads();
 #pragma unroll
-    for (int l = NB - 1; l >= 0; --l) {
-      if (l < nbp) {
-        C acc = P[q][l];
-#pragma unroll
-        for (int m = l + 1; m < NB; ++m)
-          if (m < nbp) acc = csub(acc, cmul(P[q][m], LU[m][l]));
-        P[q][l] = acc;
-      }
-    }
-#pragma unroll
-    for (int l = 0; l < NB; ++l) {
-      if (l >= nbp) continue;
-      C w = P[q][l];
-      if (pj >= 0) {
-        Ab[r + (size_t)(c0 + l) * n] = w;
-        if (l == pj) w.x -= R(1);
-      } else {
-        w = cneg(w);
-        Ab[r + (size_t)(c0 + l) * n] = w;
-      }
-      P[q][l] = w;  // row r of Z
-      if (sw == 0) Zb[r + (size_t)l * n] = w;
-      if (last) Zhb[r + (size_t)(c0 - s0 + l) * n] = w;
-    }
-    // ---- B(r, J) += Z(r, :) R(:, J) for the other super-panel columns J ----
-    for (int cb = s0; cb < s0 + sw; cb += 8) {
-      C x[8];
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int c = cb + k;
-        const bool on = c < s0 + sw && (c < c0 || c >= c0 + nbp);
-        x[k] = on ? Ab[r + (size_t)c * n] : C{0, 0};
-      }
-#pragma unroll
-      for (int l = 0; l < NB; ++l) {
-        if (l < nbp) {
-          const C z = P[q][l];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) {
-            const C rv = Rs[l][(cb + k - s0) & (kInvSuper - 1)];
-            x[k].x += z.x * rv.x - z.y * rv.y;
-            x[k].y += z.x * rv.y + z.y * rv.x;
-          }
-        }
-      }
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        const int c = cb + k;
-        if (c < s0 + sw && (c < c0 || c >= c0 + nbp)) {
-          Ab[r + (size_t)c * n] = x[k];
-          if (last) {  // only earlier columns of S remain here (the panel is the last one)
-            C zh = x[k];
-            if (spiv[c - s0] == r) zh.x -= R(1);
-            Zhb[r + (size_t)(c - s0) * n] = zh;
-          }
-        }
-      }
-    }
-    ug[r] = used[q];
+    for (int k = 0; k < PER; ++k) {
+      const int e = tid + NTG * k;
+      const int row = e / SWC, col = e % SWC;
+      if (row < nr && col < sw) Xb[(size_t)(cr0 + row) * n + s0 + col] = stage[row * LDST + col];
     }
   }
-  if (tid < nbp) piv_g[(long long)b * n + c0 + tid] = piv[tid];
+#pragma unroll
+  for (int i = 0; i < RT; ++i) {
+    const int r = l + 64 * i;
+    if (w == 0 && r < n) ug[r] = used[i];
+  }
+  if (tid < sw) piv_g[(long long)b * n + s0 + tid] = spiv[tid];
+  if (Rb) {  // pivot columns of A outside the rows of S, moved to Rb; element e = (c, k)
+    C *Rbb = Rb + (long long)b * kInvSuper * n;
+    int k = tid / n, c = tid - k * n;
+    const int dk = NTG / n, dc = NTG - dk * n;
+    for (int e = tid; e < sw * n; e += NTG) {
+      if (c < s0 || c >= s0 + sw) {
+        C *src = Xb + c + (size_t)spiv[k] * n;
+        Rbb[e] = *src;
+        *src = C{0, 0};
+      }
+      c += dc;
+      k += dk;
+      if (c >= n) { c -= n; ++k; }
+    }
+  }
   if (tid == 0 && s_sing && status) atomicOr(&status[b], (int)ST_SINGULAR);
 }
 
+// Ainv(c, i) = B(ip[c], pv[i]): column i of the inverse is column pv[i] of
+// the working matrix with its rows taken in the order ip (pv: pivot row of
+// each logical column of M = A^T, ip its inverse permutation).
 template <typename R>
 __global__ __launch_bounds__(NT) void k_inv_unperm(int n, const typename CT_<R>::t *B,
                                                    long long sB, typename CT_<R>::t *Out,
@@ -418,13 +456,12 @@ __global__ __launch_bounds__(NT) void k_inv_unperm(int n, const typename CT_<R>:
   __syncthreads();
   const C *Bb = B + (long long)b * sB;
   C *Ob = Out + (long long)b * sOut;
-  // columns handled by this workgroup: blockIdx.x * 8 .. +8
-  for (int cc = 0; cc < 8; ++cc) {
-    int c = blockIdx.x * 8 + cc;
-    if (c >= n) break;
-    const int src = ip[c];
-    for (int i = threadIdx.x; i < n; i += NT)
-      Ob[i + (size_t)c * n] = Bb[pv[i] + (size_t)src * n];
+  // output columns handled by this workgroup: blockIdx.x * 8 .. +8
+  for (int ii = 0; ii < 8; ++ii) {
+    const int i = blockIdx.x * 8 + ii;
+    if (i >= n) break;
+    const C *src = Bb + (size_t)pv[i] * n;
+    for (int c = threadIdx.x; c < n; c += NT) Ob[c + (size_t)i * n] = src[ip[c]];
   }
 }
 
@@ -462,47 +499,40 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
     HN_CHECK(hipGetLastError());
     return;
   }
-  typedef typename CT_<R>::t C;
-  (void)sizeof(C);
   if (n > 512) throw std::runtime_error("inverse: block order > 512 is not supported");
   if (sA < (long long)n * n || sAinv < (long long)n * n)
     throw std::runtime_error("inverse: item stride smaller than n*n");
-  if (!w.Z || w.n < n || w.batch < batch) throw std::runtime_error("inverse: workspace too small");
-  // panel width: FP32 32 up to n = 256; FP64 16 (a 32-wide FP64 panel row
-  // exceeds the register file); rows per thread RPT = 1 up to n = 256, else 2
-  const int NB = (sizeof(R) == 4 && n <= NT) ? 32 : 16;
-  long long nb_tot = (long long)n * batch;
-  hipLaunchKernelGGL(k_inv_reset<R>, dim3((unsigned)((nb_tot + NT - 1) / NT)), dim3(NT), 0, s,
-                     n, batch, w.used, active);
-  for (int s0 = 0; s0 < n; s0 += kInvSuper) {
-    const int sw = std::min(kInvSuper, n - s0);
-    for (int c0 = s0; c0 < s0 + sw; c0 += NB) {
-      const int nbp = std::min(NB, s0 + sw - c0);
-      {
-        ProfScope ps(PT_INV_PANEL, -1.0, 0.0, s);
-        if (n <= NT && NB == 32)
-          hipLaunchKernelGGL((k_inv_panel_reg<R, 32, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                             c0 - s0, s0, sw, A, sA, w.Z, n > sw ? w.Zh : nullptr, w.Rb, w.used,
-                             w.piv, active, status);
-        else if (n <= NT)
-          hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 1>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                             c0 - s0, s0, sw, A, sA, w.Z, n > sw ? w.Zh : nullptr, w.Rb, w.used,
-                             w.piv, active, status);
-        else
-          hipLaunchKernelGGL((k_inv_panel_reg<R, 16, 2>), dim3(batch), dim3(NT), 0, s, n, c0, nbp,
-                             c0 - s0, s0, sw, A, sA, w.Z, n > sw ? w.Zh : nullptr, w.Rb, w.used,
-                             w.piv, active, status);
+  if (!w.Rb || w.n < n || w.batch < batch) throw std::runtime_error("inverse: workspace too small");
+  // register tile per thread: 4 rows x 8 columns (super-panel 64) up to
+  // n = 256, 8 rows x 4 columns (super-panel 32) up to n = 512
+  const int rpt = n <= 256 ? 1 : 2;
+  const int spw = rpt == 1 ? 64 : 32;
+  for (int s0 = 0; s0 < n; s0 += spw) {
+    const int sw = std::min(spw, n - s0);
+    typename CT_<R>::t *rb = n > sw ? w.Rb : nullptr;
+    {
+      ProfScope ps(PT_INV_PANEL, -1.0, 0.0, s);
+      if (rpt == 1) {
+        allow_lds(k_inv_gj<R, 4, 8>, gj_stage_bytes<R, 4, 8>());
+        hipLaunchKernelGGL((k_inv_gj<R, 4, 8>), dim3(batch), dim3(NTG),
+                           (gj_stage_bytes<R, 4, 8>()), s, n, s0, sw, A, sA, rb, w.used,
+                           w.piv, active, status);
+      } else {
+        allow_lds(k_inv_gj<R, 8, 4>, gj_stage_bytes<R, 8, 4>());
+        hipLaunchKernelGGL((k_inv_gj<R, 8, 4>), dim3(batch), dim3(NTG),
+                           (gj_stage_bytes<R, 8, 4>()), s, n, s0, sw, A, sA, rb, w.used,
+                           w.piv, active, status);
       }
-      HN_CHECK(hipGetLastError());
     }
-    if (n > sw) {  // deferred rank-sw update of every column outside the super-panel
+    HN_CHECK(hipGetLastError());
+    if (rb) {  // deferred rank-sw update of every row of A outside the super-panel
       GemmArgs<R> g;
       g.m = n; g.n = n; g.k = sw;
-      g.A = w.Zh; g.sA = (long long)n * kInvSuper; g.lda = n; g.opA = OP_N;
-      g.B = w.Rb; g.sB = (long long)kInvSuper * n; g.ldb = kInvSuper; g.opB = OP_N;
+      g.A = rb; g.sA = (long long)kInvSuper * n; g.lda = n; g.opA = OP_N;
+      g.B = A + s0; g.sB = sA; g.ldb = n; g.opB = OP_N;
       g.C = A; g.sC = sA; g.ldc = n;
       g.alpha = {1.0, 0.0}; g.beta = {1.0, 0.0};
-      g.active = active; g.skip_c0 = s0; g.skip_c1 = s0 + sw;
+      g.active = active; g.skip_r0 = s0; g.skip_r1 = s0 + sw;
       g.batch = batch; g.tag = PT_GEMM_INV; g.prof_items = w.prof_items;
       if constexpr (sizeof(R) == 8) gemm_z(g, s);
       else gemm_c(g, s);
@@ -520,8 +550,7 @@ int inv_lds_max_n(int real_bytes) { return real_bytes == 8 ? lds_max_n<double>()
 size_t inv_work_bytes(int n, int batch, int real_bytes) {
   size_t c = 2 * (size_t)real_bytes;
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  return al((size_t)n * kInvPanel * c * batch) + 2 * al((size_t)n * kInvSuper * c * batch) +
-         al((size_t)n * batch * sizeof(int)) * 2;
+  return al((size_t)n * kInvSuper * c * batch) + al((size_t)n * batch * sizeof(int)) * 2;
 }
 
 template <typename R>
@@ -530,8 +559,6 @@ InvWork<R> inv_work_carve(void *buf, int n, int batch) {
   auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
   InvWork<R> w;
   char *p = (char *)buf;
-  w.Z = (C *)p; p += al((size_t)n * kInvPanel * sizeof(C) * batch);
-  w.Zh = (C *)p; p += al((size_t)n * kInvSuper * sizeof(C) * batch);
   w.Rb = (C *)p; p += al((size_t)n * kInvSuper * sizeof(C) * batch);
   w.used = (int *)p; p += al((size_t)n * batch * sizeof(int));
   w.piv = (int *)p;
