@@ -165,6 +165,24 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
       ci[i][j] = acc_t{0};
     }
 
+  // C values of this thread's epilogue elements: the first pass's fetched
+  // before the K loop, the second's at the start of the epilogue, so that
+  // their latency overlaps the products
+  CT *C = p.C + (long long)b * p.sC;
+  const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
+  constexpr int EQ = (64 * 32) / NT;
+  CT cpre[2][EQ];
+  auto fetch_c = [&](int j) {
+#pragma unroll
+    for (int q = 0; q < EQ; ++q) {
+      const int e = tid + q * NT;
+      const int gr = m0 + (e & 63), sc = e >> 6;
+      const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
+      cpre[j][q] = (useb && gr < M && gc < N) ? C[gr + (long long)gc * p.ldc] : CT{0, 0};
+    }
+  };
+  fetch_c(0);
+
   int cur = 0;
   if (K > 0) {
     load(0);
@@ -249,11 +267,10 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   static_assert(32 * EL <= 2 * BK * (BM + PAD), "epilogue slab does not fit");  // Ar+Ai, Br+Bi
   R *Er = &sm.Ar[0][0];
   R *Ei = &sm.Br[0][0];
-  CT *C = p.C + (long long)b * p.sC;
   CT *C2 = (p.C2 && (!p.active2 || p.active2[b] != 0)) ? p.C2 + (long long)b * p.sC2 : nullptr;
   const double ar_ = p.alpha.x, ai_ = p.alpha.y;
-  const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
   const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
+  fetch_c(1);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
     if constexpr (M4) {
@@ -280,7 +297,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
     __syncthreads();
     // slab column sc holds tile column (sc / 16) * 32 + j * 16 + sc % 16
 #pragma unroll
-    for (int q = 0; q < (64 * 32) / NT; ++q) {
+    for (int q = 0; q < EQ; ++q) {
       const int e = tid + q * NT;
       const int row = e & 63, sc = e >> 6;
       const int gr = m0 + row;
@@ -292,7 +309,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
         const long long o = gr + (long long)gc * p.ldc;
         double yr = vr, yi = vi;
         if (useb) {
-          const CT c = C[o];
+          const CT c = cpre[j][q];
           yr += p.beta.x * (double)c.x - p.beta.y * (double)c.y;
           yi += p.beta.x * (double)c.y + p.beta.y * (double)c.x;
         }

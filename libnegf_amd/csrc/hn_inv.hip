@@ -192,6 +192,16 @@ __device__ inline double2 crecip_fast(double2 b) {
 }
 __device__ inline float2 crecip_fast(float2 b) { return crecip(b); }
 
+// value held by lane `lane` (uniform) of the wave
+__device__ inline double lane_read(double v, int lane) {
+  const int lo = __builtin_amdgcn_readlane(__double2loint(v), lane);
+  const int hi = __builtin_amdgcn_readlane(__double2hiint(v), lane);
+  return __hiloint2double(hi, lo);
+}
+__device__ inline float lane_read(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+
 // rows of the LDS staging buffer of k_inv_gj (a chunk of the n x 8CT panel,
 // rows padded by one element)
 template <typename R, int RT, int CT>
@@ -216,16 +226,16 @@ constexpr size_t gj_stage_bytes() {
 // current pivot column always finds it in register 0 and no register is
 // indexed at run time; CT steps restore the order.
 //
-// Step j, pivot column owned by wave wj = j / CT:
-//   [after barrier A] every thread reads the pivot key (row p, the largest
-//   |.|_1 over the unpivoted rows) and 1/B(p, j); the eight threads holding
-//   row p publish it raw (1 in place of column j);
-//   [after barrier B] every row applies B(r, :) -= (B(r, j)/B(p, j)) B(p, :)
-//   and the pivot row becomes B(p, :)/B(p, j) (1/B(p, j) in column j).  The
-//   wave owning the next pivot column updates that column first and, before
-//   its other columns, publishes the column (the next multipliers) and its
-//   pivot candidate with the reciprocal, so the search overlaps the update.
-// LDS buffers alternate between steps.
+// Step j, pivot column owned by wave wj = j / CT, one barrier:
+//   every thread reads the pivot key (row p, the largest |.|_1 over the
+//   unpivoted rows) and 1/B(p, j); each wave reads its CT entries of row p
+//   from the lane holding them (1 in place of column j); every row applies
+//   B(r, :) -= (B(r, j)/B(p, j)) B(p, :), the multipliers of the other waves'
+//   rows coming from LDS, and the pivot row becomes B(p, :)/B(p, j) (1/B(p, j)
+//   in column j).  The wave owning the next pivot column updates that column
+//   first and, before its other columns, publishes the column (the next
+//   multipliers) and its pivot candidate with the reciprocal, so the search
+//   overlaps the update.  LDS buffers alternate between steps.
 //
 // With Ẑ = B(:, S) - E (E: ones at (pivot row, column)) the columns J outside
 // S then take B(:, J) += Ẑ B_old(piv, J).  In A's column-major layout that is
@@ -250,7 +260,6 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
   if (active && active[b] == 0) return;
   extern __shared__ __align__(16) unsigned char gsm[];
   C *stage = (C *)gsm;
-  __shared__ C prow[2][SWC];
   __shared__ C mcol[2][64 * RT];
   __shared__ unsigned redk[2];
   __shared__ C redinv[2];
@@ -318,30 +327,28 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
 
   for (int j = 0; j < sw; ++j) {
     const int bf = j & 1, wj = j / CT;
-    // ---- pivot row p and 1/B(p, j); row p published raw ----
+    // ---- pivot row p and 1/B(p, j); each wave reads its columns of row p
+    // from the lane holding it ----
     const unsigned kx = redk[bf];
     const int p = kx ? 511 - (int)(kx & 511) : -1;
     if (tid == 0 && (kx >> 10) == 0) s_sing = 1;  // zero (or NaN) pivot
     const C pinv = kx ? redinv[bf] : C{0, 0};
-    if (p < 0) {
-      if (tid < SWC) prow[bf][tid] = C{0, 0};  // cannot happen: j < n unpivoted rows remain
-      if (tid == 0) spiv[j] = 0;
-    } else if ((p & 63) == l) {
+    const int lp = p & 63, ip = p >> 6;  // p < 0 cannot happen: j < n unpivoted rows remain
+    C y[CT];
 #pragma unroll
-      for (int i = 0; i < RT; ++i)
-        if ((p >> 6) == i) {
-          used[i] = 1;
+    for (int c = 0; c < CT; ++c) y[c] = C{0, 0};
 #pragma unroll
-          for (int c = 0; c < CT; ++c)
-            prow[bf][w * CT + c] = (w == wj && c == 0) ? C{R(1), R(0)} : P[i][c];
-        }
-      if (w == wj) spiv[j] = p;
-    }
-    __syncthreads();
+    for (int i = 0; i < RT; ++i)
+      if (p >= 0 && ip == i) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) y[c] = C{lane_read(P[i][c].x, lp), lane_read(P[i][c].y, lp)};
+        if (l == lp) used[i] = 1;
+      }
+    if (w == wj) y[0] = C{R(1), R(0)};  // column j itself
+    if (w == wj && l == 0) spiv[j] = p < 0 ? 0 : p;
     // ---- B(r, :) -= l_r B(p, :), l_r = B(r, j) / B(p, j); old register k + 1
     // -> register k (the owning wave's column j, taken as 0, enters the last
-    // register against the published 1) ----
-    const C *pr = &prow[bf][w * CT];
+    // register against the 1 in y[0]) ----
     C lm[RT], x0[RT];
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
@@ -357,15 +364,15 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
       }
     }
     auto upd = [&](int k) {
-      const C y = pr[(k + 1) % CT];
+      const C yk = y[(k + 1) % CT];
 #pragma unroll
       for (int i = 0; i < RT; ++i) {
         C x = k + 1 < CT ? P[i][k + 1] : x0[i];
         const R nr = -lm[i].x, ni = -lm[i].y;
-        x.x = fma(nr, y.x, x.x);
-        x.x = fma(-ni, y.y, x.x);
-        x.y = fma(nr, y.y, x.y);
-        x.y = fma(ni, y.x, x.y);
+        x.x = fma(nr, yk.x, x.x);
+        x.x = fma(-ni, yk.y, x.x);
+        x.y = fma(nr, yk.y, x.y);
+        x.y = fma(ni, yk.x, x.y);
         P[i][k] = x;
       }
     };
