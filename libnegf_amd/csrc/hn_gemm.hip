@@ -439,6 +439,22 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
       cr[i][q] = 0.0;
       ci[i][q] = 0.0;
     }
+  // C values of this thread's epilogue elements: the first pass's fetched
+  // before the K loop, the second's at the start of the epilogue
+  CT *C = p.C + (long long)b * p.sC;
+  const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
+  constexpr int EQ = (64 * 64) / NT;
+  CT cpre[2][EQ];
+  auto fetch_c = [&](int j) {
+#pragma unroll
+    for (int q = 0; q < EQ; ++q) {
+      const int e = tid + q * NT;
+      const int gr = m0 + (e & 63), sc = e >> 6;
+      const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
+      cpre[j][q] = (useb && gr < M && gc < N) ? C[gr + (long long)gc * p.ldc] : CT{0, 0};
+    }
+  };
+  fetch_c(0);
   int cur = 0;
   if (K > 0) {
     load(0);
@@ -497,11 +513,10 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
   static_assert(2 * 64 * EL * sizeof(double) <= sizeof(SmemW2), "epilogue slab does not fit");
   double *Er = &sm2.buf[0].Ar[0][0];
   double *Ei = Er + 64 * EL;
-  CT *C = p.C + (long long)b * p.sC;
   CT *C2 = (p.C2 && (!p.active2 || p.active2[b] != 0)) ? p.C2 + (long long)b * p.sC2 : nullptr;
   const double ar_ = p.alpha.x, ai_ = p.alpha.y;
-  const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
   const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
+  fetch_c(1);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
 #pragma unroll
@@ -516,7 +531,7 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
     __syncthreads();
     // slab column sc holds tile column (sc / 16) * 32 + j * 16 + sc % 16
 #pragma unroll
-    for (int q = 0; q < (64 * 64) / NT; ++q) {
+    for (int q = 0; q < EQ; ++q) {
       const int e = tid + q * NT;
       const int row = e & 63, sc = e >> 6;
       const int gr = m0 + row;
@@ -528,7 +543,7 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
         const long long o = gr + (long long)gc * p.ldc;
         double yr = vr, yi = vi;
         if (useb) {
-          const CT c = C[o];
+          const CT c = cpre[j][q];
           yr += p.beta.x * c.x - p.beta.y * c.y;
           yi += p.beta.x * c.y + p.beta.y * c.x;
         }
