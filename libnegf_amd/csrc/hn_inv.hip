@@ -279,27 +279,39 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
     for (int c = 0; c < CT; ++c) P[i][c] = C{0, 0};
   }
   if (tid == 0) s_sing = 0;
-  // ---- load the panel, CR rows at a time: coalesced along rows of M ----
-  for (int cr0 = 0; cr0 < n; cr0 += CR) {
-    const int nr = min(CR, n - cr0);
+  // ---- load the panel (at most two chunks of CR rows, both chunks' global
+  // loads issued first), coalesced along rows of M ----
+  static_assert(2 * CR >= 64 * RT, "two staging chunks cover the panel");
+  C ld[2][PER];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = tid + NTG * k;
-      const int row = e / SWC, col = e % SWC;
-      if (row < nr)
-        stage[row * LDST + col] = col < sw ? Xb[(size_t)(cr0 + row) * n + s0 + col] : C{0, 0};
+      const int row = h * CR + e / SWC, col = e % SWC;
+      ld[h][k] = (row < n && col < sw) ? Xb[(size_t)row * n + s0 + col] : C{0, 0};
     }
-    __syncthreads();
 #pragma unroll
-    for (int i = 0; i < RT; ++i) {
-      const int r = l + 64 * i;
-      if (r >= cr0 && r < cr0 + nr) {
-        const C *src = stage + (r - cr0) * LDST + w * CT;
+  for (int h = 0; h < 2; ++h) {
+    const int cr0 = h * CR;
+    if (cr0 < n) {  // uniform
 #pragma unroll
-        for (int c = 0; c < CT; ++c) P[i][c] = src[c];
+      for (int k = 0; k < PER; ++k) {
+        const int e = tid + NTG * k;
+        stage[(e / SWC) * LDST + e % SWC] = ld[h][k];
       }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < RT; ++i) {
+        const int r = l + 64 * i;
+        if (r >= cr0 && r < cr0 + CR) {
+          const C *src = stage + (r - cr0) * LDST + w * CT;
+#pragma unroll
+          for (int c = 0; c < CT; ++c) P[i][c] = src[c];
+        }
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 
   // the owning wave's column (register 0) -> multipliers in mcol[bb] and the
@@ -422,17 +434,25 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
   if (tid < sw) piv_g[(long long)b * n + s0 + tid] = spiv[tid];
   if (Rb) {  // pivot columns of A outside the rows of S, moved to Rb; element e = (c, k)
     C *Rbb = Rb + (long long)b * kInvSuper * n;
-    int k = tid / n, c = tid - k * n;
-    const int dk = NTG / n, dc = NTG - dk * n;
-    for (int e = tid; e < sw * n; e += NTG) {
-      if (c < s0 || c >= s0 + sw) {
-        C *src = Xb + c + (size_t)spiv[k] * n;
-        Rbb[e] = *src;
-        *src = C{0, 0};
+    constexpr int GU = 8;  // loads in flight per thread
+    const float inv_n = 1.0f / (float)n;  // e / n below is exact: e < 2^15, n <= 512
+    const int tot = sw * n;
+    for (int base = 0; base < tot; base += NTG * GU) {
+      C v[GU];
+      long long o[GU];
+#pragma unroll
+      for (int u = 0; u < GU; ++u) {
+        const int e = base + tid + NTG * u;
+        const int k = (int)(((float)e + 0.5f) * inv_n), c = e - k * n;
+        o[u] = (e < tot && (c < s0 || c >= s0 + sw)) ? c + (long long)spiv[k] * n : -1;
+        v[u] = o[u] >= 0 ? Xb[o[u]] : C{0, 0};
       }
-      c += dc;
-      k += dk;
-      if (c >= n) { c -= n; ++k; }
+#pragma unroll
+      for (int u = 0; u < GU; ++u)
+        if (o[u] >= 0) {
+          Rbb[base + tid + NTG * u] = v[u];
+          Xb[o[u]] = C{0, 0};
+        }
     }
   }
   if (tid == 0 && s_sing && status) atomicOr(&status[b], (int)ST_SINGULAR);
