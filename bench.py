@@ -77,15 +77,26 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": n, "copy_bytes_written": n, "launches": 5}), flush=True)
 
 
-def traffic_from_profiles(kernel_tag: str):
+def traffic_from_profiles(kernel_tag: str, workload_tag):
     """HBM bytes per launch of the dominant kernel from the committed PMC summary
-    (profiles/pmc_<kernel>_*.txt, line 'traffic_bytes_per_launch: <x>'), or None."""
-    import glob
-    for f in sorted(glob.glob(os.path.join(ROOT, "profiles", f"pmc_{kernel_tag}_*.txt"))):
+    of this workload (profiles/pmc_<kernel>_<workload>.txt, line
+    'traffic_bytes_per_launch: <x>'), or None."""
+    if not workload_tag:
+        return None, None
+    f = os.path.join(ROOT, "profiles", f"pmc_{kernel_tag}_{workload_tag}.txt")
+    if os.path.exists(f):
         for line in open(f):
             if line.startswith("traffic_bytes_per_launch:"):
                 return float(line.split(":", 1)[1]), os.path.basename(f)
     return None, None
+
+
+def kernel_label(tag_name: str, nmax: int) -> str:
+    """Kernel behind a profile tag: the FP64 GEMMs of blocks below 128 run the
+    64 x 64 tiling k_gemm<double, U> instead of the 64 x 128 k_gemm_w<U>."""
+    if nmax < 128 and tag_name.startswith("k_gemm_w<"):
+        return "k_gemm<double, " + tag_name[len("k_gemm_w<"):]
+    return tag_name
 
 
 def log(*a):
@@ -396,6 +407,7 @@ def main():
         return
 
     # ------------------------------------------------------------ workload ----
+    pmc_workload = None
     if a.config == "c3":
         slab, kgrid, Egrid, eta = wl.config3(nbl=a.nbl or None)
         nsteps = a.warmup + a.steps
@@ -411,6 +423,9 @@ def main():
         workload = dict(workload="kslab_n256_nbl40_k16x16_E500", n=slab.n, nbl=s0.nbl,
                         kgrid="16x16", nE=len(Egrid), eta=eta, overlap=True,
                         items_per_step=per_step, lanes=a.lanes)
+        # the committed PMC summaries were taken on exactly this launch shape
+        if s0.nbl == 40 and per_step == 500 and prec == hip.FP64 and a.lanes == 1:
+            pmc_workload = "kslab_n256_nbl40_E500"
     else:
         s0, Egrid, eta = wl.config1() if a.config == "c1" else wl.config2(nbl=a.nbl or None)
         ds = hip.DeviceStructure(s0)
@@ -463,7 +478,8 @@ def main():
     roofline, kern = None, None
     if not a.no_profile:
         hip.profile_enable(False)
-        kern = {hip.PT_NAMES[t]: hip.profile_read(t) for t in hip.PT_NAMES}
+        nmax = int(max(s0.bsz))
+        kern = {kernel_label(hip.PT_NAMES[t], nmax): hip.profile_read(t) for t in hip.PT_NAMES}
         hip.profile_clear()
         for k, v in kern.items():
             log(f"  {k:22s} launches {v['launches']:6d}  total {v['ms']:9.2f} ms  "
@@ -478,7 +494,7 @@ def main():
             fl = v["flops"] / v["launches"]
             achieved = fl / avg_s / 1e12
             tag = re.sub(r"[<>, ]+", "_", name.replace("double", "")).strip("_").replace("__", "_")
-            traffic, tsrc = traffic_from_profiles(tag)
+            traffic, tsrc = traffic_from_profiles(tag, pmc_workload)
             roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
                         "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS,
                         "traffic": traffic, "kernel": name,
