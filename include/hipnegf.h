@@ -137,8 +137,17 @@ enum hipnegf_block_kind {
   HIPNEGF_S00 = 7,
   HIPNEGF_H01 = 8, /* contact c: PL1 -> PL2 (away from the device)          */
   HIPNEGF_S01 = 9,
-  HIPNEGF_HMC = 10, /* contact c: device block cblk -> PL1   bsz x nc        */
-  HIPNEGF_SMC = 11
+  HIPNEGF_HMC = 10, /* contact c: device block cblk -> surface + PL1
+                       bsz x (ns + nc)                                       */
+  HIPNEGF_SMC = 11,
+  /* contact c with a surface block of ns orbitals before its two PLs
+   * (surface_green, contselfenergy.F90:186-239): */
+  HIPNEGF_HSS = 12, /* surface                       ns x ns                 */
+  HIPNEGF_SSS = 13,
+  HIPNEGF_HS1 = 14, /* surface -> PL1                ns x nc                 */
+  HIPNEGF_SS1 = 15,
+  HIPNEGF_H1S = 16, /* PL1 -> surface                nc x ns                 */
+  HIPNEGF_S1S = 17
 };
 /* smode bits: 1 = S(i,i) given (else identity), 2 = S(i,i+1)/S(i+1,i) given
  * (else zero), 4 = contact S blocks given (else S00 = I, S01 = SMC = 0). */
@@ -162,6 +171,14 @@ const char *hipnegf_last_error(void);
  * cblk[ncont] with PL size nc[ncont], nslots identical slots (e.g. one per k). */
 int hipnegf_struct_create(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
                           int smode, int nslots, hipnegf_struct *st);
+/* Same with contact surface blocks: ns[ncont] orbitals between the device and
+ * each contact's first PL (mat_C_start - mat_B_start, ln_structure.F90:119-123;
+ * 0: none).  The contact's surface Green's function is then of order ns + nc:
+ * GS = inv([[z Sss - Hss, z Ss1 - Hs1], [z S1s - H1s, inv(g_PL)]])
+ * (contselfenergy.F90:220-239). */
+int hipnegf_struct_create_surf(int nbl, const int *bsz, int ncont, const int *cblk,
+                               const int *nc, const int *ns, int smode, int nslots,
+                               hipnegf_struct *st);
 int hipnegf_struct_destroy(hipnegf_struct st);
 /* copy one block (column-major interleaved complex double) from host memory */
 int hipnegf_struct_set_block(hipnegf_struct st, int slot, int kind, int index,

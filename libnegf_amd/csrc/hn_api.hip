@@ -172,7 +172,9 @@ hn::Engine &engine(Session &s) {
 }
 
 // Device structure of the working H/S: the tridiagonal PL blocks
-// (zcsr2blk_sod) and the contact blocks (extract_cont), no surface blocks.
+// (zcsr2blk_sod) and the contact blocks (extract_cont, ln_extract.F90:41-76):
+// the surface block (surfstart..surfend, may be empty) and the two PLs after
+// it; HMC/SMC span the surface and the first PL.
 std::unique_ptr<hn::DevStruct> build_struct(Session &s) {
   if (!s.structure_set) throw std::runtime_error("structure not initialised (negf_init_structure)");
   if (s.cur >= (int)s.H.size() || !s.H[s.cur].present()) throw std::runtime_error("H not set");
@@ -186,19 +188,19 @@ std::unique_ptr<hn::DevStruct> build_struct(Session &s) {
     ple[i] = s.plend[i];
     bsz[i] = ple[i] - pls[i] + 1;
   }
-  std::vector<int> nc(s.ncont);
+  std::vector<int> nc(s.ncont), ns(s.ncont);
   for (int c = 0; c < s.ncont; ++c) {
-    const int surfdim = (s.surfend[c] + 1) - s.surfstart[c];
-    if (surfdim != 0)
-      throw std::runtime_error("contacts with a surface block (surfend >= surfstart) are not supported");
-    const int ncdim = s.contend[c] - s.surfstart[c] + 1;
+    const int surfdim = (s.surfend[c] + 1) - s.surfstart[c];  // mat_C_start - mat_B_start
+    if (surfdim < 0) throw std::runtime_error("contact surface end before its start");
+    const int ncdim = s.contend[c] - s.surfend[c];           // the two PLs after the surface
     if (ncdim <= 0 || ncdim % 2) throw std::runtime_error("contact must hold 2 principal layers");
     nc[c] = ncdim / 2;
+    ns[c] = surfdim;
     if (s.cblk[c] < 1 || s.cblk[c] > nbl) throw std::runtime_error("cblk out of range");
   }
   const int smode = sid ? 0 : (hn::S_DIAG | hn::S_OFFDIAG | hn::S_CONTACT);
-  std::unique_ptr<hn::DevStruct> st(
-      new hn::DevStruct(nbl, bsz.data(), s.ncont, s.cblk.data(), nc.data(), smode, 1));
+  std::unique_ptr<hn::DevStruct> st(new hn::DevStruct(nbl, bsz.data(), s.ncont, s.cblk.data(),
+                                                      nc.data(), smode, 1, ns.data()));
   auto put = [&](int kind, int idx, const std::vector<double2> &h) {
     const long long n = st->block_elems(kind, idx);
     if ((long long)h.size() != n) throw std::runtime_error("block size mismatch");
@@ -218,15 +220,25 @@ std::unique_ptr<hn::DevStruct> build_struct(Session &s) {
     }
   }
   for (int c = 0; c < s.ncont; ++c) {
-    const int c1 = s.surfstart[c], n = nc[c];
+    const int c0 = s.surfstart[c], q = ns[c], c1 = c0 + q, n = nc[c];  // surface at c0, PL1 at c1
     const int b = s.cblk[c] - 1;
     put(hn::BK_H00, c, extract(H, c1, c1 + n - 1, c1, c1 + n - 1));
     put(hn::BK_H01, c, extract(H, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1));
-    put(hn::BK_HMC, c, extract(H, pls[b], ple[b], c1, c1 + n - 1));
+    put(hn::BK_HMC, c, extract(H, pls[b], ple[b], c0, c1 + n - 1));
+    if (q > 0) {
+      put(hn::BK_HSS, c, extract(H, c0, c1 - 1, c0, c1 - 1));
+      put(hn::BK_HS1, c, extract(H, c0, c1 - 1, c1, c1 + n - 1));
+      put(hn::BK_H1S, c, extract(H, c1, c1 + n - 1, c0, c1 - 1));
+    }
     if (S) {
       put(hn::BK_S00, c, extract(*S, c1, c1 + n - 1, c1, c1 + n - 1));
       put(hn::BK_S01, c, extract(*S, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1));
-      put(hn::BK_SMC, c, extract(*S, pls[b], ple[b], c1, c1 + n - 1));
+      put(hn::BK_SMC, c, extract(*S, pls[b], ple[b], c0, c1 + n - 1));
+      if (q > 0) {
+        put(hn::BK_SSS, c, extract(*S, c0, c1 - 1, c0, c1 - 1));
+        put(hn::BK_SS1, c, extract(*S, c0, c1 - 1, c1, c1 + n - 1));
+        put(hn::BK_S1S, c, extract(*S, c1, c1 + n - 1, c0, c1 - 1));
+      }
     }
   }
   return st;
@@ -653,7 +665,7 @@ SparseAcc place_blocks(Session &s, const hn::DevStruct &st, const std::vector<do
   }
   const Csr &H = s.H[s.cur];
   for (int k = 0; k < ncont; ++k) {
-    const int cb = st.cblk[k] - 1, nd = st.bsz[cb], nc = st.nc[k];
+    const int cb = st.cblk[k] - 1, nd = st.bsz[cb], nc = st.ngs(k);  // surface + PL1 columns
     const int r0 = start[cb], c0 = s.surfstart[k] - 1;
     const std::vector<double2> hm = extract(H, r0 + 1, r0 + nd, c0 + 1, c0 + nc);
     std::vector<double2> sm;

@@ -439,6 +439,16 @@ int hipnegf_struct_create(int nbl, const int *bsz, int ncont, const int *cblk, c
     return 0;
   });
 }
+int hipnegf_struct_create_surf(int nbl, const int *bsz, int ncont, const int *cblk,
+                               const int *nc, const int *ns, int smode, int nslots,
+                               hipnegf_struct *st) {
+  return guard([&] {
+    auto *x = new hipnegf_struct_s;
+    x->st.reset(new hn::DevStruct(nbl, bsz, ncont, cblk, nc, smode, nslots, ns));
+    *st = x;
+    return 0;
+  });
+}
 int hipnegf_struct_destroy(hipnegf_struct st) {
   return guard([&] { delete st; return 0; });
 }

@@ -15,6 +15,9 @@ namespace hn {
 enum BlockKind : int {
   BK_HD = 0, BK_SD, BK_HU, BK_SU, BK_HL, BK_SL,  // device: diag, upper (i,i+1), lower (i+1,i)
   BK_H00, BK_S00, BK_H01, BK_S01, BK_HMC, BK_SMC, // contacts
+  // contact surface block (ns orbitals before the two PLs): surface, surface
+  // -> PL1 and PL1 -> surface; HMC/SMC then span surface + PL1
+  BK_HSS, BK_SSS, BK_HS1, BK_SS1, BK_H1S, BK_S1S,
   BK_COUNT
 };
 
@@ -26,6 +29,9 @@ enum SMode : int { S_DIAG = 1, S_OFFDIAG = 2, S_CONTACT = 4 };
 struct DevStruct {
   int nbl = 0, ncont = 0, smode = 0, nslots = 0;
   std::vector<int> bsz, cblk, nc;
+  std::vector<int> ns;  // contact surface block sizes (0: none)
+  // order of the contact's surface Green's function: surface + one PL
+  int ngs(int c) const { return nc[c] + ns[c]; }
   std::vector<long long> off[BK_COUNT];
   long long slot_elems = 0;
   double2 *pool = nullptr;
@@ -56,14 +62,14 @@ struct DevStruct {
   bool owned = true;       // pool allocated (and freed) by this object
 
   DevStruct(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc, int smode,
-            int nslots);
+            int nslots, const int *ns = nullptr);
   // Same layout on caller-provided memory of at least layout_bytes(...) bytes,
   // zeroed on `zs` (stream-ordered; not freed by the destructor).
   DevStruct(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc, int smode,
-            int nslots, void *external, hipStream_t zs);
+            int nslots, void *external, hipStream_t zs, const int *ns = nullptr);
   ~DevStruct();
   static size_t layout_bytes(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
-                             int smode, int nslots);
+                             int smode, int nslots, const int *ns = nullptr);
   long long block_elems(int kind, int idx) const;
   BlkRef ref(int kind, int idx) const {
     BlkRef r;

@@ -32,7 +32,8 @@ namespace hn {
 namespace {
 // Block offsets of one slot (256-byte aligned blocks); returns the slot size.
 long long layout(int nbl, const std::vector<int> &bsz, int ncont, const std::vector<int> &cblk,
-                 const std::vector<int> &nc, int smode, std::vector<long long> *off) {
+                 const std::vector<int> &nc, const std::vector<int> &ns, int smode,
+                 std::vector<long long> *off) {
   long long o = 0;
   auto put = [&](int kind, long long n) {
     if (off) off[kind].push_back(o);
@@ -48,14 +49,21 @@ long long layout(int nbl, const std::vector<int> &bsz, int ncont, const std::vec
     for (int i = 0; i + 1 < nbl; ++i) put(BK_SL, (long long)bsz[i] * bsz[i + 1]);
   }
   for (int c = 0; c < ncont; ++c) {
-    long long n2 = (long long)nc[c] * nc[c], nm = (long long)bsz[cblk[c] - 1] * nc[c];
+    long long n2 = (long long)nc[c] * nc[c], nm = (long long)bsz[cblk[c] - 1] * (nc[c] + ns[c]);
+    long long q2 = (long long)ns[c] * ns[c], q1 = (long long)ns[c] * nc[c];
     put(BK_H00, n2);
     put(BK_H01, n2);
     put(BK_HMC, nm);
+    put(BK_HSS, q2);
+    put(BK_HS1, q1);
+    put(BK_H1S, q1);
     if (smode & S_CONTACT) {
       put(BK_S00, n2);
       put(BK_S01, n2);
       put(BK_SMC, nm);
+      put(BK_SSS, q2);
+      put(BK_SS1, q1);
+      put(BK_S1S, q1);
     }
   }
   return o;
@@ -67,33 +75,36 @@ unsigned long long next_uid() {
 }
 
 void check_shape(int nbl, const std::vector<int> &bsz, int ncont, const std::vector<int> &cblk,
-                 const std::vector<int> &nc, int nslots) {
+                 const std::vector<int> &nc, const std::vector<int> &ns, int nslots) {
   if (nbl < 1) throw std::runtime_error("structure: nbl < 1");
   if (ncont < 0 || ncont > kMaxCont) throw std::runtime_error("structure: bad ncont");
   if (nslots < 1) throw std::runtime_error("structure: nslots < 1");
   for (int i = 0; i < nbl; ++i)
     if (bsz[i] < 1) throw std::runtime_error("structure: empty block");
   for (int c = 0; c < ncont; ++c)
-    if (cblk[c] < 1 || cblk[c] > nbl || nc[c] < 1)
+    if (cblk[c] < 1 || cblk[c] > nbl || nc[c] < 1 || ns[c] < 0)
       throw std::runtime_error("structure: bad contact description");
 }
 }  // namespace
 
 size_t DevStruct::layout_bytes(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
-                               int smode, int nslots) {
+                               int smode, int nslots, const int *ns) {
   std::vector<int> b(bsz, bsz + nbl), cb(cblk, cblk + ncont), n(nc, nc + ncont);
-  check_shape(nbl, b, ncont, cb, n, nslots);
-  return (size_t)layout(nbl, b, ncont, cb, n, smode, nullptr) * nslots * sizeof(double2);
+  std::vector<int> q = ns ? std::vector<int>(ns, ns + ncont) : std::vector<int>(ncont, 0);
+  check_shape(nbl, b, ncont, cb, n, q, nslots);
+  return (size_t)layout(nbl, b, ncont, cb, n, q, smode, nullptr) * nslots * sizeof(double2);
 }
 
 DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, const int *nc_,
-                     int smode_, int nslots_)
+                     int smode_, int nslots_, const int *ns_)
     : nbl(nbl_), ncont(ncont_), smode(smode_), nslots(nslots_) {
   bsz.assign(bsz_, bsz_ + std::max(nbl, 0));
   cblk.assign(cblk_, cblk_ + std::max(ncont, 0));
   nc.assign(nc_, nc_ + std::max(ncont, 0));
-  check_shape(nbl, bsz, ncont, cblk, nc, nslots);
-  slot_elems = layout(nbl, bsz, ncont, cblk, nc, smode, off);
+  if (ns_) ns.assign(ns_, ns_ + std::max(ncont, 0));
+  else ns.assign(std::max(ncont, 0), 0);
+  check_shape(nbl, bsz, ncont, cblk, nc, ns, nslots);
+  slot_elems = layout(nbl, bsz, ncont, cblk, nc, ns, smode, off);
   uid = next_uid();
   HN_CHECK(hipGetDevice(&device));
   HN_CHECK(hipMalloc(&pool, bytes()));
@@ -101,13 +112,15 @@ DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, co
 }
 
 DevStruct::DevStruct(int nbl_, const int *bsz_, int ncont_, const int *cblk_, const int *nc_,
-                     int smode_, int nslots_, void *external, hipStream_t zs)
+                     int smode_, int nslots_, void *external, hipStream_t zs, const int *ns_)
     : nbl(nbl_), ncont(ncont_), smode(smode_), nslots(nslots_) {
   bsz.assign(bsz_, bsz_ + std::max(nbl, 0));
   cblk.assign(cblk_, cblk_ + std::max(ncont, 0));
   nc.assign(nc_, nc_ + std::max(ncont, 0));
-  check_shape(nbl, bsz, ncont, cblk, nc, nslots);
-  slot_elems = layout(nbl, bsz, ncont, cblk, nc, smode, off);
+  if (ns_) ns.assign(ns_, ns_ + std::max(ncont, 0));
+  else ns.assign(std::max(ncont, 0), 0);
+  check_shape(nbl, bsz, ncont, cblk, nc, ns, nslots);
+  slot_elems = layout(nbl, bsz, ncont, cblk, nc, ns, smode, off);
   uid = next_uid();
   HN_CHECK(hipGetDevice(&device));
   pool = (double2 *)external;
@@ -139,7 +152,9 @@ long long DevStruct::block_elems(int kind, int idx) const {
     case BK_HD: case BK_SD: return (long long)bsz[idx] * bsz[idx];
     case BK_HU: case BK_SU: case BK_HL: case BK_SL: return (long long)bsz[idx] * bsz[idx + 1];
     case BK_H00: case BK_S00: case BK_H01: case BK_S01: return (long long)nc[idx] * nc[idx];
-    case BK_HMC: case BK_SMC: return (long long)bsz[cblk[idx] - 1] * nc[idx];
+    case BK_HMC: case BK_SMC: return (long long)bsz[cblk[idx] - 1] * ngs(idx);
+    case BK_HSS: case BK_SSS: return (long long)ns[idx] * ns[idx];
+    case BK_HS1: case BK_SS1: case BK_H1S: case BK_S1S: return (long long)ns[idx] * nc[idx];
   }
   return 0;
 }
@@ -147,15 +162,15 @@ long long DevStruct::block_elems(int kind, int idx) const {
 int DevStruct::nmax() const {
   int m = 0;
   for (int b : bsz) m = std::max(m, b);
-  for (int c : nc) m = std::max(m, c);
+  for (int c = 0; c < ncont; ++c) m = std::max(m, ngs(c));
   return m;
 }
 
 bool DevStruct::uniform() const {
   for (int b : bsz)
     if (b != bsz[0]) return false;
-  for (int c : nc)
-    if (c != bsz[0]) return false;
+  for (int c = 0; c < ncont; ++c)
+    if (nc[c] != bsz[0]) return false;
   return true;
 }
 
@@ -349,7 +364,7 @@ void Engine::decimate_z(int nc, int CB, double2 *Ao, double2 *Aos, double2 *Bo, 
 bool Engine::mirror_contacts(DevStruct &st) {
   if (st.mirror_state >= 0) return st.mirror_state == 1;
   st.mirror_state = 0;
-  if (st.ncont != 2 || st.nc[0] != st.nc[1]) return false;
+  if (st.ncont != 2 || st.nc[0] != st.nc[1] || st.ns[0] || st.ns[1]) return false;
   const int n = st.nc[0];
   int *flags = nullptr;
   HN_CHECK(hipMalloc(&flags, sizeof(int) * st.nslots));
@@ -518,7 +533,8 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
   const int ncont = st.ncont;
   auto self_energy = [&](int c, const double2 *GS) {
     // SelfEnergy: Tlc = z SMC - HMC, Tp = z* SMC - HMC, Sigma = Tlc GS Tp^H
-    const int nc = st.nc[c], nd = st.bsz[st.cblk[c] - 1];
+    // (GS of order ngs: surface block + one PL)
+    const int nc = st.ngs(c), nd = st.bsz[st.cblk[c] - 1];
     const long long s2 = (long long)nc * nc, sm = (long long)nd * nc;
     if (GSout) copy_batched(GSout + (long long)c * B * sGS, sGS, GS, s2, s2, B, nullptr, stream_);
     size_t mark = arena_.used;
@@ -537,7 +553,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     arena_.used = mark;
   };
   auto fictitious = [&](int c, double2 *Go) {  // GS = -i pi dos I
-    const int nc = st.nc[c];
+    const int nc = st.ngs(c);
     fill_diag(Go, (long long)nc * nc, nc, Z(0.0, -3.141592653589793 * st.contact_dos[c]), B, stream_);
   };
   const bool shared = o.share_mirror && ncont == 2 && !st.fict[0] && !st.fict[1] && mirror_contacts(st);
@@ -545,7 +561,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
   // ---- surface Green's function cache ----
   int ncmax = 0;
   for (int c = 0; c < ncont; ++c)
-    if (!st.fict[c]) ncmax = std::max(ncmax, st.nc[c]);
+    if (!st.fict[c]) ncmax = std::max(ncmax, st.ngs(c));
   // (complex per-item energies are not cached: keys hold the real energy only)
   const bool cache = o.sgf_mode != 0 && !o.dEi && ncmax > 0 && sgf_prepare((long long)ncmax * ncmax);
   std::vector<double> hE;
@@ -567,7 +583,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     std::memcpy(&k.dbits, &o.delta, sizeof(double));
     k.slot = hS[b];
     k.contact = c + (shared ? 64 : 0);  // the shared recursion rounds differently
-    k.nc = st.nc[c];
+    k.nc = st.ngs(c);
     return k;
   };
   if (cache && o.sgf_mode == 1) {  // ReadOldSGF = 0: load when every item is cached
@@ -580,7 +596,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
       double2 *Go = arena_.take<double2>((long long)ncmax * ncmax * B);
       std::vector<int> hst((size_t)B * ncont, 0), idx(B);
       for (int c = 0; c < ncont; ++c) {
-        const long long s2 = (long long)st.nc[c] * st.nc[c];
+        const long long s2 = (long long)st.ngs(c) * st.ngs(c);
         if (st.fict[c]) {
           fictitious(c, Go);
         } else {
@@ -605,7 +621,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
   }
   auto store = [&](int c, const double2 *Go) {
     if (!cache || st.fict[c]) return;
-    const long long s2 = (long long)st.nc[c] * st.nc[c];
+    const long long s2 = (long long)st.ngs(c) * st.ngs(c);
     std::vector<int> hs(B), idx(B, -1);
     HN_CHECK(hipMemcpyAsync(hs.data(), status_c + (long long)c * B, sizeof(int) * B,
                             hipMemcpyDeviceToHost, stream_));
@@ -649,11 +665,40 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     arena_.used = mark;
     return;
   }
+  // contacts with a surface block (contselfenergy.F90:220-239): with Go the
+  // decimated PL function, GS = inv([[z Sss - Hss, z Ss1 - Hs1],
+  //                                  [z S1s - H1s, inv(Go)]]) of order ns + nc
+  auto surface = [&](int c, const double2 *Go, double2 *GS) {
+    const int nc = st.nc[c], ns = st.ns[c], ng = nc + ns;
+    const long long s2 = (long long)nc * nc, g2 = (long long)ng * ng, q1 = (long long)ns * nc;
+    int *stc = status_c + (long long)c * B;
+    size_t mark = arena_.used;
+    double2 *W = arena_.take<double2>(s2 * B), *Gi = arena_.take<double2>(s2 * B);
+    double2 *T = arena_.take<double2>(g2 * B);
+    double2 *Q = arena_.take<double2>(std::max((long long)ns * ns, q1) * B);
+    copy_batched(W, s2, Go, s2, s2, B, nullptr, stream_);
+    inverse(nc, W, s2, Gi, s2, B, nullptr, stc, PREC_FP64, o);
+    place_block(T, g2, ng, ns, ns, Gi, s2, nc, nc, B, stream_);
+    FormArgs f;
+    f.slot = dslot; f.E = dE; f.eta = o.delta; f.Ei = o.dEi; f.batch = B; f.out = Q;
+    auto put = [&](int hk, int sk, int rows, int cols, int r0, int c0) {
+      f.rows = rows; f.cols = cols; f.s_out = (long long)rows * cols;
+      f.H = st.ref(hk, c); f.S = sc ? st.ref(sk, c) : BlkRef();
+      f.s_identity = (!sc && hk == BK_HSS) ? 1 : 0;
+      form_block(f, stream_);
+      place_block(T, g2, ng, r0, c0, Q, f.s_out, rows, cols, B, stream_);
+    };
+    put(BK_HSS, BK_SSS, ns, ns, 0, 0);
+    put(BK_HS1, BK_SS1, ns, nc, 0, ns);
+    put(BK_H1S, BK_S1S, nc, ns, ns, 0);
+    inverse(ng, T, g2, GS, g2, B, nullptr, stc, PREC_FP64, o);
+    arena_.used = mark;
+  };
   for (int c = 0; c < ncont; ++c) {
     const int nc = st.nc[c];
-    const long long s2 = (long long)nc * nc;
+    const long long s2 = (long long)nc * nc, g2 = (long long)st.ngs(c) * st.ngs(c);
     size_t mark = arena_.used;
-    double2 *Go = arena_.take<double2>(s2 * B);
+    double2 *Go = arena_.take<double2>(std::max(s2, g2) * B);
     if (st.fict[c]) {
       fictitious(c, Go);  // ncyc stays 0
     } else {
@@ -671,6 +716,11 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
       f.conjz = 1; f.dagger = 1; f.out = Co;
       form_block(f, stream_);
       decimate_z(nc, B, Ao, Aos, Bo, Co, Go, o, ncyc_c + (long long)c * B, status_c + (long long)c * B);
+      if (st.ns[c] > 0) {
+        double2 *GS = arena_.take<double2>(g2 * B);
+        surface(c, Go, GS);
+        copy_batched(Go, g2, GS, g2, g2, B, nullptr, stream_);
+      }
       store(c, Go);
     }
     self_energy(c, Go);
@@ -1416,8 +1466,8 @@ std::vector<long long> Engine::density_layout(const DevStruct &st) {
   for (int i = 0; i < st.nbl; ++i) put((long long)st.bsz[i] * st.bsz[i]);
   for (int i = 0; i + 1 < st.nbl; ++i) put((long long)st.bsz[i] * st.bsz[i + 1]);
   for (int i = 0; i + 1 < st.nbl; ++i) put((long long)st.bsz[i] * st.bsz[i + 1]);
-  for (int c = 0; c < st.ncont; ++c) put((long long)st.bsz[st.cblk[c] - 1] * st.nc[c]);
-  for (int c = 0; c < st.ncont; ++c) put((long long)st.bsz[st.cblk[c] - 1] * st.nc[c]);
+  for (int c = 0; c < st.ncont; ++c) put((long long)st.bsz[st.cblk[c] - 1] * st.ngs(c));
+  for (int c = 0; c < st.ncont; ++c) put((long long)st.bsz[st.cblk[c] - 1] * st.ngs(c));
   off.push_back(o);
   return off;
 }
@@ -1491,7 +1541,7 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
   auto form_tlc = [&](int k, double2 *out, int conjz) {  // z SMC - HMC at the contacts' z
     FormArgs t;
     t.slot = dslot; t.E = dE; t.eta = o.delta; t.Ei = o.dEi; t.batch = B;
-    t.rows = bs(st.cblk[k] - 1); t.cols = st.nc[k];
+    t.rows = bs(st.cblk[k] - 1); t.cols = st.ngs(k);
     t.H = st.ref(BK_HMC, k); t.S = sc ? st.ref(BK_SMC, k) : BlkRef(); t.s_identity = 0;
     t.conjz = conjz; t.out = out; t.s_out = s2;
     form_block(t, stream_);
@@ -1562,7 +1612,7 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
     // ---- calculate_Gr_outer ----
     if (d.outer > 0) {
       for (int k = 0; k < ncont; ++k) {
-        const int cb = st.cblk[k] - 1, nd = bs(cb), nc = st.nc[k];
+        const int cb = st.cblk[k] - 1, nd = bs(cb), nc = st.ngs(k);
         double2 *GSk = GS + (long long)k * SB;
         form_tlc(k, U, 0);  // Tlc
         gemm_z(G(nd, nc, nc, U, s2, nd, OP_N, GSk, s2, nc, OP_N, W1, s2, nd, one, zero, B, nullptr), stream_);
@@ -1648,7 +1698,7 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
   // ---- calculate_Gn_outer ----
   if (d.outer > 0) {
     for (int k = 0; k < ncont; ++k) {
-      const int cb = st.cblk[k] - 1, nd = bs(cb), nc = st.nc[k];
+      const int cb = st.cblk[k] - 1, nd = bs(cb), nc = st.ngs(k);
       double2 *GSk = GS + (long long)k * SB;
       form_tlc(k, U, 0);  // Tlc at E + i delta
       gemm_z(G(nd, nc, nc, U, s2, nd, OP_N, GSk, s2, nc, OP_C, W1, s2, nd, one, zero, B, nullptr), stream_);
@@ -2020,12 +2070,13 @@ void Engine::scba_mw(DevStruct &st, int nitems, const double *dE, const int *dsl
   if (nitems <= 0) return;
   if (st.ncont < 1) throw std::runtime_error("SCBA needs at least one contact");
   HN_CHECK(hipSetDevice(device_));
-  const int n = st.nmax(), nbl = st.nbl;
-  const size_t b2 = (size_t)n * n * sizeof(double2);
+  // n: the (uniform) PL order; nm >= n also covers contact surface blocks
+  const int n = st.bsz[0], nm = st.nmax(), nbl = st.nbl;
+  const size_t b2 = (size_t)nm * nm * sizeof(double2);
   // G^r/G^n block lists (9 nbl) + sweep temporaries (8) + contact decimation and
   // self-energy workspace (12 per contact) + inverse workspace + per-orbital vectors
   const size_t per_item = b2 * (size_t)(9 * nbl + 12 * st.ncont + 16) +
-                          inv_work_bytes(n, std::max(st.ncont, 1), 8) + (size_t)n * nbl * 48 + 1024;
+                          inv_work_bytes(nm, std::max(st.ncont, 1), 8) + (size_t)n * nbl * 48 + 1024;
   const int B = pick_batch(st, nitems, o, per_item);
   last_batch_ = B;
   arena_.reserve(per_item * (size_t)B + (1 << 22));

@@ -99,7 +99,7 @@ double kslab_overlap_scale(int n) {
 void gen_kslab(DevStruct &st, int slot, unsigned long long seed, double kx, double ky,
                int with_overlap, hipStream_t s) {
   const int nbl = st.nbl;
-  if (!st.uniform() || st.ncont != 2 || st.cblk[0] != nbl || st.cblk[1] != 1)
+  if (!st.uniform() || st.ncont != 2 || st.cblk[0] != nbl || st.cblk[1] != 1 || st.ns[0] || st.ns[1])
     throw std::runtime_error("gen_kslab: structure must have equal PL/contact sizes, "
                              "contact 1 on the last block and contact 2 on block 1");
   if (with_overlap && st.smode != (S_DIAG | S_OFFDIAG | S_CONTACT))

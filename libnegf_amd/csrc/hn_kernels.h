@@ -121,6 +121,10 @@ void copy_indexed(double2 *dst, long long sd, const int *didx, const double2 *sr
                   const int *sidx, long long n, int batch, hipStream_t st);
 // out_b = diag(v) (n x n) for every item
 void fill_diag(double2 *out, long long so, int n, double2 v, int batch, hipStream_t st);
+// dst_b(r0 + i, c0 + j) = src_b(i, j) for i < rows, j < cols (src column-major
+// with leading dimension rows, dst with leading dimension ldd)
+void place_block(double2 *dst, long long sd, int ldd, int r0, int c0, const double2 *src,
+                 long long ss, int rows, int cols, int batch, hipStream_t st);
 // dst[i] = src[i] (kernel copies, usable inside captured graphs)
 void copy_real(double *dst, const double *src, int n, hipStream_t st);
 void copy_int(int *dst, const int *src, int n, hipStream_t st);

@@ -503,6 +503,16 @@ __global__ __launch_bounds__(NT) void k_fill_diag(double2 *out, long long so, in
   out[(long long)b * so + e] = (r == c) ? v : make_double2(0.0, 0.0);
 }
 
+__global__ __launch_bounds__(NT) void k_place(double2 *dst, long long sd, int ldd, int r0, int c0,
+                                              const double2 *src, long long ss, int rows,
+                                              int cols) {
+  const int b = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)rows * cols) return;
+  const int r = (int)(e % rows), c = (int)(e / rows);
+  dst[(long long)b * sd + (r0 + r) + (long long)(c0 + c) * ldd] = src[(long long)b * ss + e];
+}
+
 template <typename T>
 __global__ __launch_bounds__(NT) void k_copy1(T *dst, const T *src, int n) {
   const int i = blockIdx.x * NT + threadIdx.x;
@@ -750,6 +760,13 @@ void copy_indexed(double2 *dst, long long sd, const int *didx, const double2 *sr
 void fill_diag(double2 *out, long long so, int n, double2 v, int batch, hipStream_t st) {
   if (batch <= 0) return;
   hipLaunchKernelGGL(k_fill_diag, grid2((long long)n * n, batch), dim3(NT), 0, st, out, so, n, v);
+  HN_CHECK(hipGetLastError());
+}
+void place_block(double2 *dst, long long sd, int ldd, int r0, int c0, const double2 *src,
+                 long long ss, int rows, int cols, int batch, hipStream_t st) {
+  if (batch <= 0 || rows <= 0 || cols <= 0) return;
+  hipLaunchKernelGGL(k_place, grid2((long long)rows * cols, batch), dim3(NT), 0, st, dst, sd, ldd,
+                     r0, c0, src, ss, rows, cols);
   HN_CHECK(hipGetLastError());
 }
 void copy_real(double *dst, const double *src, int n, hipStream_t st) {

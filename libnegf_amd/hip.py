@@ -20,6 +20,7 @@ _LIB = None
 
 # block kinds / modes (hipnegf.h)
 HD, SD, HU, SU, HL, SL, H00, S00, H01, S01, HMC, SMC = range(12)
+HSS, SSS, HS1, SS1, H1S, S1S = range(12, 18)  # contact surface blocks
 S_DIAG, S_OFFDIAG, S_CONTACT = 1, 2, 4
 FP64, MIXED = 0, 1
 ST_OK, ST_SINGULAR, ST_NOT_CONVERGED, ST_NONFINITE = 0, 1, 2, 4
@@ -57,6 +58,7 @@ def load():
     L.hipnegf_engine_stream.argtypes = [vp]
     L.hipnegf_engine_stream.restype = vp
     L.hipnegf_struct_create.argtypes = [i, vp, i, vp, vp, i, i, C.POINTER(vp)]
+    L.hipnegf_struct_create_surf.argtypes = [i, vp, i, vp, vp, vp, i, i, C.POINTER(vp)]
     L.hipnegf_struct_destroy.argtypes = [vp]
     L.hipnegf_struct_set_block.argtypes = [vp, i, i, i, vp]
     L.hipnegf_struct_set_block_device.argtypes = [vp, i, i, i, vp]
@@ -201,9 +203,14 @@ class DeviceStructure:
         bsz = np.array(s0.bsz, dtype=np.int32)
         cblk = np.array([c.cblk for c in s0.contacts], dtype=np.int32)
         nc = np.array([c.nc for c in s0.contacts], dtype=np.int32)
+        ns = np.array([c.ns for c in s0.contacts], dtype=np.int32)
         self._h = C.c_void_p()
-        check(load().hipnegf_struct_create(s0.nbl, _ptr(bsz), len(cblk), _ptr(cblk), _ptr(nc),
-                                           smode, self.nslots, C.byref(self._h)))
+        if ns.any():
+            check(load().hipnegf_struct_create_surf(s0.nbl, _ptr(bsz), len(cblk), _ptr(cblk), _ptr(nc),
+                                                    _ptr(ns), smode, self.nslots, C.byref(self._h)))
+        else:
+            check(load().hipnegf_struct_create(s0.nbl, _ptr(bsz), len(cblk), _ptr(cblk), _ptr(nc),
+                                               smode, self.nslots, C.byref(self._h)))
         for k, s in enumerate(structs):
             self.set_slot(k, s)
 
@@ -256,11 +263,19 @@ class DeviceStructure:
             self.set_block(slot, H00, c, ct.H00)
             self.set_block(slot, H01, c, ct.H01)
             self.set_block(slot, HMC, c, ct.HMC)
+            if ct.ns:
+                self.set_block(slot, HSS, c, ct.Hss)
+                self.set_block(slot, HS1, c, ct.Hs1)
+                self.set_block(slot, H1S, c, ct.H1s)
             if self.smode & S_CONTACT:
-                n = ct.nc
+                n, q = ct.nc, ct.ns
                 self.set_block(slot, S00, c, ct.S00 if ct.S00 is not None else np.eye(n))
                 self.set_block(slot, S01, c, ct.S01 if ct.S01 is not None else np.zeros((n, n)))
                 self.set_block(slot, SMC, c, ct.SMC if ct.SMC is not None else np.zeros_like(ct.HMC))
+                if q:
+                    self.set_block(slot, SSS, c, ct.Sss if ct.Sss is not None else np.eye(q))
+                    self.set_block(slot, SS1, c, ct.Ss1 if ct.Ss1 is not None else np.zeros((q, n)))
+                    self.set_block(slot, S1S, c, ct.S1s if ct.S1s is not None else np.zeros((n, q)))
 
     @property
     def handle(self):
@@ -369,7 +384,7 @@ def density_blocks(ds: DeviceStructure, acc: np.ndarray) -> dict:
     off = density_layout(ds)
     b = t.bsz
     nd = [b[c.cblk - 1] for c in t.contacts]
-    nc = [c.nc for c in t.contacts]
+    nc = [c.nc + c.ns for c in t.contacts]  # outer blocks span surface + PL1
     shapes = ([(n, n) for n in b] + [(b[i], b[i + 1]) for i in range(t.nbl - 1)] +
               [(b[i + 1], b[i]) for i in range(t.nbl - 1)] + [(nd[k], nc[k]) for k in range(t.ncont)] +
               [(nc[k], nd[k]) for k in range(t.ncont)])

@@ -29,23 +29,34 @@ class Contact:
     cblk: int                       # 1-based device block the contact attaches to
     H00: np.ndarray                 # nc x nc
     H01: np.ndarray                 # nc x nc, PL1 -> PL2 (away from the device)
-    HMC: np.ndarray                 # nd x nc, device block cblk -> PL1
+    HMC: np.ndarray                 # nd x (ns + nc), device block cblk -> surface + PL1
     S00: Optional[np.ndarray] = None
     S01: Optional[np.ndarray] = None
     SMC: Optional[np.ndarray] = None
+    # optional surface block of ns orbitals between the device and PL1
+    # (surface_green, contselfenergy.F90:186-239)
+    Hss: Optional[np.ndarray] = None    # ns x ns
+    Hs1: Optional[np.ndarray] = None    # ns x nc, surface -> PL1
+    H1s: Optional[np.ndarray] = None    # nc x ns, PL1 -> surface
+    Sss: Optional[np.ndarray] = None
+    Ss1: Optional[np.ndarray] = None
+    S1s: Optional[np.ndarray] = None
 
     def __post_init__(self):
         self.H00, self.H01, self.HMC = _f(self.H00), _f(self.H01), _f(self.HMC)
-        if self.S00 is not None:
-            self.S00 = _f(self.S00)
-        if self.S01 is not None:
-            self.S01 = _f(self.S01)
-        if self.SMC is not None:
-            self.SMC = _f(self.SMC)
+        for name in ("S00", "S01", "SMC", "Hss", "Hs1", "H1s", "Sss", "Ss1", "S1s"):
+            if getattr(self, name) is not None:
+                setattr(self, name, _f(getattr(self, name)))
+        if self.Hss is not None and (self.Hs1 is None or self.H1s is None):
+            raise ValueError("a surface block needs Hss, Hs1 and H1s")
 
     @property
     def nc(self) -> int:
         return self.H00.shape[0]
+
+    @property
+    def ns(self) -> int:
+        return 0 if self.Hss is None else self.Hss.shape[0]
 
 
 @dataclass

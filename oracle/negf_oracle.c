@@ -265,12 +265,15 @@ int or_decimation(int n, zc *Go, zc *Ao, zc *Bo, zc *Co, int *okco, double sgfac
   return ncyc;
 }
 
-/* surface_green Ao/Bo/Co (contselfenergy.F90:207-210) + decimation, then
- * Tlc = z SMC - HMC, Tcl = (z* SMC - HMC)^dag (:475-479) and
- * Sigma = Tlc GS Tcl (SelfEnergy_dns :552-566). */
-int or_contact_sigma(zc z, int nc, int nd, const zc *H00, const zc *S00, const zc *H01,
-                     const zc *S01, const zc *HMC, const zc *SMC, zc *Sigma, zc *GS,
-                     int *okco, double sgfacc) {
+/* surface_green Ao/Bo/Co (contselfenergy.F90:207-210) + decimation, the
+ * surface block step (:220-239) when ns > 0, then Tlc = z SMC - HMC,
+ * Tcl = (z* SMC - HMC)^dag (:475-479) and Sigma = Tlc GS Tcl
+ * (SelfEnergy_dns :552-566). */
+int or_contact_sigma_surf(zc z, int nc, int ns, int nd, const zc *H00, const zc *S00,
+                          const zc *H01, const zc *S01, const zc *Hss, const zc *Sss,
+                          const zc *Hs1, const zc *Ss1, const zc *H1s, const zc *S1s,
+                          const zc *HMC, const zc *SMC, zc *Sigma, zc *GS, int *okco,
+                          double sgfacc) {
   size_t nn = (size_t)nc * nc;
   zc *Ao = zalloc(nn), *Bo = zalloc(nn), *Co = zalloc(nn), *Cp = zalloc(nn), *Go = zalloc(nn);
   for (int j = 0; j < nc; ++j)
@@ -284,7 +287,36 @@ int or_contact_sigma(zc z, int nc, int nd, const zc *H00, const zc *S00, const z
     }
   zdag(nc, nc, Cp, Co);
   int ncyc = or_decimation(nc, Go, Ao, Bo, Co, okco, sgfacc);
-  if (GS) memcpy(GS, Go, nn * sizeof(zc));
+  const int ng = nc + ns;
+  if (ns > 0) {
+    /* gt = [[z Sss - Hss, z Ss1 - Hs1], [z S1s - H1s, inv(Go)]], GS = inv(gt) */
+    size_t gg = (size_t)ng * ng;
+    zc *gt = zalloc(gg), *Gi = zalloc(nn), *G2 = zalloc(gg);
+    or_zinv(nc, Go, Gi);
+    for (int j = 0; j < ng; ++j)
+      for (int i = 0; i < ng; ++i) {
+        zc v;
+        if (i < ns && j < ns) {
+          size_t t = i + (size_t)j * ns;
+          v = z * (Sss ? Sss[t] : (i == j ? 1.0 : 0.0)) - Hss[t];
+        } else if (i < ns) {
+          size_t t = i + (size_t)(j - ns) * ns;
+          v = z * (Ss1 ? Ss1[t] : 0.0) - Hs1[t];
+        } else if (j < ns) {
+          size_t t = (i - ns) + (size_t)j * nc;
+          v = z * (S1s ? S1s[t] : 0.0) - H1s[t];
+        } else {
+          v = Gi[(i - ns) + (size_t)(j - ns) * nc];
+        }
+        gt[i + (size_t)j * ng] = v;
+      }
+    or_zinv(ng, gt, G2);
+    free(Go);
+    Go = G2;
+    free(gt); free(Gi);
+  }
+  if (GS) memcpy(GS, Go, (size_t)ng * ng * sizeof(zc));
+  nc = ng; /* the self-energy below couples the device block to surface + PL1 */
   size_t md = (size_t)nd * nc;
   zc *Tlc = zalloc(md), *Tp = zalloc(md), *Tcl = zalloc(md), *TG = zalloc(md);
   for (size_t t = 0; t < md; ++t) {
@@ -298,6 +330,28 @@ int or_contact_sigma(zc z, int nc, int nd, const zc *H00, const zc *S00, const z
   free(Ao); free(Bo); free(Co); free(Cp); free(Go);
   free(Tlc); free(Tp); free(Tcl); free(TG);
   return ncyc;
+}
+
+int or_contact_sigma(zc z, int nc, int nd, const zc *H00, const zc *S00, const zc *H01,
+                     const zc *S01, const zc *HMC, const zc *SMC, zc *Sigma, zc *GS,
+                     int *okco, double sgfacc) {
+  return or_contact_sigma_surf(z, nc, 0, nd, H00, S00, H01, S01, 0, 0, 0, 0, 0, 0, HMC, SMC,
+                               Sigma, GS, okco, sgfacc);
+}
+
+/* order of contact c's surface Green's function (surface + one PL) */
+static int ngs_of(const or_struct *s, int c) { return s->nc[c] + (s->ns ? s->ns[c] : 0); }
+
+/* contact c of a structure at z: Sigma (nd x nd), GS (ngs x ngs, may be NULL) */
+static int contact_sigma_of(const or_struct *s, int c, zc z, zc *Sigma, zc *GS, int *okco,
+                            double sgfacc) {
+  int nd = s->bsz[s->cblk[c] - 1], ns = s->ns ? s->ns[c] : 0;
+#define OPT(a) ((a) ? (a)[c] : 0)
+  return or_contact_sigma_surf(z, s->nc[c], ns, nd, s->H00[c], OPT(s->S00), s->H01[c], OPT(s->S01),
+                               ns ? s->Hss[c] : 0, ns ? OPT(s->Sss) : 0, ns ? s->Hs1[c] : 0,
+                               ns ? OPT(s->Ss1) : 0, ns ? s->H1s[c] : 0, ns ? OPT(s->S1s) : 0,
+                               s->HMC[c], OPT(s->SMC), Sigma, GS, okco, sgfacc);
+#undef OPT
 }
 
 /* ======================================================================= */
@@ -632,9 +686,7 @@ double or_transmission(const or_struct *s, double E, double delta, int ni, int n
   for (int c = 0; c < s->ncont; ++c) {
     int nd = s->bsz[s->cblk[c] - 1];
     Sig[c] = zalloc((size_t)nd * nd);
-    int ncyc = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
-                                s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
-                                0, 0, sgfacc);
+    int ncyc = contact_sigma_of(s, c, z, Sig[c], 0, 0, sgfacc);
     if (ncyc_out) ncyc_out[c] = ncyc;
   }
   double T = or_transmission_sigma(s, E, Sig, ni, nf, eps_drop);
@@ -654,9 +706,7 @@ int or_transmission_dos(const or_struct *s, double E, double delta, double dos_d
   for (int c = 0; c < s->ncont; ++c) {
     int nd = s->bsz[s->cblk[c] - 1];
     Sig[c] = zalloc((size_t)nd * nd);
-    int ncyc = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
-                                s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
-                                0, 0, sgfacc);
+    int ncyc = contact_sigma_of(s, c, z, Sig[c], 0, 0, sgfacc);
     if (ncyc_out) ncyc_out[c] = ncyc;
   }
   int r = or_transmission_dos_sigma(s, E, dos_delta, Sig, npairs, ni, nf, ntun, tun, nldos, ptr,
@@ -942,9 +992,7 @@ int or_scba_meir_wingreen(const or_struct *s, double E, double delta, const doub
   for (int c = 0; c < s->ncont; ++c) {
     int nd = s->bsz[s->cblk[c] - 1];
     Sig[c] = zalloc((size_t)nd * nd);
-    or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
-                     s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c], 0, 0,
-                     sgfacc);
+    contact_sigma_of(s, c, z, Sig[c], 0, 0, sgfacc);
   }
   int it = or_scba_mw_sigma(s, E, Sig, coupling, niter, tol, frm, ref, curr, err_out, gn_diag,
                             gr_diag, eps_drop);
@@ -1068,10 +1116,11 @@ int or_real_axis_grid(double mumin, double mumax, double kbt_max, int n_kt, int 
   return np;
 }
 
-/* Tlc = z SMC - HMC (nd x nc), Tcl = (conj(z) SMC - HMC)^dag (nc x nd):
+/* Tlc = z SMC - HMC (nd x ng), Tcl = (conj(z) SMC - HMC)^dag (ng x nd), ng the
+ * order of the contact's surface function (surface + one PL):
  * compute_contacts_dns, contselfenergy.F90:470-475 */
 static void t_blocks(const or_struct *s, int c, zc z, zc *Tlc, zc *Tcl) {
-  int nc = s->nc[c], nd = s->bsz[s->cblk[c] - 1];
+  int nc = ngs_of(s, c), nd = s->bsz[s->cblk[c] - 1];
   zc *Tp = zalloc((size_t)nd * nc);
   for (size_t t = 0; t < (size_t)nd * nc; ++t) {
     zc sm = (s->SMC && s->SMC[c]) ? s->SMC[c][t] : 0.0;
@@ -1095,10 +1144,8 @@ int or_gr_point(const or_struct *s, double zr, double zi, zc *const *Gdo, zc *co
   for (int c = 0; c < ncont; ++c) {
     int nd = s->bsz[s->cblk[c] - 1];
     Sig[c] = zalloc((size_t)nd * nd);
-    GS[c] = zalloc((size_t)s->nc[c] * s->nc[c]);
-    int n = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
-                             s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
-                             GS[c], 0, sgfacc);
+    GS[c] = zalloc((size_t)ngs_of(s, c) * ngs_of(s, c));
+    int n = contact_sigma_of(s, c, z, Sig[c], GS[c], 0, sgfacc);
     if (ncyc) ncyc[c] = n;
   }
   esh_t e = build_esh(s, z, Sig);
@@ -1109,7 +1156,7 @@ int or_gr_point(const or_struct *s, double zr, double zi, zc *const *Gdo, zc *co
   gr_11(&e, g, Gd[0]);
   if (nbl > 1) gr_tridiag(&e, g, 2, nbl, Gd, Gu, Gl);
   for (int c = 0; c < ncont; ++c) {
-    int nc = s->nc[c], cb = s->cblk[c] - 1, nd = e.b[cb];
+    int nc = ngs_of(s, c), cb = s->cblk[c] - 1, nd = e.b[cb];
     zc *Tlc = zalloc((size_t)nd * nc), *Tcl = zalloc((size_t)nc * nd);
     zc *w1 = zalloc((size_t)nd * nc), *w2 = zalloc((size_t)nc * nd);
     t_blocks(s, c, z, Tlc, Tcl);
@@ -1157,10 +1204,8 @@ int or_gn_point(const or_struct *s, double E, double delta, const double *frm, i
   for (int c = 0; c < ncont; ++c) {
     int nd = s->bsz[s->cblk[c] - 1];
     Sig[c] = zalloc((size_t)nd * nd);
-    GS[c] = zalloc((size_t)s->nc[c] * s->nc[c]);
-    int n = or_contact_sigma(z, s->nc[c], nd, s->H00[c], s->S00 ? s->S00[c] : 0, s->H01[c],
-                             s->S01 ? s->S01[c] : 0, s->HMC[c], s->SMC ? s->SMC[c] : 0, Sig[c],
-                             GS[c], 0, sgfacc);
+    GS[c] = zalloc((size_t)ngs_of(s, c) * ngs_of(s, c));
+    int n = contact_sigma_of(s, c, z, Sig[c], GS[c], 0, sgfacc);
     if (ncyc) ncyc[c] = n;
   }
   esh_t e = build_esh(s, E, Sig);
@@ -1194,7 +1239,7 @@ int or_gn_point(const or_struct *s, double E, double delta, const double *frm, i
   }
   gn_tridiag(&e, g, Gd, Gu, Gl, Snd, Snu, Snl, Gnd, Gnu, Gnl);
   for (int k = 0; k < ncont; ++k) {
-    int nc = s->nc[k], cb = s->cblk[k] - 1, nd = e.b[cb];
+    int nc = ngs_of(s, k), cb = s->cblk[k] - 1, nd = e.b[cb];
     zc *Tlc = zalloc((size_t)nd * nc), *Tcl = zalloc((size_t)nc * nd);
     zc *gA = zalloc((size_t)nc * nc), *w2 = zalloc((size_t)nd * nc), *w3 = zalloc((size_t)nd * nc);
     zc *w1 = zalloc((size_t)nd * nc), *A = zalloc((size_t)nc * nc);

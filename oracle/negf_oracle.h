@@ -40,6 +40,17 @@ int or_decimation(int n, zc *Go, zc *Ao, zc *Bo, zc *Co, int *okco, double sgfac
 int or_contact_sigma(zc z, int nc, int nd, const zc *H00, const zc *S00, const zc *H01,
                      const zc *S01, const zc *HMC, const zc *SMC, zc *Sigma, zc *GS,
                      int *okco, double sgfacc);
+/* Same for a contact with a surface block of ns orbitals before its two PLs
+ * (surface_green, contselfenergy.F90:186-239): with Go the decimated PL
+ * function, GS = inv([[z Sss - Hss, z Ss1 - Hs1], [z S1s - H1s, inv(Go)]])
+ * of order ng = ns + nc.  Hss,Sss: ns x ns; Hs1,Ss1: ns x nc; H1s,S1s:
+ * nc x ns (S blocks NULL: Sss = I, the others 0).  HMC,SMC: nd x ng.
+ * GS: ng x ng (may be NULL). */
+int or_contact_sigma_surf(zc z, int nc, int ns, int nd, const zc *H00, const zc *S00,
+                          const zc *H01, const zc *S01, const zc *Hss, const zc *Sss,
+                          const zc *Hs1, const zc *Ss1, const zc *H1s, const zc *S1s,
+                          const zc *HMC, const zc *SMC, zc *Sigma, zc *GS, int *okco,
+                          double sgfacc);
 
 /* ---- device structure --------------------------------------------------- */
 typedef struct {
@@ -55,7 +66,11 @@ typedef struct {
   const int *cblk;         /* [ncont] 1-based block of each contact           */
   const int *nc;           /* [ncont] contact PL size                         */
   const zc *const *H00, *const *S00, *const *H01, *const *S01; /* [ncont]     */
-  const zc *const *HMC, *const *SMC;                            /* [ncont]     */
+  const zc *const *HMC, *const *SMC;   /* [ncont] nd x (ns + nc)               */
+  /* contact surface blocks (NULL ns: none): ns[ncont] sizes; per contact
+   * Hss, Sss (ns x ns), Hs1, Ss1 (ns x nc), H1s, S1s (nc x ns); S NULL as above */
+  const int *ns;
+  const zc *const *Hss, *const *Sss, *const *Hs1, *const *Ss1, *const *H1s, *const *S1s;
 } or_struct;
 
 /* Per-energy coherent transmission between contacts ni -> nf (1-based),

@@ -60,6 +60,9 @@ def lib():
         L.or_contact_sigma.restype = C.c_int
         L.or_contact_sigma.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int] + [C.c_void_p] * 8 + \
             [C.c_void_p, C.c_double]
+        L.or_contact_sigma_surf.restype = C.c_int
+        L.or_contact_sigma_surf.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, C.c_int] + \
+            [C.c_void_p] * 14 + [C.c_void_p, C.c_double]
         L.or_decimation.restype = C.c_int
         L.or_decimation.argtypes = [C.c_int] + [C.c_void_p] * 5 + [C.c_double]
         L.or_zinv.restype = C.c_int
@@ -113,7 +116,9 @@ class _OrStruct(C.Structure):
                 ("Hu", C.c_void_p), ("Su", C.c_void_p), ("Hl", C.c_void_p), ("Sl", C.c_void_p),
                 ("ncont", C.c_int), ("cblk", C.c_void_p), ("nc", C.c_void_p),
                 ("H00", C.c_void_p), ("S00", C.c_void_p), ("H01", C.c_void_p),
-                ("S01", C.c_void_p), ("HMC", C.c_void_p), ("SMC", C.c_void_p)]
+                ("S01", C.c_void_p), ("HMC", C.c_void_p), ("SMC", C.c_void_p),
+                ("ns", C.c_void_p), ("Hss", C.c_void_p), ("Sss", C.c_void_p), ("Hs1", C.c_void_p),
+                ("Ss1", C.c_void_p), ("H1s", C.c_void_p), ("S1s", C.c_void_p)]
 
 
 def _cz(a: np.ndarray) -> np.ndarray:
@@ -165,6 +170,13 @@ class OracleStruct:
         st.S00 = parr([c.S00 for c in cs], len(cs)) if anyS else None
         st.S01 = parr([c.S01 for c in cs], len(cs)) if anyS else None
         st.SMC = parr([c.SMC for c in cs], len(cs)) if anyS else None
+        if any(c.ns for c in cs):
+            ns = np.array([c.ns for c in cs], dtype=np.int32)
+            keep.append(ns)
+            st.ns = ns.ctypes.data
+            for name in ("Hss", "Sss", "Hs1", "Ss1", "H1s", "S1s"):
+                mats = [getattr(c, name) for c in cs]
+                setattr(st, name, parr(mats, len(cs)) if any(m is not None for m in mats) else None)
         self._keep = keep
         self.c = st
         self.s = s
@@ -196,15 +208,17 @@ def transmission_batch(s, E: np.ndarray, delta: float, ni: int = 1, nf: int = 2,
 
 
 def contact_sigma(c, nd: int, z: complex, okco: Optional[np.ndarray] = None):
-    """Returns (Sigma nd x nd, GS nc x nc, ncyc)."""
+    """Returns (Sigma nd x nd, GS (ns+nc) x (ns+nc), ncyc)."""
+    ng = c.nc + c.ns
     Sig = np.zeros((nd, nd), dtype=np.complex128, order="F")
-    GS = np.zeros((c.nc, c.nc), dtype=np.complex128, order="F")
-    bufs = [_cz(c.H00), None if c.S00 is None else _cz(c.S00), _cz(c.H01),
-            None if c.S01 is None else _cz(c.S01), _cz(c.HMC), None if c.SMC is None else _cz(c.SMC)]
+    GS = np.zeros((ng, ng), dtype=np.complex128, order="F")
+    opt = lambda a: None if a is None else _cz(a)
+    bufs = [_cz(c.H00), opt(c.S00), _cz(c.H01), opt(c.S01), opt(c.Hss), opt(c.Sss), opt(c.Hs1),
+            opt(c.Ss1), opt(c.H1s), opt(c.S1s), _cz(c.HMC), opt(c.SMC)]
     ptrs = [None if b is None else b.ctypes.data for b in bufs]
     ok = None if okco is None else okco.ctypes.data
-    ncyc = lib().or_contact_sigma(z.real, z.imag, c.nc, nd, *ptrs, Sig.ctypes.data,
-                                  GS.ctypes.data, ok, SGFACC)
+    ncyc = lib().or_contact_sigma_surf(z.real, z.imag, c.nc, c.ns, nd, *ptrs, Sig.ctypes.data,
+                                       GS.ctypes.data, ok, SGFACC)
     return Sig, GS, ncyc
 
 
@@ -365,7 +379,7 @@ def _blocks(shapes):
 def _point_shapes(s):
     b = s.bsz
     nd = [b[c.cblk - 1] for c in s.contacts]
-    nc = [c.nc for c in s.contacts]
+    nc = [c.nc + c.ns for c in s.contacts]  # outer blocks span surface + PL1
     return ([(n, n) for n in b], [(b[i], b[i + 1]) for i in range(len(b) - 1)],
             [(b[i + 1], b[i]) for i in range(len(b) - 1)], [(nd[k], nc[k]) for k in range(len(nc))],
             [(nc[k], nd[k]) for k in range(len(nc))])
@@ -412,8 +426,9 @@ def _t_pattern(s, k):
 def _add_point(s, N, cstart, P, T, pat, blk, w, lower, masked_outer, tl_pattern=None):
     """Adds w * blocks onto the pattern: the device part through blk2csr
     (iterative.F90:1375-1450) on the S pattern P (rows/columns of the device
-    only), the outer blocks at (block cb rows, contact PL1 columns) and, with
-    lower, at the transposed position (calculate_Gr_outer / calculate_Gn_outer)."""
+    only), the outer blocks at (block cb rows, contact surface + PL1 columns)
+    and, with lower, at the transposed position (calculate_Gr_outer /
+    calculate_Gn_outer)."""
     off = s.block_offsets()
     nd = s.ndev
     rr, cc = np.nonzero(P[:nd, :nd])
@@ -432,7 +447,7 @@ def _add_point(s, N, cstart, P, T, pat, blk, w, lower, masked_outer, tl_pattern=
     pat |= P
     for k, con in enumerate(s.contacts):
         cb = con.cblk - 1
-        r0, c0, nc, ndk = off[cb], cstart[k], con.nc, s.bsz[cb]
+        r0, c0, nc, ndk = off[cb], cstart[k], con.nc + con.ns, s.bsz[cb]
         tp = _t_pattern(s, k)
         up = blk["Glc"][k]
         m = tp & (up != 0)
@@ -453,7 +468,8 @@ def density_matrix(s, N, cstart, P, contour=None, real=None, frm_of=None, ref=No
     it (contour_int, integrations.F90:844-897; real_axis_int, :1093-1176):
     T = sum_i w_i Gr(z_i) on the pattern, rho = i (T - T^H), then
     rho += sum_j w_j Gn(E_j).  P: N x N boolean S pattern; cstart[k]: 0-based
-    first column of contact k's first PL; frm_of(E) -> frm[ncont+1]."""
+    first column of contact k (its surface block, else its first PL);
+    frm_of(E) -> frm[ncont+1]."""
     rho = np.zeros((N, N), dtype=np.complex128)
     pat = np.zeros((N, N), dtype=bool)
     pat[np.arange(N), np.arange(N)] = True

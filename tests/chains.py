@@ -52,11 +52,13 @@ def brute_T(s, E, sigmas, ci=0, cf=1):
     return float(np.real(np.trace(gamma(sigmas[cf]) @ Gfi @ gamma(sigmas[ci]) @ Gfi.conj().T)))
 
 
-def from_dense(H, S, plend, surfstart, contend, cblk):
+def from_dense(H, S, plend, surfstart, contend, cblk, surfend=None):
     """BlockStructure of a dense PL-ordered H (and S, or None for identity) with
-    the reference's structure description (1-based, no surface blocks):
-    device PLs end at plend, contact c occupies surfstart[c]..contend[c]
-    (two principal layers) and couples to PL cblk[c]."""
+    the reference's structure description (1-based): device PLs end at plend,
+    contact c occupies surfstart[c]..contend[c] -- a surface block
+    surfstart..surfend (empty by default, surfend = surfstart - 1) and two
+    principal layers -- and couples to PL cblk[c] through the surface and the
+    first PL (ln_extract.F90:41-76)."""
     pls = [0] + [int(p) for p in plend[:-1]]
     ple = [int(p) for p in plend]
     blk = lambda M, a, b, c, d: np.array(M[a:b, c:d], dtype=np.complex128)
@@ -71,14 +73,19 @@ def from_dense(H, S, plend, surfstart, contend, cblk):
         Sl = [blk(S, pls[i + 1], ple[i + 1], pls[i], ple[i]) for i in range(nbl - 1)]
     cs = []
     for c in range(len(cblk)):
-        a = int(surfstart[c]) - 1
+        s0 = int(surfstart[c]) - 1                                   # surface start (0-based)
+        a = s0 if surfend is None else int(surfend[c])               # first PL start
         n = (int(contend[c]) - a) // 2
         b = int(cblk[c]) - 1
         kw = dict(cblk=int(cblk[c]), H00=blk(H, a, a + n, a, a + n), H01=blk(H, a, a + n, a + n, a + 2 * n),
-                  HMC=blk(H, pls[b], ple[b], a, a + n))
+                  HMC=blk(H, pls[b], ple[b], s0, a + n))
+        if a > s0:
+            kw.update(Hss=blk(H, s0, a, s0, a), Hs1=blk(H, s0, a, a, a + n), H1s=blk(H, a, a + n, s0, a))
         if S is not None:
             kw.update(S00=blk(S, a, a + n, a, a + n), S01=blk(S, a, a + n, a + n, a + 2 * n),
-                      SMC=blk(S, pls[b], ple[b], a, a + n))
+                      SMC=blk(S, pls[b], ple[b], s0, a + n))
+            if a > s0:
+                kw.update(Sss=blk(S, s0, a, s0, a), Ss1=blk(S, s0, a, a, a + n), S1s=blk(S, a, a + n, s0, a))
         cs.append(Contact(**kw))
     return BlockStructure(Hd=Hd, Hu=Hu, Hl=Hl, contacts=cs, Sd=Sd, Su=Su, Sl=Sl, name="dense")
 
