@@ -17,8 +17,8 @@
 // A(lane % 16, lane / 16) and B(lane / 16, 4q + lane % 4).  The products are
 // the same four-term FP64 dot products as with the 16x16x4 form, and the
 // results are bitwise identical to it.
-// Two tilings: 64 x 128 per workgroup (each of the 4 waves 64 x 32) for
-// K >= 128 and N >= 128, 64 x 64 (each wave 32 x 32) otherwise.  FP32 keeps
+// Two tilings: 64 x 128 per workgroup (each of 8 waves 32 x 32, 512 threads)
+// for K >= 128 and N >= 128, 64 x 64 (each of 4 waves 32 x 32) otherwise.  FP32 keeps
 // v_mfma_f32_16x16x4 on the 64 x 64 tiling.  The epilogue stages the tile
 // through LDS so C is read and written along columns (coalesced): the rank-16
 // updates of the blocked inverse are pure C streaming.
@@ -330,9 +330,11 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   }
 }
 
-// ---- FP64 wide tile: 64 x 128 per workgroup, each wave 64 x 32 (4 row
-// strips x 8 column quads of 4x4x4 instructions) ----------------------------
-constexpr int WBN = 128, WPA = 2, WPB = 2;
+// ---- FP64 wide tile: 64 x 128 per workgroup of 512 threads, each of the
+// eight waves 32 x 32 (2 row strips x 8 column quads of 4x4x4 instructions):
+// two waves per SIMD, so one wave's LDS reads and barrier waits overlap the
+// other's matrix-core work ----------------------------------------------------
+constexpr int WBN = 128, WPA = 2, WPB = 2, NTW = 512;
 struct SmemW {
   double Ar[BK][BM + WPA];
   double Ai[BK][BM + WPA];
@@ -344,7 +346,7 @@ struct SmemW2 {
 };
 
 template <int USE>
-__global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, int tiles_n,
+__global__ __launch_bounds__(NTW) void k_gemm_w(GemmArgs<double> p, int tiles_m, int tiles_n,
                                                int total, int total_pad) {
   typedef double2 CT;
   const int logical = xcd_remap(blockIdx.x, total_pad);
@@ -369,43 +371,44 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
   const bool a_n = p.opA == OP_N, b_n = p.opB == OP_N;
   const double sa = (p.opA == OP_C) ? -1.0 : 1.0, sb = (p.opB == OP_C) ? -1.0 : 1.0;
 
-  CT ra[4], rb[8];
+  CT ra[2], rb[4];
+  const int wm = wave >> 2, wn = wave & 3;
   const bool full_mn = (m0 + BM <= M) && (n0 + WBN <= N);
   auto load = [&](int k0) {
     if (full_mn && k0 + BK <= K) {  // interior tile: no per-element bounds tests
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
+      for (int r = 0; r < 2; ++r) {
         int i, kk;
-        if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
-        else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+        if (a_n) { i = tid & 63; kk = (tid >> 6) + 8 * r; }
+        else { kk = tid & 15; i = (tid >> 4) + 32 * r; }
         const int gi = m0 + i, gk = k0 + kk;
         ra[r] = a_n ? A[gi + (long long)gk * p.lda] : A[gk + (long long)gi * p.lda];
       }
 #pragma unroll
-      for (int r = 0; r < 8; ++r) {
+      for (int r = 0; r < 4; ++r) {
         int j, kk;
-        if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
-        else { j = tid & 127; kk = (tid >> 7) + 2 * r; }
+        if (b_n) { kk = tid & 15; j = (tid >> 4) + 32 * r; }
+        else { j = tid & 127; kk = (tid >> 7) + 4 * r; }
         const int gj = n0 + j, gk = k0 + kk;
         rb[r] = b_n ? B[gk + (long long)gj * p.ldb] : B[gj + (long long)gk * p.ldb];
       }
       return;
     }
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 2; ++r) {
       int i, kk;
-      if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
-      else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+      if (a_n) { i = tid & 63; kk = (tid >> 6) + 8 * r; }
+      else { kk = tid & 15; i = (tid >> 4) + 32 * r; }
       const int gi = m0 + i, gk = k0 + kk;
       CT v = {0, 0};
       if (gi < M && gk < K) v = a_n ? A[gi + (long long)gk * p.lda] : A[gk + (long long)gi * p.lda];
       ra[r] = v;
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < 4; ++r) {
       int j, kk;
-      if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
-      else { j = tid & 127; kk = (tid >> 7) + 2 * r; }
+      if (b_n) { kk = tid & 15; j = (tid >> 4) + 32 * r; }
+      else { j = tid & 127; kk = (tid >> 7) + 4 * r; }
       const int gj = n0 + j, gk = k0 + kk;
       CT v = {0, 0};
       if (gj < N && gk < K) v = b_n ? B[gk + (long long)gj * p.ldb] : B[gj + (long long)gk * p.ldb];
@@ -414,26 +417,26 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
   };
   auto store = [&](SmemW &sm) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 2; ++r) {
       int i, kk;
-      if (a_n) { i = tid & 63; kk = (tid >> 6) + 4 * r; }
-      else { kk = tid & 15; i = (tid >> 4) + 16 * r; }
+      if (a_n) { i = tid & 63; kk = (tid >> 6) + 8 * r; }
+      else { kk = tid & 15; i = (tid >> 4) + 32 * r; }
       sm.Ar[kk][i] = ra[r].x;
       sm.Ai[kk][i] = sa * ra[r].y;
     }
 #pragma unroll
-    for (int r = 0; r < 8; ++r) {
+    for (int r = 0; r < 4; ++r) {
       int j, kk;
-      if (b_n) { kk = tid & 15; j = (tid >> 4) + 16 * r; }
-      else { j = tid & 127; kk = (tid >> 7) + 2 * r; }
+      if (b_n) { kk = tid & 15; j = (tid >> 4) + 32 * r; }
+      else { j = tid & 127; kk = (tid >> 7) + 4 * r; }
       sm.Br[kk][j] = rb[r].x;
       sm.Bi[kk][j] = sb * rb[r].y;
     }
   };
 
-  double cr[4][8], ci[4][8];
+  double cr[2][8], ci[2][8];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int q = 0; q < 8; ++q) {
       cr[i][q] = 0.0;
@@ -443,18 +446,17 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
   // before the K loop, the second's at the start of the epilogue
   CT *C = p.C + (long long)b * p.sC;
   const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
-  constexpr int EQ = (64 * 64) / NT;
+  constexpr int EQ = (64 * 64) / NTW;
   CT cpre[2][EQ];
   auto fetch_c = [&](int j) {
 #pragma unroll
     for (int q = 0; q < EQ; ++q) {
-      const int e = tid + q * NT;
+      const int e = tid + q * NTW;
       const int gr = m0 + (e & 63), sc = e >> 6;
       const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
       cpre[j][q] = (useb && gr < M && gc < N) ? C[gr + (long long)gc * p.ldc] : CT{0, 0};
     }
   };
-  fetch_c(0);
   int cur = 0;
   if (K > 0) {
     load(0);
@@ -466,18 +468,18 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
     SmemW &sm = sm2.buf[cur];
     // operand fragments of sub-step s + 1 are read from LDS while the MFMAs of
     // sub-step s run (two register sets)
-    double fa[2][4], fai[2][4], fb[2][8], fbi[2][8];
+    double fa[2][2], fai[2][2], fb[2][8], fbi[2][8];
     auto frag = [&](int ks, int slot) {
       const int kr = ks + (lane >> 4), c16 = lane & 15;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        fa[slot][i] = sm.Ar[kr][i * 16 + c16];
-        fai[slot][i] = sm.Ai[kr][i * 16 + c16];
+      for (int i = 0; i < 2; ++i) {
+        fa[slot][i] = sm.Ar[kr][wm * 32 + i * 16 + c16];
+        fai[slot][i] = sm.Ai[kr][wm * 32 + i * 16 + c16];
       }
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
-        fb[slot][q] = sm.Br[kr][wave * 32 + q * 4 + (lane & 3)];
-        fbi[slot][q] = sm.Bi[kr][wave * 32 + q * 4 + (lane & 3)];
+        fb[slot][q] = sm.Br[kr][wn * 32 + q * 4 + (lane & 3)];
+        fbi[slot][q] = sm.Bi[kr][wn * 32 + q * 4 + (lane & 3)];
       }
     };
     frag(0, 0);
@@ -486,14 +488,14 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
       const int sl = s & 1;
       if (s + 1 < BK / 4) frag(4 * (s + 1), sl ^ 1);
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(fa[sl][i], fb[sl][q], cr[i][q], 0, 0, 0);
           ci[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(fa[sl][i], fbi[sl][q], ci[i][q], 0, 0, 0);
         }
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(-fai[sl][i], fbi[sl][q], cr[i][q], 0, 0, 0);
@@ -516,15 +518,16 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
   CT *C2 = (p.C2 && (!p.active2 || p.active2[b] != 0)) ? p.C2 + (long long)b * p.sC2 : nullptr;
   const double ar_ = p.alpha.x, ai_ = p.alpha.y;
   const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
+  fetch_c(0);
   fetch_c(1);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
-        const int row = i * 16 + 4 * ((lane >> 2) & 3) + (lane >> 4);
-        const int sc = wave * 16 + 4 * q + (lane & 3);
+        const int row = wm * 32 + i * 16 + 4 * ((lane >> 2) & 3) + (lane >> 4);
+        const int sc = wn * 16 + 4 * q + (lane & 3);
         Er[sc * EL + row] = cr[i][4 * j + q];
         Ei[sc * EL + row] = ci[i][4 * j + q];
       }
@@ -532,7 +535,7 @@ __global__ __launch_bounds__(NT) void k_gemm_w(GemmArgs<double> p, int tiles_m, 
     // slab column sc holds tile column (sc / 16) * 32 + j * 16 + sc % 16
 #pragma unroll
     for (int q = 0; q < EQ; ++q) {
-      const int e = tid + q * NT;
+      const int e = tid + q * NTW;
       const int row = e & 63, sc = e >> 6;
       const int gr = m0 + row;
       const int gc = n0 + (sc >> 4) * 32 + j * 16 + (sc & 15);
@@ -591,13 +594,13 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   if constexpr (sizeof(R) == 8) {
     if (W) {
       if (use == PT_GEMM_SWEEP)
-        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_SWEEP>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_SWEEP>), dim3(total_pad), dim3(NTW), 0, s, a, tiles_m,
                            tiles_n, (int)total, total_pad);
       else if (use == PT_GEMM_INV)
-        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_INV>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_INV>), dim3(total_pad), dim3(NTW), 0, s, a, tiles_m,
                            tiles_n, (int)total, total_pad);
       else
-        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_MASKED>), dim3(total_pad), dim3(NT), 0, s, a, tiles_m,
+        hipLaunchKernelGGL((k_gemm_w<PT_GEMM_MASKED>), dim3(total_pad), dim3(NTW), 0, s, a, tiles_m,
                            tiles_n, (int)total, total_pad);
       HN_CHECK(hipGetLastError());
       return;
