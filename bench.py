@@ -93,8 +93,9 @@ def traffic_from_profiles(kernel_tag: str, workload_tag):
 
 def kernel_label(tag_name: str, nmax: int) -> str:
     """Kernel behind a profile tag: the FP64 GEMMs of blocks below 128 run the
-    64 x 64 tiling k_gemm<double, U> instead of the 64 x 128 k_gemm_w<U>."""
-    if nmax < 128 and tag_name.startswith("k_gemm_w<"):
+    64 x 64 tiling k_gemm<double, U> instead of the 64 x 128 k_gemm_w<U>, and so
+    do the inverse's trailing updates of blocks above 256 (rank 32 < 64)."""
+    if tag_name.startswith("k_gemm_w<") and (nmax < 128 or (tag_name == "k_gemm_w<2>" and nmax > 256)):
         return "k_gemm<double, " + tag_name[len("k_gemm_w<"):]
     return tag_name
 

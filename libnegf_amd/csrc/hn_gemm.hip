@@ -570,7 +570,7 @@ __global__ __launch_bounds__(NTW) void k_gemm_w(GemmArgs<double> p, int tiles_m,
 template <typename R>
 void launch(const GemmArgs<R> &a, hipStream_t s) {
   if (a.m <= 0 || a.n <= 0 || a.batch <= 0) return;
-  const bool W = sizeof(R) == 8 && a.k >= 128 && a.n >= 128;
+  const bool W = sizeof(R) == 8 && a.k >= 64 && a.n >= 128;
   int tiles_m = (a.m + BM - 1) / BM, tiles_n = (a.n + (W ? WBN : BN) - 1) / (W ? WBN : BN);
   long long total = (long long)tiles_m * tiles_n * a.batch;
   if (total > (1LL << 30)) throw std::runtime_error("gemm: grid too large");

@@ -121,7 +121,7 @@ PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV, PT_INV_PANEL, PT_INV_SMALL = range(5
 # Kernel serving each tag for FP64 blocks with K, N >= 128 (the 64 x 128 tiling);
 # smaller GEMMs of a tag run on k_gemm<double, tag>.
 PT_NAMES = {PT_GEMM_MASKED: "k_gemm_w<0>", PT_GEMM_SWEEP: "k_gemm_w<1>",
-            PT_GEMM_INV: "k_gemm<double, 2>", PT_INV_PANEL: "k_inv_gj",
+            PT_GEMM_INV: "k_gemm_w<2>", PT_INV_PANEL: "k_inv_gj",
             PT_INV_SMALL: "k_inv_lds"}
 
 
