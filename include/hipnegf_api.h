@@ -10,7 +10,7 @@
  * Also provided: the local DOS alongside T and the density matrix (contour
  * and real-axis integration of the same block solver).  Entry points outside
  * that path (EFA / quasi-equilibrium densities, k-point meshes, inelastic
- * models, the surface-function disk cache, the UPT reader) are not provided.
+ * models, the UPT reader) are not provided.
  *
  * Errors: the reference stops the process (error stop); here an entry point
  * that fails records a message (negf_get_last_error) and leaves its outputs
@@ -128,6 +128,18 @@ void negf_associate_energy_current(int *handler, int currents_shape[2], double *
 void negf_read_hs(int *handler, const char *real_path, const char *imag_path, int target_matrix);
 void negf_set_output(int *handler, const char *out_path);
 void negf_write_tunneling_and_dos(int *handler);
+
+/* Scratch folder of the surface-function disk cache (negf_set_scratch,
+ * libnegf_api.f90:314-334; default "./GS/", which the caller creates).  With
+ * lnparams.sgfcache = 0 (disk) and readold_t_sgfs 0 (load, else compute and
+ * save) or 2 (compute and save), the Landauer, LDOS and dephasing solves read
+ * and write g_s of every decimated contact and energy point there, one file
+ * Mat<u|d><contact:2>_00_<kpoint>_<point>.dat each, in the reference's
+ * format (ln_cache.F90:458-594, outmatrix.F90:187-245: Fortran unformatted
+ * sequential records of int32 i, int32 j, complex(dp) value, for the entries
+ * that are not zero); sgfcache = 1 keeps them in device memory, 2 never
+ * caches.  The density paths use the device-memory cache. */
+void negf_set_scratch(int *handler, const char *scratch_path);
 
 /* Density matrix (negf_solve_density_dft, negf_get_dm, negf_set_outer;
  * libnegf_api.f90:780-791, 836-861, 1052-1065): compute_density_dft

@@ -15,6 +15,8 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -66,6 +68,7 @@ struct Session {
   bool tun_set = false;
   std::vector<int> tun_proj;
   std::string out_path = "./";  // negf%out_path (set_outpath appends '/')
+  std::string scratch = "./GS/";  // negf%scratch_path (lib_param.F90:607; set_scratch appends '/')
   // density matrix (negf%rho, CSR 0-based here) and negf%outer_blocks
   int outer = 2;
   int rho_n = 0;
@@ -322,9 +325,11 @@ hn::DevStruct &device_struct(Session &s) {
 hn::SolveOpts opts_of(Session &s) {
   hn::SolveOpts o;
   // readOldT_SGFs (lib_param.F90): 0 load cached g_s (else compute and save),
-  // 1 compute, 2 compute and save -- kept in device memory here
+  // 1 compute, 2 compute and save.  SGFcache 1 keeps them in device memory,
+  // 0 in files of the scratch folder (staged through the device cache, see
+  // DiskSgf), 2 nowhere (TMatrixCacheDummy: never cached, so always computed)
   const int r = s.p.readold_t_sgfs;
-  o.sgf_mode = (r == 0) ? 1 : (r == 2) ? 2 : 0;
+  o.sgf_mode = s.p.sgfcache == 2 ? 0 : (r == 0) ? 1 : (r == 2) ? 2 : 0;
   if (o.sgf_mode && !s.sgf_cache_on) {
     engine(s).set_sgf_cache(2ull << 30);
     s.sgf_cache_on = true;
@@ -334,6 +339,141 @@ hn::SolveOpts opts_of(Session &s) {
   o.max_batch = s.max_batch;
   return o;
 }
+
+// ---------------------------------------------------------------------------
+// Disk cache of surface Green's functions (SGFcache = 0; ln_cache.F90
+// TMatrixCacheDisk :458-594, outmat_c / inmat_c outmatrix.F90:187-245): one
+// file per (spin, contact, k-point, energy point) in the scratch folder,
+// named Mat<u|d><contact, 2 digits>_00_<k-point>_<point>.dat.  A file is a
+// Fortran unformatted sequential file holding one record per entry of g_s
+// that is not zero (outmat_c's threshold EPS is never set by the library, so
+// it is 0), in column order: int32 i, int32 j (1-based), complex(dp) g(i,j).
+// As gfortran writes it, each record is framed by its byte length (int32, 24)
+// before and after.  Reading starts from zeros and sets the entries listed.
+// ---------------------------------------------------------------------------
+std::string label_str(int v) {  // get_string_label, ln_cache.F90:669-679
+  if (v > 9999999) throw std::runtime_error("ERROR: label too large (> 9999999)");
+  const int w = v > 999999 ? 7 : v > 99999 ? 6 : v > 9999 ? 5 : v > 999 ? 4 : v > 99 ? 3 : 2;
+  char b[16];
+  if (v < 0 && v > -10) std::snprintf(b, sizeof(b), "%s", "**");  // i2.2 overflow, as Fortran prints it
+  else std::snprintf(b, sizeof(b), "%0*d", w, v);
+  return b;
+}
+
+std::string sgf_filename(const Session &s, int contact, int point) {  // disk_indices_to_filename
+  if (s.p.spin != 1 && s.p.spin != 2) throw std::runtime_error("surface-function files: spin must be 1 or 2");
+  char row[8];
+  std::snprintf(row, sizeof(row), "%02d", contact);
+  return s.scratch + "Mat" + (s.p.spin == 1 ? "u" : "d") + row + "_00_" + label_str(s.p.kpoint) + "_" +
+         label_str(point) + ".dat";
+}
+
+void sgf_write_file(const std::string &fn, const double2 *g, int n) {
+  FILE *f = std::fopen(fn.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot open " + fn + " (does the scratch folder exist?)");
+  std::vector<char> buf;
+  buf.reserve((size_t)n * n * 32);
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) {
+      const double2 v = g[(size_t)j * n + i];
+      if (!(std::fabs(v.x) > 0.0 || std::fabs(v.y) > 0.0)) continue;
+      const int32_t len = 24, ii = i + 1, jj = j + 1;
+      char r[32];
+      std::memcpy(r, &len, 4);
+      std::memcpy(r + 4, &ii, 4);
+      std::memcpy(r + 8, &jj, 4);
+      std::memcpy(r + 12, &v.x, 8);
+      std::memcpy(r + 20, &v.y, 8);
+      std::memcpy(r + 28, &len, 4);
+      buf.insert(buf.end(), r, r + 32);
+    }
+  const bool ok = std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  if (std::fclose(f) != 0 || !ok) throw std::runtime_error("cannot write " + fn);
+}
+
+bool sgf_read_file(const std::string &fn, std::vector<double2> &g, int n) {
+  FILE *f = std::fopen(fn.c_str(), "rb");
+  if (!f) return false;
+  g.assign((size_t)n * n, double2{0.0, 0.0});
+  char r[32];
+  size_t got;
+  while ((got = std::fread(r, 1, 32, f)) > 0) {
+    int32_t l0, l1, i, j;
+    double2 v;
+    std::memcpy(&l0, r, 4);
+    std::memcpy(&l1, r + 28, 4);
+    std::memcpy(&i, r + 4, 4);
+    std::memcpy(&j, r + 8, 4);
+    std::memcpy(&v.x, r + 12, 8);
+    std::memcpy(&v.y, r + 20, 8);
+    if (got != 32 || l0 != 24 || l1 != 24 || i < 1 || i > n || j < 1 || j > n) {
+      std::fclose(f);
+      throw std::runtime_error("malformed surface-function file " + fn);
+    }
+    g[(size_t)(j - 1) * n + (i - 1)] = v;
+  }
+  std::fclose(f);
+  return true;
+}
+
+// Wraps one real-axis solve of npoints energies (pt_path = point index + 1,
+// integrations.F90:1637): with SGFcache 0 and readOldT_SGFs 0 the files that
+// exist are loaded into the device cache before the solve (a point whose
+// files are all there is not recomputed, ncyc 0); after the solve g_s is
+// written for every point that had no file (readOldT_SGFs 0) or for every
+// point (readOldT_SGFs 2), as surface_green does (contselfenergy.F90:161-259).
+struct DiskSgf {
+  Session &s;
+  hn::DevStruct &st;
+  hn::Engine &eng;
+  hn::SolveOpts o;
+  bool on = false;
+  std::vector<char> save;  // (point, contact) written after the solve
+  DiskSgf(Session &s_, hn::DevStruct &st_, hn::Engine &eng_, const hn::SolveOpts &o_)
+      : s(s_), st(st_), eng(eng_), o(o_) {
+    on = s.p.sgfcache == 0 && o.sgf_mode != 0;
+    if (!on) return;
+    const int n = s.npoints, nc = st.ncont;
+    long long ent = 0;
+    int live = 0;
+    for (int c = 0; c < nc; ++c)
+      if (!st.fict[c]) {
+        ent = std::max(ent, (long long)st.ngs(c) * st.ngs(c));
+        ++live;
+      }
+    if (!live) { on = false; return; }
+    // every entry of this solve stays resident until it is written
+    const size_t need = (size_t)ent * sizeof(double2) * (size_t)n * live + ((size_t)16 << 20);
+    if (need > ((size_t)64 << 30))
+      throw std::runtime_error("surface-function files: more than 64 GiB of g_s in one solve");
+    if (eng.sgf_cache_bytes() < need) {
+      eng.set_sgf_cache(need);
+      s.sgf_cache_on = true;
+    }
+    save.assign((size_t)n * nc, 0);
+    std::vector<double2> g;
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < nc; ++c) {
+        if (st.fict[c]) continue;
+        const bool have = o.sgf_mode == 1 && sgf_read_file(sgf_filename(s, c + 1, i + 1), g, st.ngs(c));
+        if (have) eng.sgf_insert(st, o, 0, s.en[i], c, g.data(), 0);
+        else save[(size_t)i * nc + c] = 1;
+      }
+  }
+  void finish() {
+    if (!on) return;
+    const int n = s.npoints, nc = st.ncont;
+    std::vector<double2> g;
+    for (int i = 0; i < n; ++i)
+      for (int c = 0; c < nc; ++c) {
+        if (!save[(size_t)i * nc + c]) continue;
+        g.resize((size_t)st.ngs(c) * st.ngs(c));
+        int stat = 0;
+        if (eng.sgf_fetch(st, o, 0, s.en[i], c, g.data(), &stat))
+          sgf_write_file(sgf_filename(s, c + 1, i + 1), g.data(), st.ngs(c));
+      }
+  }
+};
 
 // electron_current, integrations.F90:2427-2457
 void electron_current(Session &s) {
@@ -401,8 +541,10 @@ void compute_landauer_dos(Session &s, const std::vector<int> &ni, const std::vec
     d.idx = dP.as<int>() + ptr.size();
   }
   hn::SolveOpts o = opts_of(s);
+  DiskSgf disk(s, *st, eng, o);
   eng.transmission_dos(*st, n, dE.as<double>(), dsl.as<int>(), o, s.npairs, ni.data(), nf.data(), d,
                        dTp.as<double>(), dL.as<double>(), dN.as<int>(), dS.as<int>());
+  disk.finish();
   std::vector<double> T((size_t)n * s.npairs), L((size_t)n * nout);
   HN_CHECK(hipMemcpy(T.data(), dTp.p, sizeof(double) * T.size(), hipMemcpyDeviceToHost));
   HN_CHECK(hipMemcpy(L.data(), dL.p, sizeof(double) * L.size(), hipMemcpyDeviceToHost));
@@ -447,9 +589,11 @@ void compute_landauer(Session &s) {
   HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
   DevBuf dTp(sizeof(double) * n * s.npairs);
   hn::SolveOpts o = opts_of(s);
+  DiskSgf disk(s, *st, eng, o);
   // every lead pair in one batched solve: contacts and gsmr blocks are shared
   eng.transmission_pairs(*st, n, dE.as<double>(), dsl.as<int>(), o, s.npairs, ni.data(), nf.data(),
                          dTp.as<double>(), dN.as<int>(), dS.as<int>());
+  disk.finish();
   std::vector<double> T((size_t)n * s.npairs);
   HN_CHECK(hipMemcpy(T.data(), dTp.p, sizeof(double) * n * s.npairs, hipMemcpyDeviceToHost));
   HN_CHECK(hipMemcpy(s.item_status.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
@@ -489,8 +633,10 @@ void compute_dephasing_transmission(Session &s) {
   so.frm_per_item = 0;
   so.ref = s.ncont + 1;
   hn::SolveOpts o = opts_of(s);
+  DiskSgf disk(s, *st, eng, o);
   eng.scba_mw(*st, n, dE.as<double>(), dsl.as<int>(), o, so, dC.as<double>(), dI.as<int>(),
               dErr.as<double>(), dS.as<int>(), nullptr, nullptr);
+  disk.finish();
   std::vector<double> cm((size_t)n * s.ncont);
   HN_CHECK(hipMemcpy(cm.data(), dC.p, sizeof(double) * n * s.ncont, hipMemcpyDeviceToHost));
   s.item_status.assign(n, 0);
@@ -1167,6 +1313,11 @@ void negf_read_hs(int *handler, const char *real_path, const char *imag_path, in
 }
 
 // set_outpath (libnegf.F90:943-949)
+// set_scratch, libnegf.F90:922-940 (the folder must exist, as in the reference)
+void negf_set_scratch(int *handler, const char *scratch_path) {
+  api(handler, [&](Session &s) { s.scratch = cstr(scratch_path) + "/"; });
+}
+
 void negf_set_output(int *handler, const char *out_path) {
   api(handler, [&](Session &s) { s.out_path = cstr(out_path) + "/"; });
 }

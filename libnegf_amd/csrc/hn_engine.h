@@ -213,6 +213,15 @@ class Engine {
   void set_sgf_cache(size_t bytes);
   long long sgf_hits() const { return sgf_hits_; }
   long long sgf_stores() const { return sgf_stores_; }
+  size_t sgf_cache_bytes() const { return sgf_bytes_; }
+  // One cached g_s (column-major, order ngs(c)) of item (slot, E) under the
+  // key a solve with options o uses: copied to host memory (false when it is
+  // not cached) or inserted from it, with its decimation status.  These serve
+  // the C API's disk cache of surface functions.
+  bool sgf_fetch(DevStruct &st, const SolveOpts &o, int slot, double E, int c, double2 *host,
+                 int *status);
+  void sgf_insert(DevStruct &st, const SolveOpts &o, int slot, double E, int c,
+                  const double2 *host, int status);
   // Sub-engines (own stream and workspace) for concurrent item groups.
   Engine &lane(int i);
   size_t workspace_bytes() const { return arena_.cap; }
@@ -254,6 +263,9 @@ class Engine {
   unsigned long long sgf_clock_ = 0;
   long long sgf_hits_ = 0, sgf_stores_ = 0;
   bool sgf_prepare(long long entry_elems);
+  SgfKey sgf_key(const DevStruct &st, double E, double delta, int slot, int c, bool shared) const;
+  bool sgf_shared(DevStruct &st, const SolveOpts &o);
+  static long long sgf_entry_of(const DevStruct &st);  // elements per entry: max ngs^2
   int sgf_slot_for(const SgfKey &k);  // insert (evicting LRU), returns entry index
   struct GraphEntry {
     std::vector<long long> key;

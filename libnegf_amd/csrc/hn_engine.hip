@@ -524,6 +524,56 @@ int Engine::sgf_slot_for(const SgfKey &k) {
   return victim;
 }
 
+Engine::SgfKey Engine::sgf_key(const DevStruct &st, double E, double delta, int slot, int c,
+                               bool shared) const {
+  SgfKey k;
+  k.uid = st.uid;
+  k.version = st.version;
+  std::memcpy(&k.ebits, &E, sizeof(double));
+  std::memcpy(&k.dbits, &delta, sizeof(double));
+  k.slot = slot;
+  k.contact = c + (shared ? 64 : 0);  // the shared recursion rounds differently
+  k.nc = st.ngs(c);
+  return k;
+}
+
+bool Engine::sgf_shared(DevStruct &st, const SolveOpts &o) {
+  return o.share_mirror && st.ncont == 2 && !st.fict[0] && !st.fict[1] && mirror_contacts(st);
+}
+
+long long Engine::sgf_entry_of(const DevStruct &st) {
+  long long m = 0;
+  for (int c = 0; c < st.ncont; ++c)
+    if (!st.fict[c]) m = std::max(m, (long long)st.ngs(c) * st.ngs(c));
+  return m;
+}
+
+bool Engine::sgf_fetch(DevStruct &st, const SolveOpts &o, int slot, double E, int c, double2 *host,
+                       int *status) {
+  if (c < 0 || c >= st.ncont || st.fict[c]) return false;
+  if (!sgf_pool_ || sgf_entry_elems_ < sgf_entry_of(st)) return false;
+  auto it = sgf_map_.find(sgf_key(st, E, o.delta, slot, c, sgf_shared(st, o)));
+  if (it == sgf_map_.end()) return false;
+  const long long s2 = (long long)st.ngs(c) * st.ngs(c);
+  HN_CHECK(hipStreamSynchronize(stream_));
+  HN_CHECK(hipMemcpy(host, sgf_pool_ + (long long)it->second * sgf_entry_elems_,
+                     sizeof(double2) * s2, hipMemcpyDeviceToHost));
+  if (status) *status = sgf_status_[it->second];
+  return true;
+}
+
+void Engine::sgf_insert(DevStruct &st, const SolveOpts &o, int slot, double E, int c,
+                        const double2 *host, int status) {
+  if (c < 0 || c >= st.ncont || st.fict[c]) throw std::runtime_error("sgf_insert: not a decimated contact");
+  if (!sgf_prepare(sgf_entry_of(st))) throw std::runtime_error("sgf_insert: no surface-function cache");
+  const int idx = sgf_slot_for(sgf_key(st, E, o.delta, slot, c, sgf_shared(st, o)));
+  sgf_status_[idx] = status;
+  const long long s2 = (long long)st.ngs(c) * st.ngs(c);
+  HN_CHECK(hipStreamSynchronize(stream_));
+  HN_CHECK(hipMemcpy(sgf_pool_ + (long long)idx * sgf_entry_elems_, host, sizeof(double2) * s2,
+                     hipMemcpyHostToDevice));
+}
+
 // Contact self-energies of all contacts for B items.
 // Sig: ncont x B blocks of nd x nd (nd = bsz[cblk-1]), stride sSig, contact-major.
 void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
@@ -556,7 +606,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     const int nc = st.ngs(c);
     fill_diag(Go, (long long)nc * nc, nc, Z(0.0, -3.141592653589793 * st.contact_dos[c]), B, stream_);
   };
-  const bool shared = o.share_mirror && ncont == 2 && !st.fict[0] && !st.fict[1] && mirror_contacts(st);
+  const bool shared = sgf_shared(st, o);
 
   // ---- surface Green's function cache ----
   int ncmax = 0;
@@ -575,17 +625,7 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     HN_CHECK(hipStreamSynchronize(stream_));
     didx = arena_.take<int>(B);
   }
-  auto key = [&](int b, int c) {
-    SgfKey k;
-    k.uid = st.uid;
-    k.version = st.version;
-    std::memcpy(&k.ebits, &hE[b], sizeof(double));
-    std::memcpy(&k.dbits, &o.delta, sizeof(double));
-    k.slot = hS[b];
-    k.contact = c + (shared ? 64 : 0);  // the shared recursion rounds differently
-    k.nc = st.ngs(c);
-    return k;
-  };
+  auto key = [&](int b, int c) { return sgf_key(st, hE[b], o.delta, hS[b], c, shared); };
   if (cache && o.sgf_mode == 1) {  // ReadOldSGF = 0: load when every item is cached
     bool all = true;
     for (int c = 0; c < ncont && all; ++c)

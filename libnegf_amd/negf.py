@@ -168,6 +168,7 @@ class NEGF:
         contend, surfend, plend, cblk = _i32(contend), _i32(surfend), _i32(plend), _i32(cblk)
         surfstart = surfend + 1 if surfstart is None else _i32(surfstart)
         self._call("negf_init_contacts", C.c_int(ncont))
+        self.ncont = int(ncont)
         self._call("negf_init_structure", C.c_int(ncont), hip._ptr(surfstart), hip._ptr(surfend),
                    hip._ptr(contend), C.c_int(npl), hip._ptr(plend), hip._ptr(cblk))
         self.get_params()
@@ -203,6 +204,11 @@ class NEGF:
 
     def set_output(self, path: str):
         self._call("negf_set_output", C.c_char_p(path.encode()))
+
+    def set_scratch(self, path: str):
+        """Folder of the surface-function files (``negf_set_scratch``,
+        libnegf_api.f90:314-334); used when ``params.sgfcache == 0``."""
+        self._call("negf_set_scratch", C.c_char_p(path.encode()))
 
     def write_tun_and_dos(self):
         """tunneling_*.dat and localDOS_*.dat in the output path."""
@@ -310,3 +316,14 @@ class NEGF:
         st = np.zeros(n.value, dtype=np.int32)
         self._call("negf_get_item_status", C.byref(n), hip._ptr(st), None, C.c_int(1))
         return st
+
+    def get_item_ncyc(self):
+        """Decimation iteration counts of the last solve, npoints x ncont
+        entries (0 for a surface function loaded from a cache)."""
+        n = C.c_int()
+        d = np.zeros(1, dtype=np.int32)
+        self._call("negf_get_item_status", C.byref(n), hip._ptr(d), None, C.c_int(0))
+        st = np.zeros(n.value, dtype=np.int32)
+        nc = np.zeros(n.value * max(getattr(self, "ncont", 0), 1), dtype=np.int32)
+        self._call("negf_get_item_status", C.byref(n), hip._ptr(st), hip._ptr(nc), C.c_int(1))
+        return nc
