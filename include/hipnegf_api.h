@@ -88,6 +88,17 @@ void negf_init_structure(int *handler, int ncont, int *surfstart, int *surfend, 
                          int npl, int *plend, int *cblk);
 
 /* parameters (libnegf_api.f90:338-365) */
+/* Structure helpers (libnegf_api.f90:190-269, 102-116): negf_contact_blocks
+ * finds, from the rows surfstart..contend of H, the device PL each contact
+ * couples to (find_cblocks, libnegf.F90:2172-2234; an error when two);
+ * negf_get_pls returns npl and, with copy != 0, plend, ncont and cblk;
+ * negf_get_version prints the library version. */
+void negf_contact_blocks(int *handler, int ncont, int *surfstart, int *surfend, int *contend,
+                         int npl, int *plend, int *cblk);
+void negf_get_pls(int *handler, int *npl, int *ncont, int *plend, int *cblk, int copy);
+void negf_get_version(int *handler);
+/* min_or_max of the density paths (libnegf_api.f90:1172-1185) */
+void negf_set_reference(int *handler, int *minmax);
 void negf_set_params(int *handler, struct lnparams *params);
 void negf_get_params(int *handler, struct lnparams *params);
 

@@ -1083,6 +1083,66 @@ void negf_init_structure(int *handler, int ncont, int *surfstart, int *surfend, 
   });
 }
 
+// negf_contact_blocks -> find_cblocks (libnegf.F90:2172-2234): for each
+// contact the device PL holding every device column that the rows
+// surfstart..contend of H touch; an error when they span two PLs.
+void negf_contact_blocks(int *handler, int ncont, int *surfstart, int *surfend, int *contend,
+                         int npl, int *plend, int *cblk) {
+  (void)surfend;
+  api(handler, [&](Session &s) {
+    if (s.cur >= (int)s.H.size() || !s.H[s.cur].present())
+      throw std::runtime_error("Error: H not created before invoking negf_contact_block");
+    if (npl <= 0) throw std::runtime_error("negf_contact_blocks: npl must be positive");
+    const Csr &H = s.H[s.cur];
+    const int last = plend[npl - 1];
+    std::vector<int> out(ncont);
+    for (int j = 0; j < ncont; ++j) {
+      int mx = 0, mn = 400000000;
+      for (int k = surfstart[j]; k <= contend[j]; ++k) {
+        if (k < 1 || k > H.n) throw std::runtime_error("negf_contact_blocks: contact rows outside H");
+        for (int i = H.ptr[k - 1]; i < H.ptr[k]; ++i) {
+          const int col = H.col[i] + 1;
+          if (col <= last) { mn = std::min(mn, col); mx = std::max(mx, col); }
+        }
+      }
+      for (int k = 0; k < npl; ++k)
+        if (mx <= plend[k]) {
+          out[j] = k + 1;
+          if (mn >= (k ? plend[k - 1] + 1 : 1)) break;
+          throw std::runtime_error("ERROR: contact interacting with more than one block (contact " +
+                                   std::to_string(j + 1) + ")");
+        }
+    }
+    std::copy(out.begin(), out.end(), cblk);
+  });
+}
+
+// negf_get_pls (libnegf_api.f90:245-269): npl, and with copy != 0 plend,
+// ncont and cblk of the structure
+void negf_get_pls(int *handler, int *npl, int *ncont, int *plend, int *cblk, int copy) {
+  api(handler, [&](Session &s) {
+    *npl = s.npl;
+    if (!copy) return;
+    std::copy(s.plend.begin(), s.plend.end(), plend);
+    *ncont = s.ncont;
+    std::copy(s.cblk.begin(), s.cblk.end(), cblk);
+  });
+}
+
+// negf_set_reference (libnegf_api.f90:1172-1185): min_or_max of the density
+// paths (0 the contact of lowest mu is the reference, 1 the highest)
+void negf_set_reference(int *handler, int *minmax) {
+  api(handler, [&](Session &s) { s.p.min_or_max = *minmax; });
+}
+
+// negf_get_version (negf_version, libnegf.F90:1295-1303)
+void negf_get_version(int *handler) {
+  api(handler, [&](Session &) {
+    std::printf("(libnegf) version: 1.%20s  %10s\n", "hipnegf", __DATE__);
+    std::fflush(stdout);
+  });
+}
+
 void negf_set_params(int *handler, struct lnparams *params) {
   api(handler, [&](Session &s) { s.p = *params; });
 }

@@ -173,6 +173,33 @@ class NEGF:
                    hip._ptr(contend), C.c_int(npl), hip._ptr(plend), hip._ptr(cblk))
         self.get_params()
 
+    def contact_blocks(self, ncont, surfstart, surfend, contend, npl, plend):
+        """Device PL each contact couples to, found from H (``negf_contact_blocks``)."""
+        surfstart, surfend, contend, plend = _i32(surfstart), _i32(surfend), _i32(contend), _i32(plend)
+        cblk = np.zeros(ncont, dtype=np.int32)
+        self._call("negf_contact_blocks", C.c_int(ncont), hip._ptr(surfstart), hip._ptr(surfend),
+                   hip._ptr(contend), C.c_int(npl), hip._ptr(plend), hip._ptr(cblk))
+        return cblk
+
+    def get_pls(self):
+        """(plend, cblk) of the structure (``negf_get_pls``)."""
+        npl, nc = C.c_int(), C.c_int()
+        d = np.zeros(1, dtype=np.int32)
+        self._call("negf_get_pls", C.byref(npl), C.byref(nc), hip._ptr(d), hip._ptr(d), C.c_int(0))
+        plend = np.zeros(max(npl.value, 1), dtype=np.int32)
+        cblk = np.zeros(MAXCONT, dtype=np.int32)
+        self._call("negf_get_pls", C.byref(npl), C.byref(nc), hip._ptr(plend), hip._ptr(cblk), C.c_int(1))
+        return plend[:npl.value], cblk[:nc.value]
+
+    def set_reference(self, minmax: int):
+        """Reference contact of the density paths: 0 lowest mu, 1 highest."""
+        v = C.c_int(minmax)
+        self._call("negf_set_reference", C.byref(v))
+        self.params.min_or_max = minmax
+
+    def get_version(self):
+        self._call("negf_get_version")
+
     def set_elph_dephasing(self, coupling, niter: int):
         cp = np.ascontiguousarray(coupling, dtype=np.float64)
         self._call("negf_set_elph_dephasing", hip._ptr(cp), C.c_int(cp.size), C.c_int(niter))

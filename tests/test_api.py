@@ -74,6 +74,39 @@ def test_defaults_and_host_errors():
     n.close()
 
 
+def test_contact_blocks_get_pls_and_reference():
+    """find_cblocks known answers on the 60-site chain of 4 PLs (contact 1
+    rows 61..80 couple to site 60 in PL 4, contact 2 rows 81..100 to site 1 in
+    PL 1), the two-PL error when a contact row reaches into a second PL, and
+    the structure/reference accessors.  Host only."""
+    import scipy.sparse as sps
+    from libnegf_amd.negf import NEGF, NegfError
+    H = chain_dense()
+    n = NEGF()
+    n.set_h(sps.csr_matrix(H))
+    plend = [15, 30, 45, 60]
+    cb = n.contact_blocks(2, [61, 81], [60, 80], [80, 100], 4, plend)
+    assert list(cb) == [4, 1]
+    # the same PLs split finer: site 60 is then in PL 8, site 1 in PL 1
+    cb = n.contact_blocks(2, [61, 81], [60, 80], [80, 100], 8, [8, 15, 23, 30, 38, 45, 53, 60])
+    assert list(cb) == [8, 1]
+    H2 = H.copy()
+    H2[60, 44] = H2[44, 60] = 0.1  # contact 1 row 61 also touches site 45 (PL 3)
+    m = NEGF()
+    m.set_h(sps.csr_matrix(H2))
+    with pytest.raises(NegfError, match="more than one block"):
+        m.contact_blocks(2, [61, 81], [60, 80], [80, 100], 4, plend)
+    m.close()
+    n.init_structure(2, [80, 100], [60, 80], 4, plend, [4, 1], surfstart=[61, 81])
+    pe, cbk = n.get_pls()
+    assert list(pe) == plend and list(cbk) == [4, 1]
+    assert n.params.min_or_max == 1
+    n.set_reference(0)
+    n.get_params()
+    assert n.params.min_or_max == 0
+    n.close()
+
+
 def write_petsc(path_re, path_im, M):
     """PETSc ASCII pair as read_H reads it (input_output.F90:52-60, 84-100):
     4 header lines, then one "j i value" line per nonzero (1-based)."""
