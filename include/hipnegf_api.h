@@ -9,7 +9,7 @@
  * unchanged for the Landauer / Meir-Wingreen / elastic-dephasing path.
  * Also provided: the local DOS alongside T and the density matrix (contour
  * and real-axis integration of the same block solver).  Entry points outside
- * that path (EFA / quasi-equilibrium densities, k-point meshes, inelastic
+ * that path (the quasi-equilibrium density, k-point meshes, inelastic
  * models, the UPT reader) are not provided.
  *
  * Errors: the reference stops the process (error stop); here an entry point
@@ -165,6 +165,15 @@ void negf_solve_density_dft(int *handler);
 void negf_get_dm(int *handler, int *nnz, int *nrow, int *rowpnt, int *colind, double *re_nzval,
                  double *im_nzval, int copy);
 void negf_set_outer(int *handler, int *outer);
+/* Electron (particle 1) or hole (-1) density of compute_density_efa
+ * (negf_density_efa, libnegf_api.f90:724-749; libnegf.F90:1690-1791):
+ * contour from Ec - DeltaEc (holes: Ev + DeltaEv) with Np_n (Np_p) points
+ * and n_poles poles around mu_n (mu_p) of contact 1, then np_real trapezoid
+ * points on the real axis (integrations.F90:281-592, 1346-1462; holes
+ * integrate A - G^n).  density[k] = Re rho(k, k) for k < ndofs; rho stays
+ * readable with negf_get_dm.  contacts = true (the bulk contact density) is
+ * not provided and reports an error. */
+void negf_density_efa(int *handler, int *ndofs, double *density, int *particle, bool *contacts);
 
 /* Local DOS alongside the transmission (negf_init_ldos / negf_set_ldos_intervals /
  * negf_set_ldos_indexes / negf_associate_ldos, libnegf_api.f90:941-1021):

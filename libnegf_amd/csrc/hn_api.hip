@@ -830,72 +830,118 @@ SparseAcc place_blocks(Session &s, const hn::DevStruct &st, const std::vector<do
   return A;
 }
 
+// set_ref_cont (libnegf.F90:2132-2157): reference contact and its mu
+void set_ref_cont(const Session &s, int &refcont, double &muref) {
+  int imin = 0, imax = 0;
+  for (int c = 1; c < s.ncont; ++c) {
+    if (s.p.mu[c] < s.p.mu[imin]) imin = c;
+    if (s.p.mu[c] > s.p.mu[imax]) imax = c;
+  }
+  if (s.p.min_or_max == 0) { muref = s.p.mu[imin]; refcont = imin + 1; }
+  else if (s.p.min_or_max == 1) { muref = s.p.mu[imax]; refcont = imax + 1; }
+  else { muref = s.p.mu[imax]; refcont = s.ncont + 1; }
+}
+
+// solve options of the density paths: readOldDM_SGFs selects the cache use
+hn::SolveOpts dm_opts(Session &s) {
+  hn::SolveOpts o = opts_of(s);
+  const int r = s.p.readold_dm_sgfs;
+  o.sgf_mode = s.p.sgfcache == 2 ? 0 : (r == 0) ? 1 : (r == 2) ? 2 : 0;
+  if (o.sgf_mode && !s.sgf_cache_on) {
+    engine(s).set_sgf_cache(2ull << 30);
+    s.sgf_cache_on = true;
+  }
+  return o;
+}
+
+// rho += i (T - T^H) (zspectral) or rho += sign T, over T's entries
+void add_spectral(SparseAcc &rho, const SparseAcc &T, int N) {
+  for (int r = 0; r < N; ++r)
+    for (const auto &e : T.rows[r]) {
+      const double2 v = e.second;
+      rho.add(r, e.first, make_double2(-v.y, v.x));   // i T(r,c)
+      rho.add(e.first, r, make_double2(-v.y, -v.x));  // -i conj(T(r,c)) at (c,r)
+    }
+}
+void add_plain(SparseAcc &rho, const SparseAcc &T, int N, double sign) {
+  for (int r = 0; r < N; ++r)
+    for (const auto &e : T.rows[r]) rho.add(r, e.first, make_double2(sign * e.second.x, sign * e.second.y));
+}
+
+// negf%rho as 0-based CSR
+void store_rho(Session &s, SparseAcc &rho, int N) {
+  rho.finalize();
+  s.rho_n = N;
+  s.rho_ptr.assign(N + 1, 0);
+  s.rho_col.clear();
+  s.rho_val.clear();
+  for (int r = 0; r < N; ++r) {
+    for (const auto &e : rho.rows[r]) {
+      s.rho_col.push_back(e.first);
+      s.rho_val.push_back(e.second);
+    }
+    s.rho_ptr[r + 1] = (int)s.rho_col.size();
+  }
+}
+
+// One batched pass of the density engine over the points z with weights w
+// (mode 0: sum w Gr on the pattern; 1: sum w Gn with occupations frm).
+std::vector<double2> density_run(Session &s, hn::DevStruct &st, hn::Engine &eng, const hn::SolveOpts &o,
+                               long long tot, int refcont, int mode, const std::vector<cplx> &z,
+                               const std::vector<cplx> &w, const std::vector<double> &frm,
+                               bool real_device = false) {
+  const int n = (int)z.size();
+  std::vector<double> Er(n), Ei(n);
+  std::vector<double2> ww(n);
+  for (int i = 0; i < n; ++i) {
+    Er[i] = z[i].real();
+    Ei[i] = z[i].imag();
+    ww[i] = make_double2(w[i].real(), w[i].imag());
+  }
+  DevBuf dE(sizeof(double) * n), dEi(sizeof(double) * n), dsl(sizeof(int) * n), dw(sizeof(double2) * n),
+      dF(sizeof(double) * std::max<size_t>(frm.size(), 1)), dA(sizeof(double2) * std::max<long long>(tot, 1)),
+      dS(sizeof(int) * n);
+  HN_CHECK(hipMemcpy(dE.p, Er.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dEi.p, Ei.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
+  HN_CHECK(hipMemcpy(dw.p, ww.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+  if (!frm.empty()) HN_CHECK(hipMemcpy(dF.p, frm.data(), sizeof(double) * frm.size(), hipMemcpyHostToDevice));
+  HN_CHECK(hipMemset(dA.p, 0, sizeof(double2) * tot));
+  hn::DensityOpts d;
+  d.mode = mode;
+  d.w = dw.as<double2>();
+  d.frm = mode == 1 ? dF.as<double>() : nullptr;
+  d.ref = refcont;
+  d.outer = s.outer;
+  d.status = dS.as<int>();
+  d.real_device = real_device ? 1 : 0;
+  eng.density(st, n, dE.as<double>(), mode == 0 ? dEi.as<double>() : nullptr, dsl.as<int>(), o, d,
+              dA.as<double2>());
+  std::vector<double2> acc(tot);
+  HN_CHECK(hipMemcpy(acc.data(), dA.p, sizeof(double2) * tot, hipMemcpyDeviceToHost));
+  std::vector<int> stv(n);
+  HN_CHECK(hipMemcpy(stv.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  s.item_status.insert(s.item_status.end(), stv.begin(), stv.end());
+  return acc;
+}
+
 void compute_density(Session &s) {
   if (s.ncont < 1) throw std::runtime_error("density matrix needs at least one contact");
   hn::DevStruct *st = &device_struct(s);
   hn::Engine &eng = engine(s);
   const int ncont = s.ncont, N = s.H[s.cur].n;
-  // set_ref_cont
   int refcont;
   double muref;
-  {
-    int imin = 0, imax = 0;
-    for (int c = 1; c < ncont; ++c) {
-      if (s.p.mu[c] < s.p.mu[imin]) imin = c;
-      if (s.p.mu[c] > s.p.mu[imax]) imax = c;
-    }
-    if (s.p.min_or_max == 0) { muref = s.p.mu[imin]; refcont = imin + 1; }
-    else if (s.p.min_or_max == 1) { muref = s.p.mu[imax]; refcont = imax + 1; }
-    else { muref = s.p.mu[imax]; refcont = ncont + 1; }
-  }
+  set_ref_cont(s, refcont, muref);
   const bool energy = s.p.dore[0] == 'E';
-  hn::SolveOpts o = opts_of(s);
-  {  // readOldDM_SGFs for the density paths
-    const int r = s.p.readold_dm_sgfs;
-    o.sgf_mode = (r == 0) ? 1 : (r == 2) ? 2 : 0;
-    if (o.sgf_mode && !s.sgf_cache_on) {
-      eng.set_sgf_cache(2ull << 30);
-      s.sgf_cache_on = true;
-    }
-  }
+  hn::SolveOpts o = dm_opts(s);
   const std::vector<long long> lay = hn::Engine::density_layout(*st);
   const long long tot = lay.back();
   SparseAcc rho(N);
   s.item_status.clear();
   auto run = [&](int mode, const std::vector<cplx> &z, const std::vector<cplx> &w,
                  const std::vector<double> &frm) {
-    const int n = (int)z.size();
-    std::vector<double> Er(n), Ei(n);
-    std::vector<double2> ww(n);
-    for (int i = 0; i < n; ++i) {
-      Er[i] = z[i].real();
-      Ei[i] = z[i].imag();
-      ww[i] = make_double2(w[i].real(), w[i].imag());
-    }
-    DevBuf dE(sizeof(double) * n), dEi(sizeof(double) * n), dsl(sizeof(int) * n), dw(sizeof(double2) * n),
-        dF(sizeof(double) * std::max<size_t>(frm.size(), 1)), dA(sizeof(double2) * std::max<long long>(tot, 1)),
-        dS(sizeof(int) * n);
-    HN_CHECK(hipMemcpy(dE.p, Er.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HN_CHECK(hipMemcpy(dEi.p, Ei.data(), sizeof(double) * n, hipMemcpyHostToDevice));
-    HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
-    HN_CHECK(hipMemcpy(dw.p, ww.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
-    if (!frm.empty()) HN_CHECK(hipMemcpy(dF.p, frm.data(), sizeof(double) * frm.size(), hipMemcpyHostToDevice));
-    HN_CHECK(hipMemset(dA.p, 0, sizeof(double2) * tot));
-    hn::DensityOpts d;
-    d.mode = mode;
-    d.w = dw.as<double2>();
-    d.frm = mode == 1 ? dF.as<double>() : nullptr;
-    d.ref = refcont;
-    d.outer = s.outer;
-    d.status = dS.as<int>();
-    eng.density(*st, n, dE.as<double>(), mode == 0 ? dEi.as<double>() : nullptr, dsl.as<int>(), o, d,
-                dA.as<double2>());
-    std::vector<double2> acc(tot);
-    HN_CHECK(hipMemcpy(acc.data(), dA.p, sizeof(double2) * tot, hipMemcpyDeviceToHost));
-    std::vector<int> stv(n);
-    HN_CHECK(hipMemcpy(stv.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
-    s.item_status.insert(s.item_status.end(), stv.begin(), stv.end());
-    return acc;
+    return density_run(s, *st, eng, o, tot, refcont, mode, z, w, frm);
   };
   // en_grid as the reference leaves it: the last integration path's points
   s.en.clear();
@@ -918,13 +964,7 @@ void compute_density(Session &s) {
       s.en_im.push_back(v.imag());
     }
     const std::vector<double2> acc = run(0, z, w, {});
-    SparseAcc T = place_blocks(s, *st, acc, lay, N, s.outer == 2);
-    for (int r = 0; r < N; ++r)
-      for (const auto &e : T.rows[r]) {  // zspectral: i (T - T^H)
-        const double2 v = e.second;
-        rho.add(r, e.first, make_double2(-v.y, v.x));       // i T(r,c)
-        rho.add(e.first, r, make_double2(-v.y, -v.x));      // -i conj(T(r,c)) at (c,r)
-      }
+    add_spectral(rho, place_blocks(s, *st, acc, lay, N, s.outer == 2), N);
   }
   // ---- real_axis_int: rho += sum w Gn on the pattern ----
   if (s.p.np_real > 0) {
@@ -952,21 +992,154 @@ void compute_density(Session &s) {
     }
     o.delta = s.p.delta;
     const std::vector<double2> acc = run(1, z, w, frm);
-    SparseAcc Tn = place_blocks(s, *st, acc, lay, N, s.outer == 2);
-    for (int r = 0; r < N; ++r)
-      for (const auto &e : Tn.rows[r]) rho.add(r, e.first, e.second);
+    add_plain(rho, place_blocks(s, *st, acc, lay, N, s.outer == 2), N, 1.0);
   }
-  rho.finalize();
-  s.rho_n = N;
-  s.rho_ptr.assign(N + 1, 0);
-  s.rho_col.clear();
-  s.rho_val.clear();
-  for (int r = 0; r < N; ++r) {
-    for (const auto &e : rho.rows[r]) {
-      s.rho_col.push_back(e.first);
-      s.rho_val.push_back(e.second);
+  store_rho(s, rho, N);
+}
+
+// trapez, integrations.F90:1529-1547
+void trapez(double x1, double x2, std::vector<double> &x, std::vector<double> &w, int n) {
+  if (n < 2) throw std::runtime_error("trapezoid rule needs at least 2 points (np_real)");
+  const double d = (x2 - x1) / (double)(n - 1);
+  x.assign(n, 0.0);
+  w.assign(n, d);
+  w[0] = w[n - 1] = 0.5 * d;
+  for (int i = 1; i <= n; ++i) x[i - 1] = (x1 * (n - i) + x2 * (i - 1)) / (n - 1);
+}
+
+// Electron (particle 1) or hole (-1) contour of contour_int_n_def /
+// contour_int_p_def (integrations.F90:281-426, 447-592): from the band edge
+// Eb (Ec - DeltaEc, or Ev + DeltaEv) up to Eb + i Lambda, across to
+// muref +- Omega + i Lambda, then the Fermi poles (none when the edge lies
+// beyond muref); holes weight 1 - f and run the other way.
+void efa_contour_grid(int particle, double Eb, double muref, double kbT, int n_kt, int np1, int np2,
+                      int npoles, double g_spin, std::vector<cplx> &z, std::vector<cplx> &w) {
+  const cplx I(0.0, 1.0);
+  const double Omega = n_kt * kbT;
+  const double Lambda = npoles == 0 ? 0.5 * kbT * kPi : 2.0 * npoles * kbT * kPi;
+  if (Eb < muref + 0.001 && Eb > muref - 0.001) Eb = muref - particle * kbT;
+  const int Np = (particle == 1 ? Eb > muref : Eb < muref) ? 0 : npoles;
+  z.clear();
+  w.clear();
+  cplx z1 = Eb, z2 = Eb + I * Lambda;
+  for (int seg = 0; seg < 2; ++seg) {
+    if (seg == 1) {
+      z1 = z2;
+      z2 = muref + particle * Omega + I * Lambda;
     }
-    s.rho_ptr[r + 1] = (int)s.rho_col.size();
+    const cplx zd = z2 - z1;
+    std::vector<double> x, wt;
+    gauleg(0.0, 1.0, x, wt, seg == 0 ? np1 : np2);
+    for (size_t i = 0; i < x.size(); ++i) {
+      const cplx Ec = z1 + x[i] * zd;
+      const cplx ff = particle == 1 ? fermi_c(Ec, muref, kbT) : fermi_c(-Ec, -muref, kbT);
+      z.push_back(Ec);
+      w.push_back((double)particle * g_spin * zd * ff * wt[i] / (2.0 * kPi));
+    }
+  }
+  for (int i = 1; i <= Np; ++i) {
+    z.push_back(muref + I * kbT * kPi * (2.0 * i - 1.0));
+    w.push_back(-(double)particle * I * kbT * g_spin);
+  }
+}
+
+// compute_density_efa (libnegf.F90:1690-1791) without the bulk contact
+// density: the electron (hole) contour with muref = mu_n (mu_p) of contact 1,
+// then the real-axis trapezoid of real_axis_int_{n,p}_def
+// (integrations.F90:1346-1462) through real_axis_int (:1093-1176), whose
+// hole integrand is compute_Gp = A - Gn with A = i (Gr - Gr^H) (:2405-2420);
+// q(k) = Re rho(k, k).  rho is kept for negf_get_dm.
+void compute_density_efa(Session &s, int particle, double *q, int ndofs) {
+  if (particle != 1 && particle != -1)
+    throw std::runtime_error("libNEGF error. In compute_density_efa, unknown particle");
+  if (s.ncont < 1) throw std::runtime_error("density needs at least one contact");
+  hn::DevStruct *st = &device_struct(s);
+  hn::Engine &eng = engine(s);
+  const int ncont = s.ncont, N = s.H[s.cur].n;
+  if (ndofs > N) throw std::runtime_error("compute_density_efa: ndofs larger than H");
+  int refcont;
+  double muref;
+  set_ref_cont(s, refcont, muref);
+  muref = particle == 1 ? s.p.mu_n[0] : s.p.mu_p[0];
+  const int *np = particle == 1 ? s.p.np_n : s.p.np_p;
+  const bool contour = np[0] + np[1] + s.p.n_poles > 0;
+  if (!contour) refcont = ncont + 1;
+  const bool energy = s.p.dore[0] == 'E';
+  hn::SolveOpts o = dm_opts(s);
+  const std::vector<long long> lay = hn::Engine::density_layout(*st);
+  const long long tot = lay.back();
+  auto run = [&](int mode, const std::vector<cplx> &z, const std::vector<cplx> &w,
+                 const std::vector<double> &frm, bool real_device = false) {
+    return place_blocks(s, *st, density_run(s, *st, eng, o, tot, refcont, mode, z, w, frm, real_device),
+                        lay, N, s.outer == 2);
+  };
+  SparseAcc rho(N);
+  s.item_status.clear();
+  s.en.clear();
+  s.en_im.clear();
+  bool any = false;
+  if (contour) {
+    // kbT of the reference contact (the contact of max mu when there is none)
+    int kc = refcont - 1;
+    if (refcont > ncont) {
+      kc = 0;
+      for (int c = 1; c < ncont; ++c) if (s.p.mu[c] > s.p.mu[kc]) kc = c;
+    }
+    std::vector<cplx> z, w;
+    const double Eb = particle == 1 ? s.p.ec - s.p.deltaec : s.p.ev + s.p.deltaev;
+    efa_contour_grid(particle, Eb, muref, s.p.kbt_dm[kc], s.p.n_kt, np[0], np[1], s.p.n_poles, s.p.g_spin,
+                     z, w);
+    if (energy)
+      for (size_t i = 0; i < z.size(); ++i) w[i] *= z[i];
+    for (const cplx &v : z) {
+      s.en.push_back(v.real());
+      s.en_im.push_back(v.imag());
+    }
+    if (!z.empty()) {
+      add_spectral(rho, run(0, z, w, {}), N);
+      any = true;
+    }
+  }
+  if (s.p.np_real > 0) {
+    double kmax = s.p.kbt_dm[0], mun = s.p.mu_n[0], mup = s.p.mu_p[0];
+    for (int c = 1; c < ncont; ++c) {
+      kmax = std::max(kmax, s.p.kbt_dm[c]);
+      mun = std::max(mun, s.p.mu_n[c]);
+      mup = std::min(mup, s.p.mu_p[c]);
+    }
+    const double Omega = s.p.n_kt * kmax;
+    std::vector<double> x, wt;
+    if (particle == 1) trapez(s.p.ec - s.p.deltaec, mun + Omega, x, wt, s.p.np_real);
+    else trapez(mup - Omega, s.p.ev + s.p.deltaev, x, wt, s.p.np_real);
+    const int n = s.p.np_real;
+    std::vector<cplx> z(n), w(n);
+    std::vector<double> frm((size_t)n * (ncont + 1), 0.0);
+    s.en.clear();
+    s.en_im.clear();
+    for (int i = 0; i < n; ++i) {
+      z[i] = cplx(x[i], s.p.delta);
+      w[i] = s.p.g_spin * wt[i] / (2.0 * kPi);
+      if (energy) w[i] *= x[i];
+      for (int c = 0; c < ncont; ++c) frm[(size_t)i * (ncont + 1) + c] = fermi(x[i], s.p.mu[c], s.p.kbt_dm[c]);
+      s.en.push_back(x[i]);
+      s.en_im.push_back(s.p.delta);
+    }
+    o.delta = s.p.delta;
+    if (particle == 1) {
+      add_plain(rho, run(1, z, w, frm), N, 1.0);
+    } else {  // Gp = A - Gn, A from the G^r that comes with G^n (device at real E)
+      add_spectral(rho, run(0, z, w, {}, true), N);
+      add_plain(rho, run(1, z, w, frm), N, -1.0);
+    }
+    any = true;
+  }
+  store_rho(s, rho, N);
+  for (int k = 0; k < ndofs; ++k) {
+    double v = 0.0;
+    if (any)
+      for (int i = s.rho_ptr[k]; i < s.rho_ptr[k + 1]; ++i)
+        if (s.rho_col[i] == k) v += s.rho_val[i].x;
+    q[k] = v;
   }
 }
 
@@ -1432,6 +1605,15 @@ void negf_write_tunneling_and_dos(int *handler) {
       }
       std::fclose(f);
     }
+  });
+}
+
+// negf_density_efa (libnegf_api.f90:724-749): all arguments by reference
+void negf_density_efa(int *handler, int *ndofs, double *density, int *particle, bool *contacts) {
+  api(handler, [&](Session &s) {
+    if (*contacts)
+      throw std::runtime_error("negf_density_efa: the bulk contact density (contacts = true) is not provided");
+    compute_density_efa(s, *particle, density, *ndofs);
   });
 }
 

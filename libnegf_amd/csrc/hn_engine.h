@@ -144,6 +144,8 @@ struct DensityOpts {
   int ref = 0;                  // mode 1: 1-based reference contact (ncont + 1: none)
   int outer = 2;                // 0: device blocks, 1: + device/contact, 2: + contact/device
   int *status = nullptr;        // optional device [nitems]: per-item status words
+  int real_device = 0;          // mode 0: the device at real E, only the contacts at E + i Ei
+                                // (the G^r returned beside G^n, iterative.F90:263-276)
 };
 
 struct SCBAOpts {

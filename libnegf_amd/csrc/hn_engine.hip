@@ -1528,7 +1528,7 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
   } else {
     o.dEi = nullptr;
   }
-  const double *Ei_dev = gn ? nullptr : dEi;
+  const double *Ei_dev = (gn || d.real_device) ? nullptr : dEi;
   arena_.reset();
   double2 *Sig = arena_.take<double2>(SB * ncs), *GS = arena_.take<double2>(SB * ncs);
   int *ncyc_c = arena_.take<int>((size_t)B * ncs), *status_c = arena_.take<int>((size_t)B * ncs);

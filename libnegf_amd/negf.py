@@ -247,6 +247,15 @@ class NEGF:
         self.set_params()
         self._call("negf_solve_density_dft")
 
+    def density_efa(self, ndofs: int, particle: int, contacts: bool = False):
+        """compute_density_efa: electron (particle 1) or hole (-1) density per
+        orbital, Re diag(rho) of the first ndofs orbitals."""
+        self.set_params()
+        q = np.zeros(ndofs)
+        self._call("negf_density_efa", C.byref(C.c_int(ndofs)), hip._ptr(q), C.byref(C.c_int(particle)),
+                   C.byref(C.c_bool(contacts)))
+        return q
+
     def set_outer(self, outer: int):
         """0: device blocks only, 1: + device/contact, 2: + contact/device (default)."""
         self._call("negf_set_outer", C.byref(C.c_int(outer)))

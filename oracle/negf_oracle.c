@@ -1116,6 +1116,59 @@ int or_real_axis_grid(double mumin, double mumax, double kbt_max, int n_kt, int 
   return np;
 }
 
+/* contour_int_n_def (integrations.F90:281-426) for particle 1 and
+ * contour_int_p_def (:447-592) for particle -1, written segment by segment
+ * as there: Eb is Ec - DeltaEc (electrons) or Ev + DeltaEv (holes). */
+int or_efa_contour_grid(int particle, double Eb, double muref, double kbT, int n_kt, int np1, int np2,
+                        int npoles, double g_spin, zc *z, zc *w) {
+  double Omega = n_kt * kbT, Lambda;
+  if (npoles == 0) Lambda = 0.5 * kbT * M_PI;
+  else Lambda = 2.0 * npoles * kbT * M_PI;
+  if (particle == 1) {
+    if (Eb < muref + 0.001 && Eb > muref - 0.001) Eb = muref - kbT;
+    if (Eb > muref) npoles = 0;
+  } else {
+    if (Eb < muref + 1e-3 && Eb > muref - 1e-3) Eb = muref + kbT;
+    if (Eb < muref) npoles = 0;
+  }
+  int nmax = np1 > np2 ? np1 : np2, k = 0;
+  double *x = (double *)calloc(nmax + 1, sizeof(double)), *wt = (double *)calloc(nmax + 1, sizeof(double));
+  zc z1 = Eb, z2 = Eb + I * Lambda, zd = z2 - z1;
+  or_gauleg(0.0, 1.0, x, wt, np1);
+  for (int i = 0; i < np1; ++i, ++k) {
+    zc Ec = z1 + x[i] * zd;
+    zc ff = particle == 1 ? or_fermi_c(Ec, muref, kbT) : or_fermi_c(-Ec, -muref, kbT);
+    z[k] = Ec;
+    w[k] = (particle == 1 ? 1.0 : -1.0) * g_spin * zd * ff * wt[i] / (2.0 * M_PI);
+  }
+  z1 = z2;
+  z2 = (particle == 1 ? muref + Omega : muref - Omega) + I * Lambda;
+  zd = z2 - z1;
+  or_gauleg(0.0, 1.0, x, wt, np2);
+  for (int i = 0; i < np2; ++i, ++k) {
+    zc Ec = z1 + x[i] * zd;
+    zc ff = particle == 1 ? or_fermi_c(Ec, muref, kbT) : or_fermi_c(-Ec, -muref, kbT);
+    z[k] = Ec;
+    w[k] = (particle == 1 ? 1.0 : -1.0) * g_spin * zd * ff * wt[i] / (2.0 * M_PI);
+  }
+  for (int i = 1; i <= npoles; ++i, ++k) {
+    z[k] = muref + I * kbT * M_PI * (2.0 * i - 1.0);
+    w[k] = (particle == 1 ? -1.0 : 1.0) * I * kbT * g_spin;
+  }
+  free(x);
+  free(wt);
+  return k;
+}
+
+/* trapez (integrations.F90:1529-1547) */
+void or_trapez(double x1, double x2, double *x, double *w, int n) {
+  double d = (x2 - x1) / (1.0 * (n - 1));
+  for (int i = 0; i < n; ++i) w[i] = d;
+  w[0] = d * 0.5;
+  w[n - 1] = d * 0.5;
+  for (int i = 1; i <= n; ++i) x[i - 1] = (x1 * (n - i) + x2 * (i - 1)) / (n - 1);
+}
+
 /* Tlc = z SMC - HMC (nd x ng), Tcl = (conj(z) SMC - HMC)^dag (ng x nd), ng the
  * order of the contact's surface function (surface + one PL):
  * compute_contacts_dns, contselfenergy.F90:470-475 */
@@ -1136,10 +1189,25 @@ static void t_blocks(const or_struct *s, int c, zc z, zc *Tlc, zc *Tcl) {
  * (iterative.F90:1463-1573): Glc[k] = -Gr(cb,cb) Tlc GS (nd x nc),
  * Gcl[k] = -GS Tcl Gr(cb,cb) (nc x nd), unmasked.  Blocks are caller
  * allocated (Gd[nbl], Gu/Gl[nbl-1]; Glc/Gcl may be NULL). */
+static int gr_point_at(const or_struct *s, zc z, zc zdev, zc *const *Gdo, zc *const *Guo,
+                       zc *const *Glo, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc);
+
 int or_gr_point(const or_struct *s, double zr, double zi, zc *const *Gdo, zc *const *Guo,
                 zc *const *Glo, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc) {
-  int nbl = s->nbl, ncont = s->ncont;
   zc z = zr + I * zi;
+  return gr_point_at(s, z, z, Gdo, Guo, Glo, Glc, Gcl, ncyc, sgfacc);
+}
+
+/* G^r as calculate_Gn_neq_components returns it beside G^n
+ * (iterative.F90:263-276): contacts at E + i delta, the device at real E */
+int or_gr_point_real(const or_struct *s, double E, double delta, zc *const *Gdo, zc *const *Guo,
+                     zc *const *Glo, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc) {
+  return gr_point_at(s, E + I * delta, E, Gdo, Guo, Glo, Glc, Gcl, ncyc, sgfacc);
+}
+
+static int gr_point_at(const or_struct *s, zc z, zc zdev, zc *const *Gdo, zc *const *Guo,
+                       zc *const *Glo, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc) {
+  int nbl = s->nbl, ncont = s->ncont;
   zc **Sig = (zc **)calloc(ncont ? ncont : 1, sizeof(zc *)), **GS = (zc **)calloc(ncont ? ncont : 1, sizeof(zc *));
   for (int c = 0; c < ncont; ++c) {
     int nd = s->bsz[s->cblk[c] - 1];
@@ -1148,7 +1216,7 @@ int or_gr_point(const or_struct *s, double zr, double zi, zc *const *Gdo, zc *co
     int n = contact_sigma_of(s, c, z, Sig[c], GS[c], 0, sgfacc);
     if (ncyc) ncyc[c] = n;
   }
-  esh_t e = build_esh(s, z, Sig);
+  esh_t e = build_esh(s, zdev, Sig);
   zc **g = (zc **)calloc(nbl, sizeof(zc *)), **Gd = (zc **)calloc(nbl, sizeof(zc *));
   zc **Gu = (zc **)calloc(nbl, sizeof(zc *)), **Gl = (zc **)calloc(nbl, sizeof(zc *));
   gsmr_blocks(&e, 2, g);

@@ -142,12 +142,21 @@ int or_contour_grid(double Elow, double muref, double kbT, int n_kt, int np1, in
 /* real_axis_int_def without inelastic terms (integrations.F90:989-1084) */
 int or_real_axis_grid(double mumin, double mumax, double kbt_max, int n_kt, int np, double delta,
                       double g_spin, zc *z, zc *w);
+/* contour_int_n_def / contour_int_p_def (integrations.F90:281-426, 447-592)
+ * for particle 1 / -1; returns the number of points */
+int or_efa_contour_grid(int particle, double Eb, double muref, double kbT, int n_kt, int np1, int np2,
+                        int npoles, double g_spin, zc *z, zc *w);
+void or_trapez(double x1, double x2, double *x, double *w, int n); /* integrations.F90:1529-1547 */
 /* G^r blocks at complex z (contacts at z): Gd[nbl], Gu[nbl-1] = Gr(i,i+1),
  * Gl[nbl-1] = Gr(i+1,i), outer Glc[k] = -Gr(cb,cb) Tlc GS (nd x nc) and
  * Gcl[k] = -GS Tcl Gr(cb,cb) (nc x nd) (calculate_Gr + calculate_Gr_outer,
  * iterative.F90:90-148, 1463-1573); caller-allocated, Gu/Gl/Glc/Gcl may be NULL. */
 int or_gr_point(const or_struct *s, double zr, double zi, zc *const *Gd, zc *const *Gu,
                 zc *const *Gl, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc);
+/* the same with the contacts at E + i delta and the device at real E: the
+ * G^r of calculate_Gn_neq_components (iterative.F90:263-276) */
+int or_gr_point_real(const or_struct *s, double E, double delta, zc *const *Gdo, zc *const *Guo,
+                     zc *const *Glo, zc *const *Glc, zc *const *Gcl, int *ncyc, double sgfacc);
 /* G^n blocks on the real axis (calculate_Gn_neq_components + calculate_Gn_outer,
  * iterative.F90:159-279, 1576-1665): contacts at E + i delta, ESH at E;
  * frm[ncont+1], ref 1-based (ncont+1: none); Gnlc masked by Tlc != 0,

@@ -91,11 +91,17 @@ def lib():
         L.or_contour_grid.restype = C.c_int
         L.or_contour_grid.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int, C.c_int,
                                       C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+        L.or_efa_contour_grid.restype = C.c_int
+        L.or_efa_contour_grid.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
+                                          C.c_int, C.c_int, C.c_double, C.c_void_p, C.c_void_p]
+        L.or_trapez.argtypes = [C.c_double, C.c_double, C.c_void_p, C.c_void_p, C.c_int]
         L.or_real_axis_grid.restype = C.c_int
         L.or_real_axis_grid.argtypes = [C.c_double, C.c_double, C.c_double, C.c_int, C.c_int,
                                         C.c_double, C.c_double, C.c_void_p, C.c_void_p]
         L.or_gr_point.restype = C.c_int
         L.or_gr_point.argtypes = [C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 6 + [C.c_double]
+        L.or_gr_point_real.restype = C.c_int
+        L.or_gr_point_real.argtypes = [C.c_void_p, C.c_double, C.c_double] + [C.c_void_p] * 6 + [C.c_double]
         L.or_gn_point.restype = C.c_int
         L.or_gn_point.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int] + \
             [C.c_void_p] * 6 + [C.c_double, C.c_double]
@@ -370,6 +376,68 @@ def real_axis_grid(mumin, mumax, kbt_max, n_kt, npts, delta, g_spin):
     return z[:npts], w[:npts]
 
 
+def efa_contour_grid(particle, Eb, muref, kbT, n_kt, np1, np2, npoles, g_spin):
+    """contour_int_n_def (particle 1) / contour_int_p_def (-1): (z, w)."""
+    n = np1 + np2 + npoles
+    z = np.zeros(max(n, 1), dtype=np.complex128)
+    w = np.zeros(max(n, 1), dtype=np.complex128)
+    k = lib().or_efa_contour_grid(particle, Eb, muref, kbT, n_kt, np1, np2, npoles, g_spin,
+                                  z.ctypes.data, w.ctypes.data)
+    return z[:k], w[:k]
+
+
+def trapez(x1, x2, n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().or_trapez(x1, x2, x.ctypes.data, w.ctypes.data, n)
+    return x, w
+
+
+def density_efa(s, N, cstart, P, particle, p, outer=2):
+    """compute_density_efa (libnegf.F90:1690-1791, no bulk contact density)
+    from the parameters p (a dict of the lnparams fields used): the contour of
+    contour_int_{n,p}_def, then the real-axis trapezoid of real_axis_int_{n,p}_def
+    (integrations.F90:1346-1462) whose hole integrand is A - Gn
+    (compute_Gp, :2405-2420) with A = i (Gr - Gr^H) of the G^r that comes
+    with G^n (device at real E, contacts at E + i delta).  Returns (rho, pattern, q = Re diag rho)."""
+    nc = len(p["mu"])
+    mu = list(p["mu"])
+    imin, imax = int(np.argmin(mu)), int(np.argmax(mu))
+    ref = {0: imin + 1, 1: imax + 1}.get(p["min_or_max"], nc + 1)   # set_ref_cont
+    muref = p["mu_n"][0] if particle == 1 else p["mu_p"][0]
+    npts = p["np_n"] if particle == 1 else p["np_p"]
+    contour = npts[0] + npts[1] + p["n_poles"] > 0
+    if not contour:
+        ref = nc + 1
+    dore = p.get("dore", "D")
+    rho = np.zeros((N, N), dtype=np.complex128)
+    pat = np.eye(N, dtype=bool)
+    if contour:
+        kc = ref - 1 if ref <= nc else imax
+        Eb = p["ec"] - p["deltaec"] if particle == 1 else p["ev"] + p["deltaev"]
+        z, w = efa_contour_grid(particle, Eb, muref, p["kbt_dm"][kc], p["n_kt"], npts[0], npts[1],
+                                p["n_poles"], p["g_spin"])
+        if len(z):
+            r, pt = density_matrix(s, N, cstart, P, contour=(z, w), outer=outer, dore=dore)
+            rho, pat = rho + r, pat | pt
+    if p["np_real"] > 0:
+        Om = p["n_kt"] * max(p["kbt_dm"])
+        if particle == 1:
+            x, wt = trapez(p["ec"] - p["deltaec"], max(p["mu_n"]) + Om, p["np_real"])
+        else:
+            x, wt = trapez(min(p["mu_p"]) - Om, p["ev"] + p["deltaev"], p["np_real"])
+        zr = x + 1j * p["delta"]
+        wr = p["g_spin"] * wt / (2 * np.pi) * (x if dore == "E" else 1.0)
+        frm_of = lambda E: [fermi(E, mu[c], p["kbt_dm"][c]) for c in range(nc)] + [0.0]
+        g, pt = density_matrix(s, N, cstart, P, real=(zr, wr), frm_of=frm_of, ref=ref, outer=outer,
+                               delta=p["delta"])
+        if particle == 1:
+            rho, pat = rho + g, pat | pt
+        else:
+            a, pa = density_matrix(s, N, cstart, P, contour=(zr, wr + 0j), outer=outer, real_device=True)
+            rho, pat = rho + a - g, pat | pa | pt
+    return rho, pat, np.real(np.diag(rho)).copy()
+
+
 def _blocks(shapes):
     arrs = [np.zeros(sh, dtype=np.complex128, order="F") for sh in shapes]
     ptr = (C.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])
@@ -385,8 +453,9 @@ def _point_shapes(s):
             [(nc[k], nd[k]) for k in range(len(nc))])
 
 
-def gr_point(s, z: complex):
-    """G^r blocks at complex z with the outer blocks (unmasked)."""
+def gr_point(s, z: complex, real_device: bool = False):
+    """G^r blocks at complex z with the outer blocks (unmasked); with
+    real_device the device is at Re z and only the contacts at z."""
     os_ = OracleStruct(s)
     shd, shu, shl, shlc, shcl = _point_shapes(s)
     Gd, pd, kd = _blocks(shd)
@@ -395,7 +464,8 @@ def gr_point(s, z: complex):
     Glc, plc, klc = _blocks(shlc)
     Gcl, pcl, kcl = _blocks(shcl)
     ncyc = np.zeros(max(s.ncont, 1), dtype=np.int32)
-    lib().or_gr_point(os_.ptr, z.real, z.imag, pd, pu, pl, plc, pcl, ncyc.ctypes.data, SGFACC)
+    fn = lib().or_gr_point_real if real_device else lib().or_gr_point
+    fn(os_.ptr, z.real, z.imag, pd, pu, pl, plc, pcl, ncyc.ctypes.data, SGFACC)
     return dict(Gd=Gd, Gu=Gu, Gl=Gl, Glc=Glc, Gcl=Gcl, ncyc=ncyc[:s.ncont])
 
 
@@ -463,7 +533,7 @@ def _add_point(s, N, cstart, P, T, pat, blk, w, lower, masked_outer, tl_pattern=
 
 
 def density_matrix(s, N, cstart, P, contour=None, real=None, frm_of=None, ref=None, outer=2,
-                   dore="D", delta=1e-4, eps=0.0):
+                   dore="D", delta=1e-4, eps=0.0, real_device=False):
     """Dense N x N density matrix and its pattern as compute_density_dft builds
     it (contour_int, integrations.F90:844-897; real_axis_int, :1093-1176):
     T = sum_i w_i Gr(z_i) on the pattern, rho = i (T - T^H), then
@@ -477,7 +547,7 @@ def density_matrix(s, N, cstart, P, contour=None, real=None, frm_of=None, ref=No
         T = np.zeros((N, N), dtype=np.complex128)
         pt = np.eye(N, dtype=bool)
         for z, w in zip(*contour):
-            blk = gr_point(s, complex(z))
+            blk = gr_point(s, complex(z), real_device)
             ww = w * z if dore == "E" else w
             _add_point(s, N, cstart, P, T, pt, blk, ww, outer == 2, False)
         rho = 1j * (T - T.conj().T)
