@@ -9,8 +9,8 @@
  * unchanged for the Landauer / Meir-Wingreen / elastic-dephasing path.
  * Also provided: the local DOS alongside T and the density matrix (contour
  * and real-axis integration of the same block solver).  Entry points outside
- * that path (the quasi-equilibrium density, k-point meshes, inelastic
- * models, the UPT reader) are not provided.
+ * that path (k-point meshes, inelastic models, the UPT reader) are not
+ * provided.
  *
  * Errors: the reference stops the process (error stop); here an entry point
  * that fails records a message (negf_get_last_error) and leaves its outputs
@@ -174,6 +174,14 @@ void negf_set_outer(int *handler, int *outer);
  * readable with negf_get_dm.  contacts = true (the bulk contact density) is
  * not provided and reports an error. */
 void negf_density_efa(int *handler, int *ndofs, double *density, int *particle, bool *contacts);
+/* Quasi-equilibrium electron (particle 1) or hole (-1) density per orbital
+ * (negf_density_quasi_equilibrium, libnegf_api.f90:753-775; quasiEq_int_n/p,
+ * integrations.F90:3002-3258) for band-edge profiles Ec/Ev and quasi-Fermi
+ * levels mu_n/mu_p of ndofs orbitals: a contour over every orbital's window
+ * (Np(2) horizontal and Np(1) vertical points) and the poles of each
+ * orbital's own level; density[k] = Re i (t_k - conj t_k). */
+void negf_density_quasi_equilibrium(int *handler, int *ndofs, double *density, int *particle, double *Ec,
+                                    double *Ev, double *mu_n, double *mu_p);
 
 /* Local DOS alongside the transmission (negf_init_ldos / negf_set_ldos_intervals /
  * negf_set_ldos_indexes / negf_associate_ldos, libnegf_api.f90:941-1021):

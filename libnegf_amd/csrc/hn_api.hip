@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -1606,6 +1607,91 @@ void negf_write_tunneling_and_dos(int *handler) {
       std::fclose(f);
     }
   });
+}
+
+// compute_density_quasiEq (libnegf.F90:1801-1830) = quasiEq_int_n / _p
+// (integrations.F90:3002-3258): for orbital k, q_k = Re i (t_k - conj t_k)
+// with t_k = sum_i ww_i(k) G^r_kk(z_i) over a horizontal segment at height
+// Lambda spanning every orbital's band window (Np(2) points), a vertical
+// segment at the window's low (electrons) or high (holes) end (Np(1) points)
+// and the Fermi poles of the orbital's own quasi-Fermi level.  The weights
+// depend on the orbital only through that level, so orbitals sharing a level
+// share one batched pass over the points; rho is not kept.
+void compute_density_quasi(Session &s, int particle, double *q, int ndofs, const double *Ec,
+                           const double *Ev, const double *mun, const double *mup) {
+  if (particle != 1 && particle != -1)
+    throw std::runtime_error("libNEGF error. In compute_density_quasiEq, unknown particle");
+  if (s.ncont < 1) throw std::runtime_error("density needs at least one contact");
+  hn::DevStruct *st = &device_struct(s);
+  hn::Engine &eng = engine(s);
+  const int ncont = s.ncont, N = s.H[s.cur].n;
+  if (ndofs < 1 || ndofs > N) throw std::runtime_error("compute_density_quasiEq: ndofs outside 1..size of H");
+  const cplx I(0.0, 1.0);
+  double kbT = s.p.kbt_dm[0];
+  for (int c = 1; c < ncont; ++c) kbT = std::max(kbT, s.p.kbt_dm[c]);
+  const double Lambda = s.p.n_poles == 0 ? 0.5 * kbT * kPi : 2.0 * s.p.n_poles * kbT * kPi;
+  const double Omega = s.p.n_kt * kbT;
+  const double *lev = particle == 1 ? mun : mup;
+  double lo = 1e300, hi = -1e300;  // min(minE), max(maxE)
+  for (int k = 0; k < ndofs; ++k) {
+    lo = std::min(lo, particle == 1 ? Ec[k] - s.p.deltaec : Ev[k] - Omega);
+    hi = std::max(hi, particle == 1 ? Ec[k] + Omega : Ev[k] + s.p.deltaev);
+  }
+  // the two segments: (start, end, points) in the order of the reference
+  const int *np = particle == 1 ? s.p.np_n : s.p.np_p;
+  struct Seg { cplx a, b; int n; };
+  std::vector<Seg> sg;
+  if (particle == 1) {
+    sg.push_back({lo + I * Lambda, hi + I * Lambda, np[1]});
+    sg.push_back({cplx(lo, 0.0), lo + I * Lambda, np[0]});
+  } else {
+    sg.push_back({hi + I * Lambda, lo + I * Lambda, np[1]});
+    sg.push_back({cplx(hi, 0.0), hi + I * Lambda, np[0]});
+  }
+  std::vector<cplx> zs, base;  // segment points and their level-free weight factor
+  for (const Seg &g : sg) {
+    std::vector<double> x, wt;
+    gauleg(0.0, 1.0, x, wt, g.n);
+    const cplx zd = g.b - g.a;
+    for (size_t i = 0; i < x.size(); ++i) {
+      zs.push_back(g.a + x[i] * zd);
+      base.push_back((double)particle * s.p.g_spin * s.p.kwght * wt[i] * zd / (2.0 * kPi));
+    }
+  }
+  std::map<double, std::vector<int>> groups;
+  for (int k = 0; k < ndofs; ++k) groups[lev[k]].push_back(k);
+  hn::SolveOpts o = dm_opts(s);
+  const std::vector<long long> lay = hn::Engine::density_layout(*st);
+  const long long tot = lay.back();
+  s.item_status.clear();
+  for (const auto &gr : groups) {
+    const double mu = gr.first;
+    std::vector<cplx> z = zs, w(zs.size());
+    for (size_t i = 0; i < zs.size(); ++i)
+      w[i] = base[i] * (particle == 1 ? fermi_c(zs[i], mu, kbT) : fermi_c(-zs[i], -mu, kbT));
+    for (int i = 1; i <= s.p.n_poles; ++i) {
+      z.push_back(mu + I * kbT * kPi * (2.0 * i - 1.0));
+      w.push_back(-(double)particle * I * kbT * s.p.g_spin);
+    }
+    if (z.empty()) {
+      for (int k : gr.second) q[k] = 0.0;
+      continue;
+    }
+    SparseAcc T = place_blocks(s, *st, density_run(s, *st, eng, o, tot, s.ncont + 1, 0, z, w, {}), lay, N,
+                               s.outer == 2);
+    for (int k : gr.second) {
+      cplx t(0.0, 0.0);
+      for (const auto &e : T.rows[k])
+        if (e.first == k) t += cplx(e.second.x, e.second.y);
+      q[k] = -2.0 * t.imag();  // Re i (t - conj t)
+    }
+  }
+}
+
+// negf_density_quasi_equilibrium (libnegf_api.f90:753-775): all by reference
+void negf_density_quasi_equilibrium(int *handler, int *ndofs, double *density, int *particle, double *Ec,
+                                    double *Ev, double *mu_n, double *mu_p) {
+  api(handler, [&](Session &s) { compute_density_quasi(s, *particle, density, *ndofs, Ec, Ev, mu_n, mu_p); });
 }
 
 // negf_density_efa (libnegf_api.f90:724-749): all arguments by reference

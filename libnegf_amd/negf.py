@@ -256,6 +256,16 @@ class NEGF:
                    C.byref(C.c_bool(contacts)))
         return q
 
+    def density_quasi_eq(self, particle: int, Ec, Ev, mu_n, mu_p):
+        """compute_density_quasiEq: electron (1) or hole (-1) density of each
+        orbital for band-edge profiles Ec/Ev and quasi-Fermi levels mu_n/mu_p."""
+        self.set_params()
+        Ec, Ev, mu_n, mu_p = (np.ascontiguousarray(a, dtype=np.float64) for a in (Ec, Ev, mu_n, mu_p))
+        q = np.zeros(Ec.size)
+        self._call("negf_density_quasi_equilibrium", C.byref(C.c_int(Ec.size)), hip._ptr(q),
+                   C.byref(C.c_int(particle)), hip._ptr(Ec), hip._ptr(Ev), hip._ptr(mu_n), hip._ptr(mu_p))
+        return q
+
     def set_outer(self, outer: int):
         """0: device blocks only, 1: + device/contact, 2: + contact/device (default)."""
         self._call("negf_set_outer", C.byref(C.c_int(outer)))

@@ -438,6 +438,59 @@ def density_efa(s, N, cstart, P, particle, p, outer=2):
     return rho, pat, np.real(np.diag(rho)).copy()
 
 
+def density_quasi(s, particle, p, Ec, Ev, mun, mup):
+    """quasiEq_int_n / quasiEq_int_p (integrations.F90:3002-3258), written
+    loop for loop: per contour point the diagonal of G^r (device blocks; the
+    contact rows carry no diagonal entry), weighted per orbital with its own
+    quasi-Fermi level, then the poles of every orbital's level."""
+    nz = len(Ec)
+    kbT = max(p["kbt_dm"])
+    Lam = 0.5 * kbT * np.pi if p["n_poles"] == 0 else 2.0 * p["n_poles"] * kbT * np.pi
+    Om = p["n_kt"] * kbT
+    kw = p.get("kwght", 1.0)
+    g = p["g_spin"]
+    if particle == 1:
+        minE = np.asarray(Ec) - p["deltaec"]
+        maxE = np.asarray(Ec) + Om
+        segs = [(minE.min() + 1j * Lam, maxE.max() + 1j * Lam, p["np_n"][1]),
+                (minE.min() + 0j, minE.min() + 1j * Lam, p["np_n"][0])]
+        lev, sgn = np.asarray(mun), 1.0
+    else:
+        minE = np.asarray(Ev) - Om
+        maxE = np.asarray(Ev) + p["deltaev"]
+        segs = [(maxE.max() + 1j * Lam, minE.min() + 1j * Lam, p["np_p"][1]),
+                (maxE.max() + 0j, maxE.max() + 1j * Lam, p["np_p"][0])]
+        lev, sgn = np.asarray(mup), -1.0
+    bs = s.bsz
+    off = np.concatenate([[0], np.cumsum(bs)])
+
+    def diag(z):
+        G = gr_point(s, complex(z))["Gd"]
+        d = np.zeros(nz, dtype=np.complex128)
+        for k in range(min(nz, off[-1])):
+            b = int(np.searchsorted(off, k, side="right")) - 1
+            d[k] = G[b][k - off[b], k - off[b]]
+        return d
+
+    def ferm(z, mu):
+        return 1.0 / (np.exp((z - mu) / kbT) + 1.0)
+
+    temp = np.zeros(nz, dtype=np.complex128)
+    for a, b, n in segs:
+        x, w = gauleg(0.0, 1.0, n)
+        zd = b - a
+        for i in range(n):
+            Ez = a + x[i] * zd
+            d = diag(Ez)
+            ff = ferm(Ez, lev) if particle == 1 else ferm(-Ez, -lev)
+            temp += d * (sgn * g * kw * w[i] * zd * ff / (2 * np.pi))
+    for k in range(nz):
+        for i in range(1, p["n_poles"] + 1):
+            Ez = lev[k] + 1j * kbT * np.pi * (2 * i - 1)
+            temp[k] += diag(Ez)[k] * (-sgn * 1j * kbT * g)
+    return np.real(1j * (temp - np.conj(temp)))
+
+
 def _blocks(shapes):
     arrs = [np.zeros(sh, dtype=np.complex128, order="F") for sh in shapes]
     ptr = (C.c_void_p * max(len(arrs), 1))(*[a.ctypes.data for a in arrs])

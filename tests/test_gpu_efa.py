@@ -80,3 +80,34 @@ def test_efa_errors(gpu):
     with pytest.raises(NegfError, match="bulk contact density"):
         n.density_efa(60, 1, contacts=True)
     n.close()
+
+
+@pytest.mark.parametrize("particle", [1, -1])
+def test_quasi_equilibrium_profiles(gpu, particle):
+    """negf_density_quasi_equilibrium against the oracle (quasiEq_int_n/p):
+    overlap chain, band-edge profiles, three quasi-Fermi levels, kwght 0.5."""
+    hip, eng, orc = gpu
+    rng = np.random.default_rng(11)
+    H = chain_dense(onsite=0.3 * rng.standard_normal(60))
+    S = np.eye(100, dtype=np.complex128)
+    for i in range(1, 100):
+        if H[i, i - 1] != 0:
+            S[i, i - 1] = S[i - 1, i] = 0.07
+    plend = [12, 25, 40, 60]
+    n = _session(H, S, plend, [4, 1], [61, 81], [80, 100])
+    p = n.params
+    p.deltaec, p.deltaev = 1.5, 1.2
+    p.kbt_dm[0], p.kbt_dm[1] = 0.02, 0.03
+    p.np_n[0], p.np_n[1] = 10, 24
+    p.np_p[0], p.np_p[1] = 9, 22
+    p.n_poles, p.n_kt, p.kwght = 3, 10, 0.5
+    nz = 70  # past the device: contact rows have no diagonal entry
+    Ec = 1.0 + 0.2 * rng.standard_normal(nz)
+    Ev = -1.0 + 0.2 * rng.standard_normal(nz)
+    lev = np.choose(np.arange(nz) % 3, [-0.3, 0.1, 0.4]).astype(float)
+    q = n.density_quasi_eq(particle, Ec, Ev, lev, lev)
+    s = from_dense(H, S, plend, [61, 81], [80, 100], [4, 1])
+    qref = orc.density_quasi(s, particle, dict(_params(n.params, 2), kwght=0.5), Ec, Ev, lev, lev)
+    assert np.abs(q - qref).max() <= RTOL * np.abs(qref).max()
+    assert np.all(q[60:] == 0.0)
+    n.close()

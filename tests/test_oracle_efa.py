@@ -85,3 +85,41 @@ def test_chain_hole_real_axis_limits(oracle):
     _, _, p_high = _efa(oracle, -1, mu=[2.7, 2.7], **kw)    # contacts full: p = O(delta) weight
     assert np.abs(p_low[:60] - 2.0).max() < 2e-2
     assert p_high[:60].max() < 2e-2 and p_high[:60].min() > -1e-10
+
+
+# ---- quasi-equilibrium densities (quasiEq_int_n / _p) ----
+def _qp(**kw):
+    p = dict(P0, np_n=[60, 300], np_p=[60, 300], kwght=1.0)
+    p.update(kw)
+    return p
+
+
+def test_quasi_eq_chain_closed_form(oracle):
+    """Uniform levels on the ideal chain: the windows cover the band, so
+    n = 2 int rho f and p = 2 - n per site."""
+    s = chain_structure()
+    mu, kT = 0.3, 0.05
+    n_ref = 2 * quad(lambda t: _fermi(2 * np.cos(t), mu, kT) / np.pi, 0, np.pi, limit=500, epsabs=1e-13)[0]
+    nz = 60
+    one = np.ones(nz)
+    p = _qp(kbt_dm=[kT, kT], deltaec=5.2, deltaev=5.2)
+    n = oracle.density_quasi(s, 1, p, 2.2 * one, -2.2 * one, mu * one, mu * one)
+    h = oracle.density_quasi(s, -1, p, 2.2 * one, -2.2 * one, mu * one, mu * one)
+    assert np.abs(n - n_ref).max() < 1e-8
+    assert np.abs(h - (2.0 - n_ref)).max() < 1e-8
+
+
+def test_quasi_eq_per_orbital_levels(oracle):
+    """An orbital's density depends on its own level only: a two-level
+    profile equals, site by site, the uniform runs at each level."""
+    s = chain_structure()
+    nz = 60
+    one = np.ones(nz)
+    p = _qp(kbt_dm=[0.05, 0.05], deltaec=5.2, deltaev=5.2, np_n=[20, 60], np_p=[20, 60])
+    lev = np.where(np.arange(nz) < 25, -0.4, 0.6)
+    for part in (1, -1):
+        prof = oracle.density_quasi(s, part, p, 2.2 * one, -2.2 * one, lev, lev)
+        for m in (-0.4, 0.6):
+            uni = oracle.density_quasi(s, part, p, 2.2 * one, -2.2 * one, m * one, m * one)
+            sel = lev == m
+            assert np.abs(prof[sel] - uni[sel]).max() < 1e-13
