@@ -97,6 +97,15 @@ void negf_contact_blocks(int *handler, int ncont, int *surfstart, int *surfend, 
                          int npl, int *plend, int *cblk);
 void negf_get_pls(int *handler, int *npl, int *ncont, int *plend, int *cblk, int copy);
 void negf_get_version(int *handler);
+/* Diagnostics (libnegf_api.f90:545-567, 1188-1200, 1274-1286, 1462-1475):
+ * negf_print_mat writes H.dat / S.dat into the scratch path in printcsr's
+ * text format (2i8, 2f20.10 per stored entry); negf_write_partition,
+ * negf_print_tnegf and negf_mem_stats print the partition, the parameters
+ * and the memory in use to stdout. */
+void negf_print_mat(int *handler);
+void negf_write_partition(int *handler);
+void negf_print_tnegf(int *handler);
+void negf_mem_stats(int *handler);
 /* min_or_max of the density paths (libnegf_api.f90:1172-1185) */
 void negf_set_reference(int *handler, int *minmax);
 void negf_set_params(int *handler, struct lnparams *params);

@@ -1309,6 +1309,93 @@ void negf_set_reference(int *handler, int *minmax) {
   api(handler, [&](Session &s) { s.p.min_or_max = *minmax; });
 }
 
+// negf_print_mat (libnegf_api.f90:545-567): H.dat and S.dat in the scratch
+// path as printcsr writes them (zprint_csrcoo 'c', sparsekit_drv.F90:1127-1159):
+// one line per stored entry, row by row, format (2i8,2f20.10), 1-based
+void negf_print_mat(int *handler) {
+  api(handler, [&](Session &s) {
+    if (s.cur >= (int)s.H.size() || !s.H[s.cur].present()) throw std::runtime_error("negf_print_mat: H not set");
+    auto dump = [&](const std::string &fn, const Csr *A, int n) {
+      FILE *f = std::fopen(fn.c_str(), "w");
+      if (!f) throw std::runtime_error("cannot open " + fn);
+      for (int r = 0; r < n; ++r) {
+        if (A) {
+          for (int i = A->ptr[r]; i < A->ptr[r + 1]; ++i)
+            std::fprintf(f, "%8d%8d%20.10f%20.10f\n", r + 1, A->col[i] + 1, A->v[i].x, A->v[i].y);
+        } else {
+          std::fprintf(f, "%8d%8d%20.10f%20.10f\n", r + 1, r + 1, 1.0, 0.0);  // set_s_id's identity
+        }
+      }
+      std::fclose(f);
+    };
+    const Csr &H = s.H[s.cur];
+    dump(s.scratch + "H.dat", &H, H.n);
+    const bool sid = s.cur >= (int)s.S.size() || s.s_id[s.cur] || !s.S[s.cur].present();
+    dump(s.scratch + "S.dat", sid ? nullptr : &s.S[s.cur], H.n);
+  });
+}
+
+// negf_write_partition -> negf_partition_info (libnegf.F90:1306-1313)
+void negf_write_partition(int *handler) {
+  api(handler, [&](Session &s) {
+    std::printf(" (LibNEGF) Partitioning:\n Number of blocks: %12d\n Contact interactions:", s.npl);
+    for (int c : s.cblk) std::printf("%12d", c);
+    std::printf("\n");
+    std::fflush(stdout);
+  });
+}
+
+// negf_print_tnegf -> print_all_vars (lib_param.F90:674-740): the session's
+// parameters, one "name= value" line each
+void negf_print_tnegf(int *handler) {
+  api(handler, [&](Session &s) {
+    const lnparams &p = s.p;
+    auto arr = [&](const char *name, const double *v) {
+      std::printf(" %s=", name);
+      for (int c = 0; c < std::max(s.ncont, 1); ++c) std::printf(" %.15g", v[c]);
+      std::printf("\n");
+    };
+    std::printf(" verbose= %d\n scratch= \"%s\"\n output= \"%s\"\n", p.verbose, s.scratch.c_str(),
+                s.out_path.c_str());
+    arr("mu", p.mu);
+    arr("mu_n", p.mu_n);
+    arr("mu_p", p.mu_p);
+    for (int c = 0; c < s.ncont; ++c)
+      std::printf(" Contact Parameters for contact %d\n mu= %.15g\n WideBand= %s\n DOS= %.15g\n mu_n= %.15g\n"
+                  " mu_p= %.15g\n Density Matrix T= %.15g\n Transmission T= %.15g\n",
+                  c + 1, p.mu[c], p.fictcont[c] ? "T" : "F", p.contact_dos[c], p.mu_n[c], p.mu_p[c], p.kbt_dm[c],
+                  p.kbt_t[c]);
+    std::printf(" Contour Parameters:\n Ec= %.15g\n Ev= %.15g\n DEc= %.15g\n DEv= %.15g\n ReadoldDM_SGFs= %d\n"
+                " ReadoldT_SGF= %d\n Np_n= %d %d\n Np_p= %d %d\n nkT= %d\n nPoles= %d\n min_or_max= %d\n",
+                p.ec, p.ev, p.deltaec, p.deltaev, p.readold_dm_sgfs, p.readold_t_sgfs, p.np_n[0], p.np_n[1],
+                p.np_p[0], p.np_p[1], p.n_kt, p.n_poles, p.min_or_max);
+    std::printf(" Transmission Parameters:\n Emin= %.15g\n Emax= %.15g\n Estep= %.15g\n g_spin= %.15g\n"
+                " delta= %.15g\n dos_delta= %.15g\n DorE= %c\n",
+                p.emin, p.emax, p.estep, p.g_spin, p.delta, p.dos_delta, p.dore[0]);
+    std::printf(" Internal variables:\n kp= %d\n wght= %.15g\n outer_blocks= %d\n", p.kpoint, p.kwght, s.outer);
+    std::fflush(stdout);
+  });
+}
+
+// negf_mem_stats (libnegf_api.f90:1462-1475; writePeakInfo / writeMemInfo):
+// the device memory in use and free, and the host's peak resident size
+void negf_mem_stats(int *handler) {
+  api(handler, [&](Session &) {
+    size_t fr = 0, tot = 0;
+    HN_CHECK(hipMemGetInfo(&fr, &tot));
+    long peak_kb = 0;
+    if (FILE *f = std::fopen("/proc/self/status", "r")) {
+      char line[256];
+      while (std::fgets(line, sizeof(line), f))
+        if (std::strncmp(line, "VmHWM:", 6) == 0) peak_kb = std::atol(line + 6);
+      std::fclose(f);
+    }
+    std::printf(" Peak host memory: %.3f MB\n Device memory in use: %.3f MB of %.3f MB\n", peak_kb / 1024.0,
+                (tot - fr) / 1048576.0, tot / 1048576.0);
+    std::fflush(stdout);
+  });
+}
+
 // negf_get_version (negf_version, libnegf.F90:1295-1303)
 void negf_get_version(int *handler) {
   api(handler, [&](Session &) {

@@ -200,6 +200,20 @@ class NEGF:
     def get_version(self):
         self._call("negf_get_version")
 
+    def print_mat(self):
+        """H.dat and S.dat into the scratch path (printcsr format)."""
+        self._call("negf_print_mat")
+
+    def write_partition(self):
+        self._call("negf_write_partition")
+
+    def print_tnegf(self):
+        self.set_params()
+        self._call("negf_print_tnegf")
+
+    def mem_stats(self):
+        self._call("negf_mem_stats")
+
     def set_elph_dephasing(self, coupling, niter: int):
         cp = np.ascontiguousarray(coupling, dtype=np.float64)
         self._call("negf_set_elph_dephasing", hip._ptr(cp), C.c_int(cp.size), C.c_int(niter))

@@ -107,6 +107,42 @@ def test_contact_blocks_get_pls_and_reference():
     n.close()
 
 
+def test_print_mat_partition_and_params(tmp_path, capfd):
+    """negf_print_mat writes printcsr's lines (2i8, 2f20.10; sparsekit_drv.F90:
+    1127-1159) for every stored entry of H and S (identity from set_s_id);
+    write_partition / print_tnegf print the structure and parameters.  Host
+    only."""
+    import scipy.sparse as sps
+    from libnegf_amd.negf import NEGF
+    H = chain_dense()
+    H[3, 3] = -0.25 + 1e-12
+    n = NEGF()
+    n.set_h(sps.csr_matrix(H))
+    n.set_s_id(100)
+    n.init_structure(2, [80, 100], [60, 80], 4, [15, 30, 45, 60], [4, 1], surfstart=[61, 81])
+    n.set_scratch(str(tmp_path))
+    n.print_mat()
+    lines = open(tmp_path / "H.dat").read().splitlines()
+    Hc = sps.csr_matrix(H)
+    assert len(lines) == Hc.nnz
+    assert all(re.fullmatch(r"[ \d]{8}[ \d]{8}[ \-\d.]{20}[ \-\d.]{20}", l) for l in lines)
+    got = np.array([[float(l[16:36]), float(l[36:56])] for l in lines])
+    ij = np.array([[int(l[:8]), int(l[8:16])] for l in lines])
+    coo = Hc.tocoo()
+    np.testing.assert_array_equal(ij, np.c_[coo.row + 1, coo.col + 1])
+    np.testing.assert_allclose(got[:, 0], coo.data.real, rtol=0, atol=5.1e-11)
+    assert lines[[k for k in range(len(lines)) if ij[k, 0] == 4 and ij[k, 1] == 4][0]][16:36] == "       -0.2500000000"
+    S = open(tmp_path / "S.dat").read().splitlines()
+    assert len(S) == 100 and S[0] == "       1       1        1.0000000000        0.0000000000"
+    capfd.readouterr()
+    n.write_partition()
+    n.print_tnegf()
+    out = capfd.readouterr().out
+    assert "Number of blocks:" in out and out.split("Contact interactions:")[1].split()[:2] == ["4", "1"]
+    assert "Emin=" in out and "nPoles=" in out and f'scratch= "{tmp_path}/"' in out
+    n.close()
+
+
 def write_petsc(path_re, path_im, M):
     """PETSc ASCII pair as read_H reads it (input_output.F90:52-60, 84-100):
     4 header lines, then one "j i value" line per nonzero (1-based)."""
@@ -188,6 +224,16 @@ def test_dephasing_current_scenarios(gpu_negf, coupling, niter, estep, lo, hi):
     assert np.abs(ec[:, ::7].T - ref).max() <= 1e-10 * np.abs(ref).max()
     np.testing.assert_array_equal(tr[0], ec[0])
     assert (n.get_item_status() == 0).all()
+    n.close()
+
+
+@pytest.mark.gpu
+def test_mem_stats(gpu_negf, capfd):
+    NEGF, _ = gpu_negf
+    n = NEGF()
+    n.mem_stats()
+    out = capfd.readouterr().out
+    assert "Device memory in use:" in out and "Peak host memory:" in out
     n.close()
 
 
