@@ -356,7 +356,7 @@ std::string label_str(int v) {  // get_string_label, ln_cache.F90:669-679
   if (v > 9999999) throw std::runtime_error("ERROR: label too large (> 9999999)");
   const int w = v > 999999 ? 7 : v > 99999 ? 6 : v > 9999 ? 5 : v > 999 ? 4 : v > 99 ? 3 : 2;
   char b[16];
-  if (v < 0 && v > -10) std::snprintf(b, sizeof(b), "%s", "**");  // i2.2 overflow, as Fortran prints it
+  if (v < 0) std::snprintf(b, sizeof(b), "%s", "**");  // a sign does not fit i2.2: Fortran prints **
   else std::snprintf(b, sizeof(b), "%0*d", w, v);
   return b;
 }
