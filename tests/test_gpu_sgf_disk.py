@@ -123,3 +123,59 @@ def test_sgf_memory_and_dummy_modes_write_nothing(gpu_negf, tmp_path):
         np.testing.assert_allclose(n.get_transmission(), T0, rtol=1e-12, atol=0)
         n.close()
     assert os.listdir(tmp_path) == []
+
+
+@pytest.mark.gpu
+def test_sgf_files_ldos_dephasing_and_surface(gpu_negf, tmp_path):
+    """The LDOS and the dephasing (SCBA) solves save and reload the same
+    files with identical results; a contact with a surface block writes g_s of
+    order ns + nc, equal to the oracle's."""
+    NEGF, orc = gpu_negf
+    H = _H()
+    for mode in ("ldos", "deph"):
+        d = tmp_path / mode
+        d.mkdir()
+        out = []
+        for readold in (2, 0):
+            if mode == "ldos":
+                n = _session(NEGF, H, d, readold)
+                n.set_ldos_intervals(np.array([1, 31]), np.array([30, 60]))
+            else:  # the SCBA path wants PLs of the contacts' size: 6 PLs of 10 sites
+                n = _chain_session(NEGF, chain_dense(), [10, 20, 30, 40, 50, 60], [6, 1], -1.0, 1.0, 0.25)
+                n.params.sgfcache, n.params.readold_t_sgfs = 0, readold
+                n.set_scratch(str(d))
+                n.set_elph_dephasing(np.full(60, 0.05), 5)
+            n.solve_landauer()
+            out.append((n.get_transmission(), n.get_ldos() if mode == "ldos" else n.get_energy_current()))
+            if readold == 0 and mode == "ldos":
+                assert (n.get_item_ncyc() == 0).all()
+            n.close()
+        assert len(os.listdir(d)) == 18
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        np.testing.assert_array_equal(out[0][1], out[1][1])
+    # a surface block on contact 1
+    import scipy.sparse as sps
+    from tests.surfaces import surface_pair
+    from tests.test_gpu_surface import _dense_with_surface
+    A, _ = surface_pair(W=4, nbl=4, ns=3, seed=5)
+    Hs, plend, surfstart, surfend, contend, cblk = _dense_with_surface(A)
+    d = tmp_path / "surf"
+    d.mkdir()
+    n = NEGF()
+    n.set_h(sps.csr_matrix(Hs))
+    n.set_s_id(Hs.shape[0])
+    n.init_structure(2, np.array(contend), np.array(surfend), len(plend), np.array(plend), np.array(cblk),
+                     surfstart=np.array(surfstart))
+    p = n.params
+    p.emin, p.emax, p.estep, p.delta = -1.5, 1.5, 0.5, 1e-3
+    p.sgfcache, p.readold_t_sgfs = 0, 2
+    n.set_scratch(str(d))
+    n.solve_landauer()
+    E, _ = n.get_energies()
+    n.close()
+    D = from_dense(Hs, None, plend, surfstart, contend, cblk, surfend=surfend)
+    c0 = D.contacts[0]
+    ng = c0.nc + c0.ns
+    g, _ = read_records(d / "Matu01_00_01_03.dat", ng)
+    _, GS, _ = orc.contact_sigma(c0, D.bsz[c0.cblk - 1], complex(E[2], 1e-3))
+    assert np.abs(g - GS).max() <= 1e-10 * np.abs(GS).max()
