@@ -59,6 +59,9 @@ struct Session {
   // device copy of the working H/S, rebuilt only after H, S or the partition change
   std::unique_ptr<hn::DevStruct> dev;
   long long gen = 0, dev_gen = -1;
+  // the same structure padded to one block size, for the SCBA path
+  std::unique_ptr<hn::DevStruct> dev_pad;
+  long long dev_pad_gen = -1;
   bool sgf_cache_on = false;
   // LDOS projections (negf%dos_proj / ndos_proj, libnegf.F90:971-1041), their
   // result ldos_mat (npoints x ndos, column-major) and tun_proj (:1044-1058)
@@ -179,7 +182,12 @@ hn::Engine &engine(Session &s) {
 // (zcsr2blk_sod) and the contact blocks (extract_cont, ln_extract.F90:41-76):
 // the surface block (surfstart..surfend, may be empty) and the two PLs after
 // it; HMC/SMC span the surface and the first PL.
-std::unique_ptr<hn::DevStruct> build_struct(Session &s) {
+// With pad > 0 every PL and contact layer is stored as a pad x pad block:
+// the physical block in the top-left corner, and each padding orbital
+// decoupled with H = -1, S = 0 on its diagonal, so that E S - H = 1 there and
+// the padding adds exact zeros to every product the physical entries see.
+// This serves the SCBA path, whose sweep runs on blocks of one size.
+std::unique_ptr<hn::DevStruct> build_struct(Session &s, int pad) {
   if (!s.structure_set) throw std::runtime_error("structure not initialised (negf_init_structure)");
   if (s.cur >= (int)s.H.size() || !s.H[s.cur].present()) throw std::runtime_error("H not set");
   const Csr &H = s.H[s.cur];
@@ -202,42 +210,92 @@ std::unique_ptr<hn::DevStruct> build_struct(Session &s) {
     ns[c] = surfdim;
     if (s.cblk[c] < 1 || s.cblk[c] > nbl) throw std::runtime_error("cblk out of range");
   }
-  const int smode = sid ? 0 : (hn::S_DIAG | hn::S_OFFDIAG | hn::S_CONTACT);
-  std::unique_ptr<hn::DevStruct> st(new hn::DevStruct(nbl, bsz.data(), s.ncont, s.cblk.data(),
-                                                      nc.data(), smode, 1, ns.data()));
-  auto put = [&](int kind, int idx, const std::vector<double2> &h) {
+  if (pad > 0)
+    for (int c = 0; c < s.ncont; ++c)
+      if (ns[c]) throw std::runtime_error("SCBA path requires equal PL and contact sizes with a surface block");
+  const bool padS = pad > 0 && sid;  // padding needs an explicit S
+  const int smode = (sid && !padS) ? 0 : (hn::S_DIAG | hn::S_OFFDIAG | hn::S_CONTACT);
+  std::vector<int> bsz_d = bsz, nc_d = nc;
+  if (pad > 0) {
+    bsz_d.assign(nbl, pad);
+    nc_d.assign(s.ncont, pad);
+  }
+  std::unique_ptr<hn::DevStruct> st(new hn::DevStruct(nbl, bsz_d.data(), s.ncont, s.cblk.data(),
+                                                      nc_d.data(), smode, 1, ns.data()));
+  // kinds whose padding diagonal is -1 (H) or 0 (S); off-diagonal blocks pad with zeros
+  auto diag_kind = [](int kind) { return kind == hn::BK_HD || kind == hn::BK_H00; };
+  auto put = [&](int kind, int idx, const std::vector<double2> &h0) {
+    const long long n = st->block_elems(kind, idx);
+    std::vector<double2> h = h0;
+    if (pad > 0 && (long long)h0.size() != n) {  // top-left corner of a pad x pad block
+      const int r = (int)std::lround(std::sqrt((double)h0.size()));
+      const bool sq = (long long)r * r == (long long)h0.size();
+      int rows = r, cols = r;
+      if (!sq || (kind == hn::BK_HU || kind == hn::BK_HL || kind == hn::BK_SU || kind == hn::BK_SL ||
+                  kind == hn::BK_HMC || kind == hn::BK_SMC))
+        throw std::runtime_error("padded block of unknown shape");
+      h.assign((size_t)pad * pad, make_double2(0.0, 0.0));
+      for (int j = 0; j < cols; ++j)
+        for (int i = 0; i < rows; ++i) h[i + (size_t)j * pad] = h0[i + (size_t)j * rows];
+      if (diag_kind(kind))
+        for (int i = rows; i < pad; ++i) h[i + (size_t)i * pad] = make_double2(-1.0, 0.0);
+    }
+    if ((long long)h.size() != n) throw std::runtime_error("block size mismatch");
+    HN_CHECK(hipMemcpy(st->pool + st->off[kind][idx], h.data(), n * sizeof(double2),
+                       hipMemcpyHostToDevice));
+  };
+  // rectangular blocks: rows x cols in the top-left corner of pad x pad
+  auto putr = [&](int kind, int idx, const std::vector<double2> &h0, int rows, int cols) {
+    if (pad <= 0) return put(kind, idx, h0);
+    std::vector<double2> h((size_t)pad * pad, make_double2(0.0, 0.0));
+    for (int j = 0; j < cols; ++j)
+      for (int i = 0; i < rows; ++i) h[i + (size_t)j * pad] = h0[i + (size_t)j * rows];
     const long long n = st->block_elems(kind, idx);
     if ((long long)h.size() != n) throw std::runtime_error("block size mismatch");
     HN_CHECK(hipMemcpy(st->pool + st->off[kind][idx], h.data(), n * sizeof(double2),
                        hipMemcpyHostToDevice));
   };
+  auto eye = [](int n) {
+    std::vector<double2> e((size_t)n * n, make_double2(0.0, 0.0));
+    for (int i = 0; i < n; ++i) e[i + (size_t)i * n] = make_double2(1.0, 0.0);
+    return e;
+  };
   for (int i = 0; i < nbl; ++i) {
     put(hn::BK_HD, i, extract(H, pls[i], ple[i], pls[i], ple[i]));
     if (S) put(hn::BK_SD, i, extract(*S, pls[i], ple[i], pls[i], ple[i]));
+    else if (padS) put(hn::BK_SD, i, eye(bsz[i]));
   }
   for (int i = 0; i + 1 < nbl; ++i) {
-    put(hn::BK_HU, i, extract(H, pls[i], ple[i], pls[i + 1], ple[i + 1]));
-    put(hn::BK_HL, i, extract(H, pls[i + 1], ple[i + 1], pls[i], ple[i]));
+    putr(hn::BK_HU, i, extract(H, pls[i], ple[i], pls[i + 1], ple[i + 1]), bsz[i], bsz[i + 1]);
+    putr(hn::BK_HL, i, extract(H, pls[i + 1], ple[i + 1], pls[i], ple[i]), bsz[i + 1], bsz[i]);
     if (S) {
-      put(hn::BK_SU, i, extract(*S, pls[i], ple[i], pls[i + 1], ple[i + 1]));
-      put(hn::BK_SL, i, extract(*S, pls[i + 1], ple[i + 1], pls[i], ple[i]));
+      putr(hn::BK_SU, i, extract(*S, pls[i], ple[i], pls[i + 1], ple[i + 1]), bsz[i], bsz[i + 1]);
+      putr(hn::BK_SL, i, extract(*S, pls[i + 1], ple[i + 1], pls[i], ple[i]), bsz[i + 1], bsz[i]);
+    } else if (padS) {
+      putr(hn::BK_SU, i, std::vector<double2>((size_t)bsz[i] * bsz[i + 1]), bsz[i], bsz[i + 1]);
+      putr(hn::BK_SL, i, std::vector<double2>((size_t)bsz[i] * bsz[i + 1]), bsz[i + 1], bsz[i]);
     }
   }
   for (int c = 0; c < s.ncont; ++c) {
     const int c0 = s.surfstart[c], q = ns[c], c1 = c0 + q, n = nc[c];  // surface at c0, PL1 at c1
     const int b = s.cblk[c] - 1;
     put(hn::BK_H00, c, extract(H, c1, c1 + n - 1, c1, c1 + n - 1));
-    put(hn::BK_H01, c, extract(H, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1));
-    put(hn::BK_HMC, c, extract(H, pls[b], ple[b], c0, c1 + n - 1));
+    putr(hn::BK_H01, c, extract(H, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1), n, n);
+    putr(hn::BK_HMC, c, extract(H, pls[b], ple[b], c0, c1 + n - 1), bsz[b], q + n);
     if (q > 0) {
       put(hn::BK_HSS, c, extract(H, c0, c1 - 1, c0, c1 - 1));
       put(hn::BK_HS1, c, extract(H, c0, c1 - 1, c1, c1 + n - 1));
       put(hn::BK_H1S, c, extract(H, c1, c1 + n - 1, c0, c1 - 1));
     }
+    if (padS) {
+      putr(hn::BK_S00, c, eye(n), n, n);
+      putr(hn::BK_S01, c, std::vector<double2>((size_t)n * n), n, n);
+      putr(hn::BK_SMC, c, std::vector<double2>((size_t)bsz[b] * n), bsz[b], n);
+    }
     if (S) {
-      put(hn::BK_S00, c, extract(*S, c1, c1 + n - 1, c1, c1 + n - 1));
-      put(hn::BK_S01, c, extract(*S, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1));
-      put(hn::BK_SMC, c, extract(*S, pls[b], ple[b], c0, c1 + n - 1));
+      putr(hn::BK_S00, c, extract(*S, c1, c1 + n - 1, c1, c1 + n - 1), n, n);
+      putr(hn::BK_S01, c, extract(*S, c1, c1 + n - 1, c1 + n, c1 + 2 * n - 1), n, n);
+      putr(hn::BK_SMC, c, extract(*S, pls[b], ple[b], c0, c1 + n - 1), bsz[b], q + n);
       if (q > 0) {
         put(hn::BK_SSS, c, extract(*S, c0, c1 - 1, c0, c1 - 1));
         put(hn::BK_SS1, c, extract(*S, c0, c1 - 1, c1, c1 + n - 1));
@@ -299,7 +357,7 @@ std::vector<int> pairs_of(const int *v) {
 // the partition); fictitious-contact parameters are applied on every call.
 hn::DevStruct &device_struct(Session &s) {
   if (!s.dev || s.dev_gen != s.gen) {
-    s.dev = build_struct(s);
+    s.dev = build_struct(s, 0);
     s.dev_gen = s.gen;
   }
   hn::DevStruct &d = *s.dev;
@@ -321,6 +379,29 @@ hn::DevStruct &device_struct(Session &s) {
     }
   }
   return d;
+}
+
+// The structure the SCBA path runs on: the device copy itself when every PL
+// and contact layer has one size, else a copy padded to the largest (see
+// build_struct); pad returns that size (0: not padded).
+hn::DevStruct &scba_struct(Session &s, int &pad) {
+  hn::DevStruct &d = device_struct(s);
+  pad = 0;
+  if (d.uniform()) return d;
+  for (int b : d.bsz) pad = std::max(pad, b);
+  for (int c = 0; c < d.ncont; ++c) pad = std::max(pad, d.nc[c]);
+  if (!s.dev_pad || s.dev_pad_gen != s.gen) {
+    s.dev_pad = build_struct(s, pad);
+    s.dev_pad_gen = s.gen;
+  }
+  hn::DevStruct &q = *s.dev_pad;
+  for (int c = 0; c < q.ncont; ++c)
+    if (q.fict[c] != d.fict[c] || q.contact_dos[c] != d.contact_dos[c]) {
+      q.fict[c] = d.fict[c];
+      q.contact_dos[c] = d.contact_dos[c];
+      q.touched();
+    }
+  return q;
 }
 
 hn::SolveOpts opts_of(Session &s) {
@@ -423,6 +504,9 @@ bool sgf_read_file(const std::string &fn, std::vector<double2> &g, int n) {
 // files are all there is not recomputed, ncyc 0); after the solve g_s is
 // written for every point that had no file (readOldT_SGFs 0) or for every
 // point (readOldT_SGFs 2), as surface_green does (contselfenergy.F90:161-259).
+// On a structure padded for the SCBA path the files hold the physical
+// top-left block; the padding orbitals' g_s is exactly 1 on the diagonal
+// (E S - H = 1 there, decoupled), which a load restores.
 struct DiskSgf {
   Session &s;
   hn::DevStruct &st;
@@ -430,6 +514,7 @@ struct DiskSgf {
   hn::SolveOpts o;
   bool on = false;
   std::vector<char> save;  // (point, contact) written after the solve
+  int phys(int c) const { return s.dev ? s.dev->ngs(c) : st.ngs(c); }
   DiskSgf(Session &s_, hn::DevStruct &st_, hn::Engine &eng_, const hn::SolveOpts &o_)
       : s(s_), st(st_), eng(eng_), o(o_) {
     on = s.p.sgfcache == 0 && o.sgf_mode != 0;
@@ -456,9 +541,21 @@ struct DiskSgf {
     for (int i = 0; i < n; ++i)
       for (int c = 0; c < nc; ++c) {
         if (st.fict[c]) continue;
-        const bool have = o.sgf_mode == 1 && sgf_read_file(sgf_filename(s, c + 1, i + 1), g, st.ngs(c));
-        if (have) eng.sgf_insert(st, o, 0, s.en[i], c, g.data(), 0);
-        else save[(size_t)i * nc + c] = 1;
+        const int np = phys(c), ng = st.ngs(c);
+        const bool have = o.sgf_mode == 1 && sgf_read_file(sgf_filename(s, c + 1, i + 1), g, np);
+        if (have) {
+          if (ng != np) {  // expand into the padded block
+            std::vector<double2> e((size_t)ng * ng, make_double2(0.0, 0.0));
+            for (int j = 0; j < ng; ++j)
+              for (int r = 0; r < ng; ++r)
+                e[r + (size_t)j * ng] = (r < np && j < np) ? g[r + (size_t)j * np]
+                                        : make_double2(r == j ? 1.0 : 0.0, 0.0);
+            g.swap(e);
+          }
+          eng.sgf_insert(st, o, 0, s.en[i], c, g.data(), 0);
+        } else {
+          save[(size_t)i * nc + c] = 1;
+        }
       }
   }
   void finish() {
@@ -468,10 +565,14 @@ struct DiskSgf {
     for (int i = 0; i < n; ++i)
       for (int c = 0; c < nc; ++c) {
         if (!save[(size_t)i * nc + c]) continue;
-        g.resize((size_t)st.ngs(c) * st.ngs(c));
+        const int np = phys(c), ng = st.ngs(c);
+        g.resize((size_t)ng * ng);
         int stat = 0;
-        if (eng.sgf_fetch(st, o, 0, s.en[i], c, g.data(), &stat))
-          sgf_write_file(sgf_filename(s, c + 1, i + 1), g.data(), st.ngs(c));
+        if (!eng.sgf_fetch(st, o, 0, s.en[i], c, g.data(), &stat)) continue;
+        if (ng != np)  // the physical top-left block, column by column
+          for (int j = 0; j < np; ++j)
+            for (int r = 0; r < np; ++r) g[r + (size_t)j * np] = g[r + (size_t)j * ng];
+        sgf_write_file(sgf_filename(s, c + 1, i + 1), g.data(), np);
       }
   }
 };
@@ -614,17 +715,27 @@ void compute_dephasing_transmission(Session &s) {
   std::vector<double> frm(s.ncont + 1, 0.0);
   frm[ni[0] - 1] = 1.0;
   frm[nf[0] - 1] = 0.0;
-  hn::DevStruct *st = &device_struct(s);
+  int pad = 0;
+  hn::DevStruct *st = &scba_struct(s, pad);
+  const hn::DevStruct &phys = *s.dev;
   int ndev = 0;
-  for (int b : st->bsz) ndev += b;
+  for (int b : phys.bsz) ndev += b;
   if ((int)s.coupling.size() < ndev) throw std::runtime_error("dephasing coupling shorter than the device");
+  // the coupling in the device layout the SCBA runs on (zero on padding orbitals)
+  std::vector<double> cpl(s.coupling.begin(), s.coupling.begin() + ndev);
+  if (pad > 0) {
+    cpl.assign((size_t)pad * phys.nbl, 0.0);
+    for (int i = 0, o0 = 0; i < phys.nbl; o0 += phys.bsz[i], ++i)
+      for (int r = 0; r < phys.bsz[i]; ++r) cpl[(size_t)i * pad + r] = s.coupling[o0 + r];
+  }
+  const int ndev_run = (int)cpl.size();
   hn::Engine &eng = engine(s);
   DevBuf dE(sizeof(double) * n), dsl(sizeof(int) * n), dC(sizeof(double) * n * s.ncont),
       dI(sizeof(int) * n), dErr(sizeof(double) * n), dS(sizeof(int) * n),
-      dCp(sizeof(double) * ndev), dF(sizeof(double) * (s.ncont + 1));
+      dCp(sizeof(double) * ndev_run), dF(sizeof(double) * (s.ncont + 1));
   HN_CHECK(hipMemcpy(dE.p, s.en.data(), sizeof(double) * n, hipMemcpyHostToDevice));
   HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
-  HN_CHECK(hipMemcpy(dCp.p, s.coupling.data(), sizeof(double) * ndev, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dCp.p, cpl.data(), sizeof(double) * ndev_run, hipMemcpyHostToDevice));
   HN_CHECK(hipMemcpy(dF.p, frm.data(), sizeof(double) * (s.ncont + 1), hipMemcpyHostToDevice));
   hn::SCBAOpts so;
   so.coupling = dCp.as<double>();

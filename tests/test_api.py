@@ -238,6 +238,55 @@ def test_mem_stats(gpu_negf, capfd):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("overlap", [False, True])
+def test_dephasing_nonuniform_pls(gpu_negf, overlap, tmp_path):
+    """PLs of 15 sites with contact layers of 10: the SCBA runs on the
+    structure padded to 15 (decoupled padding orbitals) and matches the
+    oracle's unpadded SCBA at 1e-10; its surface-function files hold the
+    physical 10 x 10 g_s and reload bitwise."""
+    import scipy.sparse as sps
+    NEGF, orc = gpu_negf
+    ons = np.zeros(60)
+    ons[25] = ons[35] = 1.0
+    H = chain_dense(onsite=ons)
+    S = None
+    if overlap:
+        S = np.eye(100, dtype=np.complex128)
+        for i in range(1, 100):
+            if H[i, i - 1] != 0:
+                S[i, i - 1] = S[i - 1, i] = 0.05
+    plend = [15, 30, 45, 60]
+    res = []
+    for readold in (2, 0):
+        n = NEGF()
+        n.set_h(sps.csr_matrix(H))
+        if overlap:
+            n.set_s(sps.csr_matrix(S))
+        else:
+            n.set_s_id(100)
+        n.init_structure(2, np.array([80, 100]), np.array([60, 80]), 4, np.array(plend), np.array([4, 1]),
+                         surfstart=np.array([61, 81]))
+        p = n.params
+        p.emin, p.emax, p.estep = -1.0, 1.0, 0.25
+        p.kbt_t[0] = p.kbt_t[1] = 1e-3
+        p.sgfcache, p.readold_t_sgfs = 0, readold
+        n.set_scratch(str(tmp_path))
+        n.set_elph_dephasing(np.full(60, 0.05), 10)
+        n.solve_landauer()
+        E, _ = n.get_energies()
+        res.append(n.get_energy_current())
+        assert (n.get_item_status() == 0).all()
+        n.close()
+    np.testing.assert_array_equal(res[0], res[1])
+    g = np.fromfile(tmp_path / "Matu01_00_01_03.dat", dtype=np.int32).reshape(-1, 8)
+    assert (g[:, 0] == 24).all() and g[:, 1].max() == 10 and g[:, 2].max() == 10
+    s = from_dense(H, S, plend, [61, 81], [80, 100], [4, 1])
+    ref = np.array([orc.scba_mw(s, e, 1e-4, np.full(60, 0.05), 10, 1e-7, [1.0, 0.0, 0.0], 3)["curr"]
+                    for e in E])
+    assert np.abs(res[0].T - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
 def test_landauer_double_barrier(gpu_negf):
     """tests/cpython/hello_ctypes.py structure: 4 PLs of 15 sites, barriers on
     sites 26 and 36, contact 1 on PL 4, contact 2 on PL 1; T(E) and the
