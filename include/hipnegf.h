@@ -293,7 +293,7 @@ int hipnegf_density_accumulate(hipnegf_engine e, hipnegf_struct st, int nitems, 
  * structure (get_tun_mask, iterative_cpu.F90:585-617).  n = 0 clears it. */
 int hipnegf_struct_set_tun_proj(hipnegf_struct st, int n, const int *idx);
 
-/* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):/* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):
+/* Elastic dephasing SCBA + Meir-Wingreen current per item (host arrays):
  * calculate_elastic_scba (iterative.F90:289-363) with the ElPhonDephD model
  * (elphdd.F90; coupling per device orbital, squared internally), then
  * iterative_meir_wingreen (iterative.F90:382-414).
