@@ -288,7 +288,7 @@ int hipnegf_density_accumulate(hipnegf_engine e, hipnegf_struct st, int nitems, 
                                const double *w, const double *frm, int ref, int outer,
                                double *acc, int *status);
 
-/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058): only the listed 1-based/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058): only the listed 1-based
+/* tun_proj (set_tun_indexes, libnegf.F90:1044-1058): only the listed 1-based
  * device orbitals of the traced block enter every transmission trace of this
  * structure (get_tun_mask, iterative_cpu.F90:585-617).  n = 0 clears it. */
 int hipnegf_struct_set_tun_proj(hipnegf_struct st, int n, const int *idx);
