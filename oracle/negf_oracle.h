@@ -179,4 +179,16 @@ double or_transmission_batch(const or_struct *s, int npts, const double *E, doub
 
 int or_omp_max_threads(void);
 
+/* ---- double-double accuracy reference (negf_dd.c) ------------------------ */
+/* or_transmission's fast path (two contacts, the lower on block 1, square
+ * blocks of one size, no surface block) evaluated in double-double.
+ * T_out[2] = (high, low) parts of T; ncyc_out[2] (may be NULL).  Returns 0,
+ * -1 for a structure outside that scope, or a zero-pivot code. */
+int dd_transmission(const or_struct *s, double E, double delta, double sgfacc, double *T_out,
+                    int *ncyc_out);
+/* the same for npts energies, OpenMP over points (T_out[2*npts],
+ * ncyc_out[ncont*npts]) */
+int dd_transmission_batch(const or_struct *s, int npts, const double *E, double delta,
+                          double sgfacc, double *T_out, int *ncyc_out, int nthreads);
+
 #endif

@@ -105,6 +105,9 @@ def lib():
         L.or_gn_point.restype = C.c_int
         L.or_gn_point.argtypes = [C.c_void_p, C.c_double, C.c_double, C.c_void_p, C.c_int] + \
             [C.c_void_p] * 6 + [C.c_double, C.c_double]
+        L.dd_transmission_batch.restype = C.c_int
+        L.dd_transmission_batch.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_double,
+                                            C.c_double, C.c_void_p, C.c_void_p, C.c_int]
         _LIB = L
     return _LIB
 
@@ -211,6 +214,20 @@ def transmission_batch(s, E: np.ndarray, delta: float, ni: int = 1, nf: int = 2,
     secs = lib().or_transmission_batch(os_.ptr, len(E), E.ctypes.data, delta, ni, nf,
                                        T.ctypes.data, ncyc.ctypes.data, nthreads, SGFACC)
     return T, ncyc, secs
+
+
+def transmission_dd(s, E: np.ndarray, delta: float, nthreads: int = 0):
+    """T(E) of the fast path in double-double arithmetic (negf_dd.c): returns
+    (T_hi, T_lo, ncyc[npts, 2]); T_hi + T_lo is the value."""
+    os_ = OracleStruct(s)
+    E = np.ascontiguousarray(np.atleast_1d(E), dtype=np.float64)
+    T = np.zeros((len(E), 2))
+    ncyc = np.zeros((len(E), s.ncont), dtype=np.int32)
+    r = lib().dd_transmission_batch(os_.ptr, len(E), E.ctypes.data, delta, SGFACC, T.ctypes.data,
+                                    ncyc.ctypes.data, nthreads or (os.cpu_count() or 1))
+    if r != 0:
+        raise ValueError("dd_transmission: structure outside the fast-path scope or singular block")
+    return T[:, 0].copy(), T[:, 1].copy(), ncyc
 
 
 def contact_sigma(c, nd: int, z: complex, okco: Optional[np.ndarray] = None):
