@@ -433,9 +433,9 @@ void *hipnegf_engine_stream(hipnegf_engine e) { return (void *)e->eng->stream();
 int hipnegf_struct_create(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
                           int smode, int nslots, hipnegf_struct *st) {
   return guard([&] {
-    auto *x = new hipnegf_struct_s;
+    std::unique_ptr<hipnegf_struct_s> x(new hipnegf_struct_s);
     x->st.reset(new hn::DevStruct(nbl, bsz, ncont, cblk, nc, smode, nslots));
-    *st = x;
+    *st = x.release();
     return 0;
   });
 }
@@ -443,9 +443,9 @@ int hipnegf_struct_create_surf(int nbl, const int *bsz, int ncont, const int *cb
                                const int *nc, const int *ns, int smode, int nslots,
                                hipnegf_struct *st) {
   return guard([&] {
-    auto *x = new hipnegf_struct_s;
+    std::unique_ptr<hipnegf_struct_s> x(new hipnegf_struct_s);
     x->st.reset(new hn::DevStruct(nbl, bsz, ncont, cblk, nc, smode, nslots, ns));
-    *st = x;
+    *st = x.release();
     return 0;
   });
 }

@@ -84,6 +84,18 @@ void check_shape(int nbl, const std::vector<int> &bsz, int ncont, const std::vec
   for (int c = 0; c < ncont; ++c)
     if (cblk[c] < 1 || cblk[c] > nbl || nc[c] < 1 || ns[c] < 0)
       throw std::runtime_error("structure: bad contact description");
+  // every block the solves invert (device PLs, contact PLs, surface + PL) is
+  // inverted by the batched inverse, which handles orders up to kMaxBlockOrder
+  for (int i = 0; i < nbl; ++i)
+    if (bsz[i] > kMaxBlockOrder)
+      throw std::runtime_error("structure: PL " + std::to_string(i + 1) + " has " +
+                               std::to_string(bsz[i]) + " orbitals; blocks up to " +
+                               std::to_string(kMaxBlockOrder) + " are supported");
+  for (int c = 0; c < ncont; ++c)
+    if (nc[c] + ns[c] > kMaxBlockOrder)
+      throw std::runtime_error("structure: contact " + std::to_string(c + 1) + " has " +
+                               std::to_string(nc[c] + ns[c]) + " orbitals (surface + PL); up to " +
+                               std::to_string(kMaxBlockOrder) + " are supported");
 }
 }  // namespace
 
@@ -1043,7 +1055,11 @@ void Engine::t_batch(DevStruct &st, int B, const double *dE, const int *dslot,
     copy_real(Ef, dE, B, stream_);
     copy_int(Sf, dslot, B, stream_);
     fill_int(Stf, B, 0, stream_);
-    const std::vector<long long> key = {(long long)(uintptr_t)st.pool, st.slot_elems, st.nbl, st.smode,
+    // st.uid is unique per structure for the life of the process, so a graph
+    // captured for a destroyed structure (whose pool address may be reused
+    // with other contact blocks) never matches again; it ages out of the cache
+    const std::vector<long long> key = {(long long)st.uid, (long long)(uintptr_t)st.pool,
+                                        st.slot_elems, st.nbl, st.smode,
                                         st.nmax(), B, pni[0], pnf[0], (long long)(uintptr_t)arena_.base,
                                         (long long)arena_.used, o.share_mirror,
                                         (long long)(uintptr_t)st.tun_mask,

@@ -85,6 +85,20 @@ def load():
     L.hipnegf_CmultMat.argtypes = L.hipnegf_ZmultMat.argtypes
     L.hipnegf_Zinverse.argtypes = [vp, vp, vp, vp, C.c_size_t]
     L.hipnegf_Cinverse.argtypes = L.hipnegf_Zinverse.argtypes
+    L.hipnegf_createMat.argtypes = [C.POINTER(vp), C.c_size_t]
+    L.hipnegf_deleteMat.argtypes = [C.POINTER(vp)]
+    L.hipnegf_mallocAsync.argtypes = [C.POINTER(vp), C.c_size_t]
+    L.hipnegf_freeAsync.argtypes = [C.POINTER(vp)]
+    L.hipnegf_copyMatH2D.argtypes = [vp, vp, C.c_size_t]
+    L.hipnegf_copyMatH2D_async.argtypes = [vp, vp, C.c_size_t]
+    L.hipnegf_copyMatD2H.argtypes = [vp, vp, C.c_size_t]
+    L.hipnegf_meminfo.argtypes = [C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)]
+    L.hipnegf_getDeviceCount.argtypes = [C.POINTER(i)]
+    L.hipnegf_getDevice.argtypes = [C.POINTER(i)]
+    L.hipnegf_setDevice.argtypes = [i]
+    L.hipnegf_solverInit.argtypes = [C.POINTER(vp)]
+    L.hipnegf_solverFinalize.argtypes = [vp]
+    L.hipnegf_Cmatcopy.argtypes = [vp, vp, vp, C.c_size_t]
     L.hipnegf_Zmatsum.argtypes = [vp, C.c_size_t, C.c_size_t, vp, vp, vp, vp, vp, i]
     L.hipnegf_Cmatsum.argtypes = L.hipnegf_Zmatsum.argtypes
     L.hipnegf_Zinitmat.argtypes = [vp, C.c_size_t]

@@ -74,6 +74,20 @@ def test_defaults_and_host_errors():
     n.close()
 
 
+def test_block_order_limit_is_rejected(lib):
+    """Blocks above the inverse's 512 orbitals are refused when the structure
+    is created, before any device call (host only)."""
+    h = C.c_void_p()
+    for bsz, nc in (([64, 600, 64], [64, 64]), ([64, 64], [513, 64])):
+        b = np.array(bsz, np.int32)
+        cb = np.array([len(bsz), 1], np.int32)
+        n = np.array(nc, np.int32)
+        rc = lib.hipnegf_struct_create(len(bsz), b.ctypes.data, 2, cb.ctypes.data, n.ctypes.data,
+                                       0, 1, C.byref(h))
+        assert rc == -1
+        assert b"512" in lib.hipnegf_last_error()
+
+
 def test_contact_blocks_get_pls_and_reference():
     """find_cblocks known answers on the 60-site chain of 4 PLs (contact 1
     rows 61..80 couple to site 60 in PL 4, contact 2 rows 81..100 to site 1 in
