@@ -82,6 +82,16 @@ int hipnegf_ZmultMat(hipnegf_handle h, size_t m, size_t n, size_t k, const void 
 int hipnegf_Cinverse(hipnegf_handle hb, hipnegf_handle hs, void *A, void *Ainv, size_t n);
 int hipnegf_Zinverse(hipnegf_handle hb, hipnegf_handle hs, void *A, void *Ainv, size_t n);
 
+/* hipnegf_Zinverse without any device allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * temporaries live in one library-owned device array per GPU (5 MiB, held
+ * until the process ends), so n <= 512 (larger n returns -1 with
+ * hipnegf_last_error set).  Calls that overlap on one GPU share that array
+ * and race; serialise them.  Same kernels and results as hipnegf_Zinverse;
+ * the current device must be the handle's. */
+int hipnegf_Zinverse_noalloc(hipnegf_handle hb, void *A, void *Ainv, size_t n);
+
 /* C = alpha op(A) + beta op(B); cu_{C,Z}matsum (gpuroutines.cu:567-617):
  * dagger 0: A, B m x n; 1: op(A) = A^H (A stored n x m); 2: op(B) = B^H. */
 int hipnegf_Cmatsum(hipnegf_handle h, size_t m, size_t n, const void *alpha, const void *A,

@@ -5,6 +5,7 @@
 // every compute call enqueues on the context's stream and waits for it.
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -234,6 +235,70 @@ int inv_batched(hipnegf_engine e, int n, void *A, long long sA, void *Ainv, long
 }
 
 
+// ---- allocation-free block inverse ------------------------------------------
+// hipnegf_Zinverse_noalloc keeps its temporaries in `scratch`, one array per
+// GPU that lives until the process ends: the working copy of A (the blocked
+// inverse overwrites it), the singular flag and the blocked inverse's
+// workspace, for one block of order <= kNoAllocMaxN.  Host code never passes
+// the array's name to a kernel: the device address of each GPU's copy comes
+// from hipGetSymbolAddress, resolved once per device (at hipnegf_blasInit, or
+// at the first call on a device) and kept in g_scratch_dev.
+constexpr int kNoAllocMaxN = 512;
+constexpr size_t kScratchBytes = (size_t)5 << 20;
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+void *scratch_ptr(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) throw std::runtime_error("device index out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[dev]) {
+    int cur = -1;
+    HN_CHECK(hipGetDevice(&cur));
+    if (cur != dev) throw std::runtime_error("scratch: device " + std::to_string(dev) +
+                                             " is not the current device");
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[dev] = p;
+  }
+  return g_scratch_dev[dev];
+}
+
+// Same kernels and result as inverse1<double>; every temporary in scratch.
+int inverse_noalloc(hipnegf_handle h, void *A, void *Ainv, size_t n) {
+  if (n == 0) return 0;
+  if (!h || !h->eng) throw std::runtime_error("hipnegf_Zinverse_noalloc: null handle");
+  if (n > (size_t)kNoAllocMaxN)
+    throw std::runtime_error("hipnegf_Zinverse_noalloc: block order " + std::to_string(n) +
+                             " exceeds the scratch limit of " + std::to_string(kNoAllocMaxN));
+  int cur = -1;
+  HN_CHECK(hipGetDevice(&cur));
+  if (cur != h->eng->device())
+    throw std::runtime_error("hipnegf_Zinverse_noalloc: the current device is not the handle's");
+  auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t bytes = n * n * sizeof(double2);
+  const bool blocked = (int)n > hn::inv_lds_max_n(8);
+  const size_t need = al(bytes) + 256 + (blocked ? hn::inv_work_bytes((int)n, 1, 8) : 0);
+  if (need > kScratchBytes) throw std::runtime_error("hipnegf_Zinverse_noalloc: scratch too small");
+  char *base = (char *)scratch_ptr(cur);
+  double2 *work = (double2 *)base;
+  int *status = (int *)(base + al(bytes));
+  hn::InvWork<double> w;
+  if (blocked) w = hn::inv_work_carve<double>(base + al(bytes) + 256, (int)n, 1);
+  hipStream_t s = sof(h);
+  HN_CHECK(hipMemcpyAsync(work, A, bytes, hipMemcpyDeviceToDevice, s));
+  hn::fill_int(status, 1, 0, s);
+  hn::inv_z((int)n, work, (long long)n * n, (double2 *)Ainv, (long long)n * n, 1, nullptr, status,
+            w, s);
+  int hs = 0;
+  HN_CHECK(hipMemcpyAsync(&hs, status, sizeof(int), hipMemcpyDeviceToHost, s));
+  HN_CHECK(hipStreamSynchronize(s));
+  return hs ? 1 : 0;
+}
+
 hn::SolveOpts to_opts(const hipnegf_opts *o) {
   hn::SolveOpts so;
   if (o) {
@@ -304,6 +369,9 @@ int hipnegf_blasInit(hipnegf_handle *h) {
     HN_CHECK(hipGetDevice(&dev));
     auto *c = new hipnegf_ctx;
     c->eng.reset(new hn::Engine(dev));
+    // resolve this device's scratch address once, at set-up; a failure here
+    // is reported by the first hipnegf_Zinverse_noalloc call instead
+    try { scratch_ptr(dev); } catch (const std::exception &) {}
     *h = c;
     return 0;
   });
@@ -328,6 +396,9 @@ int hipnegf_Cinverse(hipnegf_handle hb, hipnegf_handle, void *A, void *Ainv, siz
 }
 int hipnegf_Zinverse(hipnegf_handle hb, hipnegf_handle, void *A, void *Ainv, size_t n) {
   return guard([&] { return inverse1<double>(hb, A, Ainv, n); });
+}
+int hipnegf_Zinverse_noalloc(hipnegf_handle hb, void *A, void *Ainv, size_t n) {
+  return guard([&] { return inverse_noalloc(hb, A, Ainv, n); });
 }
 int hipnegf_Cmatsum(hipnegf_handle h, size_t m, size_t n, const void *alpha, const void *A,
                     const void *beta, const void *B, void *C, int dagger) {

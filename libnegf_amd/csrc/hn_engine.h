@@ -162,6 +162,7 @@ class Engine {
   explicit Engine(int device);
   ~Engine();
   hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
 
   // Coherent transmission for nitems (E, slot) points.  All pointers device.
   // T[nitems], ncyc[nitems*ncont], status[nitems]; the pair is (o.ni, o.nf).

@@ -85,6 +85,7 @@ def load():
     L.hipnegf_CmultMat.argtypes = L.hipnegf_ZmultMat.argtypes
     L.hipnegf_Zinverse.argtypes = [vp, vp, vp, vp, C.c_size_t]
     L.hipnegf_Cinverse.argtypes = L.hipnegf_Zinverse.argtypes
+    L.hipnegf_Zinverse_noalloc.argtypes = [vp, vp, vp, C.c_size_t]
     L.hipnegf_createMat.argtypes = [C.POINTER(vp), C.c_size_t]
     L.hipnegf_deleteMat.argtypes = [C.POINTER(vp)]
     L.hipnegf_mallocAsync.argtypes = [C.POINTER(vp), C.c_size_t]
