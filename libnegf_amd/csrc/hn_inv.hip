@@ -150,10 +150,10 @@ __global__ __launch_bounds__(NT) void k_inv_lds(int n, typename CT_<R>::t *A, lo
 // ---------------------------------------------------------------------------
 constexpr int NTG = 512;  // threads of the Gauss-Jordan kernel: two halves of 256 rows
 
-// Pivot search key of an unpivoted row r (r < 512): the high word of the
-// FP64 |.|_1 (exponent and 10 mantissa bits; NaN ranks as inf), a flag bit,
-// and 511 - r, so that a larger magnitude wins and ties within 2^-10 go to
-// the smaller row index; 0 marks rows that cannot be chosen.
+// Encoding of a chosen pivot row r (r < 512) with its FP64 |.|_1: the high
+// word of the magnitude (exponent and 10 mantissa bits), a flag bit and
+// 511 - r; the row is read back from the low bits and a magnitude below
+// 2^-1012 (the high word below 0x400) flags a zero pivot.
 __device__ inline unsigned pivot_key(double v, int r) {
   const unsigned hi = min((unsigned)((unsigned long long)__double_as_longlong(v) >> 32), 0x7FF00000u);
   return (hi & ~0x3FFu) | 0x200u | (unsigned)(511 - r);
@@ -161,21 +161,6 @@ __device__ inline unsigned pivot_key(double v, int r) {
 __device__ inline unsigned pivot_key(float v, int r) {
   return pivot_key((double)v, r);
 }
-template <int CTRL> __device__ inline unsigned dpp_max(unsigned k) {
-  return max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, CTRL, 0xF, 0xF, false));
-}
-// maximum key over the 64 lanes of a wave (every lane active): DPP butterflies
-// inside each row of 16 lanes, then the four row maxima read as scalars
-__device__ inline unsigned wave_max_key(unsigned k) {
-  k = dpp_max<0xB1>(k);   // quad_perm [1,0,3,2]
-  k = dpp_max<0x4E>(k);   // quad_perm [2,3,0,1]
-  k = dpp_max<0x124>(k);  // row_ror:4
-  k = dpp_max<0x128>(k);  // row_ror:8
-  const unsigned a = __builtin_amdgcn_readlane((int)k, 0), b = __builtin_amdgcn_readlane((int)k, 16);
-  const unsigned c = __builtin_amdgcn_readlane((int)k, 32), d = __builtin_amdgcn_readlane((int)k, 48);
-  return max(max(a, b), max(c, d));
-}
-
 // 1/b for a pivot: b scaled by a power of two (exact) so that |b'|^2 lies in
 // [1/4, 2), the hardware reciprocal of |b'|^2 refined by two Newton-Raphson
 // steps (r <- r + r (1 - d r)), and conj(b') r scaled back; 0 for b == 0.
@@ -201,6 +186,223 @@ __device__ inline double lane_read(double v, int lane) {
 __device__ inline float lane_read(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
+
+// ---------------------------------------------------------------------------
+// register-resident LU inverse, n <= 64 (complex double)
+// ---------------------------------------------------------------------------
+// The reference's algorithm: ZGETRF (partial pivoting; the pivot is the first
+// largest |Re| + |Im| of the column in the current row order, as IZAMAX
+// picks it) followed by ZGETRS against the identity (inversions.F90:807-839).
+//
+// One workgroup of NW waves per item.  Lane l holds physical row l of A and
+// of Y = L^-1 (the forward substitution of ZGETRS against P I; column j of Y
+// belongs to the row pivoted at step j, so Y is lower triangular in pivot
+// order); wave w holds columns w*CT .. w*CT+CT-1 of both in registers.  Rows
+// are never moved: pos (per lane) is the row's current position in LAPACK's
+// row order, and a pivot step exchanges two positions.
+//
+// Step k (one barrier): every lane reads the pivot row p and 1/A(p,k)
+// published by the wave owning column k, and its own entry A(r,k) from LDS;
+// each wave reads row p of its columns through a per-wave LDS slot, and
+// every unpivoted row applies row_r -= (A(r,k)/A(p,k)) row_p to its A
+// columns right of k and to its Y columns up to k -- the elimination of
+// ZGETRF and the forward substitution of ZGETRS in one pass.  Waves whose A
+// columns all lie left of k, or whose Y columns all lie right of it, skip
+// that half.  Every step writes each updated A value one register lower, so
+// the wave owning column k always finds it in register 0; the owner of
+// column k+1 searches its pivot before the Y update.  Column k of U goes to
+// LDS as soon as it is final.  The pivot search is exact: the largest
+// |Re| + |Im| as a 64-bit key, then the smallest position among the lanes
+// holding it.
+//
+// Then each wave back-substitutes U X = Y on its Y columns on its own (U and
+// the pivot reciprocals from LDS, no barriers); row pos(l) of X is in lane l,
+// and column j of X is column perm(j) of the inverse (the physical row
+// pivoted at step j).
+
+// 64-bit key helpers for the exact pivot search: DPP butterflies inside each
+// row of 16 lanes on both halves, then the four row results read as scalars.
+template <int CTRL> __device__ inline unsigned long long dpp64(unsigned long long k) {
+  const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(k >> 32), CTRL, 0xF, 0xF, false);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)k, CTRL, 0xF, 0xF, false);
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ inline unsigned long long rl64(unsigned long long k, int lane) {
+  const unsigned hi = __builtin_amdgcn_readlane((int)(k >> 32), lane);
+  const unsigned lo = __builtin_amdgcn_readlane((int)(unsigned)k, lane);
+  return ((unsigned long long)hi << 32) | lo;
+}
+__device__ inline unsigned long long wave_max64(unsigned long long k) {
+  k = max(k, dpp64<0xB1>(k));   // quad_perm [1,0,3,2]
+  k = max(k, dpp64<0x4E>(k));   // quad_perm [2,3,0,1]
+  k = max(k, dpp64<0x124>(k));  // row_ror:4
+  k = max(k, dpp64<0x128>(k));  // row_ror:8
+  return max(max(rl64(k, 0), rl64(k, 16)), max(rl64(k, 32), rl64(k, 48)));
+}
+template <int CTRL> __device__ inline unsigned dpp_min(unsigned k) {
+  return min(k, (unsigned)__builtin_amdgcn_update_dpp(-1, (int)k, CTRL, 0xF, 0xF, false));
+}
+__device__ inline unsigned wave_min32(unsigned k) {
+  k = dpp_min<0xB1>(k);
+  k = dpp_min<0x4E>(k);
+  k = dpp_min<0x124>(k);
+  k = dpp_min<0x128>(k);
+  const unsigned a = __builtin_amdgcn_readlane((int)k, 0), b = __builtin_amdgcn_readlane((int)k, 16);
+  const unsigned c = __builtin_amdgcn_readlane((int)k, 32), d = __builtin_amdgcn_readlane((int)k, 48);
+  return min(min(a, b), min(c, d));
+}
+
+template <int NW, int CT>
+__global__ __launch_bounds__(64 * NW) void k_lu_inv_reg(int n, const double2 *__restrict__ A,
+                                                        long long sA, double2 *__restrict__ Ainv,
+                                                        long long sAinv,
+                                                        const int *__restrict__ active,
+                                                        int *__restrict__ status) {
+  typedef double2 C;
+  const int b = blockIdx.x;
+  if (active && active[b] == 0) return;
+  extern __shared__ __align__(16) unsigned char lsm[];
+  C *U = (C *)lsm;  // U(r, c) at c * 64 + r (physical row r), columns 0..n-1
+  __shared__ C mcol[2][64];
+  __shared__ int s_piv[2];
+  __shared__ C s_uinv[64];         // 1 / U(k, k), k = 0..n-1
+  __shared__ int s_perm[64];
+  __shared__ C bcast[NW][2 * CT];  // per-wave broadcast of one row
+  __shared__ int s_sing;
+  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  const C *Ab = A + (long long)b * sA;
+  C a[CT], y[CT];
+#pragma unroll
+  for (int t = 0; t < CT; ++t) {
+    const int c = w * CT + t;
+    a[t] = (l < n && c < n) ? Ab[l + (size_t)c * n] : C{0, 0};
+    y[t] = C{0, 0};
+  }
+  int pos = l;
+  bool done = l >= n;
+  if (tid == 0) s_sing = 0;
+
+  // pivot of the owner's register-0 column (column col) among the unpivoted
+  // rows: the largest |Re| + |Im| (exact, as a 64-bit key), ties to the
+  // smaller current position
+  auto candidate = [&](int bb, int col) {
+    const double v = cabs1(a[0]);
+    const unsigned long long key =
+        done ? 0ull : (unsigned long long)__double_as_longlong(v) + 1ull;
+    const unsigned long long kmax = wave_max64(key);
+    const unsigned pmin = wave_min32(key == kmax ? (unsigned)pos : 0xFFFFFFFFu);
+    const unsigned long long hit = __ballot(key == kmax && (unsigned)pos == pmin);
+    const int r = __ffsll((long long)hit) - 1;
+    mcol[bb][l] = a[0];
+    if (l == r) {
+      s_piv[bb] = r;
+      s_uinv[col] = crecip_fast(a[0]);
+      if (kmax <= 1ull) s_sing = 1;  // the largest candidate is zero
+    }
+  };
+  if (w == 0) candidate(0, 0);
+  __syncthreads();
+
+  C *bc = bcast[w];
+  for (int k = 0; k < n; ++k) {
+    const int bf = k & 1, wk = k / CT;
+    const int p = s_piv[bf];
+    const C pinv = s_uinv[k];
+    // position exchange of LAPACK's row interchange k <-> position of p
+    const int pos_p = __builtin_amdgcn_readlane(pos, p);
+    if (pos == k) pos = pos_p;
+    if (l == p) { pos = k; done = true; }
+    if (tid == 0) s_perm[k] = p;
+    const C mraw = (w == wk) ? a[0] : mcol[bf][l];
+    const C m = done ? C{0, 0} : cmul(mraw, pinv);
+    if (w == wk) U[(size_t)k * 64 + l] = a[0];  // column k of U is final
+    // column k of Y = L^-1 starts with the 1 of the pivot row
+    if (l == p) {
+#pragma unroll
+      for (int t = 0; t < CT; ++t)
+        if (w * CT + t == k) y[t] = C{1.0, 0.0};
+    }
+    // waves whose A columns all lie left of k hold no live A values any more,
+    // waves whose Y columns all lie right of k have nothing to update there
+    const bool a_live = w * CT + CT - 1 > k;
+    const bool y_live = w * CT <= k;
+    // row p of this wave's columns, through LDS
+    if (l == p) {
+#pragma unroll
+      for (int t = 0; t < CT; ++t) { bc[t] = a[t]; bc[CT + t] = y[t]; }
+    }
+    __builtin_amdgcn_wave_barrier();
+    // A: new register i = old register i + 1 (column w*CT + (i+1+k) mod CT),
+    // updated when that column lies right of k; the pivot row comes from LDS
+    auto upd = [&](C x, int src) {
+      const int c = w * CT + (src + k) % CT;
+      const C mm = (c > k) ? m : C{0, 0};
+      const C yv = bc[src];
+      x.x = fma(-mm.x, yv.x, x.x);
+      x.x = fma(mm.y, yv.y, x.x);
+      x.y = fma(-mm.x, yv.y, x.y);
+      x.y = fma(-mm.y, yv.x, x.y);
+      return x;
+    };
+    if (a_live) {
+      const C a_first = a[0];
+#pragma unroll
+      for (int i = 0; i + 1 < CT; ++i) a[i] = upd(a[i + 1], i + 1);
+      a[CT - 1] = upd(a_first, 0);
+    }
+    if (k + 1 < n && w == (k + 1) / CT) candidate(bf ^ 1, k + 1);
+    if (y_live) {
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        C x = y[t];
+        const C yv = bc[CT + t];
+        x.x = fma(-m.x, yv.x, x.x);
+        x.x = fma(m.y, yv.y, x.x);
+        x.y = fma(-m.x, yv.y, x.y);
+        x.y = fma(-m.y, yv.x, x.y);
+        y[t] = x;
+      }
+    }
+    __syncthreads();
+  }
+
+  // back substitution U X = Y, position i = n-1 .. 0; X(i, :) ends in the
+  // registers of the lane whose row holds position i
+  for (int i = n - 1; i >= 0; --i) {
+    const int q = s_perm[i];
+    const C uinv = s_uinv[i];
+    const C coef = (pos < i) ? U[(size_t)i * 64 + l] : C{0, 0};
+    if (l == q) {
+#pragma unroll
+      for (int t = 0; t < CT; ++t) {
+        y[t] = cmul(y[t], uinv);
+        bc[t] = y[t];
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const C xq = bc[t];
+      C x = y[t];
+      x.x = fma(-coef.x, xq.x, x.x);
+      x.x = fma(coef.y, xq.y, x.x);
+      x.y = fma(-coef.x, xq.y, x.y);
+      x.y = fma(-coef.y, xq.x, x.y);
+      y[t] = x;  // coef = 0 on lane q and on rows at or below position i
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  // column j of X = U^-1 L^-1 is column perm(j) of the inverse
+  C *Ob = Ainv + (long long)b * sAinv;
+  if (l < n)
+#pragma unroll
+    for (int t = 0; t < CT; ++t) {
+      const int j = w * CT + t;
+      if (j < n) Ob[pos + (size_t)s_perm[j] * n] = y[t];
+    }
+  if (tid == 0 && s_sing && status) atomicOr(&status[b], (int)ST_SINGULAR);
+}
+
 
 // rows of the LDS staging buffer of k_inv_gj (a chunk of the n x 8CT panel,
 // rows padded by one element)
@@ -317,19 +519,27 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
   // the owning wave's column (register 0) -> multipliers in mcol[bb] and the
   // pivot candidate (largest key over the wave's unpivoted rows) with its
   // reciprocal in redk[bb] / redinv[bb]
+  // exact search: the largest |Re| + |Im| (its FP64 bit pattern as a 64-bit
+  // key), ties to the smallest row; the winner publishes its row as a
+  // pivot_key (decoded by every wave) and its reciprocal
   auto candidate = [&](int bb) {
-    unsigned kb = 0;
+    unsigned long long kb = 0;
+    int rb = 0x7fffffff;
+    double vb = 0.0;
     C cv = C{0, 0};
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
       const int r = l + 64 * i;
       if (r < n) mcol[bb][r] = P[i][0];
-      const unsigned key = used[i] ? 0u : pivot_key(cabs1(P[i][0]), r);
-      if (key > kb) { kb = key; cv = P[i][0]; }
+      const double v = (double)cabs1(P[i][0]);
+      const unsigned long long key =
+          used[i] ? 0ull : (unsigned long long)__double_as_longlong(v) + 1ull;
+      if (key > kb) { kb = key; rb = r; vb = v; cv = P[i][0]; }  // rows grow with i
     }
-    const unsigned km = wave_max_key(kb);
-    if (km != 0 && kb == km) {  // keys of distinct rows differ: one lane
-      redk[bb] = km;
+    const unsigned long long km = wave_max64(kb);
+    const unsigned rmin = wave_min32(kb == km ? (unsigned)rb : 0xFFFFFFFFu);
+    if (km != 0 && kb == km && (unsigned)rb == rmin) {
+      redk[bb] = pivot_key(vb, rb);
       redinv[bb] = crecip_fast(cv);
     }
     if (km == 0 && l == 0) redk[bb] = 0;
@@ -517,6 +727,27 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
               long long sAinv, int batch, const int *active, int *status, const InvWork<R> &w,
               hipStream_t s) {
   if (n <= 0 || batch <= 0) return;
+  if constexpr (sizeof(R) == 8) {
+    if (n <= 64) {
+      ProfScope ps(PT_INV_SMALL, -1.0, 0.0, s);
+      const size_t lds = (size_t)n * 64 * sizeof(double2);
+      if (n <= 16) {
+        allow_lds(k_lu_inv_reg<1, 16>, lds);
+        hipLaunchKernelGGL((k_lu_inv_reg<1, 16>), dim3(batch), dim3(64), lds, s, n, A, sA, Ainv,
+                           sAinv, active, status);
+      } else if (n <= 32) {
+        allow_lds(k_lu_inv_reg<2, 16>, lds);
+        hipLaunchKernelGGL((k_lu_inv_reg<2, 16>), dim3(batch), dim3(128), lds, s, n, A, sA, Ainv,
+                           sAinv, active, status);
+      } else {
+        allow_lds(k_lu_inv_reg<4, 16>, lds);
+        hipLaunchKernelGGL((k_lu_inv_reg<4, 16>), dim3(batch), dim3(256), lds, s, n, A, sA, Ainv,
+                           sAinv, active, status);
+      }
+      HN_CHECK(hipGetLastError());
+      return;
+    }
+  }
   if (n <= lds_max_n<R>()) {
     size_t lds = lds_bytes_small<R>(n);
     allow_lds(k_inv_lds<R>, lds);

@@ -61,13 +61,18 @@ int or_set_backend(int which) {
   if (which == 1) {
     if (!load_mkl()) return -1;
     g_backend = 1;
+  } else if (which == 2) {
+    g_backend = 2;
   } else {
     g_backend = 0;
   }
   return g_backend;
 }
 int or_get_backend(void) { return g_backend; }
-const char *or_backend_name(void) { return g_backend == 1 ? "MKL (libmkl_rt, sequential)" : "built-in"; }
+const char *or_backend_name(void) {
+  return g_backend == 1 ? "MKL (libmkl_rt, sequential)"
+                        : g_backend == 2 ? "built-in, Gauss-Jordan inverse" : "built-in";
+}
 
 static inline zc opA(char t, const zc *A, int lda, int i, int l) {
   /* element (i,l) of op(A) */
@@ -112,8 +117,49 @@ static inline double cabs1(zc z) { return fabs(creal(z)) + fabs(cimag(z)); }
 
 /* ZGETRF (partial pivoting, pivot = first max of |Re|+|Im| as IZAMAX) +
  * ZGETRS against the identity: zinv, inversions.F90:807-839. */
+/* Accuracy experiments only (backend 2): the inverse by in-place Gauss-Jordan
+ * elimination with partial pivoting on the transpose (rows of A^T, i.e.
+ * columns of A, chosen by the largest |Re|+|Im|, ties to the smaller index),
+ * the scheme of the GPU's blocked inverse; the dense products stay built-in. */
+static int zinv_gj_transposed(int n, const zc *A, zc *Ainv) {
+  zc *M = (zc *)malloc(sizeof(zc) * (size_t)n * n);
+  int *pv = (int *)malloc(sizeof(int) * n), *used = (int *)calloc(n, sizeof(int));
+  int info = 0;
+  for (int j = 0; j < n; ++j)
+    for (int i = 0; i < n; ++i) M[i + (size_t)j * n] = A[j + (size_t)i * n]; /* M = A^T */
+  for (int j = 0; j < n; ++j) {
+    int p = -1;
+    double best = -1.0;
+    for (int r = 0; r < n; ++r)
+      if (!used[r] && cabs1(M[r + (size_t)j * n]) > best) { best = cabs1(M[r + (size_t)j * n]); p = r; }
+    used[p] = 1;
+    pv[j] = p;
+    if (best == 0.0 && !info) info = j + 1;
+    const zc inv = best == 0.0 ? 0.0 : 1.0 / M[p + (size_t)j * n];
+    for (int c = 0; c < n; ++c) M[p + (size_t)c * n] *= inv;
+    M[p + (size_t)j * n] = inv;
+    for (int r = 0; r < n; ++r) {
+      if (r == p) continue;
+      const zc f = M[r + (size_t)j * n];
+      if (f == 0.0) continue;
+      for (int c = 0; c < n; ++c)
+        if (c != j) M[r + (size_t)c * n] -= f * M[p + (size_t)c * n];
+      M[r + (size_t)j * n] = -f * inv;
+    }
+  }
+  /* M holds inv(A^T) with rows in pivot order: inv(A^T)(i, c) = M(pv[i], pvinv[c]) */
+  int *ip = (int *)malloc(sizeof(int) * n);
+  for (int c = 0; c < n; ++c) ip[pv[c]] = c;
+  for (int c = 0; c < n; ++c)
+    for (int i = 0; i < n; ++i) /* Ainv = inv(A^T)^T */
+      Ainv[c + (size_t)i * n] = M[pv[i] + (size_t)ip[c] * n];
+  free(M); free(pv); free(used); free(ip);
+  return info;
+}
+
 int or_zinv(int n, const zc *A, zc *Ainv) {
   if (n <= 0) return 0;
+  if (g_backend == 2) return zinv_gj_transposed(n, A, Ainv);
   zc *LU = (zc *)malloc(sizeof(zc) * (size_t)n * n);
   int *ipiv = (int *)malloc(sizeof(int) * (size_t)n);
   memcpy(LU, A, sizeof(zc) * (size_t)n * n);

@@ -113,8 +113,10 @@ def lib():
 
 
 def set_backend(which: str) -> str:
-    """'builtin' or 'mkl'; returns the backend name actually in use."""
-    r = lib().or_set_backend(1 if which == "mkl" else 0)
+    """'builtin', 'mkl', or 'gj' (built-in products with the Gauss-Jordan
+    inverse of the GPU's blocked path, for accuracy experiments); returns the
+    backend name actually in use."""
+    r = lib().or_set_backend({"mkl": 1, "gj": 2}.get(which, 0))
     if r < 0:
         raise RuntimeError("MKL backend requested but libmkl_rt.so could not be loaded")
     return lib().or_backend_name().decode()

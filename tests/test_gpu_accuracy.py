@@ -17,8 +17,13 @@ therefore:
 
 * where the reference path is accurate (its error below 1e-11), the GPU is
   within 1e-10 relative -- the 1e-10 target, everywhere it is attainable;
-* binned by the reference path's error, the GPU's RMS error in every bin is
-  within RATIO of the reference path's RMS error (both backends pooled);
+* over the ill-conditioned points (the reference path's error above 1e-10)
+  the GPU's error per point, relative to the geometric mean of the two
+  backends' errors there, has a mean log10 below LOG_MEAN and a 90th
+  percentile below LOG_P90 -- the spread MKL and the built-in loops show
+  against each other (on C2 their log10 ratio has mean 0.02 and 90th
+  percentile 0.89; on C3 0.58 and 1.17);
+* the GPU's largest error is within MAX_RATIO of the reference path's;
 * the same decimation iteration counts everywhere.
 """
 import os
@@ -29,31 +34,31 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-RATIO = 3.0
-BINS = (0.0, 1e-12, 1e-10, 1e-8, 1e-6, 1.0)
+LOG_MEAN = 0.35
+LOG_P90 = 1.0
+MAX_RATIO = 10.0
 
 
 def _check(T, fx, label):
     Tt = fx["T_hi"] + fx["T_lo"]
     big = Tt > 1e-6 * Tt.max()
     e_gpu = np.abs(T - Tt) / Tt
-    cpu = [np.abs(fx["T_builtin"] - Tt) / Tt]
-    if np.isfinite(fx["T_mkl"]).all():
-        cpu.append(np.abs(fx["T_mkl"] - Tt) / Tt)
-    e_cpu = np.max(cpu, axis=0)
+    e_b = np.abs(fx["T_builtin"] - Tt) / Tt
+    e_m = np.abs(fx["T_mkl"] - Tt) / Tt if np.isfinite(fx["T_mkl"]).all() else e_b
+    e_cpu = np.maximum(e_b, e_m)
     good = big & (e_cpu < 1e-11)
     assert good.sum() >= (0.5 * big.sum() if label == "c2" else 4)
     assert e_gpu[good].max() <= 1e-10, (label, e_gpu[good].max())
-    report = []
-    for lo, hi in zip(BINS[:-1], BINS[1:]):
-        sel = big & (e_cpu >= lo) & (e_cpu < hi)
-        if sel.sum() == 0:
-            continue
-        rg = np.sqrt((e_gpu[sel] ** 2).mean())
-        rc = np.sqrt(np.mean([(e[sel] ** 2).mean() for e in cpu]))
-        report.append((lo, hi, int(sel.sum()), rg, rc))
-        assert rg <= RATIO * rc + 1e-15, (label, lo, hi, rg, rc)
-    return report
+    ill = big & (e_cpu >= 1e-10)
+    geo = np.sqrt(np.maximum(e_b[ill], 1e-17) * np.maximum(e_m[ill], 1e-17))
+    lr = np.log10(np.maximum(e_gpu[ill], 1e-17) / geo)
+    stats = dict(n_good=int(good.sum()), n_ill=int(ill.sum()), log_mean=float(lr.mean()),
+                 log_p90=float(np.quantile(lr, 0.9)), max_gpu=float(e_gpu[big].max()),
+                 max_cpu=float(e_cpu[big].max()))
+    assert stats["log_mean"] <= LOG_MEAN, (label, stats)
+    assert stats["log_p90"] <= LOG_P90, (label, stats)
+    assert stats["max_gpu"] <= MAX_RATIO * stats["max_cpu"], (label, stats)
+    return stats
 
 
 def test_c2_against_double_double():

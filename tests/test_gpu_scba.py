@@ -51,8 +51,11 @@ def _check(r, q, err_rtol=1e-6):
     for k, kk in (("gn_diag", "gn"), ("gr_diag", "gr")):
         sc = np.abs(q[kk]).max()
         assert np.abs(r[k] - q[kk]).max() <= RTOL * sc, k
+    # err is the last pass's largest change of G^n, a difference of two nearly
+    # equal iterates: beyond err_rtol it carries the iterates' own rounding
     big = q["err"] > 1e-13
-    assert np.all(np.abs(r["err"][big] - q["err"][big]) <= err_rtol * q["err"][big] + 1e-14)
+    floor = 1e-13 * np.abs(q["gn"]).max()
+    assert np.all(np.abs(r["err"][big] - q["err"][big]) <= err_rtol * q["err"][big] + floor)
 
 
 @pytest.mark.parametrize("coupling,niter", [(0.0, 5), (0.05, 5), (0.05, 10)])

@@ -35,12 +35,30 @@ def _oracle(s, E, eta, threads=0):
     return T, ncyc
 
 
-def _compare(T, Tref, rtol=RTOL):
+def _compare(T, Tref, rtol=RTOL, spread=None):
+    """|T - Tref| <= rtol relative; where the reference path's own FP64
+    rounding (spread: |T_builtin - T_mkl| / T per point) exceeds rtol, the bar
+    is four times that spread instead."""
+    bar = np.full(len(Tref), rtol) if spread is None else np.maximum(rtol, 4.0 * spread)
     big = Tref > 1e-6 * Tref.max()
     rel = np.abs(T[big] - Tref[big]) / Tref[big]
-    assert rel.max() <= rtol, (rel.max(), np.argmax(rel))
+    assert (rel <= bar[big]).all(), (rel.max(), np.argmax(rel / bar[big]))
     assert np.abs(T[~big] - Tref[~big]).max(initial=0) <= rtol * Tref.max()
     return rel.max()
+
+
+def _spread(s, E, eta, ni=1, nf=2):
+    """Per-point relative difference of the oracle's two dense backends (the
+    FP64 noise floor of the reference path at each energy), or None without MKL."""
+    from oracle import pyoracle
+    try:
+        pyoracle.set_backend("mkl")
+    except RuntimeError:
+        return None
+    Tm, _, _ = pyoracle.transmission_batch(s, np.asarray(E), eta, ni=ni, nf=nf)
+    pyoracle.set_backend("builtin")
+    Tb, _, _ = pyoracle.transmission_batch(s, np.asarray(E), eta, ni=ni, nf=nf)
+    return np.abs(Tb - Tm) / np.maximum(np.abs(Tm), 1e-300)
 
 
 def test_c1_full_grid(gpu):
@@ -184,8 +202,10 @@ def test_three_terminal_pairs(gpu):
     pyoracle.build()
     pyoracle.set_backend("builtin")
     for j, (a, b) in enumerate(pairs):
+        sp = _spread(s, E, 1e-5, ni=a, nf=b)   # up to 2.6e-10 near E = +-0.65
+        pyoracle.set_backend("builtin")
         Tref, nref, _ = pyoracle.transmission_batch(s, E, 1e-5, ni=a, nf=b)
-        _compare(T[:, j], Tref)
+        _compare(T[:, j], Tref, spread=sp)
         assert (ncyc == nref).all()
     # a single pair through the single-pair entry point takes the same path
     T13, _, _ = hip.solve_transmission(eng, ds, E, delta=1e-5, ni=1, nf=3)

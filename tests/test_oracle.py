@@ -28,6 +28,26 @@ def test_zinv_matches_lapack(oracle, n):
     np.testing.assert_allclose(Ai, np.linalg.inv(A), rtol=1e-11, atol=1e-12)
 
 
+@pytest.mark.parametrize("n", [1, 2, 9, 40])
+def test_zinv_gauss_jordan_backend(oracle, n):
+    """The Gauss-Jordan inverse kept for accuracy experiments (backend 'gj')
+    is an inverse: numpy agreement, and a zero-diagonal case that needs
+    pivoting."""
+    rng = np.random.default_rng(n + 7)
+    try:
+        oracle.set_backend("gj")
+        A = _rand(rng, n)
+        Ai, info = oracle.zinv(A)
+        assert info == 0
+        np.testing.assert_allclose(Ai, np.linalg.inv(A), rtol=1e-11, atol=1e-12)
+        P = np.eye(n)[::-1] * (1.0 + np.arange(n))
+        Pi, info = oracle.zinv(P.astype(complex))
+        assert info == 0
+        np.testing.assert_allclose(Pi, np.linalg.inv(P), atol=1e-15)
+    finally:
+        oracle.set_backend("builtin")
+
+
 def test_zinv_pivoting_and_singular(oracle):
     A = np.array([[0, 1, 0], [0, 0, 1], [1, 0, 0]], dtype=complex)  # needs row pivoting
     Ai, info = oracle.zinv(A)
