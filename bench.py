@@ -33,7 +33,12 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--items", type=int, default=0, help="items per step (0: config default)")
     ap.add_argument("--nbl", type=int, default=0, help="override the number of layers")
-    ap.add_argument("--cpu-items", type=int, default=48)
+    ap.add_argument("--cpu-items", type=int, default=48,
+                    help="CPU baseline sample: spread over up to 8 timed steps (k-points)")
+    ap.add_argument("--mixed-items", type=int, default=500,
+                    help="items of the mixed-precision leg (one step, after the FP64 timing; 0: off)")
+    ap.add_argument("--mixed-ns", type=int, default=2,
+                    help="FP64 Newton-Schulz steps per block inverse of the mixed leg")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--plain-items", type=int, default=4,
                     help="items timed one at a time (unbatched FP64 reference mode)")
@@ -47,6 +52,8 @@ def parse():
                     help="no per-launch HIP events in the timed region (no roofline field)")
     ap.add_argument("--copy-calib", action="store_true",
                     help="only run a plain 256 MiB device copy (PMC byte-counter calibration)")
+    ap.add_argument("--mfma-calib", action="store_true",
+                    help="only run register-only FP64 MFMA issue loops (PMC utilisation calibration)")
     ap.add_argument("--serve", action="store_true",
                     help="serving workload: Poisson request stream through the server loop")
     ap.add_argument("--requests", type=int, default=24, help="requests per serving pass")
@@ -102,6 +109,45 @@ def kernel_label(tag_name: str, nmax: int) -> str:
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def truth_parity(results, last, kidx, nsteps):
+    """GPU, MKL and built-in-loop errors against the double-double truth at the
+    points of tests/golden/dd_c3.npz whose k-points this run solved (warm-up
+    steps and the last timed step), and the log10 ratio of the GPU's error to
+    the geometric mean of the two CPU backends' over the ill-conditioned ones."""
+    f = os.path.join(ROOT, "tests", "golden", "dd_c3.npz")
+    if not os.path.exists(f):
+        return {}
+    fx = np.load(f)
+    have = {kidx[i]: results[i][0] for i in results}
+    have[kidx[nsteps - 1]] = last[0]
+    rows = [j for j, k in enumerate(fx["k"]) if int(k) in have]
+    if not rows:
+        return {}
+    Tt = fx["T_hi"][rows] + fx["T_lo"][rows]
+    Tg = np.array([have[int(fx["k"][j])][int(fx["idx"][j])] for j in rows])
+    eg = np.abs(Tg - Tt) / Tt
+    em = np.abs(fx["T_mkl"][rows] - Tt) / Tt
+    eb = np.abs(fx["T_builtin"][rows] - Tt) / Tt
+    ill = np.maximum(em, eb) >= 1e-10
+    lr = np.log10(np.maximum(eg[ill], 1e-17) / np.sqrt(np.maximum(em[ill], 1e-17) * np.maximum(eb[ill], 1e-17)))
+    return dict(truth_points=len(rows), truth_kpoints=sorted({int(fx["k"][j]) for j in rows}),
+                max_rel_err_vs_truth_gpu=float(eg.max()), max_rel_err_vs_truth_mkl=float(em.max()),
+                max_rel_err_vs_truth_builtin=float(eb.max()),
+                median_rel_err_vs_truth_gpu=float(np.median(eg)),
+                median_rel_err_vs_truth_mkl=float(np.median(em)),
+                gpu_over_cpu_log10_mean_ill=float(lr.mean()) if len(lr) else None)
 
 
 def fermi(E, mu, kT):
@@ -396,6 +442,16 @@ def main():
     if a.copy_calib:
         copy_calibration()
         return
+    if a.mfma_calib:
+        import ctypes as C
+        L = hip.load()
+        r = {}
+        for form, name, iters in ((0, "v_mfma_f64_4x4x4", 2000), (1, "v_mfma_f64_16x16x4", 500)):
+            tf = C.c_double()
+            hip.check(L.hipnegf_calib_mfma(form, iters, 2048, C.byref(tf)))
+            r[name] = tf.value
+        print(json.dumps({"mfma_issue_tflops": r, "peak_fp64_tflops": PEAK_FP64_TFLOPS}), flush=True)
+        return
     prec = hip.FP64 if a.precision == "fp64" else hip.MIXED
     if a.config == "c4":
         bench_c4(a, hip, wl, torch)
@@ -523,6 +579,45 @@ def main():
         plain = a.plain_items / (time.perf_counter() - t1)
         log(f"plain (one item per launch) FP64: {plain:.3f} points/s")
 
+    # ------------------------------------------------- mixed precision leg ----
+    mixed = None
+    if a.mixed_items and prec == hip.FP64 and a.config == "c3":
+        i_last = nsteps - 1
+        nm = min(a.mixed_items, per_step)
+        Tmx = torch.zeros(nm, dtype=torch.float64, device=dev)
+        Nmx = torch.zeros(nm * ncont, dtype=torch.int32, device=dev)
+        Smx = torch.zeros(nm, dtype=torch.int32, device=dev)
+        mo = hip._opts(eta, 1, 2, hip.MIXED, 0, ns_steps=a.mixed_ns, lanes=a.lanes)
+
+        def mstep():
+            hip.check(L.hipnegf_solve_transmission_device(
+                eng.handle, ds.handle, nm, C.c_void_p(dE[i_last].data_ptr()),
+                C.c_void_p(dS[i_last].data_ptr()), C.byref(mo), C.c_void_p(Tmx.data_ptr()),
+                C.c_void_p(Nmx.data_ptr()), C.c_void_p(Smx.data_ptr())))
+        mstep()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        mstep()
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t1
+        Tm = Tmx.cpu().numpy()
+        T64 = last[0][:nm]
+        big = T64 > 1e-6 * T64.max()
+        mixed = dict(points_per_s=nm / secs, speedup_vs_fp64=(nm / secs) / value,
+                     ns_steps=a.mixed_ns, items=nm,
+                     max_rel_err_vs_fp64_gpu=float((np.abs(Tm[big] - T64[big]) / T64[big]).max()),
+                     status_nonzero=int((Smx.cpu().numpy() != 0).sum()))
+        f = os.path.join(ROOT, "tests", "golden", "dd_c3.npz")
+        if os.path.exists(f) and s0.nbl == 40 and per_step == 500:
+            fx = np.load(f)
+            rows = [j for j, k in enumerate(fx["k"]) if int(k) == kidx[i_last] and fx["idx"][j] < nm]
+            if rows:
+                Tt = fx["T_hi"][rows] + fx["T_lo"][rows]
+                mixed["max_rel_err_vs_truth"] = float(
+                    (np.abs(Tm[fx["idx"][rows]] - Tt) / Tt).max())
+                mixed["truth_points"] = len(rows)
+        log(f"mixed precision (FP32 sweep + {a.mixed_ns} FP64 Newton-Schulz steps): {mixed}")
+
     # --------------------------------------------------------- CPU baseline ----
     cpu = None
     parity = None
@@ -534,21 +629,44 @@ def main():
             backend = pyoracle.set_backend("mkl")
         except RuntimeError:
             backend = pyoracle.set_backend("builtin")
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1
-        threads = min(threads, 16)
-        i_last = nsteps - 1
-        sel = np.linspace(0, per_step - 1, min(a.cpu_items, per_step)).astype(int)
-        s_last = structs[steps_slot[i_last][0]]
-        Tc, nc, secs = pyoracle.transmission_batch(s_last, steps_E[i_last][sel], eta, nthreads=threads)
-        cpu = dict(value=len(sel) / secs, unit="energy-points/s", cores=threads, kind=kind,
-                   sample=f"{len(sel)} (E,k) points of the last timed step (one k-point), "
-                          f"oracle with {backend}, OpenMP over points, 1 BLAS thread each",
+        # every host core this job has (the box grants a 16-core share per GPU)
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16)
+        # the sample: the same number of energies from each of up to 8 timed
+        # steps (distinct k-points for the slab), spread over the energy grid
+        timed = list(range(a.warmup, nsteps))
+        picks = [timed[int(round(x))] for x in np.linspace(0, len(timed) - 1, min(8, len(timed)))]
+        picks = sorted(set(picks))
+        per = max(1, a.cpu_items // len(picks))
+        Tc_all, Tg_all, nc_ok, secs = [], [], True, 0.0
+        for j, i_st in enumerate(picks):
+            sel = np.linspace(j % 2, per_step - 1 - (j + 1) % 2, per).astype(int)
+            s_i = structs[steps_slot[i_st][0]]
+            Tc, nc, sec = pyoracle.transmission_batch(s_i, steps_E[i_st][sel], eta, nthreads=threads)
+            secs += sec
+            if i_st == nsteps - 1:
+                Tg, Ng = last[0][sel], last[1][sel]
+            else:   # GPU values of this step, solved again outside the timed region
+                Tg, Ng, _ = hip.solve_transmission(eng, ds, steps_E[i_st][sel],
+                                                   slot=steps_slot[i_st][sel], delta=eta)
+            Tc_all.append(Tc)
+            Tg_all.append(Tg)
+            nc_ok = nc_ok and bool((Ng == nc).all())
+        Tc, Tg = np.concatenate(Tc_all), np.concatenate(Tg_all)
+        npts = len(Tc)
+        cpu = dict(value=npts / secs, unit="energy-points/s", cores=threads, kind=kind,
+                   sample=f"{npts} (E,k) points: {per} energies from each of {len(picks)} timed steps "
+                          f"({'k-points' if a.config == 'c3' else 'steps'} "
+                          f"{[kidx[i] for i in picks] if a.config == 'c3' else picks}), "
+                          f"oracle with {backend}, OpenMP over points, 1 BLAS thread each; "
+                          f"host CPU: {cpu_model()}",
                    seconds=secs)
-        Tg = last[0][sel]
         big = Tc > 1e-6 * Tc.max()
-        parity = dict(max_rel_err=float((np.abs(Tg[big] - Tc[big]) / Tc[big]).max()),
-                      ncyc_equal=bool((last[1][sel] == nc).all()), points=int(len(sel)))
+        parity = dict(max_rel_err_vs_mkl=float((np.abs(Tg[big] - Tc[big]) / Tc[big]).max()),
+                      ncyc_equal=nc_ok, points=int(npts))
         log(f"cpu baseline: {cpu['value']:.3f} points/s on {threads} threads; parity {parity}")
+    if a.config == "c3" and s0.nbl == 40 and per_step == 500:
+        parity = dict(parity or {}, **truth_parity(results, last, kidx, nsteps))
+        log(f"parity against the double-double truth: {parity}")
 
     out = {
         "metric": "RGF energy-points/s",
@@ -570,6 +688,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "parity": parity,
+        "mixed": mixed,
         "kernel_ms": None if kern is None else {k: round(v["ms"], 3) for k, v in kern.items()},
     }
     print(json.dumps(out), flush=True)

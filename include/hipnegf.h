@@ -406,6 +406,12 @@ int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious
 int hipnegf_profile_enable(int on);
 int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long long *launches);
 int hipnegf_profile_clear(void);
+/* FP64 matrix-core issue loop with register operands only (no counterpart in
+ * the reference; calibration of hardware counters): `blocks` workgroups of
+ * four waves, each running eight independent chains of `iters` instructions
+ * of form 0 (v_mfma_f64_4x4x4) or 1 (v_mfma_f64_16x16x4); *tflops = the
+ * achieved rate of the second of two launches. */
+int hipnegf_calib_mfma(int form, int iters, int blocks, double *tflops);
 
 #ifdef __cplusplus
 }

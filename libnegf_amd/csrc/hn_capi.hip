@@ -876,6 +876,13 @@ int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long
 int hipnegf_profile_clear(void) {
   return guard([&] { hn::profiler().clear(); return 0; });
 }
+int hipnegf_calib_mfma(int form, int iters, int blocks, double *tflops) {
+  return guard([&] {
+    if (form < 0 || form > 1 || iters < 1 || blocks < 1) throw std::runtime_error("bad arguments");
+    *tflops = hn::mfma_issue_rate(form, iters, blocks);
+    return 0;
+  });
+}
 
 struct hipnegf_server_s {
   std::unique_ptr<hn::Server> srv;

@@ -56,4 +56,8 @@ struct ProfScope {
   }
 };
 
+// FP64 matrix-core issue loop (register operands only) on the current device:
+// form 0 v_mfma_f64_4x4x4, 1 v_mfma_f64_16x16x4; returns TFLOP/s.
+double mfma_issue_rate(int form, int iters, int blocks);
+
 }  // namespace hn

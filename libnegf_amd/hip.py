@@ -118,6 +118,7 @@ def load():
     L.hipnegf_engine_sgf_stats.argtypes = [vp, C.POINTER(ll), C.POINTER(ll)]
     L.hipnegf_struct_set_fictitious.argtypes = [vp, i, i, d]
     L.hipnegf_profile_enable.argtypes = [i]
+    L.hipnegf_calib_mfma.argtypes = [i, i, i, C.POINTER(d)]
     L.hipnegf_profile_read.argtypes = [i, C.POINTER(d), C.POINTER(d), C.POINTER(d),
                                        C.POINTER(ll)]
     _LIB = L
