@@ -230,7 +230,12 @@ enum hipnegf_opt_flags {
    * cached g_s when every item of a batch is cached, else compute and store;
    * loaded items report ncyc 0), STORE = ReadOldSGF 2 (compute and store). */
   HIPNEGF_SGF_LOAD = 4,
-  HIPNEGF_SGF_STORE = 8
+  HIPNEGF_SGF_STORE = 8,
+  /* two-contact transmission with blocks of at most 16 orbitals (every block
+   * the same size, no surface block, no tun_proj mask, FP64, no g_s cache)
+   * runs as one launch per call, the whole solve of an item in one wave;
+   * this flag keeps such calls on the batched block-operation path */
+  HIPNEGF_NO_FUSED = 16
 };
 
 /* Coherent transmission T(E_i, slot_i), i < nitems (iterative.F90:1669-1780
@@ -399,7 +404,8 @@ int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious
 /* Launch timing (no counterpart in the reference; measurement hook for
  * benchmarks).  While enabled, every tagged launch records a HIP event pair on
  * its stream.  Tags: 0 masked GEMMs (decimation, SCBA), 1 RGF-sweep GEMMs,
- * 2 blocked-inverse trailing updates, 3 inverse panels, 4 small LDS inverses.
+ * 2 blocked-inverse trailing updates, 3 inverse panels, 4 small-block inverses,
+ * 5 fused small-block transmission solves.
  * hipnegf_profile_read waits for the events of `tag` and returns the summed
  * duration (ms), algorithmic flops (-1 if any launch is not counted) and
  * algorithmic bytes, and the launch count. */

@@ -30,7 +30,7 @@ CFLAGS = [
     f"-I{INCLUDE}",
 ]
 
-SOURCES = ["hn_prof.hip", "hn_gemm.hip", "hn_inv.hip", "hn_kernels.hip", "hn_engine.hip",
+SOURCES = ["hn_prof.hip", "hn_gemm.hip", "hn_inv.hip", "hn_kernels.hip", "hn_engine.hip", "hn_fused.hip",
            "hn_gen.hip", "hn_serve.hip", "hn_capi.hip", "hn_api.hip"]
 HEADERS = ["hn_common.h", "hn_prof.h", "hn_gemm.h", "hn_inv.h", "hn_kernels.h", "hn_engine.h",
            "hn_gen.h", "hn_serve.h"]

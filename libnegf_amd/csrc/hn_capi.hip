@@ -312,6 +312,7 @@ hn::SolveOpts to_opts(const hipnegf_opts *o) {
     so.eps_drop = o->eps_drop;
     so.share_mirror = (o->flags & HIPNEGF_NO_SHARED_DECIMATION) ? 0 : 1;
     so.graph = (o->flags & HIPNEGF_GRAPH) ? 1 : 0;
+    so.fused = (o->flags & HIPNEGF_NO_FUSED) ? 0 : 1;
     so.sgf_mode = (o->flags & HIPNEGF_SGF_LOAD) ? 1 : (o->flags & HIPNEGF_SGF_STORE) ? 2 : 0;
     so.lanes = o->lanes <= 0 ? 0 : std::min(o->lanes, 8);
   }

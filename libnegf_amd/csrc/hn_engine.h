@@ -85,6 +85,8 @@ struct DevStruct {
 
 enum Precision : int { PREC_FP64 = 0, PREC_MIXED = 1 };
 
+constexpr int kFusedMaxN = 16;  // largest block of the fused transmission kernel
+
 // Device arena for engine workspaces (grown on demand, never shrunk).
 struct Arena {
   char *base = nullptr;
@@ -116,6 +118,7 @@ struct SolveOpts {
   int lanes = 0;             // item groups solved concurrently, each on its own stream;
                              // 0: automatic (2 for >= 256 items of blocks >= 128, else 1)
   int graph = 0;             // replay the RGF sweep of a batch from a captured hipGraph
+  int fused = 1;             // small uniform blocks: whole solves in one launch (hn_fused.hip)
   // surface Green's function cache (surface_green + ln_cache, contselfenergy.F90:138-173,
   // 251-259): 0 compute only; 1 load if cached, else compute and store; 2 compute and store
   int sgf_mode = 0;
@@ -278,6 +281,10 @@ class Engine {
   void clear_graphs();
 
   int pick_batch(const DevStruct &st, int nitems, const SolveOpts &o, size_t per_item) const;
+  // whole two-contact transmission solves in one launch for blocks of at most
+  // kFusedMaxN orbitals (hn_fused.hip); false when the call is not eligible
+  bool t_fused(DevStruct &st, int nitems, const double *dE, const int *dslot, const SolveOpts &o,
+               int ni, int nf, double *dT, int *dncyc, int *dstatus);
   // GSout (optional): the surface Green's functions, contact-major, stride sGS
   void contacts(DevStruct &st, int B, const double *dE, const int *dslot,
                 const SolveOpts &o, double2 *Sig, long long sSig, int *ncyc_c, int *status_c,

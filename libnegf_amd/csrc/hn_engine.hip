@@ -1826,6 +1826,8 @@ void Engine::transmission_pairs(DevStruct &st, int nitems, const double *dE, con
     if (ni[p] < 1 || ni[p] > st.ncont || nf[p] < 1 || nf[p] > st.ncont || ni[p] == nf[p])
       if (st.ncont >= 2) throw std::runtime_error("transmission: bad contact pair");
   HN_CHECK(hipSetDevice(device_));
+  if (npairs == 1 && o.fused && !o.graph && t_fused(st, nitems, dE, dslot, o, ni[0], nf[0], dT, dncyc, dstatus))
+    return;
   // Two item groups on two streams measured 7 % faster on the C3 slab (n = 256)
   // than one; three or more were slower.
   const int lanes = o.lanes > 0 ? o.lanes : (nitems >= 256 && st.nmax() >= 128) ? 2 : 1;

@@ -17,8 +17,9 @@ enum ProfTag : int {
   PT_GEMM_SWEEP = 1,   // batched GEMMs of the RGF sweeps and traces
   PT_GEMM_INV = 2,     // trailing updates of the blocked inverse
   PT_INV_PANEL = 3,    // panel factorisation of the blocked inverse
-  PT_INV_SMALL = 4,    // LDS-resident small-block inverse
-  PT_COUNT = 5
+  PT_INV_SMALL = 4,    // register- or LDS-resident small-block inverse
+  PT_FUSED = 5,        // whole small-block transmission solves in one launch
+  PT_COUNT = 6
 };
 
 class Profiler {

@@ -39,6 +39,7 @@ NO_SHARED_DECIMATION = 1
 GRAPH = 2
 SGF_LOAD = 4
 SGF_STORE = 8
+NO_FUSED = 16
 
 
 def load():
@@ -133,12 +134,12 @@ def check(rc: int):
 
 
 # launch-timing tags (hipnegf_profile_*)
-PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV, PT_INV_PANEL, PT_INV_SMALL = range(5)
+PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV, PT_INV_PANEL, PT_INV_SMALL, PT_FUSED = range(6)
 # Kernel serving each tag for FP64 blocks with K, N >= 128 (the 64 x 128 tiling);
 # smaller GEMMs of a tag run on k_gemm<double, tag>.
 PT_NAMES = {PT_GEMM_MASKED: "k_gemm_w<0>", PT_GEMM_SWEEP: "k_gemm_w<1>",
             PT_GEMM_INV: "k_gemm_w<2>", PT_INV_PANEL: "k_inv_gj",
-            PT_INV_SMALL: "k_inv_lds"}
+            PT_INV_SMALL: "k_lu_inv_reg", PT_FUSED: "k_t_fused"}
 
 
 def profile_enable(on: bool):
