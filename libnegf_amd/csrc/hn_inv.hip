@@ -17,6 +17,8 @@
 // accumulated transform; the other columns of M then take ONE deferred
 // rank-|S| update on the matrix cores.  After the last super-panel,
 // M^-1(i, c) = B(pi(i), pi^-1(c)) with pi(c) the pivot row of column c.
+#include <utility>
+
 #include "hn_gemm.h"
 #include "hn_inv.h"
 #include "hn_prof.h"
@@ -187,39 +189,6 @@ __device__ inline float lane_read(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 
-// ---------------------------------------------------------------------------
-// register-resident LU inverse, n <= 64 (complex double)
-// ---------------------------------------------------------------------------
-// The reference's algorithm: ZGETRF (partial pivoting; the pivot is the first
-// largest |Re| + |Im| of the column in the current row order, as IZAMAX
-// picks it) followed by ZGETRS against the identity (inversions.F90:807-839).
-//
-// One workgroup of NW waves per item.  Lane l holds physical row l of A and
-// of Y = L^-1 (the forward substitution of ZGETRS against P I; column j of Y
-// belongs to the row pivoted at step j, so Y is lower triangular in pivot
-// order); wave w holds columns w*CT .. w*CT+CT-1 of both in registers.  Rows
-// are never moved: pos (per lane) is the row's current position in LAPACK's
-// row order, and a pivot step exchanges two positions.
-//
-// Step k (one barrier): every lane reads the pivot row p and 1/A(p,k)
-// published by the wave owning column k, and its own entry A(r,k) from LDS;
-// each wave reads row p of its columns through a per-wave LDS slot, and
-// every unpivoted row applies row_r -= (A(r,k)/A(p,k)) row_p to its A
-// columns right of k and to its Y columns up to k -- the elimination of
-// ZGETRF and the forward substitution of ZGETRS in one pass.  Waves whose A
-// columns all lie left of k, or whose Y columns all lie right of it, skip
-// that half.  Every step writes each updated A value one register lower, so
-// the wave owning column k always finds it in register 0; the owner of
-// column k+1 searches its pivot before the Y update.  Column k of U goes to
-// LDS as soon as it is final.  The pivot search is exact: the largest
-// |Re| + |Im| as a 64-bit key, then the smallest position among the lanes
-// holding it.
-//
-// Then each wave back-substitutes U X = Y on its Y columns on its own (U and
-// the pivot reciprocals from LDS, no barriers); row pos(l) of X is in lane l,
-// and column j of X is column perm(j) of the inverse (the physical row
-// pivoted at step j).
-
 // 64-bit key helpers for the exact pivot search: DPP butterflies inside each
 // row of 16 lanes on both halves, then the four row results read as scalars.
 template <int CTRL> __device__ inline unsigned long long dpp64(unsigned long long k) {
@@ -252,154 +221,269 @@ __device__ inline unsigned wave_min32(unsigned k) {
   return min(min(a, b), min(c, d));
 }
 
-template <int NW, int CT>
-__global__ __launch_bounds__(64 * NW) void k_lu_inv_reg(int n, const double2 *__restrict__ A,
-                                                        long long sA, double2 *__restrict__ Ainv,
-                                                        long long sAinv,
-                                                        const int *__restrict__ active,
-                                                        int *__restrict__ status) {
+// ---------------------------------------------------------------------------
+// LU inverse of blocks up to 64 (complex double), one workgroup per block
+// ---------------------------------------------------------------------------
+// The reference's algorithm: ZGETRF (partial pivoting; the pivot is the first
+// largest |Re| + |Im| of the column in the current row order, as IZAMAX
+// picks it) followed by ZGETRS against the identity (inversions.F90:807-839).
+//
+// G x G threads, thread (cb, rb) = threadIdx cb*G + rb holding the TB x TB
+// sub-blocks rows rb*TB.., columns cb*TB.. of A and of Y = L^-1 P (the
+// forward substitution of ZGETRS against P I: column j of Y belongs to the
+// row pivoted at step j).  The G threads of one thread column are G
+// consecutive lanes of one wave, and wave w holds the columns
+// w*64/G*TB .. +64/G*TB.  Rows are never moved: pos is a row's current
+// position in LAPACK's row order, and a pivot step exchanges two positions.
+//
+// Step k: the pivot p (row, position, reciprocal) and column k after step
+// k-1 are in LDS (U(:, k)).  The threads holding row p put its A and Y
+// values in LDS; after a barrier every thread applies
+// A(r, c) -= m_r A(p, c) and Y(r, c) -= m_r Y(p, c), m_r = A(r, k)/A(p, k),
+// to its rows -- ZGETRF's elimination and ZGETRS's forward substitution in
+// one pass.  Entries that need no update get exact zeros or are dead (A left
+// of column k is in LDS already, Y right of column k is zero), and each wave
+// skips the half that is wholly dead for its columns.  Then the thread
+// column owning column k+1 writes it to LDS and publishes its pivot: the
+// largest |Re| + |Im| over the unpivoted rows as an exact 64-bit key, ties to
+// the smallest position, reduced by DPP inside the G lanes.  Two barriers per
+// step.
+//
+// Every register index is a compile-time constant (static_for, and the k
+// loop unrolled by TB; a run-time row is read through a select chain), so
+// the blocks stay in registers.
+//
+// The columns written to LDS at the pivot steps hold every entry of U (row
+// pos(r) of U is row r of A after its own step), so the back substitution
+// U X = Y (positions i = n-1 .. 0) takes U from LDS and row i of X from the
+// threads holding it, one barrier per position.  Row pos(r) of X ends in the
+// threads of row r; column j of X is column perm(j) of the inverse (the row
+// pivoted at step j).
+
+template <typename F, int... I>
+__device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+// f(std::integral_constant<int, i>) for i = 0 .. N-1, unrolled at compile time
+template <int N, typename F> __device__ __forceinline__ void static_for(F &&f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
+
+// (key, pos) pair reduction over G consecutive lanes (G = 4, 8 or 16): the
+// larger key wins, equal keys go to the smaller pos
+template <int CTRL> __device__ inline void best_dpp(unsigned long long &k, int &p) {
+  const unsigned hi = __builtin_amdgcn_update_dpp(0, (int)(k >> 32), CTRL, 0xF, 0xF, false);
+  const unsigned lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)k, CTRL, 0xF, 0xF, false);
+  const int op = __builtin_amdgcn_update_dpp(0, p, CTRL, 0xF, 0xF, false);
+  const unsigned long long ok = ((unsigned long long)hi << 32) | lo;
+  if (ok > k || (ok == k && op < p)) {
+    k = ok;
+    p = op;
+  }
+}
+template <int G> __device__ inline void group_best(unsigned long long &k, int &p) {
+  best_dpp<0xB1>(k, p);                  // quad_perm [1,0,3,2]
+  best_dpp<0x4E>(k, p);                  // quad_perm [2,3,0,1]
+  if (G >= 8) best_dpp<0x141>(k, p);     // row_half_mirror
+  if (G >= 16) best_dpp<0x140>(k, p);    // row_mirror
+}
+// x - m y with the FMA order of the elimination updates
+__device__ inline double2 fms_c(double2 x, double2 m, double2 y) {
+  x.x = fma(-m.x, y.x, x.x);
+  x.x = fma(m.y, y.y, x.x);
+  x.y = fma(-m.x, y.y, x.y);
+  x.y = fma(-m.y, y.x, x.y);
+  return x;
+}
+
+template <int G, int TB> struct LuBlk {
+  double2 a[TB][TB], y[TB][TB];
+  int pos[TB];
+  bool done[TB];
+};
+
+// The thread column owning column col (register column JJ) writes it to U
+// (U(r, col) at col * G * TB + r) and publishes its pivot: row, position and
+// reciprocal.  Called by every thread; only the owner's wave works.
+template <int G, int TB, int JJ>
+__device__ __forceinline__ void lu_publish(int col, int tid, const LuBlk<G, TB> &B, double2 *U,
+                                           int *s_perm, int *s_pospiv, double2 *s_uinv,
+                                           int *s_sing) {
+  const int cb = tid / G, rb = tid % G;
+  const int oc = col / TB;
+  if ((tid >> 6) != (oc * G) >> 6) return;  // wave-uniform: only the owner's wave
+  unsigned long long key = 0, kk[TB];
+  int bp = 0x7fffffff;
+  static_for<TB>([&](auto I) {
+    constexpr int i = I;
+    const double2 v = B.a[i][JJ];
+    kk[i] = B.done[i] ? 0ull
+                      : (unsigned long long)__double_as_longlong(fabs(v.x) + fabs(v.y)) + 1ull;
+    if (cb == oc) U[(size_t)col * (G * TB) + rb * TB + i] = v;
+    if (kk[i] > key || (kk[i] == key && B.pos[i] < bp)) {
+      key = kk[i];
+      bp = B.pos[i];
+    }
+  });
+  group_best<G>(key, bp);
+  if (cb == oc) {
+    static_for<TB>([&](auto I) {
+      constexpr int i = I;
+      if (!B.done[i] && kk[i] == key && B.pos[i] == bp) {
+        s_perm[col] = rb * TB + i;
+        s_pospiv[col & 1] = bp;
+        s_uinv[col] = crecip_fast(B.a[i][JJ]);
+        if (key <= 1ull) *s_sing = 1;  // the largest candidate is zero
+      }
+    });
+  }
+}
+
+// element (r, j) of a register block for the uniform run-time row r < TB,
+// as a select chain that starts from zero (so it is not folded into a
+// register-indexed load, which would move the block out of registers)
+template <int TB, int J>
+__device__ __forceinline__ double2 pick0(const double2 (&v)[TB][TB], int r) {
+  double xr = 0.0, xi = 0.0;
+  static_for<TB>([&](auto I) {
+    const bool hit = r == (int)I;
+    xr = hit ? v[I][J].x : xr;
+    xi = hit ? v[I][J].y : xi;
+  });
+  return double2{xr, xi};
+}
+
+template <int G, int TB>
+__global__ __launch_bounds__(G * G) void k_lu_inv_2d(int n, const double2 *__restrict__ A,
+                                                     long long sA, double2 *__restrict__ Ainv,
+                                                     long long sAinv,
+                                                     const int *__restrict__ active,
+                                                     int *__restrict__ status) {
   typedef double2 C;
+  constexpr int LD = G * TB;          // rows per U column in LDS
+  constexpr int WC = (64 / G) * TB;   // columns per wave
+  static_assert(G * G >= 64 || G * G == 16, "thread columns inside one wave");
   const int b = blockIdx.x;
   if (active && active[b] == 0) return;
   extern __shared__ __align__(16) unsigned char lsm[];
-  C *U = (C *)lsm;  // U(r, c) at c * 64 + r (physical row r), columns 0..n-1
-  __shared__ C mcol[2][64];
-  __shared__ int s_piv[2];
-  __shared__ C s_uinv[64];         // 1 / U(k, k), k = 0..n-1
-  __shared__ int s_perm[64];
-  __shared__ C bcast[NW][2 * CT];  // per-wave broadcast of one row
+  C *U = (C *)lsm;  // U(r, k) at k * LD + r (physical row r), columns 0..n-1
+  __shared__ int s_perm[LD];
+  __shared__ C s_uinv[LD];
+  __shared__ int s_pospiv[2];
   __shared__ int s_sing;
-  const int tid = threadIdx.x, w = tid >> 6, l = tid & 63;
+  __shared__ C rowa[LD], rowy[2][LD];  // row p of A and of Y (X in the back substitution)
+  const int tid = threadIdx.x, cb = tid / G, rb = tid % G;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int c_lo = w * WC, c_hi = c_lo + WC - 1;  // this wave's columns
   const C *Ab = A + (long long)b * sA;
-  C a[CT], y[CT];
-#pragma unroll
-  for (int t = 0; t < CT; ++t) {
-    const int c = w * CT + t;
-    a[t] = (l < n && c < n) ? Ab[l + (size_t)c * n] : C{0, 0};
-    y[t] = C{0, 0};
-  }
-  int pos = l;
-  bool done = l >= n;
+  LuBlk<G, TB> B;
+  static_for<TB>([&](auto I) {
+    constexpr int i = I;
+    const int r = rb * TB + i;
+    B.pos[i] = r;
+    B.done[i] = r >= n;
+    static_for<TB>([&](auto J) {
+      constexpr int j = J;
+      const int c = cb * TB + j;
+      B.a[i][j] = (r < n && c < n) ? Ab[r + (size_t)c * n] : C{0, 0};
+      B.y[i][j] = C{0, 0};
+    });
+  });
   if (tid == 0) s_sing = 0;
-
-  // pivot of the owner's register-0 column (column col) among the unpivoted
-  // rows: the largest |Re| + |Im| (exact, as a 64-bit key), ties to the
-  // smaller current position
-  auto candidate = [&](int bb, int col) {
-    const double v = cabs1(a[0]);
-    const unsigned long long key =
-        done ? 0ull : (unsigned long long)__double_as_longlong(v) + 1ull;
-    const unsigned long long kmax = wave_max64(key);
-    const unsigned pmin = wave_min32(key == kmax ? (unsigned)pos : 0xFFFFFFFFu);
-    const unsigned long long hit = __ballot(key == kmax && (unsigned)pos == pmin);
-    const int r = __ffsll((long long)hit) - 1;
-    mcol[bb][l] = a[0];
-    if (l == r) {
-      s_piv[bb] = r;
-      s_uinv[col] = crecip_fast(a[0]);
-      if (kmax <= 1ull) s_sing = 1;  // the largest candidate is zero
-    }
-  };
-  if (w == 0) candidate(0, 0);
+  lu_publish<G, TB, 0>(0, tid, B, U, s_perm, s_pospiv, s_uinv, &s_sing);
   __syncthreads();
 
-  C *bc = bcast[w];
-  for (int k = 0; k < n; ++k) {
-    const int bf = k & 1, wk = k / CT;
-    const int p = s_piv[bf];
-    const C pinv = s_uinv[k];
-    // position exchange of LAPACK's row interchange k <-> position of p
-    const int pos_p = __builtin_amdgcn_readlane(pos, p);
-    if (pos == k) pos = pos_p;
-    if (l == p) { pos = k; done = true; }
-    if (tid == 0) s_perm[k] = p;
-    const C mraw = (w == wk) ? a[0] : mcol[bf][l];
-    const C m = done ? C{0, 0} : cmul(mraw, pinv);
-    if (w == wk) U[(size_t)k * 64 + l] = a[0];  // column k of U is final
-    // column k of Y = L^-1 starts with the 1 of the pivot row
-    if (l == p) {
-#pragma unroll
-      for (int t = 0; t < CT; ++t)
-        if (w * CT + t == k) y[t] = C{1.0, 0.0};
-    }
-    // waves whose A columns all lie left of k hold no live A values any more,
-    // waves whose Y columns all lie right of k have nothing to update there
-    const bool a_live = w * CT + CT - 1 > k;
-    const bool y_live = w * CT <= k;
-    // row p of this wave's columns, through LDS
-    if (l == p) {
-#pragma unroll
-      for (int t = 0; t < CT; ++t) { bc[t] = a[t]; bc[CT + t] = y[t]; }
-    }
-    __builtin_amdgcn_wave_barrier();
-    // A: new register i = old register i + 1 (column w*CT + (i+1+k) mod CT),
-    // updated when that column lies right of k; the pivot row comes from LDS
-    auto upd = [&](C x, int src) {
-      const int c = w * CT + (src + k) % CT;
-      const C mm = (c > k) ? m : C{0, 0};
-      const C yv = bc[src];
-      x.x = fma(-mm.x, yv.x, x.x);
-      x.x = fma(mm.y, yv.y, x.x);
-      x.y = fma(-mm.x, yv.y, x.y);
-      x.y = fma(-mm.y, yv.x, x.y);
-      return x;
-    };
-    if (a_live) {
-      const C a_first = a[0];
-#pragma unroll
-      for (int i = 0; i + 1 < CT; ++i) a[i] = upd(a[i + 1], i + 1);
-      a[CT - 1] = upd(a_first, 0);
-    }
-    if (k + 1 < n && w == (k + 1) / CT) candidate(bf ^ 1, k + 1);
-    if (y_live) {
-#pragma unroll
-      for (int t = 0; t < CT; ++t) {
-        C x = y[t];
-        const C yv = bc[CT + t];
-        x.x = fma(-m.x, yv.x, x.x);
-        x.x = fma(m.y, yv.y, x.x);
-        x.y = fma(-m.x, yv.y, x.y);
-        x.y = fma(-m.y, yv.x, x.y);
-        y[t] = x;
+  for (int k0 = 0; k0 < n; k0 += TB) {
+    static_for<TB>([&](auto KK) {
+      constexpr int kk = KK;
+      const int k = k0 + kk;
+      if (k >= n) return;  // uniform
+      const int p = s_perm[k], pos_p = s_pospiv[k & 1];
+      const C pinv = s_uinv[k];
+      const int prb = p / TB, pi = p - prb * TB;
+      C m[TB];
+      static_for<TB>([&](auto I) {
+        constexpr int i = I;
+        const int r = rb * TB + i;
+        if (r == p) {
+          B.pos[i] = k;
+          B.done[i] = true;
+          // column k of Y starts with the 1 of the pivot row
+          if (cb == k0 / TB) B.y[i][kk] = C{1.0, 0.0};
+        } else if (B.pos[i] == k) {
+          B.pos[i] = pos_p;
+        }
+        m[i] = B.done[i] ? C{0, 0} : cmul(U[(size_t)k * LD + r], pinv);
+      });
+      if (rb == prb) {
+        const int ps = __builtin_amdgcn_readfirstlane(pi);
+        static_for<TB>([&](auto J) {
+          rowa[cb * TB + J] = pick0<TB, J>(B.a, ps);
+          rowy[0][cb * TB + J] = pick0<TB, J>(B.y, ps);
+        });
       }
-    }
-    __syncthreads();
+      __syncthreads();
+      if (k < c_hi) {  // some of this wave's A columns lie right of k
+        static_for<TB>([&](auto J) {
+          constexpr int j = J;
+          const C u = rowa[cb * TB + j];
+          static_for<TB>([&](auto I) { B.a[I][j] = fms_c(B.a[I][j], m[I], u); });
+        });
+      }
+      if (k >= c_lo) {  // some of this wave's Y columns lie at or left of k
+        static_for<TB>([&](auto J) {
+          constexpr int j = J;
+          const C yu = rowy[0][cb * TB + j];
+          static_for<TB>([&](auto I) { B.y[I][j] = fms_c(B.y[I][j], m[I], yu); });
+        });
+      }
+      if (k + 1 < n)
+        lu_publish<G, TB, (kk + 1) % TB>(k + 1, tid, B, U, s_perm, s_pospiv, s_uinv, &s_sing);
+      __syncthreads();
+    });
   }
 
-  // back substitution U X = Y, position i = n-1 .. 0; X(i, :) ends in the
-  // registers of the lane whose row holds position i
+  // back substitution U X = Y, position i = n-1 .. 0: row i of X through LDS
+  // (two alternating buffers, one barrier per position)
   for (int i = n - 1; i >= 0; --i) {
     const int q = s_perm[i];
     const C uinv = s_uinv[i];
-    const C coef = (pos < i) ? U[(size_t)i * 64 + l] : C{0, 0};
-    if (l == q) {
-#pragma unroll
-      for (int t = 0; t < CT; ++t) {
-        y[t] = cmul(y[t], uinv);
-        bc[t] = y[t];
-      }
+    const int qrb = q / TB, qi = q - qrb * TB;
+    C *xb = rowy[i & 1] + cb * TB;
+    if (rb == qrb) {  // row i of X = row q of Y / U(i, i), kept unscaled in registers
+      const int qs = __builtin_amdgcn_readfirstlane(qi);
+      static_for<TB>([&](auto J) { xb[J] = cmul(pick0<TB, J>(B.y, qs), uinv); });
     }
-    __builtin_amdgcn_wave_barrier();
-#pragma unroll
-    for (int t = 0; t < CT; ++t) {
-      const C xq = bc[t];
-      C x = y[t];
-      x.x = fma(-coef.x, xq.x, x.x);
-      x.x = fma(coef.y, xq.y, x.x);
-      x.y = fma(-coef.x, xq.y, x.y);
-      x.y = fma(-coef.y, xq.x, x.y);
-      y[t] = x;  // coef = 0 on lane q and on rows at or below position i
-    }
-    __builtin_amdgcn_wave_barrier();
+    C coef[TB];
+    static_for<TB>([&](auto T) {
+      coef[T] = (B.pos[T] < i) ? U[(size_t)i * LD + rb * TB + T] : C{0, 0};
+    });
+    __syncthreads();
+    static_for<TB>([&](auto J) {
+      const C xq = xb[J];
+      // coef = 0 on row q and below position i
+      static_for<TB>([&](auto T) { B.y[T][J] = fms_c(B.y[T][J], coef[T], xq); });
+    });
   }
+  // rows of X: the scaling by 1 / U(pos, pos) left out above (a row is not
+  // updated after its own position)
+  static_for<TB>([&](auto I) {
+    if (rb * TB + I < n) {
+      const C ui = s_uinv[B.pos[I]];
+      static_for<TB>([&](auto J) { B.y[I][J] = cmul(B.y[I][J], ui); });
+    }
+  });
   // column j of X = U^-1 L^-1 is column perm(j) of the inverse
   C *Ob = Ainv + (long long)b * sAinv;
-  if (l < n)
-#pragma unroll
-    for (int t = 0; t < CT; ++t) {
-      const int j = w * CT + t;
-      if (j < n) Ob[pos + (size_t)s_perm[j] * n] = y[t];
+  static_for<TB>([&](auto J) {
+    const int c = cb * TB + J;
+    if (c < n) {
+      const int pc = s_perm[c];
+      static_for<TB>([&](auto I) {
+        if (rb * TB + I < n) Ob[B.pos[I] + (size_t)pc * n] = B.y[I][J];
+      });
     }
+  });
   if (tid == 0 && s_sing && status) atomicOr(&status[b], (int)ST_SINGULAR);
 }
 
@@ -730,18 +814,18 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
   if constexpr (sizeof(R) == 8) {
     if (n <= 64) {
       ProfScope ps(PT_INV_SMALL, -1.0, 0.0, s);
-      const size_t lds = (size_t)n * 64 * sizeof(double2);
       if (n <= 16) {
-        allow_lds(k_lu_inv_reg<1, 16>, lds);
-        hipLaunchKernelGGL((k_lu_inv_reg<1, 16>), dim3(batch), dim3(64), lds, s, n, A, sA, Ainv,
+        const size_t lds = (size_t)n * 16 * sizeof(double2);
+        hipLaunchKernelGGL((k_lu_inv_2d<8, 2>), dim3(batch), dim3(64), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
       } else if (n <= 32) {
-        allow_lds(k_lu_inv_reg<2, 16>, lds);
-        hipLaunchKernelGGL((k_lu_inv_reg<2, 16>), dim3(batch), dim3(128), lds, s, n, A, sA, Ainv,
+        const size_t lds = (size_t)n * 32 * sizeof(double2);
+        hipLaunchKernelGGL((k_lu_inv_2d<8, 4>), dim3(batch), dim3(64), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
       } else {
-        allow_lds(k_lu_inv_reg<4, 16>, lds);
-        hipLaunchKernelGGL((k_lu_inv_reg<4, 16>), dim3(batch), dim3(256), lds, s, n, A, sA, Ainv,
+        const size_t lds = (size_t)n * 64 * sizeof(double2);
+        allow_lds(k_lu_inv_2d<16, 4>, lds);
+        hipLaunchKernelGGL((k_lu_inv_2d<16, 4>), dim3(batch), dim3(256), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
       }
       HN_CHECK(hipGetLastError());

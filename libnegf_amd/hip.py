@@ -139,7 +139,7 @@ PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV, PT_INV_PANEL, PT_INV_SMALL, PT_FUSED
 # smaller GEMMs of a tag run on k_gemm<double, tag>.
 PT_NAMES = {PT_GEMM_MASKED: "k_gemm_w<0>", PT_GEMM_SWEEP: "k_gemm_w<1>",
             PT_GEMM_INV: "k_gemm_w<2>", PT_INV_PANEL: "k_inv_gj",
-            PT_INV_SMALL: "k_lu_inv_reg", PT_FUSED: "k_t_fused"}
+            PT_INV_SMALL: "k_lu_inv_2d", PT_FUSED: "k_t_fused"}
 
 
 def profile_enable(on: bool):

@@ -273,14 +273,17 @@ def test_graph_replay_matches_eager(gpu, kind):
         s = wl.KSlab.make(n=120, nbl=4, seed=6).at_k(0.5, 0.1)  # blocked inverse
     ds = hip.DeviceStructure(s)
     grids = [np.linspace(-2.0, 2.0, 17), np.linspace(-1.1, 0.9, 17), np.linspace(0.2, 2.6, 17)]
-    eager = [hip.solve_transmission(eng, ds, E, delta=1e-5) for E in grids]
+    # the graph replays the block-operation path: compare with that path run
+    # eagerly (blocks of 12 would otherwise take the fused one-launch solve)
+    nf = hip.NO_FUSED
+    eager = [hip.solve_transmission(eng, ds, E, delta=1e-5, flags=nf) for E in grids]
     for rep in range(3):
         for E, (T0, n0, s0) in zip(grids, eager):
-            T1, n1, s1 = hip.solve_transmission(eng, ds, E, delta=1e-5, flags=hip.GRAPH)
+            T1, n1, s1 = hip.solve_transmission(eng, ds, E, delta=1e-5, flags=hip.GRAPH | nf)
             np.testing.assert_array_equal(T1, T0)
             np.testing.assert_array_equal(n1, n0)
             np.testing.assert_array_equal(s1, s0)
-            Te, _, _ = hip.solve_transmission(eng, ds, E[::-1], delta=1e-5)  # eager work between
+            Te, _, _ = hip.solve_transmission(eng, ds, E[::-1], delta=1e-5, flags=nf)  # eager work between
             np.testing.assert_array_equal(Te[::-1], T0)
 
 
