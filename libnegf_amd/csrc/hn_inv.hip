@@ -229,36 +229,39 @@ __device__ inline unsigned wave_min32(unsigned k) {
 // picks it) followed by ZGETRS against the identity (inversions.F90:807-839).
 //
 // G x G threads, thread (cb, rb) = threadIdx cb*G + rb holding the TB x TB
-// sub-blocks rows rb*TB.., columns cb*TB.. of A and of Y = L^-1 P (the
-// forward substitution of ZGETRS against P I: column j of Y belongs to the
-// row pivoted at step j).  The G threads of one thread column are G
-// consecutive lanes of one wave, and wave w holds the columns
-// w*64/G*TB .. +64/G*TB.  Rows are never moved: pos is a row's current
-// position in LAPACK's row order, and a pivot step exchanges two positions.
+// sub-block rows rb*TB.., columns cb*TB.. in registers.  The G threads of one
+// thread column are G consecutive lanes of one wave.  Rows are never moved:
+// pos is a row's current position in LAPACK's row order, and a pivot step
+// exchanges two positions.  One register block holds both factors in place:
+// columns right of the current step hold A, columns at or left of it hold
+// Y = L^-1 P (the forward substitution of ZGETRS against P I; column j of Y
+// belongs to the row pivoted at step j, so the pivoted rows' entries of Y
+// right of their own step are zero).
 //
 // Step k: the pivot p (row, position, reciprocal) and column k after step
-// k-1 are in LDS (U(:, k)).  The threads holding row p put its A and Y
-// values in LDS; after a barrier every thread applies
-// A(r, c) -= m_r A(p, c) and Y(r, c) -= m_r Y(p, c), m_r = A(r, k)/A(p, k),
-// to its rows -- ZGETRF's elimination and ZGETRS's forward substitution in
-// one pass.  Entries that need no update get exact zeros or are dead (A left
-// of column k is in LDS already, Y right of column k is zero), and each wave
-// skips the half that is wholly dead for its columns.  Then the thread
-// column owning column k+1 writes it to LDS and publishes its pivot: the
+// k-1 are in LDS.  The thread column owning column k turns that register
+// column into Y's column k (1 on row p, 0 elsewhere).  Every thread takes row
+// p for its columns from the lane of its own thread column that holds it
+// (ds_bpermute inside the wave) and applies X(r, c) -= m_r X(p, c),
+// m_r = A(r, k)/A(p, k), to its unpivoted rows -- ZGETRF's elimination on
+// A's columns and ZGETRS's forward substitution on Y's, in one pass with no
+// branches.  Then the thread column owning column k+1 publishes it: the
+// entries of pivoted rows (U above the diagonal) into a triangular LDS array
+// by position, the others for the next multipliers, and the pivot -- the
 // largest |Re| + |Im| over the unpivoted rows as an exact 64-bit key, ties to
-// the smallest position, reduced by DPP inside the G lanes.  Two barriers per
+// the smallest position, reduced by DPP inside the G lanes.  One barrier per
 // step.
 //
-// Every register index is a compile-time constant (static_for, and the k
-// loop unrolled by TB; a run-time row is read through a select chain), so
-// the blocks stay in registers.
+// Every register index is a compile-time constant (static_for, the k loop
+// unrolled by TB, and run-time rows read through select chains), so the
+// block stays in registers.
 //
-// The columns written to LDS at the pivot steps hold every entry of U (row
-// pos(r) of U is row r of A after its own step), so the back substitution
-// U X = Y (positions i = n-1 .. 0) takes U from LDS and row i of X from the
-// threads holding it, one barrier per position.  Row pos(r) of X ends in the
-// threads of row r; column j of X is column perm(j) of the inverse (the row
-// pivoted at step j).
+// Back substitution U X = Y, positions i = n-1 .. 0, without barriers: row i
+// of X (the row pivoted at step i, divided by U(i, i)) from the lanes of the
+// same thread column, U's column i from the triangular array; the division
+// of a row's own registers is applied at the end (a row is not updated after
+// its own position).  Row pos(r) of X ends in the threads of row r; column j of X is
+// column perm(j) of the inverse (the row pivoted at step j).
 
 template <typename F, int... I>
 __device__ __forceinline__ void static_for_impl(F &&f, std::integer_sequence<int, I...>) {
@@ -295,51 +298,25 @@ __device__ inline double2 fms_c(double2 x, double2 m, double2 y) {
   x.y = fma(-m.y, y.x, x.y);
   return x;
 }
+// value `v` of lane `src` (lane index within the wave), for every lane
+__device__ inline double bperm(double v, int src) {
+  const int lo = __builtin_amdgcn_ds_bpermute(src << 2, __double2loint(v));
+  const int hi = __builtin_amdgcn_ds_bpermute(src << 2, __double2hiint(v));
+  return __hiloint2double(hi, lo);
+}
+__device__ inline double2 bperm(double2 v, int src) {
+  return double2{bperm(v.x, src), bperm(v.y, src)};
+}
+// offset of U(i, c), i < c, in the triangular LDS array (by position)
+__device__ inline int utri(int i, int c) { return (c * (c - 1)) / 2 + i; }
 
-template <int G, int TB> struct LuBlk {
-  double2 a[TB][TB], y[TB][TB];
+template <int TB> struct LuBlk {
+  double2 v[TB][TB];
   int pos[TB];
   bool done[TB];
 };
 
-// The thread column owning column col (register column JJ) writes it to U
-// (U(r, col) at col * G * TB + r) and publishes its pivot: row, position and
-// reciprocal.  Called by every thread; only the owner's wave works.
-template <int G, int TB, int JJ>
-__device__ __forceinline__ void lu_publish(int col, int tid, const LuBlk<G, TB> &B, double2 *U,
-                                           int *s_perm, int *s_pospiv, double2 *s_uinv,
-                                           int *s_sing) {
-  const int cb = tid / G, rb = tid % G;
-  const int oc = col / TB;
-  if ((tid >> 6) != (oc * G) >> 6) return;  // wave-uniform: only the owner's wave
-  unsigned long long key = 0, kk[TB];
-  int bp = 0x7fffffff;
-  static_for<TB>([&](auto I) {
-    constexpr int i = I;
-    const double2 v = B.a[i][JJ];
-    kk[i] = B.done[i] ? 0ull
-                      : (unsigned long long)__double_as_longlong(fabs(v.x) + fabs(v.y)) + 1ull;
-    if (cb == oc) U[(size_t)col * (G * TB) + rb * TB + i] = v;
-    if (kk[i] > key || (kk[i] == key && B.pos[i] < bp)) {
-      key = kk[i];
-      bp = B.pos[i];
-    }
-  });
-  group_best<G>(key, bp);
-  if (cb == oc) {
-    static_for<TB>([&](auto I) {
-      constexpr int i = I;
-      if (!B.done[i] && kk[i] == key && B.pos[i] == bp) {
-        s_perm[col] = rb * TB + i;
-        s_pospiv[col & 1] = bp;
-        s_uinv[col] = crecip_fast(B.a[i][JJ]);
-        if (key <= 1ull) *s_sing = 1;  // the largest candidate is zero
-      }
-    });
-  }
-}
-
-// element (r, j) of a register block for the uniform run-time row r < TB,
+// element (r, J) of the register block for the uniform run-time row r < TB,
 // as a select chain that starts from zero (so it is not folded into a
 // register-indexed load, which would move the block out of registers)
 template <int TB, int J>
@@ -353,6 +330,47 @@ __device__ __forceinline__ double2 pick0(const double2 (&v)[TB][TB], int r) {
   return double2{xr, xi};
 }
 
+// The thread column owning column col (register column JJ) publishes it:
+// pivoted rows' entries into the triangular U array, unpivoted rows' into
+// colbuf (the next multipliers), and the pivot's row, position and
+// reciprocal.  Called by every thread; only the owner's wave works.
+template <int G, int TB, int JJ>
+__device__ __forceinline__ void lu_publish(int col, int tid, const LuBlk<TB> &B, double2 *Ut,
+                                           double2 *colbuf, int *s_perm, int *s_pospiv,
+                                           double2 *s_uinv, int *s_sing) {
+  const int cb = tid / G, rb = tid % G;
+  const int oc = col / TB;
+  if ((tid >> 6) != (oc * G) >> 6) return;  // wave-uniform: only the owner's wave
+  unsigned long long key = 0, kk[TB];
+  int bp = 0x7fffffff;
+  static_for<TB>([&](auto I) {
+    constexpr int i = I;
+    const double2 v = B.v[i][JJ];
+    kk[i] = B.done[i] ? 0ull
+                      : (unsigned long long)__double_as_longlong(fabs(v.x) + fabs(v.y)) + 1ull;
+    if (cb == oc) {
+      if (B.done[i]) Ut[utri(B.pos[i], col)] = v;
+      else colbuf[rb * TB + i] = v;
+    }
+    if (kk[i] > key || (kk[i] == key && B.pos[i] < bp)) {
+      key = kk[i];
+      bp = B.pos[i];
+    }
+  });
+  group_best<G>(key, bp);
+  if (cb == oc) {
+    static_for<TB>([&](auto I) {
+      constexpr int i = I;
+      if (!B.done[i] && kk[i] == key && B.pos[i] == bp) {
+        s_perm[col] = rb * TB + i;
+        s_pospiv[col & 1] = bp;
+        s_uinv[col] = crecip_fast(B.v[i][JJ]);
+        if (key <= 1ull) *s_sing = 1;  // the largest candidate is zero
+      }
+    });
+  }
+}
+
 template <int G, int TB>
 __global__ __launch_bounds__(G * G) void k_lu_inv_2d(int n, const double2 *__restrict__ A,
                                                      long long sA, double2 *__restrict__ Ainv,
@@ -360,37 +378,33 @@ __global__ __launch_bounds__(G * G) void k_lu_inv_2d(int n, const double2 *__res
                                                      const int *__restrict__ active,
                                                      int *__restrict__ status) {
   typedef double2 C;
-  constexpr int LD = G * TB;          // rows per U column in LDS
-  constexpr int WC = (64 / G) * TB;   // columns per wave
+  constexpr int LD = G * TB;  // rows (and columns) of the padded block
   static_assert(G * G >= 64 || G * G == 16, "thread columns inside one wave");
   const int b = blockIdx.x;
   if (active && active[b] == 0) return;
   extern __shared__ __align__(16) unsigned char lsm[];
-  C *U = (C *)lsm;  // U(r, k) at k * LD + r (physical row r), columns 0..n-1
+  C *Ut = (C *)lsm;  // U(i, c), i < c (positions), at utri(i, c)
   __shared__ int s_perm[LD];
   __shared__ C s_uinv[LD];
   __shared__ int s_pospiv[2];
   __shared__ int s_sing;
-  __shared__ C rowa[LD], rowy[2][LD];  // row p of A and of Y (X in the back substitution)
+  __shared__ C colbuf[2][LD];
   const int tid = threadIdx.x, cb = tid / G, rb = tid % G;
-  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int c_lo = w * WC, c_hi = c_lo + WC - 1;  // this wave's columns
+  const int gbase = (tid & 63) - rb;  // lane of this thread column's rb = 0
   const C *Ab = A + (long long)b * sA;
-  LuBlk<G, TB> B;
+  LuBlk<TB> B;
   static_for<TB>([&](auto I) {
     constexpr int i = I;
     const int r = rb * TB + i;
     B.pos[i] = r;
     B.done[i] = r >= n;
     static_for<TB>([&](auto J) {
-      constexpr int j = J;
-      const int c = cb * TB + j;
-      B.a[i][j] = (r < n && c < n) ? Ab[r + (size_t)c * n] : C{0, 0};
-      B.y[i][j] = C{0, 0};
+      const int c = cb * TB + J;
+      B.v[i][J] = (r < n && c < n) ? Ab[r + (size_t)c * n] : C{0, 0};
     });
   });
   if (tid == 0) s_sing = 0;
-  lu_publish<G, TB, 0>(0, tid, B, U, s_perm, s_pospiv, s_uinv, &s_sing);
+  lu_publish<G, TB, 0>(0, tid, B, Ut, colbuf[0], s_perm, s_pospiv, s_uinv, &s_sing);
   __syncthreads();
 
   for (int k0 = 0; k0 < n; k0 += TB) {
@@ -401,6 +415,7 @@ __global__ __launch_bounds__(G * G) void k_lu_inv_2d(int n, const double2 *__res
       const int p = s_perm[k], pos_p = s_pospiv[k & 1];
       const C pinv = s_uinv[k];
       const int prb = p / TB, pi = p - prb * TB;
+      const bool own_k = cb == k0 / TB;
       C m[TB];
       static_for<TB>([&](auto I) {
         constexpr int i = I;
@@ -408,69 +423,49 @@ __global__ __launch_bounds__(G * G) void k_lu_inv_2d(int n, const double2 *__res
         if (r == p) {
           B.pos[i] = k;
           B.done[i] = true;
-          // column k of Y starts with the 1 of the pivot row
-          if (cb == k0 / TB) B.y[i][kk] = C{1.0, 0.0};
         } else if (B.pos[i] == k) {
           B.pos[i] = pos_p;
         }
-        m[i] = B.done[i] ? C{0, 0} : cmul(U[(size_t)k * LD + r], pinv);
+        m[i] = B.done[i] ? C{0, 0} : cmul(colbuf[k & 1][r], pinv);
+        // register column k becomes Y's column k: 1 on the pivot row
+        if (own_k) B.v[i][kk] = C{r == p ? 1.0 : 0.0, 0.0};
       });
-      if (rb == prb) {
-        const int ps = __builtin_amdgcn_readfirstlane(pi);
-        static_for<TB>([&](auto J) {
-          rowa[cb * TB + J] = pick0<TB, J>(B.a, ps);
-          rowy[0][cb * TB + J] = pick0<TB, J>(B.y, ps);
-        });
-      }
-      __syncthreads();
-      if (k < c_hi) {  // some of this wave's A columns lie right of k
-        static_for<TB>([&](auto J) {
-          constexpr int j = J;
-          const C u = rowa[cb * TB + j];
-          static_for<TB>([&](auto I) { B.a[I][j] = fms_c(B.a[I][j], m[I], u); });
-        });
-      }
-      if (k >= c_lo) {  // some of this wave's Y columns lie at or left of k
-        static_for<TB>([&](auto J) {
-          constexpr int j = J;
-          const C yu = rowy[0][cb * TB + j];
-          static_for<TB>([&](auto I) { B.y[I][j] = fms_c(B.y[I][j], m[I], yu); });
-        });
-      }
+      // row p for this thread's columns, from the lane of this thread column
+      // that holds it (same wave)
+      const int ps = __builtin_amdgcn_readfirstlane(pi);
+      static_for<TB>([&](auto J) {
+        const C u = bperm(pick0<TB, J>(B.v, ps), gbase + prb);
+        static_for<TB>([&](auto I) { B.v[I][J] = fms_c(B.v[I][J], m[I], u); });
+      });
       if (k + 1 < n)
-        lu_publish<G, TB, (kk + 1) % TB>(k + 1, tid, B, U, s_perm, s_pospiv, s_uinv, &s_sing);
+        lu_publish<G, TB, (kk + 1) % TB>(k + 1, tid, B, Ut, colbuf[(k + 1) & 1], s_perm,
+                                         s_pospiv, s_uinv, &s_sing);
       __syncthreads();
     });
   }
 
-  // back substitution U X = Y, position i = n-1 .. 0: row i of X through LDS
-  // (two alternating buffers, one barrier per position)
+  // back substitution U X = Y, position i = n-1 .. 0, each wave on its own
+  // columns: row i of X (row q of Y / U(i, i), left unscaled in the registers
+  // of row q) from the lane of this thread column that holds row q
   for (int i = n - 1; i >= 0; --i) {
     const int q = s_perm[i];
     const C uinv = s_uinv[i];
-    const int qrb = q / TB, qi = q - qrb * TB;
-    C *xb = rowy[i & 1] + cb * TB;
-    if (rb == qrb) {  // row i of X = row q of Y / U(i, i), kept unscaled in registers
-      const int qs = __builtin_amdgcn_readfirstlane(qi);
-      static_for<TB>([&](auto J) { xb[J] = cmul(pick0<TB, J>(B.y, qs), uinv); });
-    }
+    const int qrb = q / TB, qs = __builtin_amdgcn_readfirstlane(q - qrb * TB);
     C coef[TB];
     static_for<TB>([&](auto T) {
-      coef[T] = (B.pos[T] < i) ? U[(size_t)i * LD + rb * TB + T] : C{0, 0};
+      coef[T] = (B.pos[T] < i) ? Ut[utri(B.pos[T], i)] : C{0, 0};
     });
-    __syncthreads();
     static_for<TB>([&](auto J) {
-      const C xq = xb[J];
+      const C xq = bperm(cmul(pick0<TB, J>(B.v, qs), uinv), gbase + qrb);
       // coef = 0 on row q and below position i
-      static_for<TB>([&](auto T) { B.y[T][J] = fms_c(B.y[T][J], coef[T], xq); });
+      static_for<TB>([&](auto T) { B.v[T][J] = fms_c(B.v[T][J], coef[T], xq); });
     });
   }
-  // rows of X: the scaling by 1 / U(pos, pos) left out above (a row is not
-  // updated after its own position)
+  // rows of X: the scaling by 1 / U(pos, pos) left out above
   static_for<TB>([&](auto I) {
     if (rb * TB + I < n) {
       const C ui = s_uinv[B.pos[I]];
-      static_for<TB>([&](auto J) { B.y[I][J] = cmul(B.y[I][J], ui); });
+      static_for<TB>([&](auto J) { B.v[I][J] = cmul(B.v[I][J], ui); });
     }
   });
   // column j of X = U^-1 L^-1 is column perm(j) of the inverse
@@ -480,7 +475,7 @@ __global__ __launch_bounds__(G * G) void k_lu_inv_2d(int n, const double2 *__res
     if (c < n) {
       const int pc = s_perm[c];
       static_for<TB>([&](auto I) {
-        if (rb * TB + I < n) Ob[B.pos[I] + (size_t)pc * n] = B.y[I][J];
+        if (rb * TB + I < n) Ob[B.pos[I] + (size_t)pc * n] = B.v[I][J];
       });
     }
   });
@@ -814,20 +809,17 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
   if constexpr (sizeof(R) == 8) {
     if (n <= 64) {
       ProfScope ps(PT_INV_SMALL, -1.0, 0.0, s);
-      if (n <= 16) {
-        const size_t lds = (size_t)n * 16 * sizeof(double2);
+      // the triangular U array: n (n - 1) / 2 entries
+      const size_t lds = std::max<size_t>(1, (size_t)n * (n - 1) / 2) * sizeof(double2);
+      if (n <= 16)
         hipLaunchKernelGGL((k_lu_inv_2d<8, 2>), dim3(batch), dim3(64), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
-      } else if (n <= 32) {
-        const size_t lds = (size_t)n * 32 * sizeof(double2);
+      else if (n <= 32)
         hipLaunchKernelGGL((k_lu_inv_2d<8, 4>), dim3(batch), dim3(64), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
-      } else {
-        const size_t lds = (size_t)n * 64 * sizeof(double2);
-        allow_lds(k_lu_inv_2d<16, 4>, lds);
+      else
         hipLaunchKernelGGL((k_lu_inv_2d<16, 4>), dim3(batch), dim3(256), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
-      }
       HN_CHECK(hipGetLastError());
       return;
     }
