@@ -371,11 +371,19 @@ typedef struct {
   double chunk_flops;         /* target algorithmic flops per chunk; 0: 2e12 */
   long long mem_budget;       /* bytes; 0: 85 % of free HBM */
   long long worker_workspace; /* bytes per worker; 0: automatic */
+  long long sgf_cache;        /* bytes of the surface-function cache the workers share
+                                 (requests then load cached g_s, else compute and store
+                                 them: ReadOldSGF = 0); 0: none */
+  int keep_structures;        /* 1: generated structures stay resident after their
+                                 requests (least recently used evicted when admission
+                                 needs the memory); 0: freed when their last request ends */
+  int reserved2;
 } hipnegf_server_opts;
 int hipnegf_server_create(int device, const hipnegf_server_opts *o, hipnegf_server *s);
 /* finishes the admitted requests, drops queued ones, joins the workers */
 int hipnegf_server_destroy(hipnegf_server s);
-/* request on a generated slab (built in HBM at admission, freed when done) */
+/* request on a generated slab (built in HBM at its first admission; kept
+ * resident for later requests with the same arguments when keep_structures) */
 int hipnegf_server_submit_kslab(hipnegf_server s, unsigned long long seed, int n, int nbl,
                                 double kx, double ky, int with_overlap, int nE, const double *E,
                                 const hipnegf_opts *o, unsigned long long *id);
@@ -390,6 +398,9 @@ int hipnegf_server_wait(hipnegf_server s, unsigned long long id, double *T, int 
                         double *times);
 /* {submitted, completed, running, queued} */
 int hipnegf_server_stats(hipnegf_server s, long long *out);
+/* {requests served on an already resident generated structure, resident
+ * generated structures, surface functions loaded from the cache, stored} */
+int hipnegf_server_cache_stats(hipnegf_server s, long long *out);
 
 /* Device memory for the engine's surface Green's function cache (0: off).
  * Entries are keyed by (structure and its contents version, slot, E, eta,

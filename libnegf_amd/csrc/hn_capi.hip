@@ -917,6 +917,8 @@ int hipnegf_server_create(int device, const hipnegf_server_opts *o, hipnegf_serv
       if (o->chunk_flops > 0) so.chunk_flops = o->chunk_flops;
       so.mem_budget = (size_t)std::max(0LL, o->mem_budget);
       so.worker_ws = (size_t)std::max(0LL, o->worker_workspace);
+      so.sgf_cache = (size_t)std::max(0LL, o->sgf_cache);
+      so.keep_structs = o->keep_structures ? 1 : 0;
     }
     auto *x = new hipnegf_server_s;
     try {
@@ -983,6 +985,10 @@ int hipnegf_server_wait(hipnegf_server s, unsigned long long id, double *T, int 
 
 int hipnegf_server_stats(hipnegf_server s, long long *out) {
   return guard([&] { s->srv->stats(out); return 0; });
+}
+
+int hipnegf_server_cache_stats(hipnegf_server s, long long *out) {
+  return guard([&] { s->srv->cache_stats(out); return 0; });
 }
 
 }  // extern "C"

@@ -2,6 +2,7 @@
 #pragma once
 #include <functional>
 #include <memory>
+#include <mutex>
 #include <unordered_map>
 #include <vector>
 
@@ -160,6 +161,54 @@ struct SCBAOpts {
   int ref = 0;                       // 1-based reference contact (ncont+1: none)
 };
 
+// Surface Green's function cache (the role of ln_cache's TMatrixCacheMem for
+// surface_green, contselfenergy.F90:138-173, 251-259): g_s of (structure and
+// its contents version, slot, E, eta, contact) in one device buffer of
+// fixed-size entries, least recently used out first.  One engine owns it, or
+// several engines of one device share it (the serving workers): every use
+// holds `mu`, and the device copies into or out of the pool are complete
+// before it is released.
+struct SgfCache {
+  struct Key {
+    unsigned long long uid, version, ebits, dbits;
+    int slot, contact, nc;
+    bool operator==(const Key &o) const {
+      return uid == o.uid && version == o.version && ebits == o.ebits && dbits == o.dbits &&
+             slot == o.slot && contact == o.contact && nc == o.nc;
+    }
+  };
+  struct KeyHash {
+    size_t operator()(const Key &k) const {
+      size_t h = std::hash<unsigned long long>()(k.uid * 1000003ULL ^ k.version);
+      h ^= std::hash<unsigned long long>()(k.ebits) + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
+      h ^= std::hash<unsigned long long>()(k.dbits ^ ((unsigned long long)k.slot << 20) ^
+                                          ((unsigned long long)k.contact << 8) ^ (unsigned long long)k.nc);
+      return h;
+    }
+  };
+  std::mutex mu;
+  double2 *pool = nullptr;
+  size_t bytes = 0;
+  long long entry_elems = 0;
+  int capacity = 0;
+  std::unordered_map<Key, int, KeyHash> map;
+  std::vector<Key> keys;
+  std::vector<unsigned long long> used;  // LRU stamps
+  std::vector<int> valid;
+  std::vector<int> status;  // decimation status bits stored with each entry
+  unsigned long long clock = 0;
+  long long hits = 0, stores = 0;
+  SgfCache(int device, size_t bytes);
+  ~SgfCache();
+  SgfCache(const SgfCache &) = delete;
+  SgfCache &operator=(const SgfCache &) = delete;
+  // entries of at least entry_elems elements (re-partitions, dropping every
+  // entry, when a larger size is asked for); false: no capacity.  Caller holds mu.
+  bool prepare(long long entry_elems);
+  // the entry of k, inserted (evicting the least recently used) if absent.  Caller holds mu.
+  int slot_for(const Key &k);
+};
+
 class Engine {
  public:
   explicit Engine(int device);
@@ -217,9 +266,12 @@ class Engine {
   int last_batch() const { return last_batch_; }
   // device memory for the surface Green's function cache (0 disables it)
   void set_sgf_cache(size_t bytes);
-  long long sgf_hits() const { return sgf_hits_; }
-  long long sgf_stores() const { return sgf_stores_; }
-  size_t sgf_cache_bytes() const { return sgf_bytes_; }
+  // use a cache shared with other engines of the same device (null: none)
+  void attach_sgf_cache(std::shared_ptr<SgfCache> c);
+  const std::shared_ptr<SgfCache> &sgf_cache() const { return sgf_; }
+  long long sgf_hits() const;
+  long long sgf_stores() const;
+  size_t sgf_cache_bytes() const;
   // One cached g_s (column-major, order ngs(c)) of item (slot, E) under the
   // key a solve with options o uses: copied to host memory (false when it is
   // not cached) or inserted from it, with its decimation status.  These serve
@@ -239,40 +291,12 @@ class Engine {
   int *h_count_ = nullptr;  // pinned
   int last_batch_ = 0;
   std::vector<std::unique_ptr<Engine>> lanes_;
-  // surface Green's function cache: fixed-size entries in one device buffer, LRU
-  struct SgfKey {
-    unsigned long long uid, version, ebits, dbits;
-    int slot, contact, nc;
-    bool operator==(const SgfKey &o) const {
-      return uid == o.uid && version == o.version && ebits == o.ebits && dbits == o.dbits &&
-             slot == o.slot && contact == o.contact && nc == o.nc;
-    }
-  };
-  struct SgfKeyHash {
-    size_t operator()(const SgfKey &k) const {
-      size_t h = std::hash<unsigned long long>()(k.uid * 1000003ULL ^ k.version);
-      h ^= std::hash<unsigned long long>()(k.ebits) + 0x9e3779b97f4a7c15ULL + (h << 6) + (h >> 2);
-      h ^= std::hash<unsigned long long>()(k.dbits ^ ((unsigned long long)k.slot << 20) ^
-                                          ((unsigned long long)k.contact << 8) ^ (unsigned long long)k.nc);
-      return h;
-    }
-  };
-  double2 *sgf_pool_ = nullptr;
-  size_t sgf_bytes_ = 0;
-  long long sgf_entry_elems_ = 0;
-  int sgf_capacity_ = 0;
-  std::unordered_map<SgfKey, int, SgfKeyHash> sgf_map_;
-  std::vector<SgfKey> sgf_keys_;
-  std::vector<unsigned long long> sgf_used_;  // LRU stamps
-  std::vector<int> sgf_valid_;
-  std::vector<int> sgf_status_;  // decimation status bits stored with each entry
-  unsigned long long sgf_clock_ = 0;
-  long long sgf_hits_ = 0, sgf_stores_ = 0;
-  bool sgf_prepare(long long entry_elems);
-  SgfKey sgf_key(const DevStruct &st, double E, double delta, int slot, int c, bool shared) const;
+  // surface Green's function cache (own or shared with other engines)
+  std::shared_ptr<SgfCache> sgf_;
+  SgfCache::Key sgf_key(const DevStruct &st, double E, double delta, int slot, int c,
+                        bool shared) const;
   bool sgf_shared(DevStruct &st, const SolveOpts &o);
   static long long sgf_entry_of(const DevStruct &st);  // elements per entry: max ngs^2
-  int sgf_slot_for(const SgfKey &k);  // insert (evicting LRU), returns entry index
   struct GraphEntry {
     std::vector<long long> key;
     hipGraphExec_t exec = nullptr;

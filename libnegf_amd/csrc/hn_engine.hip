@@ -212,7 +212,10 @@ Engine::Engine(int device) : device_(device) {
 }
 
 Engine &Engine::lane(int i) {
-  while ((int)lanes_.size() <= i) lanes_.emplace_back(new Engine(device_));
+  while ((int)lanes_.size() <= i) {
+    lanes_.emplace_back(new Engine(device_));
+    if (sgf_) lanes_.back()->attach_sgf_cache(sgf_);
+  }
   return *lanes_[i];
 }
 
@@ -224,7 +227,6 @@ void Engine::clear_graphs() {
 Engine::~Engine() {
   (void)hipStreamSynchronize(stream_);
   clear_graphs();
-  if (sgf_pool_) (void)hipFree(sgf_pool_);
   if (h_count_) (void)hipHostFree(h_count_);
   (void)hipStreamDestroy(stream_);
 }
@@ -483,62 +485,77 @@ void Engine::decimate_shared(int nc, int B, double2 *Ao, double2 *Aos0, double2 
 }
 
 // ---------------------------------------------------------------------------
-// Surface Green's function cache (the role of ln_cache's TMatrixCacheMem for
-// surface_green, contselfenergy.F90:138-173, 251-259): g_s of (structure
-// version, slot, E, eta, contact) kept in one device buffer of fixed-size
-// entries, least recently used out first.
+// Surface Green's function cache
 // ---------------------------------------------------------------------------
-void Engine::set_sgf_cache(size_t bytes) {
-  HN_CHECK(hipStreamSynchronize(stream_));
-  if (sgf_pool_) HN_CHECK(hipFree(sgf_pool_));
-  sgf_pool_ = nullptr;
-  sgf_bytes_ = bytes;
-  sgf_entry_elems_ = 0;
-  sgf_capacity_ = 0;
-  sgf_map_.clear();
-  sgf_keys_.clear();
-  sgf_used_.clear();
-  sgf_valid_.clear();
-  if (bytes) HN_CHECK(hipMalloc(&sgf_pool_, bytes));
+SgfCache::SgfCache(int device, size_t nbytes) : bytes(nbytes) {
+  HN_CHECK(hipSetDevice(device));
+  if (bytes) HN_CHECK(hipMalloc(&pool, bytes));
+}
+SgfCache::~SgfCache() {
+  if (pool) (void)hipFree(pool);
 }
 
-bool Engine::sgf_prepare(long long entry_elems) {
-  if (!sgf_pool_) return false;
-  if (entry_elems > sgf_entry_elems_) {  // (re)partition for larger blocks; drops the entries
-    sgf_entry_elems_ = entry_elems;
-    sgf_capacity_ = (int)std::min<size_t>(sgf_bytes_ / ((size_t)entry_elems * sizeof(double2)),
-                                          1u << 30);
-    sgf_map_.clear();
-    sgf_keys_.assign(sgf_capacity_, SgfKey{});
-    sgf_used_.assign(sgf_capacity_, 0);
-    sgf_valid_.assign(sgf_capacity_, 0);
-    sgf_status_.assign(sgf_capacity_, 0);
+bool SgfCache::prepare(long long elems) {
+  if (!pool) return false;
+  if (elems > entry_elems) {  // (re)partition for larger blocks; drops the entries
+    entry_elems = elems;
+    capacity = (int)std::min<size_t>(bytes / ((size_t)elems * sizeof(double2)), 1u << 30);
+    map.clear();
+    keys.assign(capacity, Key{});
+    used.assign(capacity, 0);
+    valid.assign(capacity, 0);
+    status.assign(capacity, 0);
   }
-  return sgf_capacity_ > 0;
+  return capacity > 0;
 }
 
-int Engine::sgf_slot_for(const SgfKey &k) {
-  auto it = sgf_map_.find(k);
-  if (it != sgf_map_.end()) {
-    sgf_used_[it->second] = ++sgf_clock_;
+int SgfCache::slot_for(const Key &k) {
+  auto it = map.find(k);
+  if (it != map.end()) {
+    used[it->second] = ++clock;
     return it->second;
   }
   int victim = 0;
-  for (int i = 0; i < sgf_capacity_; ++i) {
-    if (!sgf_valid_[i]) { victim = i; break; }
-    if (sgf_used_[i] < sgf_used_[victim]) victim = i;
+  for (int i = 0; i < capacity; ++i) {
+    if (!valid[i]) { victim = i; break; }
+    if (used[i] < used[victim]) victim = i;
   }
-  if (sgf_valid_[victim]) sgf_map_.erase(sgf_keys_[victim]);
-  sgf_keys_[victim] = k;
-  sgf_valid_[victim] = 1;
-  sgf_used_[victim] = ++sgf_clock_;
-  sgf_map_[k] = victim;
+  if (valid[victim]) map.erase(keys[victim]);
+  keys[victim] = k;
+  valid[victim] = 1;
+  used[victim] = ++clock;
+  map[k] = victim;
   return victim;
 }
 
-Engine::SgfKey Engine::sgf_key(const DevStruct &st, double E, double delta, int slot, int c,
-                               bool shared) const {
-  SgfKey k;
+void Engine::set_sgf_cache(size_t bytes) {
+  HN_CHECK(hipStreamSynchronize(stream_));
+  sgf_.reset();
+  if (bytes) sgf_ = std::make_shared<SgfCache>(device_, bytes);
+  for (auto &l : lanes_) l->attach_sgf_cache(sgf_);
+}
+
+void Engine::attach_sgf_cache(std::shared_ptr<SgfCache> c) {
+  HN_CHECK(hipStreamSynchronize(stream_));
+  sgf_ = std::move(c);
+  for (auto &l : lanes_) l->attach_sgf_cache(sgf_);
+}
+
+long long Engine::sgf_hits() const {
+  if (!sgf_) return 0;
+  std::lock_guard<std::mutex> g(sgf_->mu);
+  return sgf_->hits;
+}
+long long Engine::sgf_stores() const {
+  if (!sgf_) return 0;
+  std::lock_guard<std::mutex> g(sgf_->mu);
+  return sgf_->stores;
+}
+size_t Engine::sgf_cache_bytes() const { return sgf_ ? sgf_->bytes : 0; }
+
+SgfCache::Key Engine::sgf_key(const DevStruct &st, double E, double delta, int slot, int c,
+                              bool shared) const {
+  SgfCache::Key k;
   k.uid = st.uid;
   k.version = st.version;
   std::memcpy(&k.ebits, &E, sizeof(double));
@@ -562,27 +579,32 @@ long long Engine::sgf_entry_of(const DevStruct &st) {
 
 bool Engine::sgf_fetch(DevStruct &st, const SolveOpts &o, int slot, double E, int c, double2 *host,
                        int *status) {
-  if (c < 0 || c >= st.ncont || st.fict[c]) return false;
-  if (!sgf_pool_ || sgf_entry_elems_ < sgf_entry_of(st)) return false;
-  auto it = sgf_map_.find(sgf_key(st, E, o.delta, slot, c, sgf_shared(st, o)));
-  if (it == sgf_map_.end()) return false;
+  if (c < 0 || c >= st.ncont || st.fict[c] || !sgf_) return false;
+  SgfCache &sc = *sgf_;
+  std::lock_guard<std::mutex> g(sc.mu);
+  if (!sc.pool || sc.entry_elems < sgf_entry_of(st)) return false;
+  auto it = sc.map.find(sgf_key(st, E, o.delta, slot, c, sgf_shared(st, o)));
+  if (it == sc.map.end()) return false;
   const long long s2 = (long long)st.ngs(c) * st.ngs(c);
   HN_CHECK(hipStreamSynchronize(stream_));
-  HN_CHECK(hipMemcpy(host, sgf_pool_ + (long long)it->second * sgf_entry_elems_,
+  HN_CHECK(hipMemcpy(host, sc.pool + (long long)it->second * sc.entry_elems,
                      sizeof(double2) * s2, hipMemcpyDeviceToHost));
-  if (status) *status = sgf_status_[it->second];
+  if (status) *status = sc.status[it->second];
   return true;
 }
 
 void Engine::sgf_insert(DevStruct &st, const SolveOpts &o, int slot, double E, int c,
                         const double2 *host, int status) {
   if (c < 0 || c >= st.ncont || st.fict[c]) throw std::runtime_error("sgf_insert: not a decimated contact");
-  if (!sgf_prepare(sgf_entry_of(st))) throw std::runtime_error("sgf_insert: no surface-function cache");
-  const int idx = sgf_slot_for(sgf_key(st, E, o.delta, slot, c, sgf_shared(st, o)));
-  sgf_status_[idx] = status;
+  if (!sgf_) throw std::runtime_error("sgf_insert: no surface-function cache");
+  SgfCache &sc = *sgf_;
+  std::lock_guard<std::mutex> g(sc.mu);
+  if (!sc.prepare(sgf_entry_of(st))) throw std::runtime_error("sgf_insert: no surface-function cache");
+  const int idx = sc.slot_for(sgf_key(st, E, o.delta, slot, c, sgf_shared(st, o)));
+  sc.status[idx] = status;
   const long long s2 = (long long)st.ngs(c) * st.ngs(c);
   HN_CHECK(hipStreamSynchronize(stream_));
-  HN_CHECK(hipMemcpy(sgf_pool_ + (long long)idx * sgf_entry_elems_, host, sizeof(double2) * s2,
+  HN_CHECK(hipMemcpy(sc.pool + (long long)idx * sc.entry_elems, host, sizeof(double2) * s2,
                      hipMemcpyHostToDevice));
 }
 
@@ -625,11 +647,11 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
   for (int c = 0; c < ncont; ++c)
     if (!st.fict[c]) ncmax = std::max(ncmax, st.ngs(c));
   // (complex per-item energies are not cached: keys hold the real energy only)
-  const bool cache = o.sgf_mode != 0 && !o.dEi && ncmax > 0 && sgf_prepare((long long)ncmax * ncmax);
+  const bool want = o.sgf_mode != 0 && !o.dEi && ncmax > 0 && sgf_;
   std::vector<double> hE;
   std::vector<int> hS;
   int *didx = nullptr;
-  if (cache) {
+  if (want) {
     hE.resize(B);
     hS.resize(B);
     HN_CHECK(hipMemcpyAsync(hE.data(), dE, sizeof(double) * B, hipMemcpyDeviceToHost, stream_));
@@ -638,30 +660,35 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     didx = arena_.take<int>(B);
   }
   auto key = [&](int b, int c) { return sgf_key(st, hE[b], o.delta, hS[b], c, shared); };
-  if (cache && o.sgf_mode == 1) {  // ReadOldSGF = 0: load when every item is cached
-    bool all = true;
+  if (want && o.sgf_mode == 1) {  // ReadOldSGF = 0: load when every item is cached
+    SgfCache &sc = *sgf_;
+    std::unique_lock<std::mutex> lk(sc.mu);
+    bool all = sc.prepare((long long)ncmax * ncmax);
     for (int c = 0; c < ncont && all; ++c)
       if (!st.fict[c])
-        for (int b = 0; b < B && all; ++b) all = sgf_map_.count(key(b, c)) > 0;
+        for (int b = 0; b < B && all; ++b) all = sc.map.count(key(b, c)) > 0;
     if (all) {
       size_t mark = arena_.used;
-      double2 *Go = arena_.take<double2>((long long)ncmax * ncmax * B);
+      double2 *Go = arena_.take<double2>((long long)ncmax * ncmax * B * ncont);
       std::vector<int> hst((size_t)B * ncont, 0), idx(B);
       for (int c = 0; c < ncont; ++c) {
         const long long s2 = (long long)st.ngs(c) * st.ngs(c);
-        if (st.fict[c]) {
-          fictitious(c, Go);
-        } else {
-          for (int b = 0; b < B; ++b) {
-            idx[b] = sgf_slot_for(key(b, c));  // a hit: refreshes the entry's LRU stamp
-            hst[(size_t)c * B + b] = sgf_status_[idx[b]];
-          }
-          HN_CHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
-          copy_indexed(Go, s2, nullptr, sgf_pool_, sgf_entry_elems_, didx, s2, B, stream_);
-          sgf_hits_ += B;
+        double2 *Gc = Go + (long long)c * ncmax * ncmax * B;
+        if (st.fict[c]) continue;
+        for (int b = 0; b < B; ++b) {
+          idx[b] = sc.slot_for(key(b, c));  // a hit: refreshes the entry's LRU stamp
+          hst[(size_t)c * B + b] = sc.status[idx[b]];
         }
-        self_energy(c, Go);
+        HN_CHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
+        copy_indexed(Gc, s2, nullptr, sc.pool, sc.entry_elems, didx, s2, B, stream_);
         HN_CHECK(hipStreamSynchronize(stream_));  // idx is reused by the next contact
+        sc.hits += B;
+      }
+      lk.unlock();  // the surface functions are copied out
+      for (int c = 0; c < ncont; ++c) {
+        double2 *Gc = Go + (long long)c * ncmax * ncmax * B;
+        if (st.fict[c]) fictitious(c, Gc);
+        self_energy(c, Gc);
       }
       // loaded surface functions report ncyc = 0 (contselfenergy.F90:158, 259)
       HN_CHECK(hipMemcpyAsync(status_c, hst.data(), sizeof(int) * B * ncont, hipMemcpyHostToDevice,
@@ -672,21 +699,24 @@ void Engine::contacts(DevStruct &st, int B, const double *dE, const int *dslot,
     }
   }
   auto store = [&](int c, const double2 *Go) {
-    if (!cache || st.fict[c]) return;
+    if (!want || st.fict[c]) return;
+    SgfCache &sc = *sgf_;
     const long long s2 = (long long)st.ngs(c) * st.ngs(c);
     std::vector<int> hs(B), idx(B, -1);
     HN_CHECK(hipMemcpyAsync(hs.data(), status_c + (long long)c * B, sizeof(int) * B,
                             hipMemcpyDeviceToHost, stream_));
     HN_CHECK(hipStreamSynchronize(stream_));
+    std::lock_guard<std::mutex> g(sc.mu);
+    if (!sc.prepare((long long)ncmax * ncmax)) return;
     int stored = 0;
-    for (int b = 0; b < B && stored < sgf_capacity_; ++b, ++stored) {
-      idx[b] = sgf_slot_for(key(b, c));
-      sgf_status_[idx[b]] = hs[b];
+    for (int b = 0; b < B && stored < sc.capacity; ++b, ++stored) {
+      idx[b] = sc.slot_for(key(b, c));
+      sc.status[idx[b]] = hs[b];
     }
     HN_CHECK(hipMemcpyAsync(didx, idx.data(), sizeof(int) * B, hipMemcpyHostToDevice, stream_));
-    copy_indexed(sgf_pool_, sgf_entry_elems_, didx, Go, s2, nullptr, s2, B, stream_);
+    copy_indexed(sc.pool, sc.entry_elems, didx, Go, s2, nullptr, s2, B, stream_);
     HN_CHECK(hipStreamSynchronize(stream_));
-    sgf_stores_ += stored;
+    sc.stores += stored;
   };
 
   if (shared) {

@@ -1,6 +1,7 @@
 // hn_serve.hip -- persistent request loop (see hn_serve.h).
 #include <algorithm>
 #include <cmath>
+#include <cstring>
 
 #include "hn_gen.h"
 #include "hn_serve.h"
@@ -39,6 +40,11 @@ Server::Server(int device, const ServeOpts &o) : device_(device), o_(o) {
   if (!o_.worker_ws) o_.worker_ws = std::min<size_t>(budget_ / (4 * (size_t)o_.workers), 32ull << 30);
   if (budget_ <= o_.worker_ws * (size_t)o_.workers) throw std::runtime_error("server: HBM budget too small");
   budget_ -= o_.worker_ws * (size_t)o_.workers;  // the rest holds request structures and buffers
+  if (o_.sgf_cache) {
+    if (budget_ <= o_.sgf_cache) throw std::runtime_error("server: HBM budget too small for the cache");
+    sgf_ = std::make_shared<SgfCache>(device_, o_.sgf_cache);
+    budget_ -= o_.sgf_cache;
+  }
   for (int w = 0; w < o_.workers; ++w) threads_.emplace_back(&Server::worker, this, w);
 }
 
@@ -54,6 +60,10 @@ Server::~Server() {
     Req &r = *kv.second;
     if (r.mem) (void)hipFree(r.mem);
   }
+  for (auto &kv : residents_) {
+    kv.second->st.reset();
+    if (kv.second->mem) (void)hipFree(kv.second->mem);
+  }
 }
 
 unsigned long long Server::submit(const ReqStruct &rs, const std::vector<double> &E,
@@ -68,12 +78,18 @@ unsigned long long Server::submit(const ReqStruct &rs, const std::vector<double>
   r->E = E;
   r->o = o;
   r->o.lanes = 1;  // the workers themselves are the concurrent streams
+  if (sgf_ && r->o.sgf_mode == 0) r->o.sgf_mode = 1;  // load if cached, else compute and store
   r->nE = (int)E.size();
   r->t_submit = clk::now();
   std::lock_guard<std::mutex> g(mu_);
   if (stop_) throw std::runtime_error("server: stopped");
   r->id = next_id_++;
   r->footprint = footprint(*r);
+  if (rs.gen) {
+    SlabShape sh(rs.n, rs.nbl, rs.overlap);
+    r->struct_bytes = align256(DevStruct::layout_bytes(rs.nbl, sh.bsz.data(), 2, sh.cblk.data(),
+                                                       sh.nc.data(), sh.smode, 1));
+  }
   Req *p = r.get();
   reqs_[p->id] = std::move(r);
   queue_.push_back(p);
@@ -83,37 +99,50 @@ unsigned long long Server::submit(const ReqStruct &rs, const std::vector<double>
 }
 
 size_t Server::footprint(const Req &r) const {
-  size_t b = 0;
-  int ncont = 2;
-  if (r.rs.gen) {
-    SlabShape sh(r.rs.n, r.rs.nbl, r.rs.overlap);
-    b += align256(DevStruct::layout_bytes(r.rs.nbl, sh.bsz.data(), 2, sh.cblk.data(), sh.nc.data(),
-                                          sh.smode, 1));
-  } else {
-    ncont = std::max(r.rs.resident->ncont, 1);
-  }
+  const int ncont = r.rs.gen ? 2 : std::max(r.rs.resident->ncont, 1);
   const size_t n = (size_t)r.nE;
-  b += align256(n * sizeof(double)) * 2 + align256(n * sizeof(int) * ncont) +
-       2 * align256(n * sizeof(int));
-  return b;
+  return align256(n * sizeof(double)) * 2 + align256(n * sizeof(int) * ncont) +
+         2 * align256(n * sizeof(int));
 }
 
-void Server::admit(Req &r, Engine &eng) {
+Server::GenKey Server::gen_key(const ReqStruct &rs) {
+  GenKey k;
+  k.seed = rs.seed;
+  std::memcpy(&k.kx, &rs.kx, sizeof(double));
+  std::memcpy(&k.ky, &rs.ky, sizeof(double));
+  k.n = rs.n;
+  k.nbl = rs.nbl;
+  k.overlap = rs.overlap;
+  return k;
+}
+
+void Server::evict_idle(size_t need, std::vector<std::unique_ptr<Resident>> &freed) {
+  while (used_ + need > budget_) {
+    auto victim = residents_.end();
+    for (auto it = residents_.begin(); it != residents_.end(); ++it)
+      if (it->second->ready && it->second->refs == 0 &&
+          (victim == residents_.end() || it->second->last < victim->second->last))
+        victim = it;
+    if (victim == residents_.end()) return;
+    used_ -= victim->second->bytes;
+    freed.push_back(std::move(victim->second));
+    residents_.erase(victim);
+  }
+}
+
+void Server::admit(Req &r, Engine &eng, bool build) {
   hipStream_t s = eng.stream();
+  if (build) {
+    SlabShape sh(r.rs.n, r.rs.nbl, r.rs.overlap);
+    Resident &g = *r.gst;
+    HN_CHECK(hipMalloc(&g.mem, g.bytes));
+    g.st.reset(new DevStruct(r.rs.nbl, sh.bsz.data(), 2, sh.cblk.data(), sh.nc.data(), sh.smode, 1,
+                             g.mem, s));
+    gen_kslab(*g.st, 0, r.rs.seed, r.rs.kx, r.rs.ky, r.rs.overlap, s);
+  }
   HN_CHECK(hipMallocAsync(&r.mem, r.footprint, s));
   char *p = (char *)r.mem;
-  if (r.rs.gen) {
-    SlabShape sh(r.rs.n, r.rs.nbl, r.rs.overlap);
-    const size_t sb = DevStruct::layout_bytes(r.rs.nbl, sh.bsz.data(), 2, sh.cblk.data(),
-                                              sh.nc.data(), sh.smode, 1);
-    r.owned.reset(new DevStruct(r.rs.nbl, sh.bsz.data(), 2, sh.cblk.data(), sh.nc.data(), sh.smode, 1,
-                                p, s));
-    p += align256(sb);
-    gen_kslab(*r.owned, 0, r.rs.seed, r.rs.kx, r.rs.ky, r.rs.overlap, s);
-    r.st = r.owned.get();
-  } else {
-    r.st = r.rs.resident;
-  }
+  r.st = r.rs.gen ? r.gst->st.get() : r.rs.resident;
   r.ncont = std::max(r.st->ncont, 1);
   const size_t n = (size_t)r.nE;
   r.dE = (double *)p; p += align256(n * sizeof(double));
@@ -122,7 +151,7 @@ void Server::admit(Req &r, Engine &eng) {
   r.dS = (int *)p; p += align256(n * sizeof(int));
   r.dSl = (int *)p;
   HN_CHECK(hipMemcpyAsync(r.dE, r.E.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-  fill_int(r.dSl, r.nE, r.rs.slot, s);
+  fill_int(r.dSl, r.nE, r.rs.gen ? 0 : r.rs.slot, s);
   HN_CHECK(hipStreamSynchronize(s));
   {
     std::lock_guard<std::mutex> g(g_prepare_mu);
@@ -157,7 +186,6 @@ void Server::finish(Req &r, Engine &eng) {
   if (r.mem) HN_CHECK(hipFreeAsync(r.mem, s));
   HN_CHECK(hipStreamSynchronize(s));
   r.mem = nullptr;
-  r.owned.reset();
   r.st = nullptr;
   r.res.t_latency = secs(r.t_submit, clk::now());
 }
@@ -167,6 +195,7 @@ void Server::worker(int w) {
   std::unique_ptr<Engine> eng;
   try {
     eng.reset(new Engine(device_));
+    if (sgf_) eng->attach_sgf_cache(sgf_);
   } catch (const std::exception &e) {
     std::fprintf(stderr, "hipnegf server: worker start failed: %s\n", e.what());
     return;
@@ -176,18 +205,58 @@ void Server::worker(int w) {
     // ---- admission (FCFS, by HBM footprint) ----
     if (!stop_ && !queue_.empty()) {
       Req *r = queue_.front();
-      const bool fits = used_ + r->footprint <= budget_;
-      const bool never = r->footprint > budget_;
+      Resident *res = nullptr;
+      bool build = false, wait_build = false;
+      size_t need = r->footprint;
+      if (r->rs.gen) {
+        auto it = residents_.find(gen_key(r->rs));
+        if (it != residents_.end()) {
+          res = it->second.get();
+          wait_build = !res->ready;  // another worker is generating it
+        } else {
+          build = true;
+          need += r->struct_bytes;
+        }
+      }
+      std::vector<std::unique_ptr<Resident>> freed;
+      if (!wait_build && used_ + need > budget_) evict_idle(need, freed);
+      const bool never = need > budget_;
+      const bool fits = !wait_build && used_ + need <= budget_;
+      if (!freed.empty()) {  // device memory of evicted structures, outside the lock
+        lk.unlock();
+        for (auto &f : freed) {
+          f->st.reset();
+          if (f->mem) (void)hipFree(f->mem);
+        }
+        freed.clear();
+        lk.lock();
+        continue;  // re-examine the queue
+      }
       if (fits || never) {
         queue_.pop_front();
         r->state = ADMITTING;
-        if (!never) used_ += r->footprint;
+        if (!never) {
+          used_ += need;
+          if (build) {
+            auto nr = std::make_unique<Resident>();
+            nr->bytes = r->struct_bytes;
+            res = nr.get();
+            residents_[gen_key(r->rs)] = std::move(nr);
+          } else if (res) {
+            ++struct_hits_;
+          }
+          if (res) {
+            ++res->refs;
+            res->last = ++res_clock_;
+          }
+          r->gst = res;
+        }
         lk.unlock();
         std::string err;
         if (never) err = "request does not fit the HBM budget";
         else {
           try {
-            admit(*r, *eng);
+            admit(*r, *eng, build);
           } catch (const std::exception &e) {
             err = e.what();
           }
@@ -199,8 +268,24 @@ void Server::worker(int w) {
         lk.lock();
         r->t_admit = clk::now();
         r->res.t_queue = secs(r->t_submit, r->t_admit);
+        if (build && r->gst) r->gst->ready = err.empty();
         if (!err.empty()) {
-          if (!never) used_ -= r->footprint;
+          if (!never) {
+            used_ -= r->footprint;
+            if (r->gst) {
+              --r->gst->refs;
+              if (build) {  // drop the structure that failed to build
+                auto it = residents_.find(gen_key(r->rs));
+                std::unique_ptr<Resident> gone = std::move(it->second);
+                residents_.erase(it);
+                used_ -= gone->bytes;
+                lk.unlock();
+                gone->st.reset();
+                if (gone->mem) (void)hipFree(gone->mem);
+                lk.lock();
+              }
+            }
+          }
           r->state = DONE;
           ++n_completed_;
           cv_done_.notify_all();
@@ -253,8 +338,24 @@ void Server::worker(int w) {
         }
         lk.lock();
         used_ -= best->footprint;
+        Resident *g = best->gst;
+        best->gst = nullptr;
         best->state = DONE;
         ++n_completed_;
+        if (g) {
+          --g->refs;
+          g->last = ++res_clock_;
+          if (!o_.keep_structs && g->refs == 0) {  // not kept: free it now
+            auto it = residents_.find(gen_key(best->rs));
+            std::unique_ptr<Resident> gone = std::move(it->second);
+            residents_.erase(it);
+            used_ -= gone->bytes;
+            lk.unlock();
+            gone->st.reset();
+            if (gone->mem) (void)hipFree(gone->mem);
+            lk.lock();
+          }
+        }
         cv_done_.notify_all();
         cv_work_.notify_all();
       }
@@ -282,6 +383,19 @@ ReqResult Server::wait(unsigned long long id) {
   ReqResult out = std::move(r->res);
   reqs_.erase(it);
   return out;
+}
+
+void Server::cache_stats(long long out[4]) {
+  std::lock_guard<std::mutex> g(mu_);
+  out[0] = struct_hits_;
+  out[1] = (long long)residents_.size();
+  out[2] = 0;
+  out[3] = 0;
+  if (sgf_) {
+    std::lock_guard<std::mutex> gc(sgf_->mu);
+    out[2] = sgf_->hits;
+    out[3] = sgf_->stores;
+  }
 }
 
 void Server::stats(long long out[4]) {

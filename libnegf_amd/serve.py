@@ -18,7 +18,8 @@ from . import hip
 class ServerOpts(C.Structure):
     _fields_ = [("workers", C.c_int), ("policy", C.c_int), ("max_chunk", C.c_int),
                 ("reserved", C.c_int), ("chunk_flops", C.c_double), ("mem_budget", C.c_longlong),
-                ("worker_workspace", C.c_longlong)]
+                ("worker_workspace", C.c_longlong), ("sgf_cache", C.c_longlong),
+                ("keep_structures", C.c_int), ("reserved2", C.c_int)]
 
 
 FCFS, LAS = 0, 1
@@ -36,6 +37,7 @@ def _lib():
         L.hipnegf_server_poll.argtypes = [vp, u64, C.POINTER(i)]
         L.hipnegf_server_wait.argtypes = [vp, u64, vp, vp, vp, vp]
         L.hipnegf_server_stats.argtypes = [vp, vp]
+        L.hipnegf_server_cache_stats.argtypes = [vp, vp]
         L.hipnegf_struct_generate_kslab.argtypes = [vp, vp, i, u64, d, d, i]
         L.hipnegf_struct_get_block.argtypes = [vp, i, i, i, vp]
         L._serve_bound = True
@@ -44,9 +46,11 @@ def _lib():
 
 class Server:
     def __init__(self, device: int = 0, workers: int = 2, policy: int = FCFS, max_chunk: int = 0,
-                 chunk_flops: float = 0.0, mem_budget: int = 0, worker_workspace: int = 0):
+                 chunk_flops: float = 0.0, mem_budget: int = 0, worker_workspace: int = 0,
+                 sgf_cache: int = 0, keep_structures: bool = True):
         L = _lib()
-        o = ServerOpts(workers, policy, max_chunk, 0, chunk_flops, mem_budget, worker_workspace)
+        o = ServerOpts(workers, policy, max_chunk, 0, chunk_flops, mem_budget, worker_workspace,
+                       sgf_cache, 1 if keep_structures else 0, 0)
         h = C.c_void_p()
         hip.check(L.hipnegf_server_create(device, C.byref(o), C.byref(h)))
         self._h = h
@@ -64,9 +68,12 @@ class Server:
             pass
 
     def submit_kslab(self, seed: int, n: int, nbl: int, E: np.ndarray, k=(0.0, 0.0),
-                     overlap: bool = True, delta: float = 1e-5) -> int:
+                     overlap: bool = True, delta: float = 1e-5, precision: int = hip.FP64,
+                     ns_steps: int = 0) -> int:
+        """precision=hip.MIXED: FP32 sweep with ns_steps FP64 Newton-Schulz
+        steps per block inverse (the decimation stays FP64)."""
         E = np.ascontiguousarray(E, dtype=np.float64)
-        o = hip._opts(delta, 1, 2, hip.FP64, 0)
+        o = hip._opts(delta, 1, 2, precision, 0, ns_steps=ns_steps)
         rid = C.c_ulonglong()
         hip.check(_lib().hipnegf_server_submit_kslab(self._h, seed, n, nbl, float(k[0]), float(k[1]),
                                                      1 if overlap else 0, len(E), hip._ptr(E),
@@ -99,6 +106,12 @@ class Server:
                                              hip._ptr(t)))
         return dict(T=T, ncyc=ncyc, status=st, t_queue=t[0], latency=t[1], t_gpu=t[2],
                     chunks=int(t[3]))
+
+    def cache_stats(self) -> dict:
+        out = np.zeros(4, dtype=np.int64)
+        hip.check(_lib().hipnegf_server_cache_stats(self._h, hip._ptr(out)))
+        return dict(structure_hits=int(out[0]), resident_structures=int(out[1]),
+                    sgf_hits=int(out[2]), sgf_stores=int(out[3]))
 
     def stats(self) -> dict:
         out = np.zeros(4, dtype=np.int64)

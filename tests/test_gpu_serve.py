@@ -110,3 +110,76 @@ def test_admission_rejects_oversized(gpu):
     with pytest.raises(hip.HipNegfError, match="HBM budget"):
         srv.wait(rid)
     srv.close()
+
+
+def test_alone_equals_mix_large_blocks_and_deep_stacks(gpu):
+    """Requests at the ends of the serving distribution -- a 512-orbital PL
+    and a 160-PL stack -- with two middle-sized ones: alone, then all four
+    submitted together, bitwise the same T(E) and decimation counts."""
+    hip, serve, eng = gpu
+    reqs = [dict(seed=811, n=512, nbl=10, E=np.linspace(-1.2, 1.3, 6), k=(0.4, -0.7)),
+            dict(seed=812, n=32, nbl=160, E=np.linspace(-1.5, 1.5, 12), k=(1.1, 0.2)),
+            dict(seed=813, n=128, nbl=20, E=np.linspace(-1.0, 1.0, 10), k=(-2.0, 0.9)),
+            dict(seed=814, n=64, nbl=40, E=np.linspace(-0.8, 1.6, 16), k=(0.0, 2.5))]
+    srv = serve.Server(workers=3, max_chunk=4)
+    alone = [srv.wait(srv.submit_kslab(r["seed"], r["n"], r["nbl"], r["E"], k=r["k"])) for r in reqs]
+    ids = [srv.submit_kslab(r["seed"], r["n"], r["nbl"], r["E"], k=r["k"]) for r in reqs]
+    mixed = [srv.wait(i) for i in ids]
+    srv.close()
+    for a, m in zip(alone, mixed):
+        assert (a["status"] == 0).all() and (m["status"] == 0).all()
+        np.testing.assert_array_equal(a["T"], m["T"])
+        np.testing.assert_array_equal(a["ncyc"], m["ncyc"])
+
+
+def test_resident_structures_and_shared_surface_cache(gpu):
+    """A generated structure stays resident after its request; the workers
+    share one surface-function cache, so the same request again is served
+    from the resident structure with every g_s loaded (ncyc = 0, as for
+    ReadOldSGF loads) and the same T; a new grid on the structure computes
+    (and T equals the direct entry point's).  Without keep_structures the
+    structure is freed with its last request."""
+    hip, serve, eng = gpu
+    seed, n, nbl, k = 901, 48, 6, (0.6, -1.4)
+    E = np.linspace(-1.4, 1.4, 20)
+    srv = serve.Server(workers=2, max_chunk=8, sgf_cache=1 << 30)
+    r1 = srv.wait(srv.submit_kslab(seed, n, nbl, E, k=k))
+    c1 = srv.cache_stats()
+    r2 = srv.wait(srv.submit_kslab(seed, n, nbl, E, k=k))
+    c2 = srv.cache_stats()
+    E3 = np.linspace(-0.9, 0.7, 11)
+    r3 = srv.wait(srv.submit_kslab(seed, n, nbl, E3, k=k))
+    srv.close()
+    assert (r1["status"] == 0).all() and (r2["status"] == 0).all() and (r3["status"] == 0).all()
+    assert c1["resident_structures"] == 1 and c1["structure_hits"] == 0
+    assert c1["sgf_stores"] == 2 * len(E) and c1["sgf_hits"] == 0
+    np.testing.assert_array_equal(r2["T"], r1["T"])
+    assert (r2["ncyc"] == 0).all() and (r1["ncyc"] > 0).all()
+    assert c2["structure_hits"] == 1 and c2["sgf_hits"] == 2 * len(E)
+    ds, _ = serve.kslab_structure(eng, seed, n, nbl, k=k)
+    for r, grid in ((r1, E), (r3, E3)):
+        T, ncyc, _ = hip.solve_transmission(eng, ds, grid, delta=1e-5)
+        np.testing.assert_array_equal(r["T"], T)
+        np.testing.assert_array_equal(r["ncyc"], ncyc)
+    srv = serve.Server(workers=1, keep_structures=False)
+    srv.wait(srv.submit_kslab(seed, n, nbl, E3, k=k))
+    assert srv.cache_stats()["resident_structures"] == 0
+    srv.close()
+
+
+def test_mixed_precision_request(gpu):
+    """A request in mixed precision (FP32 sweep, two FP64 Newton-Schulz steps
+    per block inverse, FP64 decimation) against the same request in FP64:
+    within the serving tolerance of 1e-8 relative."""
+    hip, serve, eng = gpu
+    seed, n, nbl, k = 931, 96, 8, (0.2, 0.9)
+    E = np.linspace(-1.2, 1.2, 16)
+    srv = serve.Server(workers=2)
+    r64 = srv.wait(srv.submit_kslab(seed, n, nbl, E, k=k))
+    rmx = srv.wait(srv.submit_kslab(seed, n, nbl, E, k=k, precision=hip.MIXED, ns_steps=2))
+    srv.close()
+    assert (r64["status"] == 0).all() and (rmx["status"] == 0).all()
+    big = r64["T"] > 1e-6 * r64["T"].max()
+    rel = np.abs(rmx["T"] - r64["T"])[big] / r64["T"][big]
+    assert rel.max() <= 1e-8, rel.max()
+    np.testing.assert_array_equal(rmx["ncyc"], r64["ncyc"])
