@@ -56,8 +56,15 @@ def parse():
                     help="only run register-only FP64 MFMA issue loops (PMC utilisation calibration)")
     ap.add_argument("--serve", action="store_true",
                     help="serving workload: Poisson request stream through the server loop")
-    ap.add_argument("--requests", type=int, default=24, help="requests per serving pass")
-    ap.add_argument("--rates", default="0.5,1.0,1.5",
+    ap.add_argument("--requests", type=int, default=50, help="requests per serving pass")
+    ap.add_argument("--serve-precision", default="mixed", choices=["fp64", "mixed"],
+                    help="request precision in the serving passes (mixed: FP32 sweep, 2 FP64 "
+                         "Newton-Schulz steps per block inverse)")
+    ap.add_argument("--sgf-cache-gb", type=float, default=4.0,
+                    help="surface-function cache shared by the serving workers (GiB)")
+    ap.add_argument("--repeat", type=float, default=0.0,
+                    help="fraction of serving requests that repeat an earlier request of the pass")
+    ap.add_argument("--rates", default="0.8,1.0",
                     help="arrival rates as multiples of the measured saturation rate")
     ap.add_argument("--workers", type=int, default=3)
     ap.add_argument("--policy", type=int, default=0, help="0 FCFS, 1 least attained service")
@@ -169,14 +176,26 @@ def bench_serve(a, hip, wl, torch):
     ner = tuple(int(x) for x in a.ne_range.split(","))
     eta = 1e-5
 
+    prec = hip.MIXED if a.serve_precision == "mixed" else hip.FP64
+    ns = 2 if prec == hip.MIXED else 0
+
     def specs(base):
-        return [wl.serving_spec(base + i, nch, nblr, ner) for i in range(a.requests)]
+        """(seed, n, nbl, nE, kx, ky) per request; with --repeat a request may
+        repeat an earlier one of the pass (same structure and grid)."""
+        rng = np.random.default_rng(base)
+        out = []
+        for i in range(a.requests):
+            if out and rng.random() < a.repeat:
+                out.append(out[int(rng.integers(len(out)))])
+            else:
+                out.append((base + i,) + tuple(wl.serving_spec(base + i, nch, nblr, ner)))
+        return out
 
     def flops(sp):
-        n, nbl, nE = sp[0], sp[1], sp[2]
+        n, nbl, nE = sp[1], sp[2], sp[3]
         return (56.0 * 2 * 25 + 24.0 * nbl + 64.0) * n ** 3 * nE
 
-    srv = serve.Server(workers=a.workers, policy=a.policy)
+    srv = serve.Server(workers=a.workers, policy=a.policy, sgf_cache=int(a.sgf_cache_gb * (1 << 30)))
     # warm-up: one small request per PL size (first-use costs: code objects, workspaces)
     for n in nch:
         srv.wait(srv.submit_kslab(1, n, 4, np.linspace(-1, 1, 8), delta=eta))
@@ -187,14 +206,14 @@ def bench_serve(a, hip, wl, torch):
         def submitter():
             t0 = time.perf_counter()
             t_next = t0
-            for i, (n, nbl, nE, kx, ky) in enumerate(sp):
+            for i, (seed, n, nbl, nE, kx, ky) in enumerate(sp):
                 t_next += gaps[i]
                 d = t_next - time.perf_counter()
                 if d > 0:
                     time.sleep(d)
                 t_sub[i] = time.perf_counter()
-                ids[i] = srv.submit_kslab(base + i, n, nbl, wl.energy_grid(nE, -2.0, 2.0), k=(kx, ky),
-                                          delta=eta)
+                ids[i] = srv.submit_kslab(seed, n, nbl, wl.energy_grid(nE, -2.0, 2.0), k=(kx, ky),
+                                          delta=eta, precision=prec, ns_steps=ns)
 
         th = threading.Thread(target=submitter)
         t0 = time.perf_counter()
@@ -204,16 +223,23 @@ def bench_serve(a, hip, wl, torch):
             while ids[i] is None:
                 time.sleep(0.001)
             res.append(srv.wait(ids[i]))
-            log(f"  request {i}: n={sp[i][0]} nbl={sp[i][1]} nE={sp[i][2]} latency {res[-1]['latency']:.2f}s "
+            log(f"  request {i}: n={sp[i][1]} nbl={sp[i][2]} nE={sp[i][3]} latency {res[-1]['latency']:.2f}s "
                 f"(queue {res[-1]['t_queue']:.2f}s, {res[-1]['chunks']} chunks)")
         th.join()
         wall = time.perf_counter() - t0
         lat = np.array([r["latency"] for r in res])
-        pts = sum(s[2] for s in sp)
+        pts = sum(s[3] for s in sp)
         bad = sum(int((r["status"] != 0).sum()) for r in res)
+        c0 = dict(cache_before)
+        cache_before.update(srv.cache_stats())
+        cache = {k: cache_before[k] - c0.get(k, 0) for k in ("structure_hits", "sgf_hits", "sgf_stores")}
         return dict(wall=wall, p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                    p99_over_p50=float(np.percentile(lat, 99) / np.percentile(lat, 50)),
                     mean=float(lat.mean()), points_per_s=pts / wall, requests=len(sp),
-                    tflops_algorithmic=sum(flops(s) for s in sp) / wall / 1e12, bad_items=bad)
+                    tflops_algorithmic=sum(flops(s) for s in sp) / wall / 1e12, bad_items=bad,
+                    cache=cache)
+
+    cache_before = dict(srv.cache_stats())
 
     sp = specs(100_000)
     burst = run(sp, 100_000, [0.0] * len(sp))
@@ -234,11 +260,14 @@ def bench_serve(a, hip, wl, torch):
     srv.close()
     out = {"metric": "RGF energy-points/s", "value": burst["points_per_s"], "unit": "energy-points/s",
            "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": burst["wall"] * 1e3,
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "c64+c128" if prec == hip.MIXED else "c128",
            "data": "synthetic (per-request seeds; slabs generated on the device)",
            "config": dict(workload="c5_serving_poisson", requests_per_pass=a.requests,
                           n_choices=list(nch), nbl_range=list(nblr), ne_range=list(ner),
-                          workers=a.workers, policy=["fcfs", "las"][a.policy]),
+                          workers=a.workers, policy=["fcfs", "las"][a.policy],
+                          precision=a.serve_precision, ns_steps=ns, sgf_cache_gb=a.sgf_cache_gb,
+                          repeat=a.repeat),
            "saturation": burst, "latency_by_rate": rates}
     print(json.dumps(out), flush=True)
 
