@@ -309,6 +309,31 @@ def test_mixed_precision_modes(gpu):
     assert errs[2] <= errs[1] + 1e-12
 
 
+def test_mixed_precision_against_oracle(gpu):
+    """The mixed mode used in serving (FP32 sweep, two FP64 Newton-Schulz
+    steps per block inverse, FP64 decimation) against the FP64 oracle on a
+    k-slab with overlap at eta = 1e-5: the mixed result is within the 1e-8
+    tolerance of the serving workload of the FP64 result, and adds at most
+    1e-8 to the FP64 engine's own distance from the oracle (which, at the
+    ill-conditioned energies, is the FP64 rounding floor of the problem)."""
+    hip, eng = gpu
+    s = wl.KSlab.make(n=128, nbl=8, seed=29).at_k(-0.4, 1.7)
+    E = np.linspace(-1.4, 1.4, 15)
+    ds = hip.DeviceStructure(s)
+    Tm, nm, st = hip.solve_transmission(eng, ds, E, delta=1e-5, precision=hip.MIXED, ns_steps=2)
+    T64, _, _ = hip.solve_transmission(eng, ds, E, delta=1e-5)
+    Tref, nref = _oracle(s, E, 1e-5)
+    assert (st == 0).all()
+    np.testing.assert_array_equal(nm, nref)
+    big = Tref > 1e-6 * Tref.max()
+    e_mx = float((np.abs(Tm - Tref)[big] / Tref[big]).max())
+    e_64 = float((np.abs(T64 - Tref)[big] / Tref[big]).max())
+    e_mf = float((np.abs(Tm - T64)[big] / T64[big]).max())
+    print("max rel err vs oracle: mixed", e_mx, "FP64", e_64, "; mixed vs FP64", e_mf)
+    assert e_mf <= 1e-8
+    assert e_mx <= e_64 + 1e-8
+
+
 def test_sgf_cache_load_and_store(gpu):
     """Surface Green's function cache (ReadOldSGF semantics): a LOAD solve on
     energies stored before reproduces T bitwise, reports ncyc 0 for the loaded
