@@ -2,14 +2,18 @@
 //
 // Complex products use four real MFMAs:  Cr += Ar Br - Ai Bi,
 // Ci += Ar Bi + Ai Br.  K is walked in steps of 16 complex elements staged
-// through LDS as separate real/imaginary planes (k-major).  The LDS stage is
-// double-buffered: while the matrix cores work on step t from one buffer,
-// step t+1 (prefetched into registers during step t-1) is stored into the
-// other and step t+2 is fetched from memory -- one barrier per K step.
+// through LDS as separate real/imaginary planes (k-major).  In the wide
+// kernel the LDS stage is double-buffered: while the matrix cores work on
+// step t from one buffer, step t+1 (prefetched into registers during step
+// t-1) is stored into the other and step t+2 is fetched from memory -- one
+// barrier per K step.  The narrow kernel (64 x 64, the small blocks) keeps
+// one stage and fetches its C tile in the epilogue: 33 KiB of LDS and 126
+// VGPRs, so four workgroups share a CU (1.1x faster at n = 64 than two
+// double-buffered ones).
 //
 // FP64 uses v_mfma_f64_4x4x4 (four independent 4x4x4 blocks per
-// instruction), which sustains ~72 TF/s on MI355X against ~49 TF/s for
-// v_mfma_f64_16x16x4 (measured with register-only issue loops).  Lane
+// instruction), which sustains 67 TF/s on MI355X against 46 TF/s for
+// v_mfma_f64_16x16x4 (register-only issue loops).  Lane
 // t = 16k + 4b + a supplies A_b(a, k) and lane t = 16k + 4b + c supplies
 // B_b(k, c); lane 16a + 4b + c receives D_b(a, c).  Taking A_b = rows
 // 4b..4b+3 of a 16-row strip and the same four columns as B_b for every
@@ -38,7 +42,7 @@ namespace hn {
 namespace {
 
 constexpr int BM = 64, BN = 64, BK = 16, NT = 256;
-constexpr int PAD = 16;  // row padding (elements) of the LDS planes
+constexpr int PAD = 2;  // row padding (elements) of the LDS planes
 
 template <typename R> struct V4;
 template <> struct V4<double> { typedef double __attribute__((ext_vector_type(4))) t; };
@@ -75,9 +79,6 @@ template <typename R> struct Smem {
   R Br[BK][BN + PAD];
   R Bi[BK][BN + PAD];
 };
-template <typename R> struct Smem2 {
-  Smem<R> buf[2];
-};
 
 // USE only separates the uses (masked, sweep, inverse update) in profiles.
 template <typename R, int USE>
@@ -96,7 +97,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   if ((m0 >= p.skip_r0 && m0 + BM <= p.skip_r1) || (n0 >= p.skip_c0 && n0 + BN <= p.skip_c1))
     return;  // wholly inside a skipped range
 
-  __shared__ Smem2<R> sm2;
+  __shared__ Smem<R> sm1;
 
   const CT *A = p.A + (long long)b * p.sA;
   const CT *B = p.B + (long long)b * p.sB;
@@ -165,9 +166,8 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
       ci[i][j] = acc_t{0};
     }
 
-  // C values of this thread's epilogue elements: the first pass's fetched
-  // before the K loop, the second's at the start of the epilogue, so that
-  // their latency overlaps the products
+  // C values of this thread's epilogue elements, fetched at the start of the
+  // epilogue
   CT *C = p.C + (long long)b * p.sC;
   const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
   constexpr int EQ = (64 * 32) / NT;
@@ -181,17 +181,12 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
       cpre[j][q] = (useb && gr < M && gc < N) ? C[gr + (long long)gc * p.ldc] : CT{0, 0};
     }
   };
-  fetch_c(0);
-
-  int cur = 0;
-  if (K > 0) {
-    load(0);
-    store(sm2.buf[0]);
-    __syncthreads();
-    if (BK < K) load(BK);
-  }
+  if (K > 0) load(0);
   for (int k0 = 0; k0 < K; k0 += BK) {
-    Smem<R> &sm = sm2.buf[cur];
+    Smem<R> &sm = sm1;
+    store(sm);
+    __syncthreads();
+    if (k0 + BK < K) load(k0 + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + (lane >> 4), c16 = lane & 15;
@@ -251,14 +246,9 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
           }
       }
     }
-    if (k0 + BK < K) {
-      store(sm2.buf[cur ^ 1]);          // next K step into the other buffer
-      if (k0 + 2 * BK < K) load(k0 + 2 * BK);
-    }
     __syncthreads();
-    cur ^= 1;
   }
-  Smem<R> &sm = sm2.buf[0];
+  Smem<R> &sm = sm1;
 
   // ---- epilogue through LDS: the four waves stage a 64 x 32 slab (tile
   // columns j*16..j*16+15 of both wave columns) per pass, then every thread
@@ -270,6 +260,7 @@ __global__ __launch_bounds__(NT) void k_gemm(GemmArgs<R> p, int tiles_m, int til
   CT *C2 = (p.C2 && (!p.active2 || p.active2[b] != 0)) ? p.C2 + (long long)b * p.sC2 : nullptr;
   const double ar_ = p.alpha.x, ai_ = p.alpha.y;
   const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
+  fetch_c(0);
   fetch_c(1);
 #pragma unroll
   for (int j = 0; j < 2; ++j) {
@@ -442,8 +433,8 @@ __global__ __launch_bounds__(NTW) void k_gemm_w(GemmArgs<double> p, int tiles_m,
       cr[i][q] = 0.0;
       ci[i][q] = 0.0;
     }
-  // C values of this thread's epilogue elements: the first pass's fetched
-  // before the K loop, the second's at the start of the epilogue
+  // C values of this thread's epilogue elements, fetched at the start of the
+  // epilogue (a prefetch before the K loop would not fit the registers)
   CT *C = p.C + (long long)b * p.sC;
   const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
   constexpr int EQ = (64 * 64) / NTW;
