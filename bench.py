@@ -105,15 +105,6 @@ def traffic_from_profiles(kernel_tag: str, workload_tag):
     return None, None
 
 
-def kernel_label(tag_name: str, nmax: int) -> str:
-    """Kernel behind a profile tag: the FP64 GEMMs of blocks below 128 run the
-    64 x 64 tiling k_gemm<double, U> instead of the 64 x 128 k_gemm_w<U>, and so
-    do the inverse's trailing updates of blocks above 256 (rank 32 < 64)."""
-    if tag_name.startswith("k_gemm_w<") and (nmax < 128 or (tag_name == "k_gemm_w<2>" and nmax > 256)):
-        return "k_gemm<double, " + tag_name[len("k_gemm_w<"):]
-    return tag_name
-
-
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -300,7 +291,7 @@ def bench_c4(a, hip, wl, torch):
     kern = None
     if not a.no_profile:
         hip.profile_enable(False)
-        kern = {hip.PT_NAMES[t]: hip.profile_read(t) for t in hip.PT_NAMES}
+        kern = hip.profile_kernels()
         hip.profile_clear()
     value = a.steps * per_step / elapsed
     it = r["iters"]
@@ -308,12 +299,16 @@ def bench_c4(a, hip, wl, torch):
         f"SCBA passes mean {it.mean():.1f} max {it.max()}; status nonzero {(r['status'] != 0).sum()}")
     roofline = None
     if kern:
-        v = kern[hip.PT_NAMES[hip.PT_GEMM_SWEEP]]
-        if v["flops"] > 0:
+        # the sweep GEMMs (whichever tile served them) carry the SCBA's counted flops
+        sweep = {k: v for k, v in kern.items() if k.endswith(f"{hip.PT_GEMM_SWEEP}>") and v["launches"]
+                 and v["flops"] > 0}
+        if sweep:
+            name = max(sweep, key=lambda k: sweep[k]["ms"])
+            v = sweep[name]
             avg = v["ms"] * 1e-3 / v["launches"]
             ach = v["flops"] / v["launches"] / avg / 1e12
             roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                        "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": hip.PT_NAMES[hip.PT_GEMM_SWEEP],
+                        "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name,
                         "avg_launch_us": avg * 1e6, "launches": v["launches"],
                         "share_of_step_time": v["ms"] * 1e-3 / elapsed}
     cpu = None
@@ -399,7 +394,7 @@ def bench_path(a, hip, wl, torch):
     roofline = None
     if not a.no_profile:
         hip.profile_enable(False)
-        kern = {hip.PT_NAMES[t]: hip.profile_read(t) for t in hip.PT_NAMES}
+        kern = hip.profile_kernels()
         hip.profile_clear()
         counted = {k: v for k, v in kern.items() if v["flops"] > 0 and v["ms"] > 0}
         if counted:
@@ -564,8 +559,7 @@ def main():
     roofline, kern = None, None
     if not a.no_profile:
         hip.profile_enable(False)
-        nmax = int(max(s0.bsz))
-        kern = {kernel_label(hip.PT_NAMES[t], nmax): hip.profile_read(t) for t in hip.PT_NAMES}
+        kern = hip.profile_kernels()
         hip.profile_clear()
         for k, v in kern.items():
             log(f"  {k:22s} launches {v['launches']:6d}  total {v['ms']:9.2f} ms  "
@@ -719,6 +713,7 @@ def main():
         "parity": parity,
         "mixed": mixed,
         "kernel_ms": None if kern is None else {k: round(v["ms"], 3) for k, v in kern.items()},
+        "gemm_tuner": hip.gemm_tune_stats(),
     }
     print(json.dumps(out), flush=True)
 

@@ -412,6 +412,15 @@ int hipnegf_engine_sgf_stats(hipnegf_engine e, long long *hits, long long *store
  * g_s = -i pi dos I instead of the decimation; contact is 0-based. */
 int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious, double dos);
 
+/* FP64 GEMM tile choice (no counterpart in the reference): -1 the autotuner
+ * (default: the first launch of each shape class times the 64 x 64 and the
+ * 64 x 128 tiles and keeps the faster; HIPNEGF_GEMM_TUNE=0 in the environment
+ * switches it off), 0 always 64 x 64, 1 always 64 x 128.  Both tilings give
+ * bitwise the same results. */
+int hipnegf_gemm_force_variant(int variant);
+/* {shape classes tuned, of which chose the 64 x 128 tile, tuner on} */
+int hipnegf_gemm_tune_stats(long long *out);
+
 /* Launch timing (no counterpart in the reference; measurement hook for
  * benchmarks).  While enabled, every tagged launch records a HIP event pair on
  * its stream.  Tags: 0 masked GEMMs (decimation, SCBA), 1 RGF-sweep GEMMs,
@@ -422,6 +431,11 @@ int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious
  * algorithmic bytes, and the launch count. */
 int hipnegf_profile_enable(int on);
 int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long long *launches);
+/* the same over the launches of `tag` served by one kernel variant: for the
+ * GEMM tags 0 the 64 x 64 FP64 tile (k_gemm<double, tag>), 1 the 64 x 128 FP64
+ * tile (k_gemm_w<tag>), 2 the FP32 tile (k_gemm<float, tag>); 0 for the others */
+int hipnegf_profile_read_variant(int tag, int variant, double *ms, double *flops, double *bytes,
+                                 long long *launches);
 int hipnegf_profile_clear(void);
 /* FP64 matrix-core issue loop with register operands only (no counterpart in
  * the reference; calibration of hardware counters): `blocks` workgroups of

@@ -874,6 +874,15 @@ int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long
     return 0;
   });
 }
+int hipnegf_profile_read_variant(int tag, int variant, double *ms, double *flops, double *bytes,
+                                 long long *launches) {
+  return guard([&] {
+    if (tag < 0 || tag >= hn::PT_COUNT) throw std::runtime_error("bad profile tag");
+    if (variant < 0 || variant > 2) throw std::runtime_error("bad kernel variant");
+    hn::profiler().summary(tag, ms, flops, bytes, launches, variant);
+    return 0;
+  });
+}
 int hipnegf_profile_clear(void) {
   return guard([&] { hn::profiler().clear(); return 0; });
 }
@@ -985,6 +994,18 @@ int hipnegf_server_wait(hipnegf_server s, unsigned long long id, double *T, int 
 
 int hipnegf_server_stats(hipnegf_server s, long long *out) {
   return guard([&] { s->srv->stats(out); return 0; });
+}
+
+int hipnegf_gemm_force_variant(int variant) {
+  return guard([&] {
+    if (variant < -1 || variant > 1) throw std::runtime_error("variant must be -1, 0 or 1");
+    hn::gemm_force_variant(variant);
+    return 0;
+  });
+}
+
+int hipnegf_gemm_tune_stats(long long *out) {
+  return guard([&] { hn::gemm_tune_stats(out); return 0; });
 }
 
 int hipnegf_server_cache_stats(hipnegf_server s, long long *out) {

@@ -32,7 +32,11 @@
 // remapped such that all tiles of one item land on the same XCD and share its
 // L2 (the A and B panels of an item are re-read by every tile).
 #include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <mutex>
 #include <type_traits>
+#include <unordered_map>
 
 #include "hn_gemm.h"
 #include "hn_prof.h"
@@ -559,29 +563,11 @@ __global__ __launch_bounds__(NTW) void k_gemm_w(GemmArgs<double> p, int tiles_m,
 }
 
 template <typename R>
-void launch(const GemmArgs<R> &a, hipStream_t s) {
-  if (a.m <= 0 || a.n <= 0 || a.batch <= 0) return;
-  const bool W = sizeof(R) == 8 && a.k >= 64 && a.n >= 128;
+void run(bool W, int use, const GemmArgs<R> &a, hipStream_t s) {
   int tiles_m = (a.m + BM - 1) / BM, tiles_n = (a.n + (W ? WBN : BN) - 1) / (W ? WBN : BN);
   long long total = (long long)tiles_m * tiles_n * a.batch;
   if (total > (1LL << 30)) throw std::runtime_error("gemm: grid too large");
   int total_pad = (int)((total + 7) / 8 * 8);
-  const int use = a.tag >= 0 ? a.tag : (a.active ? PT_GEMM_MASKED : PT_GEMM_SWEEP);
-  // algorithmic work: 8 real flops per complex multiply-add over the columns written
-  const int skipped = std::max(0, std::min(a.skip_c1, a.n) - std::max(a.skip_c0, 0));
-  const int skipped_r = std::max(0, std::min(a.skip_r1, a.m) - std::max(a.skip_r0, 0));
-  const double ncol = (double)(a.n - skipped) * (double)(a.m - skipped_r) / (double)a.m;
-  const double items = a.prof_items >= 0 ? (double)a.prof_items
-                      : a.prof_items == -2 ? 1.0
-                      : ((a.active || a.active2) ? -1.0 : (double)a.batch);
-  const double flops = items < 0 ? -1.0 : 8.0 * a.m * ncol * a.k * items;
-  const double cb = (double)sizeof(typename GemmArgs<R>::CT);
-  const bool rc = a.beta.x != 0.0 || a.beta.y != 0.0;
-  const double nb = items < 0 ? (double)a.batch : items;
-  const double bytes = cb * ((a.sA ? nb : 1.0) * a.m * a.k + (a.sB ? nb : 1.0) * a.k * a.n +
-                             (rc ? 2.0 : 1.0) * nb * a.m * ncol);
-  ProfScope ps(use, flops, bytes, s);
-  if (a.prof_token) *a.prof_token = ps.tok;
   if constexpr (sizeof(R) == 8) {
     if (W) {
       if (use == PT_GEMM_SWEEP)
@@ -609,9 +595,133 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   HN_CHECK(hipGetLastError());
 }
 
+// ---- tile-shape autotuner (FP64) ------------------------------------------
+// The first launch of each shape class -- (m, n, k, operand forms, beta and
+// dual-output and mask presence, skipped ranges, use, batch rounded down to
+// a power of two) -- times the 64 x 64 four-wave tile and the 64 x 128
+// eight-wave tile on the real operands, writing into a scratch output, and
+// keeps the faster for every later launch of that class.  Both tilings run
+// the same MFMA chains in the same order, so the choice never changes a
+// result.  Off (the fixed rule: wide for k >= 64 and n >= 128) while a
+// stream is being captured into a graph, with HIPNEGF_GEMM_TUNE=0, or when a
+// variant is forced (gemm_force_variant).
+struct TuneKey {
+  int m, n, k, opA, opB, flags, use, bb;
+  bool operator==(const TuneKey &o) const {
+    return m == o.m && n == o.n && k == o.k && opA == o.opA && opB == o.opB && flags == o.flags &&
+           use == o.use && bb == o.bb;
+  }
+};
+struct TuneKeyHash {
+  size_t operator()(const TuneKey &k) const {
+    size_t h = (size_t)k.m * 1000003u ^ (size_t)k.n * 10007u ^ (size_t)k.k * 101u;
+    return h ^ ((size_t)k.opA << 40) ^ ((size_t)k.opB << 44) ^ ((size_t)k.flags << 48) ^
+           ((size_t)k.use << 56) ^ ((size_t)k.bb << 60);
+  }
+};
+std::mutex g_tune_mu;
+std::unordered_map<TuneKey, int, TuneKeyHash> g_tune;  // 0: 64 x 64, 1: 64 x 128
+std::atomic<int> g_force{-1};
+
+bool tune_on() {
+  static const int on = [] {
+    const char *e = std::getenv("HIPNEGF_GEMM_TUNE");
+    return e ? std::atoi(e) : 1;
+  }();
+  return on != 0;
+}
+
+bool pick_wide(const GemmArgs<double> &a, int use, hipStream_t s) {
+  const int f = g_force.load();
+  if (f >= 0) return f == 1;
+  const bool rule = a.k >= 64 && a.n >= 128;
+  if (a.n < 64 || a.k < 16 || !tune_on()) return rule;
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return rule;
+  int bb = 0;
+  while ((2 << bb) <= a.batch) ++bb;
+  const int flags = ((a.beta.x != 0.0 || a.beta.y != 0.0) ? 1 : 0) | (a.C2 ? 2 : 0) |
+                    (a.active ? 4 : 0) | (a.active2 ? 8 : 0) |
+                    ((a.skip_c1 > a.skip_c0 || a.skip_r1 > a.skip_r0) ? 16 : 0) |
+                    (a.sA == 0 ? 32 : 0) | (a.sB == 0 ? 64 : 0);
+  const TuneKey key{a.m, a.n, a.k, a.opA, a.opB, flags, use, bb};
+  {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) return it->second == 1;
+  }
+  // scratch outputs of the same layout (the real C and C2 are not touched)
+  auto span = [&](long long sC, int ldc) {
+    return (long long)(a.batch - 1) * sC + (long long)ldc * (a.n - 1) + a.m;
+  };
+  GemmArgs<double> t = a;
+  double2 *c1 = nullptr, *c2 = nullptr;
+  HN_CHECK(hipMallocAsync(&c1, sizeof(double2) * span(a.sC, a.ldc), s));
+  t.C = c1;
+  if (a.C2) {
+    HN_CHECK(hipMallocAsync(&c2, sizeof(double2) * span(a.sC2, a.ldc2), s));
+    t.C2 = c2;
+  }
+  hipEvent_t e0, e1;
+  HN_CHECK(hipEventCreate(&e0));
+  HN_CHECK(hipEventCreate(&e1));
+  float ms[2] = {0.0f, 0.0f};
+  for (int v = 0; v < 2; ++v) {
+    run<double>(v == 1, use, t, s);  // first use of the code object
+    HN_CHECK(hipEventRecord(e0, s));
+    run<double>(v == 1, use, t, s);
+    run<double>(v == 1, use, t, s);
+    HN_CHECK(hipEventRecord(e1, s));
+    HN_CHECK(hipEventSynchronize(e1));
+    HN_CHECK(hipEventElapsedTime(&ms[v], e0, e1));
+  }
+  HN_CHECK(hipEventDestroy(e0));
+  HN_CHECK(hipEventDestroy(e1));
+  HN_CHECK(hipFreeAsync(c1, s));
+  if (c2) HN_CHECK(hipFreeAsync(c2, s));
+  const int choice = ms[1] < ms[0] ? 1 : 0;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tune[key] = choice;
+  return choice == 1;
+}
+
+template <typename R>
+void launch(const GemmArgs<R> &a, hipStream_t s) {
+  if (a.m <= 0 || a.n <= 0 || a.batch <= 0) return;
+  const int use = a.tag >= 0 ? a.tag : (a.active ? PT_GEMM_MASKED : PT_GEMM_SWEEP);
+  bool W = false;
+  if constexpr (sizeof(R) == 8) W = pick_wide(a, use, s);
+  // algorithmic work: 8 real flops per complex multiply-add over the columns written
+  const int skipped = std::max(0, std::min(a.skip_c1, a.n) - std::max(a.skip_c0, 0));
+  const int skipped_r = std::max(0, std::min(a.skip_r1, a.m) - std::max(a.skip_r0, 0));
+  const double ncol = (double)(a.n - skipped) * (double)(a.m - skipped_r) / (double)a.m;
+  const double items = a.prof_items >= 0 ? (double)a.prof_items
+                      : a.prof_items == -2 ? 1.0
+                      : ((a.active || a.active2) ? -1.0 : (double)a.batch);
+  const double flops = items < 0 ? -1.0 : 8.0 * a.m * ncol * a.k * items;
+  const double cb = (double)sizeof(typename GemmArgs<R>::CT);
+  const bool rc = a.beta.x != 0.0 || a.beta.y != 0.0;
+  const double nb = items < 0 ? (double)a.batch : items;
+  const double bytes = cb * ((a.sA ? nb : 1.0) * a.m * a.k + (a.sB ? nb : 1.0) * a.k * a.n +
+                             (rc ? 2.0 : 1.0) * nb * a.m * ncol);
+  ProfScope ps(use, flops, bytes, s, sizeof(R) == 8 ? (W ? 1 : 0) : 2);
+  if (a.prof_token) *a.prof_token = ps.tok;
+  run<R>(W, use, a, s);
+}
+
 }  // namespace
 
 void gemm_z(const GemmArgs<double> &a, hipStream_t s) { launch<double>(a, s); }
 void gemm_c(const GemmArgs<float> &a, hipStream_t s) { launch<float>(a, s); }
+
+void gemm_force_variant(int v) { g_force.store(v < 0 ? -1 : (v ? 1 : 0)); }
+
+void gemm_tune_stats(long long out[3]) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  out[0] = (long long)g_tune.size();
+  out[1] = 0;
+  for (auto &kv : g_tune) out[1] += kv.second;
+  out[2] = tune_on() ? 1 : 0;
+}
 
 }  // namespace hn

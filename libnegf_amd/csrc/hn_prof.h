@@ -26,16 +26,20 @@ class Profiler {
  public:
   void enable(bool on);
   bool on() const { return on_; }
-  int begin(int tag, double flops, double bytes, hipStream_t s);
+  // variant: which kernel served the tag (GEMMs: 0 the 64 x 64 FP64 tile,
+  // 1 the 64 x 128 FP64 tile, 2 the FP32 tile; 0 elsewhere)
+  int begin(int tag, double flops, double bytes, hipStream_t s, int variant = 0);
   void end(int token, hipStream_t s);
   // multiply a launch's recorded flops and bytes (recorded per item) by its item count
   void scale(int token, double items);
-  // Waits for the recorded events; sums over launches of `tag`.
-  void summary(int tag, double *ms, double *flops, double *bytes, long long *launches);
+  // Waits for the recorded events; sums over launches of `tag` (and of
+  // `variant`, unless it is negative).
+  void summary(int tag, double *ms, double *flops, double *bytes, long long *launches,
+               int variant = -1);
   void clear();
 
  private:
-  struct Rec { int tag; double flops, bytes; hipEvent_t a, b; };
+  struct Rec { int tag, variant; double flops, bytes; hipEvent_t a, b; };
   bool on_ = false;
   std::mutex mu_;
   std::vector<Rec> recs_;
@@ -49,8 +53,8 @@ Profiler &profiler();
 struct ProfScope {
   int tok = -1;
   hipStream_t s;
-  ProfScope(int tag, double flops, double bytes, hipStream_t st) : s(st) {
-    if (profiler().on()) tok = profiler().begin(tag, flops, bytes, st);
+  ProfScope(int tag, double flops, double bytes, hipStream_t st, int variant = 0) : s(st) {
+    if (profiler().on()) tok = profiler().begin(tag, flops, bytes, st, variant);
   }
   ~ProfScope() {
     if (tok >= 0) profiler().end(tok, s);

@@ -24,10 +24,10 @@ hipEvent_t Profiler::get() {
   return e;
 }
 
-int Profiler::begin(int tag, double flops, double bytes, hipStream_t s) {
+int Profiler::begin(int tag, double flops, double bytes, hipStream_t s, int variant) {
   std::lock_guard<std::mutex> g(mu_);
   if (!on_) return -1;
-  Rec r{tag, flops, bytes, get(), get()};
+  Rec r{tag, variant, flops, bytes, get(), get()};
   HN_CHECK(hipEventRecord(r.a, s));
   recs_.push_back(r);
   return (int)recs_.size() - 1;
@@ -46,13 +46,14 @@ void Profiler::scale(int token, double items) {
   recs_[token].bytes *= items;
 }
 
-void Profiler::summary(int tag, double *ms, double *flops, double *bytes, long long *launches) {
+void Profiler::summary(int tag, double *ms, double *flops, double *bytes, long long *launches,
+                       int variant) {
   std::lock_guard<std::mutex> g(mu_);
   double t = 0.0, f = 0.0, by = 0.0;
   long long n = 0;
   bool f_known = true;
   for (const Rec &r : recs_) {
-    if (r.tag != tag) continue;
+    if (r.tag != tag || (variant >= 0 && r.variant != variant)) continue;
     HN_CHECK(hipEventSynchronize(r.b));
     float e = 0.0f;
     HN_CHECK(hipEventElapsedTime(&e, r.a, r.b));

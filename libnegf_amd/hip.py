@@ -142,6 +142,21 @@ PT_NAMES = {PT_GEMM_MASKED: "k_gemm_w<0>", PT_GEMM_SWEEP: "k_gemm_w<1>",
             PT_INV_SMALL: "k_lu_inv_2d", PT_FUSED: "k_t_fused"}
 
 
+def gemm_force_variant(v: int):
+    """FP64 GEMM tile: -1 autotuned, 0 always 64 x 64, 1 always 64 x 128."""
+    L = load()
+    L.hipnegf_gemm_force_variant.argtypes = [C.c_int]
+    check(L.hipnegf_gemm_force_variant(int(v)))
+
+
+def gemm_tune_stats() -> dict:
+    out = np.zeros(3, dtype=np.int64)
+    L = load()
+    L.hipnegf_gemm_tune_stats.argtypes = [C.c_void_p]
+    check(L.hipnegf_gemm_tune_stats(_ptr(out)))
+    return dict(shapes=int(out[0]), wide=int(out[1]), on=bool(out[2]))
+
+
 def profile_enable(on: bool):
     check(load().hipnegf_profile_enable(1 if on else 0))
 
@@ -154,6 +169,26 @@ def profile_read(tag: int) -> dict:
     ms, fl, by, n = C.c_double(), C.c_double(), C.c_double(), C.c_longlong()
     check(load().hipnegf_profile_read(tag, C.byref(ms), C.byref(fl), C.byref(by), C.byref(n)))
     return dict(ms=ms.value, flops=fl.value, bytes=by.value, launches=n.value)
+
+
+def profile_kernels() -> dict:
+    """Launch statistics per kernel actually run: the GEMM tags split by the
+    tile that served them (the autotuner may pick either FP64 tile)."""
+    L = load()
+    L.hipnegf_profile_read_variant.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                               C.c_void_p, C.c_void_p]
+    out = {}
+    for t, name in PT_NAMES.items():
+        if t in (PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV):
+            for v, kname in ((1, f"k_gemm_w<{t}>"), (0, f"k_gemm<double, {t}>"), (2, f"k_gemm<float, {t}>")):
+                ms, fl, by, n = C.c_double(), C.c_double(), C.c_double(), C.c_longlong()
+                check(L.hipnegf_profile_read_variant(t, v, C.byref(ms), C.byref(fl), C.byref(by),
+                                                     C.byref(n)))
+                if n.value or v == 1:
+                    out[kname] = dict(ms=ms.value, flops=fl.value, bytes=by.value, launches=n.value)
+        else:
+            out[name] = profile_read(t)
+    return out
 
 
 def _ptr(a: np.ndarray):

@@ -150,3 +150,41 @@ def test_cinverse_batched(gpu, n):
         a = A[b].astype(np.complex128)
         res = np.abs(a @ x - np.eye(n)).max() / (np.abs(a).max() * np.abs(x).max() * n)
         assert res < 1e-5, res
+
+
+@pytest.mark.parametrize("m,n,k,beta", [(64, 64, 64, 0.0), (128, 128, 128, 1.0), (256, 256, 64, 1.0),
+                                        (100, 200, 72, 0.0)])
+def test_gemm_tile_variants_bitwise(gpu, m, n, k, beta):
+    """The autotuner's two FP64 tilings (64 x 64 four-wave, 64 x 128 eight-wave)
+    run the same MFMA chains in the same order: bitwise the same C, so the
+    tuner's choice never changes a result.  Then the tuner itself records the
+    shape class."""
+    torch, hip, eng = gpu
+    rng = np.random.default_rng(m + n + k)
+    batch = 5
+    A = rng.standard_normal((batch, m * k)) + 1j * rng.standard_normal((batch, m * k))
+    B = rng.standard_normal((batch, k * n)) + 1j * rng.standard_normal((batch, k * n))
+    C0 = rng.standard_normal((batch, m * n)) + 1j * rng.standard_normal((batch, m * n))
+    dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
+    al = np.array([1.0, 0.5])
+    be = np.array([beta, 0.0])
+    out = []
+    try:
+        for v in (0, 1, -1):
+            hip.gemm_force_variant(v)
+            dC = torch.from_numpy(C0.copy()).cuda()
+            hip.check(hip.load().hipnegf_zgemm_batched(eng.handle, m, n, k, hip._ptr(al), dA.data_ptr(),
+                                                        m * k, m, 0, dB.data_ptr(), k * n, k, 0,
+                                                        hip._ptr(be), dC.data_ptr(), m * n, m, batch))
+            eng.sync()
+            out.append(dC.cpu().numpy())
+    finally:
+        hip.gemm_force_variant(-1)
+    np.testing.assert_array_equal(out[0], out[1])
+    np.testing.assert_array_equal(out[2], out[0])
+    ref = np.stack([(al[0] + 1j * al[1]) * (A[b].reshape((m, k), order="F") @ B[b].reshape((k, n), order="F"))
+                    + beta * C0[b].reshape((m, n), order="F") for b in range(batch)])
+    got = np.stack([out[0][b].reshape((m, n), order="F") for b in range(batch)])
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    st = hip.gemm_tune_stats()
+    assert st["on"] and st["shapes"] >= 1
