@@ -201,6 +201,12 @@ void negf_init_ldos(int *handler, int nldos);
 void negf_set_ldos_intervals(int *handler, int nldos, int *istart, int *iend);
 void negf_set_ldos_indexes(int *handler, int ildos, int idx_size, int *idx);
 void negf_associate_ldos(int *handler, int ldos_shape[2], double **ldos_pointer);
+/* compute_ldos (libnegf.F90:1893-1901, Fortran API only in the reference):
+ * ldos_mat(i, l) = sum over projection l's orbitals of -g_spin Im diag(G^r S) / pi
+ * at E_i + i dos_delta on the transmission grid (ldos_int, integrations.F90:212-258;
+ * G^r with its device/contact blocks, no kwght).  Fails with elastic
+ * interactions and scba_niter > 0. */
+void negf_compute_ldos(int *handler);
 
 /* extensions of this implementation */
 void negf_set_precision(int *handler, int precision); /* hipnegf_precision (FP64 default) */

@@ -1037,6 +1037,58 @@ std::vector<double2> density_run(Session &s, hn::DevStruct &st, hn::Engine &eng,
   return acc;
 }
 
+// compute_ldos (libnegf.F90:1893-1901) = extract_cont + tunneling_int_def +
+// ldos_int (integrations.F90:212-258): per point of the transmission grid,
+// G^r at Ec = E + i dos_delta (device and contacts), with the device/contact
+// blocks (compute_Gr(negf, outer = 1), G^r kept on the pattern of S), then
+// ldos_mat(i, l) = sum over the orbitals of projection l of
+// -g_spin Im(diag(G^r S)) / pi.  Orbitals past the device have no G^r rows
+// (their diagonal is 0).  The elastic SCBA loop of compute_Gr is not run
+// here: with elastic interactions and scba_niter > 0 this fails.
+void compute_ldos(Session &s) {
+  if (s.deph && s.scba_niter > 0)
+    throw std::runtime_error("compute_ldos: elastic interactions (scba_niter > 0) are not supported");
+  if (s.ndos < 1) throw std::runtime_error("compute_ldos: no LDOS projections (init_ldos)");
+  energy_grid(s);
+  const int n = s.npoints;
+  s.ldos_cols = s.ndos;
+  s.ldos.assign((size_t)n * s.ndos, 0.0);
+  s.item_status.assign(n, 0);
+  if (n <= 0) return;
+  hn::DevStruct *st = &device_struct(s);
+  hn::Engine &eng = engine(s);
+  const int ndev = st->ndev(), N = s.H[s.cur].n;
+  for (int l = 0; l < s.ndos; ++l)
+    for (int i2 : s.dos_proj[l])
+      if (i2 < 1 || i2 > N) throw std::runtime_error("LDOS orbital index outside 1..size of H");
+  const hn::SolveOpts o = dm_opts(s);
+  std::vector<double> Ei(n, s.p.dos_delta);
+  DevBuf dE(sizeof(double) * n), dEi(sizeof(double) * n), dsl(sizeof(int) * n), dS(sizeof(int) * n),
+      dG(sizeof(double) * (size_t)n * ndev);
+  HN_CHECK(hipMemcpy(dE.p, s.en.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dEi.p, Ei.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemset(dsl.p, 0, sizeof(int) * n));
+  HN_CHECK(hipMemset(dG.p, 0, sizeof(double) * (size_t)n * ndev));
+  hn::DensityOpts d;
+  d.mode = 0;
+  d.outer = 1;
+  d.status = dS.as<int>();
+  d.gs_diag = dG.as<double>();
+  d.gs_ld = ndev;
+  d.gs_scale = -s.p.g_spin / kPi;
+  eng.density(*st, n, dE.as<double>(), dEi.as<double>(), dsl.as<int>(), o, d, nullptr);
+  std::vector<double> g((size_t)n * ndev);
+  HN_CHECK(hipMemcpy(g.data(), dG.p, sizeof(double) * g.size(), hipMemcpyDeviceToHost));
+  HN_CHECK(hipMemcpy(s.item_status.data(), dS.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+  for (int l = 0; l < s.ndos; ++l)
+    for (int i = 0; i < n; ++i) {
+      double v = 0.0;
+      for (int i2 : s.dos_proj[l])
+        if (i2 <= ndev) v += g[(size_t)i * ndev + i2 - 1];
+      s.ldos[(size_t)l * n + i] = v;
+    }
+}
+
 void compute_density(Session &s) {
   if (s.ncont < 1) throw std::runtime_error("density matrix needs at least one contact");
   hn::DevStruct *st = &device_struct(s);
@@ -1593,6 +1645,11 @@ void negf_set_kpoint(int *handler, int kpoint) {
 
 void negf_solve_landauer(int *handler) {
   api(handler, [](Session &s) { compute_current(s); });
+}
+
+// compute_ldos (libnegf.F90:1893-1901; the reference's C API has no binding)
+void negf_compute_ldos(int *handler) {
+  api(handler, [](Session &s) { compute_ldos(s); });
 }
 
 void negf_calculate_dephasing_transmission(int *handler) {

@@ -150,6 +150,13 @@ struct DensityOpts {
   int *status = nullptr;        // optional device [nitems]: per-item status words
   int real_device = 0;          // mode 0: the device at real E, only the contacts at E + i Ei
                                 // (the G^r returned beside G^n, iterative.F90:263-276)
+  // mode 0, optional: per item, gs_scale * Im diag(G^r S) of every device orbital
+  // (G^r on the pattern of the device + contact S, outer >= 1 adding G^r_DC S_CD;
+  // ldos_int, integrations.F90:241-246) ADDED into gs_diag[item * gs_ld + orbital]
+  // (device, zeroed by the caller).  With gs_diag set, acc and w may be null.
+  double *gs_diag = nullptr;
+  long long gs_ld = 0;
+  double gs_scale = 0.0;
 };
 
 struct SCBAOpts {

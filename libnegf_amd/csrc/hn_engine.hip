@@ -1595,7 +1595,19 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
   const double2 one = Z(1), zero = Z(0), mone = Z(-1);
   const int iu = nbl, il = 2 * nbl - 1, ilc = 3 * nbl - 2, icl = 3 * nbl - 2 + ncont;
   auto add = [&](int idx, const double2 *X) {
-    accumulate(acc + lay[idx], X, s2, lay[idx + 1] - lay[idx], w, B, stream_);
+    if (acc) accumulate(acc + lay[idx], X, s2, lay[idx + 1] - lay[idx], w, B, stream_);
+  };
+  // diag(G^r S): row block ri of X (rows x cols) against S(c, r) of the given kind
+  std::vector<int> dev_off(nbl + 1, 0);
+  for (int i = 0; i < nbl; ++i) dev_off[i + 1] = dev_off[i] + st.bsz[i];
+  auto gsd = [&](int ri, const double2 *X, int rows, int cols, BlkRef S, int kind) {
+    if (!d.gs_diag || gn) return;
+    DiagProdArgs a;
+    a.X = X; a.sX = s2; a.rows = rows; a.cols = cols;
+    a.S = S; a.s_kind = kind; a.slot = dslot;
+    a.out = d.gs_diag + dev_off[ri]; a.sout = d.gs_ld;
+    a.scale = d.gs_scale; a.batch = B;
+    diag_prod(a, stream_);
   };
   auto form_diag = [&](int i, double2 *out) {
     FormArgs f;
@@ -1671,6 +1683,7 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
     }
     inverse(a, D, s2, cur, s2, B, nullptr, dstatus, PREC_FP64, o);
     if (!gn) add(0, cur);
+    gsd(0, cur, a, a, sd ? st.ref(BK_SD, 0) : BlkRef(), sd ? 2 : 1);
     if (!gn && keep[0]) copy_batched(keep[0], s2, cur, s2, (long long)a * a, B, nullptr, stream_);
   }
   for (int i = 1; i < nbl; ++i) {
@@ -1688,6 +1701,11 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
       add(il + i - 1, Gl);
       add(iu + i - 1, Gu);
       add(i, dst);
+      if (sof) {  // Gr(i-1,i) S(i,i-1) on block i-1, Gr(i,i-1) S(i-1,i) on block i
+        gsd(i - 1, Gu, nmm, ni_, st.ref(BK_SL, i - 1), 2);
+        gsd(i, Gl, ni_, nmm, st.ref(BK_SU, i - 1), 2);
+      }
+      gsd(i, dst, ni_, ni_, sd ? st.ref(BK_SD, i) : BlkRef(), sd ? 2 : 1);
       if (keep[i]) copy_batched(keep[i], s2, dst, s2, (long long)ni_ * ni_, B, nullptr, stream_);
       std::swap(cur, nxt);
     } else {
@@ -1704,6 +1722,7 @@ void Engine::density_batch(DevStruct &st, int B, const double *dE, const double 
         gemm_z(G(nd, nc, nc, U, s2, nd, OP_N, GSk, s2, nc, OP_N, W1, s2, nd, one, zero, B, nullptr), stream_);
         gemm_z(G(nd, nc, nd, keep[cb], s2, nd, OP_N, W1, s2, nd, OP_N, W2, s2, nd, mone, zero, B, nullptr), stream_);
         add(ilc + k, W2);
+        if (sc) gsd(cb, W2, nd, nc, st.ref(BK_SMC, k), 3);  // Gr_DC S_CD, S_CD = S_DC^H
         if (d.outer == 2) {
           form_tlc(k, L, 1);  // conj(z) SMC - HMC; Tcl = its conjugate transpose
           gemm_z(G(nc, nd, nc, GSk, s2, nc, OP_N, L, s2, nd, OP_C, W1, s2, nc, one, zero, B, nullptr), stream_);
@@ -1812,7 +1831,9 @@ void Engine::density(DevStruct &st, int nitems, const double *dE, const double *
   if (d.mode == 0 && !dEi) throw std::runtime_error("density: contour points need imaginary parts");
   if (d.mode == 1 && (!d.frm || d.ref < 1 || d.ref > st.ncont + 1))
     throw std::runtime_error("density: real-axis points need occupations and a reference contact");
-  if (!d.w) throw std::runtime_error("density: weights missing");
+  if (!d.w && (acc || !d.gs_diag)) throw std::runtime_error("density: weights missing");
+  if (d.gs_diag && (d.mode != 0 || d.gs_ld < st.ndev()))
+    throw std::runtime_error("density: diag(G S) needs contour mode and gs_ld >= device orbitals");
   HN_CHECK(hipSetDevice(device_));
   const int n = st.nmax();
   const size_t b2 = (size_t)n * n * sizeof(double2);
@@ -1831,7 +1852,8 @@ void Engine::density(DevStruct &st, int nitems, const double *dE, const double *
     const int nb = std::min(B, nitems - i0);
     DensityOpts di = d;
     if (d.status) di.status = d.status + i0;
-    density_batch(st, nb, dE + i0, dEi ? dEi + i0 : nullptr, dslot + i0, o, di, d.w + i0,
+    if (d.gs_diag) di.gs_diag = d.gs_diag + (long long)i0 * d.gs_ld;
+    density_batch(st, nb, dE + i0, dEi ? dEi + i0 : nullptr, dslot + i0, o, di, d.w ? d.w + i0 : nullptr,
                   d.frm ? d.frm + (long long)i0 * (st.ncont + 1) : nullptr, acc, lay);
   }
   HN_CHECK(hipStreamSynchronize(stream_));

@@ -7,9 +7,9 @@
 // step t from one buffer, step t+1 (prefetched into registers during step
 // t-1) is stored into the other and step t+2 is fetched from memory -- one
 // barrier per K step.  The narrow kernel (64 x 64, the small blocks) keeps
-// one stage and fetches its C tile in the epilogue: 33 KiB of LDS and 126
-// VGPRs, so four workgroups share a CU (1.1x faster at n = 64 than two
-// double-buffered ones).
+// one stage and fetches its C tile in the epilogue: 33 KiB of LDS, 126 VGPRs
+// and 64 accumulation registers, two waves per SIMD (1.1x faster at n = 64
+// than the double-buffered form, which held the same occupancy).
 //
 // FP64 uses v_mfma_f64_4x4x4 (four independent 4x4x4 blocks per
 // instruction), which sustains 67 TF/s on MI355X against 46 TF/s for
@@ -600,7 +600,8 @@ void run(bool W, int use, const GemmArgs<R> &a, hipStream_t s) {
 // dual-output and mask presence, skipped ranges, use, batch rounded down to
 // a power of two) -- times the 64 x 64 four-wave tile and the 64 x 128
 // eight-wave tile on the real operands, writing into a scratch output, and
-// keeps the faster for every later launch of that class.  Both tilings run
+// keeps the fixed rule's tile unless the other is at least 3 % faster, for
+// every later launch of that class.  Both tilings run
 // the same MFMA chains in the same order, so the choice never changes a
 // result.  Off (the fixed rule: wide for k >= 64 and n >= 128) while a
 // stream is being captured into a graph, with HIPNEGF_GEMM_TUNE=0, or when a
@@ -679,7 +680,9 @@ bool pick_wide(const GemmArgs<double> &a, int use, hipStream_t s) {
   HN_CHECK(hipEventDestroy(e1));
   HN_CHECK(hipFreeAsync(c1, s));
   if (c2) HN_CHECK(hipFreeAsync(c2, s));
-  const int choice = ms[1] < ms[0] ? 1 : 0;
+  // the fixed rule's tile unless the other is clearly faster (3 %): shapes
+  // where the two are within timing noise keep one choice from run to run
+  const int choice = rule ? (ms[0] < 0.97f * ms[1] ? 0 : 1) : (ms[1] < 0.97f * ms[0] ? 1 : 0);
   std::lock_guard<std::mutex> g(g_tune_mu);
   g_tune[key] = choice;
   return choice == 1;

@@ -90,6 +90,22 @@ void trace_prod_conj(const double2 *X, const double2 *G, long long sX, long long
 // local DOS of a diagonal block: d_b[i] = -Im(M_b(i,i)) / pi  (iterative.F90:1907-1918)
 void dos_diag(const double2 *M, long long sM, int n, int batch, double *d, long long sd,
               hipStream_t st);
+// One term of diag(G^r S) (ldos_int, integrations.F90:241-246):
+// out_b[r] += scale * Im(sum_c X_b(r, c) S(c, r)) for r < rows, X_b rows x cols
+// (ld rows).  S(c, r) by s_kind: 0 absent (nothing added), 1 identity
+// (X_b(r, r)), 2 a cols x rows block (element c + r * cols), 3 the conjugate
+// of a rows x cols block (conj of element r + c * rows: S_CD = S_DC^H).
+struct DiagProdArgs {
+  const double2 *X = nullptr; long long sX = 0;
+  int rows = 0, cols = 0;
+  BlkRef S;
+  int s_kind = 0;
+  const int *slot = nullptr;
+  double *out = nullptr; long long sout = 0;
+  double scale = 0.0;
+  int batch = 0;
+};
+void diag_prod(const DiagProdArgs &a, hipStream_t st);
 // LDOS projections (iterative.F90:1926-1944): out_b[l] = sum of d_b[idx[k] - 1] over
 // k in [ptr[l], ptr[l+1]) with idx[k] <= ndev (1-based, device arrays); full != 0:
 // out_b[i] = d_b[i] for every device orbital

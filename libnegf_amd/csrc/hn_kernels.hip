@@ -620,6 +620,39 @@ void dos_diag(const double2 *M, long long sM, int n, int batch, double *d, long 
   HN_CHECK(hipGetLastError());
 }
 
+__global__ __launch_bounds__(NT) void k_diag_prod(DiagProdArgs a) {
+  const int b = blockIdx.y;
+  const int r = blockIdx.x * NT + threadIdx.x;
+  if (r >= a.rows) return;
+  const double2 *X = a.X + (long long)b * a.sX;
+  double im = 0.0;
+  if (a.s_kind == 1) {
+    im = X[r + (long long)r * a.rows].y;
+  } else {
+    const int sl = a.slot ? a.slot[b] : 0;
+    const double2 *Sb = a.S.pool + (long long)sl * a.S.slot_stride + a.S.off;
+    for (int c = 0; c < a.cols; ++c) {
+      const double2 x = X[r + (long long)c * a.rows];
+      double2 v;
+      if (a.s_kind == 2) {
+        v = Sb[c + (long long)r * a.cols];
+      } else {
+        v = Sb[r + (long long)c * a.rows];
+        v.y = -v.y;
+      }
+      im += x.x * v.y + x.y * v.x;
+    }
+  }
+  a.out[(long long)b * a.sout + r] += a.scale * im;
+}
+
+void diag_prod(const DiagProdArgs &a, hipStream_t st) {
+  if (a.s_kind == 0 || a.rows <= 0 || a.batch <= 0) return;
+  if (a.s_kind != 1 && a.S.off < 0) throw std::runtime_error("diag_prod: S block absent");
+  hipLaunchKernelGGL(k_diag_prod, dim3((a.rows + NT - 1) / NT, a.batch), dim3(NT), 0, st, a);
+  HN_CHECK(hipGetLastError());
+}
+
 __global__ __launch_bounds__(NT) void k_accumulate(double2 *acc, const double2 *X, long long sX,
                                                    long long n, const double2 *w, int batch) {
   const long long e = (long long)blockIdx.x * NT + threadIdx.x;

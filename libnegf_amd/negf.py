@@ -339,6 +339,12 @@ class NEGF:
         """(npairs, npoints): one row per lead pair, as the reference's class."""
         return self._associated("negf_associate_transmission")
 
+    def compute_ldos(self):
+        """compute_ldos: ldos_mat from -g_spin Im diag(G^r S) / pi at E + i dos_delta
+        on the transmission grid (libnegf.F90:1893-1901)."""
+        self.set_params()
+        self._call("negf_compute_ldos")
+
     def get_ldos(self):
         """(nldos, npoints): local DOS of each projection, as the reference's class."""
         return self._associated("negf_associate_ldos")
