@@ -304,6 +304,7 @@ void Engine::inverse(int n, double2 *A, long long sA, double2 *Ainv, long long s
     arena_.used = mark;
     return;
   }
+  w.prof_items = active ? prof_items : -1;
   inv_z(n, A, sA, Ainv, sAinv, batch, active, status, w, stream_);
 }
 

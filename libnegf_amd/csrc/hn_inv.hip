@@ -806,9 +806,14 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
               long long sAinv, int batch, const int *active, int *status, const InvWork<R> &w,
               hipStream_t s) {
   if (n <= 0 || batch <= 0) return;
+  // algorithmic work of the one-launch inverses: 8 n^3 flops and the block read
+  // and written per item (items unknown for a mask without a count: -1)
+  const double items = w.prof_items >= 0 ? (double)w.prof_items : (active ? -1.0 : (double)batch);
+  const double sm_fl = items < 0 ? -1.0 : 8.0 * (double)n * n * n * items;
+  const double sm_by = items < 0 ? 0.0 : 2.0 * sizeof(typename CT_<R>::t) * (double)n * n * items;
   if constexpr (sizeof(R) == 8) {
     if (n <= 64) {
-      ProfScope ps(PT_INV_SMALL, -1.0, 0.0, s);
+      ProfScope ps(PT_INV_SMALL, sm_fl, sm_by, s);
       // the triangular U array: n (n - 1) / 2 entries
       const size_t lds = std::max<size_t>(1, (size_t)n * (n - 1) / 2) * sizeof(double2);
       if (n <= 16)
@@ -827,7 +832,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
   if (n <= lds_max_n<R>()) {
     size_t lds = lds_bytes_small<R>(n);
     allow_lds(k_inv_lds<R>, lds);
-    ProfScope ps(PT_INV_SMALL, -1.0, 0.0, s);
+    ProfScope ps(PT_INV_SMALL, sm_fl, sm_by, s);
     hipLaunchKernelGGL(k_inv_lds<R>, dim3(batch), dim3(NT), lds, s, n, A, sA, Ainv, sAinv,
                        active, status);
     HN_CHECK(hipGetLastError());
