@@ -77,7 +77,7 @@ int hipnegf_ZmultMat(hipnegf_handle h, size_t m, size_t n, size_t k, const void 
 /* Ainv = inverse(A), n x n, A unchanged; cu_{C,Z}inverse (gpuroutines.cu:449-523).
  * The reference's caller pre-sets Ainv = I (cudautils.F90:650); not required here.
  * Returns 0, or 1 if the matrix is exactly singular (no process exit).
- * n <= 512 (the reference's ZGETRF has no limit): larger n returns -1 with
+ * n <= 1024 (the reference's ZGETRF has no limit): larger n returns -1 with
  * hipnegf_last_error set. */
 int hipnegf_Cinverse(hipnegf_handle hb, hipnegf_handle hs, void *A, void *Ainv, size_t n);
 int hipnegf_Zinverse(hipnegf_handle hb, hipnegf_handle hs, void *A, void *Ainv, size_t n);
@@ -181,7 +181,7 @@ const char *hipnegf_last_error(void);
 /* Device structure: nbl principal layers of sizes bsz[nbl] (the PL partition
  * of ln_structure.F90:80-150), ncont contacts attached to 1-based blocks
  * cblk[ncont] with PL size nc[ncont], nslots identical slots (e.g. one per k).
- * Every PL and every contact (surface + PL) holds at most 512 orbitals, the
+ * Every PL and every contact (surface + PL) holds at most 1024 orbitals, the
  * largest block order of the batched inverse; larger ones return -1. */
 int hipnegf_struct_create(int nbl, const int *bsz, int ncont, const int *cblk, const int *nc,
                           int smode, int nslots, hipnegf_struct *st);

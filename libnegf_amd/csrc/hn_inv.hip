@@ -7,7 +7,7 @@
 // in-place Gauss-Jordan elimination with partial pivoting entirely in LDS,
 // followed by the column interchanges that undo the row pivoting.
 //
-// Large blocks (up to n = 512): in-place Gauss-Jordan with lazy pivoting on
+// Large blocks (up to n = 1024): in-place Gauss-Jordan with lazy pivoting on
 // the row-major view M = A^T, so that rows of M are contiguous columns of A
 // (inverting M and reading the result back column-major gives A^-1; the row
 // pivoting of M is column pivoting of A).  Rows are never moved; the pivot
@@ -152,13 +152,14 @@ __global__ __launch_bounds__(NT) void k_inv_lds(int n, typename CT_<R>::t *A, lo
 // ---------------------------------------------------------------------------
 constexpr int NTG = 512;  // threads of the Gauss-Jordan kernel: two halves of 256 rows
 
-// Encoding of a chosen pivot row r (r < 512) with its FP64 |.|_1: the high
-// word of the magnitude (exponent and 10 mantissa bits), a flag bit and
-// 511 - r; the row is read back from the low bits and a magnitude below
-// 2^-1012 (the high word below 0x400) flags a zero pivot.
+// Encoding of a chosen pivot row r (r < 1024) with its FP64 |.|_1 (the row
+// itself comes from the exact 64-bit search): the high word of the magnitude
+// (exponent and 9 mantissa bits), a flag bit and 1023 - r; the row is read
+// back from the low bits and a magnitude below 2^-1011 (the high word below
+// 0x800) flags a zero pivot.
 __device__ inline unsigned pivot_key(double v, int r) {
   const unsigned hi = min((unsigned)((unsigned long long)__double_as_longlong(v) >> 32), 0x7FF00000u);
-  return (hi & ~0x3FFu) | 0x200u | (unsigned)(511 - r);
+  return (hi & ~0x7FFu) | 0x400u | (unsigned)(1023 - r);
 }
 __device__ inline unsigned pivot_key(float v, int r) {
   return pivot_key((double)v, r);
@@ -483,12 +484,28 @@ __global__ __launch_bounds__(G * G) void k_lu_inv_2d(int n, const double2 *__res
 }
 
 
+// "row already pivoted" flags of a lane's RT register rows: an array, or at
+// 16 rows per lane the bits of one register (the panel then holds 128)
+template <int RT, bool BITS = (RT >= 16)> struct RowFlags {
+  int f[RT];
+  __device__ __forceinline__ int get(int i) const { return f[i]; }
+  __device__ __forceinline__ void set(int i, int v) { f[i] = v; }
+};
+template <int RT> struct RowFlags<RT, true> {
+  static_assert(RT <= 32, "one 32-bit mask");
+  unsigned m = 0;
+  __device__ __forceinline__ int get(int i) const { return (int)((m >> i) & 1u); }
+  __device__ __forceinline__ void set(int i, int v) { m = v ? (m | (1u << i)) : (m & ~(1u << i)); }
+};
+
 // rows of the LDS staging buffer of k_inv_gj (a chunk of the n x 8CT panel,
-// rows padded by one element)
+// rows padded by one element); at 16 rows per lane the multiplier buffers
+// (2 x 64 RT elements) take their share of the budget
 template <typename R, int RT, int CT>
 constexpr int gj_stage_rows() {
   constexpr int rowb = (8 * CT + 1) * 2 * (int)sizeof(R);
-  constexpr int cr = ((140 * 1024) / rowb) & ~63;
+  constexpr int budget = 140 * 1024 - (RT >= 16 ? 2 * 64 * RT * 2 * (int)sizeof(R) : 0);
+  constexpr int cr = (budget / rowb) & ~63;
   return cr < 64 * RT ? cr : 64 * RT;
 }
 template <typename R, int RT, int CT>
@@ -551,31 +568,36 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
   int *ug = used_g + (long long)b * n;
 
   C P[RT][CT];
-  int used[RT];
+  RowFlags<RT> used;
 #pragma unroll
   for (int i = 0; i < RT; ++i) {
     const int r = l + 64 * i;
-    used[i] = r >= n ? 1 : (s0 == 0 ? 0 : ug[r]);
+    used.set(i, r >= n ? 1 : (s0 == 0 ? 0 : ug[r]));
 #pragma unroll
     for (int c = 0; c < CT; ++c) P[i][c] = C{0, 0};
   }
   if (tid == 0) s_sing = 0;
-  // ---- load the panel (at most two chunks of CR rows, both chunks' global
-  // loads issued first), coalesced along rows of M ----
-  static_assert(2 * CR >= 64 * RT, "two staging chunks cover the panel");
-  C ld[2][PER];
+  // ---- load the panel in chunks of CR rows, NPF chunks' global loads
+  // issued together (one at 16 rows per lane, where the panel itself holds
+  // 128 registers), coalesced along rows of M ----
+  static_assert(CR % 64 == 0, "a lane's rows fall in one chunk each");
+  constexpr int NCH = (64 * RT + CR - 1) / CR;
+  constexpr int NPF = RT >= 16 ? 1 : 2;
 #pragma unroll
-  for (int h = 0; h < 2; ++h)
+  for (int h0 = 0; h0 < NCH; h0 += NPF) {
+  C ld[NPF][PER];
+#pragma unroll
+  for (int h = 0; h < NPF; ++h)
 #pragma unroll
     for (int k = 0; k < PER; ++k) {
       const int e = tid + NTG * k;
-      const int row = h * CR + e / SWC, col = e % SWC;
-      ld[h][k] = (row < n && col < sw) ? Xb[(size_t)row * n + s0 + col] : C{0, 0};
+      const int row = (h0 + h) * CR + e / SWC, col = e % SWC;
+      ld[h][k] = (h0 + h < NCH && row < n && col < sw) ? Xb[(size_t)row * n + s0 + col] : C{0, 0};
     }
 #pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int cr0 = h * CR;
-    if (cr0 < n) {  // uniform
+  for (int h = 0; h < NPF; ++h) {
+    const int cr0 = (h0 + h) * CR;
+    if (h0 + h < NCH && cr0 < n) {  // uniform
 #pragma unroll
       for (int k = 0; k < PER; ++k) {
         const int e = tid + NTG * k;
@@ -593,6 +615,7 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
       }
       __syncthreads();
     }
+  }
   }
 
   // the owning wave's column (register 0) -> multipliers in mcol[bb] and the
@@ -612,7 +635,7 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
       if (r < n) mcol[bb][r] = P[i][0];
       const double v = (double)cabs1(P[i][0]);
       const unsigned long long key =
-          used[i] ? 0ull : (unsigned long long)__double_as_longlong(v) + 1ull;
+          used.get(i) ? 0ull : (unsigned long long)__double_as_longlong(v) + 1ull;
       if (key > kb) { kb = key; rb = r; vb = v; cv = P[i][0]; }  // rows grow with i
     }
     const unsigned long long km = wave_max64(kb);
@@ -631,8 +654,8 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
     // ---- pivot row p and 1/B(p, j); each wave reads its columns of row p
     // from the lane holding it ----
     const unsigned kx = redk[bf];
-    const int p = kx ? 511 - (int)(kx & 511) : -1;
-    if (tid == 0 && (kx >> 10) == 0) s_sing = 1;  // zero (or NaN) pivot
+    const int p = kx ? 1023 - (int)(kx & 1023) : -1;
+    if (tid == 0 && (kx >> 11) == 0) s_sing = 1;  // zero (or NaN) pivot
     const C pinv = kx ? redinv[bf] : C{0, 0};
     const int lp = p & 63, ip = p >> 6;  // p < 0 cannot happen: j < n unpivoted rows remain
     C y[CT];
@@ -643,13 +666,45 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
       if (p >= 0 && ip == i) {
 #pragma unroll
         for (int c = 0; c < CT; ++c) y[c] = C{lane_read(P[i][c].x, lp), lane_read(P[i][c].y, lp)};
-        if (l == lp) used[i] = 1;
+        if (l == lp) used.set(i, 1);
       }
     if (w == wj) y[0] = C{R(1), R(0)};  // column j itself
     if (w == wj && l == 0) spiv[j] = p < 0 ? 0 : p;
     // ---- B(r, :) -= l_r B(p, :), l_r = B(r, j) / B(p, j); old register k + 1
     // -> register k (the owning wave's column j, taken as 0, enters the last
     // register against the 1 in y[0]) ----
+    if constexpr (RT >= 16) {
+      // 16 rows per lane: row by row (multiplier, then the row's CT registers),
+      // so no per-row arrays are held; the next column is published after the
+      // whole update.  The same operations on every element as below.
+#pragma unroll
+      for (int i = 0; i < RT; ++i) {
+        const int r = l + 64 * i;
+        const C m = (w == wj) ? P[i][0] : (r < n ? mcol[bf][r] : C{0, 0});
+        C lmi = cmul(m, pinv);
+        C x0i = (w == wj) ? C{0, 0} : P[i][0];
+        if (r == p) {
+          lmi = C{-pinv.x, -pinv.y};
+          x0i = C{0, 0};
+#pragma unroll
+          for (int c = 1; c < CT; ++c) P[i][c] = C{0, 0};
+        }
+        const R nr = -lmi.x, ni = -lmi.y;
+#pragma unroll
+        for (int k = 0; k < CT; ++k) {
+          const C yk = y[(k + 1) % CT];
+          C x = k + 1 < CT ? P[i][k + 1] : x0i;
+          x.x = fma(nr, yk.x, x.x);
+          x.x = fma(-ni, yk.y, x.x);
+          x.y = fma(nr, yk.y, x.y);
+          x.y = fma(ni, yk.x, x.y);
+          P[i][k] = x;
+        }
+      }
+      if (j + 1 < sw && w == (j + 1) / CT) candidate(bf ^ 1);
+      __syncthreads();
+      continue;
+    }
     C lm[RT], x0[RT];
 #pragma unroll
     for (int i = 0; i < RT; ++i) {
@@ -718,13 +773,13 @@ __global__ __launch_bounds__(NTG) void k_inv_gj(int n, int s0, int sw,
 #pragma unroll
   for (int i = 0; i < RT; ++i) {
     const int r = l + 64 * i;
-    if (w == 0 && r < n) ug[r] = used[i];
+    if (w == 0 && r < n) ug[r] = used.get(i);
   }
   if (tid < sw) piv_g[(long long)b * n + s0 + tid] = spiv[tid];
   if (Rb) {  // pivot columns of A outside the rows of S, moved to Rb; element e = (c, k)
     C *Rbb = Rb + (long long)b * kInvSuper * n;
     constexpr int GU = 8;  // loads in flight per thread
-    const float inv_n = 1.0f / (float)n;  // e / n below is exact: e < 2^15, n <= 512
+    const float inv_n = 1.0f / (float)n;  // e / n below is exact: e < 2^15, n <= 1024
     const int tot = sw * n;
     for (int base = 0; base < tot; base += NTG * GU) {
       C v[GU];
@@ -838,14 +893,15 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
     HN_CHECK(hipGetLastError());
     return;
   }
-  if (n > 512) throw std::runtime_error("inverse: block order > 512 is not supported");
+  if (n > 1024) throw std::runtime_error("inverse: block order > 1024 is not supported");
   if (sA < (long long)n * n || sAinv < (long long)n * n)
     throw std::runtime_error("inverse: item stride smaller than n*n");
   if (!w.Rb || w.n < n || w.batch < batch) throw std::runtime_error("inverse: workspace too small");
   // register tile per thread: 4 rows x 8 columns (super-panel 64) up to
-  // n = 256, 8 rows x 4 columns (super-panel 32) up to n = 512
-  const int rpt = n <= 256 ? 1 : 2;
-  const int spw = rpt == 1 ? 64 : 32;
+  // n = 256, 8 rows x 4 columns (super-panel 32) up to n = 512, 16 rows x 2
+  // columns (super-panel 16) up to n = 1024
+  const int rpt = n <= 256 ? 1 : n <= 512 ? 2 : 4;
+  const int spw = rpt == 1 ? 64 : rpt == 2 ? 32 : 16;
   for (int s0 = 0; s0 < n; s0 += spw) {
     const int sw = std::min(spw, n - s0);
     typename CT_<R>::t *rb = n > sw ? w.Rb : nullptr;
@@ -856,10 +912,15 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
         hipLaunchKernelGGL((k_inv_gj<R, 4, 8>), dim3(batch), dim3(NTG),
                            (gj_stage_bytes<R, 4, 8>()), s, n, s0, sw, A, sA, rb, w.used,
                            w.piv, active, status);
-      } else {
+      } else if (rpt == 2) {
         allow_lds(k_inv_gj<R, 8, 4>, gj_stage_bytes<R, 8, 4>());
         hipLaunchKernelGGL((k_inv_gj<R, 8, 4>), dim3(batch), dim3(NTG),
                            (gj_stage_bytes<R, 8, 4>()), s, n, s0, sw, A, sA, rb, w.used,
+                           w.piv, active, status);
+      } else {
+        allow_lds(k_inv_gj<R, 16, 2>, gj_stage_bytes<R, 16, 2>());
+        hipLaunchKernelGGL((k_inv_gj<R, 16, 2>), dim3(batch), dim3(NTG),
+                           (gj_stage_bytes<R, 16, 2>()), s, n, s0, sw, A, sA, rb, w.used,
                            w.piv, active, status);
       }
     }

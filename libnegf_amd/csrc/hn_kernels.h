@@ -5,7 +5,7 @@
 namespace hn {
 
 constexpr int kMaxCont = 10;  // MAXNCONT, lnParams.h
-constexpr int kMaxBlockOrder = 512;  // largest block the batched inverse handles
+constexpr int kMaxBlockOrder = 1024;  // largest block the batched inverse handles
 constexpr size_t kNone = ~(size_t)0;
 
 // Where a block lives: item b reads pool + slot[b] * slot_stride + off.

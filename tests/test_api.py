@@ -75,17 +75,17 @@ def test_defaults_and_host_errors():
 
 
 def test_block_order_limit_is_rejected(lib):
-    """Blocks above the inverse's 512 orbitals are refused when the structure
+    """Blocks above the inverse's 1024 orbitals are refused when the structure
     is created, before any device call (host only)."""
     h = C.c_void_p()
-    for bsz, nc in (([64, 600, 64], [64, 64]), ([64, 64], [513, 64])):
+    for bsz, nc in (([64, 1100, 64], [64, 64]), ([64, 64], [1025, 64])):
         b = np.array(bsz, np.int32)
         cb = np.array([len(bsz), 1], np.int32)
         n = np.array(nc, np.int32)
         rc = lib.hipnegf_struct_create(len(bsz), b.ctypes.data, 2, cb.ctypes.data, n.ctypes.data,
                                        0, 1, C.byref(h))
         assert rc == -1
-        assert b"512" in lib.hipnegf_last_error()
+        assert b"1024" in lib.hipnegf_last_error()
 
 
 def test_contact_blocks_get_pls_and_reference():

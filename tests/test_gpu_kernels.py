@@ -98,7 +98,7 @@ def _inv_case(rng, n, kind):
 
 
 @pytest.mark.parametrize("n", [1, 2, 5, 10, 16, 17, 31, 32, 33, 63, 64, 96, 100, 101, 128, 130, 200, 256, 300, 383,
-                               449, 512])
+                               449, 512, 513, 700, 1024])
 @pytest.mark.parametrize("kind", ["rand", "perm", "green"])
 def test_zinverse_batched(gpu, n, kind):
     torch, hip, eng = gpu
@@ -123,7 +123,7 @@ def test_zinverse_batched(gpu, n, kind):
 
 def test_zinverse_singular_flag(gpu):
     torch, hip, eng = gpu
-    for n in (8, 160):
+    for n in (8, 160, 700):
         A = np.ones((n, n), dtype=np.complex128)
         dA = torch.from_numpy(A.ravel(order="F").copy()).cuda().reshape(1, -1)
         dX = torch.zeros_like(dA)
@@ -133,7 +133,7 @@ def test_zinverse_singular_flag(gpu):
         assert int(st.cpu()[0]) & hip.ST_SINGULAR
 
 
-@pytest.mark.parametrize("n", [10, 64, 128, 200, 256, 320, 512])
+@pytest.mark.parametrize("n", [10, 64, 128, 200, 256, 320, 512, 777])
 def test_cinverse_batched(gpu, n):
     torch, hip, eng = gpu
     rng = np.random.default_rng(n)
