@@ -657,10 +657,26 @@ bool pick_wide(const GemmArgs<double> &a, int use, hipStream_t s) {
   };
   GemmArgs<double> t = a;
   double2 *c1 = nullptr, *c2 = nullptr;
-  HN_CHECK(hipMallocAsync(&c1, sizeof(double2) * span(a.sC, a.ldc), s));
+  // without room for the scratch outputs (a caller near its HBM budget, or
+  // less than twice their size plus 1 GiB free) this launch takes the fixed
+  // rule untuned; a later one may tune the class
+  const size_t need = sizeof(double2) * ((size_t)span(a.sC, a.ldc) + (a.C2 ? (size_t)span(a.sC2, a.ldc2) : 0));
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || 2 * need + (1ull << 30) > free_b) {
+    (void)hipGetLastError();
+    return rule;
+  }
+  if (hipMallocAsync(&c1, sizeof(double2) * span(a.sC, a.ldc), s) != hipSuccess) {
+    (void)hipGetLastError();
+    return rule;
+  }
   t.C = c1;
   if (a.C2) {
-    HN_CHECK(hipMallocAsync(&c2, sizeof(double2) * span(a.sC2, a.ldc2), s));
+    if (hipMallocAsync(&c2, sizeof(double2) * span(a.sC2, a.ldc2), s) != hipSuccess) {
+      (void)hipGetLastError();
+      HN_CHECK(hipFreeAsync(c1, s));
+      return rule;
+    }
     t.C2 = c2;
   }
   hipEvent_t e0, e1;
