@@ -842,15 +842,18 @@ size_t lds_bytes_small(int n) {
   return (size_t)n * n * sizeof(C) + 2 * (size_t)n * sizeof(C) + (size_t)n * sizeof(int);
 }
 
-// largest n for the one-workgroup LDS kernel: what fits, and in FP64 at most
-// 68 -- above that the blocked path is faster (2000 items: n = 80 in 2.2 ms
-// against 4.1 ms, n = 99 in 2.7 against 7.9; n = 65 in 1.9 against 1.7)
+// largest n for the one-workgroup LDS kernel (FP32): what fits
 template <typename R>
 int lds_max_n() {
   int n = 1;
   while (n + 1 <= NT && lds_bytes_small<R>(n + 1) <= (size_t)kLdsBudget) ++n;
-  return sizeof(R) == 8 ? std::min(n, 68) : n;
+  return n;
 }
+// largest FP64 order of the register LU kernel: 16 x 16 threads with 6 x 6
+// blocks (250 VGPRs, the U array 73 KiB at n = 96, two workgroups per CU).
+// Above it the blocked path; 2000 items of order 65 / 80 / 96 take 1.0 / 1.2 /
+// 1.7 ms here against 1.7 / 2.2 / 2.6 ms on the LDS or blocked kernels.
+constexpr int kLuMaxN = 96;
 
 template <typename Kern>
 void allow_lds(Kern k, size_t bytes) {
@@ -870,7 +873,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
   const double sm_fl = items < 0 ? -1.0 : 8.0 * (double)n * n * n * items;
   const double sm_by = items < 0 ? 0.0 : 2.0 * sizeof(typename CT_<R>::t) * (double)n * n * items;
   if constexpr (sizeof(R) == 8) {
-    if (n <= 64) {
+    if (n <= kLuMaxN) {
       ProfScope ps(PT_INV_SMALL, sm_fl, sm_by, s);
       // the triangular U array: n (n - 1) / 2 entries
       const size_t lds = std::max<size_t>(1, (size_t)n * (n - 1) / 2) * sizeof(double2);
@@ -880,14 +883,23 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
       else if (n <= 32)
         hipLaunchKernelGGL((k_lu_inv_2d<8, 4>), dim3(batch), dim3(64), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
-      else
+      else if (n <= 64)
         hipLaunchKernelGGL((k_lu_inv_2d<16, 4>), dim3(batch), dim3(256), lds, s, n, A, sA, Ainv,
                            sAinv, active, status);
+      else if (n <= 80) {
+        allow_lds(k_lu_inv_2d<16, 5>, lds);
+        hipLaunchKernelGGL((k_lu_inv_2d<16, 5>), dim3(batch), dim3(256), lds, s, n, A, sA, Ainv,
+                           sAinv, active, status);
+      } else {
+        allow_lds(k_lu_inv_2d<16, 6>, lds);
+        hipLaunchKernelGGL((k_lu_inv_2d<16, 6>), dim3(batch), dim3(256), lds, s, n, A, sA, Ainv,
+                           sAinv, active, status);
+      }
       HN_CHECK(hipGetLastError());
       return;
     }
   }
-  if (n <= lds_max_n<R>()) {
+  if (sizeof(R) == 4 && n <= lds_max_n<R>()) {  // FP32; FP64 above kLuMaxN is blocked
     size_t lds = lds_bytes_small<R>(n);
     allow_lds(k_inv_lds<R>, lds);
     ProfScope ps(PT_INV_SMALL, sm_fl, sm_by, s);
@@ -948,7 +960,7 @@ void inv_impl(int n, typename CT_<R>::t *A, long long sA, typename CT_<R>::t *Ai
 
 }  // namespace
 
-int inv_lds_max_n(int real_bytes) { return real_bytes == 8 ? lds_max_n<double>() : lds_max_n<float>(); }
+int inv_lds_max_n(int real_bytes) { return real_bytes == 8 ? kLuMaxN : lds_max_n<float>(); }
 
 size_t inv_work_bytes(int n, int batch, int real_bytes) {
   size_t c = 2 * (size_t)real_bytes;

@@ -97,8 +97,8 @@ def _inv_case(rng, n, kind):
         return (0.3 + 1e-3j) * np.eye(n) - H
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 10, 16, 17, 31, 32, 33, 63, 64, 96, 100, 101, 128, 130, 200, 256, 300, 383,
-                               449, 512, 513, 700, 1024])
+@pytest.mark.parametrize("n", [1, 2, 5, 10, 16, 17, 31, 32, 33, 63, 64, 65, 72, 80, 81, 95, 96, 97, 100, 101, 128,
+                               130, 200, 256, 300, 383, 449, 512, 513, 700, 1024])
 @pytest.mark.parametrize("kind", ["rand", "perm", "green"])
 def test_zinverse_batched(gpu, n, kind):
     torch, hip, eng = gpu
