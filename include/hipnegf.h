@@ -413,12 +413,14 @@ int hipnegf_engine_sgf_stats(hipnegf_engine e, long long *hits, long long *store
 int hipnegf_struct_set_fictitious(hipnegf_struct st, int contact, int fictitious, double dos);
 
 /* FP64 GEMM tile choice (no counterpart in the reference): -1 the autotuner
- * (default: the first launch of each shape class times the 64 x 64 and the
- * 64 x 128 tiles and keeps the faster; HIPNEGF_GEMM_TUNE=0 in the environment
- * switches it off), 0 always 64 x 64, 1 always 64 x 128.  Both tilings give
- * bitwise the same results. */
+ * (default: the first launch of each shape class times the 64 x 64, the
+ * 64 x 128 and the 32 x 32 single-wave tiles and keeps the fixed rule's tile
+ * unless another is 3 % faster; HIPNEGF_GEMM_TUNE=0 in the environment
+ * switches it off), 0 always 64 x 64, 1 always 64 x 128, 2 always 32 x 32.
+ * The tilings give bitwise the same results. */
 int hipnegf_gemm_force_variant(int variant);
-/* {shape classes tuned, of which chose the 64 x 128 tile, tuner on} */
+/* {shape classes tuned, of which chose the 64 x 128 tile, tuner on, of which
+ * chose the 32 x 32 tile} */
 int hipnegf_gemm_tune_stats(long long *out);
 
 /* Launch timing (no counterpart in the reference; measurement hook for
@@ -433,7 +435,8 @@ int hipnegf_profile_enable(int on);
 int hipnegf_profile_read(int tag, double *ms, double *flops, double *bytes, long long *launches);
 /* the same over the launches of `tag` served by one kernel variant: for the
  * GEMM tags 0 the 64 x 64 FP64 tile (k_gemm<double, tag>), 1 the 64 x 128 FP64
- * tile (k_gemm_w<tag>), 2 the FP32 tile (k_gemm<float, tag>); 0 for the others */
+ * tile (k_gemm_w<tag>), 2 the FP32 tile (k_gemm<float, tag>), 3 the 32 x 32
+ * single-wave FP64 tile (k_gemm_s<tag>); 0 for the others */
 int hipnegf_profile_read_variant(int tag, int variant, double *ms, double *flops, double *bytes,
                                  long long *launches);
 int hipnegf_profile_clear(void);

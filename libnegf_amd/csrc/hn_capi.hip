@@ -878,7 +878,7 @@ int hipnegf_profile_read_variant(int tag, int variant, double *ms, double *flops
                                  long long *launches) {
   return guard([&] {
     if (tag < 0 || tag >= hn::PT_COUNT) throw std::runtime_error("bad profile tag");
-    if (variant < 0 || variant > 2) throw std::runtime_error("bad kernel variant");
+    if (variant < 0 || variant > 3) throw std::runtime_error("bad kernel variant");
     hn::profiler().summary(tag, ms, flops, bytes, launches, variant);
     return 0;
   });
@@ -998,7 +998,7 @@ int hipnegf_server_stats(hipnegf_server s, long long *out) {
 
 int hipnegf_gemm_force_variant(int variant) {
   return guard([&] {
-    if (variant < -1 || variant > 1) throw std::runtime_error("variant must be -1, 0 or 1");
+    if (variant < -1 || variant > 2) throw std::runtime_error("variant must be -1, 0, 1 or 2");
     hn::gemm_force_variant(variant);
     return 0;
   });

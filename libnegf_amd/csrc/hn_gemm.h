@@ -35,9 +35,10 @@ struct GemmArgs {
 
 void gemm_z(const GemmArgs<double> &a, hipStream_t s);  // complex FP64
 void gemm_c(const GemmArgs<float> &a, hipStream_t s);   // complex FP32 (FP32 accumulation)
-// FP64 tile choice: -1 autotuned (default), 0 always the 64 x 64 tile, 1 always 64 x 128
+// FP64 tile choice: -1 autotuned (default), 0 always the 64 x 64 tile, 1 always 64 x 128,
+// 2 always the 32 x 32 single-wave tile
 void gemm_force_variant(int v);
-// {shape classes tuned, of which took the 64 x 128 tile, tuner on}
-void gemm_tune_stats(long long out[3]);
+// {shape classes tuned, of which took the 64 x 128 tile, tuner on, of which took 32 x 32}
+void gemm_tune_stats(long long out[4]);
 
 }  // namespace hn

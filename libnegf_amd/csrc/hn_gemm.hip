@@ -562,9 +562,183 @@ __global__ __launch_bounds__(NTW) void k_gemm_w(GemmArgs<double> p, int tiles_m,
   }
 }
 
+// ---- FP64 single-wave tile: 32 x 32 per 64-thread workgroup (the work of
+// one wave of the 64 x 64 tile, same fragments and the same MFMA order, so
+// results are bitwise identical to it).  Workgroups are independent (no
+// barrier couples four waves), so at two waves per SIMD eight tiles per CU
+// hide each other's operand and C latencies; the epilogue writes the
+// accumulators straight to memory (16 rows x 4 columns per instruction).
+constexpr int SBT = 32, NTS = 64;
+struct SmemS {
+  double Ar[BK][SBT + PAD];
+  double Ai[BK][SBT + PAD];
+  double Br[BK][SBT + PAD];
+  double Bi[BK][SBT + PAD];
+};
+
+template <int USE>
+__global__ __launch_bounds__(NTS) void k_gemm_s(GemmArgs<double> p, int tiles_m, int tiles_n,
+                                               int total, int total_pad) {
+  typedef double2 CT;
+  const int logical = xcd_remap(blockIdx.x, total_pad);
+  if (logical >= total) return;
+  const int tiles = tiles_m * tiles_n;
+  const int b = logical / tiles;
+  const int t = logical - b * tiles;
+  if (p.active && p.active[b] == 0) return;
+  const int tm = t % tiles_m, tn = t / tiles_m;
+  const int m0 = tm * SBT, n0 = tn * SBT;
+  if ((m0 >= p.skip_r0 && m0 + SBT <= p.skip_r1) || (n0 >= p.skip_c0 && n0 + SBT <= p.skip_c1))
+    return;  // wholly inside a skipped range
+
+  __shared__ SmemS sm;
+  const CT *A = p.A + (long long)b * p.sA;
+  const CT *B = p.B + (long long)b * p.sB;
+  const int lane = threadIdx.x;
+  const int M = p.m, N = p.n, K = p.k;
+  const bool a_n = p.opA == OP_N, b_n = p.opB == OP_N;
+  const double sa = (p.opA == OP_C) ? -1.0 : 1.0, sb = (p.opB == OP_C) ? -1.0 : 1.0;
+
+  // global -> register staging of one K step (8 complex of A and of B per lane)
+  CT ra[8], rb[8];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int i, kk;
+      if (a_n) { i = lane & 31; kk = (lane >> 5) + 2 * r; }
+      else { kk = lane & 15; i = (lane >> 4) + 4 * r; }
+      const int gi = m0 + i, gk = k0 + kk;
+      CT v = {0, 0};
+      if (gi < M && gk < K) v = a_n ? A[gi + (long long)gk * p.lda] : A[gk + (long long)gi * p.lda];
+      ra[r] = v;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int j, kk;
+      if (b_n) { kk = lane & 15; j = (lane >> 4) + 4 * r; }
+      else { j = lane & 31; kk = (lane >> 5) + 2 * r; }
+      const int gj = n0 + j, gk = k0 + kk;
+      CT v = {0, 0};
+      if (gj < N && gk < K) v = b_n ? B[gk + (long long)gj * p.ldb] : B[gj + (long long)gk * p.ldb];
+      rb[r] = v;
+    }
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int i, kk;
+      if (a_n) { i = lane & 31; kk = (lane >> 5) + 2 * r; }
+      else { kk = lane & 15; i = (lane >> 4) + 4 * r; }
+      sm.Ar[kk][i] = ra[r].x;
+      sm.Ai[kk][i] = sa * ra[r].y;
+    }
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      int j, kk;
+      if (b_n) { kk = lane & 15; j = (lane >> 4) + 4 * r; }
+      else { j = lane & 31; kk = (lane >> 5) + 2 * r; }
+      sm.Br[kk][j] = rb[r].x;
+      sm.Bi[kk][j] = sb * rb[r].y;
+    }
+  };
+
+  double cr[2][8], ci[2][8];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { cr[i][q] = 0.0; ci[i][q] = 0.0; }
+
+  if (K > 0) load(0);
+  for (int k0 = 0; k0 < K; k0 += BK) {
+    store();
+    __syncthreads();
+    if (k0 + BK < K) load(k0 + BK);
+#pragma unroll
+    for (int ks = 0; ks < BK; ks += 4) {
+      const int kr = ks + (lane >> 4), c16 = lane & 15;
+      double ar[2], ai[2], nai[2], br[8], bi[8];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        ar[i] = sm.Ar[kr][i * 16 + c16];
+        ai[i] = sm.Ai[kr][i * 16 + c16];
+        nai[i] = -ai[i];
+      }
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        br[q] = sm.Br[kr][q * 4 + (lane & 3)];
+        bi[q] = sm.Bi[kr][q * 4 + (lane & 3)];
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(ar[i], br[q], cr[i][q], 0, 0, 0);
+          ci[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(ar[i], bi[q], ci[i][q], 0, 0, 0);
+        }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          cr[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(nai[i], bi[q], cr[i][q], 0, 0, 0);
+          ci[i][q] = __builtin_amdgcn_mfma_f64_4x4x4f64(ai[i], br[q], ci[i][q], 0, 0, 0);
+        }
+    }
+    __syncthreads();
+  }
+
+  // ---- epilogue: lane holds rows row0 + 16 i, columns col0 + 4 q ----
+  CT *C = p.C + (long long)b * p.sC;
+  CT *C2 = (p.C2 && (!p.active2 || p.active2[b] != 0)) ? p.C2 + (long long)b * p.sC2 : nullptr;
+  const bool useb = (p.beta.x != 0.0 || p.beta.y != 0.0);
+  const bool useb2 = (p.beta2.x != 0.0 || p.beta2.y != 0.0);
+  const double ar_ = p.alpha.x, ai_ = p.alpha.y;
+  const int row0 = m0 + 4 * ((lane >> 2) & 3) + (lane >> 4);
+  const int col0 = n0 + (lane & 3);
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    CT cv[8];
+    const int gr = row0 + i * 16;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int gc = col0 + 4 * q;
+      cv[q] = (useb && gr < M && gc < N) ? C[gr + (long long)gc * p.ldc] : CT{0, 0};
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const int gc = col0 + 4 * q;
+      if (gr < M && gc < N && !(gc >= p.skip_c0 && gc < p.skip_c1) &&
+          !(gr >= p.skip_r0 && gr < p.skip_r1)) {
+        const double xr = cr[i][q], xi = ci[i][q];
+        const double vr = ar_ * xr - ai_ * xi, vi = ar_ * xi + ai_ * xr;
+        double yr = vr, yi = vi;
+        if (useb) {
+          const CT c = cv[q];
+          yr += p.beta.x * c.x - p.beta.y * c.y;
+          yi += p.beta.x * c.y + p.beta.y * c.x;
+        }
+        C[gr + (long long)gc * p.ldc] = CT{yr, yi};
+        if (C2) {
+          const long long o2 = gr + (long long)gc * p.ldc2;
+          double zr = vr, zi = vi;
+          if (useb2) {
+            const CT c = C2[o2];
+            zr += p.beta2.x * c.x - p.beta2.y * c.y;
+            zi += p.beta2.x * c.y + p.beta2.y * c.x;
+          }
+          C2[o2] = CT{zr, zi};
+        }
+      }
+    }
+  }
+}
+
+// V: 0 the 64 x 64 tile, 1 the 64 x 128 tile, 2 the 32 x 32 single-wave tile
+// (FP64 only; FP32 always takes the 64 x 64 tile)
 template <typename R>
-void run(bool W, int use, const GemmArgs<R> &a, hipStream_t s) {
-  int tiles_m = (a.m + BM - 1) / BM, tiles_n = (a.n + (W ? WBN : BN) - 1) / (W ? WBN : BN);
+void run(int V, int use, const GemmArgs<R> &a, hipStream_t s) {
+  const bool W = V == 1, S1 = V == 2;
+  const int tr = S1 ? SBT : BM, tc = W ? WBN : S1 ? SBT : BN;
+  int tiles_m = (a.m + tr - 1) / tr, tiles_n = (a.n + tc - 1) / tc;
   long long total = (long long)tiles_m * tiles_n * a.batch;
   if (total > (1LL << 30)) throw std::runtime_error("gemm: grid too large");
   int total_pad = (int)((total + 7) / 8 * 8);
@@ -578,6 +752,19 @@ void run(bool W, int use, const GemmArgs<R> &a, hipStream_t s) {
                            tiles_n, (int)total, total_pad);
       else
         hipLaunchKernelGGL((k_gemm_w<PT_GEMM_MASKED>), dim3(total_pad), dim3(NTW), 0, s, a, tiles_m,
+                           tiles_n, (int)total, total_pad);
+      HN_CHECK(hipGetLastError());
+      return;
+    }
+    if (S1) {
+      if (use == PT_GEMM_SWEEP)
+        hipLaunchKernelGGL((k_gemm_s<PT_GEMM_SWEEP>), dim3(total_pad), dim3(NTS), 0, s, a, tiles_m,
+                           tiles_n, (int)total, total_pad);
+      else if (use == PT_GEMM_INV)
+        hipLaunchKernelGGL((k_gemm_s<PT_GEMM_INV>), dim3(total_pad), dim3(NTS), 0, s, a, tiles_m,
+                           tiles_n, (int)total, total_pad);
+      else
+        hipLaunchKernelGGL((k_gemm_s<PT_GEMM_MASKED>), dim3(total_pad), dim3(NTS), 0, s, a, tiles_m,
                            tiles_n, (int)total, total_pad);
       HN_CHECK(hipGetLastError());
       return;
@@ -598,12 +785,12 @@ void run(bool W, int use, const GemmArgs<R> &a, hipStream_t s) {
 // ---- tile-shape autotuner (FP64) ------------------------------------------
 // The first launch of each shape class -- (m, n, k, operand forms, beta and
 // dual-output and mask presence, skipped ranges, use, batch rounded down to
-// a power of two) -- times the 64 x 64 four-wave tile and the 64 x 128
-// eight-wave tile on the real operands, writing into a scratch output, and
-// keeps the fixed rule's tile unless the other is at least 3 % faster, for
-// every later launch of that class.  Both tilings run
-// the same MFMA chains in the same order, so the choice never changes a
-// result.  Off (the fixed rule: wide for k >= 64 and n >= 128) while a
+// a power of two) -- times the 64 x 64 four-wave tile, the 64 x 128
+// eight-wave tile (n >= 64) and the 32 x 32 single-wave tile on the real
+// operands, writing into a scratch output, and keeps the fixed rule's tile
+// unless another is at least 3 % faster, for every later launch of that
+// class.  The tilings run the same MFMA chains in the same order, so the
+// choice never changes a result.  Off (the fixed rule: wide for k >= 64 and n >= 128) while a
 // stream is being captured into a graph, with HIPNEGF_GEMM_TUNE=0, or when a
 // variant is forced (gemm_force_variant).
 struct TuneKey {
@@ -632,11 +819,11 @@ bool tune_on() {
   return on != 0;
 }
 
-bool pick_wide(const GemmArgs<double> &a, int use, hipStream_t s) {
+int pick_variant(const GemmArgs<double> &a, int use, hipStream_t s) {
   const int f = g_force.load();
-  if (f >= 0) return f == 1;
-  const bool rule = a.k >= 64 && a.n >= 128;
-  if (a.n < 64 || a.k < 16 || !tune_on()) return rule;
+  if (f >= 0) return f;
+  const int rule = a.k >= 64 && a.n >= 128 ? 1 : 0;
+  if (a.n < 16 || a.m < 16 || a.k < 16 || !tune_on()) return rule;
   hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
   if (hipStreamIsCapturing(s, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return rule;
   int bb = 0;
@@ -649,7 +836,7 @@ bool pick_wide(const GemmArgs<double> &a, int use, hipStream_t s) {
   {
     std::lock_guard<std::mutex> g(g_tune_mu);
     auto it = g_tune.find(key);
-    if (it != g_tune.end()) return it->second == 1;
+    if (it != g_tune.end()) return it->second;
   }
   // scratch outputs of the same layout (the real C and C2 are not touched)
   auto span = [&](long long sC, int ldc) {
@@ -682,12 +869,16 @@ bool pick_wide(const GemmArgs<double> &a, int use, hipStream_t s) {
   hipEvent_t e0, e1;
   HN_CHECK(hipEventCreate(&e0));
   HN_CHECK(hipEventCreate(&e1));
-  float ms[2] = {0.0f, 0.0f};
-  for (int v = 0; v < 2; ++v) {
-    run<double>(v == 1, use, t, s);  // first use of the code object
+  float ms[3] = {0.0f, 0.0f, 0.0f};
+  for (int v = 0; v < 3; ++v) {
+    if (v == 1 && a.n < 64) {  // the 64 x 128 tile would be mostly empty
+      ms[v] = 1e30f;
+      continue;
+    }
+    run<double>(v, use, t, s);  // first use of the code object
     HN_CHECK(hipEventRecord(e0, s));
-    run<double>(v == 1, use, t, s);
-    run<double>(v == 1, use, t, s);
+    run<double>(v, use, t, s);
+    run<double>(v, use, t, s);
     HN_CHECK(hipEventRecord(e1, s));
     HN_CHECK(hipEventSynchronize(e1));
     HN_CHECK(hipEventElapsedTime(&ms[v], e0, e1));
@@ -696,20 +887,23 @@ bool pick_wide(const GemmArgs<double> &a, int use, hipStream_t s) {
   HN_CHECK(hipEventDestroy(e1));
   HN_CHECK(hipFreeAsync(c1, s));
   if (c2) HN_CHECK(hipFreeAsync(c2, s));
-  // the fixed rule's tile unless the other is clearly faster (3 %): shapes
-  // where the two are within timing noise keep one choice from run to run
-  const int choice = rule ? (ms[0] < 0.97f * ms[1] ? 0 : 1) : (ms[1] < 0.97f * ms[0] ? 1 : 0);
+  // the fixed rule's tile unless another is clearly faster (3 %): shapes
+  // where the tiles are within timing noise keep one choice from run to run
+  int best = 0;
+  for (int v = 1; v < 3; ++v)
+    if (ms[v] < ms[best]) best = v;
+  const int choice = ms[best] < 0.97f * ms[rule] ? best : rule;
   std::lock_guard<std::mutex> g(g_tune_mu);
   g_tune[key] = choice;
-  return choice == 1;
+  return choice;
 }
 
 template <typename R>
 void launch(const GemmArgs<R> &a, hipStream_t s) {
   if (a.m <= 0 || a.n <= 0 || a.batch <= 0) return;
   const int use = a.tag >= 0 ? a.tag : (a.active ? PT_GEMM_MASKED : PT_GEMM_SWEEP);
-  bool W = false;
-  if constexpr (sizeof(R) == 8) W = pick_wide(a, use, s);
+  int V = 0;
+  if constexpr (sizeof(R) == 8) V = pick_variant(a, use, s);
   // algorithmic work: 8 real flops per complex multiply-add over the columns written
   const int skipped = std::max(0, std::min(a.skip_c1, a.n) - std::max(a.skip_c0, 0));
   const int skipped_r = std::max(0, std::min(a.skip_r1, a.m) - std::max(a.skip_r0, 0));
@@ -723,9 +917,10 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
   const double nb = items < 0 ? (double)a.batch : items;
   const double bytes = cb * ((a.sA ? nb : 1.0) * a.m * a.k + (a.sB ? nb : 1.0) * a.k * a.n +
                              (rc ? 2.0 : 1.0) * nb * a.m * ncol);
-  ProfScope ps(use, flops, bytes, s, sizeof(R) == 8 ? (W ? 1 : 0) : 2);
+  // profile variant: 0 the 64 x 64 FP64 tile, 1 64 x 128, 2 FP32, 3 the 32 x 32 single-wave tile
+  ProfScope ps(use, flops, bytes, s, sizeof(R) == 8 ? (V == 2 ? 3 : V) : 2);
   if (a.prof_token) *a.prof_token = ps.tok;
-  run<R>(W, use, a, s);
+  run<R>(V, use, a, s);
 }
 
 }  // namespace
@@ -733,13 +928,16 @@ void launch(const GemmArgs<R> &a, hipStream_t s) {
 void gemm_z(const GemmArgs<double> &a, hipStream_t s) { launch<double>(a, s); }
 void gemm_c(const GemmArgs<float> &a, hipStream_t s) { launch<float>(a, s); }
 
-void gemm_force_variant(int v) { g_force.store(v < 0 ? -1 : (v ? 1 : 0)); }
+void gemm_force_variant(int v) { g_force.store(v < 0 || v > 2 ? -1 : v); }
 
-void gemm_tune_stats(long long out[3]) {
+void gemm_tune_stats(long long out[4]) {
   std::lock_guard<std::mutex> g(g_tune_mu);
   out[0] = (long long)g_tune.size();
-  out[1] = 0;
-  for (auto &kv : g_tune) out[1] += kv.second;
+  out[1] = out[3] = 0;
+  for (auto &kv : g_tune) {
+    out[1] += kv.second == 1;
+    out[3] += kv.second == 2;
+  }
   out[2] = tune_on() ? 1 : 0;
 }
 

@@ -143,18 +143,18 @@ PT_NAMES = {PT_GEMM_MASKED: "k_gemm_w<0>", PT_GEMM_SWEEP: "k_gemm_w<1>",
 
 
 def gemm_force_variant(v: int):
-    """FP64 GEMM tile: -1 autotuned, 0 always 64 x 64, 1 always 64 x 128."""
+    """FP64 GEMM tile: -1 autotuned, 0 always 64 x 64, 1 always 64 x 128, 2 always 32 x 32."""
     L = load()
     L.hipnegf_gemm_force_variant.argtypes = [C.c_int]
     check(L.hipnegf_gemm_force_variant(int(v)))
 
 
 def gemm_tune_stats() -> dict:
-    out = np.zeros(3, dtype=np.int64)
+    out = np.zeros(4, dtype=np.int64)
     L = load()
     L.hipnegf_gemm_tune_stats.argtypes = [C.c_void_p]
     check(L.hipnegf_gemm_tune_stats(_ptr(out)))
-    return dict(shapes=int(out[0]), wide=int(out[1]), on=bool(out[2]))
+    return dict(shapes=int(out[0]), wide=int(out[1]), on=bool(out[2]), single_wave=int(out[3]))
 
 
 def profile_enable(on: bool):
@@ -180,7 +180,8 @@ def profile_kernels() -> dict:
     out = {}
     for t, name in PT_NAMES.items():
         if t in (PT_GEMM_MASKED, PT_GEMM_SWEEP, PT_GEMM_INV):
-            for v, kname in ((1, f"k_gemm_w<{t}>"), (0, f"k_gemm<double, {t}>"), (2, f"k_gemm<float, {t}>")):
+            for v, kname in ((1, f"k_gemm_w<{t}>"), (0, f"k_gemm<double, {t}>"), (2, f"k_gemm<float, {t}>"),
+                             (3, f"k_gemm_s<{t}>")):
                 ms, fl, by, n = C.c_double(), C.c_double(), C.c_double(), C.c_longlong()
                 check(L.hipnegf_profile_read_variant(t, v, C.byref(ms), C.byref(fl), C.byref(by),
                                                      C.byref(n)))

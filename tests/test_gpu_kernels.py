@@ -152,37 +152,44 @@ def test_cinverse_batched(gpu, n):
         assert res < 1e-5, res
 
 
-@pytest.mark.parametrize("m,n,k,beta", [(64, 64, 64, 0.0), (128, 128, 128, 1.0), (256, 256, 64, 1.0),
-                                        (100, 200, 72, 0.0)])
-def test_gemm_tile_variants_bitwise(gpu, m, n, k, beta):
-    """The autotuner's two FP64 tilings (64 x 64 four-wave, 64 x 128 eight-wave)
-    run the same MFMA chains in the same order: bitwise the same C, so the
-    tuner's choice never changes a result.  Then the tuner itself records the
-    shape class."""
+@pytest.mark.parametrize("m,n,k,beta,ops", [(64, 64, 64, 0.0, (0, 0)), (128, 128, 128, 1.0, (0, 0)),
+                                            (256, 256, 64, 1.0, (0, 0)), (100, 200, 72, 0.0, (0, 0)),
+                                            (32, 32, 32, 1.0, (0, 0)), (48, 40, 24, 1.0, (1, 2)),
+                                            (70, 90, 33, 0.0, (2, 1)), (64, 64, 64, 1.0, (2, 2))])
+def test_gemm_tile_variants_bitwise(gpu, m, n, k, beta, ops):
+    """The autotuner's three FP64 tilings (64 x 64 four-wave, 64 x 128
+    eight-wave, 32 x 32 single-wave) run the same MFMA chains in the same
+    order: bitwise the same C for every operand form, so the tuner's choice
+    never changes a result.  Then the tuner itself records the shape class."""
     torch, hip, eng = gpu
-    rng = np.random.default_rng(m + n + k)
+    rng = np.random.default_rng(m + n + k + 7 * ops[0] + ops[1])
     batch = 5
-    A = rng.standard_normal((batch, m * k)) + 1j * rng.standard_normal((batch, m * k))
-    B = rng.standard_normal((batch, k * n)) + 1j * rng.standard_normal((batch, k * n))
+    opa, opb = ops
+    ra, ca = (m, k) if opa == 0 else (k, m)   # stored shapes (column-major, ld = rows)
+    rb, cb = (k, n) if opb == 0 else (n, k)
+    A = rng.standard_normal((batch, ra * ca)) + 1j * rng.standard_normal((batch, ra * ca))
+    B = rng.standard_normal((batch, rb * cb)) + 1j * rng.standard_normal((batch, rb * cb))
     C0 = rng.standard_normal((batch, m * n)) + 1j * rng.standard_normal((batch, m * n))
     dA, dB = torch.from_numpy(A).cuda(), torch.from_numpy(B).cuda()
     al = np.array([1.0, 0.5])
     be = np.array([beta, 0.0])
     out = []
     try:
-        for v in (0, 1, -1):
+        for v in (0, 1, 2, -1):
             hip.gemm_force_variant(v)
             dC = torch.from_numpy(C0.copy()).cuda()
             hip.check(hip.load().hipnegf_zgemm_batched(eng.handle, m, n, k, hip._ptr(al), dA.data_ptr(),
-                                                        m * k, m, 0, dB.data_ptr(), k * n, k, 0,
+                                                        ra * ca, ra, opa, dB.data_ptr(), rb * cb, rb, opb,
                                                         hip._ptr(be), dC.data_ptr(), m * n, m, batch))
             eng.sync()
             out.append(dC.cpu().numpy())
     finally:
         hip.gemm_force_variant(-1)
-    np.testing.assert_array_equal(out[0], out[1])
-    np.testing.assert_array_equal(out[2], out[0])
-    ref = np.stack([(al[0] + 1j * al[1]) * (A[b].reshape((m, k), order="F") @ B[b].reshape((k, n), order="F"))
+    for o in out[1:]:
+        np.testing.assert_array_equal(o, out[0])
+    op = lambda X, o: X if o == 0 else (X.T if o == 1 else X.conj().T)
+    ref = np.stack([(al[0] + 1j * al[1]) * (op(A[b].reshape((ra, ca), order="F"), opa)
+                                            @ op(B[b].reshape((rb, cb), order="F"), opb))
                     + beta * C0[b].reshape((m, n), order="F") for b in range(batch)])
     got = np.stack([out[0][b].reshape((m, n), order="F") for b in range(batch)])
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
