@@ -876,11 +876,37 @@ static void gn_tridiag(const esh_t *e, zc *const *g, zc *const *Grd, zc *const *
   }
 }
 
+/* Inelastic self-energy blocks of one (E,k) point, added to ESH and Sigma_n
+ * by ElPhonInel's add_sigma_r / add_sigma_n (elphinel.F90:540-621):
+ * Srd, Snd [nbl]; Sru, Srl, Snu [nbl-1] (used when tridiag != 0). */
+typedef struct {
+  const zc *const *Srd, *const *Sru, *const *Srl, *const *Snd, *const *Snu;
+  int tridiag;
+} inel_in_t;
+/* G^r / G^n tridiagonal blocks of the last pass, as cache_Gr / cache_Gn store
+ * them for the inelastic model (iterative.F90:574-618, set_Gr_inel :643-668). */
+typedef struct {
+  zc *const *Grd, *const *Gru, *const *Grl, *const *Gnd, *const *Gnu, *const *Gnl;
+} inel_out_t;
+
+static int scba_core(const or_struct *s, double E, zc *const *Sigma, const double *coupling,
+                     int niter, double tol, const double *frm, int ref, double *curr,
+                     double *err_out, zc *gn_diag, zc *gr_diag, double eps_drop,
+                     const inel_in_t *in, const inel_out_t *out);
+
 /* calculate_elastic_scba (iterative.F90:289-363) + ElPhonDephD (elphdd.F90)
  * + iterative_meir_wingreen (iterative.F90:382-414). */
 int or_scba_mw_sigma(const or_struct *s, double E, zc *const *Sigma, const double *coupling,
                      int niter, double tol, const double *frm, int ref, double *curr,
                      double *err_out, zc *gn_diag, zc *gr_diag, double eps_drop) {
+  return scba_core(s, E, Sigma, coupling, niter, tol, frm, ref, curr, err_out, gn_diag, gr_diag,
+                   eps_drop, NULL, NULL);
+}
+
+static int scba_core(const or_struct *s, double E, zc *const *Sigma, const double *coupling,
+                     int niter, double tol, const double *frm, int ref, double *curr,
+                     double *err_out, zc *gn_diag, zc *gr_diag, double eps_drop,
+                     const inel_in_t *in, const inel_out_t *out) {
   int nbl = s->nbl, ncont = s->ncont;
   int ndev = 0;
   int *off = (int *)calloc(nbl + 1, sizeof(int));
@@ -914,6 +940,15 @@ int or_scba_mw_sigma(const or_struct *s, double E, zc *const *Sigma, const doubl
     if (iter > 0) /* add_sigma_r, elphdd.F90:108-128 */
       for (int i = 0; i < nbl; ++i)
         for (int d = 0; d < s->bsz[i]; ++d) e.d[i][d + (size_t)d * s->bsz[i]] -= sig_r[off[i] + d];
+    if (in) /* inelastic add_sigma_r, elphinel.F90:540-579 */
+      for (int i = 0; i < nbl; ++i) {
+        size_t m = (size_t)s->bsz[i] * s->bsz[i];
+        for (size_t t = 0; t < m; ++t) e.d[i][t] -= in->Srd[i][t];
+        if (in->tridiag && i + 1 < nbl) {
+          size_t mo = (size_t)s->bsz[i] * s->bsz[i + 1];
+          for (size_t t = 0; t < mo; ++t) { e.u[i][t] -= in->Sru[i][t]; e.l[i][t] -= in->Srl[i][t]; }
+        }
+      }
     for (int i = 0; i < nbl; ++i) { free(g[i]); g[i] = 0; }
     gsmr_blocks(&e, 2, g);
     for (int i = 0; i < nbl; ++i) { free(Grd[i]); Grd[i] = 0; free(Gnd[i]); Gnd[i] = 0; }
@@ -941,6 +976,20 @@ int or_scba_mw_sigma(const or_struct *s, double E, zc *const *Sigma, const doubl
     if (iter > 0) /* add_sigma_n, elphdd.F90:135-156 (uses sigma_n of previous pass) */
       for (int i = 0; i < nbl; ++i)
         for (int d = 0; d < s->bsz[i]; ++d) Snd[i][d + (size_t)d * s->bsz[i]] += sig_n[off[i] + d];
+    if (in) /* inelastic add_sigma_n, elphinel.F90:585-621: Sigma_n(i+1,i) = Sigma_n(i,i+1)^H */
+      for (int i = 0; i < nbl; ++i) {
+        int ni = s->bsz[i];
+        for (size_t t = 0; t < (size_t)ni * ni; ++t) Snd[i][t] += in->Snd[i][t];
+        if (in->tridiag && i + 1 < nbl) {
+          int nj = s->bsz[i + 1];
+          for (int c = 0; c < nj; ++c)
+            for (int r = 0; r < ni; ++r) {
+              zc v = in->Snu[i][r + (size_t)c * ni];
+              Snu[i][r + (size_t)c * ni] += v;
+              Snl[i][c + (size_t)r * nj] += conj(v);
+            }
+        }
+      }
     for (int j = 0; j < ncont; ++j) {
       double fd = frm[j] - frm[ref - 1];
       if (j != ref - 1 && fabs(fd) > eps_drop) {
@@ -1017,6 +1066,19 @@ int or_scba_mw_sigma(const or_struct *s, double E, zc *const *Sigma, const doubl
   if (gr_diag)
     for (int i = 0; i < nbl; ++i)
       for (int d = 0; d < s->bsz[i]; ++d) gr_diag[off[i] + d] = Grd[i][d + (size_t)d * s->bsz[i]];
+  if (out)
+    for (int i = 0; i < nbl; ++i) {
+      size_t m = (size_t)s->bsz[i] * s->bsz[i];
+      memcpy(out->Grd[i], Grd[i], m * sizeof(zc));
+      memcpy(out->Gnd[i], Gnd[i], m * sizeof(zc));
+      if (i + 1 < nbl) {
+        size_t mo = (size_t)s->bsz[i] * s->bsz[i + 1];
+        memcpy(out->Gru[i], Gru[i], mo * sizeof(zc));
+        memcpy(out->Grl[i], Grl[i], mo * sizeof(zc));
+        memcpy(out->Gnu[i], Gnu[i], mo * sizeof(zc));
+        memcpy(out->Gnl[i], Gnl[i], mo * sizeof(zc));
+      }
+    }
   if (err_out) *err_out = err;
   for (int i = 0; i < nbl; ++i) { free(g[i]); free(Grd[i]); free(Gnd[i]); free(Snd[i]); }
   for (int i = 0; i + 1 < nbl; ++i) {
@@ -1045,6 +1107,302 @@ int or_scba_meir_wingreen(const or_struct *s, double E, double delta, const doub
   for (int c = 0; c < s->ncont; ++c) free(Sig[c]);
   free(Sig);
   return it;
+}
+
+/* ======================================================================= */
+/* inelastic electron-phonon SCBA (elphinel.F90, self_energy.F90)           */
+/* ======================================================================= */
+
+/* set_reciprocal_vectors, elphinel.F90:1118-1143: rec(:,j) = 2 pi a_k x a_l / vol */
+static void reciprocal(const double *a, double *rec) {
+  const double *a1 = a, *a2 = a + 3, *a3 = a + 6;
+  double c[3][3] = {{a2[1] * a3[2] - a2[2] * a3[1], a2[2] * a3[0] - a2[0] * a3[2], a2[0] * a3[1] - a2[1] * a3[0]},
+                    {a3[1] * a1[2] - a3[2] * a1[1], a3[2] * a1[0] - a3[0] * a1[2], a3[0] * a1[1] - a3[1] * a1[0]},
+                    {a1[1] * a2[2] - a1[2] * a2[1], a1[2] * a2[0] - a1[0] * a2[2], a1[0] * a2[1] - a1[1] * a2[0]}};
+  double vol = a1[0] * c[0][0] + a1[1] * c[0][1] + a1[2] * c[0][2];
+  for (int j = 0; j < 3; ++j)
+    for (int r = 0; r < 3; ++r) rec[r + 3 * j] = 2.0 * M_PI * c[j][r] / vol;
+}
+
+/* absolute k-points: rec . kfrac, or 0 when all lattice vectors are 0
+ * (prepare_POKmat, elphinel.F90:345-350) */
+static void abs_kpoints(int nk, const double *kfrac, const double *latt, double *kabs) {
+  int zero = 1;
+  for (int t = 0; t < 9; ++t) zero &= latt[t] == 0.0;
+  double rec[9];
+  if (!zero) reciprocal(latt, rec);
+  for (int k = 0; k < nk; ++k)
+    for (int r = 0; r < 3; ++r) {
+      double v = 0.0;
+      if (!zero)
+        for (int j = 0; j < 3; ++j) v += rec[r + 3 * j] * kfrac[j + 3 * k];
+      kabs[r + 3 * k] = v;
+    }
+}
+
+/* prepare_POKmat (elphinel.F90:318-406) without equivalent k-points:
+ * K[iz + ndz (iK + nk iQ)] = Ce w(iQ) (2Q^2 + q0^2 (1 - b z)) exp(-b z) / (4 b^3),
+ * Q = k(iK) - k(iQ), b = sqrt(q0^2 + Q^2), z = iz dz,
+ * Ce = coupling wq / 2 / area (1/eps_inf - 1/eps0). */
+void or_inel_kmat_po(int ndz, double dz, int nk, const double *kfrac, const double *kw,
+                     const double *latt, double coupling, double wq, double area, double eps0,
+                     double eps_inf, double q0, double *K) {
+  double *ka = (double *)calloc(3 * (size_t)nk, sizeof(double));
+  abs_kpoints(nk, kfrac, latt, ka);
+  double Ce = coupling * wq / 2.0 / area * (1.0 / eps_inf - 1.0 / eps0);
+  for (int iQ = 0; iQ < nk; ++iQ)
+    for (int iK = 0; iK < nk; ++iK) {
+      double Q2 = 0.0;
+      for (int r = 0; r < 3; ++r) {
+        double d = ka[r + 3 * iK] - ka[r + 3 * iQ];
+        Q2 += d * d;
+      }
+      double bb = sqrt(q0 * q0 + Q2);
+      for (int iz = 0; iz < ndz; ++iz) {
+        double z = iz * dz;
+        double Kf = (2.0 * Q2 + q0 * q0 * (1.0 - bb * z)) * exp(-bb * z) / (4.0 * bb * bb * bb);
+        K[iz + (size_t)ndz * (iK + (size_t)nk * iQ)] = Ce * kw[iQ] * Kf;
+      }
+    }
+  free(ka);
+}
+
+/* prepare_NonPOKmat (elphinel.F90:411-519) without equivalent k-points:
+ * K = coupling w(iQ) D0 (k.q)^2 exp(-|q| z) / (2 |q|^2 |k|^2); 0 if k or q = 0. */
+void or_inel_kmat_nonpo(int ndz, double dz, int nk, const double *kfrac, const double *kw,
+                        const double *latt, double coupling, double D0, double *K) {
+  double *ka = (double *)calloc(3 * (size_t)nk, sizeof(double));
+  abs_kpoints(nk, kfrac, latt, ka);
+  for (int iQ = 0; iQ < nk; ++iQ) {
+    const double *kq = ka + 3 * iQ;
+    double kq2 = kq[0] * kq[0] + kq[1] * kq[1] + kq[2] * kq[2];
+    for (int iK = 0; iK < nk; ++iK) {
+      const double *kk = ka + 3 * iK;
+      double kk2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+      double QQ = kk[0] * kq[0] + kk[1] * kq[1] + kk[2] * kq[2];
+      double Q2 = QQ * QQ;
+      for (int iz = 0; iz < ndz; ++iz) {
+        double v = 0.0;
+        if (kq2 != 0.0 && kk2 != 0.0) {
+          double Kf = D0 * Q2 * exp(-sqrt(kq2) * (iz * dz)) / (2.0 * kq2 * kk2);
+          v = coupling * kw[iQ] * Kf;
+        }
+        K[iz + (size_t)ndz * (iK + (size_t)nk * iQ)] = v;
+      }
+    }
+  }
+  free(ka);
+}
+
+/* Per-point block stores: point p = iK * ne + iE; the diagonal blocks of a
+ * point sit at p * nd + od[i], the off-diagonal ones at p * no + oo[i]. */
+typedef struct {
+  int nbl;
+  const int *b;
+  size_t od[4096], oo[4096], nd, no;
+} blk_layout;
+
+static int layout_init(blk_layout *L, int nbl, const int *b) {
+  if (nbl > 4096) return -1;
+  L->nbl = nbl;
+  L->b = b;
+  L->nd = L->no = 0;
+  for (int i = 0; i < nbl; ++i) {
+    L->od[i] = L->nd;
+    L->nd += (size_t)b[i] * b[i];
+    L->oo[i] = L->no;
+    if (i + 1 < nbl) L->no += (size_t)b[i] * b[i + 1];
+  }
+  return 0;
+}
+
+/* selfenergy (self_energy.F90:41-321) on one process for one block pair:
+ * Sig(iE,iK)_rc (+)= sum_iQ K(|zr_r - zc_c|, iK, iQ) (fm G(iE-s, iQ)_rc + fp G(iE+s, iQ)_rc),
+ * G outside the grid = 0.  G / Sig: element e of point p at X[p * sp + e]. */
+static void selfenergy_blk(int ne, int nk, const double *K, int ndz, int rows, int cols,
+                           const int *izr, const int *izc, int shift, zc fm, zc fp, const zc *G,
+                           size_t sp, zc *Sig, int accumulate) {
+  for (int iE = 0; iE < ne; ++iE)
+    for (int iK = 0; iK < nk; ++iK) {
+      zc *S = Sig + (size_t)(iK * ne + iE) * sp;
+      if (!accumulate)
+        for (int e = 0; e < rows * cols; ++e) S[e] = 0.0;
+      for (int iQ = 0; iQ < nk; ++iQ) {
+        const zc *Gm = iE - shift >= 0 ? G + (size_t)(iQ * ne + iE - shift) * sp : NULL;
+        const zc *Gp = iE + shift < ne ? G + (size_t)(iQ * ne + iE + shift) * sp : NULL;
+        for (int c = 0; c < cols; ++c)
+          for (int r = 0; r < rows; ++r) {
+            int e = r + c * rows;
+            zc sb = (Gm ? fm * Gm[e] : 0.0) + (Gp ? fp * Gp[e] : 0.0);
+            int dz = abs(izr[r] - izc[c]);
+            S[e] += K[dz + (size_t)ndz * (iK + (size_t)nk * iQ)] * sb;
+          }
+      }
+    }
+}
+
+/* compute_Sigma_n then compute_Sigma_r (elphinel.F90:942-1100, 729-938) for
+ * every point from the G stores (layout above, one store per block kind).
+ * Sigma_n: (i,i) and (i,i+1) from G^n with (Nq, Nq+1); Sigma_r: (i,i), (i,i+1),
+ * (i+1,i) from G^r with (Nq+1, Nq) plus G^n with (i/2, -i/2).  iz[ndev]:
+ * z bin of every device orbital; |iz_r - iz_c| < ndz is required. */
+int or_inel_sigma(int nbl, const int *bsz, int ne, int nk, const double *K, int ndz,
+                  const int *iz, int shift, double Nq, int tridiag, const zc *Grd,
+                  const zc *Gru, const zc *Grl, const zc *Gnd, const zc *Gnu, const zc *Gnl,
+                  zc *Srd, zc *Sru, zc *Srl, zc *Snd, zc *Snu) {
+  blk_layout L;
+  if (layout_init(&L, nbl, bsz)) return -1;
+  int *off = (int *)calloc(nbl + 1, sizeof(int));
+  for (int i = 0; i < nbl; ++i) off[i + 1] = off[i] + bsz[i];
+  for (int i = 0; i < off[nbl]; ++i)
+    for (int j = 0; j < off[nbl]; ++j)
+      if (abs(iz[i] - iz[j]) >= ndz) { free(off); return -2; }
+  const zc nq = Nq, nq1 = Nq + 1.0, ih = 0.5 * I;
+  for (int i = 0; i < nbl; ++i) {
+    int n = bsz[i];
+    const int *zr = iz + off[i];
+    selfenergy_blk(ne, nk, K, ndz, n, n, zr, zr, shift, nq, nq1, Gnd + L.od[i], L.nd, Snd + L.od[i], 0);
+    if (tridiag && i + 1 < nbl) {
+      const int *zc_ = iz + off[i + 1];
+      selfenergy_blk(ne, nk, K, ndz, n, bsz[i + 1], zr, zc_, shift, nq, nq1, Gnu + L.oo[i], L.no,
+                     Snu + L.oo[i], 0);
+    }
+  }
+  for (int i = 0; i < nbl; ++i) {
+    int n = bsz[i];
+    const int *zr = iz + off[i];
+    selfenergy_blk(ne, nk, K, ndz, n, n, zr, zr, shift, nq1, nq, Grd + L.od[i], L.nd, Srd + L.od[i], 0);
+    selfenergy_blk(ne, nk, K, ndz, n, n, zr, zr, shift, ih, -ih, Gnd + L.od[i], L.nd, Srd + L.od[i], 1);
+    if (tridiag && i + 1 < nbl) {
+      int m = bsz[i + 1];
+      const int *zc_ = iz + off[i + 1];
+      selfenergy_blk(ne, nk, K, ndz, n, m, zr, zc_, shift, nq1, nq, Gru + L.oo[i], L.no, Sru + L.oo[i], 0);
+      selfenergy_blk(ne, nk, K, ndz, n, m, zr, zc_, shift, ih, -ih, Gnu + L.oo[i], L.no, Sru + L.oo[i], 1);
+      selfenergy_blk(ne, nk, K, ndz, m, n, zc_, zr, shift, nq1, nq, Grl + L.oo[i], L.no, Srl + L.oo[i], 0);
+      selfenergy_blk(ne, nk, K, ndz, m, n, zc_, zr, shift, ih, -ih, Gnl + L.oo[i], L.no, Srl + L.oo[i], 1);
+    }
+  }
+  free(off);
+  return 0;
+}
+
+/* meir_wingreen with an inelastic model (integrations.F90:1804-1966): outer
+ * SCBA pass p = 0, 1, ... while not converged and p <= niter_inel:
+ *   every (E,k) point: contacts at E + i delta, calculate_elastic_scba with the
+ *   inelastic Sigma of pass p-1 added (none at p = 0), iterative_meir_wingreen;
+ *   curr_mat(iE,c) = sum_k kw(k) I_c(E,k); compute_Sigmas_inelastic from the
+ *   cached G^r/G^n blocks; currents(c) = g_spin integrate_el_meir_wingreen;
+ *   check_J_convergence (scba.F90:150-173): converged if max|J - J_old| < tol
+ *   (pass 0 only stores J_old).
+ * sk[nk]: the structure at each k-point (same partition), kw[nk] weights,
+ * E[ne] the uniform grid, frm[ne * (ncont+1)] occupations per energy with
+ * frm[iE*(ncont+1) + ref-1] the reference, coupling[ndev] the elastic dephasing
+ * coupling (may be NULL: none) with niter_ela / tol_ela.  Outputs: curr_mat
+ * [ne * ncont] (iE + ne c), currents[ncont], err_out (last max|dJ|),
+ * iters_ela[ne*nk] (may be NULL: elastic passes of the last outer pass),
+ * stores[11] (NULL, or entries NULL to skip): Grd, Gru, Grl, Gnd, Gnu, Gnl of
+ * the last pass and Srd, Sru, Srl, Snd, Snu computed from them, in the
+ * per-point layout.  Returns the number of outer passes, or < 0 on bad input. */
+int or_inel_meir_wingreen(int nk, const or_struct *const *sk, const double *kw, int ne,
+                          const double *E, double delta, const double *coupling, int niter_ela,
+                          double tol_ela, const double *frm, int ref, const double *K, int ndz,
+                          const int *iz, int shift, double Nq, int tridiag, int niter_inel,
+                          double tol_inel, int integration, double estep, double g_spin,
+                          double *curr_mat, double *currents, double *err_out, int *iters_ela,
+                          zc *const *stores, double sgfacc, double eps_drop, int nthreads) {
+  const or_struct *s0 = sk[0];
+  int nbl = s0->nbl, ncont = s0->ncont, np = ne * nk;
+  blk_layout L;
+  if (layout_init(&L, nbl, s0->bsz)) return -1;
+  int ndev = 0;
+  for (int i = 0; i < nbl; ++i) ndev += s0->bsz[i];
+  double *cpl = (double *)calloc(ndev ? ndev : 1, sizeof(double));
+  if (coupling) memcpy(cpl, coupling, sizeof(double) * ndev);
+  /* contact self-energies of every point (the same each pass) */
+  zc **Sig = (zc **)calloc((size_t)np * ncont, sizeof(zc *));
+#ifdef _OPENMP
+  if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+#endif
+  for (int p = 0; p < np; ++p) {
+    const or_struct *s = sk[p / ne];
+    for (int c = 0; c < ncont; ++c) {
+      int nd = s->bsz[s->cblk[c] - 1];
+      Sig[(size_t)p * ncont + c] = zalloc((size_t)nd * nd);
+      contact_sigma_of(s, c, E[p % ne] + I * delta, Sig[(size_t)p * ncont + c], 0, 0, sgfacc);
+    }
+  }
+  size_t NDs = (size_t)np * L.nd, NOs = (size_t)np * (L.no ? L.no : 1);
+  zc *Grd = zalloc(NDs), *Gnd = zalloc(NDs), *Srd = zalloc(NDs), *Snd = zalloc(NDs);
+  zc *Gru = zalloc(NOs), *Grl = zalloc(NOs), *Gnu = zalloc(NOs), *Gnl = zalloc(NOs);
+  zc *Sru = zalloc(NOs), *Srl = zalloc(NOs), *Snu = zalloc(NOs);
+  double *cur = (double *)calloc((size_t)np * ncont, sizeof(double));
+  int *it_pt = (int *)calloc(np, sizeof(int));
+  double *Jold = (double *)calloc(ncont, sizeof(double));
+  double *colv = (double *)calloc(ne, sizeof(double));
+  int have_old = 0, converged = 0, pass = 0, rc = 0;
+  double err = 0.0;
+  while (!converged && pass <= niter_inel) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic) num_threads(nthreads)
+#endif
+    for (int p = 0; p < np; ++p) {
+      const or_struct *s = sk[p / ne];
+      zc **pd = (zc **)calloc(6 * (size_t)nbl + 5 * (size_t)nbl, sizeof(zc *));
+      zc **gd = pd, **gu = pd + nbl, **gl = pd + 2 * nbl, **nd_ = pd + 3 * nbl,
+         **nu = pd + 4 * nbl, **nl = pd + 5 * nbl;
+      zc **sd = pd + 6 * nbl, **su = pd + 7 * nbl, **sl = pd + 8 * nbl, **snd = pd + 9 * nbl,
+         **snu = pd + 10 * nbl;
+      for (int i = 0; i < nbl; ++i) {
+        size_t d = (size_t)p * L.nd + L.od[i], o = (size_t)p * L.no + L.oo[i];
+        gd[i] = Grd + d; nd_[i] = Gnd + d; sd[i] = Srd + d; snd[i] = Snd + d;
+        if (i + 1 < nbl) {
+          gu[i] = Gru + o; gl[i] = Grl + o; nu[i] = Gnu + o; nl[i] = Gnl + o;
+          su[i] = Sru + o; sl[i] = Srl + o; snu[i] = Snu + o;
+        }
+      }
+      inel_in_t in = {(const zc *const *)sd, (const zc *const *)su, (const zc *const *)sl,
+                      (const zc *const *)snd, (const zc *const *)snu, tridiag};
+      inel_out_t out = {gd, gu, gl, nd_, nu, nl};
+      it_pt[p] = scba_core(s, E[p % ne], Sig + (size_t)p * ncont, cpl, niter_ela, tol_ela,
+                           frm + (size_t)(p % ne) * (ncont + 1), ref, cur + (size_t)p * ncont, NULL,
+                           NULL, NULL, eps_drop, pass > 0 ? &in : NULL, &out);
+      free(pd);
+    }
+    for (size_t t = 0; t < (size_t)ne * ncont; ++t) curr_mat[t] = 0.0;
+    for (int iK = 0; iK < nk; ++iK)
+      for (int iE = 0; iE < ne; ++iE)
+        for (int c = 0; c < ncont; ++c)
+          curr_mat[iE + (size_t)ne * c] += cur[(size_t)(iK * ne + iE) * ncont + c] * kw[iK];
+    rc = or_inel_sigma(nbl, s0->bsz, ne, nk, K, ndz, iz, shift, Nq, tridiag, Grd, Gru, Grl, Gnd,
+                       Gnu, Gnl, Srd, Sru, Srl, Snd, Snu);
+    if (rc) break;
+    for (int c = 0; c < ncont; ++c) {
+      for (int iE = 0; iE < ne; ++iE) colv[iE] = curr_mat[iE + (size_t)ne * c];
+      currents[c] = or_integrate_el_mw(integration, colv, ne, estep) * g_spin;
+    }
+    if (have_old) {
+      err = 0.0;
+      for (int c = 0; c < ncont; ++c) err = fmax(err, fabs(Jold[c] - currents[c]));
+      converged = err < tol_inel;
+    }
+    for (int c = 0; c < ncont; ++c) Jold[c] = currents[c];
+    have_old = 1;
+    pass++;
+  }
+  if (iters_ela) memcpy(iters_ela, it_pt, sizeof(int) * np);
+  if (stores) { /* the last pass's G blocks and the Sigma blocks computed from them */
+    zc *src[11] = {Grd, Gru, Grl, Gnd, Gnu, Gnl, Srd, Sru, Srl, Snd, Snu};
+    for (int t = 0; t < 11; ++t)
+      if (stores[t]) memcpy(stores[t], src[t], sizeof(zc) * ((t == 0 || t == 3 || t == 6 || t == 9) ? NDs : (size_t)np * L.no));
+  }
+  if (err_out) *err_out = err;
+  for (size_t t = 0; t < (size_t)np * ncont; ++t) free(Sig[t]);
+  free(Sig);
+  free(Grd); free(Gnd); free(Srd); free(Snd); free(Gru); free(Grl); free(Gnu); free(Gnl);
+  free(Sru); free(Srl); free(Snu);
+  free(cur); free(it_pt); free(Jold); free(colv); free(cpl);
+  return rc ? -2 : pass;
 }
 
 /* ======================================================================= */

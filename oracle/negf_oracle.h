@@ -191,4 +191,32 @@ int dd_transmission(const or_struct *s, double E, double delta, double sgfacc, d
 int dd_transmission_batch(const or_struct *s, int npts, const double *E, double delta,
                           double sgfacc, double *T_out, int *ncyc_out, int nthreads);
 
+/* ---- inelastic electron-phonon SCBA (elphinel.F90, self_energy.F90) ---- */
+/* K tables K[iz + ndz (iK + nk iQ)] of prepare_POKmat / prepare_NonPOKmat
+ * (elphinel.F90:318-519, no equivalent k-points).  kfrac[3*nk] fractional
+ * k-points, latt[9] lattice vectors column-wise (all 0: k = 0). */
+void or_inel_kmat_po(int ndz, double dz, int nk, const double *kfrac, const double *kw,
+                     const double *latt, double coupling, double wq, double area, double eps0,
+                     double eps_inf, double q0, double *K);
+void or_inel_kmat_nonpo(int ndz, double dz, int nk, const double *kfrac, const double *kw,
+                        const double *latt, double coupling, double D0, double *K);
+/* compute_Sigma_n + compute_Sigma_r for every point p = iK*ne + iE from the
+ * per-point G stores (diagonal blocks of a point contiguous, then the
+ * off-diagonal ones); returns 0, -2 if a z-bin difference reaches ndz. */
+int or_inel_sigma(int nbl, const int *bsz, int ne, int nk, const double *K, int ndz,
+                  const int *iz, int shift, double Nq, int tridiag, const zc *Grd,
+                  const zc *Gru, const zc *Grl, const zc *Gnd, const zc *Gnu, const zc *Gnl,
+                  zc *Srd, zc *Sru, zc *Srl, zc *Snd, zc *Snu);
+/* meir_wingreen with the inelastic model (integrations.F90:1804-1966);
+ * stores[11] (may be NULL) receive the last pass's G^r/G^n blocks (d, u, l)
+ * and the Sigma_r (d, u, l) / Sigma_n (d, u) blocks computed from them.
+ * Returns the number of outer SCBA passes. */
+int or_inel_meir_wingreen(int nk, const or_struct *const *sk, const double *kw, int ne,
+                          const double *E, double delta, const double *coupling, int niter_ela,
+                          double tol_ela, const double *frm, int ref, const double *K, int ndz,
+                          const int *iz, int shift, double Nq, int tridiag, int niter_inel,
+                          double tol_inel, int integration, double estep, double g_spin,
+                          double *curr_mat, double *currents, double *err_out, int *iters_ela,
+                          zc *const *stores, double sgfacc, double eps_drop, int nthreads);
+
 #endif

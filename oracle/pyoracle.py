@@ -635,3 +635,123 @@ def density_matrix(s, N, cstart, P, contour=None, real=None, frm_of=None, ref=No
         rho = rho + T
         pat = pat | pt
     return rho, pat
+
+
+# ---- inelastic electron-phonon SCBA (elphinel.F90, self_energy.F90) ----------
+_ZERO_LATT = np.zeros(9)
+
+
+def inel_kmat_po(ndz: int, dz: float, kfrac: np.ndarray, kw: np.ndarray, coupling: float,
+                 wq: float, area: float, eps0: float, eps_inf: float, q0: float,
+                 latt: Optional[np.ndarray] = None) -> np.ndarray:
+    """prepare_POKmat: K[iz, iK, iQ] (returned as an (ndz, nk, nk) Fortran-order array)."""
+    kf = np.ascontiguousarray(np.asarray(kfrac, dtype=np.float64).reshape(-1, 3))
+    nk = kf.shape[0]
+    w = np.ascontiguousarray(kw, dtype=np.float64)
+    la = np.ascontiguousarray(_ZERO_LATT if latt is None else np.asarray(latt, float).T.ravel())
+    K = np.zeros(ndz * nk * nk)
+    L = lib()
+    L.or_inel_kmat_po.argtypes = [C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p] + \
+        [C.c_double] * 6 + [C.c_void_p]
+    L.or_inel_kmat_po(ndz, dz, nk, kf.ctypes.data, w.ctypes.data, la.ctypes.data, coupling, wq, area,
+                      eps0, eps_inf, q0, K.ctypes.data)
+    return K.reshape((ndz, nk, nk), order="F")
+
+
+def inel_kmat_nonpo(ndz: int, dz: float, kfrac: np.ndarray, kw: np.ndarray, coupling: float,
+                    D0: float, latt: Optional[np.ndarray] = None) -> np.ndarray:
+    """prepare_NonPOKmat: K[iz, iK, iQ]."""
+    kf = np.ascontiguousarray(np.asarray(kfrac, dtype=np.float64).reshape(-1, 3))
+    nk = kf.shape[0]
+    w = np.ascontiguousarray(kw, dtype=np.float64)
+    la = np.ascontiguousarray(_ZERO_LATT if latt is None else np.asarray(latt, float).T.ravel())
+    K = np.zeros(ndz * nk * nk)
+    L = lib()
+    L.or_inel_kmat_nonpo.argtypes = [C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_double, C.c_double, C.c_void_p]
+    L.or_inel_kmat_nonpo(ndz, dz, nk, kf.ctypes.data, w.ctypes.data, la.ctypes.data, coupling, D0,
+                         K.ctypes.data)
+    return K.reshape((ndz, nk, nk), order="F")
+
+
+def _blk_sizes(bsz):
+    nd = sum(b * b for b in bsz)
+    no = sum(bsz[i] * bsz[i + 1] for i in range(len(bsz) - 1))
+    return nd, no
+
+
+def inelastic_mw(structs, kw, E: np.ndarray, delta: float, frm: np.ndarray, ref: int,
+                 K: np.ndarray, iz: np.ndarray, shift: int, Nq: float, tridiag: bool,
+                 niter: int, tol: float, coupling: Optional[np.ndarray] = None, niter_ela: int = 0,
+                 tol_ela: float = 1e-7, integration: int = 0, g_spin: float = 2.0,
+                 stores: bool = False, nthreads: int = 0, eps_drop: float = 0.0) -> dict:
+    """meir_wingreen with an inelastic model.  structs[nk]: the device at each
+    k-point; E: uniform grid; frm[ne, ncont+1].  Returns dict(curr_mat[ne, ncont],
+    currents[ncont], passes, err, iters_ela[nk, ne]) and, with stores=True, the
+    per-point block stores (flat, point p = iK*ne + iE)."""
+    structs = list(structs)
+    nk, ne = len(structs), len(E)
+    keep = [OracleStruct(s) for s in structs]
+    arr = (C.c_void_p * nk)(*[C.cast(C.pointer(k.c), C.c_void_p) for k in keep])
+    s0 = structs[0]
+    ncont = s0.ncont
+    Ev = np.ascontiguousarray(E, dtype=np.float64)
+    w = np.ascontiguousarray(kw, dtype=np.float64)
+    fr = np.ascontiguousarray(np.asarray(frm, dtype=np.float64).reshape(ne, ncont + 1))
+    Kc = np.ascontiguousarray(np.asarray(K, dtype=np.float64).ravel(order="F"))
+    ndz = np.asarray(K).shape[0]
+    izc = np.ascontiguousarray(iz, dtype=np.int32)
+    cp = None if coupling is None else np.ascontiguousarray(coupling, dtype=np.float64)
+    cm = np.zeros(ne * ncont)
+    cur = np.zeros(ncont)
+    err = np.zeros(1)
+    its = np.zeros(ne * nk, dtype=np.int32)
+    nd, no = _blk_sizes(list(s0.bsz))
+    st = None
+    sp = None
+    if stores:
+        st = [np.zeros(ne * nk * (nd if t in (0, 3, 6, 9) else max(no, 1)), dtype=np.complex128)
+              for t in range(11)]
+        sp = (C.c_void_p * 11)(*[x.ctypes.data for x in st])
+    L = lib()
+    L.or_inel_meir_wingreen.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p, C.c_double,
+                                        C.c_void_p, C.c_int, C.c_double, C.c_void_p, C.c_int,
+                                        C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_double, C.c_int,
+                                        C.c_int, C.c_double, C.c_int, C.c_double, C.c_double,
+                                        C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_double, C.c_double, C.c_int]
+    estep = float(E[1] - E[0]) if ne > 1 else 0.0
+    p = L.or_inel_meir_wingreen(nk, C.cast(arr, C.c_void_p), w.ctypes.data, ne, Ev.ctypes.data, delta,
+                                None if cp is None else cp.ctypes.data, niter_ela, tol_ela,
+                                fr.ctypes.data, ref, Kc.ctypes.data, ndz, izc.ctypes.data, shift, Nq,
+                                1 if tridiag else 0, niter, tol, integration, estep, g_spin,
+                                cm.ctypes.data, cur.ctypes.data, err.ctypes.data, its.ctypes.data,
+                                None if sp is None else C.cast(sp, C.c_void_p), SGFACC, eps_drop, nthreads)
+    if p < 0:
+        raise RuntimeError(f"or_inel_meir_wingreen failed ({p})")
+    out = dict(curr_mat=cm.reshape((ne, ncont), order="F"), currents=cur, passes=p,
+               err=float(err[0]), iters_ela=its.reshape(nk, ne))
+    if stores:
+        out["stores"] = dict(zip(["Grd", "Gru", "Grl", "Gnd", "Gnu", "Gnl", "Srd", "Sru", "Srl", "Snd", "Snu"], st))
+    return out
+
+
+def inel_sigma(bsz, ne: int, nk: int, K: np.ndarray, iz: np.ndarray, shift: int, Nq: float,
+               tridiag: bool, G: Sequence[np.ndarray]) -> List[np.ndarray]:
+    """compute_Sigma_n + compute_Sigma_r from the per-point G stores
+    G = [Grd, Gru, Grl, Gnd, Gnu, Gnl]; returns [Srd, Sru, Srl, Snd, Snu]."""
+    nd, no = _blk_sizes(list(bsz))
+    out = [np.zeros(ne * nk * (nd if t in (0, 3) else max(no, 1)), dtype=np.complex128) for t in range(5)]
+    b = np.array(bsz, dtype=np.int32)
+    Kc = np.ascontiguousarray(np.asarray(K, dtype=np.float64).ravel(order="F"))
+    izc = np.ascontiguousarray(iz, dtype=np.int32)
+    g = [np.ascontiguousarray(x, dtype=np.complex128) for x in G]
+    L = lib()
+    L.or_inel_sigma.argtypes = [C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                C.c_int, C.c_double, C.c_int] + [C.c_void_p] * 11
+    rc = L.or_inel_sigma(len(b), b.ctypes.data, ne, nk, Kc.ctypes.data, np.asarray(K).shape[0],
+                         izc.ctypes.data, shift, Nq, 1 if tridiag else 0, *[x.ctypes.data for x in g],
+                         *[x.ctypes.data for x in out])
+    if rc:
+        raise RuntimeError(f"or_inel_sigma failed ({rc})")
+    return out
