@@ -326,6 +326,46 @@ int hipnegf_solve_scba_mw(hipnegf_engine e, hipnegf_struct st, int nitems, const
                           double *curr, int *iters, double *err, int *status, double *gn_diag,
                           double *gr_diag);
 
+/* meir_wingreen with an inelastic electron-phonon model (host arrays):
+ * the outer SCBA of integrations.F90:1804-1966 over the (E,k) points of a
+ * uniform grid E[ne] (ne >= 2) and the first nk structure slots (k weights
+ * kweights[nk]).  Each pass runs calculate_elastic_scba per point
+ * (coupling[ndev] elastic dephasing, may be NULL; niter_ela / tol_ela) with
+ * the inelastic self-energies of the previous pass (ElPhonInel add_sigma_r /
+ * add_sigma_n, elphinel.F90:540-621), then rebuilds them (compute_Sigma_n /
+ * compute_Sigma_r, elphinel.F90:729-1100; selfenergy, self_energy.F90:41-321):
+ *   Sigma(E,k)_rc = sum_q K[|iz_r - iz_c| + ndz (k + nk q)]
+ *                   (fm G(E - shift dE, q)_rc + fp G(E + shift dE, q)_rc)
+ * on the diagonal blocks (and (i,i+1), (i+1,i) if tridiag), G outside the grid
+ * = 0, with (fm, fp) = (Nq, Nq+1) on G^n for Sigma_n and (Nq+1, Nq) on G^r
+ * plus (i/2, -i/2) on G^n for Sigma_r.  frm[ne*(ncont+1)] occupations per
+ * energy (frm[iE*(ncont+1) + ref-1] the reference).  Stops when
+ * max_c |J_c - J_c(previous)| < tol (check_J_convergence, scba.F90:150-173),
+ * J_c = g_spin * integrate_el_meir_wingreen(curr_mat(:,c)) (integration 0
+ * trapezoidal, 1 Simpson 1/3, 2 Simpson 3/8), or after pass niter.
+ * K[ndz*nk*nk]: hipnegf_inel_kmat_po / _nonpo; iz[ndev]: z bin of each device
+ * orbital (bin differences must stay below ndz).  Equal PL and contact sizes.
+ * Out: curr_mat[ne*ncont] (iE + ne*c, sum_k kweights[k] I_c(E,k)),
+ * currents[ncont], passes, err (last max |dJ|), status[ne*nk] and
+ * iters[ne*nk] (item k*ne + iE; may be NULL). */
+int hipnegf_solve_inelastic_mw(hipnegf_engine e, hipnegf_struct st, int ne, const double *E,
+                               int nk, const double *kweights, const hipnegf_opts *o,
+                               const double *coupling, int niter_ela, double tol_ela,
+                               const double *frm, int ref, const double *K, int ndz, const int *iz,
+                               int shift, double Nq, int tridiag, int niter, double tol,
+                               int integration, double g_spin, double *curr_mat, double *currents,
+                               int *passes, double *err, int *status, int *iters);
+/* K tables of the polar-optical (prepare_POKmat, elphinel.F90:318-406) and
+ * non-polar optical (prepare_NonPOKmat, :411-519) models, without equivalent
+ * k-points: K[iz + ndz*(iK + nk*iQ)] for z = iz*dz.  kfrac[3*nk] fractional
+ * k-points, latt[9] the lattice vectors column-wise (all 0: k = 0).  Host only. */
+int hipnegf_inel_kmat_po(int ndz, double dz, int nk, const double *kfrac, const double *kweights,
+                         const double *latt, double coupling, double wq, double cell_area,
+                         double eps0, double eps_inf, double q0, double *K);
+int hipnegf_inel_kmat_nonpo(int ndz, double dz, int nk, const double *kfrac,
+                            const double *kweights, const double *latt, double coupling, double D0,
+                            double *K);
+
 /* Batched kernels for tests and callers with their own drivers (device
  * pointers, item b at X + b*strideX elements): */
 int hipnegf_zgemm_batched(hipnegf_engine e, int m, int n, int k, const double *alpha,

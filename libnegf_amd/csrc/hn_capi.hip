@@ -3,6 +3,7 @@
 // Block operations keep the synchronous semantics the reference's Fortran
 // relies on (blocking copies, host-returning reductions: gpuroutines.cu:248-996):
 // every compute call enqueues on the context's stream and waits for it.
+#include <cmath>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -726,6 +727,129 @@ int hipnegf_solve_scba_mw(hipnegf_engine e, hipnegf_struct st, int nitems, const
     if (status) HN_CHECK(hipMemcpy(status, p[7], sizeof(int) * nitems, hipMemcpyDeviceToHost));
     if (gn_diag) HN_CHECK(hipMemcpy(gn_diag, p[8], sizeof(double2) * (size_t)nitems * ndev, hipMemcpyDeviceToHost));
     if (gr_diag) HN_CHECK(hipMemcpy(gr_diag, p[9], sizeof(double2) * (size_t)nitems * ndev, hipMemcpyDeviceToHost));
+    return 0;
+  });
+}
+
+int hipnegf_solve_inelastic_mw(hipnegf_engine e, hipnegf_struct st, int ne, const double *E,
+                               int nk, const double *kweights, const hipnegf_opts *o,
+                               const double *coupling, int niter_ela, double tol_ela,
+                               const double *frm, int ref, const double *K, int ndz, const int *iz,
+                               int shift, double Nq, int tridiag, int niter, double tol,
+                               int integration, double g_spin, double *curr_mat, double *currents,
+                               int *passes, double *err, int *status, int *iters) {
+  return guard([&] {
+    hn::DevStruct &S = *st->st;
+    if (ne < 2) throw std::runtime_error("inelastic SCBA needs a grid of at least 2 energies");
+    if (nk < 1 || nk > S.nslots) throw std::runtime_error("k-point count out of range");
+    if (ref < 1 || ref > S.ncont + 1) throw std::runtime_error("reference contact out of range");
+    if (niter < 0 || niter_ela < 0) throw std::runtime_error("niter < 0");
+    const double dE = E[1] - E[0];
+    for (int i = 1; i < ne; ++i)
+      if (std::fabs((E[i] - E[i - 1]) - dE) > 1e-9 * std::max(1.0, std::fabs(dE)))
+        throw std::runtime_error("inelastic SCBA needs a uniform energy grid");
+    int ndev = 0;
+    for (int b : S.bsz) ndev += b;
+    void *pc = nullptr;
+    struct F { void **p; ~F() { if (*p) (void)hipFree(*p); } } fr{&pc};
+    if (coupling) {
+      HN_CHECK(hipMalloc(&pc, sizeof(double) * ndev));
+      HN_CHECK(hipMemcpy(pc, coupling, sizeof(double) * ndev, hipMemcpyHostToDevice));
+    }
+    hn::SCBAOpts so;
+    so.coupling = (const double *)pc;
+    so.niter = niter_ela;
+    so.tol = tol_ela;
+    so.ref = ref;
+    hn::InelOpts io;
+    io.ne = ne; io.nk = nk; io.kw = kweights; io.K = K; io.ndz = ndz; io.iz = iz;
+    io.shift = shift; io.Nq = Nq; io.tridiag = tridiag ? 1 : 0; io.niter = niter; io.tol = tol;
+    io.integration = integration; io.estep = dE; io.g_spin = g_spin;
+    int np = 0;
+    double er = 0.0;
+    e->eng->inel_mw(S, E, frm, to_opts(o), so, io, curr_mat, currents, &np, &er, status, iters);
+    if (passes) *passes = np;
+    if (err) *err = er;
+    return 0;
+  });
+}
+
+// set_reciprocal_vectors (elphinel.F90:1118-1143) and the absolute k-points
+// of prepare_POKmat / prepare_NonPOKmat (:345-350): rec . kfrac, or 0 when
+// every lattice vector is 0
+static void inel_kabs(int nk, const double *kfrac, const double *latt, std::vector<double> &ka) {
+  ka.assign(3 * (size_t)nk, 0.0);
+  bool zero = true;
+  for (int t = 0; t < 9; ++t) zero = zero && latt[t] == 0.0;
+  if (zero) return;
+  const double *a1 = latt, *a2 = latt + 3, *a3 = latt + 6;
+  auto cross = [](const double *u, const double *v, double *w) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+  };
+  double c[3][3];
+  cross(a2, a3, c[0]);
+  cross(a3, a1, c[1]);
+  cross(a1, a2, c[2]);
+  const double vol = a1[0] * c[0][0] + a1[1] * c[0][1] + a1[2] * c[0][2];
+  const double tp = 2.0 * 3.14159265358979323846;
+  for (int k = 0; k < nk; ++k)
+    for (int r = 0; r < 3; ++r) {
+      double v = 0.0;
+      for (int j = 0; j < 3; ++j) v += tp * c[j][r] / vol * kfrac[j + 3 * k];
+      ka[r + 3 * (size_t)k] = v;
+    }
+}
+
+int hipnegf_inel_kmat_po(int ndz, double dz, int nk, const double *kfrac, const double *kweights,
+                         const double *latt, double coupling, double wq, double cell_area,
+                         double eps0, double eps_inf, double q0, double *K) {
+  return guard([&] {
+    if (ndz < 1 || nk < 1) throw std::runtime_error("empty K table");
+    std::vector<double> ka;
+    inel_kabs(nk, kfrac, latt, ka);
+    const double Ce = coupling * wq / 2.0 / cell_area * (1.0 / eps_inf - 1.0 / eps0);
+    for (int iQ = 0; iQ < nk; ++iQ)
+      for (int iK = 0; iK < nk; ++iK) {
+        double Q2 = 0.0;
+        for (int r = 0; r < 3; ++r) {
+          const double d = ka[r + 3 * (size_t)iK] - ka[r + 3 * (size_t)iQ];
+          Q2 += d * d;
+        }
+        const double bb = std::sqrt(q0 * q0 + Q2);
+        for (int z = 0; z < ndz; ++z) {
+          const double zz = z * dz;
+          const double Kf = (2.0 * Q2 + q0 * q0 * (1.0 - bb * zz)) * std::exp(-bb * zz) / (4.0 * bb * bb * bb);
+          K[z + (size_t)ndz * (iK + (size_t)nk * iQ)] = Ce * kweights[iQ] * Kf;
+        }
+      }
+    return 0;
+  });
+}
+
+int hipnegf_inel_kmat_nonpo(int ndz, double dz, int nk, const double *kfrac,
+                            const double *kweights, const double *latt, double coupling, double D0,
+                            double *K) {
+  return guard([&] {
+    if (ndz < 1 || nk < 1) throw std::runtime_error("empty K table");
+    std::vector<double> ka;
+    inel_kabs(nk, kfrac, latt, ka);
+    for (int iQ = 0; iQ < nk; ++iQ) {
+      const double *kq = &ka[3 * (size_t)iQ];
+      const double kq2 = kq[0] * kq[0] + kq[1] * kq[1] + kq[2] * kq[2];
+      for (int iK = 0; iK < nk; ++iK) {
+        const double *kk = &ka[3 * (size_t)iK];
+        const double kk2 = kk[0] * kk[0] + kk[1] * kk[1] + kk[2] * kk[2];
+        const double QQ = kk[0] * kq[0] + kk[1] * kq[1] + kk[2] * kq[2];
+        for (int z = 0; z < ndz; ++z) {
+          double v = 0.0;
+          if (kq2 != 0.0 && kk2 != 0.0)
+            v = coupling * kweights[iQ] * (D0 * QQ * QQ * std::exp(-std::sqrt(kq2) * (z * dz)) / (2.0 * kq2 * kk2));
+          K[z + (size_t)ndz * (iK + (size_t)nk * iQ)] = v;
+        }
+      }
+    }
     return 0;
   });
 }

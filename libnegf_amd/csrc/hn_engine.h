@@ -168,6 +168,37 @@ struct SCBAOpts {
   int ref = 0;                       // 1-based reference contact (ncont+1: none)
 };
 
+// Inelastic electron-phonon model (ElPhonInel: elphinel.F90, self_energy.F90)
+// in meir_wingreen's outer SCBA loop (integrations.F90:1804-1966).  Items are
+// the (E,k) points of a uniform energy grid: item = iK * ne + iE.
+struct InelOpts {
+  int ne = 0, nk = 0;
+  const double *kw = nullptr;  // host [nk] k weights of the curr_mat sum
+  const double *K = nullptr;   // host [ndz * nk * nk], K[iz + ndz (iK + nk iQ)]
+  int ndz = 0;
+  const int *iz = nullptr;     // host [ndev] z bin of each device orbital
+  int shift = 0;               // nint(wq / dE): energy points per phonon quantum
+  double Nq = 0.0;             // Bose occupation of the phonon mode
+  int tridiag = 0;             // with the (i,i+1) / (i+1,i) self-energy blocks
+  int niter = 0;               // outer passes run while p <= niter and not converged
+  double tol = 1e-7;           // max |J - J_old| over contacts (check_J_convergence)
+  int integration = 0;         // integrate_el_meir_wingreen rule (0 trapz, 1 simpson13, 2 simpson38)
+  double estep = 0.0, g_spin = 2.0;
+};
+
+// Block stores of one inelastic solve, [block][item][n*n] with item stride n*n.
+struct InelIO {
+  int i0 = 0, NI = 0, tridiag = 0;
+  double2 *G[6] = {};        // out: Grd, Gru, Grl, Gnd, Gnu, Gnl of each item's last pass
+  const double2 *S[5] = {};  // in: Srd, Sru, Srl, Snd, Snu (null: none, outer pass 0)
+  double2 *sigc = nullptr;   // contact self-energies [c][item][n*n] kept across passes
+  int *stc = nullptr;        // their status [c][item]
+  bool sigc_valid = false;
+};
+
+// integrate_el_meir_wingreen (integrations.F90:2774-2832)
+double integrate_mw(int integration, const double *Iv, int npts, double estep);
+
 // Surface Green's function cache (the role of ln_cache's TMatrixCacheMem for
 // surface_green, contselfenergy.F90:138-173, 251-259): g_s of (structure and
 // its contents version, slot, E, eta, contact) in one device buffer of
@@ -266,6 +297,16 @@ class Engine {
                const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters, double *derr,
                int *dstatus, double2 *gn_diag, double2 *gr_diag);
 
+  // meir_wingreen with the inelastic model: E[ne] (host, uniform grid),
+  // frm[ne * (ncont+1)] (host) occupations per energy, so.coupling the elastic
+  // dephasing (device, may be null: none), so.niter / so.tol its inner loop.
+  // Out (host): curr_mat[ne * ncont] (iE + ne c, k-weighted), currents[ncont],
+  // passes, err (last max |dJ|), status / iters [ne * nk] (contacts | inverses
+  // over all passes; inner passes of the last outer pass).
+  void inel_mw(DevStruct &st, const double *E, const double *frm, const SolveOpts &o,
+               const SCBAOpts &so, const InelOpts &io, double *curr_mat, double *currents,
+               int *passes, double *err, int *status, int *iters);
+
   // Single decimation with host matrices (cu_Zdecimation semantics).
   int decimation_host(int n, double2 *hGo, const double2 *hAo, const double2 *hBo,
                       const double2 *hCo, double acc, int crit, int *ncyc);
@@ -350,7 +391,8 @@ class Engine {
                     int *dstatus);
   void scba_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                   const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
-                  double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag);
+                  double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag,
+                  const InelIO *io = nullptr);
 };
 
 }  // namespace hn

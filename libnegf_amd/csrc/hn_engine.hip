@@ -1971,7 +1971,8 @@ int Engine::decimation_host(int n, double2 *hGo, const double2 *hAo, const doubl
 // ---------------------------------------------------------------------------
 void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot,
                         const SolveOpts &o, const SCBAOpts &so, double *dcurr, int *diters,
-                        double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag) {
+                        double *derr, int *dstatus, double2 *gn_diag, double2 *gr_diag,
+                        const InelIO *io) {
   if (!st.uniform()) throw std::runtime_error("SCBA path requires equal PL and contact sizes");
   const int nbl = st.nbl, ncont = st.ncont, n = st.bsz[0];
   const long long s2 = (long long)n * n, SB = s2 * B;
@@ -1983,7 +1984,33 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
   fill_int(ncyc_c, B * std::max(ncont, 1), 0, stream_);
   fill_int(status_c, B * std::max(ncont, 1), 0, stream_);
   fill_int(dstatus, B, 0, stream_);
-  contacts(st, B, dE, dslot, o, Sig, s2, ncyc_c, status_c);
+  const int nc1 = std::max(ncont, 1);
+  if (io && io->sigc_valid) {  // contact self-energies of an earlier outer pass
+    for (int c = 0; c < ncont; ++c) {
+      copy_batched(Sig + c * SB, s2, io->sigc + ((long long)c * io->NI + io->i0) * s2, s2, s2, B,
+                   nullptr, stream_);
+      HN_CHECK(hipMemcpyAsync(status_c + (size_t)c * B, io->stc + (size_t)c * io->NI + io->i0,
+                              sizeof(int) * B, hipMemcpyDeviceToDevice, stream_));
+    }
+  } else {
+    contacts(st, B, dE, dslot, o, Sig, s2, ncyc_c, status_c);
+    if (io && io->sigc)
+      for (int c = 0; c < ncont; ++c) {
+        copy_batched(io->sigc + ((long long)c * io->NI + io->i0) * s2, s2, Sig + c * SB, s2, s2, B,
+                     nullptr, stream_);
+        HN_CHECK(hipMemcpyAsync(io->stc + (size_t)c * io->NI + io->i0, status_c + (size_t)c * B,
+                                sizeof(int) * B, hipMemcpyDeviceToDevice, stream_));
+      }
+  }
+  (void)nc1;
+  // inelastic stores: block i of kind t for the items of this batch
+  auto ist = [&](int t, int i) -> const double2 * {
+    return io && io->S[t] ? io->S[t] + ((long long)i * io->NI + io->i0) * s2 : nullptr;
+  };
+  auto ost = [&](int t, int i) -> double2 * {
+    return io->G[t] + ((long long)i * io->NI + io->i0) * s2;
+  };
+  const bool inel_off = io && io->tridiag && io->S[1];
   double2 *Gam = arena_.take<double2>(SB * std::max(ncont, 1));
   for (int c = 0; c < ncont; ++c) gamma_batched(Sig + c * SB, s2, Gam + c * SB, s2, n, B, stream_);
 
@@ -2025,6 +2052,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     if (cs.size() >= 1) { f.sig = Sig + cs[0] * SB; f.s_sig = s2; }
     if (cs.size() == 2) { f.sig2 = Sig + cs[1] * SB; f.s_sig2 = s2; }
     f.sig_diag = sig_r + (long long)i * n; f.s_sig_diag = ndev;  // add_sigma_r
+    f.sig3 = ist(0, i); f.s_sig3 = s2;                            // inelastic add_sigma_r
     form_block(f, stream_);
   };
   auto form_off = [&](int i, double2 *u, double2 *l) {  // u = ESH(i,i+1), l = ESH(i+1,i)
@@ -2035,10 +2063,12 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     f.s_out = s2;
     if (u) {
       f.H = st.ref(BK_HU, i); f.S = sof ? st.ref(BK_SU, i) : BlkRef(); f.out = u;
+      f.sig = inel_off ? ist(1, i) : nullptr; f.s_sig = s2;
       form_block(f, stream_);
     }
     if (l) {
       f.H = st.ref(BK_HL, i); f.S = sof ? st.ref(BK_SL, i) : BlkRef(); f.out = l;
+      f.sig = inel_off ? ist(2, i) : nullptr; f.s_sig = s2;
       form_block(f, stream_);
     }
   };
@@ -2103,6 +2133,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
         ++a.nq;
       }
       a.frm = so.frm; a.frm_stride = frm_stride; a.ref = so.ref; a.eps = o.eps_drop;
+      a.X = ist(3, i); a.sX = s2;  // inelastic add_sigma_n
       a.active = act; a.batch = B;
       sn_form(a, stream_);
     }
@@ -2114,6 +2145,13 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
         form_off(i, U, nullptr);
         mm(U, OP_N, gns, OP_N, W1, one, zero);
         mm(W1, OP_N, U, OP_C, blk(Sn, i), one, one);
+        if (inel_off) {  // Sigma_n(i,i) -= Sigma_n(i,i+1) g^H U^H + U g Sigma_n(i+1,i)
+          const double2 *snu = ist(4, i);
+          mm(snu, OP_N, blk(g, i + 1), OP_C, W1, one, zero);
+          mm(W1, OP_N, U, OP_C, blk(Sn, i), mone, one);
+          mm(U, OP_N, blk(g, i + 1), OP_N, W1, one, zero);
+          mm(W1, OP_N, snu, OP_C, blk(Sn, i), mone, one);
+        }
         if (i > 0) {
           mm(blk(g, i), OP_N, blk(Sn, i), OP_N, W1, one, zero);
           mm(W1, OP_N, blk(g, i), OP_C, gns, one, zero);
@@ -2128,10 +2166,16 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     for (int i = 0; i + 1 < nbl; ++i) {
       form_off(i, nullptr, L);  // L = ESH(i+1,i)
       mm(blk(Sn, i + 1), OP_N, blk(Gru, i), OP_C, W1, one, zero);
+      if (inel_off) mm(ist(4, i), OP_C, blk(Grd, i), OP_C, W1, one, one);  // Sigma_n(i+1,i) Gr(i,i)^H
       mm(L, OP_N, blk(Gnd, i), OP_N, W1, mone, one);
       mm(blk(g, i + 1), OP_N, W1, OP_N, Gnl, one, zero);
       dagger_batched(Gnl, s2, Gnu, s2, n, n, B, act, stream_);
+      if (io) {  // cache_Gn of the off-diagonal blocks (items still iterating)
+        copy_batched(ost(5, i), s2, Gnl, s2, s2, B, act, stream_);
+        copy_batched(ost(4, i), s2, Gnu, s2, s2, B, act, stream_);
+      }
       mm(blk(Sn, i + 1), OP_N, blk(Grd, i + 1), OP_C, W1, one, zero);
+      if (inel_off) mm(ist(4, i), OP_C, blk(Grl, i), OP_C, W1, one, one);  // Sigma_n(i+1,i) Gr(i+1,i)^H
       mm(L, OP_N, Gnu, OP_N, W1, mone, one);
       mm(blk(g, i + 1), OP_N, W1, OP_N, blk(Gnd, i + 1), one, zero);
       if (s_general) {  // off-diagonal S entries enter the convergence test
@@ -2150,6 +2194,16 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     }
     scba_state(err, so.tol, pass, so.niter, act, diters, derr, B, stream_);
   }
+  // ---- cache_Gr / cache_Gn for the inelastic model (each item's last pass) ----
+  if (io)
+    for (int i = 0; i < nbl; ++i) {
+      copy_batched(ost(0, i), s2, blk(Grd, i), s2, s2, B, nullptr, stream_);
+      copy_batched(ost(3, i), s2, blk(Gnd, i), s2, s2, B, nullptr, stream_);
+      if (i + 1 < nbl) {
+        copy_batched(ost(1, i), s2, blk(Gru, i), s2, s2, B, nullptr, stream_);
+        copy_batched(ost(2, i), s2, blk(Grl, i), s2, s2, B, nullptr, stream_);
+      }
+    }
   // ---- Meir-Wingreen current per contact ----
   for (int c = 0; c < ncont; ++c) {
     const int b = st.cblk[c] - 1;
@@ -2201,6 +2255,198 @@ void Engine::scba_mw(DevStruct &st, int nitems, const double *dE, const int *dsl
                gr_diag ? gr_diag + (long long)i0 * ndev : nullptr);
   }
   HN_CHECK(hipStreamSynchronize(stream_));
+}
+
+// integrate_el_meir_wingreen, integrations.F90:2774-2832: trapezoidal,
+// Simpson 1/3 or Simpson 3/8 weights on the uniform grid
+double integrate_mw(int integration, const double *Iv, int npts, double estep) {
+  std::vector<double> w(npts, 0.0);
+  if (integration == 0) {
+    for (int i = 0; i < npts; ++i) w[i] = estep;
+    w[0] = 0.5 * estep;
+    w[npts - 1] = 0.5 * estep;
+  } else if (integration == 1) {
+    for (int i = 1; i <= npts; ++i) {
+      const int m = (i - 1) % 2 + 1;
+      w[i - 1] = estep / 3.0 * (double)(m * m);
+    }
+    for (int i = 3; i <= npts - 1; i += 2) w[i - 1] += estep / 3.0;
+  } else {
+    for (int i = 0; i < npts; ++i) w[i] = 9.0 / 8.0 * estep;
+    w[0] = 3.0 / 8.0 * estep;
+    for (int i = 4; i <= npts; i += 3) w[i - 1] = 6.0 / 8.0 * estep;
+  }
+  double c = 0.0;
+  for (int i = 0; i < npts; ++i) c += Iv[i] * w[i];
+  return c;
+}
+
+namespace {
+struct DMem {  // device allocation owned by one inelastic solve
+  void *p = nullptr;
+  explicit DMem(size_t b) { HN_CHECK(hipMalloc(&p, std::max<size_t>(b, 256))); }
+  ~DMem() { if (p) (void)hipFree(p); }
+  DMem(const DMem &) = delete;
+  DMem &operator=(const DMem &) = delete;
+  template <typename T> T *as() const { return static_cast<T *>(p); }
+};
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// meir_wingreen with an inelastic model (integrations.F90:1804-1966).
+// Outer pass p = 0, 1, ...: every (E,k) item runs the elastic SCBA of
+// scba_batch with the inelastic Sigma_r / Sigma_n of pass p-1 (none at p = 0;
+// ElPhonInel's add_sigma_r / add_sigma_n, elphinel.F90:540-621) and leaves
+// its last G^r / G^n tridiagonal blocks in the stores (cache_Gr / cache_Gn);
+// curr_mat(iE,c) = sum_k kw(k) I_c(E,k); compute_Sigma_n then compute_Sigma_r
+// (elphinel.F90:942-1100, 729-938) rebuild the self-energies of every item from
+// the stores with the energy shift and the q sum (self_energy.F90:41-321); the
+// contact currents are integrated (integrate_el_meir_wingreen) and the loop
+// stops when max |J - J_old| < tol (check_J_convergence, scba.F90:150-173).
+// Contact self-energies are computed in pass 0 and kept on the device.
+// ---------------------------------------------------------------------------
+void Engine::inel_mw(DevStruct &st, const double *E, const double *frm, const SolveOpts &o,
+                     const SCBAOpts &so, const InelOpts &io, double *curr_mat, double *currents,
+                     int *passes, double *err, int *status, int *iters) {
+  HN_CHECK(hipSetDevice(device_));
+  if (!st.uniform()) throw std::runtime_error("inelastic SCBA requires equal PL and contact sizes");
+  const int ne = io.ne, nk = io.nk, ncont = st.ncont, nbl = st.nbl, n = st.bsz[0];
+  if (ne < 1 || nk < 1) throw std::runtime_error("inelastic SCBA: empty energy grid or k set");
+  if (ncont < 1) throw std::runtime_error("SCBA needs at least one contact");
+  if (nk > st.nslots) throw std::runtime_error("inelastic SCBA: more k-points than structure slots");
+  if (ne > 65535) throw std::runtime_error("inelastic SCBA: at most 65535 energy points");
+  if (io.shift < 0) throw std::runtime_error("inelastic SCBA: negative energy shift");
+  const int ndev = n * nbl;
+  if (io.ndz < 1) throw std::runtime_error("inelastic SCBA: empty K table");
+  {
+    const auto mm = std::minmax_element(io.iz, io.iz + ndev);
+    if (*mm.second - *mm.first >= io.ndz)
+      throw std::runtime_error("inelastic SCBA: z-bin difference beyond the K table");
+  }
+  const long long NI = (long long)ne * nk, s2 = (long long)n * n;
+  const int nbu = std::max(nbl - 1, 1);
+  const size_t blk_b = sizeof(double2) * (size_t)s2 * NI;
+  // stores: G (d,u,l) x (r,n), Sigma_r (d,u,l), Sigma_n (d,u), contact self-energies
+  const size_t store_b = blk_b * ((size_t)4 * nbl + 7 * nbu + ncont);
+  size_t free_b = 0, total_b = 0;
+  HN_CHECK(hipMemGetInfo(&free_b, &total_b));
+  if (store_b + (1ull << 30) > free_b)
+    throw std::runtime_error("inelastic SCBA: block stores do not fit in device memory");
+  DMem mG0(blk_b * nbl), mG1(blk_b * nbu), mG2(blk_b * nbu), mG3(blk_b * nbl), mG4(blk_b * nbu),
+      mG5(blk_b * nbu), mS0(blk_b * nbl), mS1(blk_b * nbu), mS2(blk_b * nbu), mS3(blk_b * nbl),
+      mS4(blk_b * nbu), mSc(blk_b * ncont), mStc(sizeof(int) * NI * ncont);
+  // item tables: E, slot, occupations
+  std::vector<double> hE(NI), hF((size_t)NI * (ncont + 1));
+  std::vector<int> hS(NI);
+  for (int k = 0; k < nk; ++k)
+    for (int e = 0; e < ne; ++e) {
+      const long long it = (long long)k * ne + e;
+      hE[it] = E[e];
+      hS[it] = k;
+      for (int c = 0; c <= ncont; ++c) hF[it * (ncont + 1) + c] = frm[(size_t)e * (ncont + 1) + c];
+    }
+  DMem dEi(sizeof(double) * NI), dSl(sizeof(int) * NI), dFr(sizeof(double) * hF.size()),
+      dCur(sizeof(double) * NI * ncont), dIt(sizeof(int) * NI), dErr(sizeof(double) * NI),
+      dSt(sizeof(int) * NI), dK(sizeof(double) * (size_t)io.ndz * nk * nk), dIz(sizeof(int) * ndev),
+      dCp(sizeof(double) * ndev);
+  HN_CHECK(hipMemcpy(dEi.p, hE.data(), sizeof(double) * NI, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dSl.p, hS.data(), sizeof(int) * NI, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dFr.p, hF.data(), sizeof(double) * hF.size(), hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dK.p, io.K, sizeof(double) * (size_t)io.ndz * nk * nk, hipMemcpyHostToDevice));
+  HN_CHECK(hipMemcpy(dIz.p, io.iz, sizeof(int) * ndev, hipMemcpyHostToDevice));
+  if (so.coupling)
+    HN_CHECK(hipMemcpy(dCp.p, so.coupling, sizeof(double) * ndev, hipMemcpyDeviceToDevice));
+  else
+    HN_CHECK(hipMemset(dCp.p, 0, sizeof(double) * ndev));
+  // batch of the elastic sweeps (same per-item budget as scba_mw)
+  const int nm = st.nmax();
+  const size_t b2 = (size_t)nm * nm * sizeof(double2);
+  const size_t per_item = b2 * (size_t)(9 * nbl + 12 * ncont + 16) +
+                          inv_work_bytes(nm, ncont, 8) + (size_t)n * nbl * 48 + 1024;
+  const int B = pick_batch(st, (int)std::min<long long>(NI, 1 << 30), o, per_item);
+  last_batch_ = B;
+  arena_.reserve(per_item * (size_t)B + (1 << 22));
+
+  InelIO x;
+  x.NI = (int)NI;
+  x.tridiag = io.tridiag;
+  double2 *G[6] = {mG0.as<double2>(), mG1.as<double2>(), mG2.as<double2>(),
+                   mG3.as<double2>(), mG4.as<double2>(), mG5.as<double2>()};
+  double2 *S[5] = {mS0.as<double2>(), mS1.as<double2>(), mS2.as<double2>(), mS3.as<double2>(),
+                   mS4.as<double2>()};
+  for (int t = 0; t < 6; ++t) x.G[t] = G[t];
+  x.sigc = mSc.as<double2>();
+  x.stc = mStc.as<int>();
+  std::vector<double> cur((size_t)NI * ncont), Jold(ncont, 0.0), col(ne);
+  std::vector<int> st_all(NI, 0), st_pass(NI);
+  int p = 0;
+  bool conv = false, have_old = false;
+  double e_last = 0.0;
+  const double2 nq = {io.Nq, 0.0}, nq1 = {io.Nq + 1.0, 0.0}, ih = {0.0, 0.5}, mih = {0.0, -0.5};
+  for (; !conv && p <= io.niter; ++p) {
+    for (int t = 0; t < 5; ++t) x.S[t] = p > 0 ? S[t] : nullptr;
+    if (!io.tridiag) x.S[1] = x.S[2] = x.S[4] = nullptr;
+    x.sigc_valid = p > 0;
+    for (long long i0 = 0; i0 < NI; i0 += B) {
+      const int nb = (int)std::min<long long>(B, NI - i0);
+      x.i0 = (int)i0;
+      SCBAOpts s2o = so;
+      s2o.coupling = dCp.as<double>();
+      s2o.frm = dFr.as<double>() + i0 * (ncont + 1);
+      s2o.frm_per_item = 1;
+      scba_batch(st, nb, dEi.as<double>() + i0, dSl.as<int>() + i0, o, s2o,
+                 dCur.as<double>() + i0 * ncont, dIt.as<int>() + i0, dErr.as<double>() + i0,
+                 dSt.as<int>() + i0, nullptr, nullptr, &x);
+    }
+    HN_CHECK(hipMemcpyAsync(cur.data(), dCur.p, sizeof(double) * cur.size(), hipMemcpyDeviceToHost, stream_));
+    HN_CHECK(hipMemcpyAsync(st_pass.data(), dSt.p, sizeof(int) * NI, hipMemcpyDeviceToHost, stream_));
+    // compute_Sigma_n, then compute_Sigma_r, from this pass's G blocks
+    auto sig = [&](double2 *Sd, const double2 *Gs, int rb, int cb, double2 fm, double2 fp, int acc) {
+      InelSigArgs a;
+      a.G = Gs; a.sG = s2; a.S = Sd; a.sS = s2; a.rows = n; a.cols = n;
+      a.izr = dIz.as<int>() + (long long)rb * n; a.izc = dIz.as<int>() + (long long)cb * n;
+      a.K = dK.as<double>(); a.ndz = io.ndz; a.nk = nk; a.ne = ne; a.shift = io.shift;
+      a.fm = fm; a.fp = fp; a.accumulate = acc;
+      inel_sigma(a, stream_);
+    };
+    auto at = [&](double2 *base, int i) { return base + (long long)i * NI * s2; };
+    for (int i = 0; i < nbl; ++i) {
+      sig(at(S[3], i), at(G[3], i), i, i, nq, nq1, 0);
+      if (io.tridiag && i + 1 < nbl) sig(at(S[4], i), at(G[4], i), i, i + 1, nq, nq1, 0);
+    }
+    for (int i = 0; i < nbl; ++i) {
+      sig(at(S[0], i), at(G[0], i), i, i, nq1, nq, 0);
+      sig(at(S[0], i), at(G[3], i), i, i, ih, mih, 1);
+      if (io.tridiag && i + 1 < nbl) {
+        sig(at(S[1], i), at(G[1], i), i, i + 1, nq1, nq, 0);
+        sig(at(S[1], i), at(G[4], i), i, i + 1, ih, mih, 1);
+        sig(at(S[2], i), at(G[2], i), i + 1, i, nq1, nq, 0);
+        sig(at(S[2], i), at(G[5], i), i + 1, i, ih, mih, 1);
+      }
+    }
+    HN_CHECK(hipStreamSynchronize(stream_));
+    for (long long t = 0; t < NI; ++t) st_all[t] |= st_pass[t];
+    for (size_t t = 0; t < (size_t)ne * ncont; ++t) curr_mat[t] = 0.0;
+    for (int k = 0; k < nk; ++k)
+      for (int e = 0; e < ne; ++e)
+        for (int c = 0; c < ncont; ++c)
+          curr_mat[e + (size_t)ne * c] += cur[((size_t)k * ne + e) * ncont + c] * io.kw[k];
+    for (int c = 0; c < ncont; ++c) {
+      for (int e = 0; e < ne; ++e) col[e] = curr_mat[e + (size_t)ne * c];
+      currents[c] = integrate_mw(io.integration, col.data(), ne, io.estep) * io.g_spin;
+    }
+    if (have_old) {
+      e_last = 0.0;
+      for (int c = 0; c < ncont; ++c) e_last = std::max(e_last, std::fabs(Jold[c] - currents[c]));
+      conv = e_last < io.tol;
+    }
+    for (int c = 0; c < ncont; ++c) Jold[c] = currents[c];
+    have_old = true;
+  }
+  if (passes) *passes = p;
+  if (err) *err = e_last;
+  if (status) std::copy(st_all.begin(), st_all.end(), status);
+  if (iters) HN_CHECK(hipMemcpy(iters, dIt.p, sizeof(int) * NI, hipMemcpyDeviceToHost));
 }
 
 }  // namespace hn

@@ -33,6 +33,7 @@ struct FormArgs {
   double2 *out2 = nullptr; long long s_out2 = 0;  // optional copy
   const double2 *sig = nullptr; long long s_sig = 0;
   const double2 *sig2 = nullptr; long long s_sig2 = 0;
+  const double2 *sig3 = nullptr; long long s_sig3 = 0;  // e.g. an inelastic Sigma_r block
   const double2 *sig_diag = nullptr; long long s_sig_diag = 0;  // optional diagonal -= vector
   const int *active = nullptr;
   int batch = 0;
@@ -182,10 +183,26 @@ struct SnArgs {
   int nq = 0;
   const double *frm = nullptr; int frm_stride = 0; int ref = 0;
   double eps = 0.0;
+  const double2 *X = nullptr; long long sX = 0;  // optional full block added (inelastic Sigma_n)
   const int *active = nullptr;
   int batch = 0;
 };
 void sn_form(const SnArgs &a, hipStream_t st);
+
+// Inelastic self-energy of one block pair for every (E,k) item (self_energy.F90:41-321
+// on one process): S(iK*ne+iE)_rc (+)= sum_iQ K[|izr_r - izc_c| + ndz (iK + nk iQ)]
+//   (fm G(iQ*ne + iE-shift)_rc + fp G(iQ*ne + iE+shift)_rc), G outside the grid = 0.
+struct InelSigArgs {
+  const double2 *G = nullptr; long long sG = 0;
+  double2 *S = nullptr; long long sS = 0;
+  int rows = 0, cols = 0;
+  const int *izr = nullptr, *izc = nullptr;
+  const double *K = nullptr; int ndz = 0, nk = 0, ne = 0;
+  int shift = 0;
+  double2 fm = {0.0, 0.0}, fp = {0.0, 0.0};
+  int accumulate = 0;
+};
+void inel_sigma(const InelSigArgs &a, hipStream_t st);
 // err_b = max(err_b, max |X_b - Xold_b| over the mask), then Xold_b = X_b
 // mask mode 0: diagonal entries; 1: entries where the slot's block `mask` != 0;
 // 2: entries (r,c) where mask(r,c) != 0 or mask2(c,r) != 0 (mask2: cols x rows)

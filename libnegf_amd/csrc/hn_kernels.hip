@@ -94,6 +94,11 @@ __global__ __launch_bounds__(NT) void k_form(FormArgs a) {
     v.x -= sg.x;
     v.y -= sg.y;
   }
+  if (a.sig3) {
+    double2 sg = a.sig3[(long long)b * a.s_sig3 + e];
+    v.x -= sg.x;
+    v.y -= sg.y;
+  }
   if (a.sig_diag && r == c) {
     double2 sg = a.sig_diag[(long long)b * a.s_sig_diag + r];
     v.x -= sg.x;
@@ -403,7 +408,39 @@ __global__ __launch_bounds__(NT) void k_sn_form(SnArgs a) {
       out.y += w * g.y;
     }
   }
+  if (a.X) {
+    const double2 x = a.X[(long long)b * a.sX + e];
+    out.x += x.x;
+    out.y += x.y;
+  }
   a.Sn[(long long)b * a.sSn + e] = out;
+}
+
+// one thread per (element, energy); the k and q sums run inside
+__global__ __launch_bounds__(NT) void k_inel_sigma(InelSigArgs a) {
+  const int iE = blockIdx.y;
+  const long long e = (long long)blockIdx.x * NT + threadIdx.x;
+  if (e >= (long long)a.rows * a.cols) return;
+  const int r = (int)(e % a.rows), c = (int)(e / a.rows);
+  const int dz = abs(a.izr[r] - a.izc[c]);
+  const bool hm = iE - a.shift >= 0, hp = iE + a.shift < a.ne;
+  for (int k = 0; k < a.nk; ++k) {
+    double2 *Sp = a.S + (long long)(k * a.ne + iE) * a.sS + e;
+    double2 acc = a.accumulate ? *Sp : make_double2(0.0, 0.0);
+    for (int q = 0; q < a.nk; ++q) {
+      double2 sb = make_double2(0.0, 0.0);
+      if (hm) sb = cm(a.fm, a.G[(long long)(q * a.ne + iE - a.shift) * a.sG + e]);
+      if (hp) {
+        const double2 t = cm(a.fp, a.G[(long long)(q * a.ne + iE + a.shift) * a.sG + e]);
+        sb.x += t.x;
+        sb.y += t.y;
+      }
+      const double kv = a.K[dz + (long long)a.ndz * (k + (long long)a.nk * q)];
+      acc.x += kv * sb.x;
+      acc.y += kv * sb.y;
+    }
+    *Sp = acc;
+  }
 }
 
 __global__ __launch_bounds__(NT) void k_masked_diff(const double2 *X, double2 *Xold, long long s,
@@ -841,6 +878,11 @@ void diag_extract_masked(const double2 *M, long long sM, int n, int batch, doubl
                          long long sd, const double *scale, const int *active, hipStream_t st) {
   hipLaunchKernelGGL(k_diag_extract_m, grid2(n, batch), dim3(NT), 0, st, M, sM, n, d, sd, scale,
                      active);
+  HN_CHECK(hipGetLastError());
+}
+void inel_sigma(const InelSigArgs &a, hipStream_t st) {
+  if (a.ne <= 0 || a.rows <= 0 || a.cols <= 0) return;
+  hipLaunchKernelGGL(k_inel_sigma, grid2((long long)a.rows * a.cols, a.ne), dim3(NT), 0, st, a);
   HN_CHECK(hipGetLastError());
 }
 void sn_form(const SnArgs &a, hipStream_t st) {

@@ -494,3 +494,74 @@ def solve_scba_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, coupling: np.
         per_item, ref, _ptr(curr), _ptr(iters), _ptr(err), _ptr(st),
         None if gn is None else _ptr(gn), None if gr is None else _ptr(gr)))
     return dict(curr=curr, iters=iters, err=err, status=st, gn_diag=gn, gr_diag=gr)
+
+
+def inel_kmat_po(ndz: int, dz: float, kfrac, kweights, coupling: float, wq: float, cell_area: float,
+                 eps0: float, eps_inf: float, q0: float, latt=None) -> np.ndarray:
+    """K table of the polar-optical model (prepare_POKmat): (ndz, nk, nk), Fortran order."""
+    kf = np.ascontiguousarray(np.asarray(kfrac, dtype=np.float64).reshape(-1, 3))
+    nk = kf.shape[0]
+    w = np.ascontiguousarray(kweights, dtype=np.float64)
+    la = np.ascontiguousarray(np.zeros(9) if latt is None else np.asarray(latt, float).T.ravel())
+    K = np.zeros(ndz * nk * nk)
+    L = load()
+    L.hipnegf_inel_kmat_po.argtypes = [C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                       C.c_void_p] + [C.c_double] * 6 + [C.c_void_p]
+    check(L.hipnegf_inel_kmat_po(ndz, dz, nk, _ptr(kf), _ptr(w), _ptr(la), coupling, wq, cell_area,
+                                 eps0, eps_inf, q0, _ptr(K)))
+    return K.reshape((ndz, nk, nk), order="F")
+
+
+def inel_kmat_nonpo(ndz: int, dz: float, kfrac, kweights, coupling: float, D0: float,
+                    latt=None) -> np.ndarray:
+    """K table of the non-polar optical model (prepare_NonPOKmat)."""
+    kf = np.ascontiguousarray(np.asarray(kfrac, dtype=np.float64).reshape(-1, 3))
+    nk = kf.shape[0]
+    w = np.ascontiguousarray(kweights, dtype=np.float64)
+    la = np.ascontiguousarray(np.zeros(9) if latt is None else np.asarray(latt, float).T.ravel())
+    K = np.zeros(ndz * nk * nk)
+    L = load()
+    L.hipnegf_inel_kmat_nonpo.argtypes = [C.c_int, C.c_double, C.c_int, C.c_void_p, C.c_void_p,
+                                          C.c_void_p, C.c_double, C.c_double, C.c_void_p]
+    check(L.hipnegf_inel_kmat_nonpo(ndz, dz, nk, _ptr(kf), _ptr(w), _ptr(la), coupling, D0, _ptr(K)))
+    return K.reshape((ndz, nk, nk), order="F")
+
+
+def solve_inelastic_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, kweights, frm, ref: int,
+                       K: np.ndarray, iz, shift: int, Nq: float, tridiag: bool, niter: int,
+                       tol: float, coupling=None, niter_ela: int = 0, tol_ela: float = 1e-7,
+                       integration: int = 0, g_spin: float = 2.0, delta: float = 1e-4,
+                       max_batch: int = 0) -> dict:
+    """meir_wingreen with an inelastic model over the (E,k) grid: E uniform,
+    the first len(kweights) slots of ds as the k-points, frm[ne, ncont+1].
+    Returns dict(curr_mat[ne, ncont], currents, passes, err, status[nk, ne],
+    iters[nk, ne])."""
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    ne = len(E)
+    w = np.ascontiguousarray(kweights, dtype=np.float64)
+    nk = len(w)
+    ncont = ds.template.ncont
+    fr = np.ascontiguousarray(np.asarray(frm, dtype=np.float64).reshape(ne, ncont + 1))
+    Kc = np.ascontiguousarray(np.asarray(K, dtype=np.float64).ravel(order="F"))
+    ndz = np.asarray(K).shape[0]
+    izc = np.ascontiguousarray(iz, dtype=np.int32)
+    cp = None if coupling is None else np.ascontiguousarray(coupling, dtype=np.float64)
+    cm = np.zeros(ne * ncont)
+    cur = np.zeros(ncont)
+    npass = C.c_int()
+    err = C.c_double()
+    st = np.zeros(ne * nk, dtype=np.int32)
+    it = np.zeros(ne * nk, dtype=np.int32)
+    o = _opts(delta, 1, 2, FP64, max_batch)
+    L = load()
+    vp, i, d = C.c_void_p, C.c_int, C.c_double
+    L.hipnegf_solve_inelastic_mw.argtypes = [vp, vp, i, vp, i, vp, C.POINTER(Opts), vp, i, d, vp, i, vp,
+                                             i, vp, i, d, i, i, d, i, d, vp, vp, C.POINTER(C.c_int),
+                                             C.POINTER(C.c_double), vp, vp]
+    check(L.hipnegf_solve_inelastic_mw(
+        eng.handle, ds.handle, ne, _ptr(E), nk, _ptr(w), C.byref(o), None if cp is None else _ptr(cp),
+        niter_ela, tol_ela, _ptr(fr), ref, _ptr(Kc), ndz, _ptr(izc), shift, Nq, 1 if tridiag else 0,
+        niter, tol, integration, g_spin, _ptr(cm), _ptr(cur), C.byref(npass), C.byref(err), _ptr(st),
+        _ptr(it)))
+    return dict(curr_mat=cm.reshape((ne, ncont), order="F"), currents=cur, passes=npass.value,
+                err=err.value, status=st.reshape(nk, ne), iters=it.reshape(nk, ne))
