@@ -29,7 +29,7 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "ldos", "dm"])
+    ap.add_argument("--config", default="c3", choices=["c1", "c2", "c3", "c4", "ldos", "dm", "inel"])
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--items", type=int, default=0, help="items per step (0: config default)")
     ap.add_argument("--nbl", type=int, default=0, help="override the number of layers")
@@ -348,6 +348,108 @@ def bench_c4(a, hip, wl, torch):
     print(json.dumps(out), flush=True)
 
 
+def bench_inel(a, hip, wl, torch):
+    """Inelastic polar-optical phonon SCBA (meir_wingreen's outer loop over the
+    whole (E,k) grid): one step = one complete solve to convergence.  value =
+    (E,k) points x outer passes solved per second."""
+    c = wl.CI
+    w = wl.config_inel(nbl=a.nbl or None)
+    structs, E, kw = w["structs"], w["E"], w["kw"]
+    mu, kT = w["mu"], w["kT"]
+
+    def kmat(kfrac, kwt):
+        return hip.inel_kmat_po(w["ndz"], 1.0, kfrac, kwt, c["coupling"], c["wq"], c["area"], c["eps0"],
+                                c["eps_inf"], c["q0"], latt=w["latt"])
+
+    def occ(Eg):
+        return np.stack([fermi(Eg, mu[0], kT), fermi(Eg, mu[1], kT), np.zeros_like(Eg)], axis=1)
+
+    K = kmat(w["kfrac"], kw)
+    frm = occ(E)
+    eng = hip.Engine(0)
+    ds = hip.DeviceStructure(structs)
+    run = lambda: hip.solve_inelastic_mw(eng, ds, E, kw, frm, 3, K, w["iz"], w["shift"], w["Nq"], True,
+                                         c["niter"], c["tol"], delta=w["eta"])
+    for i in range(a.warmup):
+        t0 = time.time()
+        run()
+        log(f"warmup {i}: {time.time() - t0:.2f}s")
+    torch.cuda.synchronize()
+    if not a.no_profile:
+        hip.profile_clear()
+        hip.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        r = run()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kern = None
+    if not a.no_profile:
+        hip.profile_enable(False)
+        kern = hip.profile_kernels()
+        hip.profile_clear()
+    npts = len(E) * len(kw)
+    value = a.steps * npts * r["passes"] / elapsed
+    log(f"inel: {npts} (E,k) points x {r['passes']} passes in {elapsed / a.steps:.2f}s/step -> "
+        f"{value:.1f} point-passes/s; currents {r['currents']}; status nonzero {(r['status'] != 0).sum()}")
+    roofline = None
+    if kern:
+        counted = {k: v for k, v in kern.items() if v["flops"] > 0 and v["ms"] > 0}
+        if counted:
+            name = max(counted, key=lambda k: counted[k]["ms"])
+            v = counted[name]
+            avg = v["ms"] * 1e-3 / v["launches"]
+            ach = v["flops"] / v["launches"] / avg / 1e12
+            roofline = {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name,
+                        "avg_launch_us": avg * 1e6, "launches": v["launches"],
+                        "share_of_step_time": v["ms"] * 1e-3 / elapsed}
+    cpu = parity = None
+    if not a.no_cpu:
+        # bounded sample: the same structures at 2 k-points on a 24-point grid
+        from oracle import pyoracle
+        pyoracle.build()
+        try:
+            backend = pyoracle.set_backend("mkl")
+        except RuntimeError:
+            backend = pyoracle.set_backend("builtin")
+        threads = min(int(os.environ.get("OMP_NUM_THREADS", "0")) or os.cpu_count() or 1, 16)
+        Es = wl.energy_grid(24, E[0], E[-1])
+        shs = int(round(c["wq"] / (Es[1] - Es[0])))
+        kws = np.array([0.5, 0.5])
+        Ks = kmat(w["kfrac"][:2], kws)
+        fs = occ(Es)
+        t1 = time.perf_counter()
+        ro = pyoracle.inelastic_mw(structs[:2], kws, Es, w["eta"], fs, 3, Ks, w["iz"], shs, w["Nq"], True,
+                                   c["niter"], c["tol"], nthreads=threads)
+        secs = time.perf_counter() - t1
+        ds2 = hip.DeviceStructure(structs[:2])
+        rg = hip.solve_inelastic_mw(eng, ds2, Es, kws, fs, 3, Ks, w["iz"], shs, w["Nq"], True, c["niter"],
+                                    c["tol"], delta=w["eta"])
+        ds2.close()
+        pp = len(Es) * 2 * ro["passes"]
+        cpu = dict(value=pp / secs, unit="(E,k) point-passes/s", cores=threads, kind="port",
+                   sample=f"{len(Es)} energies x 2 k-points of the same structures ({ro['passes']} passes, "
+                          f"shift {shs}), oracle with {backend}, OpenMP over points; host CPU: {cpu_model()}",
+                   seconds=secs)
+        scale = np.abs(ro["curr_mat"]).max()
+        parity = dict(max_rel_err_curr_mat=float(np.abs(rg["curr_mat"] - ro["curr_mat"]).max() / scale),
+                      passes_equal=bool(rg["passes"] == ro["passes"]), points=len(Es) * 2)
+        log(f"cpu baseline {cpu}; parity {parity}")
+    ds.close()
+    out = {"metric": "RGF energy-points/s", "value": value, "unit": "(E,k) point-passes/s", "n_gpus": 1,
+           "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1000.0 * elapsed / a.steps,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+           "data": "synthetic (seeded generators, libnegf_amd/workloads.py)",
+           "config": dict(workload=f"inelastic_po_W{structs[0].bsz[0]}_nbl{len(structs[0].bsz)}"
+                                   f"_E{len(E)}_k{len(kw)}",
+                          shift=w["shift"], Nq=w["Nq"], passes=r["passes"], points=npts,
+                          currents=[float(x) for x in r["currents"]]),
+           "roofline": roofline, "cpu_baseline": cpu, "parity": parity,
+           "kernel_ms": None if kern is None else {k: round(v["ms"], 3) for k, v in kern.items()}}
+    print(json.dumps(out), flush=True)
+
+
 def bench_path(a, hip, wl, torch):
     """The C3 k-slab at one k-point through the other block-solver paths:
     --config ldos: T + per-layer local DOS (calculate_transmissions_and_dos,
@@ -479,6 +581,9 @@ def main():
     prec = hip.FP64 if a.precision == "fp64" else hip.MIXED
     if a.config == "c4":
         bench_c4(a, hip, wl, torch)
+        return
+    if a.config == "inel":
+        bench_inel(a, hip, wl, torch)
         return
     if a.config in ("ldos", "dm"):
         bench_path(a, hip, wl, torch)

@@ -44,6 +44,15 @@ C4 = dict(W=128, nbl=30, disorder=0.2, seed=4, nE=1000, eta=1e-5, coupling=0.1,
           bias=0.2, T=300.0, scba_tol=1e-8, scba_niter=100)
 
 
+# Inelastic (polar-optical phonon) SCBA workload: a stack of disordered
+# ribbons coupled transversally with hopping ty (Bloch phase 2 ty cos(2 pi ky)
+# on every site at nk transverse k-points), optical phonon wq, Froehlich
+# screening q0, one z bin per principal layer.
+CI = dict(W=64, nbl=20, disorder=0.2, seed=6, ty=-0.3, nk=4, nE=400, eta=1e-5, bias=0.2, T=300.0,
+          wq=0.06, Tph=300.0, coupling=40.0, eps0=12.9, eps_inf=10.9, q0=0.1, area=1.0,
+          tol=1e-6, niter=60)
+
+
 def energy_grid(nE: int, Emin: float, Emax: float) -> np.ndarray:
     """E_i = Emin + (i-1) (Emax-Emin)/(nE-1), i = 1..nE (inclusive grid)."""
     return Emin + (Emax - Emin) * np.arange(nE, dtype=np.float64) / (nE - 1)
@@ -82,6 +91,36 @@ def config2(nbl: Optional[int] = None, W: Optional[int] = None):
     c = C2
     return (ribbon(W or c["W"], nbl or c["nbl"], c["disorder"], c["seed"], name="C2"),
             energy_grid(c["nE"], c["Emin"], c["Emax"]), c["eta"])
+
+
+def config_inel(nbl: Optional[int] = None, W: Optional[int] = None, nE: Optional[int] = None,
+                nk: Optional[int] = None):
+    """Structures per transverse k, uniform energy grid, k-points (fractional,
+    along y) and weights, lattice vectors (columns), z bins per device orbital,
+    the phonon's energy shift in grid steps and Bose occupation, contact
+    chemical potentials and kT."""
+    c = CI
+    W, nbl, nE, nk = W or c["W"], nbl or c["nbl"], nE or c["nE"], nk or c["nk"]
+    base = ribbon(W, nbl, c["disorder"], c["seed"], name="CI")
+    ky = (np.arange(nk) + 0.5) / (2 * nk)  # half zone (time-reversal pairs)
+    structs = []
+    for j in range(nk):
+        d = 2.0 * c["ty"] * np.cos(2 * np.pi * ky[j]) * np.eye(W)
+        cs = [Contact(cblk=ct.cblk, H00=ct.H00 + d, H01=ct.H01, HMC=ct.HMC) for ct in base.contacts]
+        structs.append(BlockStructure(Hd=[h + d for h in base.Hd], Hu=base.Hu, Hl=base.Hl, contacts=cs,
+                                      name="CI"))
+    kfrac = np.stack([np.zeros(nk), ky, np.zeros(nk)], axis=1)
+    kw = np.full(nk, 1.0 / nk)
+    latt = np.diag([1.0, 1.0, 1.0])
+    kT = KB_EV * c["T"]
+    mu = (c["bias"] / 2.0, -c["bias"] / 2.0)
+    E = energy_grid(nE, -c["bias"] / 2 - 10 * kT, c["bias"] / 2 + 10 * kT)
+    shift = int(round(c["wq"] / (E[1] - E[0])))
+    kph = KB_EV * c["Tph"]
+    Nq = 1.0 / (np.exp(c["wq"] / kph) - 1.0)
+    iz = np.repeat(np.arange(nbl), W)
+    return dict(structs=structs, E=E, kfrac=kfrac, kw=kw, latt=latt, iz=iz, ndz=nbl, shift=shift,
+                Nq=Nq, mu=mu, kT=kT, eta=c["eta"])
 
 
 def config4(nbl: Optional[int] = None, W: Optional[int] = None):
