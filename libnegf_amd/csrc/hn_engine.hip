@@ -2014,10 +2014,15 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
   double2 *Gam = arena_.take<double2>(SB * std::max(ncont, 1));
   for (int c = 0; c < ncont; ++c) gamma_batched(Sig + c * SB, s2, Gam + c * SB, s2, n, B, stream_);
 
-  double2 *g = arena_.take<double2>(SB * nbl), *Grd = arena_.take<double2>(SB * nbl);
-  double2 *Gru = arena_.take<double2>(SB * std::max(nbl - 1, 1));
-  double2 *Grl = arena_.take<double2>(SB * std::max(nbl - 1, 1));
-  double2 *Sn = arena_.take<double2>(SB * nbl), *Gnd = arena_.take<double2>(SB * nbl);
+  // with inelastic stores covering exactly this batch, G^r and the diagonal
+  // G^n blocks are formed in the stores themselves (same [block][item] layout)
+  const bool in_store = io && io->i0 == 0 && io->NI == B;
+  double2 *g = arena_.take<double2>(SB * nbl);
+  double2 *Grd = in_store ? io->G[0] : arena_.take<double2>(SB * nbl);
+  double2 *Gru = in_store ? io->G[1] : arena_.take<double2>(SB * std::max(nbl - 1, 1));
+  double2 *Grl = in_store ? io->G[2] : arena_.take<double2>(SB * std::max(nbl - 1, 1));
+  double2 *Sn = arena_.take<double2>(SB * nbl);
+  double2 *Gnd = in_store ? io->G[3] : arena_.take<double2>(SB * nbl);
   double2 *Gnd_old = arena_.take<double2>(SB * nbl);
   const bool s_general = (st.smode & S_OFFDIAG) != 0;
   double2 *Gnu_old = s_general ? arena_.take<double2>(SB * std::max(nbl - 1, 1)) : nullptr;
@@ -2165,23 +2170,22 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     // ---- downward recursion ----
     for (int i = 0; i + 1 < nbl; ++i) {
       form_off(i, nullptr, L);  // L = ESH(i+1,i)
+      // with inelastic stores, Gn(i+1,i) / Gn(i,i+1) are formed in them directly
+      // (cache_Gn; masked launches leave converged items' blocks as they were)
+      double2 *Gnl_i = io ? ost(5, i) : Gnl, *Gnu_i = io ? ost(4, i) : Gnu;
       mm(blk(Sn, i + 1), OP_N, blk(Gru, i), OP_C, W1, one, zero);
       if (inel_off) mm(ist(4, i), OP_C, blk(Grd, i), OP_C, W1, one, one);  // Sigma_n(i+1,i) Gr(i,i)^H
       mm(L, OP_N, blk(Gnd, i), OP_N, W1, mone, one);
-      mm(blk(g, i + 1), OP_N, W1, OP_N, Gnl, one, zero);
-      dagger_batched(Gnl, s2, Gnu, s2, n, n, B, act, stream_);
-      if (io) {  // cache_Gn of the off-diagonal blocks (items still iterating)
-        copy_batched(ost(5, i), s2, Gnl, s2, s2, B, act, stream_);
-        copy_batched(ost(4, i), s2, Gnu, s2, s2, B, act, stream_);
-      }
+      mm(blk(g, i + 1), OP_N, W1, OP_N, Gnl_i, one, zero);
+      dagger_batched(Gnl_i, s2, Gnu_i, s2, n, n, B, act, stream_);
       mm(blk(Sn, i + 1), OP_N, blk(Grd, i + 1), OP_C, W1, one, zero);
       if (inel_off) mm(ist(4, i), OP_C, blk(Grl, i), OP_C, W1, one, one);  // Sigma_n(i+1,i) Gr(i+1,i)^H
-      mm(L, OP_N, Gnu, OP_N, W1, mone, one);
+      mm(L, OP_N, Gnu_i, OP_N, W1, mone, one);
       mm(blk(g, i + 1), OP_N, W1, OP_N, blk(Gnd, i + 1), one, zero);
       if (s_general) {  // off-diagonal S entries enter the convergence test
         // Gn(i+1,i) = Gn(i,i+1)^H, so |dGn(i+1,i)(r,c)| = |dGn(i,i+1)(c,r)|: one pass over
         // Gn(i,i+1) with the union of the S(i,i+1) pattern and the transposed S(i+1,i) pattern
-        masked_diff(Gnu, blk(Gnu_old, i), s2, n, n, 2, st.ref(BK_SU, i), st.ref(BK_SL, i), dslot, err,
+        masked_diff(Gnu_i, blk(Gnu_old, i), s2, n, n, 2, st.ref(BK_SU, i), st.ref(BK_SL, i), dslot, err,
                     act, B, stream_);
       }
     }
@@ -2195,7 +2199,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     scba_state(err, so.tol, pass, so.niter, act, diters, derr, B, stream_);
   }
   // ---- cache_Gr / cache_Gn for the inelastic model (each item's last pass) ----
-  if (io)
+  if (io && !in_store)
     for (int i = 0; i < nbl; ++i) {
       copy_batched(ost(0, i), s2, blk(Grd, i), s2, s2, B, nullptr, stream_);
       copy_batched(ost(3, i), s2, blk(Gnd, i), s2, s2, B, nullptr, stream_);
@@ -2401,27 +2405,26 @@ void Engine::inel_mw(DevStruct &st, const double *E, const double *frm, const So
     HN_CHECK(hipMemcpyAsync(cur.data(), dCur.p, sizeof(double) * cur.size(), hipMemcpyDeviceToHost, stream_));
     HN_CHECK(hipMemcpyAsync(st_pass.data(), dSt.p, sizeof(int) * NI, hipMemcpyDeviceToHost, stream_));
     // compute_Sigma_n, then compute_Sigma_r, from this pass's G blocks
-    auto sig = [&](double2 *Sd, const double2 *Gs, int rb, int cb, double2 fm, double2 fp, int acc) {
+    // G2 (optional): a second term of the same block, summed after G's
+    auto sig = [&](double2 *Sd, const double2 *Gs, int rb, int cb, double2 fm, double2 fp,
+                   const double2 *G2s = nullptr, double2 fm2 = {0.0, 0.0}, double2 fp2 = {0.0, 0.0}) {
       InelSigArgs a;
       a.G = Gs; a.sG = s2; a.S = Sd; a.sS = s2; a.rows = n; a.cols = n;
       a.izr = dIz.as<int>() + (long long)rb * n; a.izc = dIz.as<int>() + (long long)cb * n;
       a.K = dK.as<double>(); a.ndz = io.ndz; a.nk = nk; a.ne = ne; a.shift = io.shift;
-      a.fm = fm; a.fp = fp; a.accumulate = acc;
+      a.fm = fm; a.fp = fp; a.G2 = G2s; a.fm2 = fm2; a.fp2 = fp2; a.accumulate = 0;
       inel_sigma(a, stream_);
     };
     auto at = [&](double2 *base, int i) { return base + (long long)i * NI * s2; };
     for (int i = 0; i < nbl; ++i) {
-      sig(at(S[3], i), at(G[3], i), i, i, nq, nq1, 0);
-      if (io.tridiag && i + 1 < nbl) sig(at(S[4], i), at(G[4], i), i, i + 1, nq, nq1, 0);
+      sig(at(S[3], i), at(G[3], i), i, i, nq, nq1);
+      if (io.tridiag && i + 1 < nbl) sig(at(S[4], i), at(G[4], i), i, i + 1, nq, nq1);
     }
-    for (int i = 0; i < nbl; ++i) {
-      sig(at(S[0], i), at(G[0], i), i, i, nq1, nq, 0);
-      sig(at(S[0], i), at(G[3], i), i, i, ih, mih, 1);
+    for (int i = 0; i < nbl; ++i) {  // Sigma_r: the G^r sum, then the G^n sum
+      sig(at(S[0], i), at(G[0], i), i, i, nq1, nq, at(G[3], i), ih, mih);
       if (io.tridiag && i + 1 < nbl) {
-        sig(at(S[1], i), at(G[1], i), i, i + 1, nq1, nq, 0);
-        sig(at(S[1], i), at(G[4], i), i, i + 1, ih, mih, 1);
-        sig(at(S[2], i), at(G[2], i), i + 1, i, nq1, nq, 0);
-        sig(at(S[2], i), at(G[5], i), i + 1, i, ih, mih, 1);
+        sig(at(S[1], i), at(G[1], i), i, i + 1, nq1, nq, at(G[4], i), ih, mih);
+        sig(at(S[2], i), at(G[2], i), i + 1, i, nq1, nq, at(G[5], i), ih, mih);
       }
     }
     HN_CHECK(hipStreamSynchronize(stream_));

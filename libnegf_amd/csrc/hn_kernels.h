@@ -200,6 +200,9 @@ struct InelSigArgs {
   const double *K = nullptr; int ndz = 0, nk = 0, ne = 0;
   int shift = 0;
   double2 fm = {0.0, 0.0}, fp = {0.0, 0.0};
+  // optional second term (same block, other G): its q sum is added after G's
+  const double2 *G2 = nullptr;
+  double2 fm2 = {0.0, 0.0}, fp2 = {0.0, 0.0};
   int accumulate = 0;
 };
 void inel_sigma(const InelSigArgs &a, hipStream_t st);

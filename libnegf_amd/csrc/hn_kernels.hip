@@ -416,7 +416,12 @@ __global__ __launch_bounds__(NT) void k_sn_form(SnArgs a) {
   a.Sn[(long long)b * a.sSn + e] = out;
 }
 
-// one thread per (element, energy); the k and q sums run inside
+// one thread per (element, energy); the k and q sums run inside.  NKT > 0:
+// the nk <= NKT shifted-energy sums of each q are formed once and held in
+// registers for every k; NKT == 0: any nk, re-read per k (from L2).
+// Per output: acc = S or 0; acc += K(k,q) sb1(q) for q in order; then
+// acc += K(k,q) sb2(q) for q in order (the order of two separate passes).
+template <int NKT>
 __global__ __launch_bounds__(NT) void k_inel_sigma(InelSigArgs a) {
   const int iE = blockIdx.y;
   const long long e = (long long)blockIdx.x * NT + threadIdx.x;
@@ -424,22 +429,59 @@ __global__ __launch_bounds__(NT) void k_inel_sigma(InelSigArgs a) {
   const int r = (int)(e % a.rows), c = (int)(e / a.rows);
   const int dz = abs(a.izr[r] - a.izc[c]);
   const bool hm = iE - a.shift >= 0, hp = iE + a.shift < a.ne;
-  for (int k = 0; k < a.nk; ++k) {
-    double2 *Sp = a.S + (long long)(k * a.ne + iE) * a.sS + e;
-    double2 acc = a.accumulate ? *Sp : make_double2(0.0, 0.0);
-    for (int q = 0; q < a.nk; ++q) {
-      double2 sb = make_double2(0.0, 0.0);
-      if (hm) sb = cm(a.fm, a.G[(long long)(q * a.ne + iE - a.shift) * a.sG + e]);
-      if (hp) {
-        const double2 t = cm(a.fp, a.G[(long long)(q * a.ne + iE + a.shift) * a.sG + e]);
-        sb.x += t.x;
-        sb.y += t.y;
-      }
-      const double kv = a.K[dz + (long long)a.ndz * (k + (long long)a.nk * q)];
-      acc.x += kv * sb.x;
-      acc.y += kv * sb.y;
+  auto shifted = [&](const double2 *G, double2 fm, double2 fp, int q) {
+    double2 sb = make_double2(0.0, 0.0);
+    if (hm) sb = cm(fm, G[(long long)(q * a.ne + iE - a.shift) * a.sG + e]);
+    if (hp) {
+      const double2 t = cm(fp, G[(long long)(q * a.ne + iE + a.shift) * a.sG + e]);
+      sb.x += t.x;
+      sb.y += t.y;
     }
-    *Sp = acc;
+    return sb;
+  };
+  const double *Kz = a.K + dz;
+  if constexpr (NKT > 0) {
+    double2 s1[NKT], s2[NKT];
+#pragma unroll
+    for (int q = 0; q < NKT; ++q)
+      if (q < a.nk) {
+        s1[q] = shifted(a.G, a.fm, a.fp, q);
+        s2[q] = a.G2 ? shifted(a.G2, a.fm2, a.fp2, q) : make_double2(0.0, 0.0);
+      }
+    for (int k = 0; k < a.nk; ++k) {
+      double2 *Sp = a.S + (long long)(k * a.ne + iE) * a.sS + e;
+      double2 acc = a.accumulate ? *Sp : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int q = 0; q < NKT; ++q)
+        if (q < a.nk) {
+          const double kv = Kz[(long long)a.ndz * (k + (long long)a.nk * q)];
+          acc.x += kv * s1[q].x;
+          acc.y += kv * s1[q].y;
+        }
+      if (a.G2) {
+#pragma unroll
+        for (int q = 0; q < NKT; ++q)
+          if (q < a.nk) {
+            const double kv = Kz[(long long)a.ndz * (k + (long long)a.nk * q)];
+            acc.x += kv * s2[q].x;
+            acc.y += kv * s2[q].y;
+          }
+      }
+      *Sp = acc;
+    }
+  } else {
+    for (int k = 0; k < a.nk; ++k) {
+      double2 *Sp = a.S + (long long)(k * a.ne + iE) * a.sS + e;
+      double2 acc = a.accumulate ? *Sp : make_double2(0.0, 0.0);
+      for (int t = 0; t < (a.G2 ? 2 : 1); ++t)
+        for (int q = 0; q < a.nk; ++q) {
+          const double2 sb = t == 0 ? shifted(a.G, a.fm, a.fp, q) : shifted(a.G2, a.fm2, a.fp2, q);
+          const double kv = Kz[(long long)a.ndz * (k + (long long)a.nk * q)];
+          acc.x += kv * sb.x;
+          acc.y += kv * sb.y;
+        }
+      *Sp = acc;
+    }
   }
 }
 
@@ -882,7 +924,13 @@ void diag_extract_masked(const double2 *M, long long sM, int n, int batch, doubl
 }
 void inel_sigma(const InelSigArgs &a, hipStream_t st) {
   if (a.ne <= 0 || a.rows <= 0 || a.cols <= 0) return;
-  hipLaunchKernelGGL(k_inel_sigma, grid2((long long)a.rows * a.cols, a.ne), dim3(NT), 0, st, a);
+  const dim3 g = grid2((long long)a.rows * a.cols, a.ne);
+  if (a.nk <= 1) hipLaunchKernelGGL(k_inel_sigma<1>, g, dim3(NT), 0, st, a);
+  else if (a.nk <= 2) hipLaunchKernelGGL(k_inel_sigma<2>, g, dim3(NT), 0, st, a);
+  else if (a.nk <= 4) hipLaunchKernelGGL(k_inel_sigma<4>, g, dim3(NT), 0, st, a);
+  else if (a.nk <= 8) hipLaunchKernelGGL(k_inel_sigma<8>, g, dim3(NT), 0, st, a);
+  else if (a.nk <= 16) hipLaunchKernelGGL(k_inel_sigma<16>, g, dim3(NT), 0, st, a);
+  else hipLaunchKernelGGL(k_inel_sigma<0>, g, dim3(NT), 0, st, a);
   HN_CHECK(hipGetLastError());
 }
 void sn_form(const SnArgs &a, hipStream_t st) {
