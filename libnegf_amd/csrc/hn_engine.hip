@@ -2026,7 +2026,11 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
   double2 *Gnd_old = arena_.take<double2>(SB * nbl);
   const bool s_general = (st.smode & S_OFFDIAG) != 0;
   double2 *Gnu_old = s_general ? arena_.take<double2>(SB * std::max(nbl - 1, 1)) : nullptr;
-  double2 *D = arena_.take<double2>(SB), *U = arena_.take<double2>(SB), *L = arena_.take<double2>(SB);
+  double2 *D = arena_.take<double2>(SB);
+  // ESH(i,i+1) and ESH(i+1,i) of every block: the same in every pass (the
+  // dephasing self-energy is diagonal; an inelastic one is fixed per call)
+  double2 *Uc = arena_.take<double2>(SB * std::max(nbl - 1, 1));
+  double2 *Lc = arena_.take<double2>(SB * std::max(nbl - 1, 1));
   double2 *W1 = arena_.take<double2>(SB), *W2 = arena_.take<double2>(SB);
   double2 *gns = arena_.take<double2>(SB), *Gnl = arena_.take<double2>(SB), *Gnu = arena_.take<double2>(SB);
   double2 *sig_r = arena_.take<double2>((size_t)ndev * B), *sig_n = arena_.take<double2>((size_t)ndev * B);
@@ -2086,6 +2090,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     gemm_z(g, stream_);
   };
 
+  for (int i = 0; i + 1 < nbl; ++i) form_off(i, blk(Uc, i), blk(Lc, i));
   for (int pass = 0; pass <= so.niter; ++pass) {
     count_nonzero(act, B, dcount, stream_);
     HN_CHECK(hipMemcpyAsync(h_count_, dcount, sizeof(int), hipMemcpyDeviceToHost, stream_));
@@ -2097,7 +2102,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
       form_diag(nbl - 1, D);
       inverse(n, D, s2, blk(g, nbl - 1), s2, B, act, dstatus, PREC_FP64, o, n_active);
       for (int i = nbl - 2; i >= 1; --i) {
-        form_off(i, U, L);
+        const double2 *U = blk(Uc, i), *L = blk(Lc, i);
         form_diag(i, D);
         mm(U, OP_N, blk(g, i + 1), OP_N, W1, mone, zero);
         mm(W1, OP_N, L, OP_N, D, one, one);
@@ -2107,14 +2112,13 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     // ---- Gr(1,1) ----
     form_diag(0, D);
     if (nbl > 1) {
-      form_off(0, U, L);
-      mm(U, OP_N, blk(g, 1), OP_N, W1, mone, zero);
-      mm(W1, OP_N, L, OP_N, D, one, one);
+      mm(Uc, OP_N, blk(g, 1), OP_N, W1, mone, zero);
+      mm(W1, OP_N, Lc, OP_N, D, one, one);
     }
     inverse(n, D, s2, blk(Grd, 0), s2, B, act, dstatus, PREC_FP64, o, n_active);
     // ---- Gr tridiagonal blocks, i = 2..nbl (1-based) ----
     for (int i = 1; i < nbl; ++i) {
-      form_off(i - 1, U, L);  // U = ESH(i-1,i), L = ESH(i,i-1)
+      const double2 *U = blk(Uc, i - 1), *L = blk(Lc, i - 1);  // ESH(i-1,i), ESH(i,i-1)
       mm(blk(g, i), OP_N, L, OP_N, W1, one, zero);
       mm(W1, OP_N, blk(Grd, i - 1), OP_N, blk(Grl, i - 1), mone, zero);
       mm(U, OP_N, blk(g, i), OP_N, W2, one, zero);
@@ -2147,7 +2151,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
       mm(blk(g, nbl - 1), OP_N, blk(Sn, nbl - 1), OP_N, W1, one, zero);
       mm(W1, OP_N, blk(g, nbl - 1), OP_C, gns, one, zero);
       for (int i = nbl - 2; i >= 0; --i) {
-        form_off(i, U, nullptr);
+        const double2 *U = blk(Uc, i);
         mm(U, OP_N, gns, OP_N, W1, one, zero);
         mm(W1, OP_N, U, OP_C, blk(Sn, i), one, one);
         if (inel_off) {  // Sigma_n(i,i) -= Sigma_n(i,i+1) g^H U^H + U g Sigma_n(i+1,i)
@@ -2169,7 +2173,7 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
     fill_int(reinterpret_cast<int *>(err), 2 * B, 0, stream_);
     // ---- downward recursion ----
     for (int i = 0; i + 1 < nbl; ++i) {
-      form_off(i, nullptr, L);  // L = ESH(i+1,i)
+      const double2 *L = blk(Lc, i);  // ESH(i+1,i)
       // with inelastic stores, Gn(i+1,i) / Gn(i,i+1) are formed in them directly
       // (cache_Gn; masked launches leave converged items' blocks as they were)
       double2 *Gnl_i = io ? ost(5, i) : Gnl, *Gnu_i = io ? ost(4, i) : Gnu;
@@ -2242,9 +2246,9 @@ void Engine::scba_mw(DevStruct &st, int nitems, const double *dE, const int *dsl
   // n: the (uniform) PL order; nm >= n also covers contact surface blocks
   const int n = st.bsz[0], nm = st.nmax(), nbl = st.nbl;
   const size_t b2 = (size_t)nm * nm * sizeof(double2);
-  // G^r/G^n block lists (9 nbl) + sweep temporaries (8) + contact decimation and
+  // G^r/G^n block lists and ESH off-diagonal blocks (11 nbl) + sweep temporaries (8) + contact decimation and
   // self-energy workspace (12 per contact) + inverse workspace + per-orbital vectors
-  const size_t per_item = b2 * (size_t)(9 * nbl + 12 * st.ncont + 16) +
+  const size_t per_item = b2 * (size_t)(11 * nbl + 12 * st.ncont + 16) +
                           inv_work_bytes(nm, std::max(st.ncont, 1), 8) + (size_t)n * nbl * 48 + 1024;
   const int B = pick_batch(st, nitems, o, per_item);
   last_batch_ = B;
@@ -2365,7 +2369,7 @@ void Engine::inel_mw(DevStruct &st, const double *E, const double *frm, const So
   // batch of the elastic sweeps (same per-item budget as scba_mw)
   const int nm = st.nmax();
   const size_t b2 = (size_t)nm * nm * sizeof(double2);
-  const size_t per_item = b2 * (size_t)(9 * nbl + 12 * ncont + 16) +
+  const size_t per_item = b2 * (size_t)(11 * nbl + 12 * ncont + 16) +
                           inv_work_bytes(nm, ncont, 8) + (size_t)n * nbl * 48 + 1024;
   const int B = pick_batch(st, (int)std::min<long long>(NI, 1 << 30), o, per_item);
   last_batch_ = B;
