@@ -1984,7 +1984,6 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
   fill_int(ncyc_c, B * std::max(ncont, 1), 0, stream_);
   fill_int(status_c, B * std::max(ncont, 1), 0, stream_);
   fill_int(dstatus, B, 0, stream_);
-  const int nc1 = std::max(ncont, 1);
   if (io && io->sigc_valid) {  // contact self-energies of an earlier outer pass
     for (int c = 0; c < ncont; ++c) {
       copy_batched(Sig + c * SB, s2, io->sigc + ((long long)c * io->NI + io->i0) * s2, s2, s2, B,
@@ -2002,7 +2001,6 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
                                 sizeof(int) * B, hipMemcpyDeviceToDevice, stream_));
       }
   }
-  (void)nc1;
   // inelastic stores: block i of kind t for the items of this batch
   auto ist = [&](int t, int i) -> const double2 * {
     return io && io->S[t] ? io->S[t] + ((long long)i * io->NI + io->i0) * s2 : nullptr;
