@@ -274,7 +274,8 @@ def bench_c4(a, hip, wl, torch):
     cp = np.full(s.ndev, c["coupling"])
     eng = hip.Engine(0)
     ds = hip.DeviceStructure(s)
-    run = lambda: hip.solve_scba_mw(eng, ds, E, cp, c["scba_niter"], c["scba_tol"], frm, 3, delta=eta)
+    run = lambda: hip.solve_scba_mw(eng, ds, E, cp, c["scba_niter"], c["scba_tol"], frm, 3, delta=eta,
+                                    lanes=a.lanes)
     for i in range(a.warmup):
         t0 = time.time()
         run()
@@ -369,7 +370,7 @@ def bench_inel(a, hip, wl, torch):
     eng = hip.Engine(0)
     ds = hip.DeviceStructure(structs)
     run = lambda: hip.solve_inelastic_mw(eng, ds, E, kw, frm, 3, K, w["iz"], w["shift"], w["Nq"], True,
-                                         c["niter"], c["tol"], delta=w["eta"])
+                                         c["niter"], c["tol"], delta=w["eta"], lanes=a.lanes)
     for i in range(a.warmup):
         t0 = time.time()
         run()

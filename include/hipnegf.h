@@ -213,7 +213,8 @@ typedef struct {
   double eps_drop;   /* EPS drop tolerance (ln_precision), default 0           */
   int flags;         /* hipnegf_opt_flags                                      */
   int lanes;         /* concurrent item groups (own stream each); 1: one;
-                        0: automatic (2 for >= 256 items of blocks >= 128)    */
+                        0: automatic (transmission: 2 for >= 256 items of
+                        blocks >= 128; SCBA solves: 2 for >= 256 items)       */
 } hipnegf_opts;
 /* HIPNEGF_NO_SHARED_DECIMATION: run one Sancho-Rubio recursion per contact
  * even when two contacts are mirror images (by default such a pair shares one

@@ -472,7 +472,7 @@ def density_accumulate(eng: Engine, ds: DeviceStructure, z: np.ndarray, w: np.nd
 
 def solve_scba_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, coupling: np.ndarray,
                   niter: int, tol: float, frm, ref: int, slot=None, delta: float = 1e-4,
-                  precision: int = FP64, max_batch: int = 0, want_diag: bool = False):
+                  precision: int = FP64, max_batch: int = 0, want_diag: bool = False, lanes: int = 1):
     """Elastic-dephasing SCBA + Meir-Wingreen per energy (host arrays).
     frm: [ncont+1] shared or [n, ncont+1] per item.  Returns dict."""
     E = np.ascontiguousarray(E, dtype=np.float64)
@@ -488,7 +488,7 @@ def solve_scba_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, coupling: np.
     st = np.zeros(n, dtype=np.int32)
     gn = np.zeros((n, s.ndev), dtype=np.complex128) if want_diag else None
     gr = np.zeros((n, s.ndev), dtype=np.complex128) if want_diag else None
-    o = _opts(delta, 1, 2, precision, max_batch)
+    o = _opts(delta, 1, 2, precision, max_batch, lanes=lanes)
     check(load().hipnegf_solve_scba_mw(
         eng.handle, ds.handle, n, _ptr(E), _ptr(sl), C.byref(o), _ptr(cp), niter, tol, _ptr(fr),
         per_item, ref, _ptr(curr), _ptr(iters), _ptr(err), _ptr(st),
@@ -531,7 +531,7 @@ def solve_inelastic_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, kweights
                        K: np.ndarray, iz, shift: int, Nq: float, tridiag: bool, niter: int,
                        tol: float, coupling=None, niter_ela: int = 0, tol_ela: float = 1e-7,
                        integration: int = 0, g_spin: float = 2.0, delta: float = 1e-4,
-                       max_batch: int = 0) -> dict:
+                       max_batch: int = 0, lanes: int = 1) -> dict:
     """meir_wingreen with an inelastic model over the (E,k) grid: E uniform,
     the first len(kweights) slots of ds as the k-points, frm[ne, ncont+1].
     Returns dict(curr_mat[ne, ncont], currents, passes, err, status[nk, ne],
@@ -552,7 +552,7 @@ def solve_inelastic_mw(eng: Engine, ds: DeviceStructure, E: np.ndarray, kweights
     err = C.c_double()
     st = np.zeros(ne * nk, dtype=np.int32)
     it = np.zeros(ne * nk, dtype=np.int32)
-    o = _opts(delta, 1, 2, FP64, max_batch)
+    o = _opts(delta, 1, 2, FP64, max_batch, lanes=lanes)
     L = load()
     vp, i, d = C.c_void_p, C.c_int, C.c_double
     L.hipnegf_solve_inelastic_mw.argtypes = [vp, vp, i, vp, i, vp, C.POINTER(Opts), vp, i, d, vp, i, vp,

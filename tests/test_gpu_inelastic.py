@@ -92,6 +92,23 @@ def test_chunked_items_and_three_kpoints(gpu):
     _compare(hip, eng, orc, structs, kw, E, K, iz, 2, Nq, frm, True, max_batch=16)
 
 
+def test_lanes_and_batches_bitwise(gpu):
+    """Item groups on concurrent streams and smaller batches leave every
+    result bitwise unchanged."""
+    hip, eng, orc = gpu
+    kw = np.array([0.3, 0.7])
+    structs, E, K, iz, Nq, frm = _case(3, 2, 2, 25, 2, kw)
+    ds = hip.DeviceStructure(structs)
+    r0 = hip.solve_inelastic_mw(eng, ds, E, kw, frm, 3, K, iz, 2, Nq, True, 40, 1e-9)
+    for mb, lanes in ((0, 2), (7, 2), (0, 3), (11, 1)):
+        r1 = hip.solve_inelastic_mw(eng, ds, E, kw, frm, 3, K, iz, 2, Nq, True, 40, 1e-9, max_batch=mb,
+                                    lanes=lanes)
+        np.testing.assert_array_equal(r0["curr_mat"], r1["curr_mat"])
+        np.testing.assert_array_equal(r0["iters"], r1["iters"])
+        assert r0["passes"] == r1["passes"]
+    ds.close()
+
+
 def test_conservation_and_zero_coupling(gpu):
     hip, eng, orc = gpu
     kw = np.array([0.3, 0.7])

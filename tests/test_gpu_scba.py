@@ -153,8 +153,8 @@ def test_scba_batch_invariance(gpu):
     cp = np.full(s.ndev, 0.06)
     ds = hip.DeviceStructure(s)
     r0 = hip.solve_scba_mw(eng, ds, E, cp, 20, 1e-9, [1.0, 0.0, 0.0], 3, delta=1e-5)
-    for mb in (1, 4):
+    for mb, lanes in ((1, 1), (4, 1), (0, 2), (4, 3)):  # batches, and groups on concurrent streams
         r1 = hip.solve_scba_mw(eng, ds, E, cp, 20, 1e-9, [1.0, 0.0, 0.0], 3, delta=1e-5,
-                               max_batch=mb)
+                               max_batch=mb, lanes=lanes)
         np.testing.assert_array_equal(r0["curr"], r1["curr"])
         np.testing.assert_array_equal(r0["iters"], r1["iters"])
