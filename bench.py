@@ -467,7 +467,7 @@ def bench_path(a, hip, wl, torch):
         E = Egrid[np.linspace(0, len(Egrid) - 1, per_step).astype(int)]
         proj = [list(range(1 + n * i, 1 + n * (i + 1))) for i in range(s.nbl)]
         run = lambda sel=None: hip.solve_transmission_dos(
-            eng, ds, E if sel is None else E[sel], proj=proj, dos_delta=1e-4, delta=eta)
+            eng, ds, E if sel is None else E[sel], proj=proj, dos_delta=1e-4, delta=eta, lanes=a.lanes)
         workload = f"kslab_n{n}_nbl{s.nbl}_T_and_layer_LDOS_E{per_step}"
         unit = "energy-points/s"
     else:

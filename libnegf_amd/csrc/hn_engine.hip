@@ -1512,11 +1512,41 @@ void Engine::transmission_dos(DevStruct &st, int nitems, const double *dE, const
   if (d.nldos < 0 || (!d.full && d.nldos > 0 && (!d.ptr || !d.idx)))
     throw std::runtime_error("transmission_dos: bad LDOS projections");
   HN_CHECK(hipSetDevice(device_));
+  const long long nout = d.full ? st.ndev() : d.nldos;
+  // item groups on concurrent streams, with the same automatic rule as the
+  // transmission-only solve (every output row belongs to one item)
+  const int lanes = o.lanes > 0 ? o.lanes : (nitems >= 256 && st.nmax() >= 128) ? 2 : 1;
+  if (lanes > 1 && nitems >= 2 * lanes) {
+    prepare(st);
+    SolveOpts lo = o;
+    lo.lanes = 1;
+    if (lo.max_batch <= 0) lo.max_batch = std::max(1, (nitems + lanes - 1) / lanes);
+    const int per = (nitems + lanes - 1) / lanes;
+    std::vector<std::thread> th;
+    std::vector<std::string> err(lanes);
+    for (int l = 0; l < lanes; ++l) {
+      const int i0 = l * per, cnt = std::min(per, nitems - i0);
+      if (cnt <= 0) break;
+      Engine *e = &lane(l);
+      th.emplace_back([&, l, i0, cnt, e]() {
+        try {
+          e->transmission_dos(st, cnt, dE + i0, dslot + i0, lo, npairs, ni, nf, d,
+                              dT + (long long)i0 * npairs, dldos ? dldos + i0 * nout : nullptr,
+                              dncyc ? dncyc + (long long)i0 * st.ncont : nullptr, dstatus + i0);
+        } catch (const std::exception &x) {
+          err[l] = x.what();
+        }
+      });
+    }
+    for (auto &t : th) t.join();
+    for (auto &m : err)
+      if (!m.empty()) throw std::runtime_error(m);
+    return;
+  }
   const size_t per_item = dos_item_bytes(st, npairs, ni, nf);
   const int B = pick_batch(st, nitems, o, per_item);
   last_batch_ = B;
   arena_.reserve(per_item * (size_t)B + inv_work_bytes(st.nmax(), B, 8) + (1 << 20));
-  const long long nout = d.full ? st.ndev() : d.nldos;
   for (int i0 = 0; i0 < nitems; i0 += B) {
     const int nb = std::min(B, nitems - i0);
     dos_batch(st, nb, dE + i0, dslot + i0, o, npairs, ni, nf, d, dT + (long long)i0 * npairs,

@@ -394,7 +394,7 @@ class DosOpts(C.Structure):
 def solve_transmission_dos(eng: Engine, ds: DeviceStructure, E: np.ndarray, pairs=((1, 2),),
                            proj=None, full: bool = False, dos_delta: float = 0.0, slot=None,
                            delta: float = 1e-4, precision: int = FP64, max_batch: int = 0,
-                           flags: int = 0):
+                           flags: int = 0, lanes: int = 1):
     """Transmission and local DOS per energy (calculate_transmissions_and_dos).
     proj: list of 1-based orbital lists (LDOS projections); full: one column
     per device orbital.  Returns (T[n, npairs], ldos[n, nout], ncyc[n, ncont],
@@ -417,7 +417,7 @@ def solve_transmission_dos(eng: Engine, ds: DeviceStructure, E: np.ndarray, pair
     nc = max(ds.template.ncont, 1)
     ncyc = np.zeros((n, nc), dtype=np.int32)
     st = np.zeros(n, dtype=np.int32)
-    o = _opts(delta, 1, 2, precision, max_batch, flags=flags)
+    o = _opts(delta, 1, 2, precision, max_batch, flags=flags, lanes=lanes)
     check(load().hipnegf_solve_transmission_dos(eng.handle, ds.handle, n, _ptr(E), _ptr(sl), C.byref(o),
                                                 len(pairs), _ptr(ni), _ptr(nf), C.byref(d), _ptr(T),
                                                 _ptr(ldos), _ptr(ncyc), _ptr(st)))

@@ -101,6 +101,12 @@ def test_off_first_block_and_nonuniform(gpu):
     _cmpT(T[:, 0], Tr[:, 0])
     _cmpL(L, Lr)
     assert (ncyc == nr).all()
+    # item groups on concurrent streams: bitwise the same
+    T3, L3, n3, s3 = hip.solve_transmission_dos(eng, ds, E, pairs=[(1, 2)], full=True, dos_delta=5e-4,
+                                                delta=1e-5, lanes=3)
+    np.testing.assert_array_equal(T3, T)
+    np.testing.assert_array_equal(L3, L)
+    np.testing.assert_array_equal(n3, ncyc)
     # unequal PLs: 100-site chain with disorder, PLs of 9, 17, 11, 23 sites
     rng = np.random.default_rng(8)
     H = chain_dense(onsite=0.4 * rng.standard_normal(60))
