@@ -214,7 +214,7 @@ typedef struct {
   int flags;         /* hipnegf_opt_flags                                      */
   int lanes;         /* concurrent item groups (own stream each); 1: one;
                         0: automatic (transmission: 2 for >= 256 items of
-                        blocks >= 128; SCBA solves: 2 for >= 256 items)       */
+                        blocks >= 64; SCBA solves: 2 for >= 256 items)        */
 } hipnegf_opts;
 /* HIPNEGF_NO_SHARED_DECIMATION: run one Sancho-Rubio recursion per contact
  * even when two contacts are mirror images (by default such a pair shares one

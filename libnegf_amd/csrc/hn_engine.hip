@@ -1515,7 +1515,7 @@ void Engine::transmission_dos(DevStruct &st, int nitems, const double *dE, const
   const long long nout = d.full ? st.ndev() : d.nldos;
   // item groups on concurrent streams, with the same automatic rule as the
   // transmission-only solve (every output row belongs to one item)
-  const int lanes = o.lanes > 0 ? o.lanes : (nitems >= 256 && st.nmax() >= 128) ? 2 : 1;
+  const int lanes = o.lanes > 0 ? o.lanes : (nitems >= 256 && st.nmax() >= 64) ? 2 : 1;
   if (lanes > 1 && nitems >= 2 * lanes) {
     prepare(st);
     SolveOpts lo = o;
@@ -1912,8 +1912,8 @@ void Engine::transmission_pairs(DevStruct &st, int nitems, const double *dE, con
   if (npairs == 1 && o.fused && !o.graph && t_fused(st, nitems, dE, dslot, o, ni[0], nf[0], dT, dncyc, dstatus))
     return;
   // Two item groups on two streams measured 7 % faster on the C3 slab (n = 256)
-  // than one; three or more were slower.
-  const int lanes = o.lanes > 0 ? o.lanes : (nitems >= 256 && st.nmax() >= 128) ? 2 : 1;
+  // and 16 % on 2000 energies of n = 64 than one; three were slower.
+  const int lanes = o.lanes > 0 ? o.lanes : (nitems >= 256 && st.nmax() >= 64) ? 2 : 1;
   if (lanes > 1 && nitems >= 2 * lanes) {
     // Item groups on concurrent streams: one group's latency-bound steps (panel
     // factorisations, convergence polls) overlap the other groups' GEMMs.
