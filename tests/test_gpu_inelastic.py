@@ -92,6 +92,44 @@ def test_chunked_items_and_three_kpoints(gpu):
     _compare(hip, eng, orc, structs, kw, E, K, iz, 2, Nq, frm, True, max_batch=16)
 
 
+def test_ribbon_stack_w32_nbl10(gpu):
+    """The bench workload's model at W=32, nbl=10 (polar-optical phonons,
+    transverse k pair, one z bin per layer), 24 energies, 16 outer passes."""
+    hip, eng, orc = gpu
+    from libnegf_amd import workloads as wl
+    c = wl.CI
+    w = wl.config_inel(nbl=10, W=32, nE=24, nk=2)
+    E = w["E"]
+    shift = int(round(c["wq"] / (E[1] - E[0])))
+    K = orc.inel_kmat_po(w["ndz"], 1.0, w["kfrac"], w["kw"], c["coupling"], c["wq"], c["area"], c["eps0"],
+                         c["eps_inf"], c["q0"], latt=w["latt"])
+    frm = np.stack([fermi(E, w["mu"][0], w["kT"]), fermi(E, w["mu"][1], w["kT"]), np.zeros_like(E)], axis=1)
+    ds = hip.DeviceStructure(w["structs"])
+    g = hip.solve_inelastic_mw(eng, ds, E, w["kw"], frm, 3, K, w["iz"], shift, w["Nq"], True, 15, 1e-12)
+    ds.close()
+    r = orc.inelastic_mw(w["structs"], w["kw"], E, 1e-4, frm, 3, K, w["iz"], shift, w["Nq"], True, 15, 1e-12)
+    assert g["passes"] == r["passes"] == 16 and (g["status"] == 0).all()
+    np.testing.assert_array_equal(g["iters"], r["iters_ela"])
+    scale = np.abs(r["curr_mat"]).max()
+    err = np.abs(g["curr_mat"] - r["curr_mat"]).max() / scale
+    # Several energies sit at band edges of the stacked ribbons: there two FP64
+    # evaluations of the same algorithm differ by a few 1e-10 (the oracle's
+    # built-in loops against MKL: 4e-11 before the first inelastic pass, 3e-10
+    # after it).  Bar: 1e-10, or four times that backend spread where MKL loads.
+    bar = 1e-10
+    try:
+        orc.set_backend("mkl")
+        rm = orc.inelastic_mw(w["structs"], w["kw"], E, 1e-4, frm, 3, K, w["iz"], shift, w["Nq"], True, 15,
+                              1e-12)
+        bar = max(bar, 4.0 * np.abs(rm["curr_mat"] - r["curr_mat"]).max() / scale)
+    except RuntimeError:
+        pass
+    finally:
+        orc.set_backend("builtin")
+    assert bar <= 1e-8
+    assert err <= bar, (err, bar)
+
+
 def test_lanes_and_batches_bitwise(gpu):
     """Item groups on concurrent streams and smaller batches leave every
     result bitwise unchanged."""
