@@ -130,6 +130,17 @@ def test_ribbon_stack_w32_nbl10(gpu):
     assert err <= bar, (err, bar)
 
 
+@pytest.mark.parametrize("nk", [1, 5, 17])
+def test_kpoint_counts(gpu, nk):
+    """The self-energy kernel's q sums for 1 k-point, a non-power-of-two
+    count held in registers, and more k-points than its register tiers (17)."""
+    hip, eng, orc = gpu
+    kw = np.arange(1, nk + 1, dtype=float)
+    kw /= kw.sum()
+    structs, E, K, iz, Nq, frm = _case(2, 2, nk, 11, 1, kw)
+    _compare(hip, eng, orc, structs, kw, E, K, iz, 1, Nq, frm, True, niter=6)
+
+
 def test_lanes_and_batches_bitwise(gpu):
     """Item groups on concurrent streams and smaller batches leave every
     result bitwise unchanged."""
