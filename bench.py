@@ -33,8 +33,9 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
     ap.add_argument("--items", type=int, default=0, help="items per step (0: config default)")
     ap.add_argument("--nbl", type=int, default=0, help="override the number of layers")
-    ap.add_argument("--cpu-items", type=int, default=48,
-                    help="CPU baseline sample: spread over up to 8 timed steps (k-points)")
+    ap.add_argument("--cpu-items", type=int, default=160,
+                    help="CPU baseline sample: spread over up to 8 timed steps (k-points); "
+                         "about 12 s of host work on the default workload")
     ap.add_argument("--mixed-items", type=int, default=500,
                     help="items of the mixed-precision leg (one step, after the FP64 timing; 0: off)")
     ap.add_argument("--mixed-ns", type=int, default=2,
@@ -790,8 +791,12 @@ def main():
                           f"host CPU: {cpu_model()}",
                    seconds=secs)
         big = Tc > 1e-6 * Tc.max()
-        parity = dict(max_rel_err_vs_mkl=float((np.abs(Tg[big] - Tc[big]) / Tc[big]).max()),
-                      ncyc_equal=nc_ok, points=int(npts))
+        rel = np.abs(Tg[big] - Tc[big]) / Tc[big]
+        # the ill-conditioned energies (condition ~ 1/eta) put any two FP64
+        # evaluations far apart; the median and the count above 1e-8 show how
+        # many points the maximum stands for
+        parity = dict(max_rel_err_vs_mkl=float(rel.max()), median_rel_err_vs_mkl=float(np.median(rel)),
+                      points_above_1e8=int((rel > 1e-8).sum()), ncyc_equal=nc_ok, points=int(npts))
         log(f"cpu baseline: {cpu['value']:.3f} points/s on {threads} threads; parity {parity}")
     if a.config == "c3" and s0.nbl == 40 and per_step == 500:
         parity = dict(parity or {}, **truth_parity(results, last, kidx, nsteps))
