@@ -796,7 +796,7 @@ def main():
         # evaluations far apart; the median and the count above 1e-8 show how
         # many points the maximum stands for
         parity = dict(max_rel_err_vs_mkl=float(rel.max()), median_rel_err_vs_mkl=float(np.median(rel)),
-                      points_above_1e8=int((rel > 1e-8).sum()), ncyc_equal=nc_ok, points=int(npts))
+                      points_rel_err_gt_1e_8=int((rel > 1e-8).sum()), ncyc_equal=nc_ok, points=int(npts))
         log(f"cpu baseline: {cpu['value']:.3f} points/s on {threads} threads; parity {parity}")
     if a.config == "c3" and s0.nbl == 40 and per_step == 500:
         parity = dict(parity or {}, **truth_parity(results, last, kidx, nsteps))
