@@ -2209,10 +2209,14 @@ void Engine::scba_batch(DevStruct &st, int B, const double *dE, const int *dslot
       if (inel_off) mm(ist(4, i), OP_C, blk(Grd, i), OP_C, W1, one, one);  // Sigma_n(i+1,i) Gr(i,i)^H
       mm(L, OP_N, blk(Gnd, i), OP_N, W1, mone, one);
       mm(blk(g, i + 1), OP_N, W1, OP_N, Gnl_i, one, zero);
-      dagger_batched(Gnl_i, s2, Gnu_i, s2, n, n, B, act, stream_);
+      // Gn(i,i+1) = Gn(i+1,i)^H is formed only where it is kept or tested;
+      // otherwise the product below reads Gn(i+1,i) conjugate-transposed
+      const bool keep_gnu = io || s_general;
+      if (keep_gnu) dagger_batched(Gnl_i, s2, Gnu_i, s2, n, n, B, act, stream_);
       mm(blk(Sn, i + 1), OP_N, blk(Grd, i + 1), OP_C, W1, one, zero);
       if (inel_off) mm(ist(4, i), OP_C, blk(Grl, i), OP_C, W1, one, one);  // Sigma_n(i+1,i) Gr(i+1,i)^H
-      mm(L, OP_N, Gnu_i, OP_N, W1, mone, one);
+      if (keep_gnu) mm(L, OP_N, Gnu_i, OP_N, W1, mone, one);
+      else mm(L, OP_N, Gnl_i, OP_C, W1, mone, one);
       mm(blk(g, i + 1), OP_N, W1, OP_N, blk(Gnd, i + 1), one, zero);
       if (s_general) {  // off-diagonal S entries enter the convergence test
         // Gn(i+1,i) = Gn(i,i+1)^H, so |dGn(i+1,i)(r,c)| = |dGn(i,i+1)(c,r)|: one pass over
